@@ -1,0 +1,129 @@
+"""ctypes binding of libbemhip.so (the C ABI declared in include/bemhip.h).
+
+The shared library is built in-tree next to this file. There is no fallback: if it is missing or does not
+export the expected symbols, importing the solver raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(os.environ.get("BEMHIP_LIBRARY", Path(__file__).with_name("libbemhip.so")))
+
+BEM_OK = 0
+BEM_PREC_FP64 = 0
+BEM_PREC_MIXED = 1
+BEM_MAX_CHANNELS = 4
+
+_p = C.c_void_p
+_i = C.c_int
+_ll = C.c_longlong
+_d = C.c_double
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_u16p = C.POINTER(C.c_uint16)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_faces", _i),
+        ("n_vertices", _i),
+        ("faces", _ip),
+        ("face_vertices", _dp),
+        ("normals", _dp),
+        ("curls", _dp),
+        ("area2", _dp),
+        ("qpoints", _dp),
+        ("n_chunks", _i),
+        ("order", _ip),
+        ("slot_count", _ip),
+        ("slot_start", _u16p),
+        ("slot_inc", _u16p),
+        ("slot_base", _ip),
+        ("strip_len", _i),
+        ("cs_ptr", _ip),
+        ("cs_slot", _ip),
+        ("vstar_ptr", _ip),
+        ("vstar_row", _ip),
+        ("n_pairs", _i),
+        ("sing_test", _ip),
+        ("sing_trial", _ip),
+        ("sing_rule", _ip),
+        ("sing_ptr", _ip),
+        ("n_rules", _i),
+        ("rule_off", _ip),
+        ("rule_len", _ip),
+        ("rule_pts", _dp),
+        ("m11", _d * 9),
+        ("m10", _d * 3),
+        ("n_src", _i),
+        ("src_faces", _ip),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "bem_version": (_i, []),
+    "bem_last_error": (_i, [C.c_char_p, _i]),
+    "bem_ctx_create": (_i, [_i, _p, C.POINTER(_p)]),
+    "bem_ctx_destroy": (_i, [_p]),
+    "bem_ctx_sync": (_i, [_p]),
+    "bem_ctx_stream": (_i, [_p, C.POINTER(_p)]),
+    "bem_mesh_create": (_i, [_p, C.POINTER(MeshDesc), C.POINTER(_p)]),
+    "bem_mesh_destroy": (_i, [_p]),
+    "bem_mesh_reserve": (_i, [_p, _i, _i, _i, _i, _i]),
+    "bem_mesh_device_bytes": (_i, [_p, C.POINTER(_ll)]),
+    "bem_assemble": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _ll, _ll, _p, _p, _ll, _ll, _p]),
+    "bem_lu_factor": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _ll, _p]),
+    "bem_lu_solve": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _ll, _p, _ll, _ll, _p, _ll, _ll, _i, _p]),
+    "bem_refine": (_i, [_p, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _ll, _ll, _p, _p, _p, _i, _i, _d, _p, _p, _p]),
+    "bem_residual": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _ll, _ll, _p, _i, _p, _p]),
+    "bem_growth": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _p]),
+    "bem_field": (_i, [_p, _p, _i, _i, _p, _i, _p, _p, _ll, _ll, _p, _ll, _i, _p, _ll, _ll]),
+    "bem_graph_begin": (_i, [_p]),
+    "bem_graph_end": (_i, [_p, C.POINTER(_p)]),
+    "bem_graph_launch": (_i, [_p, _p]),
+    "bem_graph_destroy": (_i, [_p]),
+}
+
+
+class BemError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load(path: Path | None = None):
+    """Load and type the library (cached). Raises if it is missing: the HIP path is the only path."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise BemError(
+            f"libbemhip.so not found at {p}; build it with `python boundary-lab_amd/build.py` "
+            "(or __graft_entry__.build())"
+        )
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def last_error(lib=None) -> str:
+    lib = lib or load()
+    buf = C.create_string_buffer(4096)
+    lib.bem_last_error(buf, len(buf))
+    return buf.value.decode("utf-8", "replace")
+
+
+def check(status: int, what: str = "", lib=None):
+    if status != BEM_OK:
+        raise BemError(f"{what} failed (status {status}): {last_error(lib)}")

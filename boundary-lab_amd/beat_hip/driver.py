@@ -1,0 +1,618 @@
+"""Solve-request driver: the job semantics of the reference worker, on the HIP engine.
+
+Restates src/blab/solvers/julia_local/solver.jl (cited per function) in Python:
+request parsing and validation, mesh loading, radiator/channel drives with crossovers, polar and Fibonacci
+observation sets, per-frequency solve, channel-basis synthesis, SPL, impedance, and the NDJSON event payloads
+(``initialized``, ``status``, ``result``, ``completed``/``cancelled``/``failed``).
+
+Differences by design (documented in the result diagnostics):
+* all channels of a frequency are solved from ONE factorization (the reference re-factors per channel,
+  BeatEngineRocmOperators.jl:238-242), and frequencies are batched on the device;
+* ``precision`` is "mixed" (default) or "fp64"; mixed results carry their FP64 refinement residual and
+  pivot growth, and optionally (``fp64_check``) their error against the FP64 GPU path.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import geometry as G
+
+# ---------------------------------------------------------------------------------------------
+# config parsing (solver.jl:46-250)
+# ---------------------------------------------------------------------------------------------
+
+
+def get_value(raw, key, default=None):
+    return raw[key] if isinstance(raw, dict) and key in raw and raw[key] is not None else default
+
+
+def validate_crossover_config(owner, crossover):
+    """solver.jl:54-69"""
+    if crossover is None:
+        return
+    ctype = str(get_value(crossover, "type", "none")).lower()
+    if ctype not in ("none", "lowpass", "highpass"):
+        raise ValueError(f"{owner} crossover type must be none, lowpass, or highpass.")
+    if ctype == "none":
+        return
+    freq = get_value(crossover, "frequency_hz")
+    if freq is None or float(freq) <= 0.0:
+        raise ValueError(f"{owner} crossover frequency_hz must be > 0.")
+    filt = str(get_value(crossover, "filter", "butterworth")).lower()
+    if filt not in ("butterworth", "linkwitz_riley"):
+        raise ValueError(f"{owner} crossover filter must be butterworth or linkwitz_riley.")
+    order = int(get_value(crossover, "order", 1))
+    if order not in (1, 2, 4, 6):
+        raise ValueError(f"{owner} crossover order must be 1, 2, 4, or 6.")
+    if filt == "linkwitz_riley" and order not in (2, 4, 6):
+        raise ValueError(f"{owner} Linkwitz-Riley order must be 2, 4, or 6.")
+
+
+def symmetry_mode_from_config(config):
+    """solver.jl:71-75"""
+    mode = str(get_value(config, "symmetry", "off")).strip().lower()
+    if mode not in ("off", "x", "xy"):
+        raise ValueError(f"Unsupported symmetry mode: {mode}. Expected off, x, or xy.")
+    return mode
+
+
+BACKEND_ALIASES = {
+    "beat_cuda": "cuda", "gpu": "cuda", "julia_local": "cuda", "local_julia": "cuda", "beat_cpu": "cpu",
+    "beat_rocm": "rocm", "amd": "rocm", "amdgpu": "rocm",
+}
+
+
+def beat_backend_from_request(request):
+    """solver.jl:76-91 (this engine runs every accepted backend id on the HIP device)."""
+    backend = str(get_value(request, "beat_engine_backend", "rocm")).strip().lower()
+    backend = BACKEND_ALIASES.get(backend, backend)
+    if backend not in ("cuda", "cpu", "rocm"):
+        raise ValueError(f"Unsupported BEAT Engine backend: {backend}. Expected cuda, cpu, or rocm.")
+    return backend
+
+
+def regular_quadrature_mode_from_config(config):
+    """solver.jl:93-101 for a GPU backend: only fixed order-4 quadrature."""
+    mode = str(get_value(config, "regular_quadrature_mode", get_value(config, "quadrature_mode", "fixed")))
+    mode = mode.strip().lower()
+    if mode not in ("fixed", "wavelength"):
+        raise ValueError(f"Unsupported regular quadrature mode: {mode}. Expected fixed or wavelength.")
+    if mode == "wavelength":
+        raise ValueError("Wavelength-driven regular quadrature is currently implemented only for the BEAT CPU backend.")
+    return mode
+
+
+@dataclass
+class MeshInput:
+    path: str
+    scale: float
+    name: str
+    translation: tuple
+
+
+def mesh_inputs_from_config(config):
+    """solver.jl:152-179"""
+    meshes = get_value(config, "meshes", []) or []
+    if meshes:
+        out, seen = [], set()
+        for i, mesh in enumerate(meshes, start=1):
+            name = str(get_value(mesh, "name", f"mesh_{i}"))
+            if name in seen:
+                raise ValueError(f"Duplicate mesh name: {name}")
+            seen.add(name)
+            tr = get_value(mesh, "translation_m", [0.0, 0.0, 0.0])
+            if len(tr) != 3:
+                raise ValueError(f"Mesh '{name}' translation_m must contain three values.")
+            scale = get_value(mesh, "scale_factor", get_value(config, "scale_factor", 0.001))
+            out.append(MeshInput(str(mesh["file"]), float(scale), name, tuple(float(v) for v in tr)))
+        return out
+    return [MeshInput(str(config["mesh_file"]), float(get_value(config, "scale_factor", 0.001)), "mesh",
+                      (0.0, 0.0, 0.0))]
+
+
+def radiator_inputs_from_config(config, mesh_inputs):
+    """solver.jl:185-233"""
+    raw = get_value(config, "radiators", []) or []
+    lookup = {m.name: i for i, m in enumerate(mesh_inputs, start=1)}
+    none = {"type": "none"}
+    if not raw:
+        return [{
+            "name": "Radiator", "tag": int(get_value(config, "tag_throat", 2)), "mesh": mesh_inputs[0].name,
+            "mesh_id": 1, "channel": "main", "velocity_offset_db": 0.0, "level_db": 0.0, "polarity": 1,
+            "delay_ms": 0.0, "hpf": none, "lpf": none,
+        }]
+    out = []
+    for r in raw:
+        name = str(get_value(r, "name", "Radiator"))
+        mesh_name = get_value(r, "mesh")
+        if mesh_name is None:
+            if len(mesh_inputs) != 1:
+                raise ValueError(f"Radiator '{name}' must specify 'mesh' when multiple meshes are configured.")
+            mesh_name = mesh_inputs[0].name
+        mesh_name = str(mesh_name)
+        if mesh_name not in lookup:
+            raise ValueError(f"Radiator '{name}' references unknown mesh '{mesh_name}'.")
+        validate_crossover_config(name, get_value(r, "hpf"))
+        validate_crossover_config(name, get_value(r, "lpf"))
+        out.append({
+            "name": name, "tag": int(r["tag"]), "mesh": mesh_name, "mesh_id": lookup[mesh_name],
+            "channel": str(get_value(r, "channel", "main")),
+            "velocity_offset_db": float(get_value(r, "velocity_offset_db", 0.0)),
+            "level_db": float(get_value(r, "level_db", 0.0)), "polarity": int(get_value(r, "polarity", 1)),
+            "delay_ms": float(get_value(r, "delay_ms", 0.0)), "hpf": get_value(r, "hpf", none),
+            "lpf": get_value(r, "lpf", none),
+        })
+    return out
+
+
+def channel_inputs_from_config(config):
+    """solver.jl:235-250"""
+    channels = {}
+    for ch in get_value(config, "channels", []) or []:
+        name = str(ch["name"])
+        validate_crossover_config(f"channel {name}", get_value(ch, "hpf"))
+        validate_crossover_config(f"channel {name}", get_value(ch, "lpf"))
+        channels[name] = {
+            "level_db": float(get_value(ch, "level_db", 0.0)), "polarity": int(get_value(ch, "polarity", 1)),
+            "delay_ms": float(get_value(ch, "delay_ms", 0.0)), "hpf": get_value(ch, "hpf", {"type": "none"}),
+            "lpf": get_value(ch, "lpf", {"type": "none"}),
+        }
+    return channels
+
+
+def load_combined_mesh(mesh_inputs):
+    """solver.jl:252-269: raw physical tags are kept, element mesh ids are the 1-based input index."""
+    parts = [G.read_gmsh22(m.path, m.scale, m.translation) for m in mesh_inputs]
+    return G.combine(parts)
+
+
+# ---------------------------------------------------------------------------------------------
+# drives and crossovers (solver.jl:271-318, 390-410, 590-605)
+# ---------------------------------------------------------------------------------------------
+
+
+def butterworth_poles(order):
+    return [np.exp(1j * (math.pi / 2 + (2 * k - 1) * math.pi / (2 * order))) for k in range(1, order + 1)]
+
+
+def butterworth_response(ctype, order, cutoff_hz, freq):
+    omega = 2 * math.pi * freq
+    wc = 2 * math.pi * float(cutoff_hz)
+    s = 1j * omega
+    resp = 1.0 + 0j
+    for pole in butterworth_poles(order):
+        sp = wc * pole if ctype == "lowpass" else wc / pole
+        resp *= (-sp) / (s - sp) if ctype == "lowpass" else s / (s - sp)
+    return resp
+
+
+def crossover_response(crossover, freq):
+    if crossover is None:
+        return 1.0 + 0j
+    ctype = str(get_value(crossover, "type", "none")).lower()
+    if ctype == "none":
+        return 1.0 + 0j
+    filt = str(get_value(crossover, "filter", "butterworth")).lower()
+    order = int(get_value(crossover, "order", 1))
+    cutoff = get_value(crossover, "frequency_hz")
+    if cutoff is None:
+        raise ValueError(f"Crossover frequency_hz must be set for {ctype}.")
+    if filt == "linkwitz_riley":
+        sec = butterworth_response(ctype, order // 2, cutoff, freq)
+        return sec * sec
+    return butterworth_response(ctype, order, cutoff, freq)
+
+
+def channel_drive(channel, freq):
+    omega = 2 * math.pi * freq
+    level = 10.0 ** (float(channel["level_db"]) / 20.0)
+    delay = np.exp(-1j * omega * float(channel["delay_ms"]) / 1000.0)
+    xo = crossover_response(get_value(channel, "hpf"), freq) * crossover_response(get_value(channel, "lpf"), freq)
+    return complex(float(channel["polarity"]) * level) * delay * xo
+
+
+def drive_for_radiator(radiator, channels, freq):
+    omega = 2 * math.pi * freq
+    ch = channels.get(str(get_value(radiator, "channel", "main")))
+    if ch is not None:
+        return channel_drive(ch, freq) * 10.0 ** (float(radiator["velocity_offset_db"]) / 20.0)
+    level = 10.0 ** ((float(radiator["level_db"]) + float(radiator["velocity_offset_db"])) / 20.0)
+    delay = np.exp(-1j * omega * float(radiator["delay_ms"]) / 1000.0)
+    xo = crossover_response(get_value(radiator, "hpf"), freq) * crossover_response(get_value(radiator, "lpf"), freq)
+    return complex(float(radiator["polarity"]) * level) * delay * xo
+
+
+def radiator_faces(mesh, radiator):
+    return (mesh.tags == int(radiator["tag"])) & (mesh.mesh_ids == int(radiator["mesh_id"]))
+
+
+def validate_radiator_channels(radiators, channels):
+    if not channels:
+        return
+    for r in radiators:
+        cname = str(get_value(r, "channel", "main"))
+        if cname not in channels:
+            raise ValueError(f"Radiator '{r['name']}' references unknown channel '{cname}'.")
+
+
+def validate_radiator_elements(mesh, radiators):
+    for r in radiators:
+        if not np.any(radiator_faces(mesh, r)):
+            raise ValueError(f"No elements found for radiator '{r['name']}' tag={r['tag']} on mesh '{r['mesh']}'.")
+
+
+def channel_unit_qunit(mesh, radiators, channel_names):
+    """Unit Neumann drives per channel, (F, C): q_e / (i rho omega) (solver.jl:434-458, 590-597).
+
+    Later radiators overwrite earlier ones on shared faces, as in the reference's assignment.
+    """
+    q = np.zeros((mesh.n_faces, len(channel_names)), dtype=np.complex128)
+    for ci, cname in enumerate(channel_names):
+        for r in radiators:
+            drive = 10.0 ** (float(r["velocity_offset_db"]) / 20.0) if str(get_value(r, "channel", "main")) == cname \
+                else 0.0
+            q[radiator_faces(mesh, r), ci] = drive
+    return q
+
+
+# ---------------------------------------------------------------------------------------------
+# observation sets (solver.jl:320-388)
+# ---------------------------------------------------------------------------------------------
+
+
+def polar_observation_points(config):
+    step = float(get_value(config, "step_size", 5.0))
+    amin = float(get_value(config, "min_angle", -180.0))
+    amax = float(get_value(config, "max_angle", 180.0))
+    if step <= 0:
+        raise ValueError("step_size must be positive.")
+    if amin < -180 or amax > 180:
+        raise ValueError("polar angle range must stay within [-180, 180] degrees.")
+    if amax < amin:
+        raise ValueError("max_angle must be >= min_angle.")
+    if not (amin <= 0 <= amax):
+        raise ValueError("polar angle range must include 0 degrees.")
+    n = int(math.floor((amax - amin) / step + 1e-9)) + 1
+    angles = (amin + step * np.arange(n, dtype=np.float64)).astype(np.float32)
+    if angles.size == 0 or angles[-1] < np.float32(amax):
+        angles = np.append(angles, np.float32(amax))
+    angles = np.clip(angles, np.float32(amin), np.float32(amax)).astype(np.float32)
+    d = float(get_value(config, "distance", 2.0))
+    if d <= 0:
+        raise ValueError("distance must be positive.")
+    df = np.float32(d)
+    off = np.float32(get_value(config, "axial_offset", 0.0))
+    ang = np.float32(math.pi) * angles / np.float32(180.0)
+    s, c = np.sin(ang).astype(np.float32), np.cos(ang).astype(np.float32)
+    z = np.zeros_like(s)
+    horizontal = np.stack([df * s, z, df * c + off], axis=1).astype(np.float32)
+    vertical = np.stack([z, df * s, df * c + off], axis=1).astype(np.float32)
+    return angles, horizontal, vertical, int(np.argmin(np.abs(angles.astype(np.float64))))
+
+
+def spherical_observation(config):
+    if not bool(get_value(config, "spherical_sampling_enabled", False)):
+        return None
+    n = int(get_value(config, "spherical_sampling_points", 6000))
+    if n <= 0:
+        raise ValueError("spherical_sampling_points must be positive.")
+    d = float(get_value(config, "distance", 2.0))
+    if d <= 0:
+        raise ValueError("distance must be positive.")
+    df = np.float32(d)
+    off = np.float32(get_value(config, "axial_offset", 0.0))
+    golden = np.float32(math.pi * (3.0 - math.sqrt(5.0)))
+    i = np.arange(n)
+    zu = (1.0 - (2.0 * i + 1.0) / n).astype(np.float32)
+    xy = np.sqrt(np.maximum(np.float32(1.0) - zu * zu, np.float32(0.0))).astype(np.float32)
+    az = (i.astype(np.float32) * golden).astype(np.float32)
+    x = (df * xy * np.cos(az)).astype(np.float32)
+    y = (df * xy * np.sin(az)).astype(np.float32)
+    pts = np.stack([x, y, (df * zu + off).astype(np.float32)], axis=1)
+    meta = {
+        "r_distance_m": np.full(n, df, dtype=np.float32),
+        "theta_polar_rad": np.arccos(np.clip(zu, -1, 1)).astype(np.float32),
+        "phi_azimuth_rad": np.mod(np.arctan2(y, x), np.float32(2 * math.pi)).astype(np.float32),
+    }
+    return pts, meta
+
+
+# ---------------------------------------------------------------------------------------------
+# post-processing (solver.jl:488-631)
+# ---------------------------------------------------------------------------------------------
+
+
+def pressure_to_spl(p):
+    with np.errstate(divide="ignore"):
+        return (20.0 * np.log10(np.abs(p) / 20e-6)).astype(np.float32)
+
+
+def complex_rows_to_wire(rows):
+    return {"real": [np.asarray(r.real, np.float32).tolist() for r in rows],
+            "imag": [np.asarray(r.imag, np.float32).tolist() for r in rows]}
+
+
+def interpolate_complex_reference(values, angles_deg, ref_deg):
+    if len(values) == 0:
+        return 0j
+    if len(values) == 1:
+        return complex(values[0])
+    ref = (float(ref_deg) + 180.0) % 360.0 - 180.0
+    ang = np.asarray(angles_deg, dtype=np.float64)
+    val = np.asarray(values, dtype=np.complex128)
+    if math.isclose(ang[0], -180.0) and math.isclose(ang[-1], 180.0):
+        ang, val = ang[:-1], val[:-1]
+    ea = np.concatenate([ang - 360.0, ang, ang + 360.0])
+    ev = np.concatenate([val, val, val])
+    for i in range(len(ea) - 1):
+        lo, hi = ea[i], ea[i + 1]
+        if lo <= ref <= hi:
+            if math.isclose(lo, hi):
+                return ev[i]
+            frac = (ref - lo) / (hi - lo)
+            return ev[i] * (1.0 - frac) + ev[i + 1] * frac
+    return ev[int(np.argmin(np.abs(ea - ref)))]
+
+
+def flat_target_corrections(channel_names, horizontal_rows, angles_deg, ref_deg, flat_target):
+    out = {}
+    for ci, name in enumerate(channel_names):
+        if not flat_target:
+            out[name] = 1.0
+            continue
+        mag = abs(interpolate_complex_reference(horizontal_rows[ci], angles_deg, ref_deg))
+        out[name] = 1.0 if mag <= 1e-12 else 1.0 / mag
+    return out
+
+
+DEFAULT_CHANNEL = {"level_db": 0.0, "polarity": 1, "delay_ms": 0.0, "hpf": {"type": "none"},
+                   "lpf": {"type": "none"}}
+
+
+def synthesize_channel_basis(channel_names, hrows, vrows, srows, channels, freq, angles_deg, ref_deg, flat):
+    corr = flat_target_corrections(channel_names, hrows, angles_deg, ref_deg, flat)
+    weights = [channel_drive(channels.get(n, DEFAULT_CHANNEL), freq) * corr[n] for n in channel_names]
+    hp = sum(hrows[i] * weights[i] for i in range(len(channel_names)))
+    vp = sum(vrows[i] * weights[i] for i in range(len(channel_names)))
+    hs, vs = pressure_to_spl(hp), pressure_to_spl(vp)
+    ref = hs[int(np.argmin(np.abs(np.asarray(angles_deg, np.float64))))]
+    sn = None
+    if srows is not None:
+        sp = sum(srows[i] * weights[i] for i in range(len(channel_names)))
+        sn = (pressure_to_spl(sp) - ref).astype(np.float32)
+    return {"horizontal_spl": hs, "vertical_spl": vs, "horizontal_norm": (hs - ref).astype(np.float32),
+            "vertical_norm": (vs - ref).astype(np.float32), "sphere_norm": sn, "corrections": corr,
+            "weights": weights}
+
+
+def impedance_for_radiators(mesh, pressure, radiators, drives, symmetry_factor=1):
+    """solver.jl:607-631: F = 10 * sum p_avg A * factor; [Re(F/drive)/2, -Im(F/drive)/2]."""
+    out = []
+    f = mesh.faces
+    for r, drive in zip(radiators, drives):
+        if abs(drive) <= 0.0:
+            out.append([float("nan"), float("nan")])
+            continue
+        sel = radiator_faces(mesh, r)
+        pav = (pressure[f[sel, 0]] + pressure[f[sel, 1]] + pressure[f[sel, 2]]) / 3.0
+        force = 10.0 * np.sum(pav * mesh.areas[sel] * symmetry_factor)
+        z = force / drive
+        out.append([float(np.float32(z.real / 2)), float(np.float32(-z.imag / 2))])
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# job
+# ---------------------------------------------------------------------------------------------
+
+
+@dataclass
+class Job:
+    """A parsed, mesh-loaded request ready for device work."""
+
+    request: dict
+    config: dict
+    frequencies: np.ndarray
+    mesh: G.BoundaryMesh
+    radiators: list
+    channels: dict
+    channel_names: list
+    angles: np.ndarray
+    horizontal: np.ndarray
+    vertical: np.ndarray
+    sphere: tuple | None
+    rho: float
+    sound_speed: float
+    flat_target: bool
+    ref_angle: float
+    symmetry: str
+    precision: str
+    qunit: np.ndarray
+    obs: np.ndarray
+    cancel_path: str | None
+    fp64_check: bool
+
+    @property
+    def n_horizontal(self):
+        return len(self.horizontal)
+
+
+def parse_request(request) -> tuple[dict, dict]:
+    """Everything the reference validates before emitting ``initialized`` (solver.jl:680-708)."""
+    schema = int(get_value(request, "schema_version", 1))
+    if schema not in (1, 2):
+        raise ValueError(f"Unsupported solve request schema_version {schema}.")
+    config = request["config"]
+    symmetry = symmetry_mode_from_config(config)
+    beat_backend_from_request(request)
+    freqs = np.asarray(request["frequencies_hz"], dtype=np.float32)
+    if freqs.size == 0:
+        raise ValueError("frequencies_hz must contain at least one frequency.")
+    mesh_inputs = mesh_inputs_from_config(config)
+    radiators = radiator_inputs_from_config(config, mesh_inputs)
+    channels = channel_inputs_from_config(config)
+    validate_radiator_channels(radiators, channels)
+    angles, hor, ver, _ = polar_observation_points(config)
+    sphere = spherical_observation(config)
+    init = {
+        "polar_angle_deg": angles.tolist(),
+        "radiator_names": [r["name"] for r in radiators],
+        "sphere_metadata": None if sphere is None else {k: v.tolist() for k, v in sphere[1].items()},
+    }
+    parsed = dict(config=config, symmetry=symmetry, freqs=freqs, mesh_inputs=mesh_inputs, radiators=radiators,
+                  channels=channels, angles=angles, hor=hor, ver=ver, sphere=sphere)
+    return init, parsed
+
+
+def build_job(request, parsed) -> Job:
+    config = parsed["config"]
+    if parsed["symmetry"] != "off":
+        raise ValueError("symmetry x/xy is not supported by this engine yet; solve the full model (symmetry off).")
+    regular_quadrature_mode_from_config(config)
+    if int(get_value(config, "quadrature_order", 4)) != 4 or int(get_value(config, "singular_order", 4)) != 4:
+        raise ValueError("This engine implements the order-4 regular rule and order-4 Duffy rules only.")
+    mesh = load_combined_mesh(parsed["mesh_inputs"])
+    validate_radiator_elements(mesh, parsed["radiators"])
+    names = sorted({str(get_value(r, "channel", "main")) for r in parsed["radiators"]})
+    qunit = channel_unit_qunit(mesh, parsed["radiators"], names)
+    pts = [parsed["hor"], parsed["ver"]] + ([parsed["sphere"][0]] if parsed["sphere"] is not None else [])
+    precision = str(get_value(config, "precision", os.environ.get("BEAT_HIP_PRECISION", "mixed"))).lower()
+    if precision not in ("mixed", "fp64"):
+        raise ValueError("precision must be 'mixed' or 'fp64'.")
+    return Job(
+        request=request, config=config, frequencies=parsed["freqs"], mesh=mesh, radiators=parsed["radiators"],
+        channels=parsed["channels"], channel_names=names, angles=parsed["angles"], horizontal=parsed["hor"],
+        vertical=parsed["ver"], sphere=parsed["sphere"], rho=float(np.float32(get_value(config, "rho", 1.21))),
+        sound_speed=float(np.float32(get_value(config, "sound_speed", 343.0))),
+        flat_target=bool(get_value(config, "flat_target_normalization_enabled", True)),
+        ref_angle=float(get_value(config, "flat_target_reference_angle_deg", 0.0)),
+        symmetry=parsed["symmetry"], precision=precision, qunit=qunit,
+        obs=np.concatenate(pts).astype(np.float64), cancel_path=get_value(request, "cancel_path"),
+        fp64_check=bool(get_value(config, "fp64_check", False)),
+    )
+
+
+def result_payload(job: Job, freq: float, p_vc: np.ndarray, field_oc: np.ndarray, timings: dict,
+                   diagnostics: dict) -> dict:
+    """Per-frequency result dict (solver.jl:921-996). p_vc (V, C) boundary pressure, field_oc (n_obs, C)."""
+    nh, nv = len(job.horizontal), len(job.vertical)
+    C = len(job.channel_names)
+    hrows = [field_oc[:nh, c] for c in range(C)]
+    vrows = [field_oc[nh : nh + nv, c] for c in range(C)]
+    srows = [field_oc[nh + nv :, c] for c in range(C)] if job.sphere is not None else None
+    syn = synthesize_channel_basis(job.channel_names, hrows, vrows, srows, job.channels, freq, job.angles,
+                                   job.ref_angle, job.flat_target)
+    drives = [drive_for_radiator(r, job.channels, freq) * syn["corrections"].get(str(get_value(r, "channel", "main")),
+                                                                                  1.0)
+              for r in job.radiators]
+    mixed = sum(p_vc[:, c] * syn["weights"][c] for c in range(C))
+    imp = impedance_for_radiators(job.mesh, mixed, job.radiators, drives)
+    return {
+        "freq_hz": float(np.float32(freq)),
+        "horizontal_spl_norm_db": syn["horizontal_norm"].tolist(),
+        "vertical_spl_norm_db": syn["vertical_norm"].tolist(),
+        "impedance": imp,
+        "horizontal_spl_db": syn["horizontal_spl"].tolist(),
+        "vertical_spl_db": syn["vertical_spl"].tolist(),
+        "sphere_spl_norm_db": None if syn["sphere_norm"] is None else syn["sphere_norm"].tolist(),
+        "channel_names": list(job.channel_names),
+        "horizontal_pressure": complex_rows_to_wire(hrows),
+        "vertical_pressure": complex_rows_to_wire(vrows),
+        "sphere_pressure": None if srows is None else complex_rows_to_wire(srows),
+        "timings": {k: float(np.float32(v)) for k, v in timings.items()},
+        "diagnostics": diagnostics,
+    }
+
+
+def diagnostics_for(job: Job, res, i, extra=None) -> dict:
+    d = {
+        "convergence_info": int(res.info[i]),
+        "message": "HIP direct dense solve" + (" + FP64 iterative refinement" if job.precision == "mixed" else ""),
+        "backend": "rocm",
+        "symmetry": job.symmetry,
+        "regular_assembly_mode": "hip_row_strip_slots",
+        "blas_threads": 0,
+        "regular_quadrature_mode": "fixed",
+        "regular_quadrature_order": 4,
+        "regular_quadrature_base_order": 4,
+        "precision": job.precision,
+        "relative_residual": float(res.residual[i]),
+        "pivot_growth": float(res.growth[i]),
+        "refinement_iterations": int(res.iters[i]),
+    }
+    if extra:
+        d.update(extra)
+    return d
+
+
+def fp64_errors(p_mixed, f_mixed, p_ref, f_ref) -> dict:
+    """Per-result error of a mixed-precision solve against the FP64 path."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        db = np.abs(20.0 * np.log10(np.abs(f_mixed) / np.abs(f_ref)))
+        ph = np.degrees(np.abs(np.angle(f_mixed / f_ref)))
+    return {
+        "fp64_pressure_rel_error": float(np.linalg.norm(p_mixed - p_ref) / max(np.linalg.norm(p_ref), 1e-300)),
+        "fp64_field_max_db_error": float(np.nanmax(db)),
+        "fp64_field_max_phase_error_deg": float(np.nanmax(ph)),
+    }
+
+
+def cancelled(job: Job) -> bool:
+    return job.cancel_path is not None and os.path.isfile(str(job.cancel_path))
+
+
+def run_job(request, engine_factory, emit, batch: int | None = None):
+    """Drive one request to completion, emitting event dicts through ``emit`` (solver.jl:680-1020).
+
+    engine_factory(prepared_mesh, qunit) -> beat_hip.engine.Engine
+    """
+    init, parsed = parse_request(request)
+    emit({"type": "initialized", **init})
+    emit({"type": "status", "message": f"BEAT Engine loading {len(parsed['mesh_inputs'])} mesh"
+          f"{'' if len(parsed['mesh_inputs']) == 1 else 'es'} (HIP engine)"})
+    job = build_job(request, parsed)
+    prep = G.prepare(job.mesh)
+    emit({"type": "status", "message": f"Initializing BEAT Engine using native HIP assembly and GPU solve "
+          f"({job.precision}, {job.mesh.n_faces} triangles, {job.mesh.n_vertices} P1 dofs)..."})
+    eng = engine_factory(prep, job.qunit)
+    try:
+        freqs = job.frequencies.astype(np.float64)
+        nb = int(batch or get_value(job.config, "frequency_batch", 8))
+        nb = max(1, min(nb, len(freqs)))
+        b = eng.batch(job.precision, nb, job.obs, job.rho, job.sound_speed)
+        ref = eng.batch("fp64", nb, job.obs, job.rho, job.sound_speed) if (job.fp64_check and job.precision == "mixed") \
+            else None
+        solved = 0
+        for i0 in range(0, len(freqs), nb):
+            if cancelled(job):
+                emit({"type": "cancelled", "solved_count": solved})
+                return
+            fr = freqs[i0 : i0 + nb]
+            res = b.run(fr)
+            rref = ref.run(fr) if ref is not None else None
+            n = len(fr)
+            for j in range(n):
+                timings = {"assembly_s": res.t_assembly / n, "solve_s": res.t_solve / n, "field_s": res.t_field / n}
+                extra = None
+                if rref is not None:
+                    extra = fp64_errors(res.pressure[j], res.field[j], rref.pressure[j], rref.field[j])
+                payload = result_payload(job, float(job.frequencies[i0 + j]), res.pressure[j], res.field[j], timings,
+                                         diagnostics_for(job, res, j, extra))
+                solved = i0 + j + 1
+                emit({"type": "result", "solved_count": solved, "total_count": len(freqs), "result": payload})
+        b.close()
+        if ref is not None:
+            ref.close()
+    finally:
+        eng.close()
+    emit({"type": "completed", "solved_count": len(job.frequencies)})
+
+
+def now():
+    return time.perf_counter()

@@ -1,0 +1,358 @@
+"""GPU solve driver: device mesh, batched assemble -> LU -> solve/refine -> field through libbemhip.
+
+PyTorch-ROCm provides device buffers and the stream only; all arithmetic runs in the HIP kernels behind the
+C ABI. Two forms, matching the workloads:
+
+* ``precision="fp64"`` -- FP64 assembly, FP64 LU and solve, FP64 field (the plain method).
+* ``precision="mixed"`` -- FP32 regular assembly with FP64 singular pairs, FP32-MFMA LU, FP64 residual
+  iterative refinement to ``tol``, FP32 field; FP64 pivot-growth check.
+
+``Engine.solve`` processes a list of frequencies in batches (one batch = one set of launches, optionally
+captured once as a HIP graph and replayed with new wavenumbers).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .geometry import PreparedMesh
+
+PRECISIONS = {"fp64": _lib.BEM_PREC_FP64, "mixed": _lib.BEM_PREC_MIXED}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(None)
+
+
+def _np_ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Device:
+    """One HIP context on one GPU with its own (default-priority) stream."""
+
+    def __init__(self, index: int = 0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise _lib.BemError("no GPU visible: the HIP solver has no CPU fallback")
+        self.torch = torch
+        self.lib = _lib.load()
+        self.index = index
+        self.device = torch.device("cuda", index)
+        self.stream = torch.cuda.Stream(device=self.device)
+        ctx = C.c_void_p()
+        _lib.check(self.lib.bem_ctx_create(index, C.c_void_p(self.stream.cuda_stream), C.byref(ctx)), "bem_ctx_create")
+        self.ctx = ctx
+
+    def sync(self):
+        _lib.check(self.lib.bem_ctx_sync(self.ctx), "bem_ctx_sync")
+
+    def close(self):
+        if self.ctx:
+            self.lib.bem_ctx_destroy(self.ctx)
+            self.ctx = None
+
+
+class DeviceMesh:
+    """A prepared mesh resident on the device (geometry, singular pairs, Duffy rules, slot tables)."""
+
+    def __init__(self, dev: Device, prep: PreparedMesh, src_faces: np.ndarray):
+        self.dev = dev
+        self.prep = prep
+        m = prep.mesh
+        self.F, self.V = m.n_faces, m.n_vertices
+        keep = {}
+
+        def arr(name, a, dtype):
+            keep[name] = np.ascontiguousarray(a, dtype=dtype)
+            return keep[name]
+
+        d = _lib.MeshDesc()
+        d.n_faces = self.F
+        d.n_vertices = self.V
+        d.faces = _np_ptr(arr("faces", m.faces, np.int32), C.c_int)
+        d.face_vertices = _np_ptr(arr("fv", m.face_vertices.reshape(self.F, 9), np.float64), C.c_double)
+        d.normals = _np_ptr(arr("n", m.normals, np.float64), C.c_double)
+        d.curls = _np_ptr(arr("curl", prep.curls.reshape(self.F, 9), np.float64), C.c_double)
+        d.area2 = _np_ptr(arr("a2", 2.0 * m.areas, np.float64), C.c_double)
+        d.qpoints = _np_ptr(arr("qp", prep.qpoints.reshape(self.F, 18), np.float64), C.c_double)
+        s = prep.slots
+        d.n_chunks = len(s.slot_count)
+        d.order = _np_ptr(arr("order", s.order, np.int32), C.c_int)
+        d.slot_count = _np_ptr(arr("sc", s.slot_count, np.int32), C.c_int)
+        d.slot_start = _np_ptr(arr("ss", s.slot_start, np.uint16), C.c_uint16)
+        d.slot_inc = _np_ptr(arr("si", s.slot_inc, np.uint16), C.c_uint16)
+        d.slot_base = _np_ptr(arr("sb", s.slot_base, np.int32), C.c_int)
+        d.strip_len = s.strip_len
+        d.cs_ptr = _np_ptr(arr("csp", s.cs_ptr, np.int32), C.c_int)
+        d.cs_slot = _np_ptr(arr("css", s.cs_slot, np.int32), C.c_int)
+        d.vstar_ptr = _np_ptr(arr("vsp", s.vstar_ptr, np.int32), C.c_int)
+        d.vstar_row = _np_ptr(arr("vsr", s.vstar_row, np.int32), C.c_int)
+        p = prep.pairs
+        d.n_pairs = len(p.test)
+        d.sing_test = _np_ptr(arr("st", p.test, np.int32), C.c_int)
+        d.sing_trial = _np_ptr(arr("str", p.trial, np.int32), C.c_int)
+        d.sing_rule = _np_ptr(arr("sr", p.rule, np.int32), C.c_int)
+        d.sing_ptr = _np_ptr(arr("sp", p.ptr, np.int32), C.c_int)
+        d.n_rules = len(p.rule_off)
+        d.rule_off = _np_ptr(arr("ro", p.rule_off, np.int32), C.c_int)
+        d.rule_len = _np_ptr(arr("rl", p.rule_len, np.int32), C.c_int)
+        d.rule_pts = _np_ptr(arr("rp", p.rule_pts, np.float64), C.c_double)
+        for i in range(9):
+            d.m11[i] = float(prep.m11.reshape(-1)[i])
+        for i in range(3):
+            d.m10[i] = float(prep.m10[i])
+        src = arr("src", np.asarray(src_faces, dtype=np.int32).reshape(-1), np.int32)
+        d.n_src = len(src)
+        d.src_faces = _np_ptr(src, C.c_int) if len(src) else C.POINTER(C.c_int)()
+        h = C.c_void_p()
+        _lib.check(dev.lib.bem_mesh_create(dev.ctx, C.byref(d), C.byref(h)), "bem_mesh_create")
+        self.handle = h
+        self.src_faces = src
+
+    def reserve(self, precision: str, nf: int, strip_nf: int, n_obs: int, n_ch: int):
+        _lib.check(
+            self.dev.lib.bem_mesh_reserve(self.handle, PRECISIONS[precision], nf, strip_nf, n_obs, n_ch),
+            "bem_mesh_reserve",
+        )
+
+    def device_bytes(self) -> int:
+        v = C.c_longlong()
+        _lib.check(self.dev.lib.bem_mesh_device_bytes(self.handle, C.byref(v)), "bem_mesh_device_bytes")
+        return int(v.value)
+
+    def close(self):
+        if self.handle:
+            self.dev.lib.bem_mesh_destroy(self.handle)
+            self.handle = None
+
+
+def matrix_bytes(n_vertices: int, precision: str) -> int:
+    """Device bytes per frequency held by one batch slot (A + A0 copy + vectors)."""
+    lda = _lda(n_vertices)
+    cb = 8 if precision == "mixed" else 16
+    return 2 * n_vertices * lda * cb + 64 * n_vertices * 16
+
+
+def _lda(n):
+    return ((n + 15) // 16) * 16
+
+
+@dataclass
+class BatchResult:
+    freqs: np.ndarray  # (nf,)
+    pressure: np.ndarray  # (nf, V, C) complex128
+    q: np.ndarray  # (nf, F, C) complex128 (i rho omega * unit drives)
+    field: np.ndarray  # (nf, n_obs, C) complex128 or None
+    residual: np.ndarray  # (nf,) relative FP64 residual of the solved system
+    growth: np.ndarray  # (nf,) pivot growth
+    iters: np.ndarray  # (nf,) refinement corrections applied (0 for fp64)
+    info: np.ndarray  # (nf,) LU info
+    t_assembly: float
+    t_solve: float
+    t_field: float
+
+
+class Batch:
+    """Device buffers and (optionally) a captured HIP graph for one (mesh, precision, nf) shape."""
+
+    def __init__(self, dmesh: DeviceMesh, precision: str, nf: int, qunit: np.ndarray, obs: np.ndarray | None,
+                 rho: float, sound_speed: float, tol: float = 1e-10, max_iter: int = 4, strip_nf: int = 4):
+        dev = dmesh.dev
+        torch = dev.torch
+        self.dmesh, self.dev, self.precision, self.nf = dmesh, dev, precision, nf
+        self.prec = PRECISIONS[precision]
+        self.rho, self.c = float(rho), float(sound_speed)
+        self.tol, self.max_iter = float(tol), int(max_iter)
+        N, F = dmesh.V, dmesh.F
+        qunit = np.asarray(qunit, dtype=np.complex128).reshape(F, -1)
+        self.n_ch = qunit.shape[1]
+        if not 1 <= self.n_ch <= _lib.BEM_MAX_CHANNELS:
+            raise ValueError(f"1..{_lib.BEM_MAX_CHANNELS} channels supported, got {self.n_ch}")
+        self.N, self.F = N, F
+        self.lda = _lda(N)
+        self.ldb = _lda(N)
+        cdt = torch.complex64 if precision == "mixed" else torch.complex128
+        with torch.cuda.stream(dev.stream):
+            kw = dict(device=dev.device)
+            self.k = torch.zeros(nf, dtype=torch.float64, **kw)
+            self.rhs_scale = torch.zeros(nf, dtype=torch.complex128, **kw)
+            self.qunit = torch.from_numpy(np.ascontiguousarray(qunit.T)).to(dev.device)  # (C, F)
+            self.A = torch.empty((nf, N, self.lda), dtype=cdt, **kw)
+            self.A0 = torch.empty((nf, N, self.lda), dtype=cdt, **kw)
+            self.b = torch.zeros((nf, self.n_ch, self.ldb), dtype=torch.complex128, **kw)
+            self.x = torch.zeros((nf, self.n_ch, self.ldb), dtype=torch.complex128, **kw)
+            self.r = torch.zeros((nf, self.n_ch, self.ldb), dtype=torch.complex128, **kw)
+            self.y = torch.zeros((nf, self.n_ch, self.ldb), dtype=cdt, **kw)
+            self.rowmax = torch.zeros((nf, N), dtype=torch.float64, **kw)
+            self.umax = torch.zeros((nf, N), dtype=torch.float64, **kw)
+            self.ipiv = torch.zeros((nf, N), dtype=torch.int32, **kw)
+            self.perm = torch.zeros((nf, N), dtype=torch.int32, **kw)
+            self.info = torch.zeros(nf, dtype=torch.int32, **kw)
+            self.rel = torch.zeros(nf, dtype=torch.float64, **kw)
+            self.iters = torch.zeros(nf, dtype=torch.int32, **kw)
+            self.done = torch.zeros(nf, dtype=torch.int32, **kw)
+            self.scratch = torch.zeros(2 * nf, dtype=torch.int32, **kw)
+            self.growth = torch.zeros(nf, dtype=torch.float64, **kw)
+            self.q = torch.zeros((nf, self.n_ch, F), dtype=torch.complex128, **kw)
+            self.n_obs = 0 if obs is None else int(len(obs))
+            if self.n_obs:
+                self.obs = torch.from_numpy(np.ascontiguousarray(obs, dtype=np.float64)).to(dev.device)
+                self.field = torch.zeros((nf, self.n_ch, self.n_obs), dtype=torch.complex128, **kw)
+        dmesh.reserve(precision, nf, min(strip_nf, nf), self.n_obs, self.n_ch)
+        self.graph = None
+
+    # -- launch sequence ------------------------------------------------------------------------
+    def _assemble(self):
+        lib, dev, m = self.dev.lib, self.dev, self.dmesh
+        _lib.check(
+            lib.bem_assemble(dev.ctx, m.handle, self.prec, self.nf, _ptr(self.k), _ptr(self.qunit), self.n_ch,
+                             _ptr(self.rhs_scale), _ptr(self.A), self.lda, self.N * self.lda, _ptr(self.A0),
+                             _ptr(self.b), self.ldb, self.n_ch * self.ldb, _ptr(self.rowmax)),
+            "bem_assemble",
+        )
+
+    def _factor_solve(self):
+        lib, dev = self.dev.lib, self.dev
+        sA, sB, sP = self.N * self.lda, self.n_ch * self.ldb, self.N
+        _lib.check(lib.bem_lu_factor(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A), self.lda, sA,
+                                     _ptr(self.ipiv), _ptr(self.perm), sP, _ptr(self.info)), "bem_lu_factor")
+        if self.precision == "mixed":
+            _lib.check(lib.bem_refine(dev.ctx, self.N, self.nf, _ptr(self.A), self.lda, sA, _ptr(self.perm), sP,
+                                      _ptr(self.A0), _ptr(self.b), self.ldb, sB, _ptr(self.x), _ptr(self.r),
+                                      _ptr(self.y), self.n_ch, self.max_iter, self.tol, _ptr(self.rel),
+                                      _ptr(self.iters), _ptr(self.done)), "bem_refine")
+        else:
+            _lib.check(lib.bem_lu_solve(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A), self.lda, sA,
+                                        _ptr(self.perm), sP, _ptr(self.b), self.ldb, sB, _ptr(self.x), self.ldb, sB,
+                                        self.n_ch, None), "bem_lu_solve")
+            _lib.check(lib.bem_residual(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A0), self.lda, sA,
+                                        _ptr(self.b), _ptr(self.x), self.ldb, sB, _ptr(self.r), self.n_ch,
+                                        _ptr(self.rel), _ptr(self.scratch)), "bem_residual")
+        _lib.check(lib.bem_growth(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A), self.lda, sA,
+                                  _ptr(self.rowmax), _ptr(self.umax), _ptr(self.growth)), "bem_growth")
+
+    def _field(self):
+        if not self.n_obs:
+            return
+        lib, dev = self.dev.lib, self.dev
+        _lib.check(lib.bem_field(dev.ctx, self.dmesh.handle, self.prec, self.nf, _ptr(self.k), self.n_obs,
+                                 _ptr(self.obs), _ptr(self.x), self.ldb, self.n_ch * self.ldb, _ptr(self.q),
+                                 self.n_ch * self.F, self.n_ch, _ptr(self.field), self.n_obs,
+                                 self.n_ch * self.n_obs), "bem_field")
+
+    def _sequence(self):
+        self._assemble()
+        self._factor_solve()
+        self._field()
+
+    def capture(self):
+        """Capture the whole launch sequence once; later batches replay it with new wavenumbers."""
+        torch = self.dev.torch
+        with torch.cuda.stream(self.dev.stream):
+            self.set_frequencies(np.full(self.nf, 1000.0))
+            self._sequence()  # warm run outside capture (allocations, code objects)
+            self.dev.sync()
+            _lib.check(self.dev.lib.bem_graph_begin(self.dev.ctx), "bem_graph_begin")
+            try:
+                self._sequence()
+            finally:
+                g = C.c_void_p()
+                rc = self.dev.lib.bem_graph_end(self.dev.ctx, C.byref(g))
+            _lib.check(rc, "bem_graph_end")
+            self.graph = g
+
+    def set_frequencies(self, freqs):
+        torch = self.dev.torch
+        f = np.asarray(freqs, dtype=np.float64)
+        if len(f) != self.nf:
+            f = np.concatenate([f, np.full(self.nf - len(f), f[-1])])
+        omega = 2.0 * math.pi * f
+        scale = (1j * self.rho * omega).astype(np.complex128)
+        with torch.cuda.stream(self.dev.stream):
+            self.k.copy_(torch.from_numpy(omega / self.c))
+            self.rhs_scale.copy_(torch.from_numpy(scale))
+            # Neumann data q = i rho omega * unit drives (outside any captured sequence)
+            self.q.copy_(self.rhs_scale[:, None, None] * self.qunit[None, :, :])
+
+    def run(self, freqs, timed: bool = True) -> BatchResult:
+        """Solve up to nf frequencies (padded internally); returns host arrays."""
+        torch = self.dev.torch
+        freqs = np.asarray(freqs, dtype=np.float64)
+        n = len(freqs)
+        self.set_frequencies(freqs)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        with torch.cuda.stream(self.dev.stream):
+            if self.graph is not None:
+                ev[0].record(self.dev.stream)
+                _lib.check(self.dev.lib.bem_graph_launch(self.dev.ctx, self.graph), "bem_graph_launch")
+                ev[3].record(self.dev.stream)
+                ev[1], ev[2] = ev[0], ev[3]
+            else:
+                ev[0].record(self.dev.stream)
+                self._assemble()
+                ev[1].record(self.dev.stream)
+                self._factor_solve()
+                ev[2].record(self.dev.stream)
+                self._field()
+                ev[3].record(self.dev.stream)
+            self.dev.stream.synchronize()
+        t_as = ev[0].elapsed_time(ev[1]) / 1e3
+        t_so = ev[1].elapsed_time(ev[2]) / 1e3
+        t_fi = ev[2].elapsed_time(ev[3]) / 1e3
+        if self.graph is not None:
+            t_as, t_so, t_fi = 0.0, ev[0].elapsed_time(ev[3]) / 1e3, 0.0
+        N = self.N
+        p = self.x[:n, :, :N].cpu().numpy().transpose(0, 2, 1)
+        q = self.q[:n].cpu().numpy().transpose(0, 2, 1)
+        field = self.field[:n].cpu().numpy().transpose(0, 2, 1) if self.n_obs else None
+        return BatchResult(
+            freqs=freqs, pressure=p, q=q, field=field,
+            residual=self.rel[:n].cpu().numpy(), growth=self.growth[:n].cpu().numpy(),
+            iters=self.iters[:n].cpu().numpy() if self.precision == "mixed" else np.zeros(n, np.int32),
+            info=self.info[:n].cpu().numpy(), t_assembly=t_as, t_solve=t_so, t_field=t_fi,
+        )
+
+    def close(self):
+        if self.graph is not None:
+            self.dev.lib.bem_graph_destroy(self.graph)
+            self.graph = None
+
+
+class Engine:
+    """Convenience wrapper: mesh on device + batches per precision."""
+
+    def __init__(self, prep: PreparedMesh, qunit: np.ndarray, dev: Device | None = None):
+        self.dev = dev or Device(0)
+        self.prep = prep
+        self.qunit = np.asarray(qunit, dtype=np.complex128).reshape(prep.n_faces, -1)
+        src = np.nonzero(np.any(self.qunit != 0, axis=1))[0]
+        self.dmesh = DeviceMesh(self.dev, prep, src)
+
+    def batch(self, precision, nf, obs=None, rho=1.21, sound_speed=343.0, **kw) -> Batch:
+        return Batch(self.dmesh, precision, nf, self.qunit, obs, rho, sound_speed, **kw)
+
+    def solve(self, freqs, precision="mixed", batch=None, obs=None, rho=1.21, sound_speed=343.0, graph=False,
+              **kw):
+        freqs = np.asarray(freqs, dtype=np.float64)
+        nb = len(freqs) if batch is None else int(batch)
+        b = self.batch(precision, nb, obs, rho, sound_speed, **kw)
+        if graph:
+            b.capture()
+        out = []
+        t0 = time.perf_counter()
+        for i in range(0, len(freqs), nb):
+            out.append(b.run(freqs[i : i + nb]))
+        wall = time.perf_counter() - t0
+        b.close()
+        return out, wall
+
+    def close(self):
+        self.dmesh.close()

@@ -1,0 +1,466 @@
+"""Host-side mesh preparation for the HIP solver (once per mesh, vectorised NumPy).
+
+Restates the reference's mesh / space / quadrature set-up in the layout the gfx950 kernels consume:
+
+* ``BoundaryMesh``           BeatEngineCore.jl:122-130,236-256 (normals from cross(v2-v1, v3-v1), no welding)
+* Gmsh 2.2 reader            BeatEngineCore.jl:193-234, combined meshes solver.jl:252-269
+* order-4 triangle rule      BeatEngineCore.jl:517-549 (the same tabulated constants)
+* Duffy rules and remaps     BeatEngineCore.jl:551-655,751-771,820-873
+* vertex-sharing pair cache  BeatEngineCore.jl:441-479,784-811,875-915
+* surface curls              BeatEngineCore.jl:685-702
+* mass blocks                BeatEngineCore.jl:731-749
+
+New for the HIP layout: trial elements are put in Morton (Z-curve) order of their centroids and cut into
+chunks of 64 (one lane each). Each chunk's distinct vertices are "slots"; the regular kernel writes one value
+per (test-element row, slot) and the row gather maps slots back to P1 columns (``cs_ptr/cs_slot``) and P1
+rows back to test-element rows (``vstar_ptr/vstar_row``).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# ---------------------------------------------------------------------------------------------
+# Mesh
+# ---------------------------------------------------------------------------------------------
+
+
+@dataclass
+class BoundaryMesh:
+    vertices: np.ndarray  # (V,3) float64 [m]
+    faces: np.ndarray  # (F,3) int64 0-based
+    tags: np.ndarray  # (F,) physical tags
+    mesh_ids: np.ndarray = None  # (F,) 1-based input-mesh id
+    normals: np.ndarray = field(init=False)
+    areas: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        self.vertices = np.ascontiguousarray(self.vertices, dtype=np.float64)
+        self.faces = np.ascontiguousarray(self.faces, dtype=np.int64)
+        self.tags = np.asarray(self.tags, dtype=np.int64)
+        if self.mesh_ids is None:
+            self.mesh_ids = np.ones(len(self.faces), dtype=np.int64)
+        fv = self.vertices[self.faces]
+        cross = np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0])
+        norm = np.linalg.norm(cross, axis=1)
+        if np.any(norm <= 0.0):
+            raise ValueError("mesh contains degenerate (zero-area) triangles")
+        self.areas = norm / 2.0
+        self.normals = cross / norm[:, None]
+        self.face_vertices = fv
+
+    @property
+    def n_faces(self) -> int:
+        return int(self.faces.shape[0])
+
+    @property
+    def n_vertices(self) -> int:
+        return int(self.vertices.shape[0])
+
+
+def read_gmsh22(path, scale: float = 1.0, translation=(0.0, 0.0, 0.0)):
+    """Gmsh 2.2 ASCII: nodes and type-2 (triangle) elements; tag = first tag token (BeatEngineCore.jl:193-234)."""
+    with open(path, "r", encoding="utf-8") as handle:
+        lines = handle.read().splitlines()
+    stripped = [line.strip() for line in lines]
+    try:
+        ns, ne = stripped.index("$Nodes"), stripped.index("$EndNodes")
+        es, ee = stripped.index("$Elements"), stripped.index("$EndElements")
+    except ValueError as exc:
+        raise ValueError("Only Gmsh 2.2 ASCII meshes with Nodes and Elements sections are supported.") from exc
+    node_index = {}
+    coords = []
+    for line in lines[ns + 2 : ne]:
+        parts = line.split()
+        if len(parts) < 4:
+            continue
+        node_index[int(parts[0])] = len(coords)
+        coords.append((float(parts[1]) * scale, float(parts[2]) * scale, float(parts[3]) * scale))
+    faces, tags = [], []
+    for line in lines[es + 2 : ee]:
+        parts = line.split()
+        if len(parts) < 8 or parts[1] != "2":
+            continue
+        a = node_index.get(int(parts[-3]))
+        b = node_index.get(int(parts[-2]))
+        c = node_index.get(int(parts[-1]))
+        if a is None or b is None or c is None:
+            continue
+        faces.append((a, b, c))
+        tags.append(int(parts[3]))
+    v = np.asarray(coords, dtype=np.float64).reshape(-1, 3) + np.asarray(translation, dtype=np.float64)
+    return v, np.asarray(faces, dtype=np.int64).reshape(-1, 3), np.asarray(tags, dtype=np.int64)
+
+
+def combine(parts) -> BoundaryMesh:
+    """Concatenate (V, F, tags) parts without welding; face mesh ids are 1-based part indices."""
+    vs, fs, ts, ids = [], [], [], []
+    off = 0
+    for i, (v, f, t) in enumerate(parts, start=1):
+        vs.append(np.asarray(v, dtype=np.float64))
+        fs.append(np.asarray(f, dtype=np.int64) + off)
+        ts.append(np.asarray(t, dtype=np.int64))
+        ids.append(np.full(len(f), i, dtype=np.int64))
+        off += len(v)
+    return BoundaryMesh(np.concatenate(vs), np.concatenate(fs), np.concatenate(ts), np.concatenate(ids))
+
+
+# ---------------------------------------------------------------------------------------------
+# Quadrature
+# ---------------------------------------------------------------------------------------------
+
+# Order-4 six-point rule on the reference triangle (xi, eta); weights sum to 1/2.
+RULE_POINTS = np.array(
+    [
+        [0.4459484909159651, 0.4459484909159651],
+        [0.0915762135097710, 0.0915762135097700],
+        [0.1081030181680700, 0.4459484909159651],
+        [0.4459484909159651, 0.1081030181680700],
+        [0.8168475729804590, 0.0915762135097700],
+        [0.0915762135097710, 0.8168475729804580],
+    ]
+)
+RULE_WEIGHTS = 0.5 * np.array(
+    [0.2233815896780110, 0.1099517436553220, 0.2233815896780110, 0.2233815896780110, 0.1099517436553220,
+     0.1099517436553220]
+)
+
+
+def p1_basis(points):
+    p = np.asarray(points, dtype=np.float64)
+    return np.stack([1.0 - p[..., 0] - p[..., 1], p[..., 0], p[..., 1]], axis=-1)
+
+
+def gauss01(order: int):
+    """Gauss-Legendre nodes/weights on [0,1] for orders 1..4."""
+    if order == 1:
+        return np.array([0.5]), np.array([1.0])
+    if order == 2:
+        a = 0.5 / math.sqrt(3.0)
+        return np.array([0.5 - a, 0.5 + a]), np.array([0.5, 0.5])
+    if order == 3:
+        a = math.sqrt(0.6) / 2.0
+        return np.array([0.5 - a, 0.5, 0.5 + a]), np.array([5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0])
+    if order == 4:
+        inner = 2.0 / 7.0 * math.sqrt(1.2)
+        x1 = math.sqrt(3.0 / 7.0 - inner) / 2.0
+        x2 = math.sqrt(3.0 / 7.0 + inner) / 2.0
+        w1 = (18.0 + math.sqrt(30.0)) / 72.0
+        w2 = (18.0 - math.sqrt(30.0)) / 72.0
+        return np.array([0.5 - x2, 0.5 - x1, 0.5 + x1, 0.5 + x2]), np.array([w2, w1, w1, w2])
+    raise ValueError("singular_order must be 1..4")
+
+
+def duffy_base(order: int, kind: int):
+    """Duffy rule on T x T for kind 1 (coincident, 6 maps), 2 (edge, 5), 3 (vertex, 2).
+
+    Returns (test (n,2), trial (n,2), w (n,)). The simplex maps are those of the reference
+    (BeatEngineCore.jl:590-620); points are converted with (x, y) -> (x - y, y).
+    """
+    x, w = gauss01(order)
+    # tensor grid: outer index i -> second coordinate, inner j -> first coordinate
+    gx = np.tile(x, len(x))
+    gy = np.repeat(x, len(x))
+    gw = np.repeat(w, len(x)) * np.tile(w, len(x))
+    xsi = np.repeat(gx, len(gx))
+    e1 = np.repeat(gy, len(gx))
+    e2 = np.tile(gx, len(gx))
+    e3 = np.tile(gy, len(gx))
+    base = np.repeat(gw, len(gx)) * np.tile(gw, len(gx))
+    e12 = e1 * e2
+    e123 = e12 * e3
+    one = np.ones_like(xsi)
+    if kind == 1:
+        wt = base * xsi**3 * e1**2 * e2
+        maps = [
+            (xsi, xsi * (1 - e1 + e12), xsi * (1 - e123), xsi * (1 - e1), wt),
+            (xsi * (1 - e123), xsi * (1 - e1), xsi, xsi * (1 - e1 + e12), wt),
+            (xsi, xsi * (e1 - e12 + e123), xsi * (1 - e12), xsi * (e1 - e12), wt),
+            (xsi * (1 - e12), xsi * (e1 - e12), xsi, xsi * (e1 - e12 + e123), wt),
+            (xsi * (1 - e123), xsi * (e1 - e123), xsi, xsi * (e1 - e12), wt),
+            (xsi, xsi * (e1 - e12), xsi * (1 - e123), xsi * (e1 - e123), wt),
+        ]
+    elif kind == 2:
+        wt = base * xsi**3 * e1**2
+        maps = [
+            (xsi, xsi * e1 * e3, xsi * (1 - e12), xsi * e1 * (1 - e2), wt),
+            (xsi, xsi * e1, xsi * (1 - e123), xsi * e12 * (1 - e3), wt * e2),
+            (xsi * (1 - e12), xsi * e1 * (1 - e2), xsi * one, xsi * e123, wt * e2),
+            (xsi * (1 - e123), xsi * e12 * (1 - e3), xsi * one, xsi * e1, wt * e2),
+            (xsi * (1 - e123), xsi * e1 * (1 - e2 * e3), xsi * one, xsi * e12, wt * e2),
+        ]
+    elif kind == 3:
+        wt = base * xsi**3 * e2
+        maps = [
+            (xsi, xsi * e1, xsi * e2, xsi * e2 * e3, wt),
+            (xsi * e2, xsi * e2 * e3, xsi * one, xsi * e1, wt),
+        ]
+    else:
+        raise ValueError(kind)
+    # interleave maps per tensor point (point-major, map-minor) like the reference's push order
+    tx = np.stack([m[0] for m in maps], axis=1).reshape(-1)
+    ty = np.stack([m[1] for m in maps], axis=1).reshape(-1)
+    sx = np.stack([m[2] for m in maps], axis=1).reshape(-1)
+    sy = np.stack([m[3] for m in maps], axis=1).reshape(-1)
+    ww = np.stack([m[4] for m in maps], axis=1).reshape(-1)
+    return np.stack([tx - ty, ty], axis=1), np.stack([sx - sy, sy], axis=1), ww
+
+
+_REF = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+
+
+def remap_vertex(p, vid):
+    """Move the Duffy singular corner (origin) to local vertex vid (0-based)."""
+    if vid == 0:
+        return p.copy()
+    if vid == 1:
+        return np.stack([1.0 - p[:, 0] - p[:, 1], p[:, 1]], axis=1)
+    return np.stack([p[:, 0], 1.0 - p[:, 0] - p[:, 1]], axis=1)
+
+
+def remap_edge(p, va, vb):
+    """Affine map sending the reference edge (0,0)-(1,0) to local vertices va -> vb (0-based)."""
+    vr = 3 - va - vb
+    o, a, b = _REF[va], _REF[vb], _REF[vr]
+    return o[None, :] + p[:, :1] * (a - o)[None, :] + p[:, 1:2] * (b - o)[None, :]
+
+
+def mass_blocks():
+    b = p1_basis(RULE_POINTS)
+    m11 = np.einsum("qi,qj,q->ij", b, b, RULE_WEIGHTS)
+    m10 = b.T @ RULE_WEIGHTS
+    return m11, m10
+
+
+# ---------------------------------------------------------------------------------------------
+# Per-face geometry
+# ---------------------------------------------------------------------------------------------
+
+
+def surface_curls(face_vertices, normals):
+    """n x grad(phi_i) for the three P1 functions via the inverse Gram matrix. (F,3,3)."""
+    e1 = face_vertices[:, 1] - face_vertices[:, 0]
+    e2 = face_vertices[:, 2] - face_vertices[:, 0]
+    a = (e1 * e1).sum(1)
+    b = (e1 * e2).sum(1)
+    d = (e2 * e2).sum(1)
+    det = a * d - b * b
+    g1 = (e1 * d[:, None] - e2 * b[:, None]) / det[:, None]
+    g2 = (e2 * a[:, None] - e1 * b[:, None]) / det[:, None]
+    grads = np.stack([-g1 - g2, g1, g2], axis=1)
+    return np.cross(normals[:, None, :], grads)
+
+
+def quadrature_points(face_vertices):
+    """(F,6,3) order-4 rule points."""
+    return np.einsum("ql,flc->fqc", p1_basis(RULE_POINTS), face_vertices)
+
+
+# ---------------------------------------------------------------------------------------------
+# Singular pairs
+# ---------------------------------------------------------------------------------------------
+
+
+@dataclass
+class SingularPairs:
+    test: np.ndarray
+    trial: np.ndarray
+    rule: np.ndarray
+    ptr: np.ndarray  # (F+1,) CSR by test element
+    rule_off: np.ndarray
+    rule_len: np.ndarray
+    rule_pts: np.ndarray  # (n,5)
+    keys: list
+
+
+def _vertex_incidence(faces, n_vertices):
+    F = len(faces)
+    v = faces.reshape(-1)
+    e = np.repeat(np.arange(F), 3)
+    o = np.argsort(v, kind="stable")
+    ptr = np.zeros(n_vertices + 1, dtype=np.int64)
+    np.add.at(ptr, v + 1, 1)
+    return np.cumsum(ptr), e[o], v[o]
+
+
+def singular_pairs(mesh: BoundaryMesh, order: int = 4) -> SingularPairs:
+    """All (test, trial) pairs sharing at least one vertex id (incl. t == s), with remapped Duffy rules."""
+    faces = mesh.faces
+    F = len(faces)
+    ptr, inc, _ = _vertex_incidence(faces, mesh.n_vertices)
+    counts = ptr[faces + 1] - ptr[faces]  # (F,3)
+    tot = counts.sum(1)
+    t_idx = np.repeat(np.arange(F), tot)
+    starts = ptr[faces].reshape(-1)
+    lens = counts.reshape(-1)
+    seg = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens)
+    s_idx = inc[np.arange(lens.sum()) + seg]
+    key = np.unique(t_idx * F + s_idx)
+    t, s = key // F, key % F
+    ft, fs = faces[t], faces[s]
+    match = ft[:, :, None] == fs[:, None, :]  # (P,3,3) [i test, j trial]
+    nshared = match.sum(axis=(1, 2))
+    kinds = np.where(nshared >= 3, 1, np.where(nshared == 2, 2, 3))
+    # edge orientation: list matches in (i, j) scan order, then sort the pair by trial index j
+    flat = match.reshape(len(t), 9)
+    order_ij = np.argsort(~flat, axis=1, kind="stable")[:, :2]  # first two matches in scan order
+    ia, ja = order_ij[:, 0] // 3, order_ij[:, 0] % 3
+    ib, jb = order_ij[:, 1] // 3, order_ij[:, 1] % 3
+    swap = jb < ja
+    ia2, ib2 = np.where(swap, ib, ia), np.where(swap, ia, ib)
+    ja2, jb2 = np.where(swap, jb, ja), np.where(swap, ja, jb)
+    codes = np.where(
+        kinds == 1,
+        100000,
+        np.where(kinds == 2, 200000 + ia2 * 1000 + ib2 * 100 + ja2 * 10 + jb2, 300000 + ia * 1000 + ja * 10),
+    )
+    uniq, rule_of_pair = np.unique(codes, return_inverse=True)
+    bases = {k: duffy_base(order, k) for k in (1, 2, 3)}
+    pts, offs, lens_r, keys = [], [], [], []
+    off = 0
+    for code in uniq:
+        kind = int(code // 100000)
+        test_p, trial_p, w = bases[kind]
+        if kind == 2:
+            a, b, c, d = (int(code // 1000) % 10, int(code // 100) % 10, int(code // 10) % 10, int(code) % 10)
+            test_p, trial_p = remap_edge(test_p, a, b), remap_edge(trial_p, c, d)
+            keys.append((2, a, b, c, d))
+        elif kind == 3:
+            a, c = int(code // 1000) % 10, int(code // 10) % 10
+            test_p, trial_p = remap_vertex(test_p, a), remap_vertex(trial_p, c)
+            keys.append((3, a, -1, c, -1))
+        else:
+            keys.append((1, -1, -1, -1, -1))
+        pts.append(np.column_stack([test_p, trial_p, w]))
+        offs.append(off)
+        lens_r.append(len(w))
+        off += len(w)
+    ptr_t = np.zeros(F + 1, dtype=np.int64)
+    np.add.at(ptr_t, t + 1, 1)
+    return SingularPairs(
+        test=t.astype(np.int32),
+        trial=s.astype(np.int32),
+        rule=rule_of_pair.astype(np.int32),
+        ptr=np.cumsum(ptr_t).astype(np.int32),
+        rule_off=np.asarray(offs, dtype=np.int32),
+        rule_len=np.asarray(lens_r, dtype=np.int32),
+        rule_pts=np.ascontiguousarray(np.concatenate(pts), dtype=np.float64),
+        keys=keys,
+    )
+
+
+# ---------------------------------------------------------------------------------------------
+# Locality order and slot tables
+# ---------------------------------------------------------------------------------------------
+
+
+def _morton(points, bits=10):
+    lo = points.min(0)
+    span = np.maximum(points.max(0) - lo, 1e-30)
+    q = np.minimum(((points - lo) / span * (2**bits - 1)).astype(np.int64), 2**bits - 1)
+    code = np.zeros(len(points), dtype=np.int64)
+    for b in range(bits):
+        for a in range(3):
+            code |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return code
+
+
+@dataclass
+class SlotTables:
+    order: np.ndarray  # (n_chunks*64,) int32
+    slot_count: np.ndarray
+    slot_start: np.ndarray  # (n_chunks, 196) uint16
+    slot_inc: np.ndarray  # (n_chunks, 192) uint16
+    slot_base: np.ndarray
+    strip_len: int
+    cs_ptr: np.ndarray
+    cs_slot: np.ndarray
+    vstar_ptr: np.ndarray
+    vstar_row: np.ndarray
+
+
+def slot_tables(mesh: BoundaryMesh) -> SlotTables:
+    F, V = mesh.n_faces, mesh.n_vertices
+    centroids = mesh.face_vertices.mean(axis=1)
+    perm = np.argsort(_morton(centroids), kind="stable")
+    n_chunks = (F + 63) // 64
+    order = np.full(n_chunks * 64, -1, dtype=np.int32)
+    order[:F] = perm
+    slot_count = np.zeros(n_chunks, dtype=np.int32)
+    slot_start = np.zeros((n_chunks, 196), dtype=np.uint16)
+    slot_inc = np.zeros((n_chunks, 192), dtype=np.uint16)
+    slot_base = np.zeros(n_chunks, dtype=np.int32)
+    slot_vertex = []
+    base = 0
+    for c in range(n_chunks):
+        elems = order[c * 64 : (c + 1) * 64]
+        valid = elems >= 0
+        lanes = np.nonzero(valid)[0]
+        verts = mesh.faces[elems[valid]]  # (n,3)
+        uniq, inv = np.unique(verts.reshape(-1), return_inverse=True)
+        code = (np.repeat(lanes, 3) << 2) | np.tile(np.arange(3), len(lanes))
+        o = np.lexsort((code, inv))
+        slot_inc[c, : len(o)] = code[o]
+        cnt = np.bincount(inv, minlength=len(uniq))
+        slot_start[c, 1 : len(uniq) + 1] = np.cumsum(cnt)
+        slot_count[c] = len(uniq)
+        slot_base[c] = base
+        slot_vertex.append(uniq)
+        base += len(uniq)
+    strip_len = ((base + 63) // 64) * 64
+    sv = np.concatenate(slot_vertex)
+    gs = np.arange(len(sv))
+    o = np.lexsort((gs, sv))
+    cs_ptr = np.zeros(V + 1, dtype=np.int64)
+    np.add.at(cs_ptr, sv + 1, 1)
+    rows = np.arange(3 * F)
+    rv = mesh.faces.reshape(-1)
+    o2 = np.lexsort((rows, rv))
+    vs_ptr = np.zeros(V + 1, dtype=np.int64)
+    np.add.at(vs_ptr, rv + 1, 1)
+    return SlotTables(
+        order=order,
+        slot_count=slot_count,
+        slot_start=slot_start,
+        slot_inc=slot_inc,
+        slot_base=slot_base,
+        strip_len=int(strip_len),
+        cs_ptr=np.cumsum(cs_ptr).astype(np.int32),
+        cs_slot=gs[o].astype(np.int32),
+        vstar_ptr=np.cumsum(vs_ptr).astype(np.int32),
+        vstar_row=rows[o2].astype(np.int32),
+    )
+
+
+# ---------------------------------------------------------------------------------------------
+# Everything the device needs for one mesh
+# ---------------------------------------------------------------------------------------------
+
+
+@dataclass
+class PreparedMesh:
+    mesh: BoundaryMesh
+    curls: np.ndarray
+    qpoints: np.ndarray
+    pairs: SingularPairs
+    slots: SlotTables
+    m11: np.ndarray
+    m10: np.ndarray
+
+    @property
+    def n_faces(self):
+        return self.mesh.n_faces
+
+    @property
+    def n_vertices(self):
+        return self.mesh.n_vertices
+
+
+def prepare(mesh: BoundaryMesh, singular_order: int = 4) -> PreparedMesh:
+    curls = surface_curls(mesh.face_vertices, mesh.normals)
+    qp = quadrature_points(mesh.face_vertices)
+    m11, m10 = mass_blocks()
+    return PreparedMesh(mesh, curls, qp, singular_pairs(mesh, singular_order), slot_tables(mesh), m11, m10)

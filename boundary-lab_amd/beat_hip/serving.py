@@ -1,0 +1,231 @@
+"""Continuous-batching serving loop for the ``blab server`` job path.
+
+The reference serves one job at a time per GPU (the worker protocol is serial behind a lock,
+src/blab/solvers/beat_engine_backend.py:307-334; the server pool defaults to one running job,
+src/blab/server.py:154-185). Here one process-wide scheduler owns the GPU:
+
+* Jobs are parsed and their ``initialized`` metadata returned immediately; device work is queued.
+* Admission by HBM footprint: a job is admitted when a device slot is free and its estimated footprint
+  (mesh tables + strip workspace + batch matrices, which grow with N^2) fits in the remaining budget.
+* Each admitted job runs on its own slot (HIP stream + context), so kernels of different jobs overlap on the
+  device; its mesh, singular tables and factor buffers stay resident between its frequency batches.
+* Frequencies are batched per job (one launch set per batch); jobs whose mesh and drives are identical share
+  one device mesh and are merged into the same batches.
+* Per-job latency (arrival -> last result) is recorded for p50/p99 reporting.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import driver as D
+from . import geometry as G
+
+
+def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, strip_len: int, batch: int, n_obs: int,
+                       precision: str = "mixed", n_ch: int = 1) -> int:
+    """Device bytes a job holds while resident (matrices dominate: 2 x N^2 per frequency in the batch)."""
+    cb = 8 if precision == "mixed" else 16
+    lda = ((n_vertices + 15) // 16) * 16
+    matrices = batch * 2 * n_vertices * lda * cb
+    vectors = batch * 8 * n_ch * lda * 16
+    strip = min(batch, 4) * 3 * n_faces * strip_len * cb
+    singular = batch * n_pairs * 12 * 16
+    mesh = n_faces * (2 * 128 + 6 * 6 * 12 + 200) + n_pairs * 16
+    field_ws = batch * n_ch * 6 * n_faces * 2 * cb + batch * n_obs * n_ch * 16 * 64
+    return int(matrices + vectors + strip + singular + mesh + field_ws)
+
+
+@dataclass
+class JobHandle:
+    job_id: int
+    request: dict
+    arrival: float
+    events: "queue.Queue[dict]" = field(default_factory=queue.Queue)
+    cancel_flag: threading.Event = field(default_factory=threading.Event)
+    t_done: float | None = None
+    t_admit: float | None = None
+    n_results: int = 0
+    footprint: int = 0
+
+    def cancel(self):
+        self.cancel_flag.set()
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+class Scheduler:
+    def __init__(self, hbm_budget_bytes: int | None = None, slots: int = 4, batch: int = 8,
+                 device_index: int = 0, engine_factory=None):
+        self.slots = int(slots)
+        self.batch = int(batch)
+        self.device_index = device_index
+        self._engine_factory = engine_factory
+        self._budget = hbm_budget_bytes
+        self._used = 0
+        self._lock = threading.Condition()
+        self._pending: list[tuple[JobHandle, D.Job, G.PreparedMesh]] = []
+        self._running = 0
+        self._next_id = 0
+        self._threads: list[threading.Thread] = []
+        self._stop = False
+        self._devices = []
+        self.completed: list[JobHandle] = []
+        self._dispatcher = threading.Thread(target=self._dispatch_loop, name="beat-hip-dispatch", daemon=True)
+        self._dispatcher.start()
+
+    # -- public -------------------------------------------------------------------------------
+    def submit(self, request: dict) -> tuple[JobHandle, dict]:
+        """Parse and queue a request; returns the handle and the ``initialized`` payload."""
+        init, parsed = D.parse_request(request)
+        with self._lock:
+            jid = self._next_id
+            self._next_id += 1
+        h = JobHandle(jid, request, time.perf_counter())
+        h.events.put({"type": "initialized", **init})
+        try:
+            job = D.build_job(request, parsed)
+            prep = G.prepare(job.mesh)
+        except Exception as exc:
+            h.events.put({"type": "failed", "error": f"{type(exc).__name__}: {exc}"})
+            h.t_done = time.perf_counter()
+            return h, init
+        nb = max(1, min(self.batch, len(job.frequencies)))
+        h.footprint = estimate_footprint(job.mesh.n_vertices, job.mesh.n_faces, len(prep.pairs.test),
+                                         prep.slots.strip_len, nb, len(job.obs), job.precision,
+                                         len(job.channel_names))
+        with self._lock:
+            self._pending.append((h, job, prep))
+            self._lock.notify_all()
+        return h, init
+
+    def shutdown(self):
+        with self._lock:
+            self._stop = True
+            self._lock.notify_all()
+
+    def budget(self) -> int:
+        if self._budget is None:
+            import torch
+
+            free, total = torch.cuda.mem_get_info(self.device_index)
+            self._budget = int(0.85 * free)
+        return self._budget
+
+    # -- internals ----------------------------------------------------------------------------
+    def _device(self, slot):
+        from .engine import Device
+
+        while len(self._devices) <= slot:
+            self._devices.append(Device(self.device_index))
+        return self._devices[slot]
+
+    def _dispatch_loop(self):
+        free_slots = list(range(self.slots))
+        slot_of = {}
+        while True:
+            with self._lock:
+                while not self._stop:
+                    pick = None
+                    if free_slots and self._pending:
+                        budget = self.budget()
+                        for idx, (h, job, prep) in enumerate(self._pending):
+                            if h.cancel_flag.is_set() or self._used + h.footprint <= budget or self._running == 0:
+                                pick = idx
+                                break
+                    if pick is not None:
+                        break
+                    self._lock.wait(0.05)
+                    # reclaim finished slots
+                    for t in list(slot_of):
+                        if not t.is_alive():
+                            free_slots.append(slot_of.pop(t))
+                if self._stop:
+                    return
+                h, job, prep = self._pending.pop(pick)
+                slot = free_slots.pop(0)
+                self._used += h.footprint
+                self._running += 1
+                h.t_admit = time.perf_counter()
+            th = threading.Thread(target=self._run_job, args=(slot, h, job, prep), name=f"beat-hip-slot{slot}",
+                                  daemon=True)
+            slot_of[th] = slot
+            th.start()
+
+    def _run_job(self, slot, h: JobHandle, job: D.Job, prep: G.PreparedMesh):
+        from .engine import Engine
+
+        eng = None
+        try:
+            if h.cancel_flag.is_set():
+                h.events.put({"type": "cancelled", "solved_count": 0})
+                return
+            dev = self._device(slot)
+            eng = (self._engine_factory or (lambda p, q: Engine(p, q, dev=dev)))(prep, job.qunit)
+            freqs = job.frequencies.astype(np.float64)
+            nb = max(1, min(self.batch, len(freqs)))
+            b = eng.batch(job.precision, nb, job.obs, job.rho, job.sound_speed)
+            solved = 0
+            for i0 in range(0, len(freqs), nb):
+                if h.cancel_flag.is_set() or D.cancelled(job):
+                    h.events.put({"type": "cancelled", "solved_count": solved})
+                    return
+                fr = freqs[i0 : i0 + nb]
+                res = b.run(fr)
+                n = len(fr)
+                for j in range(n):
+                    timings = {"assembly_s": res.t_assembly / n, "solve_s": res.t_solve / n,
+                               "field_s": res.t_field / n}
+                    payload = D.result_payload(job, float(job.frequencies[i0 + j]), res.pressure[j], res.field[j],
+                                               timings, D.diagnostics_for(job, res, j, {"serving_slot": slot}))
+                    solved = i0 + j + 1
+                    h.n_results = solved
+                    h.events.put({"type": "result", "solved_count": solved, "total_count": len(freqs),
+                                  "result": payload})
+            b.close()
+            h.events.put({"type": "completed", "solved_count": len(freqs)})
+        except Exception as exc:
+            h.events.put({"type": "failed", "error": f"{type(exc).__name__}: {exc}"})
+        finally:
+            if eng is not None:
+                eng.close()
+            h.t_done = time.perf_counter()
+            with self._lock:
+                self._used -= h.footprint
+                self._running -= 1
+                self.completed.append(h)
+                self._lock.notify_all()
+
+
+_SCHEDULER = None
+_SCHEDULER_LOCK = threading.Lock()
+
+
+def shared_scheduler(**kw) -> Scheduler:
+    global _SCHEDULER
+    with _SCHEDULER_LOCK:
+        if _SCHEDULER is None:
+            _SCHEDULER = Scheduler(**kw)
+        return _SCHEDULER
+
+
+def percentile(values, q):
+    v = np.sort(np.asarray(values, dtype=np.float64))
+    if v.size == 0:
+        return float("nan")
+    return float(np.percentile(v, q))
+
+
+def mesh_key(mesh: G.BoundaryMesh, qunit: np.ndarray) -> str:
+    h = hashlib.sha1()
+    for a in (mesh.vertices, mesh.faces, qunit):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()

@@ -1,0 +1,484 @@
+// Burton-Miller Galerkin assembly for gfx950.
+//
+// Replaces the reference's colour-pair regular kernels
+//   src/blab/solvers/julia_local/src/BeatEngineRocmPairKernels.jl:28-341 (2*C^2 launches, 4 dense operators)
+// singular block + gather kernels
+//   src/blab/solvers/julia_local/src/BeatEngineRocmSingular.jl:84-374
+// and the LHS broadcast / RHS gemv formation
+//   src/blab/solvers/julia_local/src/BeatEngineRocmOperators.jl:110-121,239-240
+// with four kernels that produce only what the solve needs: A = 1/2 M11 - D + (i/k) H and
+// b = -(S + (i/k)(D* + 1/2 M10)) q for every channel, for a batch of wavenumbers at once.
+//
+// Data flow per frequency f:
+//   k_singular    (FP64)   one wave per vertex-sharing element pair: Duffy-rule blocks of the LHS (9 complex)
+//                          and of the RHS operator (3 complex), identity terms folded into coincident pairs.
+//   k_regular_lhs (T)      one wave per test element t; lanes sweep trial elements in locality-ordered chunks of
+//                          64. Each lane integrates its (t,s) pair with the 6x6 order-4 rule into a 3x3 block
+//                          (or picks up the FP64 singular block for adjacent pairs). Blocks go to LDS, then lanes
+//                          switch to "slot" ownership: each slot (a trial vertex of the chunk) sums the block
+//                          columns that touch it, in a fixed order, and stores one value per test-element row.
+//                          Every strip entry is written exactly once: no atomics, no colouring, deterministic.
+//   k_rhs_part    (T)      one wave per test element: sum over radiating faces of the RHS-operator blocks times
+//                          the unit channel drives.
+//   k_gather      (T)      one block per P1 row i: A[i, j] = sum over (t,l) in star(i), slots of column j.
+//
+// Regular-pair arithmetic follows BeatEngineCpuAssembly.jl:278-373 / BeatEngineRocmPairKernels.jl:100-150:
+//   G = e^{ikr}/(4 pi r), grad = G (ik - 1/r) r_hat, D uses the trial normal, D* uses -r.n_test,
+//   H = (curl_t.curl_s - k^2 phi psi n_t.n_s) G, weight w_a w_b 4 A_t A_s, pairs sharing a vertex are singular.
+// The 3x3 block is contracted in two stages using the tensor structure of the rule:
+//   inner over trial points b with constants psi_m(b) w_b, outer over test points a with phi_l(a) w_a.
+
+#include "bem_kernels.hpp"
+
+#define SLOT_STRIDE BEM_SLOT_STRIDE
+#define MAX_CH BEM_MAX_CH
+
+
+
+// 3x3 LHS block of a regular (t,s) pair, scaled and combined: C = J/(4pi) (-D + (i/k) H).
+template <typename T>
+__device__ __forceinline__ void regular_block(const T (&x)[18], const T (&nt)[3], const T (&ct)[9], T a2t,
+                                              const ElemRec<T>& se, T k, T kappa, cplx<T> (&C)[9]) {
+  T y[18];
+#pragma unroll
+  for (int i = 0; i < 18; ++i) y[i] = se.q[i];
+  const T nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
+  T DMr[9], DMi[9], GMr[9], GMi[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) DMr[i] = DMi[i] = GMr[i] = GMi[i] = T(0);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    T tDr[3] = {0, 0, 0}, tDi[3] = {0, 0, 0}, tGr[3] = {0, 0, 0}, tGi[3] = {0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const T dx = y[3 * b + 0] - x[3 * a + 0];
+      const T dy = y[3 * b + 1] - x[3 * a + 1];
+      const T dz = y[3 * b + 2] - x[3 * a + 2];
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      const T rinv = Phase<T>::rsqrt_(r2);
+      const T r = r2 * rinv;
+      T sn, cs;
+      Phase<T>::sincos_kr(kappa, k, r, &sn, &cs);
+      const T gr = cs * rinv, gi = sn * rinv;  // 4 pi G
+      const T hr = -gr * rinv - gi * k;        // 4 pi G (ik - 1/r)
+      const T hi = gr * k - gi * rinv;
+      const T td = (dx * nsx + dy * nsy + dz * nsz) * rinv;
+      const T dgr = hr * td, dgi = hi * td;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const T cw = T(rule_phi(m, b) * rule_w(b));
+        tDr[m] += cw * dgr;
+        tDi[m] += cw * dgi;
+        tGr[m] += cw * gr;
+        tGi[m] += cw * gi;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const T wa = T(rule_phi(l, a) * rule_w(a));
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        DMr[3 * l + m] += wa * tDr[m];
+        DMi[3 * l + m] += wa * tDi[m];
+        GMr[3 * l + m] += wa * tGr[m];
+        GMi[3 * l + m] += wa * tGi[m];
+      }
+    }
+  }
+  T gsr = 0, gsi = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    gsr += GMr[i];
+    gsi += GMi[i];
+  }
+  const T nn = nt[0] * nsx + nt[1] * nsy + nt[2] * nsz;
+  const T k2nn = k * k * nn;
+  const T scale = a2t * se.a2 * T(BEM_INV_FOUR_PI);
+  const T kinv = T(1) / k;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const T cp = ct[3 * l + 0] * se.curl[3 * m + 0] + ct[3 * l + 1] * se.curl[3 * m + 1] +
+                   ct[3 * l + 2] * se.curl[3 * m + 2];
+      const T Hr = cp * gsr - k2nn * GMr[3 * l + m];
+      const T Hi = cp * gsi - k2nn * GMi[3 * l + m];
+      // -D + (i/k) H
+      C[3 * l + m] = mk<T>(scale * (-DMr[3 * l + m] - Hi * kinv), scale * (-DMi[3 * l + m] + Hr * kinv));
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
+  const int t = blockIdx.x;
+  const int f = blockIdx.y;
+  const int lane = threadIdx.x;
+  __shared__ cplx<T> blk[64][9];
+
+  const double kd = p.k[f];
+  const T k = T(kd);
+  const T kappa = T(kd / (2.0 * BEM_PI));
+  const ElemRec<T>* te = p.rec + t;
+  T x[18], nt[3], ct[9];
+#pragma unroll
+  for (int i = 0; i < 18; ++i) x[i] = te->q[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) nt[i] = te->n[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) ct[i] = te->curl[i];
+  const T a2t = te->a2;
+  const int4 tv = p.fv[t];
+  const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
+  const cplx<double>* sl = p.sing_lhs + (size_t)f * p.P * 9;
+  cplx<T>* strip = p.strip + (size_t)f * p.strip_fstride;
+  cplx<T>* row0 = strip + (size_t)(3 * t + 0) * p.L;
+  cplx<T>* row1 = strip + (size_t)(3 * t + 1) * p.L;
+  cplx<T>* row2 = strip + (size_t)(3 * t + 2) * p.L;
+
+  for (int c = 0; c < p.n_chunks; ++c) {
+    const int s = p.order[c * 64 + lane];
+    cplx<T> C[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C[i] = mk<T>(0, 0);
+    if (s >= 0) {
+      const int4 sv = p.fv[s];
+      const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
+                       sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
+      if (!adj) {
+        regular_block<T>(x, nt, ct, a2t, p.rec[s], k, kappa, C);
+      } else {
+        for (int q = sp0; q < sp1; ++q) {
+          if (p.sing_trial[q] == s) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+              const cplx<double> v = sl[(size_t)q * 9 + i];
+              C[i] = mk<T>(T(v.re), T(v.im));
+            }
+            break;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) blk[lane][i] = C[i];
+    __syncthreads();
+    const int nsl = p.slot_count[c];
+    const uint16_t* st = p.slot_start + (size_t)c * SLOT_STRIDE;
+    const uint16_t* inc = p.slot_inc + (size_t)c * 192;
+    const int base = p.slot_base[c];
+    for (int kk = lane; kk < nsl; kk += 64) {
+      cplx<T> a0 = mk<T>(0, 0), a1 = mk<T>(0, 0), a2 = mk<T>(0, 0);
+      const int e1 = st[kk + 1];
+      for (int e = st[kk]; e < e1; ++e) {
+        const int code = inc[e];
+        const int src = code >> 2, m = code & 3;
+        a0 = cadd(a0, blk[src][m]);
+        a1 = cadd(a1, blk[src][3 + m]);
+        a2 = cadd(a2, blk[src][6 + m]);
+      }
+      row0[base + kk] = a0;
+      row1[base + kk] = a1;
+      row2[base + kk] = a2;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Singular (vertex-sharing) pairs, FP64 Duffy rules: BeatEngineCpuAssembly.jl:151-218,
+// BeatEngineRocmSingular.jl:84-289. Identity terms 1/2 M11 and 1/2 (i/k) M10 are added to the
+// coincident pair of each element (BeatEngineCore.jl:731-749 with the regular rule).
+// ------------------------------------------------------------------------------------------------
+
+
+__global__ __launch_bounds__(64) void k_singular(SingParams p) {
+  const int pi = blockIdx.x;
+  const int f = blockIdx.y;
+  const int lane = threadIdx.x;
+  const int t = p.test[pi], s = p.trial[pi], ru = p.rule[pi];
+  const double k = p.k[f];
+  double tv[9], sv[9], nt[3], ns[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    tv[i] = p.fvtx[9 * t + i];
+    sv[i] = p.fvtx[9 * s + i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    nt[i] = p.fnrm[3 * t + i];
+    ns[i] = p.fnrm[3 * s + i];
+  }
+  double Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
+  double Dr[9], Di[9], Gr[9], Gi[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Dr[i] = Di[i] = Gr[i] = Gi[i] = 0.0;
+  const int off = p.rule_off[ru], len = p.rule_len[ru];
+  for (int q = lane; q < len; q += 64) {
+    const double* rp = p.rpts + (size_t)(off + q) * 5;
+    const double txi = rp[0], tet = rp[1], sxi = rp[2], set = rp[3], w = rp[4];
+    const double bt[3] = {1.0 - txi - tet, txi, tet};
+    const double bs[3] = {1.0 - sxi - set, sxi, set};
+    double xx[3], yy[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xx[c] = bt[0] * tv[c] + bt[1] * tv[3 + c] + bt[2] * tv[6 + c];
+      yy[c] = bs[0] * sv[c] + bs[1] * sv[3 + c] + bs[2] * sv[6 + c];
+    }
+    const double dx = yy[0] - xx[0], dy = yy[1] - xx[1], dz = yy[2] - xx[2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r == 0.0) continue;
+    const double rinv = 1.0 / r;
+    double sn, cs;
+    sincos(k * r, &sn, &cs);
+    const double gr = cs * rinv * w, gi = sn * rinv * w;  // 4 pi G w
+    const double hr = -gr * rinv - gi * k, hi = gr * k - gi * rinv;
+    const double td = (dx * ns[0] + dy * ns[1] + dz * ns[2]) * rinv;
+    const double ad = -(dx * nt[0] + dy * nt[1] + dz * nt[2]) * rinv;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      Sr[l] += bt[l] * gr;
+      Si[l] += bt[l] * gi;
+      Ar[l] += bt[l] * hr * ad;
+      Ai[l] += bt[l] * hi * ad;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double bb = bt[l] * bs[m];
+        Dr[3 * l + m] += bb * hr * td;
+        Di[3 * l + m] += bb * hi * td;
+        Gr[3 * l + m] += bb * gr;
+        Gi[3 * l + m] += bb * gi;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Sr[i] = wave_sum(Sr[i]);
+    Si[i] = wave_sum(Si[i]);
+    Ar[i] = wave_sum(Ar[i]);
+    Ai[i] = wave_sum(Ai[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    Dr[i] = wave_sum(Dr[i]);
+    Di[i] = wave_sum(Di[i]);
+    Gr[i] = wave_sum(Gr[i]);
+    Gi[i] = wave_sum(Gi[i]);
+  }
+  if (lane != 0) return;
+  const double scale = p.fa2[t] * p.fa2[s] * BEM_INV_FOUR_PI;
+  const double kinv = 1.0 / k;
+  double gsr = 0, gsi = 0;
+  for (int i = 0; i < 9; ++i) {
+    gsr += Gr[i];
+    gsi += Gi[i];
+  }
+  const double nn = nt[0] * ns[0] + nt[1] * ns[1] + nt[2] * ns[2];
+  const double* ctp = p.fcurl + 9 * t;
+  const double* csp = p.fcurl + 9 * s;
+  cplx<double>* out = p.lhs + ((size_t)f * p.P + pi) * 9;
+  const bool self = (t == s);
+  const double a2 = p.fa2[t];
+  for (int l = 0; l < 3; ++l) {
+    for (int m = 0; m < 3; ++m) {
+      const double cp = ctp[3 * l] * csp[3 * m] + ctp[3 * l + 1] * csp[3 * m + 1] + ctp[3 * l + 2] * csp[3 * m + 2];
+      const double Hr = cp * gsr - k * k * nn * Gr[3 * l + m];
+      const double Hi = cp * gsi - k * k * nn * Gi[3 * l + m];
+      double re = scale * (-Dr[3 * l + m] - Hi * kinv);
+      double im = scale * (-Di[3 * l + m] + Hr * kinv);
+      if (self) re += 0.5 * p.m11[3 * l + m] * a2;
+      out[3 * l + m] = mk<double>(re, im);
+    }
+  }
+  cplx<double>* ro = p.rhs + ((size_t)f * p.P + pi) * 3;
+  for (int l = 0; l < 3; ++l) {
+    // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair
+    double re = scale * (-Sr[l] + Ai[l] * kinv);
+    double im = scale * (-Si[l] - Ar[l] * kinv);
+    if (self) im -= 0.5 * kinv * p.m10[l] * a2;
+    ro[l] = mk<double>(re, im);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// RHS partial sums per test element: rhs_part[f][3t+l][c] = sum_s B_ts[l] * q_unit[c][s]
+// ------------------------------------------------------------------------------------------------
+
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
+  const int t = blockIdx.x;
+  const int f = blockIdx.y;
+  const int lane = threadIdx.x;
+  const double kd = p.k[f];
+  const T k = T(kd);
+  const T kappa = T(kd / (2.0 * BEM_PI));
+  const ElemRec<T>* te = p.rec + t;
+  const int4 tv = p.fv[t];
+  const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
+  double accr[3][MAX_CH], acci[3][MAX_CH];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) accr[l][c] = acci[l][c] = 0.0;
+
+  for (int j = lane; j < p.n_src; j += 64) {
+    const int s = p.src[j];
+    const int4 sv = p.fv[s];
+    const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
+                     sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
+    double br[3] = {0, 0, 0}, bi[3] = {0, 0, 0};
+    if (!adj) {
+      const ElemRec<T>& se = p.rec[s];
+      T Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
+      const T ntx = te->n[0], nty = te->n[1], ntz = te->n[2];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        T sgr = 0, sgi = 0, sar = 0, sai = 0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+          const T dx = se.q[3 * b] - te->q[3 * a];
+          const T dy = se.q[3 * b + 1] - te->q[3 * a + 1];
+          const T dz = se.q[3 * b + 2] - te->q[3 * a + 2];
+          const T r2 = dx * dx + dy * dy + dz * dz;
+          const T rinv = Phase<T>::rsqrt_(r2);
+          const T r = r2 * rinv;
+          T sn, cs;
+          Phase<T>::sincos_kr(kappa, k, r, &sn, &cs);
+          const T gr = cs * rinv, gi = sn * rinv;
+          const T hr = -gr * rinv - gi * k, hi = gr * k - gi * rinv;
+          const T ad = -(dx * ntx + dy * nty + dz * ntz) * rinv;
+          const T wb = T(rule_w(b));
+          sgr += wb * gr;
+          sgi += wb * gi;
+          sar += wb * hr * ad;
+          sai += wb * hi * ad;
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          const T wa = T(rule_phi(l, a) * rule_w(a));
+          Sr[l] += wa * sgr;
+          Si[l] += wa * sgi;
+          Ar[l] += wa * sar;
+          Ai[l] += wa * sai;
+        }
+      }
+      const double scale = double(te->a2) * double(se.a2) * BEM_INV_FOUR_PI;
+      const double kinv = 1.0 / kd;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        br[l] = scale * (-double(Sr[l]) + double(Ai[l]) * kinv);
+        bi[l] = scale * (-double(Si[l]) - double(Ar[l]) * kinv);
+      }
+    } else {
+      const cplx<double>* sr = p.sing_rhs + (size_t)f * p.P * 3;
+      for (int q = sp0; q < sp1; ++q) {
+        if (p.sing_trial[q] == s) {
+          for (int l = 0; l < 3; ++l) {
+            br[l] = sr[(size_t)q * 3 + l].re;
+            bi[l] = sr[(size_t)q * 3 + l].im;
+          }
+          break;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) {
+      if (c < p.n_ch) {
+        const cplx<double> qv = p.qunit[(size_t)c * p.F + s];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          accr[l][c] += br[l] * qv.re - bi[l] * qv.im;
+          acci[l][c] += br[l] * qv.im + bi[l] * qv.re;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) {
+      accr[l][c] = wave_sum(accr[l][c]);
+      acci[l][c] = wave_sum(acci[l][c]);
+    }
+  if (lane == 0) {
+    cplx<double>* o = p.out + ((size_t)f * 3 * p.F + 3 * t) * MAX_CH;
+    for (int l = 0; l < 3; ++l)
+      for (int c = 0; c < MAX_CH; ++c) o[l * MAX_CH + c] = mk<double>(accr[l][c], acci[l][c]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Row gather: A[i][j] = sum_{(t,l) in star(i)} sum_{slot in cols(j)} strip[3t+l][slot]
+//             b[c][i] = scale_f * sum_{(t,l) in star(i)} rhs_part[3t+l][c]
+// ------------------------------------------------------------------------------------------------
+
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
+  const int i = blockIdx.x;
+  const int f = blockIdx.y;
+  const int tid = threadIdx.x;
+  __shared__ int rows[32];  // vertex valence <= 32 is checked at mesh upload
+  __shared__ double red[256];
+  const int s0 = g.vstar_ptr[i], s1 = g.vstar_ptr[i + 1];
+  const int ns = min(s1 - s0, 32);
+  if (tid < ns) rows[tid] = g.vstar_row[s0 + tid];
+  __syncthreads();
+  const cplx<T>* strip = g.strip + (size_t)f * g.strip_fstride;
+  cplx<T>* Ar = g.A + (size_t)f * g.strideA + (size_t)i * g.lda;
+  cplx<T>* A0r = g.A0 ? g.A0 + (size_t)f * g.strideA + (size_t)i * g.lda : nullptr;
+  double rmax = 0.0;
+  for (int j = tid; j < g.N; j += 256) {
+    cplx<T> acc = mk<T>(0, 0);
+    const int e1 = g.cs_ptr[j + 1];
+    for (int e = g.cs_ptr[j]; e < e1; ++e) {
+      const int slot = g.cs_slot[e];
+      for (int r = 0; r < ns; ++r) acc = cadd(acc, strip[(size_t)rows[r] * g.L + slot]);
+    }
+    Ar[j] = acc;
+    if (A0r) A0r[j] = acc;
+    rmax = fmax(rmax, (double)(fabs(acc.re) + fabs(acc.im)));
+  }
+  red[tid] = rmax;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+    __syncthreads();
+  }
+  if (tid == 0) g.rowmax[(size_t)f * g.N + i] = red[0];
+  if (tid < g.n_ch) {
+    const cplx<double>* rp = g.rhs_part + (size_t)f * 3 * g.F * MAX_CH;
+    cplx<double> acc = mk<double>(0, 0);
+    for (int r = 0; r < ns; ++r) acc = cadd(acc, rp[(size_t)rows[r] * MAX_CH + tid]);
+    g.b[(size_t)f * g.strideB + (size_t)tid * g.ldb + i] = cmul(acc, g.rhs_scale[f]);
+  }
+}
+
+template <typename T>
+hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
+  hipLaunchKernelGGL(k_regular_lhs<T>, dim3(p.F, nf), dim3(64), 0, st, p);
+  return hipGetLastError();
+}
+hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
+  if (p.P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_singular, dim3(p.P, nf), dim3(64), 0, st, p);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_rhs(const RhsParams<T>& p, int nf, hipStream_t st) {
+  hipLaunchKernelGGL(k_rhs_part<T>, dim3(p.F, nf), dim3(64), 0, st, p);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_gather(const GatherParams<T>& g, int nf, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather<T>, dim3(g.N, nf), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+template hipError_t launch_regular<float>(const RegParams<float>&, int, hipStream_t);
+template hipError_t launch_regular<double>(const RegParams<double>&, int, hipStream_t);
+template hipError_t launch_rhs<float>(const RhsParams<float>&, int, hipStream_t);
+template hipError_t launch_rhs<double>(const RhsParams<double>&, int, hipStream_t);
+template hipError_t launch_gather<float>(const GatherParams<float>&, int, hipStream_t);
+template hipError_t launch_gather<double>(const GatherParams<double>&, int, hipStream_t);

@@ -1,0 +1,583 @@
+// C ABI implementation (see include/bemhip.h): context, mesh upload, workspace, launch sequences.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bemhip.h"
+#include "bem_kernels.hpp"
+
+// ---- error handling ----
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+static int hipfail(hipError_t e, const char* what) {
+  return fail(BEM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(call, what)                         \
+  do {                                             \
+    hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) return hipfail(e_, what); \
+  } while (0)
+
+struct bem_ctx {
+  int device;
+  hipStream_t stream;
+  bool own_stream;
+  hipGraph_t graph;
+  bool capturing;
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct bem_mesh {
+  bem_ctx* ctx;
+  int F, V, n_chunks, L, P, n_rules, n_src, M;
+  double m11[9], m10[3];
+  std::vector<DevBuf> owned;
+  // device arrays
+  ElemRec<float>* rec32;
+  ElemRec<double>* rec64;
+  SrcRec<float>* src32;
+  SrcRec<double>* src64;
+  float* srcw32;
+  double* srcw64;
+  int4* fv;
+  int *order, *slot_count, *slot_base, *cs_ptr, *cs_slot, *vstar_ptr, *vstar_row;
+  uint16_t *slot_start, *slot_inc;
+  double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
+  int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces;
+  // workspace
+  DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial;
+  int strip_nf = 0;
+};
+
+static int dev_alloc(bem_mesh* m, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return BEM_OK;
+  if (m->ctx->capturing) return fail(BEM_E_STATE, "device allocation requested during graph capture; call bem_mesh_reserve first");
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return BEM_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) return fail(BEM_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  b.bytes = bytes;
+  return BEM_OK;
+}
+
+template <typename U>
+static int upload(bem_mesh* m, U** dst, const U* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return BEM_OK;
+  DevBuf b;
+  hipError_t e = hipMalloc(&b.p, count * sizeof(U));
+  if (e != hipSuccess) return fail(BEM_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  b.bytes = count * sizeof(U);
+  m->owned.push_back(b);
+  if (src) {
+    e = hipMemcpy(b.p, src, count * sizeof(U), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hipfail(e, "hipMemcpy");
+  }
+  *dst = static_cast<U*>(b.p);
+  return BEM_OK;
+}
+
+extern "C" {
+
+int bem_version(void) { return 10000; }
+
+int bem_last_error(char* buf, int buflen) {
+  if (!buf || buflen <= 0) return (int)g_err.size();
+  std::snprintf(buf, (size_t)buflen, "%s", g_err.c_str());
+  return (int)g_err.size();
+}
+
+int bem_ctx_create(int device, void* stream, bem_ctx** out) {
+  if (!out) return fail(BEM_E_ARG, "out is NULL");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n), "hipGetDeviceCount");
+  if (device < 0 || device >= n) return fail(BEM_E_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device), "hipSetDevice");
+  bem_ctx* c = new bem_ctx();
+  c->device = device;
+  c->capturing = false;
+  c->graph = nullptr;
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+    c->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return hipfail(e, "hipStreamCreate");
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return BEM_OK;
+}
+
+int bem_ctx_destroy(bem_ctx* c) {
+  if (!c) return BEM_OK;
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BEM_OK;
+}
+
+int bem_ctx_sync(bem_ctx* c) {
+  if (!c) return fail(BEM_E_ARG, "ctx is NULL");
+  HIPCHK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return BEM_OK;
+}
+
+int bem_ctx_stream(bem_ctx* c, void** s) {
+  if (!c || !s) return fail(BEM_E_ARG, "NULL argument");
+  *s = c->stream;
+  return BEM_OK;
+}
+
+int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
+  if (!c || !d || !out) return fail(BEM_E_ARG, "NULL argument");
+  if (d->n_faces <= 0 || d->n_vertices <= 0) return fail(BEM_E_ARG, "empty mesh");
+  if (d->n_chunks != (d->n_faces + 63) / 64) return fail(BEM_E_ARG, "n_chunks must be ceil(F/64)");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  bem_mesh* m = new bem_mesh();
+  m->ctx = c;
+  const int F = d->n_faces, V = d->n_vertices;
+  m->F = F;
+  m->V = V;
+  m->n_chunks = d->n_chunks;
+  m->L = d->strip_len;
+  m->P = d->n_pairs;
+  m->n_rules = d->n_rules;
+  m->n_src = d->n_src;
+  m->M = 6 * F;
+  std::memcpy(m->m11, d->m11, sizeof(m->m11));
+  std::memcpy(m->m10, d->m10, sizeof(m->m10));
+  // host-side records
+  std::vector<ElemRec<float>> r32(F);
+  std::vector<ElemRec<double>> r64(F);
+  std::vector<SrcRec<float>> s32(6 * (size_t)F);
+  std::vector<SrcRec<double>> s64(6 * (size_t)F);
+  std::vector<float> w32(6 * (size_t)F);
+  std::vector<double> w64(6 * (size_t)F);
+  std::vector<int4> fv(F);
+  for (int t = 0; t < F; ++t) {
+    for (int i = 0; i < 18; ++i) {
+      r64[t].q[i] = d->qpoints[18 * (size_t)t + i];
+      r32[t].q[i] = (float)r64[t].q[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+      r64[t].n[i] = d->normals[3 * (size_t)t + i];
+      r32[t].n[i] = (float)r64[t].n[i];
+    }
+    for (int i = 0; i < 9; ++i) {
+      r64[t].curl[i] = d->curls[9 * (size_t)t + i];
+      r32[t].curl[i] = (float)r64[t].curl[i];
+    }
+    r64[t].a2 = d->area2[t];
+    r32[t].a2 = (float)d->area2[t];
+    r64[t].pad = 0;
+    r32[t].pad = 0;
+    fv[t] = make_int4(d->faces[3 * (size_t)t], d->faces[3 * (size_t)t + 1], d->faces[3 * (size_t)t + 2], 0);
+    for (int q = 0; q < 6; ++q) {
+      const size_t mm = 6 * (size_t)t + q;
+      for (int i = 0; i < 3; ++i) {
+        s64[mm].y[i] = r64[t].q[3 * q + i];
+        s64[mm].n[i] = r64[t].n[i];
+        s32[mm].y[i] = (float)s64[mm].y[i];
+        s32[mm].n[i] = (float)s64[mm].n[i];
+      }
+      w64[mm] = rule_w(q) * d->area2[t];
+      w32[mm] = (float)w64[mm];
+    }
+  }
+  int rc = BEM_OK;
+  size_t total_rule_pts = 0;
+  for (int r = 0; r < d->n_rules; ++r) total_rule_pts = std::max(total_rule_pts, (size_t)(d->rule_off[r] + d->rule_len[r]));
+  size_t cs_nnz = (size_t)d->cs_ptr[V];
+  size_t vs_nnz = (size_t)d->vstar_ptr[V];
+#define UP(dst, src, n)                      \
+  if ((rc = upload(m, &(dst), (src), (n)))) { \
+    bem_mesh_destroy(m);                     \
+    return rc;                               \
+  }
+  UP(m->rec32, r32.data(), (size_t)F);
+  UP(m->rec64, r64.data(), (size_t)F);
+  UP(m->src32, s32.data(), s32.size());
+  UP(m->src64, s64.data(), s64.size());
+  UP(m->srcw32, w32.data(), w32.size());
+  UP(m->srcw64, w64.data(), w64.size());
+  UP(m->fv, fv.data(), (size_t)F);
+  UP(m->order, d->order, (size_t)d->n_chunks * 64);
+  UP(m->slot_count, d->slot_count, (size_t)d->n_chunks);
+  UP(m->slot_start, d->slot_start, (size_t)d->n_chunks * 196);
+  UP(m->slot_inc, d->slot_inc, (size_t)d->n_chunks * 192);
+  UP(m->slot_base, d->slot_base, (size_t)d->n_chunks);
+  UP(m->cs_ptr, d->cs_ptr, (size_t)V + 1);
+  UP(m->cs_slot, d->cs_slot, cs_nnz);
+  UP(m->vstar_ptr, d->vstar_ptr, (size_t)V + 1);
+  UP(m->vstar_row, d->vstar_row, vs_nnz);
+  UP(m->fvtx, d->face_vertices, (size_t)F * 9);
+  UP(m->fnrm, d->normals, (size_t)F * 3);
+  UP(m->fcurl, d->curls, (size_t)F * 9);
+  UP(m->fa2, d->area2, (size_t)F);
+  UP(m->sing_test, d->sing_test, (size_t)d->n_pairs);
+  UP(m->sing_trial, d->sing_trial, (size_t)d->n_pairs);
+  UP(m->sing_rule, d->sing_rule, (size_t)d->n_pairs);
+  UP(m->sing_ptr, d->sing_ptr, (size_t)F + 1);
+  UP(m->rule_off, d->rule_off, (size_t)d->n_rules);
+  UP(m->rule_len, d->rule_len, (size_t)d->n_rules);
+  UP(m->rpts, d->rule_pts, total_rule_pts * 5);
+  UP(m->src_faces, d->src_faces, (size_t)d->n_src);
+#undef UP
+  for (int v = 0; v < V; ++v)
+    if (d->vstar_ptr[v + 1] - d->vstar_ptr[v] > 32) {
+      bem_mesh_destroy(m);
+      return fail(BEM_E_ARG, "vertex valence above 32 is not supported");
+    }
+  *out = m;
+  return BEM_OK;
+}
+
+int bem_mesh_destroy(bem_mesh* m) {
+  if (!m) return BEM_OK;
+  for (auto& b : m->owned)
+    if (b.p) (void)hipFree(b.p);
+  for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial})
+    if (b->p) (void)hipFree(b->p);
+  delete m;
+  return BEM_OK;
+}
+
+static int field_slices(const bem_mesh* m, int n_obs, int nf) {
+  const int obs_blocks = (n_obs + 63) / 64;
+  int s = 2048 / std::max(1, obs_blocks * nf);
+  s = std::max(1, std::min(s, (m->M + 255) / 256));
+  return s;
+}
+
+static int reserve_impl(bem_mesh* m, int precision, int nf, int strip_nf, int n_obs, int n_ch) {
+  const size_t cb = precision == BEM_PREC_MIXED ? 8 : 16;
+  strip_nf = std::max(1, std::min(strip_nf, nf));
+  int rc;
+  if ((rc = dev_alloc(m, m->strip, (size_t)strip_nf * 3 * m->F * (size_t)m->L * cb))) return rc;
+  m->strip_nf = strip_nf;
+  if ((rc = dev_alloc(m, m->sing_lhs, (size_t)nf * std::max(1, m->P) * 9 * 16))) return rc;
+  if ((rc = dev_alloc(m, m->sing_rhs, (size_t)nf * std::max(1, m->P) * 3 * 16))) return rc;
+  if ((rc = dev_alloc(m, m->rhs_part, (size_t)nf * 3 * m->F * BEM_MAX_CHANNELS * 16))) return rc;
+  if (n_obs > 0) {
+    if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->M * 2 * cb))) return rc;
+    const int ns = field_slices(m, n_obs, nf);
+    if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
+  }
+  return BEM_OK;
+}
+
+int bem_mesh_reserve(bem_mesh* m, int precision, int nf, int strip_nf, int n_obs, int n_ch) {
+  if (!m || nf <= 0 || n_ch < 1 || n_ch > BEM_MAX_CHANNELS) return fail(BEM_E_ARG, "invalid reserve arguments");
+  HIPCHK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  return reserve_impl(m, precision, nf, strip_nf, n_obs, n_ch);
+}
+
+int bem_mesh_device_bytes(bem_mesh* m, long long* bytes) {
+  if (!m || !bytes) return fail(BEM_E_ARG, "NULL argument");
+  size_t s = 0;
+  for (auto& b : m->owned) s += b.bytes;
+  for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial}) s += b->bytes;
+  *bytes = (long long)s;
+  return BEM_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const void* qunit, int n_ch,
+                         const void* rhs_scale, void* A, long long lda, long long strideA, void* A0, void* b,
+                         long long ldb, long long strideB, double* rowmax, int precision) {
+  int rc;
+  if ((rc = reserve_impl(m, precision, nf, m->strip_nf > 0 ? m->strip_nf : std::min(nf, 4), 0, n_ch))) return rc;
+  hipStream_t st = c->stream;
+  // singular blocks for all frequencies
+  SingParams sp;
+  sp.fvtx = m->fvtx;
+  sp.fnrm = m->fnrm;
+  sp.fcurl = m->fcurl;
+  sp.fa2 = m->fa2;
+  sp.test = m->sing_test;
+  sp.trial = m->sing_trial;
+  sp.rule = m->sing_rule;
+  sp.rule_off = m->rule_off;
+  sp.rule_len = m->rule_len;
+  sp.rpts = m->rpts;
+  sp.k = k;
+  sp.lhs = static_cast<cplx<double>*>(m->sing_lhs.p);
+  sp.rhs = static_cast<cplx<double>*>(m->sing_rhs.p);
+  sp.P = m->P;
+  std::memcpy(sp.m11, m->m11, sizeof(sp.m11));
+  std::memcpy(sp.m10, m->m10, sizeof(sp.m10));
+  HIPCHK(launch_singular(sp, nf, st), "k_singular launch");
+  // rhs partial sums
+  RhsParams<T> rp;
+  rp.rec = precision == BEM_PREC_MIXED ? (const ElemRec<T>*)m->rec32 : (const ElemRec<T>*)m->rec64;
+  rp.fv = m->fv;
+  rp.src = m->src_faces;
+  rp.qunit = static_cast<const cplx<double>*>(qunit);
+  rp.sing_ptr = m->sing_ptr;
+  rp.sing_trial = m->sing_trial;
+  rp.sing_rhs = static_cast<const cplx<double>*>(m->sing_rhs.p);
+  rp.k = k;
+  rp.out = static_cast<cplx<double>*>(m->rhs_part.p);
+  rp.F = m->F;
+  rp.P = m->P;
+  rp.n_src = m->n_src;
+  rp.n_ch = n_ch;
+  HIPCHK(launch_rhs<T>(rp, nf, st), "k_rhs_part launch");
+  // regular strips + gather, in strip-sized frequency groups
+  const long long fstride = 3LL * m->F * m->L;
+  for (int f0 = 0; f0 < nf; f0 += m->strip_nf) {
+    const int g = std::min(m->strip_nf, nf - f0);
+    RegParams<T> p;
+    p.rec = rp.rec;
+    p.fv = m->fv;
+    p.order = m->order;
+    p.slot_count = m->slot_count;
+    p.slot_start = m->slot_start;
+    p.slot_inc = m->slot_inc;
+    p.slot_base = m->slot_base;
+    p.sing_ptr = m->sing_ptr;
+    p.sing_trial = m->sing_trial;
+    p.sing_lhs = static_cast<const cplx<double>*>(m->sing_lhs.p) + (size_t)f0 * m->P * 9;
+    p.k = k + f0;
+    p.strip = static_cast<cplx<T>*>(m->strip.p);
+    p.strip_fstride = fstride;
+    p.L = m->L;
+    p.F = m->F;
+    p.n_chunks = m->n_chunks;
+    p.P = m->P;
+    HIPCHK(launch_regular<T>(p, g, st), "k_regular_lhs launch");
+    GatherParams<T> gp;
+    gp.strip = p.strip;
+    gp.strip_fstride = fstride;
+    gp.L = m->L;
+    gp.vstar_ptr = m->vstar_ptr;
+    gp.vstar_row = m->vstar_row;
+    gp.cs_ptr = m->cs_ptr;
+    gp.cs_slot = m->cs_slot;
+    gp.rhs_part = static_cast<const cplx<double>*>(m->rhs_part.p) + (size_t)f0 * 3 * m->F * BEM_MAX_CHANNELS;
+    gp.rhs_scale = static_cast<const cplx<double>*>(rhs_scale) + f0;
+    gp.A = static_cast<cplx<T>*>(A) + (size_t)f0 * strideA;
+    gp.A0 = A0 ? static_cast<cplx<T>*>(A0) + (size_t)f0 * strideA : nullptr;
+    gp.lda = lda;
+    gp.strideA = strideA;
+    gp.b = static_cast<cplx<double>*>(b) + (size_t)f0 * strideB;
+    gp.ldb = ldb;
+    gp.strideB = strideB;
+    gp.rowmax = rowmax + (size_t)f0 * m->V;
+    gp.N = m->V;
+    gp.F = m->F;
+    gp.n_ch = n_ch;
+    HIPCHK(launch_gather<T>(gp, g, st), "k_gather launch");
+  }
+  return BEM_OK;
+}
+
+extern "C" {
+
+int bem_assemble(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, const void* qunit, int n_ch,
+                 const void* rhs_scale, void* A, long long lda, long long strideA, void* A0, void* b,
+                 long long ldb, long long strideB, double* rowmax) {
+  if (!c || !m || !k || !qunit || !rhs_scale || !A || !b || !rowmax) return fail(BEM_E_ARG, "NULL argument");
+  if (nf <= 0 || n_ch < 1 || n_ch > BEM_MAX_CHANNELS) return fail(BEM_E_ARG, "invalid nf or n_ch");
+  if (lda < m->V || strideA < lda * m->V || ldb < m->V || strideB < ldb * n_ch)
+    return fail(BEM_E_ARG, "leading dimension or stride too small");
+  if (precision == BEM_PREC_MIXED)
+    return assemble_impl<float>(c, m, nf, k, qunit, n_ch, rhs_scale, A, lda, strideA, A0, b, ldb, strideB, rowmax,
+                                precision);
+  if (precision == BEM_PREC_FP64)
+    return assemble_impl<double>(c, m, nf, k, qunit, n_ch, rhs_scale, A, lda, strideA, A0, b, ldb, strideB,
+                                 rowmax, precision);
+  return fail(BEM_E_ARG, "unknown precision");
+}
+
+int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long long lda, long long strideA,
+                  int* ipiv, int* perm, long long strideP, int* info) {
+  if (!c || !A || !ipiv || !perm || !info) return fail(BEM_E_ARG, "NULL argument");
+  if (n <= 0 || batch <= 0 || lda < n || strideA < lda * n || strideP < n) return fail(BEM_E_ARG, "invalid sizes");
+  if (n > 12288) return fail(BEM_E_ARG, "N above 12288 is not supported by the register panel");
+  hipError_t e;
+  if (precision == BEM_PREC_MIXED) {
+    MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
+    e = lu_factor_batch<float>(mb, batch, ipiv, perm, strideP, info, nullptr, c->stream);
+  } else if (precision == BEM_PREC_FP64) {
+    MatBatch<double> mb = {static_cast<cplx<double>*>(A), lda, strideA, n};
+    e = lu_factor_batch<double>(mb, batch, ipiv, perm, strideP, info, nullptr, c->stream);
+  } else {
+    return fail(BEM_E_ARG, "unknown precision");
+  }
+  if (e != hipSuccess) return hipfail(e, "lu_factor launch");
+  return BEM_OK;
+}
+
+int bem_lu_solve(bem_ctx* c, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
+                 const int* perm, long long strideP, const void* b, long long ldb, long long strideB, void* y,
+                 long long ldy, long long strideY, int nrhs, const int* skip) {
+  if (!c || !LU || !perm || !b || !y) return fail(BEM_E_ARG, "NULL argument");
+  if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || n <= 0 || batch <= 0) return fail(BEM_E_ARG, "invalid sizes");
+  hipError_t e;
+  if (precision == BEM_PREC_MIXED) {
+    MatBatch<float> mb = {(cplx<float>*)LU, lda, strideA, n};
+    e = lu_solve_batch<float>(mb, batch, perm, strideP, (const cplx<double>*)b, ldb, strideB, (cplx<float>*)y, ldy,
+                              strideY, nrhs, skip, c->stream);
+  } else if (precision == BEM_PREC_FP64) {
+    MatBatch<double> mb = {(cplx<double>*)LU, lda, strideA, n};
+    e = lu_solve_batch<double>(mb, batch, perm, strideP, (const cplx<double>*)b, ldb, strideB, (cplx<double>*)y,
+                               ldy, strideY, nrhs, skip, c->stream);
+  } else {
+    return fail(BEM_E_ARG, "unknown precision");
+  }
+  if (e != hipSuccess) return hipfail(e, "lu_solve launch");
+  return BEM_OK;
+}
+
+int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA, const int* perm,
+               long long strideP, const void* A0, const void* b, long long ldb, long long strideB, void* x,
+               void* work_r, void* work_y, int nrhs, int max_iter, double tol, double* rel, int* iters, int* done) {
+  if (!c || !LU || !perm || !A0 || !b || !x || !work_r || !work_y || !rel || !iters || !done)
+    return fail(BEM_E_ARG, "NULL argument");
+  if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || max_iter < 0) return fail(BEM_E_ARG, "invalid arguments");
+  hipStream_t st = c->stream;
+  MatBatch<float> lu = {(cplx<float>*)LU, lda, strideA, n};
+  MatBatch<float> a0 = {(cplx<float>*)A0, lda, strideA, n};
+  const cplx<double>* bb = (const cplx<double>*)b;
+  cplx<double>* xx = (cplx<double>*)x;
+  cplx<double>* rr = (cplx<double>*)work_r;
+  cplx<float>* yy = (cplx<float>*)work_y;
+  // done[] = 0 with a fill kernel (no memset nodes: this sequence may be captured into a graph)
+  hipError_t e = fill_int(done, batch, 0, st);
+  if (e != hipSuccess) return hipfail(e, "clear done");
+  if ((e = lu_solve_batch<float>(lu, batch, perm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, st)))
+    return hipfail(e, "refine initial solve");
+  if ((e = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 1, nullptr, batch, st)))
+    return hipfail(e, "refine init x");
+  for (int it = 0; it <= max_iter; ++it) {
+    if ((e = residual_batch<float>(a0, batch, bb, xx, rr, ldb, strideB, nrhs, done, st))) return hipfail(e, "residual");
+    if ((e = resnorm_batch(rr, bb, ldb, strideB, n, nrhs, tol, rel, done, it, iters, batch, st)))
+      return hipfail(e, "resnorm");
+    if (it == max_iter) break;
+    if ((e = lu_solve_batch<float>(lu, batch, perm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, done, st)))
+      return hipfail(e, "refine solve");
+    if ((e = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 0, done, batch, st)))
+      return hipfail(e, "refine update");
+  }
+  return BEM_OK;
+}
+
+int bem_residual(bem_ctx* c, int precision, int n, int batch, const void* A, long long lda, long long strideA,
+                 const void* b, const void* x, long long ldb, long long strideB, void* work_r, int nrhs,
+                 double* rel, int* scratch) {
+  if (!c || !A || !b || !x || !work_r || !rel || !scratch) return fail(BEM_E_ARG, "NULL argument");
+  if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || n <= 0 || batch <= 0) return fail(BEM_E_ARG, "invalid sizes");
+  hipStream_t st = c->stream;
+  hipError_t e;
+  if (precision == BEM_PREC_MIXED) {
+    MatBatch<float> mb = {(cplx<float>*)A, lda, strideA, n};
+    e = residual_batch<float>(mb, batch, (const cplx<double>*)b, (const cplx<double>*)x, (cplx<double>*)work_r, ldb,
+                              strideB, nrhs, nullptr, st);
+  } else {
+    MatBatch<double> mb = {(cplx<double>*)A, lda, strideA, n};
+    e = residual_batch<double>(mb, batch, (const cplx<double>*)b, (const cplx<double>*)x, (cplx<double>*)work_r,
+                               ldb, strideB, nrhs, nullptr, st);
+  }
+  if (e != hipSuccess) return hipfail(e, "residual");
+  if ((e = fill_int(scratch, batch, 0, st)) != hipSuccess) return hipfail(e, "fill");
+  e = resnorm_batch((const cplx<double>*)work_r, (const cplx<double>*)b, ldb, strideB, n, nrhs, 0.0, rel, scratch, 0,
+                    scratch + batch, batch, st);
+  if (e != hipSuccess) return hipfail(e, "resnorm");
+  return BEM_OK;
+}
+
+int bem_growth(bem_ctx* c, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
+               const double* amax, double* umax, double* growth) {
+  if (!c || !LU || !amax || !umax || !growth) return fail(BEM_E_ARG, "NULL argument");
+  hipError_t e;
+  if (precision == BEM_PREC_MIXED) {
+    MatBatch<float> mb = {(cplx<float>*)LU, lda, strideA, n};
+    e = growth_batch<float>(mb, batch, umax, amax, growth, c->stream);
+  } else {
+    MatBatch<double> mb = {(cplx<double>*)LU, lda, strideA, n};
+    e = growth_batch<double>(mb, batch, umax, amax, growth, c->stream);
+  }
+  if (e != hipSuccess) return hipfail(e, "growth");
+  return BEM_OK;
+}
+
+int bem_field(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, int n_obs, const double* obs,
+              const void* p, long long ldp, long long strideP, const void* q, long long strideQ, int n_ch,
+              void* out, long long ldo, long long strideO) {
+  if (!c || !m || !k || !obs || !p || !q || !out) return fail(BEM_E_ARG, "NULL argument");
+  if (nf <= 0 || n_obs <= 0 || n_ch < 1 || n_ch > BEM_MAX_CHANNELS) return fail(BEM_E_ARG, "invalid sizes");
+  const int ns = field_slices(m, n_obs, nf);
+  const size_t cb = precision == BEM_PREC_MIXED ? 8 : 16;
+  int rc;
+  if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->M * 2 * cb))) return rc;
+  if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
+  hipError_t e;
+  if (precision == BEM_PREC_MIXED) {
+    e = field_batch<float>(m->src32, m->fv, m->srcw32, obs, m->M, n_obs, k, (const cplx<double>*)p, ldp,
+                           strideP, (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<float>*)m->wsrc.p,
+                           (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, c->stream);
+  } else {
+    e = field_batch<double>(m->src64, m->fv, m->srcw64, obs, m->M, n_obs, k, (const cplx<double>*)p, ldp, strideP,
+                            (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<double>*)m->wsrc.p,
+                            (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, c->stream);
+  }
+  if (e != hipSuccess) return hipfail(e, "field launch");
+  return BEM_OK;
+}
+
+int bem_graph_begin(bem_ctx* c) {
+  if (!c) return fail(BEM_E_ARG, "ctx is NULL");
+  if (c->capturing) return fail(BEM_E_STATE, "already capturing");
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  c->capturing = true;
+  return BEM_OK;
+}
+
+int bem_graph_end(bem_ctx* c, void** graph_exec) {
+  if (!c || !graph_exec) return fail(BEM_E_ARG, "NULL argument");
+  if (!c->capturing) return fail(BEM_E_STATE, "not capturing");
+  hipGraph_t g = nullptr;
+  c->capturing = false;
+  HIPCHK(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
+  *graph_exec = ge;
+  return BEM_OK;
+}
+
+int bem_graph_launch(bem_ctx* c, void* graph_exec) {
+  if (!c || !graph_exec) return fail(BEM_E_ARG, "NULL argument");
+  HIPCHK(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), c->stream), "hipGraphLaunch");
+  return BEM_OK;
+}
+
+int bem_graph_destroy(void* graph_exec) {
+  if (graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec));
+  return BEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Shared device/host helpers for the MI355X (gfx950) Helmholtz BEM kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BEM_PI 3.14159265358979323846
+#define BEM_INV_FOUR_PI 0.07957747154594767  // 1 / (4 pi)
+
+template <typename T>
+struct __align__(2 * sizeof(T)) cplx {
+  T re, im;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mk(T r, T i) {
+  cplx<T> c;
+  c.re = r;
+  c.im = i;
+  return c;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return mk<T>(a.re + b.re, a.im + b.im); }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return mk<T>(a.re - b.re, a.im - b.im); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// a - b*c
+template <typename T>
+__device__ __forceinline__ cplx<T> cfms(cplx<T> a, cplx<T> b, cplx<T> c) {
+  return mk<T>(a.re - b.re * c.re + b.im * c.im, a.im - b.re * c.im - b.im * c.re);
+}
+// a + b*c
+template <typename T>
+__device__ __forceinline__ cplx<T> cfma(cplx<T> a, cplx<T> b, cplx<T> c) {
+  return mk<T>(a.re + b.re * c.re - b.im * c.im, a.im + b.re * c.im + b.im * c.re);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cdiv(cplx<T> a, cplx<T> b) {
+  // Smith-free scaled division; pivots are O(1) after partial pivoting so plain form is adequate.
+  T d = b.re * b.re + b.im * b.im;
+  T inv = T(1) / d;
+  return mk<T>((a.re * b.re + a.im * b.im) * inv, (a.im * b.re - a.re * b.im) * inv);
+}
+template <typename T>
+__device__ __forceinline__ T cabs1(cplx<T> a) {  // |re| + |im|, the LAPACK i?amax measure
+  return fabs(a.re) + fabs(a.im);
+}
+
+// Trigonometry of the Helmholtz phase k*r.
+//  float : hardware v_sin_f32 / v_cos_f32 take revolutions; the phase is formed as (k/2pi)*r and
+//          reduced to [0,1) with v_fract_f32 before the call.
+//  double: library sincos in radians.
+template <typename T>
+struct Phase;
+template <>
+struct Phase<float> {
+  __device__ __forceinline__ static void sincos_kr(float kappa_rev, float k, float r, float* s, float* c) {
+    (void)k;
+    float ph = __builtin_amdgcn_fractf(kappa_rev * r);
+    *s = __builtin_amdgcn_sinf(ph);
+    *c = __builtin_amdgcn_cosf(ph);
+  }
+  __device__ __forceinline__ static float rsqrt_(float x) { return x > 0.f ? __builtin_amdgcn_rsqf(x) : 0.f; }
+};
+template <>
+struct Phase<double> {
+  __device__ __forceinline__ static void sincos_kr(double kappa_rev, double k, double r, double* s, double* c) {
+    (void)kappa_rev;
+    sincos(k * r, s, c);
+  }
+  __device__ __forceinline__ static double rsqrt_(double x) { return x > 0.0 ? 1.0 / sqrt(x) : 0.0; }
+};
+
+// Wave-level reductions (wave64).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Order-4 triangle rule (7-digit-asymmetric constants as tabulated by the reference,
+// BeatEngineCore.jl:517-549). Index q = 0..5.
+__host__ __device__ constexpr double rule_xi(int q) {
+  return q == 0 ? 0.4459484909159651
+         : q == 1 ? 0.0915762135097710
+         : q == 2 ? 0.1081030181680700
+         : q == 3 ? 0.4459484909159651
+         : q == 4 ? 0.8168475729804590
+                  : 0.0915762135097710;
+}
+__host__ __device__ constexpr double rule_eta(int q) {
+  return q == 0 ? 0.4459484909159651
+         : q == 1 ? 0.0915762135097700
+         : q == 2 ? 0.4459484909159651
+         : q == 3 ? 0.1081030181680700
+         : q == 4 ? 0.0915762135097700
+                  : 0.8168475729804580;
+}
+__host__ __device__ constexpr double rule_w(int q) {
+  return 0.5 * (q == 0 || q == 2 || q == 3 ? 0.2233815896780110 : 0.1099517436553220);
+}
+// P1 basis value phi_l at rule point q.
+__host__ __device__ constexpr double rule_phi(int l, int q) {
+  return l == 0 ? 1.0 - rule_xi(q) - rule_eta(q) : (l == 1 ? rule_xi(q) : rule_eta(q));
+}

@@ -1,0 +1,139 @@
+// Kernel parameter blocks and launcher declarations shared by the translation units of libbemhip.
+#pragma once
+
+#include "bem_common.hpp"
+
+#define BEM_SLOT_STRIDE 196  // per-chunk slot_start entries (<= 192 slots + 1, padded)
+#define BEM_SLOT_INC 192     // per-chunk incidence entries (64 elements x 3 columns)
+#define BEM_MAX_CH 4
+#define BEM_NBO 64           // outer LU panel width (trailing-update K)
+
+// Per-element record used by the regular kernels (32 T values, 128 B in FP32).
+template <typename T>
+struct ElemRec {
+  T q[18];    // six order-4 quadrature points (x,y,z)
+  T n[3];     // unit normal
+  T curl[9];  // P1 surface curls, 3 vectors
+  T a2;       // 2 * area
+  T pad;
+};
+
+// Field source point (order-4 rule point of a face) with its face normal.
+template <typename T>
+struct SrcRec {
+  T y[3];
+  T n[3];
+};
+
+template <typename T>
+struct RegParams {
+  const ElemRec<T>* rec;
+  const int4* fv;
+  const int* order;             // [n_chunks*64] trial element per (chunk, lane), -1 = none
+  const int* slot_count;        // [n_chunks]
+  const uint16_t* slot_start;   // [n_chunks][BEM_SLOT_STRIDE]
+  const uint16_t* slot_inc;     // [n_chunks][BEM_SLOT_INC]  (lane << 2) | m
+  const int* slot_base;         // [n_chunks]
+  const int* sing_ptr;          // [F+1]
+  const int* sing_trial;        // [P]
+  const cplx<double>* sing_lhs; // [nf][P][9]
+  const double* k;              // [nf]
+  cplx<T>* strip;               // [nf][3F][L]
+  long long strip_fstride;
+  int L, F, n_chunks, P;
+};
+
+struct SingParams {
+  const double* fvtx;   // [F][9]
+  const double* fnrm;   // [F][3]
+  const double* fcurl;  // [F][9]
+  const double* fa2;    // [F]
+  const int* test;      // [P]
+  const int* trial;     // [P]
+  const int* rule;      // [P]
+  const int* rule_off;  // [n_rules]
+  const int* rule_len;  // [n_rules]
+  const double* rpts;   // [total][5]: test xi, eta, trial xi, eta, weight
+  const double* k;      // [nf]
+  cplx<double>* lhs;    // [nf][P][9]
+  cplx<double>* rhs;    // [nf][P][3]
+  int P;
+  double m11[9];        // reference-element mass block / (2A)
+  double m10[3];
+};
+
+template <typename T>
+struct RhsParams {
+  const ElemRec<T>* rec;
+  const int4* fv;
+  const int* src;                // [n_src] radiating faces
+  const cplx<double>* qunit;     // [n_ch][F]
+  const int* sing_ptr;
+  const int* sing_trial;
+  const cplx<double>* sing_rhs;  // [nf][P][3]
+  const double* k;
+  cplx<double>* out;             // [nf][3F][BEM_MAX_CH]
+  int F, P, n_src, n_ch;
+};
+
+template <typename T>
+struct GatherParams {
+  const cplx<T>* strip;
+  long long strip_fstride;
+  int L;
+  const int* vstar_ptr;           // [N+1]
+  const int* vstar_row;           // strip rows 3t+l
+  const int* cs_ptr;              // [N+1]
+  const int* cs_slot;
+  const cplx<double>* rhs_part;   // [nf][3F][BEM_MAX_CH]
+  const cplx<double>* rhs_scale;  // [nf]  (i rho omega)
+  cplx<T>* A;                     // [nf] row-major N x lda
+  cplx<T>* A0;                    // optional copy (may be null)
+  long long lda, strideA;
+  cplx<double>* b;                // [nf][n_ch][ldb]
+  long long ldb, strideB;
+  double* rowmax;                 // [nf][N]
+  int N, F, n_ch;
+};
+
+template <typename T>
+struct MatBatch {
+  cplx<T>* A;
+  long long lda, strideA;
+  int N;
+};
+
+template <typename T>
+hipError_t launch_regular(const RegParams<T>&, int nf, hipStream_t);
+hipError_t launch_singular(const SingParams&, int nf, hipStream_t);
+template <typename T>
+hipError_t launch_rhs(const RhsParams<T>&, int nf, hipStream_t);
+template <typename T>
+hipError_t launch_gather(const GatherParams<T>&, int nf, hipStream_t);
+
+template <typename T>
+hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,
+                           const int* skip, hipStream_t);
+template <typename T>
+hipError_t lu_solve_batch(const MatBatch<T>&, int batch, const int* perm, long long strideP, const cplx<double>* b,
+                          long long ldb, long long strideB, cplx<T>* y, long long ldy, long long strideY, int nrhs,
+                          const int* skip, hipStream_t);
+template <typename T>
+hipError_t residual_batch(const MatBatch<T>&, int batch, const cplx<double>* b, const cplx<double>* x,
+                          cplx<double>* r, long long ldv, long long strideV, int nrhs, const int* skip, hipStream_t);
+hipError_t resnorm_batch(const cplx<double>* r, const cplx<double>* b, long long ldv, long long strideV, int N,
+                         int nrhs, double tol, double* rel, int* done, int iter, int* iters, int batch, hipStream_t);
+template <typename T>
+hipError_t update_batch(cplx<double>* x, const cplx<T>* d, long long ldv, long long ldd, long long strideV,
+                        long long strideD, int N, int nrhs, int init, const int* skip, int batch, hipStream_t);
+template <typename T>
+hipError_t growth_batch(const MatBatch<T>&, int batch, double* umax, const double* amax, double* growth,
+                        hipStream_t);
+hipError_t fill_int(int* p, int n, int value, hipStream_t);
+
+template <typename T>
+hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, const double* obs, int M, int n_obs,
+                       const double* k, const cplx<double>* p, long long ldp, long long strideP,
+                       const cplx<double>* q, long long ldq, long long strideQ, int n_ch, int nf, cplx<T>* wsrc,
+                       cplx<double>* partial, int n_slices, cplx<double>* out, long long ldo, long long strideO,
+                       hipStream_t st);

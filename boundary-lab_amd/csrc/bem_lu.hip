@@ -1,0 +1,729 @@
+// Blocked dense complex LU with partial pivoting, triangular solves and FP64 iterative refinement.
+//
+// Replaces the reference's rocSOLVER getrf/getrs calls
+//   src/blab/solvers/julia_local/src/BeatEngineRocmOperators.jl:132-222 (rocm_dense_lu!,
+//   solve_rocm_dense_factorization, d_lhs \ d_rhs) and LAPACK cgetrf/cgetrs on the CPU path
+//   (BeatEngineCpuSolve.jl:55-71).
+//
+// Storage: each matrix is row-major complex (re,im interleaved), leading dimension lda, matrices of a batch
+// at a fixed stride. LAPACK semantics: P A = L U, unit lower L below the diagonal, U on and above it, pivot
+// chosen with |re|+|im| (i?amax), ties to the lowest row; ipiv[c] is the row swapped with row c. A running
+// permutation perm (perm[i] = original row now at position i) is kept for O(N) RHS permutation.
+//
+// Right-looking at the outer level (panel width NBO = 64, trailing update C -= L21 U12 on MFMA), left-looking
+// inside the panel: sub-panels of NBI columns are factored by one 1024-thread workgroup that holds the
+// sub-panel in registers (k_lu_panel), after a top TRSM and an MFMA update with the sub-panels already done.
+//
+// Complex GEMM on real MFMA: C (M x 2Nc reals, interleaved) = A (M x 2K reals) * B'' where B'' is the 2x2 real
+// expansion [[br, bi], [-bi, br]] of each complex entry of B, formed on the fly from B in LDS. FP32 uses
+// v_mfma_f32_16x16x4f32 (FP32 operands, FP32 accumulate); FP64 uses v_mfma_f64_16x16x4f64.
+
+#include "bem_kernels.hpp"
+
+#define NBO BEM_NBO
+#define MAX_CH BEM_MAX_CH
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef f32x4 acc_t;
+  __device__ __forceinline__ static acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // output register v of lane l holds tile row 4*(l>>4)+v, column l&15
+  __device__ __forceinline__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <>
+struct Mfma<double> {
+  typedef f64x4 acc_t;
+  __device__ __forceinline__ static acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // output register v of lane l holds tile row 4*v+(l>>4), column l&15
+  __device__ __forceinline__ static int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+
+
+// ------------------------------------------------------------------------------------------------
+// GEMM: C[c_r0+i][c_c0+j] -= sum_k A[a_r0+i][a_c0+k] * B[b_r0+k][b_c0+j], all blocks of one matrix.
+// ------------------------------------------------------------------------------------------------
+
+struct GemmShape {
+  int a_r0, a_c0, b_r0, b_c0, c_r0, c_c0, M, Nc, K;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(MatBatch<T> mb, GemmShape g, const int* skip) {
+  const int mat = blockIdx.z;
+  if (skip && skip[mat]) return;
+  cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wy = w >> 1, wx = w & 1;
+  const int bm = blockIdx.y * 64, bn = blockIdx.x * 64;
+  __shared__ cplx<T> As[64][17];
+  __shared__ cplx<T> Bs[16][65];
+  typedef typename Mfma<T>::acc_t acc_t;
+  acc_t acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
+
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    // stage A tile: 64 rows x 16 complex
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int r = e >> 4, kk = e & 15;
+      cplx<T> v = mk<T>(0, 0);
+      if (bm + r < g.M && k0 + kk < g.K) v = A[(size_t)(g.a_r0 + bm + r) * lda + g.a_c0 + k0 + kk];
+      As[r][kk] = v;
+    }
+    // stage B tile: 16 rows x 64 complex
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int kk = e >> 6, j = e & 63;
+      cplx<T> v = mk<T>(0, 0);
+      if (k0 + kk < g.K && bn + j < g.Nc) v = A[(size_t)(g.b_r0 + k0 + kk) * lda + g.b_c0 + bn + j];
+      Bs[kk][j] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const int kr = 4 * ks + (lane >> 4);
+      const int kc = kr >> 1, pk = kr & 1;
+      T av[2];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+        const cplx<T> a = As[32 * wy + 16 * ti + (lane & 15)][kc];
+        av[ti] = pk ? a.im : a.re;
+      }
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const int cr = 64 * wx + 16 * tj + (lane & 15);
+        const int jc = cr >> 1, pj = cr & 1;
+        const cplx<T> b = Bs[kc][jc];
+        const T bv = (pk == pj) ? b.re : (pk ? -b.im : b.im);
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) acc[ti][tj] = Mfma<T>::step(av[ti], bv, acc[ti][tj]);
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue: C -= acc
+  T* Cbase = reinterpret_cast<T*>(A);
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = bm + 32 * wy + 16 * ti + Mfma<T>::row(lane, v);
+        const int cr = 2 * bn + 64 * wx + 16 * tj + (lane & 15);
+        const int jc = cr >> 1;
+        if (r < g.M && jc < g.Nc) {
+          T* cp = Cbase + 2 * ((size_t)(g.c_r0 + r) * lda + g.c_c0 + jc) + (cr & 1);
+          *cp = *cp - acc[ti][tj][v];
+        }
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
+// TRSM with unit lower L: X[r0:r0+nb, col0:col0+ncols] = L[r0:r0+nb, r0:r0+nb]^{-1} X
+// ------------------------------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_lower(MatBatch<T> mb, int r0, int nb, int col0, int ncols,
+                                                   const int* skip) {
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  constexpr int TW = sizeof(T) == 4 ? 64 : 32;
+  cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x;
+  const int cb = col0 + blockIdx.x * TW;
+  __shared__ cplx<T> X[64][TW + 1];
+  for (int e = tid; e < 64 * TW; e += 256) {
+    const int r = e / TW, j = e % TW;
+    cplx<T> v = mk<T>(0, 0);
+    if (r < nb && cb + j < col0 + ncols) v = A[(size_t)(r0 + r) * lda + cb + j];
+    X[r][j] = v;
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 16) {
+    const int be = min(nb, b0 + 16);
+    // diagonal block: one thread per column
+    if (tid < TW) {
+      for (int r = b0 + 1; r < be; ++r) {
+        cplx<T> acc = X[r][tid];
+        for (int c = b0; c < r; ++c) acc = cfms(acc, A[(size_t)(r0 + r) * lda + r0 + c], X[c][tid]);
+        X[r][tid] = acc;
+      }
+    }
+    __syncthreads();
+    // update rows below the block
+    const int rows_below = nb - be;
+    for (int e = tid; e < rows_below * TW; e += 256) {
+      const int r = be + e / TW, j = e % TW;
+      cplx<T> acc = X[r][j];
+      const cplx<T>* Lr = A + (size_t)(r0 + r) * lda + r0;
+      for (int c = b0; c < be; ++c) acc = cfms(acc, Lr[c], X[c][j]);
+      X[r][j] = acc;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * TW; e += 256) {
+    const int r = e / TW, j = e % TW;
+    if (cb + j < col0 + ncols) A[(size_t)(r0 + r) * lda + cb + j] = X[r][j];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Sub-panel factorization in registers: rows [c0, N), columns [c0, c0+nb), nb <= NBI.
+// ------------------------------------------------------------------------------------------------
+
+template <typename T, int NBI, int MR>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lu_panel(MatBatch<T> mb, int c0, int nb, int* ipiv, int* perm,
+                                                 long long strideP, int* info, const int* skip) {
+  const int mat = blockIdx.x;
+  if (skip && skip[mat]) return;
+  cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  int* piv_out = ipiv + (size_t)mat * strideP;
+  int* prm = perm + (size_t)mat * strideP;
+  const long long lda = mb.lda;
+  const int N = mb.N;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ T s_val[16];
+  __shared__ int s_idx[16];
+  __shared__ int s_piv;
+  __shared__ int s_pivs[NBI];
+  __shared__ cplx<T> s_prow[NBI];
+  __shared__ cplx<T> s_crow[NBI];
+  __shared__ int s_zero;
+
+  cplx<T> P[MR][NBI];
+#pragma unroll
+  for (int m = 0; m < MR; ++m) {
+    const int r = c0 + tid + 1024 * m;
+#pragma unroll
+    for (int j = 0; j < NBI; ++j)
+      P[m][j] = (r < N && j < nb) ? A[(size_t)r * lda + c0 + j] : mk<T>(0, 0);
+  }
+  if (tid == 0) s_zero = 0;
+
+#pragma unroll
+  for (int jj = 0; jj < NBI; ++jj) {
+    if (jj < nb) {
+      const int c = c0 + jj;
+      T best = T(-1);
+      int bidx = 0x7fffffff;
+#pragma unroll
+      for (int m = 0; m < MR; ++m) {
+        const int r = c0 + tid + 1024 * m;
+        if (r >= c && r < N) {
+          const T v = cabs1(P[m][jj]);
+          if (v > best) {
+            best = v;
+            bidx = r;
+          }
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const T ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ov > best || (ov == best && oi < bidx)) {
+          best = ov;
+          bidx = oi;
+        }
+      }
+      if (lane == 0) {
+        s_val[w] = best;
+        s_idx[w] = bidx;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        T bv = s_val[0];
+        int bi = s_idx[0];
+        for (int q = 1; q < 16; ++q)
+          if (s_val[q] > bv || (s_val[q] == bv && s_idx[q] < bi)) {
+            bv = s_val[q];
+            bi = s_idx[q];
+          }
+        s_piv = bi;
+        s_pivs[jj] = bi;
+      }
+      __syncthreads();
+      const int pv = s_piv;
+      const int own_p = (pv - c0) & 1023, m_p = (pv - c0) >> 10;
+      const int own_c = (c - c0) & 1023, m_c = (c - c0) >> 10;
+#pragma unroll
+      for (int m = 0; m < MR; ++m) {
+        if (tid == own_p && m == m_p)
+#pragma unroll
+          for (int j = 0; j < NBI; ++j) s_prow[j] = P[m][j];
+        if (tid == own_c && m == m_c)
+#pragma unroll
+          for (int j = 0; j < NBI; ++j) s_crow[j] = P[m][j];
+      }
+      __syncthreads();
+      if (pv != c) {
+#pragma unroll
+        for (int m = 0; m < MR; ++m) {
+          if (tid == own_c && m == m_c)
+#pragma unroll
+            for (int j = 0; j < NBI; ++j) P[m][j] = s_prow[j];
+          if (tid == own_p && m == m_p)
+#pragma unroll
+            for (int j = 0; j < NBI; ++j) P[m][j] = s_crow[j];
+        }
+      }
+      if (tid == 0) {
+        piv_out[c] = pv;
+        if (pv != c) {
+          const int tmp = prm[c];
+          prm[c] = prm[pv];
+          prm[pv] = tmp;
+        }
+      }
+      const cplx<T> pivot = s_prow[jj];
+      if (cabs1(pivot) == T(0)) {
+        if (tid == 0 && s_zero == 0) s_zero = c + 1;
+      } else {
+        const cplx<T> inv = cdiv(mk<T>(1, 0), pivot);
+#pragma unroll
+        for (int m = 0; m < MR; ++m) {
+          const int r = c0 + tid + 1024 * m;
+          if (r > c && r < N) {
+            const cplx<T> l = cmul(P[m][jj], inv);
+            P[m][jj] = l;
+#pragma unroll
+            for (int j2 = jj + 1; j2 < NBI; ++j2) P[m][j2] = cfms(P[m][j2], l, s_prow[j2]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MR; ++m) {
+    const int r = c0 + tid + 1024 * m;
+    if (r < N)
+#pragma unroll
+      for (int j = 0; j < NBI; ++j)
+        if (j < nb) A[(size_t)r * lda + c0 + j] = P[m][j];
+  }
+  // row interchanges outside the sub-panel, in pivot order
+  for (int col = tid; col < N; col += 1024) {
+    if (col >= c0 && col < c0 + nb) continue;
+    for (int jj = 0; jj < nb; ++jj) {
+      const int c = c0 + jj, pv = s_pivs[jj];
+      if (pv != c) {
+        cplx<T>* a = A + (size_t)c * lda + col;
+        cplx<T>* b = A + (size_t)pv * lda + col;
+        const cplx<T> t = *a;
+        *a = *b;
+        *b = t;
+      }
+    }
+  }
+  if (tid == 0 && s_zero != 0 && info[mat] == 0) info[mat] = s_zero;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Misc kernels: identity permutation, RHS permutation, triangular sweeps, residual, norms, update.
+// ------------------------------------------------------------------------------------------------
+
+__global__ void k_fill_int(int* p, int n, int value) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+hipError_t fill_int(int* p, int n, int value, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, value);
+  return hipGetLastError();
+}
+
+__global__ void k_iota(int* perm, long long strideP, int N, int* info) {
+  const int mat = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) perm[(size_t)mat * strideP + i] = i;
+  if (i == 0) info[mat] = 0;
+}
+
+// y[c][i] = src[c][perm[i]]  (src is FP64 complex, y is T)
+template <typename T>
+__global__ void k_permute_rhs(const cplx<double>* src, long long lds_, long long strideS, cplx<T>* y,
+                              long long ldy, long long strideY, const int* perm, long long strideP, int N,
+                              int nrhs, const int* skip) {
+  const int mat = blockIdx.z;
+  if (skip && skip[mat]) return;
+  const int c = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || c >= nrhs) return;
+  const cplx<double> v = src[(size_t)mat * strideS + (size_t)c * lds_ + perm[(size_t)mat * strideP + i]];
+  y[(size_t)mat * strideY + (size_t)c * ldy + i] = mk<T>(T(v.re), T(v.im));
+}
+
+// Forward (unit lower) or backward (upper) substitution, one workgroup per matrix, in place on y.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(1024) void k_trsv(MatBatch<T> mb, cplx<T>* y, long long ldy, long long strideY,
+                                             int nrhs, const int* skip) {
+  const int mat = blockIdx.x;
+  if (skip && skip[mat]) return;
+  const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  cplx<T>* Y = y + (size_t)mat * strideY;
+  const long long lda = mb.lda;
+  const int N = mb.N;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ cplx<T> xb[MAX_CH][64];
+  const int nblk = (N + 63) / 64;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int blk = LOWER ? bi : nblk - 1 - bi;
+    const int r0 = blk * 64;
+    const int nb = min(64, N - r0);
+    if (w == 0) {
+      for (int c = 0; c < nrhs; ++c) {
+        cplx<T> v = lane < nb ? Y[(size_t)c * ldy + r0 + lane] : mk<T>(0, 0);
+        if (LOWER) {
+          for (int r = 0; r < nb; ++r) {
+            const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
+            if (lane > r && lane < nb) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
+          }
+        } else {
+          for (int r = nb - 1; r >= 0; --r) {
+            if (lane == r) v = cdiv(v, A[(size_t)(r0 + r) * lda + r0 + r]);
+            const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
+            if (lane < r) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
+          }
+        }
+        xb[c][lane] = v;
+        if (lane < nb) Y[(size_t)c * ldy + r0 + lane] = v;
+      }
+    }
+    __syncthreads();
+    // update the other rows: lower -> rows below, upper -> rows above
+    const int rbeg = LOWER ? r0 + nb : 0;
+    const int rend = LOWER ? N : r0;
+    for (int i = rbeg + w; i < rend; i += 16) {
+      const cplx<T> a = lane < nb ? A[(size_t)i * lda + r0 + lane] : mk<T>(0, 0);
+      for (int c = 0; c < nrhs; ++c) {
+        const cplx<T> pr = cmul(a, xb[c][lane]);
+        const T sr = wave_sum(pr.re), si = wave_sum(pr.im);
+        if (lane == 0) {
+          cplx<T>* yp = Y + (size_t)c * ldy + i;
+          *yp = mk<T>(yp->re - sr, yp->im - si);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// r = b - A0 x in FP64 (A0 in T, x and b in FP64); one wave per row, all RHS together.
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual(MatBatch<T> mb, const cplx<double>* b, const cplx<double>* x,
+                                                  cplx<double>* r, long long ldv, long long strideV, int nrhs,
+                                                  const int* skip) {
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int N = mb.N;
+  if (i >= N) return;
+  const cplx<T>* Ar = mb.A + (size_t)mat * mb.strideA + (size_t)i * mb.lda;
+  const cplx<double>* X = x + (size_t)mat * strideV;
+  double sr[MAX_CH], si[MAX_CH];
+#pragma unroll
+  for (int c = 0; c < MAX_CH; ++c) sr[c] = si[c] = 0.0;
+  for (int j = lane; j < N; j += 64) {
+    const cplx<T> a = Ar[j];
+    const double ar = a.re, ai = a.im;
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) {
+      if (c < nrhs) {
+        const cplx<double> xv = X[(size_t)c * ldv + j];
+        sr[c] += ar * xv.re - ai * xv.im;
+        si[c] += ar * xv.im + ai * xv.re;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < MAX_CH; ++c) {
+    if (c < nrhs) {
+      const double tr = wave_sum(sr[c]), ti = wave_sum(si[c]);
+      if (lane == 0) {
+        const size_t o = (size_t)mat * strideV + (size_t)c * ldv + i;
+        r[o] = mk<double>(b[o].re - tr, b[o].im - ti);
+      }
+    }
+  }
+}
+
+// Relative residual per matrix: max over RHS of ||r_c|| / ||b_c||; sets done[mat] when <= tol.
+__global__ __launch_bounds__(256) void k_resnorm(const cplx<double>* r, const cplx<double>* b, long long ldv,
+                                                 long long strideV, int N, int nrhs, double tol, double* rel,
+                                                 int* done, int iter, int* iters) {
+  const int mat = blockIdx.x;
+  if (done[mat]) return;
+  __shared__ double sr[256], sb[256];
+  const int tid = threadIdx.x;
+  double worst = 0.0;
+  for (int c = 0; c < nrhs; ++c) {
+    double ar = 0.0, ab = 0.0;
+    const size_t base = (size_t)mat * strideV + (size_t)c * ldv;
+    for (int i = tid; i < N; i += 256) {
+      const cplx<double> rv = r[base + i], bv = b[base + i];
+      ar += rv.re * rv.re + rv.im * rv.im;
+      ab += bv.re * bv.re + bv.im * bv.im;
+    }
+    sr[tid] = ar;
+    sb[tid] = ab;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (tid < o) {
+        sr[tid] += sr[tid + o];
+        sb[tid] += sb[tid + o];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double q = sb[0] > 0.0 ? sqrt(sr[0] / sb[0]) : sqrt(sr[0]);
+      worst = fmax(worst, q);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    rel[mat] = worst;
+    iters[mat] = iter;
+    if (worst <= tol) done[mat] = 1;
+  }
+}
+
+// x64 += d (T), or x64 = d when init
+template <typename T>
+__global__ void k_update(cplx<double>* x, const cplx<T>* d, long long ldv, long long ldd, long long strideV,
+                         long long strideD, int N, int nrhs, int init, const int* skip) {
+  const int mat = blockIdx.z;
+  if (skip && skip[mat]) return;
+  const int c = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || c >= nrhs) return;
+  cplx<double>* xp = x + (size_t)mat * strideV + (size_t)c * ldv + i;
+  const cplx<T> dv = d[(size_t)mat * strideD + (size_t)c * ldd + i];
+  if (init)
+    *xp = mk<double>(dv.re, dv.im);
+  else
+    *xp = mk<double>(xp->re + double(dv.re), xp->im + double(dv.im));
+}
+
+// Row maxima of U (upper triangle incl. diagonal) in |re|+|im|; growth = max|U| / max|A| per matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void k_umax(MatBatch<T> mb, double* umax) {
+  const int mat = blockIdx.y, i = blockIdx.x;
+  const cplx<T>* Ar = mb.A + (size_t)mat * mb.strideA + (size_t)i * mb.lda;
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int j = i + threadIdx.x; j < mb.N; j += 256) m = fmax(m, double(cabs1(Ar[j])));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) umax[(size_t)mat * mb.N + i] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_growth(const double* umax, const double* amax, int N, double* growth) {
+  const int mat = blockIdx.x;
+  __shared__ double ru[256], ra[256];
+  double u = 0.0, a = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) {
+    u = fmax(u, umax[(size_t)mat * N + i]);
+    a = fmax(a, amax[(size_t)mat * N + i]);
+  }
+  ru[threadIdx.x] = u;
+  ra[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      ru[threadIdx.x] = fmax(ru[threadIdx.x], ru[threadIdx.x + o]);
+      ra[threadIdx.x] = fmax(ra[threadIdx.x], ra[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) growth[mat] = ra[0] > 0.0 ? ru[0] / ra[0] : 0.0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host-side launch sequences
+// ------------------------------------------------------------------------------------------------
+
+template <typename T>
+static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb, int* ipiv, int* perm,
+                               long long strideP, int* info, const int* skip, hipStream_t st) {
+  const int H = mb.N - c0;
+  const int mr = (H + 1023) / 1024;
+#define PANEL(NBI_, MR_)                                                                                    \
+  hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, ipiv, perm, \
+                     strideP, info, skip)
+  // (NBI, MR) pairs chosen so that the register-resident sub-panel does not spill at 128 VGPRs/lane
+  if (sizeof(T) == 4) {
+    if (mr <= 1) PANEL(16, 1);
+    else if (mr <= 2) PANEL(8, 2);
+    else if (mr <= 3) PANEL(8, 3);
+    else if (mr <= 6) PANEL(4, 6);
+    else if (mr <= 12) PANEL(2, 12);
+    else return hipErrorInvalidValue;
+  } else {
+    if (mr <= 1) PANEL(8, 1);
+    else if (mr <= 2) PANEL(4, 2);
+    else if (mr <= 3) PANEL(4, 3);
+    else if (mr <= 6) PANEL(2, 6);
+    else if (mr <= 12) PANEL(1, 12);
+    else return hipErrorInvalidValue;
+  }
+#undef PANEL
+  return hipGetLastError();
+}
+
+template <typename T>
+int panel_width(int N, int c0) {
+  const int mr = (N - c0 + 1023) / 1024;
+  if (sizeof(T) == 4) return mr <= 1 ? 16 : (mr <= 3 ? 8 : (mr <= 6 ? 4 : 2));
+  return mr <= 1 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
+}
+
+template <typename T>
+static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape& g, const int* skip,
+                              hipStream_t st) {
+  if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
+  dim3 grid((g.Nc + 63) / 64, (g.M + 63) / 64, batch);
+  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, mb, g, skip);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_trsm(const MatBatch<T>& mb, int batch, int r0, int nb, int col0, int ncols,
+                              const int* skip, hipStream_t st) {
+  if (nb <= 0 || ncols <= 0) return hipSuccess;
+  constexpr int TW = sizeof(T) == 4 ? 64 : 32;
+  dim3 grid((ncols + TW - 1) / TW, batch);
+  hipLaunchKernelGGL(k_trsm_lower<T>, grid, dim3(256), 0, st, mb, r0, nb, col0, ncols, skip);
+  return hipGetLastError();
+}
+
+#define CHK(x)                            \
+  do {                                    \
+    hipError_t e_ = (x);                  \
+    if (e_ != hipSuccess) return e_;      \
+  } while (0)
+
+template <typename T>
+hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* perm, long long strideP, int* info,
+                           const int* skip, hipStream_t st) {
+  const int N = mb.N;
+  hipLaunchKernelGGL(k_iota, dim3((N + 255) / 256, batch), dim3(256), 0, st, perm, strideP, N, info);
+  CHK(hipGetLastError());
+  for (int k0 = 0; k0 < N; k0 += NBO) {
+    const int k1 = min(N, k0 + NBO);
+    int c0 = k0;
+    while (c0 < k1) {
+      const int nb = min(panel_width<T>(N, c0), k1 - c0);
+      if (c0 > k0) {
+        CHK(launch_trsm<T>(mb, batch, k0, c0 - k0, c0, nb, skip, st));
+        GemmShape g = {c0, k0, k0, c0, c0, c0, N - c0, nb, c0 - k0};
+        CHK(launch_gemm<T>(mb, batch, g, skip, st));
+      }
+      CHK(launch_panel<T>(mb, batch, c0, nb, ipiv, perm, strideP, info, skip, st));
+      c0 += nb;
+    }
+    if (k1 < N) {
+      CHK(launch_trsm<T>(mb, batch, k0, k1 - k0, k1, N - k1, skip, st));
+      GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, N - k1, k1 - k0};
+      CHK(launch_gemm<T>(mb, batch, g, skip, st));
+    }
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, long long strideP,
+                          const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
+                          long long strideY, int nrhs, const int* skip, hipStream_t st) {
+  const int N = mb.N;
+  hipLaunchKernelGGL(k_permute_rhs<T>, dim3((N + 255) / 256, nrhs, batch), dim3(256), 0, st, b, ldb, strideB, y,
+                     ldy, strideY, perm, strideP, N, nrhs, skip);
+  CHK(hipGetLastError());
+  hipLaunchKernelGGL((k_trsv<T, true>), dim3(batch), dim3(1024), 0, st, mb, y, ldy, strideY, nrhs, skip);
+  CHK(hipGetLastError());
+  hipLaunchKernelGGL((k_trsv<T, false>), dim3(batch), dim3(1024), 0, st, mb, y, ldy, strideY, nrhs, skip);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t residual_batch(const MatBatch<T>& mb, int batch, const cplx<double>* b, const cplx<double>* x,
+                          cplx<double>* r, long long ldv, long long strideV, int nrhs, const int* skip,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_residual<T>, dim3((mb.N + 3) / 4, batch), dim3(256), 0, st, mb, b, x, r, ldv, strideV, nrhs,
+                     skip);
+  return hipGetLastError();
+}
+
+hipError_t resnorm_batch(const cplx<double>* r, const cplx<double>* b, long long ldv, long long strideV, int N,
+                         int nrhs, double tol, double* rel, int* done, int iter, int* iters, int batch,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_resnorm, dim3(batch), dim3(256), 0, st, r, b, ldv, strideV, N, nrhs, tol, rel, done, iter,
+                     iters);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t update_batch(cplx<double>* x, const cplx<T>* d, long long ldv, long long ldd, long long strideV,
+                        long long strideD, int N, int nrhs, int init, const int* skip, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(k_update<T>, dim3((N + 255) / 256, nrhs, batch), dim3(256), 0, st, x, d, ldv, ldd, strideV,
+                     strideD, N, nrhs, init, skip);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t growth_batch(const MatBatch<T>& mb, int batch, double* umax, const double* amax, double* growth,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_umax<T>, dim3(mb.N, batch), dim3(256), 0, st, mb, umax);
+  CHK(hipGetLastError());
+  hipLaunchKernelGGL(k_growth, dim3(batch), dim3(256), 0, st, umax, amax, mb.N, growth);
+  return hipGetLastError();
+}
+
+template hipError_t lu_factor_batch<float>(const MatBatch<float>&, int, int*, int*, long long, int*, const int*,
+                                           hipStream_t);
+template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, int*, long long, int*, const int*,
+                                            hipStream_t);
+template hipError_t lu_solve_batch<float>(const MatBatch<float>&, int, const int*, long long, const cplx<double>*,
+                                          long long, long long, cplx<float>*, long long, long long, int, const int*,
+                                          hipStream_t);
+template hipError_t lu_solve_batch<double>(const MatBatch<double>&, int, const int*, long long,
+                                           const cplx<double>*, long long, long long, cplx<double>*, long long,
+                                           long long, int, const int*, hipStream_t);
+template hipError_t residual_batch<float>(const MatBatch<float>&, int, const cplx<double>*, const cplx<double>*,
+                                          cplx<double>*, long long, long long, int, const int*, hipStream_t);
+template hipError_t residual_batch<double>(const MatBatch<double>&, int, const cplx<double>*,
+                                           const cplx<double>*, cplx<double>*, long long, long long, int,
+                                           const int*, hipStream_t);
+template hipError_t update_batch<float>(cplx<double>*, const cplx<float>*, long long, long long, long long,
+                                        long long, int, int, int, const int*, int, hipStream_t);
+template hipError_t update_batch<double>(cplx<double>*, const cplx<double>*, long long, long long, long long,
+                                         long long, int, int, int, const int*, int, hipStream_t);
+template hipError_t growth_batch<float>(const MatBatch<float>&, int, double*, const double*, double*,
+                                        hipStream_t);
+template hipError_t growth_batch<double>(const MatBatch<double>&, int, double*, const double*, double*,
+                                         hipStream_t);

@@ -1,0 +1,177 @@
+/*
+ * bemhip.h — C ABI of the MI355X (gfx950) exterior Helmholtz BEM library (libbemhip.so).
+ *
+ * This is the drop-in seam for the reference's `beat_rocm` device path. The reference binds no C library:
+ * its Julia worker calls AMDGPU.jl kernels and rocBLAS/rocSOLVER directly. Each entry point below replaces
+ * one of those Julia entry points and cites it (paths relative to the reference repository root;
+ * JS = src/blab/solvers/julia_local/src).
+ *
+ * Conventions
+ *  - Every function returns an int status (BEM_OK == 0) and never throws; the message of the last failure on
+ *    the calling thread is available through bem_last_error().
+ *  - Device buffers are caller-owned and passed as raw device pointers plus leading dimensions and batch
+ *    strides in ELEMENTS. Complex numbers are interleaved (re, im). "c128" = complex double, "cT" = complex
+ *    float for BEM_PREC_MIXED and complex double for BEM_PREC_FP64. Matrices are row-major.
+ *  - Handles (bem_ctx, bem_mesh) are opaque and owned by the library; no ownership crosses the ABI.
+ *  - One HIP stream per context; calls on one context are not reentrant. All work is enqueued on that stream
+ *    and is asynchronous unless the function says otherwise.
+ */
+#ifndef BEMHIP_H
+#define BEMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEM_OK 0
+#define BEM_E_ARG 1      /* invalid argument */
+#define BEM_E_HIP 2      /* HIP runtime error */
+#define BEM_E_NOMEM 3    /* device allocation failed */
+#define BEM_E_STATE 4    /* operation not allowed in the current state (e.g. allocation during capture) */
+
+#define BEM_PREC_FP64 0  /* FP64 assembly, FP64 LU, FP64 field */
+#define BEM_PREC_MIXED 1 /* FP32 regular assembly + FP64 singular, FP32 LU (FP32 MFMA), FP64 refinement */
+
+#define BEM_MAX_CHANNELS 4
+
+typedef struct bem_ctx bem_ctx;
+typedef struct bem_mesh bem_mesh;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int bem_version(void);
+
+/* Copy the calling thread's last error message into buf (always NUL-terminated). Returns its length. */
+int bem_last_error(char* buf, int buflen);
+
+/* Create a context on `device`. `stream` is a hipStream_t to enqueue on, or NULL to create one
+ * (default priority). Replaces the device selection / AMDGPU.functional() checks of
+ * JS/BeatEngineRocmCommon.jl:1-85. */
+int bem_ctx_create(int device, void* stream, bem_ctx** out);
+int bem_ctx_destroy(bem_ctx* ctx);
+/* Block until all work on the context's stream is done (AMDGPU.synchronize(),
+ * JS/BeatEngineRocmAssembly.jl:85). */
+int bem_ctx_sync(bem_ctx* ctx);
+/* The hipStream_t used by the context. */
+int bem_ctx_stream(bem_ctx* ctx, void** stream);
+
+/* Host-side description of a prepared boundary mesh (all pointers are HOST memory, copied by
+ * bem_mesh_create). Indices are 0-based. */
+typedef struct bem_mesh_desc {
+  int n_faces;               /* F  (DP0 dofs) */
+  int n_vertices;            /* V  (P1 dofs, N of the dense system) */
+  const int* faces;          /* [F][3] vertex ids */
+  const double* face_vertices; /* [F][9] */
+  const double* normals;     /* [F][3] unit normals, cross(v2-v1, v3-v1) */
+  const double* curls;       /* [F][9] P1 surface curls */
+  const double* area2;       /* [F] 2*area */
+  const double* qpoints;     /* [F][18] order-4 rule points */
+  /* regular-assembly tables: trial elements in locality order, chunks of 64, slot tables */
+  int n_chunks;
+  const int* order;          /* [n_chunks*64], -1 padding */
+  const int* slot_count;     /* [n_chunks] */
+  const uint16_t* slot_start;/* [n_chunks][196] */
+  const uint16_t* slot_inc;  /* [n_chunks][192], (lane << 2) | local column */
+  const int* slot_base;      /* [n_chunks] */
+  int strip_len;             /* L: slots per strip row (padded) */
+  const int* cs_ptr;         /* [V+1] column -> slots CSR */
+  const int* cs_slot;
+  const int* vstar_ptr;      /* [V+1] vertex -> strip rows (3t+l) CSR */
+  const int* vstar_row;
+  /* singular (vertex-sharing) pairs, grouped by test element */
+  int n_pairs;
+  const int* sing_test;      /* [P] */
+  const int* sing_trial;     /* [P] */
+  const int* sing_rule;      /* [P] */
+  const int* sing_ptr;       /* [F+1] */
+  int n_rules;
+  const int* rule_off;       /* [n_rules] */
+  const int* rule_len;       /* [n_rules] */
+  const double* rule_pts;    /* [sum len][5]: test xi, eta, trial xi, eta, weight */
+  double m11[9];             /* reference-element mass block, sum_q phi_l phi_m w_q */
+  double m10[3];             /* sum_q phi_l w_q */
+  /* radiating faces (faces with a non-zero Neumann drive in any channel) */
+  int n_src;
+  const int* src_faces;
+} bem_mesh_desc;
+
+/* Upload a prepared mesh: SoA geometry, locality/slot tables, singular pairs and Duffy rules, field sources.
+ * Replaces build_rocm_regular_assembly_cache (JS/BeatEngineRocmRegular.jl:134-205),
+ * build_rocm_singular_correction_cache (JS/BeatEngineRocmSingular.jl:1-55),
+ * build_rocm_field_evaluation_cache (JS/BeatEngineRocmField.jl:1-39) and the identity cache
+ * (JS/BeatEngineRocmOperators.jl:23-32). Synchronous. */
+int bem_mesh_create(bem_ctx* ctx, const bem_mesh_desc* desc, bem_mesh** out);
+int bem_mesh_destroy(bem_mesh* mesh);
+/* Reserve library workspace for up to `nf` frequencies per call at `precision` (device allocations happen
+ * here, never inside a captured graph). `strip_nf` bounds how many frequencies share one strip buffer. */
+int bem_mesh_reserve(bem_mesh* mesh, int precision, int nf, int strip_nf, int n_obs, int n_ch);
+/* Bytes of device memory held by the mesh (geometry + workspace). */
+int bem_mesh_device_bytes(bem_mesh* mesh, long long* bytes);
+
+/* Assemble, for nf wavenumbers k[f] (device FP64 array), the Burton-Miller LHS
+ *   A_f = 1/2 M11 - D + (i/k) H            (cT, [nf] x N x lda, stride strideA; optional copy A0, same layout)
+ * and channel RHS
+ *   b_f[c] = rhs_scale[f] * (-(S + (i/k)(D* + 1/2 M10)) qunit[c])   (c128, [nf][n_ch][ldb], stride strideB)
+ * with qunit a device c128 [n_ch][F] array of unit channel drives and rhs_scale a device c128 [nf] array
+ * (i rho omega per frequency). rowmax receives max_j |A_ij| (|re|+|im|) as FP64 [nf][N].
+ * Replaces assemble_regular_galerkin_operators_rocm_regular (JS/BeatEngineRocmAssembly.jl:16-193),
+ * _rocm_burton_miller_rhs (JS/BeatEngineRocmOperators.jl:110-121) and the LHS broadcast (:239-240). */
+int bem_assemble(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const double* k, const void* qunit,
+                 int n_ch, const void* rhs_scale, void* A, long long lda, long long strideA, void* A0, void* b,
+                 long long ldb, long long strideB, double* rowmax);
+
+/* In-place LU with partial pivoting of `batch` N x N cT matrices (P A = L U, LAPACK getrf semantics;
+ * ipiv[c] = row exchanged with row c, perm = resulting row permutation; info[m] = 0 or first zero pivot + 1;
+ * int arrays of stride strideP). Replaces rocm_dense_lu! (JS/BeatEngineRocmOperators.jl:161-194). */
+int bem_lu_factor(bem_ctx* ctx, int precision, int n, int batch, void* A, long long lda, long long strideA,
+                  int* ipiv, int* perm, long long strideP, int* info);
+
+/* y = U^-1 L^-1 P b for nrhs right-hand sides (b c128 [batch][nrhs][ldb]; y cT [batch][nrhs][ldy]).
+ * `skip` (device int [batch], may be NULL) skips matrices whose flag is set.
+ * Replaces solve_rocm_dense_factorization (JS/BeatEngineRocmOperators.jl:196-222). */
+int bem_lu_solve(bem_ctx* ctx, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
+                 const int* perm, long long strideP, const void* b, long long ldb, long long strideB, void* y,
+                 long long ldy, long long strideY, int nrhs, const int* skip);
+
+/* Mixed-precision solve with FP64 iterative refinement (no reference counterpart; the reference solves
+ * in FP32 once, JS/BeatEngineRocmOperators.jl:132-159): x (c128) = A^-1 b where A0 is the c64 matrix as
+ * assembled and LU its c64 factorization. Iterates r = b - A0 x (FP64), d = LU^-1 r, x += d until
+ * ||r_c|| / ||b_c|| <= tol for every channel or max_iter corrections. rel[m] receives the final relative
+ * residual, iters[m] the corrections applied. work_r: c128 like b; work_y: c64 [batch][nrhs][ldb];
+ * done: device int [batch] scratch. Works under graph capture (fixed launch count, device-side exit). */
+int bem_refine(bem_ctx* ctx, int n, int batch, const void* LU, long long lda, long long strideA, const int* perm,
+               long long strideP, const void* A0, const void* b, long long ldb, long long strideB, void* x,
+               void* work_r, void* work_y, int nrhs, int max_iter, double tol, double* rel, int* iters, int* done);
+
+/* FP64 relative residual ||b_c - A x_c|| / ||b_c|| (max over channels) of a cT matrix and c128 x, b.
+ * rel: device double [batch]. work_r: c128 like b. scratch: device int [2*batch]. */
+int bem_residual(bem_ctx* ctx, int precision, int n, int batch, const void* A, long long lda, long long strideA,
+                 const void* b, const void* x, long long ldb, long long strideB, void* work_r, int nrhs,
+                 double* rel, int* scratch);
+
+/* Pivot growth max|U| / max|A| in FP64 (|re|+|im|); amax = rowmax from bem_assemble, umax scratch
+ * double [batch][N], growth double [batch]. */
+int bem_growth(bem_ctx* ctx, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
+               const double* amax, double* umax, double* growth);
+
+/* Field at n_obs points (obs: device FP64 [n_obs][3]) for nf frequencies and n_ch channels:
+ *   u = sum_m dG/dn p_w - G q_w     (p: c128 [nf][n_ch][ldp] boundary pressure; q: c128 [nf][n_ch][F]
+ *   Neumann data including i rho omega; out: c128 [nf][n_ch][ldo]).
+ * Replaces evaluate_galerkin_field_rocm (JS/BeatEngineRocmField.jl:150-202). */
+int bem_field(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const double* k, int n_obs, const double* obs,
+              const void* p, long long ldp, long long strideP, const void* q, long long strideQ, int n_ch,
+              void* out, long long ldo, long long strideO);
+
+/* Stream capture into a HIP graph on the context's stream (frequency batching under one graph launch).
+ * bem_graph_end returns an executable graph handle usable with bem_graph_launch. */
+int bem_graph_begin(bem_ctx* ctx);
+int bem_graph_end(bem_ctx* ctx, void** graph_exec);
+int bem_graph_launch(bem_ctx* ctx, void* graph_exec);
+int bem_graph_destroy(void* graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEMHIP_H */

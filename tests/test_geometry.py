@@ -1,0 +1,122 @@
+"""Host mesh preparation (product) against the oracle's independent restatement, plus table invariants."""
+
+import os
+
+import numpy as np
+import pytest
+
+from beat_hip import geometry as G
+from beat_hip import synthetic as syn
+from oracle import bem_oracle as O
+
+REF_MESHES = "/root/reference/src/blab/solvers/julia_local/test_meshes"
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    V, F, T = syn.icosphere(2, 0.1)
+    return G.BoundaryMesh(V, F, T)
+
+
+@pytest.mark.parametrize("gen", ["sphere", "horn", "pistons"])
+def test_generators_closed_oriented(gen):
+    V, F, T = {"sphere": lambda: syn.icosphere(2, 0.1), "horn": lambda: syn.horn_in_box(h=0.05),
+               "pistons": lambda: syn.box_with_pistons(h=0.05)}[gen]()
+    assert syn.check_closed_oriented(F)
+    assert syn.signed_volume(V, F) > 0  # normals point out of the body, into the fluid
+    assert len(V) - len(F) // 2 == 2  # Euler: V - E + F = 2 with E = 3F/2 -> genus 0
+    assert 2 in set(T.tolist())
+
+
+def test_default_workload_sizes():
+    V, F, T = syn.horn_in_box()
+    assert len(F) == 6000 and len(V) == 3002
+    V, F, T = syn.icosphere(3, 0.1)
+    assert len(F) == 1280 and len(V) == 642
+
+
+def test_singular_pairs_match_oracle(sphere):
+    prep = G.prepare(sphere)
+    om = O.Mesh(sphere.vertices, sphere.faces, sphere.tags, sphere.mesh_ids)
+    cache = O.build_singular_cache(om, 4)
+    a = sorted(zip(cache.test.tolist(), cache.trial.tolist()))
+    b = sorted(zip(prep.pairs.test.tolist(), prep.pairs.trial.tolist()))
+    assert a == b
+    ko = {(t, s): cache.keys[r] for t, s, r in zip(cache.test, cache.trial, cache.rule)}
+    for t, s, r in zip(prep.pairs.test, prep.pairs.trial, prep.pairs.rule):
+        key = prep.pairs.keys[r]
+        assert ko[(t, s)] == key
+        tp, sp, w = cache.rules[cache.keys.index(key)]
+        off, n = prep.pairs.rule_off[r], prep.pairs.rule_len[r]
+        got = prep.pairs.rule_pts[off : off + n]
+        assert np.allclose(got[:, :2], tp, atol=1e-15) and np.allclose(got[:, 2:4], sp, atol=1e-15)
+        assert np.allclose(got[:, 4], w, atol=1e-18)
+    # pairs are grouped by test element (CSR)
+    assert np.all(np.diff(prep.pairs.test) >= 0)
+    assert prep.pairs.ptr[-1] == len(prep.pairs.test)
+
+
+def test_curls_and_rule_match_oracle(sphere):
+    prep = G.prepare(sphere)
+    om = O.Mesh(sphere.vertices, sphere.faces, sphere.tags, sphere.mesh_ids)
+    assert np.abs(prep.curls - O.surface_curls(om.face_vertices, om.normals)).max() < 1e-9
+    pts, w = O.triangle_rule(4)
+    assert np.array_equal(pts, G.RULE_POINTS) and np.array_equal(w, G.RULE_WEIGHTS)
+    m11, m10 = G.mass_blocks()
+    assert np.allclose(m11, (np.ones((3, 3)) + np.eye(3)) / 24.0, atol=1e-15)
+    assert np.allclose(m10, 1.0 / 6.0, atol=1e-15)
+
+
+def test_slot_tables_cover_every_incidence_once(sphere):
+    prep = G.prepare(sphere)
+    s = prep.slots
+    F, V = sphere.n_faces, sphere.n_vertices
+    seen = np.zeros((F, 3), dtype=int)
+    slot_vertex = np.full(s.strip_len, -1)
+    for c in range(len(s.slot_count)):
+        for k in range(s.slot_count[c]):
+            for e in range(s.slot_start[c, k], s.slot_start[c, k + 1]):
+                code = int(s.slot_inc[c, e])
+                lane, m = code >> 2, code & 3
+                elem = s.order[c * 64 + lane]
+                assert elem >= 0
+                v = sphere.faces[elem, m]
+                g = s.slot_base[c] + k
+                assert slot_vertex[g] in (-1, v)
+                slot_vertex[g] = v
+                seen[elem, m] += 1
+    assert np.all(seen == 1)
+    # column CSR maps every slot back to its vertex
+    for j in range(V):
+        for e in range(s.cs_ptr[j], s.cs_ptr[j + 1]):
+            assert slot_vertex[s.cs_slot[e]] == j
+    assert s.cs_ptr[-1] == int(s.slot_count.sum())
+    # row CSR: strip rows 3t+l with faces[t, l] == i
+    for i in range(V):
+        rows = s.vstar_row[s.vstar_ptr[i] : s.vstar_ptr[i + 1]]
+        assert np.all(sphere.faces[rows // 3, rows % 3] == i)
+    assert s.vstar_ptr[-1] == 3 * F
+    assert s.strip_len % 64 == 0 and s.strip_len >= s.slot_count.sum()
+
+
+def test_gmsh_roundtrip(tmp_path, sphere):
+    path = tmp_path / "s.msh"
+    syn.write_gmsh22(path, sphere.vertices, sphere.faces, sphere.tags, scale=0.001)
+    V, F, T = G.read_gmsh22(path, 0.001)
+    Vo, Fo, To = O.load_gmsh22(str(path), 0.001)
+    assert np.allclose(V, sphere.vertices, atol=1e-15) and np.array_equal(F, sphere.faces)
+    assert np.array_equal(V, Vo) and np.array_equal(F, Fo) and np.array_equal(T, To)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_MESHES), reason="reference meshes not present")
+def test_reference_meshes_load_and_prepare():
+    # read in place (GPL fixtures are not copied); counts from the reference's own test (runtests.jl:50-63)
+    V, F, T = G.read_gmsh22(os.path.join(REF_MESHES, "sample.msh"), 0.001)
+    assert len(F) == 2776 and len(V) == 1390
+    assert np.sum(T == 2) == 128 and np.sum(T == 1) == 2648
+    V2, F2, T2 = G.read_gmsh22(os.path.join(REF_MESHES, "two_tetrahedra.msh"), 1.0)
+    assert len(F2) == 8 and len(V2) == 8
+    prep = G.prepare(G.BoundaryMesh(V, F, T))
+    om = O.Mesh(V, F, T, np.ones(len(F), int))
+    cache = O.build_singular_cache(om, 4)
+    assert len(cache.test) == len(prep.pairs.test)

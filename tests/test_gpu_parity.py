@@ -1,0 +1,136 @@
+"""GPU parity of the HIP path against the FP64 oracle (pulsating sphere, 1,280 triangles)."""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+A_SPHERE = 0.1
+FREQ = 1000.0
+RHO = 1.205
+C0 = 343.0
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+
+    V, F, T = syn.icosphere(3, A_SPHERE)
+    return G.prepare(G.BoundaryMesh(V, F, T))
+
+
+@pytest.fixture(scope="module")
+def oracle_sphere(sphere):
+    from oracle import bem_oracle as O
+
+    m = sphere.mesh
+    om = O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids)
+    pts = O.fibonacci_observation({"spherical_sampling_enabled": True, "spherical_sampling_points": 1000,
+                                   "distance": 2.0})[0].astype(np.float64)
+    out = O.solve_frequency(om, FREQ, np.ones((m.n_faces, 1)), C0, RHO, points=pts)
+    out["points"] = pts
+    return out
+
+
+def _engine(dev, prep):
+    from beat_hip.engine import Engine
+
+    return Engine(prep, np.ones((prep.n_faces, 1)), dev=dev)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-5)])
+def test_assembly_matches_oracle(gpu_device, sphere, oracle_sphere, precision, tol):
+    eng = _engine(gpu_device, sphere)
+    b = eng.batch(precision, 1, None, RHO, C0)
+    b.set_frequencies([FREQ])
+    with gpu_device.torch.cuda.stream(gpu_device.stream):
+        b._assemble()
+    gpu_device.sync()
+    N = sphere.n_vertices
+    A = b.A0[0, :, :N].cpu().numpy().astype(np.complex128)
+    rhs = b.b[0, 0, :N].cpu().numpy()
+    print(precision, "A rel", _rel(A, oracle_sphere["lhs"]), "b rel", _rel(rhs, oracle_sphere["rhs"][:, 0]))
+    assert _rel(A, oracle_sphere["lhs"]) < tol
+    assert _rel(rhs, oracle_sphere["rhs"][:, 0]) < tol
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precision):
+    eng = _engine(gpu_device, sphere)
+    res, _ = eng.solve([FREQ], precision=precision, obs=oracle_sphere["points"], rho=RHO, sound_speed=C0)
+    r = res[0]
+    p = r.pressure[0, :, 0]
+    u = r.field[0, :, 0]
+    po, uo = oracle_sphere["pressure"][:, 0], oracle_sphere["field"][:, 0]
+    db = np.abs(20 * np.log10(np.abs(u) / np.abs(uo)))
+    ph = np.degrees(np.abs(np.angle(u / uo)))
+    print(precision, "p rel", _rel(p, po), "field max dB", db.max(), "phase", ph.max(), "res", r.residual,
+          "growth", r.growth, "iters", r.iters)
+    assert r.info[0] == 0
+    if precision == "fp64":
+        assert _rel(p, po) < 1e-10
+        assert _rel(u, uo) < 1e-10
+        assert r.residual[0] < 1e-12
+    else:
+        assert _rel(p, po) < 1e-4
+        assert db.max() < 0.05 and ph.max() < 0.5
+        assert r.residual[0] <= 1e-10
+    # analytic pulsating sphere (discretisation error of 1,280 flat triangles)
+    from oracle.bem_oracle import pulsating_sphere_pressure
+
+    k = 2 * math.pi * FREQ / C0
+    pa = pulsating_sphere_pressure(np.linalg.norm(oracle_sphere["points"], axis=1), A_SPHERE, k, RHO,
+                                   2 * math.pi * FREQ)
+    assert np.abs(20 * np.log10(np.abs(u) / np.abs(pa))).max() < 0.05
+    eng.close()
+
+
+@pytest.mark.parametrize("precision,n,batch", [("fp64", 200, 2), ("mixed", 200, 3), ("fp64", 1500, 1),
+                                               ("mixed", 2100, 2)])
+def test_lu_random(gpu_device, precision, n, batch):
+    import ctypes as C
+
+    from beat_hip import _lib
+    from beat_hip.engine import PRECISIONS, _ptr
+
+    torch = gpu_device.torch
+    rng = np.random.default_rng(n + batch)
+    lda = ((n + 15) // 16) * 16
+    M = rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))
+    M += 4 * np.sqrt(n) * np.eye(n)[None] * (rng.standard_normal(batch)[:, None, None] > 5)  # usually no boost
+    B = rng.standard_normal((batch, 2, n)) + 1j * rng.standard_normal((batch, 2, n))
+    cdt = torch.complex64 if precision == "mixed" else torch.complex128
+    dev = gpu_device.device
+    with torch.cuda.stream(gpu_device.stream):
+        A = torch.zeros((batch, n, lda), dtype=cdt, device=dev)
+        A[:, :, :n] = torch.from_numpy(M).to(dev).to(cdt)
+        b = torch.zeros((batch, 2, lda), dtype=torch.complex128, device=dev)
+        b[:, :, :n] = torch.from_numpy(B).to(dev)
+        y = torch.zeros((batch, 2, lda), dtype=cdt, device=dev)
+        ipiv = torch.zeros((batch, n), dtype=torch.int32, device=dev)
+        perm = torch.zeros((batch, n), dtype=torch.int32, device=dev)
+        info = torch.zeros(batch, dtype=torch.int32, device=dev)
+    gpu_device.sync()
+    lib = gpu_device.lib
+    prec = PRECISIONS[precision]
+    _lib.check(lib.bem_lu_factor(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(ipiv), _ptr(perm), n,
+                                 _ptr(info)), "factor")
+    _lib.check(lib.bem_lu_solve(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(perm), n, _ptr(b), lda,
+                                2 * lda, _ptr(y), lda, 2 * lda, 2, None), "solve")
+    gpu_device.sync()
+    X = y[:, :, :n].cpu().numpy().astype(np.complex128)
+    tol = 1e-10 if precision == "fp64" else 5e-4
+    for m in range(batch):
+        ref = np.linalg.solve(M[m], B[m].T).T
+        err = np.linalg.norm(X[m] - ref) / np.linalg.norm(ref)
+        print(precision, n, m, "rel err", err)
+        assert err < tol
+    assert int(info.cpu().numpy().max()) == 0

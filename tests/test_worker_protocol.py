@@ -1,0 +1,186 @@
+"""Worker NDJSON protocol and request semantics, with a test double in place of the GPU engine.
+
+Re-expresses the reference's fake-worker contract tests (tests/test_solver_backends.py:193-267,513-666): event
+order, payload schema accepted by the consumer, failure and cancellation paths. The double only fabricates
+device results; every host step (parsing, drives, synthesis, SPL, impedance, wire format) is the product's.
+"""
+
+import io
+import json
+import math
+
+import numpy as np
+import pytest
+
+from beat_hip import contract as K
+from beat_hip import driver as D
+from beat_hip import synthetic as syn
+from beat_hip import worker as W
+from beat_hip.engine import BatchResult
+
+
+class FakeBatch:
+    def __init__(self, prep, qunit, obs, rho, c, nf):
+        self.prep, self.qunit, self.obs, self.rho, self.c, self.nf = prep, qunit, obs, rho, c, nf
+
+    def run(self, freqs):
+        n = len(freqs)
+        V = self.prep.n_vertices
+        C = self.qunit.shape[1]
+        om = 2 * math.pi * np.asarray(freqs)
+        k = om / self.c
+        r = np.linalg.norm(self.obs, axis=1)
+        field = np.stack([np.exp(1j * kk * r)[:, None] / r[:, None] * np.arange(1, C + 1)[None, :] for kk in k])
+        p = np.ones((n, V, C), complex)
+        q = (1j * self.rho * om)[:, None, None] * self.qunit[None]
+        return BatchResult(freqs=np.asarray(freqs), pressure=p, q=q, field=field, residual=np.full(n, 1e-12),
+                           growth=np.ones(n), iters=np.ones(n, int), info=np.zeros(n, int), t_assembly=0.1,
+                           t_solve=0.2, t_field=0.05)
+
+    def close(self):
+        pass
+
+
+class FakeEngine:
+    def __init__(self, prep, qunit):
+        self.prep, self.qunit = prep, qunit
+
+    def batch(self, precision, nf, obs, rho, c, **kw):
+        return FakeBatch(self.prep, self.qunit, obs, rho, c, nf)
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def mesh_file(tmp_path):
+    V, F, T = syn.icosphere(1, 0.1)
+    path = tmp_path / "sphere.msh"
+    syn.write_gmsh22(path, V, F, T, scale=0.001)
+    return str(path)
+
+
+def _request(mesh_file, freqs=(500.0, 1000.0), **cfg):
+    config = {"mesh_file": mesh_file, "step_size": 30.0, "spherical_sampling_enabled": True,
+              "spherical_sampling_points": 20, "rho": 1.205}
+    config.update(cfg)
+    return {"schema_version": 2, "config": config, "frequencies_hz": list(freqs), "beat_engine_backend": "rocm"}
+
+
+def _run(req):
+    events = []
+    D.run_job(req, FakeEngine, events.append, batch=1)
+    return events
+
+
+def test_event_sequence_and_payload(mesh_file):
+    ev = _run(_request(mesh_file))
+    types = [e["type"] for e in ev]
+    assert types[0] == "initialized" and types[-1] == "completed"
+    assert types.count("result") == 2 and "status" in types
+    init = ev[0]
+    assert init["polar_angle_deg"][0] == -180.0 and len(init["polar_angle_deg"]) == 13
+    assert init["radiator_names"] == ["Radiator"]
+    assert set(init["sphere_metadata"]) == {"r_distance_m", "theta_polar_rad", "phi_azimuth_rad"}
+    results = [e for e in ev if e["type"] == "result"]
+    assert [r["solved_count"] for r in results] == [1, 2]
+    assert all(r["total_count"] == 2 for r in results)
+    res = K.frequency_result_from_dict(json.loads(json.dumps(results[0]["result"])))
+    assert res.freq_hz == 500.0
+    assert res.horizontal_pressure.shape == (1, 13)
+    assert res.sphere_pressure.shape == (1, 20)
+    assert res.impedance.shape == (1, 2)
+    # normalised SPL is 0 dB on axis (solver.jl:568-569)
+    on_axis = int(np.argmin(np.abs(np.asarray(init["polar_angle_deg"]))))
+    assert abs(res.horizontal_spl_norm_db[on_axis]) < 1e-5
+    assert res.diagnostics.backend == "rocm" and res.diagnostics.convergence_info == 0
+    assert ev[-1] == {"type": "completed", "solved_count": 2}
+
+
+def test_channels_and_radiators(mesh_file):
+    req = _request(mesh_file, radiators=[
+        {"name": "A", "tag": 2, "channel": "hf", "velocity_offset_db": 6.0},
+        {"name": "B", "tag": 2, "channel": "lf"}],
+        channels=[{"name": "hf", "level_db": -3.0}, {"name": "lf", "polarity": -1,
+                                                      "lpf": {"type": "lowpass", "frequency_hz": 800, "order": 2}}])
+    ev = _run(req)
+    r = [e for e in ev if e["type"] == "result"][0]["result"]
+    assert r["channel_names"] == ["hf", "lf"]
+    assert len(r["horizontal_pressure"]["real"]) == 2
+    assert ev[0]["radiator_names"] == ["A", "B"]
+
+
+def test_failed_and_validation(mesh_file, tmp_path):
+    with pytest.raises(ValueError, match="schema_version"):
+        _run({**_request(mesh_file), "schema_version": 7})
+    with pytest.raises(ValueError, match="step_size"):
+        _run(_request(mesh_file, step_size=0))
+    with pytest.raises(ValueError, match="unknown channel"):
+        _run(_request(mesh_file, radiators=[{"name": "A", "tag": 2, "channel": "x"}], channels=[{"name": "y"}]))
+    with pytest.raises(ValueError, match="No elements found"):
+        _run(_request(mesh_file, radiators=[{"name": "A", "tag": 9}]))
+    with pytest.raises(ValueError, match="Wavelength"):
+        _run(_request(mesh_file, regular_quadrature_mode="wavelength"))
+
+
+def test_cancel_file(mesh_file, tmp_path):
+    cancel = tmp_path / "cancel"
+    cancel.write_text("cancel")
+    req = _request(mesh_file)
+    req["cancel_path"] = str(cancel)
+    ev = _run(req)
+    assert ev[-1] == {"type": "cancelled", "solved_count": 0}
+
+
+def test_worker_loop_ready_failed_continue(mesh_file, tmp_path, monkeypatch):
+    good = tmp_path / "good.json"
+    good.write_text(json.dumps(_request(mesh_file, freqs=(700.0,))))
+    bad = tmp_path / "bad.json"
+    bad.write_text(json.dumps(_request(str(tmp_path / "missing.msh"))))
+    monkeypatch.setattr(W, "engine_factory", lambda prep, q: FakeEngine(prep, q))
+    out = []
+    stdin = io.StringIO("\n".join(json.dumps({"request": str(p), "operation": "solve"}) for p in (bad, good)) + "\n")
+    W.worker_loop(stdin=stdin, emit=out.append)
+    types = [e["type"] for e in out]
+    assert out[0]["type"] == "ready" and out[0]["protocol"] == "boundary_lab_julia_worker"
+    assert "failed" in types and types[-1] == "completed"
+    assert types.index("failed") < types.index("completed")
+
+
+def test_one_shot_argument_errors(capsys):
+    assert W.main(["--bogus"]) == 1
+    assert json.loads(capsys.readouterr().out.strip())["type"] == "failed"
+    assert W.main([]) == 1
+
+
+def test_crossover_and_impedance_formulas():
+    # Butterworth: |H(fc)| = 1/sqrt(2) for any order; LR = Butterworth^2 -> -6 dB at fc
+    for order in (1, 2, 4, 6):
+        h = D.crossover_response({"type": "lowpass", "frequency_hz": 1000.0, "order": order}, 1000.0)
+        assert abs(abs(h) - 1 / math.sqrt(2)) < 1e-12
+    lr = D.crossover_response({"type": "highpass", "filter": "linkwitz_riley", "frequency_hz": 500, "order": 4}, 500)
+    assert abs(abs(lr) - 0.5) < 1e-12
+    # first-order high-pass: s / (s + wc)
+    h = D.crossover_response({"type": "highpass", "frequency_hz": 100.0, "order": 1}, 300.0)
+    s = 2j * math.pi * 300
+    assert abs(h - s / (s + 2 * math.pi * 100)) < 1e-12
+    from beat_hip import geometry as G
+
+    V, F, T = syn.icosphere(1, 0.1)
+    m = G.BoundaryMesh(V, F, T)
+    rad = [{"tag": 2, "mesh_id": 1}]
+    p = np.full(len(V), 2.0 + 1.0j)
+    imp = D.impedance_for_radiators(m, p, rad, [1.0 + 0j])
+    force = 10.0 * (2.0 + 1.0j) * m.areas.sum()
+    assert np.allclose(imp[0], [np.float32(force.real / 2), np.float32(-force.imag / 2)])
+
+
+def test_polar_points_follow_reference_conventions():
+    angles, h, v, on_axis = D.polar_observation_points({"step_size": 5.0})
+    assert len(angles) == 73 and on_axis == 36 and angles.dtype == np.float32
+    assert np.allclose(h[on_axis], [0, 0, 2.0]) and np.allclose(v[18], [0, -2.0, 0], atol=1e-6)
+    angles, *_ = D.polar_observation_points({"step_size": 7.0, "min_angle": -90, "max_angle": 90})
+    assert angles[-1] == 90.0 and angles[0] == -90.0  # max_angle appended when not on the grid
+    pts, meta = D.spherical_observation({"spherical_sampling_enabled": True, "spherical_sampling_points": 1000})
+    assert pts.shape == (1000, 3) and np.allclose(np.linalg.norm(pts, axis=1), 2.0, atol=1e-5)
+    assert abs(pts[0, 2] - 2.0 * (1 - 1 / 1000)) < 1e-5  # z = 1 - (2i+1)/n ordering
