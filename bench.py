@@ -1,0 +1,283 @@
+"""Benchmark: freq-points/s of the mixed-precision batched solver vs the plain FP64 GPU path at 6,000 triangles.
+
+Workload (synthetic, seeded): exponential horn (throat 25 mm, mouth 300 mm, length 200 mm) flush in a closed
+0.4 x 0.6 x 0.3 m box, 6,000 triangles / 3,002 P1 unknowns, +-5 % vertex jitter (seed 1234), unit normal velocity
+on the throat disk, 40 log-spaced frequencies 100 Hz - 8 kHz, polar arcs (2 x 73 points) at 2 m.
+
+One step = solving all 40 frequencies of that job on the device with the mesh resident: mixed precision
+(FP32 assembly + FP64 singular pairs, FP32-MFMA LU, FP64 refinement to 1e-10), frequencies batched and the
+launch sequence captured once as a HIP graph and replayed. value = 40 * K / (time of K steps).
+The plain FP64 path (FP64 assembly/LU/field, one frequency per launch) is timed once beside it, and every
+mixed result is compared with its FP64 counterpart (far-field dB / phase error).
+
+Also reported: the roofline of the dominant kernel (regular-pair assembly) from HIP events on the library's
+stream, the CPU baseline (C/OpenMP restatement of the BEAT CPU algorithm, FP32, LAPACK cgetrf) on a bounded
+sample, and the serving loop's p50/p99 job latency on a short mixed-size Poisson stream.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "boundary-lab_amd"):
+    sys.path.insert(0, str(p))
+
+METRIC = "freq-points/s @6k tris, mixed vs FP64 GPU (≤0.05 dB); p50/p99 job latency"
+# Algorithmic work of k_regular_lhs per quadrature-point pair (fixed from the kernel's arithmetic, FMA = 2):
+# geometry 9, rsqrt/r 2, phase 2 (+ sin, cos), Green and gradient 10, trial-normal projection 8,
+# inner contraction 24, outer contraction 12 (72 flops per test point / 6 trial points).
+FLOPS_PER_QPAIR = 66
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def horn_job():
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+    from beat_hip import driver as D
+
+    V, F, T = syn.horn_in_box()
+    mesh = G.BoundaryMesh(V, F, T)
+    prep = G.prepare(mesh)
+    qunit = (mesh.tags == 2).astype(np.complex128)[:, None]
+    angles, hor, ver, _ = D.polar_observation_points({"step_size": 5.0})
+    obs = np.concatenate([hor, ver]).astype(np.float64)
+    freqs = np.logspace(np.log10(100.0), np.log10(8000.0), 40)
+    return prep, qunit, obs, freqs
+
+
+def field_errors(fields_mixed, fields_ref):
+    """Max far-field |dB| and phase error of mixed results against the FP64 path, over all frequencies."""
+    db, ph = [], []
+    for a, b in zip(fields_mixed, fields_ref):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            db.append(np.abs(20 * np.log10(np.abs(a) / np.abs(b))).max())
+            ph.append(np.degrees(np.abs(np.angle(a / b))).max())
+    return float(max(db)), float(max(ph))
+
+
+def cpu_baseline(prep, qunit, obs, freqs, budget_s=20.0):
+    """BEAT CPU algorithm restated (FP32, C/OpenMP + LAPACK) on a bounded sample of the same job."""
+    from oracle import beat_cpu as B
+    from oracle import bem_oracle as O
+
+    m = prep.mesh
+    om = O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids)
+    cm = B.CpuMesh(om, "single")
+    done, t0 = 0, time.perf_counter()
+    sample = [freqs[len(freqs) // 2], freqs[-1], freqs[0]]
+    for f in sample:
+        cm.solve_frequency(float(f), qunit, 343.0, 1.205, obs)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "freq-points/s", "cores": B.threads(), "kind": "port",
+            "sample": f"{done} of the 40 frequencies ({', '.join(f'{f:.0f} Hz' for f in sample[:done])}) of the "
+                      f"6k-triangle horn job, BEAT CPU algorithm restated in C/OpenMP FP32 + LAPACK cgetrf"}
+
+
+def serving_sample(n_jobs, rate, seed, batch, slots):
+    """Seeded Poisson stream of mixed-size jobs through the serving scheduler; returns latency stats."""
+    import tempfile
+
+    from beat_hip import serving as S
+    from beat_hip import synthetic as syn
+
+    rng = np.random.default_rng(seed)
+    tmp = Path(tempfile.mkdtemp(prefix="beat-serve-"))
+    jobs = []
+    for j in range(n_jobs):
+        n_tri = float(np.exp(rng.uniform(np.log(1000), np.log(15000))))
+        kind = ["sphere", "horn", "pistons"][j % 3]
+        V, F, T = syn.mesh_for_triangle_count(kind, int(n_tri), seed=seed + j)
+        path = tmp / f"job{j}.msh"
+        syn.write_gmsh22(path, V, F, T, scale=1.0)
+        tags = sorted(set(T.tolist()) - {1}) or [2]
+        nsrc = min(len(tags), int(rng.integers(1, 4)))
+        rads = [{"name": f"r{i}", "tag": int(tags[i]), "channel": f"c{i}"} for i in range(nsrc)]
+        nf = int(rng.integers(8, 61))
+        fr = np.sort(np.exp(rng.uniform(np.log(100), np.log(10000), nf))).tolist()
+        req = {"schema_version": 2, "frequencies_hz": fr,
+               "config": {"mesh_file": str(path), "scale_factor": 1.0, "radiators": rads, "step_size": 5.0}}
+        jobs.append((len(F), nf, req))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    sched = S.Scheduler(slots=slots, batch=batch)
+    handles = []
+    t0 = time.perf_counter()
+    for (nt, nf, req), ta in zip(jobs, arrivals):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        h, _ = sched.submit(req)
+        handles.append(h)
+    total_f = 0
+    for h in handles:
+        while True:
+            ev = h.events.get()
+            if ev["type"] == "result":
+                total_f += 1
+            if ev["type"] in ("completed", "failed", "cancelled"):
+                if ev["type"] == "failed":
+                    raise RuntimeError(ev["error"])
+                break
+    wall = time.perf_counter() - t0
+    sched.shutdown()
+    lat = [h.latency for h in handles]
+    return {"jobs": n_jobs, "rate_jobs_per_s": rate, "freq_points": total_f, "wall_s": wall,
+            "throughput_freq_points_per_s": total_f / wall, "p50_latency_s": S.percentile(lat, 50),
+            "p99_latency_s": S.percentile(lat, 99), "tri_min": min(j[0] for j in jobs),
+            "tri_max": max(j[0] for j in jobs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=40)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=16)
+    ap.add_argument("--serving-rate", type=float, default=2.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build runs on a single MI355X); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from beat_hip.engine import Engine
+
+    log("bench: building 6k-triangle horn-in-box job")
+    prep, qunit, obs, freqs = horn_job()
+    eng = Engine(prep, qunit)
+    dev = eng.dev
+    nb = max(1, min(args.batch, len(freqs)))
+
+    # ---- mixed, batched, graph-captured -------------------------------------------------------
+    log(f"bench: mixed batched path, batch={nb}, capturing graph")
+    b = eng.batch("mixed", nb, obs, 1.205, 343.0)
+    b.capture()
+    for _ in range(args.warmup):
+        [b.run(freqs[i : i + nb]) for i in range(0, len(freqs), nb)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mixed = None
+    for _ in range(args.steps):
+        mixed = [b.run(freqs[i : i + nb]) for i in range(0, len(freqs), nb)]
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    ms_per_step = 1e3 * t_mixed / args.steps
+    value = len(freqs) * args.steps / t_mixed
+    log(f"bench: mixed {value:.2f} freq-points/s ({ms_per_step:.1f} ms per 40-frequency job)")
+    max_iters = int(max(int(r.iters.max()) for r in mixed))
+    max_res = float(max(float(r.residual.max()) for r in mixed))
+    b.close()
+
+    # ---- roofline of the regular assembly kernel (HIP events on the library stream) ---------------
+    from beat_hip import _lib
+
+    nf_t = min(8, nb)
+    bt = eng.batch("mixed", nf_t, obs, 1.205, 343.0)
+    bt.set_frequencies(freqs[-nf_t:])
+    with torch.cuda.stream(dev.stream):
+        bt._assemble()
+    dev.sync()
+    dev.timers(True)
+    reps = 3
+    with torch.cuda.stream(dev.stream):
+        for _ in range(reps):
+            bt._assemble()
+    dev.sync()
+    t_ms, units = dev.timer(_lib.TIMER_REGULAR)
+    dev.timers(False)
+    launches = max(1, units // nf_t)
+    avg_ms = t_ms / launches
+    F = prep.n_faces
+    qpairs = (F * F - len(prep.pairs.test)) * 36 * nf_t
+    achieved = qpairs * FLOPS_PER_QPAIR / (avg_ms * 1e-3) / 1e12
+    bt.close()
+    log(f"bench: k_regular_lhs<float> avg {avg_ms:.3f} ms/launch for {nf_t} freqs -> {achieved:.1f} TFLOP/s")
+
+    # ---- plain FP64 path: one frequency per launch ------------------------------------------------
+    fp64_value, db_err, ph_err = None, None, None
+    if not args.no_fp64:
+        log("bench: plain FP64 path, one frequency per launch")
+        b1 = eng.batch("fp64", 1, obs, 1.205, 343.0)
+        b1.run(freqs[:1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref = [b1.run(freqs[i : i + 1]) for i in range(len(freqs))]
+        torch.cuda.synchronize()
+        t_fp64 = time.perf_counter() - t0
+        b1.close()
+        fp64_value = len(freqs) / t_fp64
+        fm = [r.field[j] for r in mixed for j in range(len(r.freqs))]
+        fr = [r.field[0] for r in ref]
+        db_err, ph_err = field_errors(fm, fr)
+        log(f"bench: fp64 {fp64_value:.2f} freq-points/s; mixed/fp64 = {value / fp64_value:.2f}x; "
+            f"max field error {db_err:.4f} dB / {ph_err:.3f} deg")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline sample")
+        cpu = cpu_baseline(prep, qunit, obs, freqs)
+        log(f"bench: cpu {cpu['value']:.3f} freq-points/s on {cpu['cores']} threads")
+    eng.close()
+
+    serving = None
+    if args.serving_jobs > 0:
+        log(f"bench: serving stream, {args.serving_jobs} jobs at {args.serving_rate}/s")
+        serving = serving_sample(args.serving_jobs, args.serving_rate, 2024, batch=8, slots=4)
+        log(f"bench: serving p50 {serving['p50_latency_s']:.2f} s p99 {serving['p99_latency_s']:.2f} s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "freq-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": "horn-in-box 6000 tris / 3002 P1 dofs, 40 log freqs 100 Hz-8 kHz, mixed precision "
+                               "batched (hipGraph), polar 2x73 pts at 2 m", "triangles": int(F),
+                   "p1_dofs": int(prep.n_vertices), "frequencies": int(len(freqs)), "batch": nb,
+                   "precision": "mixed"},
+        "fp64_gpu_value": None if fp64_value is None else round(fp64_value, 4),
+        "mixed_over_fp64": None if fp64_value is None else round(value / fp64_value, 3),
+        "max_field_error_db": db_err,
+        "max_field_error_deg": ph_err,
+        "max_refinement_iterations": max_iters,
+        "max_relative_residual": max_res,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "k_regular_lhs<float>", "avg_launch_ms": round(avg_ms, 4),
+                     "flops_per_qpair": FLOPS_PER_QPAIR, "qpairs_per_launch": int(qpairs)},
+        "cpu_baseline": cpu,
+        "serving": serving,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

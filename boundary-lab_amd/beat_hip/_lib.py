@@ -16,6 +16,7 @@ BEM_OK = 0
 BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_MAX_CHANNELS = 4
+TIMER_REGULAR, TIMER_SINGULAR, TIMER_GATHER, TIMER_RHS, TIMER_LU, TIMER_FIELD = range(6)
 
 _p = C.c_void_p
 _i = C.c_int
@@ -71,6 +72,8 @@ SIGNATURES = {
     "bem_ctx_destroy": (_i, [_p]),
     "bem_ctx_sync": (_i, [_p]),
     "bem_ctx_stream": (_i, [_p, C.POINTER(_p)]),
+    "bem_ctx_timers": (_i, [_p, _i]),
+    "bem_ctx_timer_read": (_i, [_p, _i, C.POINTER(_d), C.POINTER(_ll)]),
     "bem_mesh_create": (_i, [_p, C.POINTER(MeshDesc), C.POINTER(_p)]),
     "bem_mesh_destroy": (_i, [_p]),
     "bem_mesh_reserve": (_i, [_p, _i, _i, _i, _i, _i]),

@@ -54,6 +54,15 @@ class Device:
     def sync(self):
         _lib.check(self.lib.bem_ctx_sync(self.ctx), "bem_ctx_sync")
 
+    def timers(self, enable: bool):
+        _lib.check(self.lib.bem_ctx_timers(self.ctx, int(bool(enable))), "bem_ctx_timers")
+
+    def timer(self, slot: int):
+        """(accumulated ms, accumulated work units) of a kernel timer slot."""
+        ms, units = C.c_double(), C.c_longlong()
+        _lib.check(self.lib.bem_ctx_timer_read(self.ctx, slot, C.byref(ms), C.byref(units)), "bem_ctx_timer_read")
+        return float(ms.value), int(units.value)
+
     def close(self):
         if self.ctx:
             self.lib.bem_ctx_destroy(self.ctx)

@@ -26,12 +26,41 @@ static int hipfail(hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hipfail(e_, what); \
   } while (0)
 
+#define BEM_TIMER_SLOTS 8
+
 struct bem_ctx {
   int device;
   hipStream_t stream;
   bool own_stream;
   hipGraph_t graph;
   bool capturing;
+  bool timing;
+  hipEvent_t ev0, ev1;
+  double timer_ms[BEM_TIMER_SLOTS];
+  long long timer_count[BEM_TIMER_SLOTS];
+};
+
+// Kernel timing on the context's stream (diagnostics): brackets one launch with events and waits for it.
+struct ScopedTimer {
+  bem_ctx* c;
+  int slot;
+  long long units;
+  bool on;
+  ScopedTimer(bem_ctx* c_, int slot_, long long units_) : c(c_), slot(slot_), units(units_) {
+    on = c->timing && !c->capturing;
+    if (on) (void)hipEventRecord(c->ev0, c->stream);
+  }
+  ~ScopedTimer() {
+    if (!on) return;
+    (void)hipEventRecord(c->ev1, c->stream);
+    if (hipEventSynchronize(c->ev1) == hipSuccess) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) {
+        c->timer_ms[slot] += ms;
+        c->timer_count[slot] += units;
+      }
+    }
+  }
 };
 
 struct DevBuf {
@@ -113,6 +142,15 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   c->device = device;
   c->capturing = false;
   c->graph = nullptr;
+  c->timing = false;
+  for (int i = 0; i < BEM_TIMER_SLOTS; ++i) {
+    c->timer_ms[i] = 0.0;
+    c->timer_count[i] = 0;
+  }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(BEM_E_HIP, "hipEventCreate failed");
+  }
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
     c->own_stream = false;
@@ -131,6 +169,8 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
 int bem_ctx_destroy(bem_ctx* c) {
   if (!c) return BEM_OK;
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
   delete c;
   return BEM_OK;
 }
@@ -138,6 +178,24 @@ int bem_ctx_destroy(bem_ctx* c) {
 int bem_ctx_sync(bem_ctx* c) {
   if (!c) return fail(BEM_E_ARG, "ctx is NULL");
   HIPCHK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return BEM_OK;
+}
+
+int bem_ctx_timers(bem_ctx* c, int enable) {
+  if (!c) return fail(BEM_E_ARG, "ctx is NULL");
+  c->timing = enable != 0;
+  if (enable)
+    for (int i = 0; i < BEM_TIMER_SLOTS; ++i) {
+      c->timer_ms[i] = 0.0;
+      c->timer_count[i] = 0;
+    }
+  return BEM_OK;
+}
+
+int bem_ctx_timer_read(bem_ctx* c, int slot, double* ms, long long* units) {
+  if (!c || !ms || !units || slot < 0 || slot >= BEM_TIMER_SLOTS) return fail(BEM_E_ARG, "invalid timer query");
+  *ms = c->timer_ms[slot];
+  *units = c->timer_count[slot];
   return BEM_OK;
 }
 
@@ -327,7 +385,10 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   sp.P = m->P;
   std::memcpy(sp.m11, m->m11, sizeof(sp.m11));
   std::memcpy(sp.m10, m->m10, sizeof(sp.m10));
-  HIPCHK(launch_singular(sp, nf, st), "k_singular launch");
+  {
+    ScopedTimer tm(c, BEM_TIMER_SINGULAR, (long long)nf);
+    HIPCHK(launch_singular(sp, nf, st), "k_singular launch");
+  }
   // rhs partial sums
   RhsParams<T> rp;
   rp.rec = precision == BEM_PREC_MIXED ? (const ElemRec<T>*)m->rec32 : (const ElemRec<T>*)m->rec64;
@@ -343,7 +404,10 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   rp.P = m->P;
   rp.n_src = m->n_src;
   rp.n_ch = n_ch;
-  HIPCHK(launch_rhs<T>(rp, nf, st), "k_rhs_part launch");
+  {
+    ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
+    HIPCHK(launch_rhs<T>(rp, nf, st), "k_rhs_part launch");
+  }
   // regular strips + gather, in strip-sized frequency groups
   const long long fstride = 3LL * m->F * m->L;
   for (int f0 = 0; f0 < nf; f0 += m->strip_nf) {
@@ -366,7 +430,10 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.F = m->F;
     p.n_chunks = m->n_chunks;
     p.P = m->P;
-    HIPCHK(launch_regular<T>(p, g, st), "k_regular_lhs launch");
+    {
+      ScopedTimer tm(c, BEM_TIMER_REGULAR, (long long)g);
+      HIPCHK(launch_regular<T>(p, g, st), "k_regular_lhs launch");
+    }
     GatherParams<T> gp;
     gp.strip = p.strip;
     gp.strip_fstride = fstride;
@@ -388,7 +455,10 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     gp.N = m->V;
     gp.F = m->F;
     gp.n_ch = n_ch;
-    HIPCHK(launch_gather<T>(gp, g, st), "k_gather launch");
+    {
+      ScopedTimer tm(c, BEM_TIMER_GATHER, (long long)g);
+      HIPCHK(launch_gather<T>(gp, g, st), "k_gather launch");
+    }
   }
   return BEM_OK;
 }
@@ -417,6 +487,7 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
   if (n <= 0 || batch <= 0 || lda < n || strideA < lda * n || strideP < n) return fail(BEM_E_ARG, "invalid sizes");
   if (n > 12288) return fail(BEM_E_ARG, "N above 12288 is not supported by the register panel");
   hipError_t e;
+  ScopedTimer tm(c, BEM_TIMER_LU, (long long)batch);
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
     e = lu_factor_batch<float>(mb, batch, ipiv, perm, strideP, info, nullptr, c->stream);
@@ -534,6 +605,7 @@ int bem_field(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, i
   if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->M * 2 * cb))) return rc;
   if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
   hipError_t e;
+  ScopedTimer tm(c, BEM_TIMER_FIELD, (long long)nf);
   if (precision == BEM_PREC_MIXED) {
     e = field_batch<float>(m->src32, m->fv, m->srcw32, obs, m->M, n_obs, k, (const cplx<double>*)p, ldp,
                            strideP, (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<float>*)m->wsrc.p,

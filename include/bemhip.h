@@ -57,6 +57,18 @@ int bem_ctx_sync(bem_ctx* ctx);
 /* The hipStream_t used by the context. */
 int bem_ctx_stream(bem_ctx* ctx, void** stream);
 
+/* Kernel timers (no reference counterpart; diagnostics for roofline measurement). While enabled, each timed
+ * launch group is bracketed by HIP events on the context's stream and waited for (not during capture).
+ * bem_ctx_timer_read returns accumulated milliseconds and accumulated work units (frequencies or matrices). */
+#define BEM_TIMER_REGULAR 0  /* regular-pair assembly kernel */
+#define BEM_TIMER_SINGULAR 1 /* Duffy singular-pair kernel */
+#define BEM_TIMER_GATHER 2   /* row gather */
+#define BEM_TIMER_RHS 3      /* RHS partial sums */
+#define BEM_TIMER_LU 4       /* whole LU factorization sequence */
+#define BEM_TIMER_FIELD 5    /* field evaluation sequence */
+int bem_ctx_timers(bem_ctx* ctx, int enable);
+int bem_ctx_timer_read(bem_ctx* ctx, int slot, double* ms, long long* units);
+
 /* Host-side description of a prepared boundary mesh (all pointers are HOST memory, copied by
  * bem_mesh_create). Indices are 0-based. */
 typedef struct bem_mesh_desc {
