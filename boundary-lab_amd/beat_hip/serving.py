@@ -79,6 +79,7 @@ class Scheduler:
         self._stop = False
         self._devices = []
         self.completed: list[JobHandle] = []
+        self.peak_running = 0
         self._dispatcher = threading.Thread(target=self._dispatch_loop, name="beat-hip-dispatch", daemon=True)
         self._dispatcher.start()
 
@@ -154,6 +155,7 @@ class Scheduler:
                 slot = free_slots.pop(0)
                 self._used += h.footprint
                 self._running += 1
+                self.peak_running = max(self.peak_running, self._running)
                 h.t_admit = time.perf_counter()
             th = threading.Thread(target=self._run_job, args=(slot, h, job, prep), name=f"beat-hip-slot{slot}",
                                   daemon=True)
@@ -168,8 +170,10 @@ class Scheduler:
             if h.cancel_flag.is_set():
                 h.events.put({"type": "cancelled", "solved_count": 0})
                 return
-            dev = self._device(slot)
-            eng = (self._engine_factory or (lambda p, q: Engine(p, q, dev=dev)))(prep, job.qunit)
+            if self._engine_factory is not None:
+                eng = self._engine_factory(prep, job.qunit)
+            else:
+                eng = Engine(prep, job.qunit, dev=self._device(slot))
             freqs = job.frequencies.astype(np.float64)
             nb = max(1, min(self.batch, len(freqs)))
             b = eng.batch(job.precision, nb, job.obs, job.rho, job.sound_speed)
