@@ -74,7 +74,7 @@ def cpu_baseline(prep, qunit, obs, freqs, budget_s=20.0):
     from oracle import beat_cpu as B
     from oracle import bem_oracle as O
 
-    m = prep.mesh
+    m = prep.original  # qunit is in the original face order
     om = O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids)
     cm = B.CpuMesh(om, "single")
     done, t0 = 0, time.perf_counter()

@@ -6,7 +6,7 @@ Mirrors BeatEngineRocmBackend (src/blab/solvers/beat_engine_backend.py:465-585) 
 (src/blab/solvers/base.py:92-110). Unlike the subprocess adapter, sessions from several server threads share one
 device scheduler (continuous batching), so ``blab server --max-running-jobs N`` overlaps jobs on the GPU.
 
-Register with Boundary Lab's registry by replacing the ``beat_rocm`` factory with
+Register with Boundary Lab's registry (src/blab/solvers/registry.py:59-71) by replacing the ``beat_rocm`` factory with
 ``lambda **kw: BeatHipRocmBackend(**kw)``.
 """
 

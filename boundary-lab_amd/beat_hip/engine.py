@@ -319,8 +319,12 @@ class Batch:
         if self.graph is not None:
             t_as, t_so, t_fi = 0.0, ev[0].elapsed_time(ev[3]) / 1e3, 0.0
         N = self.N
-        p = self.x[:n, :, :N].cpu().numpy().transpose(0, 2, 1)
-        q = self.q[:n].cpu().numpy().transpose(0, 2, 1)
+        prep = self.dmesh.prep
+        # back to the caller's vertex / face numbering
+        p = prep.unpermute_vertices(self.x[:n, :, :N].cpu().numpy().transpose(0, 2, 1), axis=1)
+        qi = self.q[:n].cpu().numpy().transpose(0, 2, 1)
+        q = np.empty_like(qi)
+        q[:, prep.fperm, :] = qi
         field = self.field[:n].cpu().numpy().transpose(0, 2, 1) if self.n_obs else None
         return BatchResult(
             freqs=freqs, pressure=p, q=q, field=field,
@@ -341,7 +345,8 @@ class Engine:
     def __init__(self, prep: PreparedMesh, qunit: np.ndarray, dev: Device | None = None):
         self.dev = dev or Device(0)
         self.prep = prep
-        self.qunit = np.asarray(qunit, dtype=np.complex128).reshape(prep.n_faces, -1)
+        # unit drives arrive in the caller's (original) face order; the device works in locality order
+        self.qunit = prep.permute_faces(np.asarray(qunit, dtype=np.complex128).reshape(prep.n_faces, -1))
         src = np.nonzero(np.any(self.qunit != 0, axis=1))[0]
         self.dmesh = DeviceMesh(self.dev, prep, src)
 

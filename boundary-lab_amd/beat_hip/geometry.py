@@ -442,6 +442,10 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
 
 @dataclass
 class PreparedMesh:
+    """Device-ready mesh. ``mesh`` is in locality order: ``fperm[i]`` / ``vperm[i]`` are the original face /
+    vertex ids of internal face / vertex ``i``; per-face inputs are permuted with ``fperm`` and per-vertex
+    outputs restored with ``unpermute_vertices``."""
+
     mesh: BoundaryMesh
     curls: np.ndarray
     qpoints: np.ndarray
@@ -449,6 +453,9 @@ class PreparedMesh:
     slots: SlotTables
     m11: np.ndarray
     m10: np.ndarray
+    fperm: np.ndarray
+    vperm: np.ndarray
+    original: BoundaryMesh
 
     @property
     def n_faces(self):
@@ -458,9 +465,41 @@ class PreparedMesh:
     def n_vertices(self):
         return self.mesh.n_vertices
 
+    def permute_faces(self, per_face):
+        """Original face order -> internal order (first axis)."""
+        return np.asarray(per_face)[self.fperm]
 
-def prepare(mesh: BoundaryMesh, singular_order: int = 4) -> PreparedMesh:
-    curls = surface_curls(mesh.face_vertices, mesh.normals)
-    qp = quadrature_points(mesh.face_vertices)
+    def unpermute_vertices(self, per_vertex, axis=0):
+        """Internal vertex order -> original order along ``axis``."""
+        a = np.asarray(per_vertex)
+        out = np.empty_like(a)
+        idx = [slice(None)] * a.ndim
+        idx[axis] = self.vperm
+        out[tuple(idx)] = a
+        return out
+
+
+def locality_order(mesh: BoundaryMesh):
+    """Faces in Morton order of their centroids; vertices in order of first use by those faces."""
+    fperm = np.argsort(_morton(mesh.face_vertices.mean(axis=1)), kind="stable")
+    flat = mesh.faces[fperm].reshape(-1)
+    ids, first = np.unique(flat, return_index=True)
+    used = ids[np.argsort(first, kind="stable")]
+    unused = np.setdiff1d(np.arange(mesh.n_vertices), ids)
+    vperm = np.concatenate([used, unused]).astype(np.int64)
+    return fperm.astype(np.int64), vperm
+
+
+def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True) -> PreparedMesh:
+    if reorder:
+        fperm, vperm = locality_order(mesh)
+    else:
+        fperm, vperm = np.arange(mesh.n_faces), np.arange(mesh.n_vertices)
+    vinv = np.empty_like(vperm)
+    vinv[vperm] = np.arange(len(vperm))
+    work = BoundaryMesh(mesh.vertices[vperm], vinv[mesh.faces[fperm]], mesh.tags[fperm], mesh.mesh_ids[fperm])
+    curls = surface_curls(work.face_vertices, work.normals)
+    qp = quadrature_points(work.face_vertices)
     m11, m10 = mass_blocks()
-    return PreparedMesh(mesh, curls, qp, singular_pairs(mesh, singular_order), slot_tables(mesh), m11, m10)
+    return PreparedMesh(work, curls, qp, singular_pairs(work, singular_order), slot_tables(work), m11, m10, fperm,
+                        vperm, mesh)

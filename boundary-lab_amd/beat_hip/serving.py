@@ -166,9 +166,10 @@ class Scheduler:
         from .engine import Engine
 
         eng = None
+        terminal = None
         try:
             if h.cancel_flag.is_set():
-                h.events.put({"type": "cancelled", "solved_count": 0})
+                terminal = {"type": "cancelled", "solved_count": 0}
                 return
             if self._engine_factory is not None:
                 eng = self._engine_factory(prep, job.qunit)
@@ -180,7 +181,7 @@ class Scheduler:
             solved = 0
             for i0 in range(0, len(freqs), nb):
                 if h.cancel_flag.is_set() or D.cancelled(job):
-                    h.events.put({"type": "cancelled", "solved_count": solved})
+                    terminal = {"type": "cancelled", "solved_count": solved}
                     return
                 fr = freqs[i0 : i0 + nb]
                 res = b.run(fr)
@@ -195,18 +196,20 @@ class Scheduler:
                     h.events.put({"type": "result", "solved_count": solved, "total_count": len(freqs),
                                   "result": payload})
             b.close()
-            h.events.put({"type": "completed", "solved_count": len(freqs)})
+            terminal = {"type": "completed", "solved_count": len(freqs)}
         except Exception as exc:
-            h.events.put({"type": "failed", "error": f"{type(exc).__name__}: {exc}"})
+            terminal = {"type": "failed", "error": f"{type(exc).__name__}: {exc}"}
         finally:
             if eng is not None:
                 eng.close()
+            # latency is final before the terminal event becomes visible to the consumer
             h.t_done = time.perf_counter()
             with self._lock:
                 self._used -= h.footprint
                 self._running -= 1
                 self.completed.append(h)
                 self._lock.notify_all()
+            h.events.put(terminal or {"type": "failed", "error": "job ended without a terminal event"})
 
 
 _SCHEDULER = None

@@ -1,6 +1,6 @@
 """Executable worker script: drop-in ``solver_script`` for Boundary Lab's BeatEngineBackend.
 
-Usage from Boundary Lab (see INTEGRATION.md):
+Usage from Boundary Lab:
     BeatEngineRocmBackend(julia_executable=sys.executable, solver_script="<repo>/boundary-lab_amd/beat_rocm_worker.py",
                           julia_project=None)
 which runs ``python beat_rocm_worker.py --worker`` and speaks the reference worker's NDJSON protocol.

@@ -35,9 +35,21 @@
 
 
 
+// phi_l(a) * w_a and psi_m(b) * w_b for the six rule points (index l*6 + a).
+struct RuleWeights {
+  double v[18];
+  constexpr RuleWeights() : v() {
+    for (int l = 0; l < 3; ++l)
+      for (int a = 0; a < 6; ++a) v[l * 6 + a] = rule_phi(l, a) * rule_w(a);
+  }
+};
+__constant__ RuleWeights c_phiw = RuleWeights();
+
 // 3x3 LHS block of a regular (t,s) pair, scaled and combined: C = J/(4pi) (-D + (i/k) H).
-template <typename T>
-__device__ __forceinline__ void regular_block(const T (&x)[18], const T (&nt)[3], const T (&ct)[9], T a2t,
+// UNROLL_A = 6 fully unrolls the test-point loop (FP32: test points live in SGPRs); UNROLL_A = 1 keeps it
+// rolled with wave-uniform scalar loads of the test point (FP64: halves register pressure, no spills).
+template <typename T, int UNROLL_A>
+__device__ __forceinline__ void regular_block(const T* __restrict__ xq, const T (&nt)[3], const T (&ct)[9], T a2t,
                                               const ElemRec<T>& se, T k, T kappa, cplx<T> (&C)[9]) {
   T y[18];
 #pragma unroll
@@ -46,14 +58,15 @@ __device__ __forceinline__ void regular_block(const T (&x)[18], const T (&nt)[3]
   T DMr[9], DMi[9], GMr[9], GMi[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) DMr[i] = DMi[i] = GMr[i] = GMi[i] = T(0);
-#pragma unroll
+#pragma unroll UNROLL_A
   for (int a = 0; a < 6; ++a) {
+    const T xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
     T tDr[3] = {0, 0, 0}, tDi[3] = {0, 0, 0}, tGr[3] = {0, 0, 0}, tGi[3] = {0, 0, 0};
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-      const T dx = y[3 * b + 0] - x[3 * a + 0];
-      const T dy = y[3 * b + 1] - x[3 * a + 1];
-      const T dz = y[3 * b + 2] - x[3 * a + 2];
+      const T dx = y[3 * b + 0] - xa;
+      const T dy = y[3 * b + 1] - ya;
+      const T dz = y[3 * b + 2] - za;
       const T r2 = dx * dx + dy * dy + dz * dz;
       const T rinv = Phase<T>::rsqrt_(r2);
       const T r = r2 * rinv;
@@ -75,7 +88,7 @@ __device__ __forceinline__ void regular_block(const T (&x)[18], const T (&nt)[3]
     }
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-      const T wa = T(rule_phi(l, a) * rule_w(a));
+      const T wa = T(c_phiw.v[l * 6 + a]);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         DMr[3 * l + m] += wa * tDr[m];
@@ -120,9 +133,7 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
   const T k = T(kd);
   const T kappa = T(kd / (2.0 * BEM_PI));
   const ElemRec<T>* te = p.rec + t;
-  T x[18], nt[3], ct[9];
-#pragma unroll
-  for (int i = 0; i < 18; ++i) x[i] = te->q[i];
+  T nt[3], ct[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) nt[i] = te->n[i];
 #pragma unroll
@@ -146,7 +157,7 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
       const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
                        sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
       if (!adj) {
-        regular_block<T>(x, nt, ct, a2t, p.rec[s], k, kappa, C);
+        regular_block<T, sizeof(T) == 4 ? 6 : 1>(te->q, nt, ct, a2t, p.rec[s], k, kappa, C);
       } else {
         for (int q = sp0; q < sp1; ++q) {
           if (p.sing_trial[q] == s) {
@@ -192,40 +203,39 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
 // ------------------------------------------------------------------------------------------------
 
 
-__global__ __launch_bounds__(64) void k_singular(SingParams p) {
-  const int pi = blockIdx.x;
-  const int f = blockIdx.y;
-  const int lane = threadIdx.x;
+// One lane per (pair, frequency) item; items run pair-major over pairs sorted by Duffy rule, frequency fastest,
+// so the lanes of a wave share one rule (uniform trip count, broadcast rule-point loads) and need no
+// cross-lane reduction: each lane integrates its pair at its wavenumber over all Duffy points in FP64.
+__global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
+  const long long item = (long long)blockIdx.x * 64 + threadIdx.x;
+  if (item >= (long long)p.P * nf) return;
+  const int pi = p.by_rule[item / nf];
+  const int f = (int)(item % nf);
   const int t = p.test[pi], s = p.trial[pi], ru = p.rule[pi];
   const double k = p.k[f];
-  double tv[9], sv[9], nt[3], ns[3];
+  double t0[3], te1[3], te2[3], s0[3], se1[3], se2[3], nt[3], ns[3];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    tv[i] = p.fvtx[9 * t + i];
-    sv[i] = p.fvtx[9 * s + i];
-  }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    nt[i] = p.fnrm[3 * t + i];
-    ns[i] = p.fnrm[3 * s + i];
+  for (int c = 0; c < 3; ++c) {
+    t0[c] = p.fvtx[9 * t + c];
+    te1[c] = p.fvtx[9 * t + 3 + c] - t0[c];
+    te2[c] = p.fvtx[9 * t + 6 + c] - t0[c];
+    s0[c] = p.fvtx[9 * s + c];
+    se1[c] = p.fvtx[9 * s + 3 + c] - s0[c];
+    se2[c] = p.fvtx[9 * s + 6 + c] - s0[c];
+    nt[c] = p.fnrm[3 * t + c];
+    ns[c] = p.fnrm[3 * s + c];
   }
   double Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
   double Dr[9], Di[9], Gr[9], Gi[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Dr[i] = Di[i] = Gr[i] = Gi[i] = 0.0;
   const int off = p.rule_off[ru], len = p.rule_len[ru];
-  for (int q = lane; q < len; q += 64) {
-    const double* rp = p.rpts + (size_t)(off + q) * 5;
+  const double* rp = p.rpts + (size_t)off * 5;
+  for (int q = 0; q < len; ++q, rp += 5) {
     const double txi = rp[0], tet = rp[1], sxi = rp[2], set = rp[3], w = rp[4];
-    const double bt[3] = {1.0 - txi - tet, txi, tet};
-    const double bs[3] = {1.0 - sxi - set, sxi, set};
-    double xx[3], yy[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      xx[c] = bt[0] * tv[c] + bt[1] * tv[3 + c] + bt[2] * tv[6 + c];
-      yy[c] = bs[0] * sv[c] + bs[1] * sv[3 + c] + bs[2] * sv[6 + c];
-    }
-    const double dx = yy[0] - xx[0], dy = yy[1] - xx[1], dz = yy[2] - xx[2];
+    const double dx = (s0[0] + sxi * se1[0] + set * se2[0]) - (t0[0] + txi * te1[0] + tet * te2[0]);
+    const double dy = (s0[1] + sxi * se1[1] + set * se2[1]) - (t0[1] + txi * te1[1] + tet * te2[1]);
+    const double dz = (s0[2] + sxi * se1[2] + set * se2[2]) - (t0[2] + txi * te1[2] + tet * te2[2]);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     if (r == 0.0) continue;
     const double rinv = 1.0 / r;
@@ -235,6 +245,9 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p) {
     const double hr = -gr * rinv - gi * k, hi = gr * k - gi * rinv;
     const double td = (dx * ns[0] + dy * ns[1] + dz * ns[2]) * rinv;
     const double ad = -(dx * nt[0] + dy * nt[1] + dz * nt[2]) * rinv;
+    const double bt[3] = {1.0 - txi - tet, txi, tet};
+    const double bs[3] = {1.0 - sxi - set, sxi, set};
+    const double dgr = hr * td, dgi = hi * td;
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       Sr[l] += bt[l] * gr;
@@ -244,31 +257,17 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p) {
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         const double bb = bt[l] * bs[m];
-        Dr[3 * l + m] += bb * hr * td;
-        Di[3 * l + m] += bb * hi * td;
+        Dr[3 * l + m] += bb * dgr;
+        Di[3 * l + m] += bb * dgi;
         Gr[3 * l + m] += bb * gr;
         Gi[3 * l + m] += bb * gi;
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    Sr[i] = wave_sum(Sr[i]);
-    Si[i] = wave_sum(Si[i]);
-    Ar[i] = wave_sum(Ar[i]);
-    Ai[i] = wave_sum(Ai[i]);
-  }
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    Dr[i] = wave_sum(Dr[i]);
-    Di[i] = wave_sum(Di[i]);
-    Gr[i] = wave_sum(Gr[i]);
-    Gi[i] = wave_sum(Gi[i]);
-  }
-  if (lane != 0) return;
   const double scale = p.fa2[t] * p.fa2[s] * BEM_INV_FOUR_PI;
   const double kinv = 1.0 / k;
   double gsr = 0, gsi = 0;
+#pragma unroll
   for (int i = 0; i < 9; ++i) {
     gsr += Gr[i];
     gsi += Gi[i];
@@ -279,21 +278,24 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p) {
   cplx<double>* out = p.lhs + ((size_t)f * p.P + pi) * 9;
   const bool self = (t == s);
   const double a2 = p.fa2[t];
+#pragma unroll
   for (int l = 0; l < 3; ++l) {
+#pragma unroll
     for (int m = 0; m < 3; ++m) {
       const double cp = ctp[3 * l] * csp[3 * m] + ctp[3 * l + 1] * csp[3 * m + 1] + ctp[3 * l + 2] * csp[3 * m + 2];
       const double Hr = cp * gsr - k * k * nn * Gr[3 * l + m];
       const double Hi = cp * gsi - k * k * nn * Gi[3 * l + m];
       double re = scale * (-Dr[3 * l + m] - Hi * kinv);
-      double im = scale * (-Di[3 * l + m] + Hr * kinv);
+      const double im = scale * (-Di[3 * l + m] + Hr * kinv);
       if (self) re += 0.5 * p.m11[3 * l + m] * a2;
       out[3 * l + m] = mk<double>(re, im);
     }
   }
   cplx<double>* ro = p.rhs + ((size_t)f * p.P + pi) * 3;
+#pragma unroll
   for (int l = 0; l < 3; ++l) {
     // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair
-    double re = scale * (-Sr[l] + Ai[l] * kinv);
+    const double re = scale * (-Sr[l] + Ai[l] * kinv);
     double im = scale * (-Si[l] - Ar[l] * kinv);
     if (self) im -= 0.5 * kinv * p.m10[l] * a2;
     ro[l] = mk<double>(re, im);
@@ -462,7 +464,8 @@ hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
 }
 hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
   if (p.P == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_singular, dim3(p.P, nf), dim3(64), 0, st, p);
+  const long long items = (long long)p.P * nf;
+  hipLaunchKernelGGL(k_singular, dim3((unsigned)((items + 63) / 64)), dim3(64), 0, st, p, nf);
   return hipGetLastError();
 }
 template <typename T>

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -84,7 +85,7 @@ struct bem_mesh {
   int *order, *slot_count, *slot_base, *cs_ptr, *cs_slot, *vstar_ptr, *vstar_row;
   uint16_t *slot_start, *slot_inc;
   double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
-  int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces;
+  int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces, *sing_by_rule;
   // workspace
   DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial;
   int strip_nf = 0;
@@ -295,6 +296,14 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   UP(m->sing_trial, d->sing_trial, (size_t)d->n_pairs);
   UP(m->sing_rule, d->sing_rule, (size_t)d->n_pairs);
   UP(m->sing_ptr, d->sing_ptr, (size_t)F + 1);
+  {
+    // pairs sorted by Duffy rule (stable), so a wave of the singular kernel shares one rule
+    std::vector<int> by_rule(d->n_pairs);
+    for (int i = 0; i < d->n_pairs; ++i) by_rule[i] = i;
+    std::stable_sort(by_rule.begin(), by_rule.end(),
+                     [d](int a, int b) { return d->sing_rule[a] < d->sing_rule[b]; });
+    UP(m->sing_by_rule, by_rule.data(), (size_t)d->n_pairs);
+  }
   UP(m->rule_off, d->rule_off, (size_t)d->n_rules);
   UP(m->rule_len, d->rule_len, (size_t)d->n_rules);
   UP(m->rpts, d->rule_pts, total_rule_pts * 5);
@@ -376,6 +385,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   sp.test = m->sing_test;
   sp.trial = m->sing_trial;
   sp.rule = m->sing_rule;
+  sp.by_rule = m->sing_by_rule;
   sp.rule_off = m->rule_off;
   sp.rule_len = m->rule_len;
   sp.rpts = m->rpts;

@@ -51,6 +51,7 @@ struct SingParams {
   const int* test;      // [P]
   const int* trial;     // [P]
   const int* rule;      // [P]
+  const int* by_rule;   // [P] pair indices sorted by rule
   const int* rule_off;  // [n_rules]
   const int* rule_len;  // [n_rules]
   const double* rpts;   // [total][5]: test xi, eta, trial xi, eta, weight

@@ -429,6 +429,92 @@ __global__ __launch_bounds__(1024) void k_trsv(MatBatch<T> mb, cplx<T>* y, long 
   }
 }
 
+// One block step of a triangular sweep, many workgroups per matrix (the multi-block sweep of k_trsv).
+// Lower (forward, unit L): blocks ascend; step `kb` applies the solved block kb to the rows of every later
+// block (one workgroup per 64-row block), and the workgroup owning block kb+1 then solves its diagonal block,
+// whose rows now carry every earlier contribution. kb = -1 only solves block 0.
+// Upper (backward, U with diagonal): the mirror image with blocks descending; kb = nblk solves the last block.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, long long ldy, long long strideY,
+                                                  int nrhs, int kb, const int* skip) {
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  const int N = mb.N;
+  const int nblk = (N + 63) / 64;
+  const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  cplx<T>* Y = y + (size_t)mat * strideY;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int rb = LOWER ? kb + 1 + (int)blockIdx.x : kb - 1 - (int)blockIdx.x;  // row block of this workgroup
+  if (rb < 0 || rb >= nblk) return;
+  const int r0 = rb * 64, nr = min(64, N - r0);
+  __shared__ cplx<T> xk[MAX_CH][64];
+  __shared__ cplx<T> yb[MAX_CH][64];  // updated rows of the block solved by this workgroup
+  const bool solve_here = LOWER ? (rb == kb + 1) : (rb == kb - 1);
+  const bool updated = kb >= 0 && kb < nblk;
+  if (updated) {
+    const int c0 = kb * 64, nc = min(64, N - c0);
+    if (tid < 64)
+      for (int c = 0; c < nrhs; ++c) xk[c][tid] = tid < nc ? Y[(size_t)c * ldy + c0 + tid] : mk<T>(0, 0);
+    __syncthreads();
+    // 4 threads per row, 16 columns each
+    const int row = tid >> 2, part = tid & 3;
+    cplx<T> acc[MAX_CH];
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) acc[c] = mk<T>(0, 0);
+    if (row < nr) {
+      const cplx<T>* Ar = A + (size_t)(r0 + row) * lda + c0 + 16 * part;
+#pragma unroll 4
+      for (int j = 0; j < 16; ++j) {
+        if (16 * part + j < nc) {
+          const cplx<T> a = Ar[j];
+#pragma unroll
+          for (int c = 0; c < MAX_CH; ++c)
+            if (c < nrhs) acc[c] = cfma(acc[c], a, xk[c][16 * part + j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) {
+      if (c < nrhs) {
+        T ar = acc[c].re, ai = acc[c].im;
+        ar += __shfl_xor(ar, 1, 64);
+        ai += __shfl_xor(ai, 1, 64);
+        ar += __shfl_xor(ar, 2, 64);
+        ai += __shfl_xor(ai, 2, 64);
+        if (part == 0 && row < nr) {
+          cplx<T>* yp = Y + (size_t)c * ldy + r0 + row;
+          const cplx<T> nv = mk<T>(yp->re - ar, yp->im - ai);
+          if (solve_here)
+            yb[c][row] = nv;
+          else
+            *yp = nv;
+        }
+      }
+    }
+  }
+  // the workgroup holding the next block to solve finishes it (updated rows are handed over in LDS)
+  if (!solve_here) return;
+  __syncthreads();
+  if (w != 0) return;
+  for (int c = 0; c < nrhs; ++c) {
+    cplx<T> v = lane < nr ? (updated ? yb[c][lane] : Y[(size_t)c * ldy + r0 + lane]) : mk<T>(0, 0);
+    if (LOWER) {
+      for (int r = 0; r < nr; ++r) {
+        const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
+        if (lane > r && lane < nr) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
+      }
+    } else {
+      for (int r = nr - 1; r >= 0; --r) {
+        if (lane == r) v = cdiv(v, A[(size_t)(r0 + r) * lda + r0 + r]);
+        const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
+        if (lane < r) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
+      }
+    }
+    if (lane < nr) Y[(size_t)c * ldy + r0 + lane] = v;
+  }
+}
+
 // r = b - A0 x in FP64 (A0 in T, x and b in FP64); one wave per row, all RHS together.
 template <typename T>
 __global__ __launch_bounds__(256) void k_residual(MatBatch<T> mb, const cplx<double>* b, const cplx<double>* x,
@@ -664,10 +750,21 @@ hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, lon
   hipLaunchKernelGGL(k_permute_rhs<T>, dim3((N + 255) / 256, nrhs, batch), dim3(256), 0, st, b, ldb, strideB, y,
                      ldy, strideY, perm, strideP, N, nrhs, skip);
   CHK(hipGetLastError());
-  hipLaunchKernelGGL((k_trsv<T, true>), dim3(batch), dim3(1024), 0, st, mb, y, ldy, strideY, nrhs, skip);
-  CHK(hipGetLastError());
-  hipLaunchKernelGGL((k_trsv<T, false>), dim3(batch), dim3(1024), 0, st, mb, y, ldy, strideY, nrhs, skip);
-  return hipGetLastError();
+  const int nblk = (N + 63) / 64;
+  // forward sweep: step kb updates blocks > kb and solves block kb+1
+  for (int kb = -1; kb < nblk - 1; ++kb) {
+    const int rows_after = nblk - 1 - kb;
+    hipLaunchKernelGGL((k_trsv_step<T, true>), dim3(rows_after, batch), dim3(256), 0, st, mb, y, ldy, strideY,
+                       nrhs, kb, skip);
+    CHK(hipGetLastError());
+  }
+  // backward sweep: step kb updates blocks < kb and solves block kb-1
+  for (int kb = nblk; kb > 0; --kb) {
+    hipLaunchKernelGGL((k_trsv_step<T, false>), dim3(kb, batch), dim3(256), 0, st, mb, y, ldy, strideY, nrhs, kb,
+                       skip);
+    CHK(hipGetLastError());
+  }
+  return hipSuccess;
 }
 
 template <typename T>

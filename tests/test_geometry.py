@@ -36,7 +36,7 @@ def test_default_workload_sizes():
 
 
 def test_singular_pairs_match_oracle(sphere):
-    prep = G.prepare(sphere)
+    prep = G.prepare(sphere, reorder=False)
     om = O.Mesh(sphere.vertices, sphere.faces, sphere.tags, sphere.mesh_ids)
     cache = O.build_singular_cache(om, 4)
     a = sorted(zip(cache.test.tolist(), cache.trial.tolist()))
@@ -57,7 +57,7 @@ def test_singular_pairs_match_oracle(sphere):
 
 
 def test_curls_and_rule_match_oracle(sphere):
-    prep = G.prepare(sphere)
+    prep = G.prepare(sphere, reorder=False)
     om = O.Mesh(sphere.vertices, sphere.faces, sphere.tags, sphere.mesh_ids)
     assert np.abs(prep.curls - O.surface_curls(om.face_vertices, om.normals)).max() < 1e-9
     pts, w = O.triangle_rule(4)
@@ -70,6 +70,7 @@ def test_curls_and_rule_match_oracle(sphere):
 def test_slot_tables_cover_every_incidence_once(sphere):
     prep = G.prepare(sphere)
     s = prep.slots
+    sphere = prep.mesh  # tables index the locality-ordered mesh
     F, V = sphere.n_faces, sphere.n_vertices
     seen = np.zeros((F, 3), dtype=int)
     slot_vertex = np.full(s.strip_len, -1)
@@ -120,3 +121,23 @@ def test_reference_meshes_load_and_prepare():
     om = O.Mesh(V, F, T, np.ones(len(F), int))
     cache = O.build_singular_cache(om, 4)
     assert len(cache.test) == len(prep.pairs.test)
+
+
+def test_locality_order_is_a_consistent_permutation(sphere):
+    prep = G.prepare(sphere)
+    assert sorted(prep.fperm.tolist()) == list(range(sphere.n_faces))
+    assert sorted(prep.vperm.tolist()) == list(range(sphere.n_vertices))
+    # internal face i is original face fperm[i] with vertices renamed by vperm
+    assert np.array_equal(prep.vperm[prep.mesh.faces], sphere.faces[prep.fperm])
+    assert np.allclose(prep.mesh.normals, sphere.normals[prep.fperm])
+    # the singular pair set maps onto the original one
+    ref = G.prepare(sphere, reorder=False).pairs
+    a = sorted(zip(prep.fperm[prep.pairs.test].tolist(), prep.fperm[prep.pairs.trial].tolist()))
+    b = sorted(zip(ref.test.tolist(), ref.trial.tolist()))
+    assert a == b
+    x = np.arange(sphere.n_vertices, dtype=float)[prep.vperm]
+    assert np.array_equal(prep.unpermute_vertices(x), np.arange(sphere.n_vertices))
+    # locality: consecutive internal vertices are close in space on average
+    d = np.linalg.norm(np.diff(prep.mesh.vertices, axis=0), axis=1).mean()
+    d0 = np.linalg.norm(np.diff(sphere.vertices, axis=0), axis=1).mean()
+    assert d <= d0
