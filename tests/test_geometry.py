@@ -141,3 +141,20 @@ def test_locality_order_is_a_consistent_permutation(sphere):
     d = np.linalg.norm(np.diff(prep.mesh.vertices, axis=0), axis=1).mean()
     d0 = np.linalg.norm(np.diff(sphere.vertices, axis=0), axis=1).mean()
     assert d <= d0
+
+
+def test_locality_order_permutes_the_operators():
+    """The oracle's operators on the renumbered mesh are the original ones with rows/columns taken by vperm."""
+    V, F, T = syn.icosphere(1, 0.1)
+    prep = G.prepare(G.BoundaryMesh(V, F, T))
+    assert not np.array_equal(prep.vperm, np.arange(len(V)))
+    q = np.arange(1.0, len(F) + 1.0)[:, None]
+    outs = []
+    for m, qq in ((prep.original, q), (prep.mesh, q[prep.fperm])):
+        om = O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids)
+        outs.append(O.solve_frequency(om, 1500.0, qq, 343.0, 1.205))
+    orig, internal = outs
+    vp = prep.vperm
+    assert np.abs(internal["lhs"] - orig["lhs"][np.ix_(vp, vp)]).max() < 1e-12 * np.abs(orig["lhs"]).max()
+    assert np.abs(internal["rhs"] - orig["rhs"][vp]).max() < 1e-12 * np.abs(orig["rhs"]).max()
+    assert np.allclose(prep.unpermute_vertices(internal["pressure"]), orig["pressure"], rtol=1e-10, atol=0)

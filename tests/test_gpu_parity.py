@@ -26,7 +26,7 @@ def sphere():
 def oracle_sphere(sphere):
     from oracle import bem_oracle as O
 
-    m = sphere.mesh
+    m = sphere.original  # oracle in the original numbering; the engine reports in it too
     om = O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids)
     pts = O.fibonacci_observation({"spherical_sampling_enabled": True, "spherical_sampling_points": 1000,
                                    "distance": 2.0})[0].astype(np.float64)
