@@ -38,6 +38,48 @@ METRIC = "freq-points/s @6k tris, mixed vs FP64 GPU (≤0.05 dB); p50/p99 job la
 FLOPS_PER_QPAIR = 66
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+PIPE_NOTE = ("compute-bound on the FP32 vector ALUs (no MFMA, no HBM limit); 157.3 TFLOP/s is MI355X's FP32 peak, "
+             "shared by the matrix pipe and the packed-FP32 vector pipe; without packed FP32 the VALU issue ceiling "
+             "is half of it")
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_regular_lhs_horn6k_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per k_regular_lhs<float> launch from the committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE per dispatch, each scaled by the factor a plain copy of known size reads), or None."""
+    try:
+        vals = {}
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if ":" in line and line.strip().startswith(("fetch_bytes_per_launch", "write_bytes_per_launch")):
+                key, val = line.strip().split(":", 1)
+                vals[key] = float(val.strip().split()[0])
+        return int(vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def traffic_probe():
+    """Workload for the two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE): a 1 GiB device-to-device copy
+    (calibration: the copy kernel fetches and writes exactly 1 GiB) followed by one eager assembly of 4
+    frequencies of the horn job (one k_regular_lhs<float> launch)."""
+    import torch
+
+    from beat_hip.engine import Engine
+
+    prep, qunit, obs, freqs = horn_job()
+    eng = Engine(prep, qunit)
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    b = eng.batch("mixed", 4, obs, 1.205, 343.0)
+    b.set_frequencies(freqs[-4:])
+    with torch.cuda.stream(eng.dev.stream):
+        b._assemble()
+    eng.dev.sync()
+    b.close()
+    eng.close()
+    log("probe: copy 1 GiB, then one 4-frequency assembly of the 6k-triangle horn job")
 
 
 def log(msg):
@@ -153,11 +195,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=16)
     ap.add_argument("--serving-rate", type=float, default=2.0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run the workload of the HBM-counter passes (copy calibration + one assembly)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build runs on a single MI355X); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.traffic_probe:
+        traffic_probe()
+        return
 
     import torch
 
@@ -189,30 +237,28 @@ def main():
     max_res = float(max(float(r.residual.max()) for r in mixed))
     b.close()
 
-    # ---- roofline of the regular assembly kernel (HIP events on the library stream) ---------------
+    # ---- per-stage times of one eager job and the roofline of the regular assembly kernel -----------
+    # HIP events on the library's stream bracket every assembly launch (one region per kernel launch)
     from beat_hip import _lib
 
-    nf_t = min(8, nb)
-    bt = eng.batch("mixed", nf_t, obs, 1.205, 343.0)
-    bt.set_frequencies(freqs[-nf_t:])
-    with torch.cuda.stream(dev.stream):
-        bt._assemble()
-    dev.sync()
+    be = eng.batch("mixed", nb, obs, 1.205, 343.0)
+    be.run(freqs[:nb])
     dev.timers(True)
-    reps = 3
-    with torch.cuda.stream(dev.stream):
-        for _ in range(reps):
-            bt._assemble()
-    dev.sync()
-    t_ms, units = dev.timer(_lib.TIMER_REGULAR)
+    for i in range(0, len(freqs), nb):
+        be.run(freqs[i : i + nb])
+    stages = {name: dev.timer(slot) for slot, name in enumerate(_lib.TIMER_NAMES)}
     dev.timers(False)
-    launches = max(1, units // nf_t)
-    avg_ms = t_ms / launches
+    be.close()
+    t_ms, units, regions = stages["regular"]
+    avg_ms = t_ms / max(1, regions)
     F = prep.n_faces
-    qpairs = (F * F - len(prep.pairs.test)) * 36 * nf_t
+    freqs_per_launch = units / max(1, regions)
+    qpairs = (F * F - len(prep.pairs.test)) * 36 * freqs_per_launch
     achieved = qpairs * FLOPS_PER_QPAIR / (avg_ms * 1e-3) / 1e12
-    bt.close()
-    log(f"bench: k_regular_lhs<float> avg {avg_ms:.3f} ms/launch for {nf_t} freqs -> {achieved:.1f} TFLOP/s")
+    traffic = measured_traffic()
+    log(f"bench: k_regular_lhs<float> avg {avg_ms:.3f} ms/launch ({freqs_per_launch:g} freqs) -> "
+        f"{achieved:.1f} TFLOP/s; stages (ms/job): "
+        + ", ".join(f"{k} {v[0]:.1f}" for k, v in stages.items()))
 
     # ---- plain FP64 path: one frequency per launch ------------------------------------------------
     fp64_value, db_err, ph_err = None, None, None
@@ -269,10 +315,13 @@ def main():
         "max_field_error_deg": ph_err,
         "max_refinement_iterations": max_iters,
         "max_relative_residual": max_res,
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_regular_lhs<float>", "avg_launch_ms": round(avg_ms, 4),
-                     "flops_per_qpair": FLOPS_PER_QPAIR, "qpairs_per_launch": int(qpairs)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "k_regular_lhs<float>", "pipe": PIPE_NOTE, "avg_launch_ms": round(avg_ms, 4),
+                     "freqs_per_launch": freqs_per_launch, "flops_per_qpair": FLOPS_PER_QPAIR,
+                     "qpairs_per_launch": int(qpairs),
+                     "frac_of_unpacked_fp32_valu": round(achieved / (PEAK_FP32_TFLOPS / 2), 4)},
+        "stages_ms_per_job": {k: round(v[0], 3) for k, v in stages.items()},
         "cpu_baseline": cpu,
         "serving": serving,
     }

@@ -58,10 +58,11 @@ class Device:
         _lib.check(self.lib.bem_ctx_timers(self.ctx, int(bool(enable))), "bem_ctx_timers")
 
     def timer(self, slot: int):
-        """(accumulated ms, accumulated work units) of a kernel timer slot."""
-        ms, units = C.c_double(), C.c_longlong()
-        _lib.check(self.lib.bem_ctx_timer_read(self.ctx, slot, C.byref(ms), C.byref(units)), "bem_ctx_timer_read")
-        return float(ms.value), int(units.value)
+        """(accumulated ms, accumulated work units, timed regions) of a kernel timer slot."""
+        ms, units, regions = C.c_double(), C.c_longlong(), C.c_longlong()
+        _lib.check(self.lib.bem_ctx_timer_read(self.ctx, slot, C.byref(ms), C.byref(units), C.byref(regions)),
+                   "bem_ctx_timer_read")
+        return float(ms.value), int(units.value), int(regions.value)
 
     def close(self):
         if self.ctx:

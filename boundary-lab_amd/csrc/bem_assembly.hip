@@ -122,16 +122,129 @@ __device__ __forceinline__ void regular_block(const T* __restrict__ xq, const T 
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
-  const int t = blockIdx.x;
-  const int f = blockIdx.y;
-  const int lane = threadIdx.x;
-  __shared__ cplx<T> blk[64][9];
+// Orbit structure of the order-4 rule: points {0,2,3} share one weight, points {1,4,5} the other, and within an
+// orbit basis function m takes one value at its own "special" point ({0,2,3}[m], {1,4,5}[m]) and a common value
+// at the other two. Hence psi_m(b) w_b = base(orbit) + [b special for m] * sp(orbit) (to the last digit of the
+// tabulated rule), and the trial contraction of six values costs 4 adds + 8 FMAs instead of 18 FMAs.
+constexpr double kBaseA = rule_phi(1, 0) * rule_w(0);
+constexpr double kSpA = rule_phi(0, 0) * rule_w(0) - kBaseA;
+constexpr double kBaseB = rule_phi(1, 1) * rule_w(1);
+constexpr double kSpB = rule_phi(0, 1) * rule_w(1) - kBaseB;
 
-  const double kd = p.k[f];
-  const T k = T(kd);
-  const T kappa = T(kd / (2.0 * BEM_PI));
+__device__ __forceinline__ void trial_contract(const float (&x)[6], float (&t)[3]) {
+  const float base = float(kBaseA) * (x[0] + x[2] + x[3]) + float(kBaseB) * (x[1] + x[4] + x[5]);
+  t[0] = base + float(kSpA) * x[0] + float(kSpB) * x[1];
+  t[1] = base + float(kSpA) * x[2] + float(kSpB) * x[4];
+  t[2] = base + float(kSpA) * x[3] + float(kSpB) * x[5];
+}
+
+// FP32 block for NFW wavenumbers at once: the pair geometry (r, 1/r, r.n_s) is shared by the frequencies.
+template <int NFW>
+__device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, const float (&nt)[3],
+                                                  const float (&ct)[9], float a2t, const ElemRec<float>& se,
+                                                  const float (&k)[NFW], const float (&kappa)[NFW],
+                                                  cplx<float> (&C)[NFW][9]) {
+  float y[18];
+#pragma unroll
+  for (int i = 0; i < 18; ++i) y[i] = se.q[i];
+  const float nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
+  float DMr[NFW][9], DMi[NFW][9], GMr[NFW][9], GMi[NFW][9];
+#pragma unroll
+  for (int j = 0; j < NFW; ++j)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) DMr[j][i] = DMi[j][i] = GMr[j][i] = GMi[j][i] = 0.f;
+#pragma unroll 1  // rolled: 185 VGPRs (two waves per SIMD) and no spills with two wavenumbers
+  for (int a = 0; a < 6; ++a) {
+    const float xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
+    float xDr[NFW][6], xDi[NFW][6], xGr[NFW][6], xGi[NFW][6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const float dx = y[3 * b + 0] - xa;
+      const float dy = y[3 * b + 1] - ya;
+      const float dz = y[3 * b + 2] - za;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float rinv = Phase<float>::rsqrt_(r2);
+      const float r = r2 * rinv;
+      const float td = (dx * nsx + dy * nsy + dz * nsz) * rinv;
+#pragma unroll
+      for (int j = 0; j < NFW; ++j) {
+        float sn, cs;
+        Phase<float>::sincos_kr(kappa[j], k[j], r, &sn, &cs);
+        const float gr = cs * rinv, gi = sn * rinv;  // 4 pi G
+        const float hr = -gr * rinv - gi * k[j];     // 4 pi G (ik - 1/r)
+        const float hi = gr * k[j] - gi * rinv;
+        xDr[j][b] = hr * td;
+        xDi[j][b] = hi * td;
+        xGr[j][b] = gr;
+        xGi[j][b] = gi;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NFW; ++j) {
+      float tDr[3], tDi[3], tGr[3], tGi[3];
+      trial_contract(xDr[j], tDr);
+      trial_contract(xDi[j], tDi);
+      trial_contract(xGr[j], tGr);
+      trial_contract(xGi[j], tGi);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const float wa = float(c_phiw.v[l * 6 + a]);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          DMr[j][3 * l + m] += wa * tDr[m];
+          DMi[j][3 * l + m] += wa * tDi[m];
+          GMr[j][3 * l + m] += wa * tGr[m];
+          GMi[j][3 * l + m] += wa * tGi[m];
+        }
+      }
+    }
+  }
+  const float nn = nt[0] * nsx + nt[1] * nsy + nt[2] * nsz;
+  const float scale = a2t * se.a2 * float(BEM_INV_FOUR_PI);
+  float cp[9];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      cp[3 * l + m] = ct[3 * l + 0] * se.curl[3 * m + 0] + ct[3 * l + 1] * se.curl[3 * m + 1] +
+                      ct[3 * l + 2] * se.curl[3 * m + 2];
+#pragma unroll
+  for (int j = 0; j < NFW; ++j) {
+    float gsr = 0.f, gsi = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      gsr += GMr[j][i];
+      gsi += GMi[j][i];
+    }
+    const float k2nn = k[j] * k[j] * nn;
+    const float kinv = 1.f / k[j];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const float Hr = cp[i] * gsr - k2nn * GMr[j][i];
+      const float Hi = cp[i] * gsi - k2nn * GMi[j][i];
+      // -D + (i/k) H
+      C[j][i] = mk<float>(scale * (-DMr[j][i] - Hi * kinv), scale * (-DMi[j][i] + Hr * kinv));
+    }
+  }
+}
+
+// One wave per (test element t, group of NFW wavenumbers); FP32 shares the pair geometry across NFW = 2
+// wavenumbers, FP64 runs one wavenumber per wave (NFW = 1).
+template <typename T, int NFW>
+__global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
+  const int t = blockIdx.x;
+  const int f0 = blockIdx.y * NFW;
+  const int nfw = min(NFW, nf - f0);  // the last group may be partial: its spare slot repeats a wavenumber
+  const int lane = threadIdx.x;
+  __shared__ cplx<T> blk[NFW][64][9];
+
+  T k[NFW], kappa[NFW];
+#pragma unroll
+  for (int j = 0; j < NFW; ++j) {
+    const double kd = p.k[f0 + min(j, nfw - 1)];
+    k[j] = T(kd);
+    kappa[j] = T(kd / (2.0 * BEM_PI));
+  }
   const ElemRec<T>* te = p.rec + t;
   T nt[3], ct[9];
 #pragma unroll
@@ -141,30 +254,33 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
   const T a2t = te->a2;
   const int4 tv = p.fv[t];
   const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
-  const cplx<double>* sl = p.sing_lhs + (size_t)f * p.P * 9;
-  cplx<T>* strip = p.strip + (size_t)f * p.strip_fstride;
-  cplx<T>* row0 = strip + (size_t)(3 * t + 0) * p.L;
-  cplx<T>* row1 = strip + (size_t)(3 * t + 1) * p.L;
-  cplx<T>* row2 = strip + (size_t)(3 * t + 2) * p.L;
 
   for (int c = 0; c < p.n_chunks; ++c) {
     const int s = p.order[c * 64 + lane];
-    cplx<T> C[9];
+    cplx<T> C[NFW][9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) C[i] = mk<T>(0, 0);
+    for (int j = 0; j < NFW; ++j)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) C[j][i] = mk<T>(0, 0);
     if (s >= 0) {
       const int4 sv = p.fv[s];
       const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
                        sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
       if (!adj) {
-        regular_block<T, sizeof(T) == 4 ? 6 : 1>(te->q, nt, ct, a2t, p.rec[s], k, kappa, C);
+        if constexpr (sizeof(T) == 4) {
+          regular_block_f32<NFW>(te->q, nt, ct, a2t, p.rec[s], k, kappa, C);
+        } else {
+          static_assert(NFW == 1, "FP64 runs one wavenumber per wave");
+          regular_block<T, 1>(te->q, nt, ct, a2t, p.rec[s], k[0], kappa[0], C[0]);
+        }
       } else {
         for (int q = sp0; q < sp1; ++q) {
           if (p.sing_trial[q] == s) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-              const cplx<double> v = sl[(size_t)q * 9 + i];
-              C[i] = mk<T>(T(v.re), T(v.im));
+            for (int j = 0; j < NFW; ++j) {
+              const cplx<double>* sl = p.sing_lhs + ((size_t)(f0 + min(j, nfw - 1)) * p.P + q) * 9;
+#pragma unroll
+              for (int i = 0; i < 9; ++i) C[j][i] = mk<T>(T(sl[i].re), T(sl[i].im));
             }
             break;
           }
@@ -172,25 +288,33 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p) {
       }
     }
 #pragma unroll
-    for (int i = 0; i < 9; ++i) blk[lane][i] = C[i];
+    for (int j = 0; j < NFW; ++j)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) blk[j][lane][i] = C[j][i];
     __syncthreads();
     const int nsl = p.slot_count[c];
     const uint16_t* st = p.slot_start + (size_t)c * SLOT_STRIDE;
     const uint16_t* inc = p.slot_inc + (size_t)c * 192;
     const int base = p.slot_base[c];
     for (int kk = lane; kk < nsl; kk += 64) {
-      cplx<T> a0 = mk<T>(0, 0), a1 = mk<T>(0, 0), a2 = mk<T>(0, 0);
-      const int e1 = st[kk + 1];
-      for (int e = st[kk]; e < e1; ++e) {
-        const int code = inc[e];
-        const int src = code >> 2, m = code & 3;
-        a0 = cadd(a0, blk[src][m]);
-        a1 = cadd(a1, blk[src][3 + m]);
-        a2 = cadd(a2, blk[src][6 + m]);
+      const int e0 = st[kk], e1 = st[kk + 1];
+#pragma unroll
+      for (int j = 0; j < NFW; ++j) {
+        if (j < nfw) {
+          cplx<T> a0 = mk<T>(0, 0), a1 = mk<T>(0, 0), a2 = mk<T>(0, 0);
+          for (int e = e0; e < e1; ++e) {
+            const int code = inc[e];
+            const int src = code >> 2, m = code & 3;
+            a0 = cadd(a0, blk[j][src][m]);
+            a1 = cadd(a1, blk[j][src][3 + m]);
+            a2 = cadd(a2, blk[j][src][6 + m]);
+          }
+          cplx<T>* rows = p.strip + (size_t)(f0 + j) * p.strip_fstride + (size_t)(3 * t) * p.L + base + kk;
+          rows[0] = a0;
+          rows[p.L] = a1;
+          rows[2 * (size_t)p.L] = a2;
+        }
       }
-      row0[base + kk] = a0;
-      row1[base + kk] = a1;
-      row2[base + kk] = a2;
     }
     __syncthreads();
   }
@@ -459,7 +583,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
 
 template <typename T>
 hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
-  hipLaunchKernelGGL(k_regular_lhs<T>, dim3(p.F, nf), dim3(64), 0, st, p);
+  constexpr int NFW = sizeof(T) == 4 ? 2 : 1;
+  hipLaunchKernelGGL((k_regular_lhs<T, NFW>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
   return hipGetLastError();
 }
 hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {

@@ -28,6 +28,7 @@ static int hipfail(hipError_t e, const char* what) {
   } while (0)
 
 #define BEM_TIMER_SLOTS 8
+#define BEM_AUX_STREAMS 3
 
 struct bem_ctx {
   int device;
@@ -39,7 +40,35 @@ struct bem_ctx {
   hipEvent_t ev0, ev1;
   double timer_ms[BEM_TIMER_SLOTS];
   long long timer_count[BEM_TIMER_SLOTS];
+  long long timer_regions[BEM_TIMER_SLOTS];
+  // auxiliary streams (default priority) for batch fan-out: independent matrices of one batch progress on
+  // separate streams so that one group's latency-bound panel phase overlaps another group's MFMA updates
+  hipStream_t aux[BEM_AUX_STREAMS];
+  hipEvent_t fork_ev, join_ev[BEM_AUX_STREAMS];
+  int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
 };
+
+// Run body(first, count, stream) on up to 1 + BEM_AUX_STREAMS contiguous sub-batches, forked from and joined
+// back into the context stream with events (capturable into a graph).
+template <typename F>
+static hipError_t fan_out(bem_ctx* c, int batch, int min_per_stream, F&& body) {
+  int S = c->fanout;
+  while (S > 1 && batch < S * min_per_stream) --S;
+  if (S == 1) return body(0, batch, c->stream);
+  hipError_t e = hipEventRecord(c->fork_ev, c->stream);
+  if (e != hipSuccess) return e;
+  for (int i = 0; i < S - 1; ++i)
+    if ((e = hipStreamWaitEvent(c->aux[i], c->fork_ev, 0)) != hipSuccess) return e;
+  for (int i = 0; i < S; ++i) {
+    const int b0 = (int)((long long)batch * i / S), b1 = (int)((long long)batch * (i + 1) / S);
+    if ((e = body(b0, b1 - b0, i == 0 ? c->stream : c->aux[i - 1])) != hipSuccess) return e;
+  }
+  for (int i = 0; i < S - 1; ++i) {
+    if ((e = hipEventRecord(c->join_ev[i], c->aux[i])) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream, c->join_ev[i], 0)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 // Kernel timing on the context's stream (diagnostics): brackets one launch with events and waits for it.
 struct ScopedTimer {
@@ -59,6 +88,7 @@ struct ScopedTimer {
       if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) {
         c->timer_ms[slot] += ms;
         c->timer_count[slot] += units;
+        c->timer_regions[slot] += 1;
       }
     }
   }
@@ -144,13 +174,23 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   c->capturing = false;
   c->graph = nullptr;
   c->timing = false;
+  c->fanout = 1 + BEM_AUX_STREAMS;
   for (int i = 0; i < BEM_TIMER_SLOTS; ++i) {
     c->timer_ms[i] = 0.0;
     c->timer_count[i] = 0;
+    c->timer_regions[i] = 0;
   }
-  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess) {
     delete c;
     return fail(BEM_E_HIP, "hipEventCreate failed");
+  }
+  for (int i = 0; i < BEM_AUX_STREAMS; ++i) {
+    if (hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) != hipSuccess) {
+      delete c;
+      return fail(BEM_E_HIP, "auxiliary stream creation failed");
+    }
   }
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
@@ -170,6 +210,11 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
 int bem_ctx_destroy(bem_ctx* c) {
   if (!c) return BEM_OK;
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  for (int i = 0; i < BEM_AUX_STREAMS; ++i) {
+    (void)hipStreamDestroy(c->aux[i]);
+    (void)hipEventDestroy(c->join_ev[i]);
+  }
+  (void)hipEventDestroy(c->fork_ev);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   delete c;
@@ -182,6 +227,14 @@ int bem_ctx_sync(bem_ctx* c) {
   return BEM_OK;
 }
 
+int bem_ctx_set_fanout(bem_ctx* c, int streams) {
+  if (!c) return fail(BEM_E_ARG, "ctx is NULL");
+  if (streams < 1 || streams > 1 + BEM_AUX_STREAMS) return fail(BEM_E_ARG, "fan-out must be 1..4 streams");
+  if (c->capturing) return fail(BEM_E_STATE, "cannot change the fan-out during graph capture");
+  c->fanout = streams;
+  return BEM_OK;
+}
+
 int bem_ctx_timers(bem_ctx* c, int enable) {
   if (!c) return fail(BEM_E_ARG, "ctx is NULL");
   c->timing = enable != 0;
@@ -189,14 +242,16 @@ int bem_ctx_timers(bem_ctx* c, int enable) {
     for (int i = 0; i < BEM_TIMER_SLOTS; ++i) {
       c->timer_ms[i] = 0.0;
       c->timer_count[i] = 0;
+      c->timer_regions[i] = 0;
     }
   return BEM_OK;
 }
 
-int bem_ctx_timer_read(bem_ctx* c, int slot, double* ms, long long* units) {
+int bem_ctx_timer_read(bem_ctx* c, int slot, double* ms, long long* units, long long* regions) {
   if (!c || !ms || !units || slot < 0 || slot >= BEM_TIMER_SLOTS) return fail(BEM_E_ARG, "invalid timer query");
   *ms = c->timer_ms[slot];
   *units = c->timer_count[slot];
+  if (regions) *regions = c->timer_regions[slot];
   return BEM_OK;
 }
 
@@ -498,12 +553,19 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
   if (n > 12288) return fail(BEM_E_ARG, "N above 12288 is not supported by the register panel");
   hipError_t e;
   ScopedTimer tm(c, BEM_TIMER_LU, (long long)batch);
+  // matrices are independent: sub-batches of at least 4 factor on separate streams
   if (precision == BEM_PREC_MIXED) {
-    MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
-    e = lu_factor_batch<float>(mb, batch, ipiv, perm, strideP, info, nullptr, c->stream);
+    e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) {
+      MatBatch<float> mb = {static_cast<cplx<float>*>(A) + (size_t)b0 * strideA, lda, strideA, n};
+      return lu_factor_batch<float>(mb, nb, ipiv + (size_t)b0 * strideP, perm + (size_t)b0 * strideP, strideP,
+                                    info + b0, nullptr, st);
+    });
   } else if (precision == BEM_PREC_FP64) {
-    MatBatch<double> mb = {static_cast<cplx<double>*>(A), lda, strideA, n};
-    e = lu_factor_batch<double>(mb, batch, ipiv, perm, strideP, info, nullptr, c->stream);
+    e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) {
+      MatBatch<double> mb = {static_cast<cplx<double>*>(A) + (size_t)b0 * strideA, lda, strideA, n};
+      return lu_factor_batch<double>(mb, nb, ipiv + (size_t)b0 * strideP, perm + (size_t)b0 * strideP, strideP,
+                                     info + b0, nullptr, st);
+    });
   } else {
     return fail(BEM_E_ARG, "unknown precision");
   }
@@ -516,6 +578,7 @@ int bem_lu_solve(bem_ctx* c, int precision, int n, int batch, const void* LU, lo
                  long long ldy, long long strideY, int nrhs, const int* skip) {
   if (!c || !LU || !perm || !b || !y) return fail(BEM_E_ARG, "NULL argument");
   if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || n <= 0 || batch <= 0) return fail(BEM_E_ARG, "invalid sizes");
+  ScopedTimer tm(c, BEM_TIMER_SOLVE, (long long)batch);
   hipError_t e;
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {(cplx<float>*)LU, lda, strideA, n};
@@ -538,30 +601,34 @@ int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long
   if (!c || !LU || !perm || !A0 || !b || !x || !work_r || !work_y || !rel || !iters || !done)
     return fail(BEM_E_ARG, "NULL argument");
   if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || max_iter < 0) return fail(BEM_E_ARG, "invalid arguments");
-  hipStream_t st = c->stream;
-  MatBatch<float> lu = {(cplx<float>*)LU, lda, strideA, n};
-  MatBatch<float> a0 = {(cplx<float>*)A0, lda, strideA, n};
-  const cplx<double>* bb = (const cplx<double>*)b;
-  cplx<double>* xx = (cplx<double>*)x;
-  cplx<double>* rr = (cplx<double>*)work_r;
-  cplx<float>* yy = (cplx<float>*)work_y;
-  // done[] = 0 with a fill kernel (no memset nodes: this sequence may be captured into a graph)
-  hipError_t e = fill_int(done, batch, 0, st);
-  if (e != hipSuccess) return hipfail(e, "clear done");
-  if ((e = lu_solve_batch<float>(lu, batch, perm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, st)))
-    return hipfail(e, "refine initial solve");
-  if ((e = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 1, nullptr, batch, st)))
-    return hipfail(e, "refine init x");
-  for (int it = 0; it <= max_iter; ++it) {
-    if ((e = residual_batch<float>(a0, batch, bb, xx, rr, ldb, strideB, nrhs, done, st))) return hipfail(e, "residual");
-    if ((e = resnorm_batch(rr, bb, ldb, strideB, n, nrhs, tol, rel, done, it, iters, batch, st)))
-      return hipfail(e, "resnorm");
-    if (it == max_iter) break;
-    if ((e = lu_solve_batch<float>(lu, batch, perm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, done, st)))
-      return hipfail(e, "refine solve");
-    if ((e = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 0, done, batch, st)))
-      return hipfail(e, "refine update");
-  }
+  ScopedTimer tm(c, BEM_TIMER_SOLVE, (long long)batch);
+  // matrices are independent: sub-batches refine on separate streams (each solve is a chain of short launches)
+  hipError_t e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) -> hipError_t {
+    MatBatch<float> lu = {(cplx<float>*)LU + (size_t)b0 * strideA, lda, strideA, n};
+    MatBatch<float> a0 = {(cplx<float>*)A0 + (size_t)b0 * strideA, lda, strideA, n};
+    const int* pm = perm + (size_t)b0 * strideP;
+    const cplx<double>* bb = (const cplx<double>*)b + (size_t)b0 * strideB;
+    cplx<double>* xx = (cplx<double>*)x + (size_t)b0 * strideB;
+    cplx<double>* rr = (cplx<double>*)work_r + (size_t)b0 * strideB;
+    cplx<float>* yy = (cplx<float>*)work_y + (size_t)b0 * strideB;
+    int* dn = done + b0;
+    hipError_t e2;
+    // done[] = 0 with a fill kernel (no memset nodes: this sequence may be captured into a graph)
+    if ((e2 = fill_int(dn, nb, 0, st))) return e2;
+    if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, st)))
+      return e2;
+    if ((e2 = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 1, nullptr, nb, st))) return e2;
+    for (int it = 0; it <= max_iter; ++it) {
+      if ((e2 = residual_batch<float>(a0, nb, bb, xx, rr, ldb, strideB, nrhs, dn, st))) return e2;
+      if ((e2 = resnorm_batch(rr, bb, ldb, strideB, n, nrhs, tol, rel + b0, dn, it, iters + b0, nb, st))) return e2;
+      if (it == max_iter) break;
+      if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, dn, st)))
+        return e2;
+      if ((e2 = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 0, dn, nb, st))) return e2;
+    }
+    return hipSuccess;
+  });
+  if (e != hipSuccess) return hipfail(e, "refine");
   return BEM_OK;
 }
 

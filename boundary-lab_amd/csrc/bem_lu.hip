@@ -151,22 +151,34 @@ __global__ __launch_bounds__(256) void k_trsm_lower(MatBatch<T> mb, int r0, int 
   const int tid = threadIdx.x;
   const int cb = col0 + blockIdx.x * TW;
   __shared__ cplx<T> X[64][TW + 1];
+  __shared__ cplx<T> Ls[64][65];  // strictly lower part of the diagonal block (the serial sweep reads only LDS)
   for (int e = tid; e < 64 * TW; e += 256) {
     const int r = e / TW, j = e % TW;
     cplx<T> v = mk<T>(0, 0);
     if (r < nb && cb + j < col0 + ncols) v = A[(size_t)(r0 + r) * lda + cb + j];
     X[r][j] = v;
   }
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    Ls[r][c] = (r < nb && c < r) ? A[(size_t)(r0 + r) * lda + r0 + c] : mk<T>(0, 0);
+  }
   __syncthreads();
-  for (int b0 = 0; b0 < nb; b0 += 16) {
-    const int be = min(nb, b0 + 16);
-    // diagonal block: one thread per column
+  constexpr int BS = sizeof(T) == 4 ? 16 : 8;  // diagonal block rows held in registers
+  for (int b0 = 0; b0 < nb; b0 += BS) {
+    const int be = min(nb, b0 + BS);
+    // diagonal block: one thread per column, the 16 unknowns in registers (L reads are independent of the
+    // substitution chain and pipeline; rows past nb are zero in Ls and never stored)
     if (tid < TW) {
-      for (int r = b0 + 1; r < be; ++r) {
-        cplx<T> acc = X[r][tid];
-        for (int c = b0; c < r; ++c) acc = cfms(acc, A[(size_t)(r0 + r) * lda + r0 + c], X[c][tid]);
-        X[r][tid] = acc;
-      }
+      cplx<T> x[BS];
+#pragma unroll
+      for (int i = 0; i < BS; ++i) x[i] = b0 + i < be ? X[b0 + i][tid] : mk<T>(0, 0);
+#pragma unroll
+      for (int i = 1; i < BS; ++i)
+#pragma unroll
+        for (int c = 0; c < i; ++c) x[i] = cfms(x[i], Ls[b0 + i][b0 + c], x[c]);
+#pragma unroll
+      for (int i = 0; i < BS; ++i)
+        if (b0 + i < be) X[b0 + i][tid] = x[i];
     }
     __syncthreads();
     // update rows below the block
@@ -174,8 +186,9 @@ __global__ __launch_bounds__(256) void k_trsm_lower(MatBatch<T> mb, int r0, int 
     for (int e = tid; e < rows_below * TW; e += 256) {
       const int r = be + e / TW, j = e % TW;
       cplx<T> acc = X[r][j];
-      const cplx<T>* Lr = A + (size_t)(r0 + r) * lda + r0;
-      for (int c = b0; c < be; ++c) acc = cfms(acc, Lr[c], X[c][j]);
+#pragma unroll
+      for (int c = 0; c < BS; ++c)
+        if (b0 + c < be) acc = cfms(acc, Ls[r][b0 + c], X[b0 + c][j]);
       X[r][j] = acc;
     }
     __syncthreads();
@@ -187,12 +200,116 @@ __global__ __launch_bounds__(256) void k_trsm_lower(MatBatch<T> mb, int r0, int 
 }
 
 // ------------------------------------------------------------------------------------------------
-// Sub-panel factorization in registers: rows [c0, N), columns [c0, c0+nb), nb <= NBI.
+// Pivot search: exact |re|+|im| argmax (LAPACK i?amax: first index among equal maxima) as a max over
+// integer keys {bits of the non-negative magnitude, ~row}; rows with no candidate carry key 0. A wave reduces
+// its keys with DPP row shifts and row broadcasts (no LDS round trips); lane 63 holds the result.
+// ------------------------------------------------------------------------------------------------
+
+template <typename T>
+struct PivKey;
+template <>
+struct PivKey<float> {
+  static constexpr int W = 2;
+  __device__ __forceinline__ static void make(float v, int row, unsigned (&k)[2]) {
+    k[0] = __float_as_uint(v);
+    k[1] = ~(unsigned)row;
+  }
+};
+template <>
+struct PivKey<double> {
+  static constexpr int W = 3;
+  __device__ __forceinline__ static void make(double v, int row, unsigned (&k)[3]) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    k[0] = (unsigned)(b >> 32);
+    k[1] = (unsigned)b;
+    k[2] = ~(unsigned)row;
+  }
+};
+
+template <int W>
+__device__ __forceinline__ bool key_less(const unsigned (&a)[W], const unsigned (&b)[W]) {
+  bool lt = false, eq = true;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    lt = lt || (eq && a[i] < b[i]);
+    eq = eq && a[i] == b[i];
+  }
+  return lt;
+}
+
+template <int CTRL, int ROWMASK, int W>
+__device__ __forceinline__ void dpp_max_step(unsigned (&k)[W]) {
+  unsigned o[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) o[i] = (unsigned)__builtin_amdgcn_update_dpp((int)k[i], (int)k[i], CTRL, ROWMASK, 0xf, false);
+  if (key_less<W>(k, o))
+#pragma unroll
+    for (int i = 0; i < W; ++i) k[i] = o[i];
+}
+
+// max over the 64 lanes, returned in every lane
+template <int W>
+__device__ __forceinline__ void wave_max_key(unsigned (&k)[W]) {
+  dpp_max_step<0x111, 0xf, W>(k);  // row_shr:1
+  dpp_max_step<0x112, 0xf, W>(k);  // row_shr:2
+  dpp_max_step<0x114, 0xf, W>(k);  // row_shr:4
+  dpp_max_step<0x118, 0xf, W>(k);  // row_shr:8  -> lane 15 of each row holds the row maximum
+  dpp_max_step<0x142, 0xa, W>(k);  // row_bcast:15 into rows 1 and 3
+  dpp_max_step<0x143, 0xc, W>(k);  // row_bcast:31 into rows 2 and 3 -> lane 63 holds the maximum
+#pragma unroll
+  for (int i = 0; i < W; ++i) k[i] = (unsigned)__builtin_amdgcn_readlane((int)k[i], 63);
+}
+
+// Composite of the row interchanges (c0+jj, piv[jj]), jj < nb, applied in order (LAPACK laswp order),
+// restricted to the rows it touches: afterwards row rows[q] holds the former row rows[src[q]]. Run by one
+// wave; rows [c0, c0+nb) come first, pivot rows beyond are appended in first-seen order. Returns the count.
+__device__ int compose_swaps(const int* piv, int c0, int nb, int* rows, int* src, int lane) {
+  for (int q = lane; q < nb; q += 64) {
+    rows[q] = c0 + q;
+    src[q] = q;
+  }
+  int na = nb;
+  for (int jj = 0; jj < nb; ++jj) {
+    const int pv = piv[jj];
+    int b;
+    if (pv < c0 + nb) {
+      b = pv - c0;
+    } else {
+      // look among the appended rows (at most nb of them; 64 lanes per pass)
+      b = -1;
+      for (int q0 = nb; q0 < na && b < 0; q0 += 64) {
+        const int q = q0 + lane;
+        const unsigned long long hit = __ballot(q < na && rows[q] == pv);
+        if (hit) b = q0 + __builtin_ctzll(hit);
+      }
+      if (b < 0) {
+        b = na++;
+        if (lane == 0) {
+          rows[b] = pv;
+          src[b] = b;
+        }
+      }
+    }
+    if (lane == 0 && b != jj) {
+      const int t = src[jj];
+      src[jj] = src[b];
+      src[b] = t;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  return na;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Sub-panel factorization in registers: rows [c0, N), columns [c0, c0+nb), nb <= NBI. The row interchanges
+// are applied to the other columns of the outer panel [k0, k1) here; columns outside the outer panel are
+// permuted once per outer panel by k_laswp.
 // ------------------------------------------------------------------------------------------------
 
 template <typename T, int NBI, int MR>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lu_panel(MatBatch<T> mb, int c0, int nb, int* ipiv, int* perm,
-                                                 long long strideP, int* info, const int* skip) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lu_panel(
+    MatBatch<T> mb, int c0, int nb, int k0, int k1, int* ipiv, int* perm, long long strideP, int* info,
+    const int* skip) {
   const int mat = blockIdx.x;
   if (skip && skip[mat]) return;
   cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
@@ -201,13 +318,16 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   const long long lda = mb.lda;
   const int N = mb.N;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  __shared__ T s_val[16];
-  __shared__ int s_idx[16];
-  __shared__ int s_piv;
+  constexpr int KW = PivKey<T>::W;
+  // per-column exchange area, double-buffered by column parity so that one barrier per column suffices:
+  // every wave publishes its best key and that row's data; the owner of row c publishes row c
+  __shared__ unsigned s_key[2][16][KW];
+  __shared__ cplx<T> s_cand[2][16][NBI];
+  __shared__ cplx<T> s_crow[2][NBI];
   __shared__ int s_pivs[NBI];
-  __shared__ cplx<T> s_prow[NBI];
-  __shared__ cplx<T> s_crow[NBI];
-  __shared__ int s_zero;
+  __shared__ int s_rows[2 * NBI];
+  __shared__ int s_src[2 * NBI];
+  __shared__ int s_na;
 
   cplx<T> P[MR][NBI];
 #pragma unroll
@@ -217,86 +337,81 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     for (int j = 0; j < NBI; ++j)
       P[m][j] = (r < N && j < nb) ? A[(size_t)r * lda + c0 + j] : mk<T>(0, 0);
   }
-  if (tid == 0) s_zero = 0;
+  int zero_col = 0;  // first breakdown column (1-based), tracked by thread 0
 
 #pragma unroll
   for (int jj = 0; jj < NBI; ++jj) {
     if (jj < nb) {
+      const int buf = jj & 1;
       const int c = c0 + jj;
-      T best = T(-1);
-      int bidx = 0x7fffffff;
+      unsigned key[KW];
+#pragma unroll
+      for (int i = 0; i < KW; ++i) key[i] = 0u;
 #pragma unroll
       for (int m = 0; m < MR; ++m) {
         const int r = c0 + tid + 1024 * m;
         if (r >= c && r < N) {
-          const T v = cabs1(P[m][jj]);
-          if (v > best) {
-            best = v;
-            bidx = r;
-          }
-        }
-      }
+          unsigned kr[KW];
+          PivKey<T>::make(cabs1(P[m][jj]), r, kr);
+          if (key_less<KW>(key, kr))
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const T ov = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bidx, o, 64);
-        if (ov > best || (ov == best && oi < bidx)) {
-          best = ov;
-          bidx = oi;
+            for (int i = 0; i < KW; ++i) key[i] = kr[i];
         }
       }
-      if (lane == 0) {
-        s_val[w] = best;
-        s_idx[w] = bidx;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        T bv = s_val[0];
-        int bi = s_idx[0];
-        for (int q = 1; q < 16; ++q)
-          if (s_val[q] > bv || (s_val[q] == bv && s_idx[q] < bi)) {
-            bv = s_val[q];
-            bi = s_idx[q];
-          }
-        s_piv = bi;
-        s_pivs[jj] = bi;
-      }
-      __syncthreads();
-      const int pv = s_piv;
-      const int own_p = (pv - c0) & 1023, m_p = (pv - c0) >> 10;
-      const int own_c = (c - c0) & 1023, m_c = (c - c0) >> 10;
+      wave_max_key<KW>(key);
+      const int wrow = (int)~key[KW - 1];  // this wave's best row (meaningless when key is 0)
+      if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < KW; ++i) s_key[buf][w][i] = key[i];
 #pragma unroll
       for (int m = 0; m < MR; ++m) {
-        if (tid == own_p && m == m_p)
+        const int r = c0 + tid + 1024 * m;
+        if (r == wrow && r < N)
 #pragma unroll
-          for (int j = 0; j < NBI; ++j) s_prow[j] = P[m][j];
-        if (tid == own_c && m == m_c)
+          for (int j = 0; j < NBI; ++j) s_cand[buf][w][j] = P[m][j];
+        if (r == c)
 #pragma unroll
-          for (int j = 0; j < NBI; ++j) s_crow[j] = P[m][j];
+          for (int j = 0; j < NBI; ++j) s_crow[buf][j] = P[m][j];
       }
       __syncthreads();
+      // every lane reduces the 16 wave keys (LDS broadcast reads)
+      unsigned best[KW];
+      int gw = 0;
+#pragma unroll
+      for (int i = 0; i < KW; ++i) best[i] = s_key[buf][0][i];
+#pragma unroll
+      for (int q = 1; q < 16; ++q) {
+        unsigned kq[KW];
+#pragma unroll
+        for (int i = 0; i < KW; ++i) kq[i] = s_key[buf][q][i];
+        if (key_less<KW>(best, kq)) {
+#pragma unroll
+          for (int i = 0; i < KW; ++i) best[i] = kq[i];
+          gw = q;
+        }
+      }
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < KW; ++i) any = any || best[i] != 0u;
+      // no candidate at all cannot happen for c < N; keep row c then (the pivot test records a breakdown)
+      const int pv = any ? (int)~best[KW - 1] : c;
+      const cplx<T>* prow = pv == c ? s_crow[buf] : s_cand[buf][gw];  // pivot row (LDS broadcast reads)
       if (pv != c) {
 #pragma unroll
         for (int m = 0; m < MR; ++m) {
-          if (tid == own_c && m == m_c)
+          const int r = c0 + tid + 1024 * m;
+          if (r == c)
 #pragma unroll
-            for (int j = 0; j < NBI; ++j) P[m][j] = s_prow[j];
-          if (tid == own_p && m == m_p)
+            for (int j = 0; j < NBI; ++j) P[m][j] = prow[j];
+          if (r == pv)
 #pragma unroll
-            for (int j = 0; j < NBI; ++j) P[m][j] = s_crow[j];
+            for (int j = 0; j < NBI; ++j) P[m][j] = s_crow[buf][j];
         }
       }
-      if (tid == 0) {
-        piv_out[c] = pv;
-        if (pv != c) {
-          const int tmp = prm[c];
-          prm[c] = prm[pv];
-          prm[pv] = tmp;
-        }
-      }
-      const cplx<T> pivot = s_prow[jj];
-      if (cabs1(pivot) == T(0)) {
-        if (tid == 0 && s_zero == 0) s_zero = c + 1;
+      if (tid == 0) s_pivs[jj] = pv;
+      const cplx<T> pivot = prow[jj];
+      if (!(cabs1(pivot) > T(0))) {  // zero or NaN pivot: breakdown, no elimination
+        if (tid == 0 && zero_col == 0) zero_col = c + 1;
       } else {
         const cplx<T> inv = cdiv(mk<T>(1, 0), pivot);
 #pragma unroll
@@ -306,11 +421,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
             const cplx<T> l = cmul(P[m][jj], inv);
             P[m][jj] = l;
 #pragma unroll
-            for (int j2 = jj + 1; j2 < NBI; ++j2) P[m][j2] = cfms(P[m][j2], l, s_prow[j2]);
+            for (int j2 = jj + 1; j2 < NBI; ++j2) P[m][j2] = cfms(P[m][j2], l, prow[j2]);
           }
         }
       }
-      __syncthreads();
     }
   }
 #pragma unroll
@@ -321,21 +435,98 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
       for (int j = 0; j < NBI; ++j)
         if (j < nb) A[(size_t)r * lda + c0 + j] = P[m][j];
   }
-  // row interchanges outside the sub-panel, in pivot order
-  for (int col = tid; col < N; col += 1024) {
-    if (col >= c0 && col < c0 + nb) continue;
-    for (int jj = 0; jj < nb; ++jj) {
-      const int c = c0 + jj, pv = s_pivs[jj];
-      if (pv != c) {
-        cplx<T>* a = A + (size_t)c * lda + col;
-        cplx<T>* b = A + (size_t)pv * lda + col;
-        const cplx<T> t = *a;
-        *a = *b;
-        *b = t;
-      }
+  __syncthreads();
+  if (w == 0) {
+    const int na = compose_swaps(s_pivs, c0, nb, s_rows, s_src, lane);
+    if (lane == 0) s_na = na;
+    // LAPACK-style ipiv and the row permutation (loads of the wave complete before its stores)
+    if (lane < nb) piv_out[c0 + lane] = s_pivs[lane];
+    const int pq = lane < na ? prm[s_rows[s_src[lane]]] : 0;
+    if (lane < na) prm[s_rows[lane]] = pq;
+    if (lane == 0 && zero_col != 0 && info[mat] == 0) info[mat] = zero_col;
+  }
+  __syncthreads();
+  // interchanges on the other columns of the outer panel: one (column, row) pair per thread, all loads first
+  const int na = s_na;
+  const int ncols = (k1 - k0) - nb;
+  constexpr int PER = (2 * NBI * BEM_NBO + 1023) / 1024;
+  T vr[PER], vi[PER];  // scalar arrays stay in registers
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + 1024 * i;
+    const int q = e % (2 * NBI), cc = e / (2 * NBI);
+    vr[i] = vi[i] = T(0);
+    if (cc < ncols && q < na) {
+      const int col = k0 + cc + (k0 + cc >= c0 ? nb : 0);
+      const cplx<T> x = A[(size_t)s_rows[s_src[q]] * lda + col];
+      vr[i] = x.re;
+      vi[i] = x.im;
     }
   }
-  if (tid == 0 && s_zero != 0 && info[mat] == 0) info[mat] = s_zero;
+  // the barrier does not wait for outstanding global loads: consume them first, so that no load of this
+  // thread can observe a store issued by another thread after the barrier
+#pragma unroll
+  for (int i = 0; i < PER; ++i) asm volatile("" ::"v"(vr[i]), "v"(vi[i]));
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + 1024 * i;
+    const int q = e % (2 * NBI), cc = e / (2 * NBI);
+    if (cc < ncols && q < na && s_src[q] != q) {
+      const int col = k0 + cc + (k0 + cc >= c0 ? nb : 0);
+      A[(size_t)s_rows[q] * lda + col] = mk<T>(vr[i], vi[i]);
+    }
+  }
+}
+
+// Row interchanges of one outer panel (pivots of rows [k0, k1)) on the columns outside it. Each workgroup
+// resolves the composite permutation in LDS, stages the touched rows of its column block in LDS and writes
+// them back permuted.
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, long long strideP, int k0, int k1,
+                                               const int* skip) {
+  constexpr int CB = sizeof(T) == 4 ? 64 : 32;  // columns per workgroup
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  const int* piv = ipiv + (size_t)mat * strideP;
+  const long long lda = mb.lda;
+  const int N = mb.N;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ int s_piv[BEM_NBO];
+  __shared__ int s_rows[2 * BEM_NBO];
+  __shared__ int s_src[2 * BEM_NBO];
+  __shared__ int s_na;
+  __shared__ cplx<T> buf[2 * BEM_NBO][CB];
+  const int nb = k1 - k0;
+  if (tid < nb) s_piv[tid] = piv[k0 + tid];
+  __syncthreads();
+  if (w == 0) {
+    const int na = compose_swaps(s_piv, k0, nb, s_rows, s_src, lane);
+    if (lane == 0) s_na = na;
+  }
+  __syncthreads();
+  const int na = s_na;
+  // column block: columns [0, k0) then [k1, N)
+  const int outside = N - nb;
+  const int cb0 = blockIdx.x * CB;
+  for (int e = tid; e < na * CB; e += 256) {
+    const int q = e / CB, jc = e % CB;
+    const int cc = cb0 + jc;
+    if (cc < outside) {
+      const int col = cc < k0 ? cc : cc + nb;
+      buf[q][jc] = A[(size_t)s_rows[s_src[q]] * lda + col];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < na * CB; e += 256) {
+    const int q = e / CB, jc = e % CB;
+    const int cc = cb0 + jc;
+    if (cc < outside && s_src[q] != q) {
+      const int col = cc < k0 ? cc : cc + nb;
+      A[(size_t)s_rows[q] * lda + col] = buf[q][jc];
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -374,62 +565,7 @@ __global__ void k_permute_rhs(const cplx<double>* src, long long lds_, long long
   y[(size_t)mat * strideY + (size_t)c * ldy + i] = mk<T>(T(v.re), T(v.im));
 }
 
-// Forward (unit lower) or backward (upper) substitution, one workgroup per matrix, in place on y.
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(1024) void k_trsv(MatBatch<T> mb, cplx<T>* y, long long ldy, long long strideY,
-                                             int nrhs, const int* skip) {
-  const int mat = blockIdx.x;
-  if (skip && skip[mat]) return;
-  const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
-  cplx<T>* Y = y + (size_t)mat * strideY;
-  const long long lda = mb.lda;
-  const int N = mb.N;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  __shared__ cplx<T> xb[MAX_CH][64];
-  const int nblk = (N + 63) / 64;
-  for (int bi = 0; bi < nblk; ++bi) {
-    const int blk = LOWER ? bi : nblk - 1 - bi;
-    const int r0 = blk * 64;
-    const int nb = min(64, N - r0);
-    if (w == 0) {
-      for (int c = 0; c < nrhs; ++c) {
-        cplx<T> v = lane < nb ? Y[(size_t)c * ldy + r0 + lane] : mk<T>(0, 0);
-        if (LOWER) {
-          for (int r = 0; r < nb; ++r) {
-            const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
-            if (lane > r && lane < nb) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
-          }
-        } else {
-          for (int r = nb - 1; r >= 0; --r) {
-            if (lane == r) v = cdiv(v, A[(size_t)(r0 + r) * lda + r0 + r]);
-            const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
-            if (lane < r) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
-          }
-        }
-        xb[c][lane] = v;
-        if (lane < nb) Y[(size_t)c * ldy + r0 + lane] = v;
-      }
-    }
-    __syncthreads();
-    // update the other rows: lower -> rows below, upper -> rows above
-    const int rbeg = LOWER ? r0 + nb : 0;
-    const int rend = LOWER ? N : r0;
-    for (int i = rbeg + w; i < rend; i += 16) {
-      const cplx<T> a = lane < nb ? A[(size_t)i * lda + r0 + lane] : mk<T>(0, 0);
-      for (int c = 0; c < nrhs; ++c) {
-        const cplx<T> pr = cmul(a, xb[c][lane]);
-        const T sr = wave_sum(pr.re), si = wave_sum(pr.im);
-        if (lane == 0) {
-          cplx<T>* yp = Y + (size_t)c * ldy + i;
-          *yp = mk<T>(yp->re - sr, yp->im - si);
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// One block step of a triangular sweep, many workgroups per matrix (the multi-block sweep of k_trsv).
+// One block step of a triangular sweep, many workgroups per matrix.
 // Lower (forward, unit L): blocks ascend; step `kb` applies the solved block kb to the rows of every later
 // block (one workgroup per 64-row block), and the workgroup owning block kb+1 then solves its diagonal block,
 // whose rows now carry every earlier contribution. kb = -1 only solves block 0.
@@ -450,6 +586,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, l
   const int r0 = rb * 64, nr = min(64, N - r0);
   __shared__ cplx<T> xk[MAX_CH][64];
   __shared__ cplx<T> yb[MAX_CH][64];  // updated rows of the block solved by this workgroup
+  __shared__ cplx<T> Dt[64][65];      // its diagonal block, transposed (padded: conflict-free column reads)
   const bool solve_here = LOWER ? (rb == kb + 1) : (rb == kb - 1);
   const bool updated = kb >= 0 && kb < nblk;
   if (updated) {
@@ -493,8 +630,13 @@ __global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, l
       }
     }
   }
-  // the workgroup holding the next block to solve finishes it (updated rows are handed over in LDS)
+  // the workgroup holding the next block to solve finishes it (updated rows are handed over in LDS); the
+  // diagonal block is staged in LDS first so the serial column sweep waits on no global load
   if (!solve_here) return;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int i = e >> 6, j = e & 63;  // coalesced along rows of A
+    Dt[j][i] = (i < nr && j < nr) ? A[(size_t)(r0 + i) * lda + r0 + j] : mk<T>(0, 0);
+  }
   __syncthreads();
   if (w != 0) return;
   for (int c = 0; c < nrhs; ++c) {
@@ -502,13 +644,13 @@ __global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, l
     if (LOWER) {
       for (int r = 0; r < nr; ++r) {
         const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
-        if (lane > r && lane < nr) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
+        if (lane > r && lane < nr) v = cfms(v, Dt[r][lane], xr);
       }
     } else {
       for (int r = nr - 1; r >= 0; --r) {
-        if (lane == r) v = cdiv(v, A[(size_t)(r0 + r) * lda + r0 + r]);
+        if (lane == r) v = cdiv(v, Dt[r][r]);
         const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
-        if (lane < r) v = cfms(v, A[(size_t)(r0 + lane) * lda + r0 + r], xr);
+        if (lane < r) v = cfms(v, Dt[r][lane], xr);
       }
     }
     if (lane < nr) Y[(size_t)c * ldy + r0 + lane] = v;
@@ -655,15 +797,15 @@ __global__ __launch_bounds__(256) void k_growth(const double* umax, const double
 // ------------------------------------------------------------------------------------------------
 
 template <typename T>
-static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb, int* ipiv, int* perm,
-                               long long strideP, int* info, const int* skip, hipStream_t st) {
+static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb, int k0, int k1, int* ipiv,
+                               int* perm, long long strideP, int* info, const int* skip, hipStream_t st) {
   const int H = mb.N - c0;
   const int mr = (H + 1023) / 1024;
 #define PANEL(NBI_, MR_)                                                                                    \
-  hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, ipiv, perm, \
-                     strideP, info, skip)
+  hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, k0, k1, ipiv, \
+                     perm, strideP, info, skip)
   // (NBI, MR) pairs chosen so that the register-resident sub-panel does not spill at 128 VGPRs/lane
-  if (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4) {
     if (mr <= 1) PANEL(16, 1);
     else if (mr <= 2) PANEL(8, 2);
     else if (mr <= 3) PANEL(8, 3);
@@ -672,7 +814,7 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
     else return hipErrorInvalidValue;
   } else {
     if (mr <= 1) PANEL(8, 1);
-    else if (mr <= 2) PANEL(4, 2);
+    else if (mr <= 2) PANEL(8, 2);
     else if (mr <= 3) PANEL(4, 3);
     else if (mr <= 6) PANEL(2, 6);
     else if (mr <= 12) PANEL(1, 12);
@@ -686,7 +828,7 @@ template <typename T>
 int panel_width(int N, int c0) {
   const int mr = (N - c0 + 1023) / 1024;
   if (sizeof(T) == 4) return mr <= 1 ? 16 : (mr <= 3 ? 8 : (mr <= 6 ? 4 : 2));
-  return mr <= 1 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
+  return mr <= 2 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
 }
 
 template <typename T>
@@ -730,8 +872,14 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
         GemmShape g = {c0, k0, k0, c0, c0, c0, N - c0, nb, c0 - k0};
         CHK(launch_gemm<T>(mb, batch, g, skip, st));
       }
-      CHK(launch_panel<T>(mb, batch, c0, nb, ipiv, perm, strideP, info, skip, st));
+      CHK(launch_panel<T>(mb, batch, c0, nb, k0, k1, ipiv, perm, strideP, info, skip, st));
       c0 += nb;
+    }
+    if (N > k1 - k0) {
+      constexpr int CB = sizeof(T) == 4 ? 64 : 32;
+      hipLaunchKernelGGL(k_laswp<T>, dim3((N - (k1 - k0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv, strideP,
+                         k0, k1, skip);
+      CHK(hipGetLastError());
     }
     if (k1 < N) {
       CHK(launch_trsm<T>(mb, batch, k0, k1 - k0, k1, N - k1, skip, st));

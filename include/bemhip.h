@@ -57,17 +57,24 @@ int bem_ctx_sync(bem_ctx* ctx);
 /* The hipStream_t used by the context. */
 int bem_ctx_stream(bem_ctx* ctx, void** stream);
 
+/* Batch fan-out (no reference counterpart; tuning). LU factorization and refinement split a batch of
+ * independent matrices into up to `streams` contiguous groups (at least 4 matrices each) that run on separate
+ * default-priority streams forked from and joined back into the context stream. Default 4; 1 disables. */
+int bem_ctx_set_fanout(bem_ctx* ctx, int streams);
+
 /* Kernel timers (no reference counterpart; diagnostics for roofline measurement). While enabled, each timed
  * launch group is bracketed by HIP events on the context's stream and waited for (not during capture).
- * bem_ctx_timer_read returns accumulated milliseconds and accumulated work units (frequencies or matrices). */
+ * bem_ctx_timer_read returns accumulated milliseconds, accumulated work units (frequencies or matrices) and the
+ * number of timed regions (one per kernel launch for the assembly slots; one per call for LU, solve, field). */
 #define BEM_TIMER_REGULAR 0  /* regular-pair assembly kernel */
 #define BEM_TIMER_SINGULAR 1 /* Duffy singular-pair kernel */
 #define BEM_TIMER_GATHER 2   /* row gather */
 #define BEM_TIMER_RHS 3      /* RHS partial sums */
 #define BEM_TIMER_LU 4       /* whole LU factorization sequence */
 #define BEM_TIMER_FIELD 5    /* field evaluation sequence */
+#define BEM_TIMER_SOLVE 6    /* triangular solves + refinement (bem_lu_solve, bem_refine) */
 int bem_ctx_timers(bem_ctx* ctx, int enable);
-int bem_ctx_timer_read(bem_ctx* ctx, int slot, double* ms, long long* units);
+int bem_ctx_timer_read(bem_ctx* ctx, int slot, double* ms, long long* units, long long* regions);
 
 /* Host-side description of a prepared boundary mesh (all pointers are HOST memory, copied by
  * bem_mesh_create). Indices are 0-based. */
