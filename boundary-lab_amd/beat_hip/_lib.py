@@ -62,6 +62,10 @@ class MeshDesc(C.Structure):
         ("m10", _d * 3),
         ("n_src", _i),
         ("src_faces", _ip),
+        ("n_transforms", _i),
+        ("transform_signs", _i * 12),
+        ("row_weights", _dp),
+        ("plane_mask", _ip),
     ]
 
 

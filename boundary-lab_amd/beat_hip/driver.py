@@ -472,12 +472,13 @@ def parse_request(request) -> tuple[dict, dict]:
 
 def build_job(request, parsed) -> Job:
     config = parsed["config"]
-    if parsed["symmetry"] != "off":
-        raise ValueError("symmetry x/xy is not supported by this engine yet; solve the full model (symmetry off).")
     regular_quadrature_mode_from_config(config)
     if int(get_value(config, "quadrature_order", 4)) != 4 or int(get_value(config, "singular_order", 4)) != 4:
         raise ValueError("This engine implements the order-4 regular rule and order-4 Duffy rules only.")
     mesh = load_combined_mesh(parsed["mesh_inputs"])
+    # x / xy symmetry: the mesh is the fundamental domain (solver.jl:718-719)
+    mesh = G.snap_symmetry_planes(mesh, parsed["symmetry"])
+    G.validate_symmetry_domain(mesh, parsed["symmetry"])
     validate_radiator_elements(mesh, parsed["radiators"])
     names = sorted({str(get_value(r, "channel", "main")) for r in parsed["radiators"]})
     qunit = channel_unit_qunit(mesh, parsed["radiators"], names)
@@ -512,7 +513,7 @@ def result_payload(job: Job, freq: float, p_vc: np.ndarray, field_oc: np.ndarray
                                                                                   1.0)
               for r in job.radiators]
     mixed = sum(p_vc[:, c] * syn["weights"][c] for c in range(C))
-    imp = impedance_for_radiators(job.mesh, mixed, job.radiators, drives)
+    imp = impedance_for_radiators(job.mesh, mixed, job.radiators, drives, G.symmetry_reduction_factor(job.symmetry))
     return {
         "freq_hz": float(np.float32(freq)),
         "horizontal_spl_norm_db": syn["horizontal_norm"].tolist(),
@@ -577,7 +578,7 @@ def run_job(request, engine_factory, emit, batch: int | None = None):
     emit({"type": "status", "message": f"BEAT Engine loading {len(parsed['mesh_inputs'])} mesh"
           f"{'' if len(parsed['mesh_inputs']) == 1 else 'es'} (HIP engine)"})
     job = build_job(request, parsed)
-    prep = G.prepare(job.mesh)
+    prep = G.prepare(job.mesh, symmetry=job.symmetry)
     emit({"type": "status", "message": f"Initializing BEAT Engine using native HIP assembly and GPU solve "
           f"({job.precision}, {job.mesh.n_faces} triangles, {job.mesh.n_vertices} P1 dofs)..."})
     eng = engine_factory(prep, job.qunit)

@@ -122,6 +122,13 @@ class DeviceMesh:
         src = arr("src", np.asarray(src_faces, dtype=np.int32).reshape(-1), np.int32)
         d.n_src = len(src)
         d.src_faces = _np_ptr(src, C.c_int) if len(src) else C.POINTER(C.c_int)()
+        tf = prep.transforms if prep.transforms is not None else np.ones((1, 3), np.int32)
+        d.n_transforms = len(tf)
+        for i, v in enumerate(np.asarray(tf, dtype=np.int32).reshape(-1)):
+            d.transform_signs[i] = int(v)
+        if prep.row_weights is not None and prep.symmetry != "off":
+            d.row_weights = _np_ptr(arr("rw", prep.row_weights, np.float64), C.c_double)
+            d.plane_mask = _np_ptr(arr("pm", prep.plane_mask, np.int32), C.c_int)
         h = C.c_void_p()
         _lib.check(dev.lib.bem_mesh_create(dev.ctx, C.byref(d), C.byref(h)), "bem_mesh_create")
         self.handle = h

@@ -113,6 +113,10 @@ def combine(parts) -> BoundaryMesh:
 # ---------------------------------------------------------------------------------------------
 
 # Order-4 six-point rule on the reference triangle (xi, eta); weights sum to 1/2.
+# Positions closer than this (model units, metres) are the same point for image-pair detection
+# (BeatEngineCore.jl:917, geometric tolerance of image_singular_candidates).
+GEOMETRY_TOLERANCE = 1e-8
+
 RULE_POINTS = np.array(
     [
         [0.4459484909159651, 0.4459484909159651],
@@ -286,8 +290,8 @@ def _vertex_incidence(faces, n_vertices):
     return np.cumsum(ptr), e[o], v[o]
 
 
-def singular_pairs(mesh: BoundaryMesh, order: int = 4) -> SingularPairs:
-    """All (test, trial) pairs sharing at least one vertex id (incl. t == s), with remapped Duffy rules."""
+def _topological_candidates(mesh: BoundaryMesh):
+    """(t, s) pairs sharing at least one vertex id (incl. t == s) and their 3x3 vertex match matrices."""
     faces = mesh.faces
     F = len(faces)
     ptr, inc, _ = _vertex_incidence(faces, mesh.n_vertices)
@@ -300,23 +304,34 @@ def singular_pairs(mesh: BoundaryMesh, order: int = 4) -> SingularPairs:
     s_idx = inc[np.arange(lens.sum()) + seg]
     key = np.unique(t_idx * F + s_idx)
     t, s = key // F, key % F
-    ft, fs = faces[t], faces[s]
-    match = ft[:, :, None] == fs[:, None, :]  # (P,3,3) [i test, j trial]
+    match = faces[t][:, :, None] == faces[s][:, None, :]  # (P,3,3) [i test, j trial]
+    return t, s, match
+
+
+def _pair_codes(match):
+    """Orientation codes of vertex-sharing pairs (BeatEngineCore.jl:453-507,820-844): kind 1 coincident,
+    2 edge (test a, b; trial a, b after ordering the trial indices), 3 vertex (test a, trial a)."""
     nshared = match.sum(axis=(1, 2))
     kinds = np.where(nshared >= 3, 1, np.where(nshared == 2, 2, 3))
     # edge orientation: list matches in (i, j) scan order, then sort the pair by trial index j
-    flat = match.reshape(len(t), 9)
+    flat = match.reshape(len(match), 9)
     order_ij = np.argsort(~flat, axis=1, kind="stable")[:, :2]  # first two matches in scan order
     ia, ja = order_ij[:, 0] // 3, order_ij[:, 0] % 3
     ib, jb = order_ij[:, 1] // 3, order_ij[:, 1] % 3
     swap = jb < ja
     ia2, ib2 = np.where(swap, ib, ia), np.where(swap, ia, ib)
     ja2, jb2 = np.where(swap, jb, ja), np.where(swap, ja, jb)
-    codes = np.where(
+    return np.where(
         kinds == 1,
         100000,
         np.where(kinds == 2, 200000 + ia2 * 1000 + ib2 * 100 + ja2 * 10 + jb2, 300000 + ia * 1000 + ja * 10),
     )
+
+
+def _build_pairs(F, t, key, codes, order):
+    """Pairs grouped by test element (CSR), sorted by trial key, with one remapped Duffy rule per code."""
+    o = np.lexsort((key, t))
+    t, key, codes = t[o], key[o], codes[o]
     uniq, rule_of_pair = np.unique(codes, return_inverse=True)
     bases = {k: duffy_base(order, k) for k in (1, 2, 3)}
     pts, offs, lens_r, keys = [], [], [], []
@@ -342,14 +357,71 @@ def singular_pairs(mesh: BoundaryMesh, order: int = 4) -> SingularPairs:
     np.add.at(ptr_t, t + 1, 1)
     return SingularPairs(
         test=t.astype(np.int32),
-        trial=s.astype(np.int32),
+        trial=key.astype(np.int32),
         rule=rule_of_pair.astype(np.int32),
         ptr=np.cumsum(ptr_t).astype(np.int32),
         rule_off=np.asarray(offs, dtype=np.int32),
         rule_len=np.asarray(lens_r, dtype=np.int32),
-        rule_pts=np.ascontiguousarray(np.concatenate(pts), dtype=np.float64),
+        rule_pts=np.ascontiguousarray(np.concatenate(pts), dtype=np.float64) if pts else np.zeros((0, 5)),
         keys=keys,
     )
+
+
+def singular_pairs(mesh: BoundaryMesh, order: int = 4, transforms=None, tol: float = GEOMETRY_TOLERANCE
+                   ) -> SingularPairs:
+    """All (test, trial) pairs sharing at least one vertex id (incl. t == s), with remapped Duffy rules.
+
+    With plane symmetry, ``transforms`` lists the sign vectors of the identity and the image transforms; the
+    image pairs (t, s) whose reflected trial element R_r s touches t geometrically (BeatEngineCore.jl:917-945,
+    BeatEngineRocmRegular.jl:103-132) are added with trial key ``s + r * F``."""
+    F = mesh.n_faces
+    t, s, match = _topological_candidates(mesh)
+    ts, keys, codes = [t], [s], [_pair_codes(match)]
+    for r, signs in enumerate(transforms or ()):
+        if r == 0:
+            continue
+        ti, si, mi = image_candidates(mesh, np.asarray(signs, float), tol)
+        if len(ti):
+            ts.append(ti)
+            keys.append(si + r * F)
+            codes.append(_pair_codes(mi))
+    return _build_pairs(F, np.concatenate(ts), np.concatenate(keys), np.concatenate(codes), order)
+
+
+def image_candidates(mesh: BoundaryMesh, signs, tol: float):
+    """(t, s, match) with match[i, j] = |x_t,i - R x_s,j| <= tol, for the pairs with at least one such vertex
+    (candidates by rounded positions as BeatEngineCore.jl:509-515,917-945)."""
+    fv = mesh.face_vertices
+    V = mesh.vertices
+    keyv = np.rint(V / tol).astype(np.int64)
+    keyr = np.rint(V * signs / tol).astype(np.int64)
+    both = np.concatenate([keyv, keyr])
+    _, inv = np.unique(both, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    gv, gr = inv[: len(V)], inv[len(V):]
+    # vertices u whose position equals the reflection of vertex v
+    order = np.argsort(gv, kind="stable")
+    sorted_g = gv[order]
+    lo = np.searchsorted(sorted_g, gr, side="left")
+    hi = np.searchsorted(sorted_g, gr, side="right")
+    vv = np.repeat(np.arange(len(V)), hi - lo)
+    uu = order[np.concatenate([np.arange(a, b) for a, b in zip(lo, hi)]) if len(vv) else np.zeros(0, np.int64)]
+    if len(vv) == 0:
+        z = np.zeros(0, np.int64)
+        return z, z, np.zeros((0, 3, 3), bool)
+    ptr, inc, _ = _vertex_incidence(mesh.faces, mesh.n_vertices)
+    tt, ss = [], []
+    for u, v in zip(uu.tolist(), vv.tolist()):
+        tu = inc[ptr[u]:ptr[u + 1]]
+        sv = inc[ptr[v]:ptr[v + 1]]
+        tt.append(np.repeat(tu, len(sv)))
+        ss.append(np.tile(sv, len(tu)))
+    key = np.unique(np.concatenate(tt) * mesh.n_faces + np.concatenate(ss))
+    t, s = key // mesh.n_faces, key % mesh.n_faces
+    d = fv[t][:, :, None, :] - (fv[s] * signs)[:, None, :, :]
+    match = np.sqrt((d * d).sum(-1)) <= tol
+    keep = match.any(axis=(1, 2))
+    return t[keep], s[keep], match[keep]
 
 
 # ---------------------------------------------------------------------------------------------
@@ -456,6 +528,10 @@ class PreparedMesh:
     fperm: np.ndarray
     vperm: np.ndarray
     original: BoundaryMesh
+    symmetry: str = "off"
+    transforms: np.ndarray = None  # (R,3) sign vectors, identity first
+    row_weights: np.ndarray = None  # (V,) P1 orbit weights (internal order)
+    plane_mask: np.ndarray = None  # (F,) bit r: face takes part in an image pair of transform r
 
     @property
     def n_faces(self):
@@ -490,7 +566,90 @@ def locality_order(mesh: BoundaryMesh):
     return fperm.astype(np.int64), vperm
 
 
-def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True) -> PreparedMesh:
+# ---------------------------------------------------------------------------------------------
+# Plane symmetry (BeatEngineCore.jl:317-439)
+# ---------------------------------------------------------------------------------------------
+
+SYMMETRY_SIGNS = {
+    "off": ((1, 1, 1),),
+    "x": ((1, 1, 1), (-1, 1, 1)),
+    "xy": ((1, 1, 1), (-1, 1, 1), (1, -1, 1), (-1, -1, 1)),
+}
+
+
+def normalized_symmetry_mode(mode) -> str:
+    m = str(mode).strip().lower()
+    if m not in SYMMETRY_SIGNS:
+        raise ValueError(f"Unsupported symmetry mode: {mode}. Expected off, x, or xy.")
+    return m
+
+
+def symmetry_reduction_factor(mode) -> int:
+    return len(SYMMETRY_SIGNS[normalized_symmetry_mode(mode)])
+
+
+def _active_axes(mode):
+    return {"off": (), "x": (0,), "xy": (0, 1)}[normalized_symmetry_mode(mode)]
+
+
+def symmetry_plane_tolerance(vertices) -> float:
+    """max(1e-9, 1e-6 x bounding-box diagonal) (BeatEngineCore.jl:358-375)."""
+    v = np.asarray(vertices, dtype=np.float64)
+    scale = float(np.linalg.norm(v.max(0) - v.min(0))) if len(v) else 0.0
+    return max(1e-9, scale * 1e-6)
+
+
+def snap_symmetry_planes(mesh: BoundaryMesh, mode) -> BoundaryMesh:
+    """Coordinates within the plane tolerance of an active symmetry plane become exactly zero
+    (BeatEngineCore.jl:377-403)."""
+    axes = _active_axes(mode)
+    if not axes:
+        return mesh
+    tol = symmetry_plane_tolerance(mesh.vertices)
+    v = np.array(mesh.vertices, dtype=np.float64)
+    for a in axes:
+        v[np.abs(v[:, a]) <= tol, a] = 0.0
+    if np.array_equal(v, mesh.vertices):
+        return mesh
+    return BoundaryMesh(v, mesh.faces, mesh.tags, mesh.mesh_ids)
+
+
+def validate_symmetry_domain(mesh: BoundaryMesh, mode) -> None:
+    """The mesh must lie in the positive half-space of every active plane (BeatEngineCore.jl:405-423)."""
+    tol = symmetry_plane_tolerance(mesh.vertices)
+    for a in _active_axes(mode):
+        bad = np.nonzero(mesh.vertices[:, a] < -tol)[0]
+        if bad.size:
+            name = "XYZ"[a]
+            raise ValueError(
+                f"Mesh is not in the positive {name} fundamental domain for {normalized_symmetry_mode(mode).upper()} "
+                f"symmetry. Vertex {bad[0] + 1} has {name.lower()}={mesh.vertices[bad[0], a]} m.")
+
+
+def p1_orbit_weights(mesh: BoundaryMesh, mode) -> np.ndarray:
+    """Row weight 2 per active plane a vertex lies on (BeatEngineCore.jl:425-439)."""
+    tol = symmetry_plane_tolerance(mesh.vertices)
+    w = np.ones(mesh.n_vertices)
+    for a in _active_axes(mode):
+        w[np.abs(mesh.vertices[:, a]) <= tol] *= 2.0
+    return w
+
+
+def plane_masks(pairs: SingularPairs, n_faces: int, n_transforms: int) -> np.ndarray:
+    """Bit r of face f is set when f takes part in an image pair of transform r (as test or trial)."""
+    mask = np.zeros(n_faces, dtype=np.int32)
+    r = pairs.trial // n_faces
+    img = r > 0
+    np.bitwise_or.at(mask, pairs.test[img], (1 << r[img]).astype(np.int32))
+    np.bitwise_or.at(mask, pairs.trial[img] % n_faces, (1 << r[img]).astype(np.int32))
+    return mask
+
+
+def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True, symmetry: str = "off"
+            ) -> PreparedMesh:
+    """Device-ready tables. With ``symmetry`` x / xy the mesh is the fundamental domain (already snapped and
+    validated, as the request driver does); image pairs join the singular pairs with trial key s + r F."""
+    symmetry = normalized_symmetry_mode(symmetry)
     if reorder:
         fperm, vperm = locality_order(mesh)
     else:
@@ -501,5 +660,8 @@ def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True) -
     curls = surface_curls(work.face_vertices, work.normals)
     qp = quadrature_points(work.face_vertices)
     m11, m10 = mass_blocks()
-    return PreparedMesh(work, curls, qp, singular_pairs(work, singular_order), slot_tables(work), m11, m10, fperm,
-                        vperm, mesh)
+    signs = SYMMETRY_SIGNS[symmetry]
+    pairs = singular_pairs(work, singular_order, transforms=signs if len(signs) > 1 else None)
+    return PreparedMesh(work, curls, qp, pairs, slot_tables(work), m11, m10, fperm, vperm, mesh, symmetry,
+                        np.asarray(signs, dtype=np.int32), p1_orbit_weights(work, symmetry),
+                        plane_masks(pairs, work.n_faces, len(signs)))

@@ -94,7 +94,7 @@ class Scheduler:
         h.events.put({"type": "initialized", **init})
         try:
             job = D.build_job(request, parsed)
-            prep = G.prepare(job.mesh)
+            prep = G.prepare(job.mesh, symmetry=job.symmetry)
         except Exception as exc:
             h.events.put({"type": "failed", "error": f"{type(exc).__name__}: {exc}"})
             h.t_done = time.perf_counter()

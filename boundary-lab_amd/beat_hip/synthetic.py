@@ -159,6 +159,56 @@ def icosphere(subdivisions: int = 3, radius: float = 0.1, tag: int = TAG_THROAT)
 # ----------------------------------------------------------------------------------------------
 
 
+def uv_sphere(n_lat: int = 12, n_lon: int = 24, radius: float = 0.1, tag: int = TAG_THROAT):
+    """Latitude/longitude sphere with outward normals, mirror-symmetric in x and y. ``n_lon`` is a multiple of 4,
+    so the planes x = 0 and y = 0 are unions of mesh edges (meridians); coordinates on them are exactly zero."""
+    if n_lon % 4 or n_lat < 2:
+        raise ValueError("n_lon must be a multiple of 4 and n_lat >= 2")
+    V = [(0.0, 0.0, radius)]
+    for i in range(1, n_lat):
+        th = math.pi * i / n_lat
+        for j in range(n_lon):
+            q = j % (n_lon // 4)  # exact zeros on the x = 0 / y = 0 meridians
+            ph = 2.0 * math.pi * j / n_lon
+            c = 0.0 if (q == 0 and (j // (n_lon // 4)) % 2 == 1) else math.cos(ph)
+            sn = 0.0 if (q == 0 and (j // (n_lon // 4)) % 2 == 0) else math.sin(ph)
+            V.append((radius * math.sin(th) * c, radius * math.sin(th) * sn, radius * math.cos(th)))
+    V.append((0.0, 0.0, -radius))
+    south = len(V) - 1
+
+    def ring(i, j):
+        return 1 + (i - 1) * n_lon + (j % n_lon)
+
+    Fl = []
+    for j in range(n_lon):
+        Fl.append((0, ring(1, j), ring(1, j + 1)))
+    for i in range(1, n_lat - 1):
+        for j in range(n_lon):
+            a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j), ring(i + 1, j + 1)
+            # quad diagonals alternate by quadrant so that x -> -x and y -> -y map the mesh onto itself
+            if (j // (n_lon // 4)) % 2 == 0:
+                Fl += [(a, c, d), (a, d, b)]
+            else:
+                Fl += [(a, c, b), (b, c, d)]
+    for j in range(n_lon):
+        Fl.append((south, ring(n_lat - 1, j + 1), ring(n_lat - 1, j)))
+    Vn, Fn = np.asarray(V, float), np.asarray(Fl, np.int64)
+    return Vn, Fn, np.full(len(Fn), tag, dtype=np.int64)
+
+
+def symmetric_part(V, F, T, mode: str):
+    """Faces of a plane-symmetric mesh inside the fundamental domain (x >= 0 for "x"; x, y >= 0 for "xy"),
+    with unused vertices dropped (vertex order kept)."""
+    axes = {"x": (0,), "xy": (0, 1)}[mode]
+    c = V[F].mean(axis=1)
+    keep = np.all(c[:, axes] > 0.0, axis=1)
+    Fk = F[keep]
+    used = np.unique(Fk)
+    remap = np.full(len(V), -1, dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    return V[used], remap[Fk], T[keep]
+
+
 def _box_with_holes(width, height, depth, bands, h, horn=None, seed=None, jitter=0.0):
     """Closed box surface; the front face (z=0, normal +z) is split into horizontal bands, each with one
     circular hole filled by either a horn (dict) or a flat piston disk.

@@ -45,7 +45,7 @@ struct RuleWeights {
 };
 __constant__ RuleWeights c_phiw = RuleWeights();
 
-// 3x3 LHS block of a regular (t,s) pair, scaled and combined: C = J/(4pi) (-D + (i/k) H).
+// 3x3 LHS block of a regular (t,s) pair, scaled and combined: C += J/(4pi) (-D + (i/k) H).
 // UNROLL_A = 6 fully unrolls the test-point loop (FP32: test points live in SGPRs); UNROLL_A = 1 keeps it
 // rolled with wave-uniform scalar loads of the test point (FP64: halves register pressure, no spills).
 template <typename T, int UNROLL_A>
@@ -116,8 +116,8 @@ __device__ __forceinline__ void regular_block(const T* __restrict__ xq, const T 
                    ct[3 * l + 2] * se.curl[3 * m + 2];
       const T Hr = cp * gsr - k2nn * GMr[3 * l + m];
       const T Hi = cp * gsi - k2nn * GMi[3 * l + m];
-      // -D + (i/k) H
-      C[3 * l + m] = mk<T>(scale * (-DMr[3 * l + m] - Hi * kinv), scale * (-DMi[3 * l + m] + Hr * kinv));
+      // -D + (i/k) H, accumulated (image transforms add into the same block)
+      C[3 * l + m] = cadd(C[3 * l + m], mk<T>(scale * (-DMr[3 * l + m] - Hi * kinv), scale * (-DMi[3 * l + m] + Hr * kinv)));
     }
   }
 }
@@ -222,15 +222,15 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
     for (int i = 0; i < 9; ++i) {
       const float Hr = cp[i] * gsr - k2nn * GMr[j][i];
       const float Hi = cp[i] * gsi - k2nn * GMi[j][i];
-      // -D + (i/k) H
-      C[j][i] = mk<float>(scale * (-DMr[j][i] - Hi * kinv), scale * (-DMi[j][i] + Hr * kinv));
+      // -D + (i/k) H, accumulated (image transforms add into the same block)
+      C[j][i] = cadd(C[j][i], mk<float>(scale * (-DMr[j][i] - Hi * kinv), scale * (-DMi[j][i] + Hr * kinv)));
     }
   }
 }
 
 // One wave per (test element t, group of NFW wavenumbers); FP32 shares the pair geometry across NFW = 2
 // wavenumbers, FP64 runs one wavenumber per wave (NFW = 1).
-template <typename T, int NFW>
+template <typename T, int NFW, bool SYM>
 __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
   const int t = blockIdx.x;
   const int f0 = blockIdx.y * NFW;
@@ -254,6 +254,7 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
   const T a2t = te->a2;
   const int4 tv = p.fv[t];
   const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
+  const int mt = p.pmask ? p.pmask[t] : 0;
 
   for (int c = 0; c < p.n_chunks; ++c) {
     const int s = p.order[c * 64 + lane];
@@ -266,23 +267,35 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
       const int4 sv = p.fv[s];
       const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
                        sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
-      if (!adj) {
-        if constexpr (sizeof(T) == 4) {
-          regular_block_f32<NFW>(te->q, nt, ct, a2t, p.rec[s], k, kappa, C);
-        } else {
-          static_assert(NFW == 1, "FP64 runs one wavenumber per wave");
-          regular_block<T, 1>(te->q, nt, ct, a2t, p.rec[s], k[0], kappa[0], C[0]);
-        }
-      } else {
-        for (int q = sp0; q < sp1; ++q) {
-          if (p.sing_trial[q] == s) {
-#pragma unroll
-            for (int j = 0; j < NFW; ++j) {
-              const cplx<double>* sl = p.sing_lhs + ((size_t)(f0 + min(j, nfw - 1)) * p.P + q) * 9;
-#pragma unroll
-              for (int i = 0; i < 9; ++i) C[j][i] = mk<T>(T(sl[i].re), T(sl[i].im));
+      const int ms = p.pmask ? (mt & p.pmask[s]) : 0;
+      // the trial element and its images under the symmetry transforms (r = 0: itself); a pair listed among
+      // the singular pairs (topologically adjacent, or an image touching t) takes its FP64 Duffy block.
+      // Without symmetry (SYM = false) the loop has one trip and the block is assigned, not accumulated.
+      const int R = SYM ? p.R : 1;
+      for (int r = 0; r < R; ++r) {
+        int qs = -1;
+        if (r == 0 ? adj : ((ms >> r) & 1)) {
+          const int key = s + r * p.F;
+          for (int q = sp0; q < sp1; ++q)
+            if (p.sing_trial[q] == key) {
+              qs = q;
+              break;
             }
-            break;
+        }
+        if (qs >= 0) {
+#pragma unroll
+          for (int j = 0; j < NFW; ++j) {
+            const cplx<double>* sl = p.sing_lhs + ((size_t)(f0 + min(j, nfw - 1)) * p.P + qs) * 9;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) C[j][i] = cadd(C[j][i], mk<T>(T(sl[i].re), T(sl[i].im)));
+          }
+        } else if (r != 0 || !adj) {
+          const ElemRec<T>& se = p.rec[(size_t)r * p.F + s];
+          if constexpr (sizeof(T) == 4) {
+            regular_block_f32<NFW>(te->q, nt, ct, a2t, se, k, kappa, C);
+          } else {
+            static_assert(NFW == 1, "FP64 runs one wavenumber per wave");
+            regular_block<T, 1>(te->q, nt, ct, a2t, se, k[0], kappa[0], C[0]);
           }
         }
       }
@@ -335,7 +348,11 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
   if (item >= (long long)p.P * nf) return;
   const int pi = p.by_rule[item / nf];
   const int f = (int)(item % nf);
-  const int t = p.test[pi], s = p.trial[pi], ru = p.rule[pi];
+  const int t = p.test[pi], key = p.trial[pi], ru = p.rule[pi];
+  // trial key = s + r F: element s reflected by transform r (r = 0: itself)
+  const int s = key % p.F, tr = key / p.F;
+  const double sg[3] = {double(p.sgn[3 * tr]), double(p.sgn[3 * tr + 1]), double(p.sgn[3 * tr + 2])};
+  const double det = sg[0] * sg[1] * sg[2];
   const double k = p.k[f];
   double t0[3], te1[3], te2[3], s0[3], se1[3], se2[3], nt[3], ns[3];
 #pragma unroll
@@ -343,11 +360,11 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
     t0[c] = p.fvtx[9 * t + c];
     te1[c] = p.fvtx[9 * t + 3 + c] - t0[c];
     te2[c] = p.fvtx[9 * t + 6 + c] - t0[c];
-    s0[c] = p.fvtx[9 * s + c];
-    se1[c] = p.fvtx[9 * s + 3 + c] - s0[c];
-    se2[c] = p.fvtx[9 * s + 6 + c] - s0[c];
+    s0[c] = sg[c] * p.fvtx[9 * s + c];
+    se1[c] = sg[c] * p.fvtx[9 * s + 3 + c] - s0[c];
+    se2[c] = sg[c] * p.fvtx[9 * s + 6 + c] - s0[c];
     nt[c] = p.fnrm[3 * t + c];
-    ns[c] = p.fnrm[3 * s + c];
+    ns[c] = sg[c] * p.fnrm[3 * s + c];
   }
   double Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
   double Dr[9], Di[9], Gr[9], Gi[9];
@@ -400,13 +417,15 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
   const double* ctp = p.fcurl + 9 * t;
   const double* csp = p.fcurl + 9 * s;
   cplx<double>* out = p.lhs + ((size_t)f * p.P + pi) * 9;
-  const bool self = (t == s);
+  const bool self = (t == key);  // the coincident pair of an element with itself (not with an image)
   const double a2 = p.fa2[t];
+  const double cs0 = det * sg[0], cs1 = det * sg[1], cs2 = det * sg[2];  // reflected curls: det * signs
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      const double cp = ctp[3 * l] * csp[3 * m] + ctp[3 * l + 1] * csp[3 * m + 1] + ctp[3 * l + 2] * csp[3 * m + 2];
+      const double cp = ctp[3 * l] * cs0 * csp[3 * m] + ctp[3 * l + 1] * cs1 * csp[3 * m + 1] +
+                        ctp[3 * l + 2] * cs2 * csp[3 * m + 2];
       const double Hr = cp * gsr - k * k * nn * Gr[3 * l + m];
       const double Hi = cp * gsi - k * k * nn * Gi[3 * l + m];
       double re = scale * (-Dr[3 * l + m] - Hi * kinv);
@@ -431,8 +450,9 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
 // ------------------------------------------------------------------------------------------------
 
 
-template <typename T>
+template <typename T, bool SYM>
 __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
+  constexpr int RHS_UNROLL_A = sizeof(T) == 4 ? 6 : 1;  // FP64: rolled test-point loop, no spills
   const int t = blockIdx.x;
   const int f = blockIdx.y;
   const int lane = threadIdx.x;
@@ -448,17 +468,31 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
 #pragma unroll
     for (int c = 0; c < MAX_CH; ++c) accr[l][c] = acci[l][c] = 0.0;
 
+  const int mt = p.pmask ? p.pmask[t] : 0;
   for (int j = lane; j < p.n_src; j += 64) {
     const int s = p.src[j];
     const int4 sv = p.fv[s];
     const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
                      sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
+    const int ms = p.pmask ? (mt & p.pmask[s]) : 0;
     double br[3] = {0, 0, 0}, bi[3] = {0, 0, 0};
-    if (!adj) {
-      const ElemRec<T>& se = p.rec[s];
+    // the radiating face and its images (r = 0: itself); singular pairs take their FP64 Duffy block
+    const int R = SYM ? p.R : 1;
+    for (int r = 0; r < R; ++r) {
+    int qs = -1;
+    if (r == 0 ? adj : ((ms >> r) & 1)) {
+      const int key = s + r * p.F;
+      for (int q = sp0; q < sp1; ++q)
+        if (p.sing_trial[q] == key) {
+          qs = q;
+          break;
+        }
+    }
+    if (qs < 0 && (r != 0 || !adj)) {
+      const ElemRec<T>& se = p.rec[(size_t)r * p.F + s];
       T Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
       const T ntx = te->n[0], nty = te->n[1], ntz = te->n[2];
-#pragma unroll
+#pragma unroll RHS_UNROLL_A
       for (int a = 0; a < 6; ++a) {
         T sgr = 0, sgi = 0, sar = 0, sai = 0;
 #pragma unroll
@@ -493,20 +527,16 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
       const double kinv = 1.0 / kd;
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        br[l] = scale * (-double(Sr[l]) + double(Ai[l]) * kinv);
-        bi[l] = scale * (-double(Si[l]) - double(Ar[l]) * kinv);
+        br[l] += scale * (-double(Sr[l]) + double(Ai[l]) * kinv);
+        bi[l] += scale * (-double(Si[l]) - double(Ar[l]) * kinv);
       }
-    } else {
+    } else if (qs >= 0) {
       const cplx<double>* sr = p.sing_rhs + (size_t)f * p.P * 3;
-      for (int q = sp0; q < sp1; ++q) {
-        if (p.sing_trial[q] == s) {
-          for (int l = 0; l < 3; ++l) {
-            br[l] = sr[(size_t)q * 3 + l].re;
-            bi[l] = sr[(size_t)q * 3 + l].im;
-          }
-          break;
-        }
+      for (int l = 0; l < 3; ++l) {
+        br[l] += sr[(size_t)qs * 3 + l].re;
+        bi[l] += sr[(size_t)qs * 3 + l].im;
       }
+    }
     }
 #pragma unroll
     for (int c = 0; c < MAX_CH; ++c) {
@@ -554,6 +584,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
   const cplx<T>* strip = g.strip + (size_t)f * g.strip_fstride;
   cplx<T>* Ar = g.A + (size_t)f * g.strideA + (size_t)i * g.lda;
   cplx<T>* A0r = g.A0 ? g.A0 + (size_t)f * g.strideA + (size_t)i * g.lda : nullptr;
+  const double wi = g.roww ? g.roww[i] : 1.0;  // P1 orbit row weight (plane symmetry)
   double rmax = 0.0;
   for (int j = tid; j < g.N; j += 256) {
     cplx<T> acc = mk<T>(0, 0);
@@ -562,6 +593,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
       const int slot = g.cs_slot[e];
       for (int r = 0; r < ns; ++r) acc = cadd(acc, strip[(size_t)rows[r] * g.L + slot]);
     }
+    if (g.roww) acc = mk<T>(acc.re * T(wi), acc.im * T(wi));
     Ar[j] = acc;
     if (A0r) A0r[j] = acc;
     rmax = fmax(rmax, (double)(fabs(acc.re) + fabs(acc.im)));
@@ -577,6 +609,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
     const cplx<double>* rp = g.rhs_part + (size_t)f * 3 * g.F * MAX_CH;
     cplx<double> acc = mk<double>(0, 0);
     for (int r = 0; r < ns; ++r) acc = cadd(acc, rp[(size_t)rows[r] * MAX_CH + tid]);
+    acc = mk<double>(acc.re * wi, acc.im * wi);
     g.b[(size_t)f * g.strideB + (size_t)tid * g.ldb + i] = cmul(acc, g.rhs_scale[f]);
   }
 }
@@ -584,7 +617,10 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
 template <typename T>
 hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
   constexpr int NFW = sizeof(T) == 4 ? 2 : 1;
-  hipLaunchKernelGGL((k_regular_lhs<T, NFW>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
+  if (p.R > 1)
+    hipLaunchKernelGGL((k_regular_lhs<T, NFW, true>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
+  else
+    hipLaunchKernelGGL((k_regular_lhs<T, NFW, false>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
   return hipGetLastError();
 }
 hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
@@ -595,7 +631,10 @@ hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
 }
 template <typename T>
 hipError_t launch_rhs(const RhsParams<T>& p, int nf, hipStream_t st) {
-  hipLaunchKernelGGL(k_rhs_part<T>, dim3(p.F, nf), dim3(64), 0, st, p);
+  if (p.R > 1)
+    hipLaunchKernelGGL((k_rhs_part<T, true>), dim3(p.F, nf), dim3(64), 0, st, p);
+  else
+    hipLaunchKernelGGL((k_rhs_part<T, false>), dim3(p.F, nf), dim3(64), 0, st, p);
   return hipGetLastError();
 }
 template <typename T>

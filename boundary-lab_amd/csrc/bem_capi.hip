@@ -102,6 +102,8 @@ struct DevBuf {
 struct bem_mesh {
   bem_ctx* ctx;
   int F, V, n_chunks, L, P, n_rules, n_src, M;
+  int R;        // symmetry transforms (identity first): element records and field sources exist per transform
+  int sgn[12];  // their sign vectors
   double m11[9], m10[3];
   std::vector<DevBuf> owned;
   // device arrays
@@ -116,6 +118,8 @@ struct bem_mesh {
   uint16_t *slot_start, *slot_inc;
   double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
   int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces, *sing_by_rule;
+  double* roww;  // [V] orbit row weights or null
+  int* pmask;    // [F] image-pair participation bits or null
   // workspace
   DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial;
   int strip_nf = 0;
@@ -265,6 +269,13 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   if (!c || !d || !out) return fail(BEM_E_ARG, "NULL argument");
   if (d->n_faces <= 0 || d->n_vertices <= 0) return fail(BEM_E_ARG, "empty mesh");
   if (d->n_chunks != (d->n_faces + 63) / 64) return fail(BEM_E_ARG, "n_chunks must be ceil(F/64)");
+  const int R = d->n_transforms <= 0 ? 1 : d->n_transforms;
+  if (R != 1 && R != 2 && R != 4) return fail(BEM_E_ARG, "n_transforms must be 1, 2 or 4");
+  for (int i = 0; i < 3 * R; ++i)
+    if (d->n_transforms > 0 && d->transform_signs[i] != 1 && d->transform_signs[i] != -1)
+      return fail(BEM_E_ARG, "transform signs must be +1 or -1");
+  for (int i = 0; i < d->n_pairs; ++i)
+    if (d->sing_trial[i] < 0 || d->sing_trial[i] >= R * d->n_faces) return fail(BEM_E_ARG, "singular trial key out of range");
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   bem_mesh* m = new bem_mesh();
   m->ctx = c;
@@ -276,14 +287,17 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   m->P = d->n_pairs;
   m->n_rules = d->n_rules;
   m->n_src = d->n_src;
-  m->M = 6 * F;
+  m->R = R;
+  m->M = 6 * F * R;
+  for (int i = 0; i < 12; ++i) m->sgn[i] = i < 3 * R && d->n_transforms > 0 ? d->transform_signs[i] : 1;
   std::memcpy(m->m11, d->m11, sizeof(m->m11));
   std::memcpy(m->m10, d->m10, sizeof(m->m10));
-  // host-side records
-  std::vector<ElemRec<float>> r32(F);
-  std::vector<ElemRec<double>> r64(F);
-  std::vector<SrcRec<float>> s32(6 * (size_t)F);
-  std::vector<SrcRec<double>> s64(6 * (size_t)F);
+  // host-side records: transform r of element t at r*F + t (reflected rule points and normal, curls times
+  // det * signs: BeatEngineCore.jl:341-356); field sources transform-major, 6 per element
+  std::vector<ElemRec<float>> r32((size_t)F * R);
+  std::vector<ElemRec<double>> r64((size_t)F * R);
+  std::vector<SrcRec<float>> s32(6 * (size_t)F * R);
+  std::vector<SrcRec<double>> s64(6 * (size_t)F * R);
   std::vector<float> w32(6 * (size_t)F);
   std::vector<double> w64(6 * (size_t)F);
   std::vector<int4> fv(F);
@@ -317,6 +331,34 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
       w32[mm] = (float)w64[mm];
     }
   }
+  for (int r = 1; r < R; ++r) {
+    const int* sg = m->sgn + 3 * r;
+    const double det = double(sg[0] * sg[1] * sg[2]);
+    for (int t = 0; t < F; ++t) {
+      ElemRec<double>& e = r64[(size_t)r * F + t];
+      e = r64[t];
+      for (int q = 0; q < 6; ++q)
+        for (int i = 0; i < 3; ++i) e.q[3 * q + i] *= sg[i];
+      for (int i = 0; i < 3; ++i) e.n[i] *= sg[i];
+      for (int l = 0; l < 3; ++l)
+        for (int i = 0; i < 3; ++i) e.curl[3 * l + i] *= det * sg[i];
+      ElemRec<float>& ef = r32[(size_t)r * F + t];
+      for (int i = 0; i < 18; ++i) ef.q[i] = (float)e.q[i];
+      for (int i = 0; i < 3; ++i) ef.n[i] = (float)e.n[i];
+      for (int i = 0; i < 9; ++i) ef.curl[i] = (float)e.curl[i];
+      ef.a2 = (float)e.a2;
+      ef.pad = 0;
+      for (int q = 0; q < 6; ++q) {
+        const size_t mm = (size_t)r * 6 * F + 6 * (size_t)t + q;
+        for (int i = 0; i < 3; ++i) {
+          s64[mm].y[i] = e.q[3 * q + i];
+          s64[mm].n[i] = e.n[i];
+          s32[mm].y[i] = (float)s64[mm].y[i];
+          s32[mm].n[i] = (float)s64[mm].n[i];
+        }
+      }
+    }
+  }
   int rc = BEM_OK;
   size_t total_rule_pts = 0;
   for (int r = 0; r < d->n_rules; ++r) total_rule_pts = std::max(total_rule_pts, (size_t)(d->rule_off[r] + d->rule_len[r]));
@@ -327,8 +369,8 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
     bem_mesh_destroy(m);                     \
     return rc;                               \
   }
-  UP(m->rec32, r32.data(), (size_t)F);
-  UP(m->rec64, r64.data(), (size_t)F);
+  UP(m->rec32, r32.data(), r32.size());
+  UP(m->rec64, r64.data(), r64.size());
   UP(m->src32, s32.data(), s32.size());
   UP(m->src64, s64.data(), s64.size());
   UP(m->srcw32, w32.data(), w32.size());
@@ -363,6 +405,10 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   UP(m->rule_len, d->rule_len, (size_t)d->n_rules);
   UP(m->rpts, d->rule_pts, total_rule_pts * 5);
   UP(m->src_faces, d->src_faces, (size_t)d->n_src);
+  m->roww = nullptr;
+  m->pmask = nullptr;
+  if (R > 1 && d->row_weights) UP(m->roww, d->row_weights, (size_t)V);
+  if (R > 1 && d->plane_mask) UP(m->pmask, d->plane_mask, (size_t)F);
 #undef UP
   for (int v = 0; v < V; ++v)
     if (d->vstar_ptr[v + 1] - d->vstar_ptr[v] > 32) {
@@ -400,7 +446,7 @@ static int reserve_impl(bem_mesh* m, int precision, int nf, int strip_nf, int n_
   if ((rc = dev_alloc(m, m->sing_rhs, (size_t)nf * std::max(1, m->P) * 3 * 16))) return rc;
   if ((rc = dev_alloc(m, m->rhs_part, (size_t)nf * 3 * m->F * BEM_MAX_CHANNELS * 16))) return rc;
   if (n_obs > 0) {
-    if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->M * 2 * cb))) return rc;
+    if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * 6 * m->F * 2 * cb))) return rc;
     const int ns = field_slices(m, n_obs, nf);
     if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
   }
@@ -450,6 +496,8 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   sp.P = m->P;
   std::memcpy(sp.m11, m->m11, sizeof(sp.m11));
   std::memcpy(sp.m10, m->m10, sizeof(sp.m10));
+  sp.F = m->F;
+  std::memcpy(sp.sgn, m->sgn, sizeof(sp.sgn));
   {
     ScopedTimer tm(c, BEM_TIMER_SINGULAR, (long long)nf);
     HIPCHK(launch_singular(sp, nf, st), "k_singular launch");
@@ -469,6 +517,8 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   rp.P = m->P;
   rp.n_src = m->n_src;
   rp.n_ch = n_ch;
+  rp.R = m->R;
+  rp.pmask = m->pmask;
   {
     ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
     HIPCHK(launch_rhs<T>(rp, nf, st), "k_rhs_part launch");
@@ -495,6 +545,8 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.F = m->F;
     p.n_chunks = m->n_chunks;
     p.P = m->P;
+    p.R = m->R;
+    p.pmask = m->pmask;
     {
       ScopedTimer tm(c, BEM_TIMER_REGULAR, (long long)g);
       HIPCHK(launch_regular<T>(p, g, st), "k_regular_lhs launch");
@@ -520,6 +572,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     gp.N = m->V;
     gp.F = m->F;
     gp.n_ch = n_ch;
+    gp.roww = m->roww;
     {
       ScopedTimer tm(c, BEM_TIMER_GATHER, (long long)g);
       HIPCHK(launch_gather<T>(gp, g, st), "k_gather launch");
@@ -679,16 +732,16 @@ int bem_field(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, i
   const int ns = field_slices(m, n_obs, nf);
   const size_t cb = precision == BEM_PREC_MIXED ? 8 : 16;
   int rc;
-  if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->M * 2 * cb))) return rc;
+  if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * 6 * m->F * 2 * cb))) return rc;
   if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
   hipError_t e;
   ScopedTimer tm(c, BEM_TIMER_FIELD, (long long)nf);
   if (precision == BEM_PREC_MIXED) {
-    e = field_batch<float>(m->src32, m->fv, m->srcw32, obs, m->M, n_obs, k, (const cplx<double>*)p, ldp,
+    e = field_batch<float>(m->src32, m->fv, m->srcw32, obs, m->M, 6 * m->F, n_obs, k, (const cplx<double>*)p, ldp,
                            strideP, (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<float>*)m->wsrc.p,
                            (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, c->stream);
   } else {
-    e = field_batch<double>(m->src64, m->fv, m->srcw64, obs, m->M, n_obs, k, (const cplx<double>*)p, ldp, strideP,
+    e = field_batch<double>(m->src64, m->fv, m->srcw64, obs, m->M, 6 * m->F, n_obs, k, (const cplx<double>*)p, ldp, strideP,
                             (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<double>*)m->wsrc.p,
                             (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, c->stream);
   }

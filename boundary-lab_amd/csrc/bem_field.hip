@@ -4,6 +4,8 @@
 // thread-per-point field kernels) and BeatEngineCpuField.jl:1-45:
 //   u(x) = sum_m [ dG/dn_y(x, y_m) p_w(m) - G(x, y_m) q_w(m) ],
 //   p_w(m) = w_m sum_l phi_l(q) p[face_l], q_w(m) = w_m q[face], y_m the order-4 rule points.
+// With x / xy plane symmetry the sources of every image transform follow those of the fundamental domain
+// (transform-major) and share its weighted values (BeatEngineCore.jl:1245-1280).
 // Design: the transfer matrix is generated on the fly (VALU bound, never stored). A block holds 64 observation
 // points (one per lane) and its four waves sweep disjoint slices of sources whose coordinates and weights are
 // wave-uniform (scalar loads). Slices are reduced in a fixed order by k_field_reduce: deterministic.
@@ -23,6 +25,7 @@ struct FieldParams {
   const double* k;            // [nf]
   cplx<double>* partial;      // [nf][n_slices][n_ch][n_obs]
   int M, n_obs, n_ch, n_slices, slice_len;
+  int M0;  // weighted sources per transform: source m uses the weights of source m % M0 (its original)
 };
 
 // weighted sources: p_w and q_w per (f, c, m); p is [nf][n_ch][ldp] FP64 boundary pressure, q likewise per face
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(256) void k_field(FieldParams<T> p) {
 #pragma unroll
     for (int c = 0; c < MAX_CH; ++c) {
       if (c < p.n_ch) {
-        const cplx<T>* ws = p.wsrc + (((size_t)f * p.n_ch + c) * p.M + m) * 2;
+        const cplx<T>* ws = p.wsrc + (((size_t)f * p.n_ch + c) * p.M0 + (m % p.M0)) * 2;
         const cplx<T> pw = ws[0], qw = ws[1];
         ur[c] += dr * pw.re - di * pw.im - (gr * qw.re - gi * qw.im);
         ui[c] += dr * pw.im + di * pw.re - (gr * qw.im + gi * qw.re);
@@ -126,13 +129,14 @@ __global__ void k_field_reduce(const cplx<double>* partial, cplx<double>* out, l
 }
 
 template <typename T>
-hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, const double* obs, int M, int n_obs,
-                       const double* k, const cplx<double>* p, long long ldp, long long strideP,
+hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, const double* obs, int M, int M0,
+                       int n_obs, const double* k, const cplx<double>* p, long long ldp, long long strideP,
                        const cplx<double>* q, long long ldq, long long strideQ, int n_ch, int nf, cplx<T>* wsrc,
                        cplx<double>* partial, int n_slices, cplx<double>* out, long long ldo, long long strideO,
                        hipStream_t st) {
-  hipLaunchKernelGGL(k_field_sources<T>, dim3((M + 255) / 256, n_ch, nf), dim3(256), 0, st, fv, src_w, p, ldp,
-                     strideP, q, ldq, strideQ, wsrc, M, n_ch);
+  // weighted sources of the fundamental domain (M0 = 6F); image sources (M = R * M0) reuse them
+  hipLaunchKernelGGL(k_field_sources<T>, dim3((M0 + 255) / 256, n_ch, nf), dim3(256), 0, st, fv, src_w, p, ldp,
+                     strideP, q, ldq, strideQ, wsrc, M0, n_ch);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   FieldParams<T> fp;
@@ -144,6 +148,7 @@ hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, con
   fp.k = k;
   fp.partial = partial;
   fp.M = M;
+  fp.M0 = M0;
   fp.n_obs = n_obs;
   fp.n_ch = n_ch;
   fp.n_slices = n_slices;
@@ -156,11 +161,11 @@ hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, con
   return hipGetLastError();
 }
 
-template hipError_t field_batch<float>(const SrcRec<float>*, const int4*, const float*, const double*, int, int,
+template hipError_t field_batch<float>(const SrcRec<float>*, const int4*, const float*, const double*, int, int, int,
                                        const double*, const cplx<double>*, long long, long long,
                                        const cplx<double>*, long long, long long, int, int, cplx<float>*,
                                        cplx<double>*, int, cplx<double>*, long long, long long, hipStream_t);
-template hipError_t field_batch<double>(const SrcRec<double>*, const int4*, const double*, const double*, int, int,
+template hipError_t field_batch<double>(const SrcRec<double>*, const int4*, const double*, const double*, int, int, int,
                                         const double*, const cplx<double>*, long long, long long,
                                         const cplx<double>*, long long, long long, int, int, cplx<double>*,
                                         cplx<double>*, int, cplx<double>*, long long, long long, hipStream_t);

@@ -41,6 +41,8 @@ struct RegParams {
   cplx<T>* strip;               // [nf][3F][L]
   long long strip_fstride;
   int L, F, n_chunks, P;
+  int R;                        // transforms (identity + images); rec holds R*F records, image r at r*F
+  const int* pmask;             // [F] bit r: face in an image pair of transform r (null: no image pairs)
 };
 
 struct SingParams {
@@ -61,6 +63,8 @@ struct SingParams {
   int P;
   double m11[9];        // reference-element mass block / (2A)
   double m10[3];
+  int F;                // trial key = s + r*F: element s reflected by transform r
+  int sgn[12];          // transform sign vectors
 };
 
 template <typename T>
@@ -75,6 +79,8 @@ struct RhsParams {
   const double* k;
   cplx<double>* out;             // [nf][3F][BEM_MAX_CH]
   int F, P, n_src, n_ch;
+  int R;                         // transforms; rec holds R*F records
+  const int* pmask;              // [F] image-pair participation bits (null: none)
 };
 
 template <typename T>
@@ -95,6 +101,7 @@ struct GatherParams {
   long long ldb, strideB;
   double* rowmax;                 // [nf][N]
   int N, F, n_ch;
+  const double* roww;             // [N] P1 orbit row weights (null: 1)
 };
 
 template <typename T>
@@ -133,7 +140,7 @@ hipError_t growth_batch(const MatBatch<T>&, int batch, double* umax, const doubl
 hipError_t fill_int(int* p, int n, int value, hipStream_t);
 
 template <typename T>
-hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, const double* obs, int M, int n_obs,
+hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, const double* obs, int M, int M0, int n_obs,
                        const double* k, const cplx<double>* p, long long ldp, long long strideP,
                        const cplx<double>* q, long long ldq, long long strideQ, int n_ch, int nf, cplx<T>* wsrc,
                        cplx<double>* partial, int n_slices, cplx<double>* out, long long ldo, long long strideO,

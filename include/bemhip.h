@@ -114,6 +114,16 @@ typedef struct bem_mesh_desc {
   /* radiating faces (faces with a non-zero Neumann drive in any channel) */
   int n_src;
   const int* src_faces;
+  /* x / xy plane symmetry (JS/BeatEngineCore.jl:317-439,917-945; JS/BeatEngineRocmRegular.jl:103-132;
+   * JS/BeatEngineRocmPairKernels.jl:442-466; JS/BeatEngineRocmOperators.jl:11-21). The mesh is the
+   * fundamental domain. n_transforms = 1 (off), 2 (x) or 4 (xy); transform r reflects by
+   * transform_signs[3r .. 3r+2] (+-1; transform 0 is the identity). Every trial element is integrated together
+   * with its images; singular pairs whose trial key sing_trial >= n_faces are image pairs (trial element
+   * key % n_faces reflected by transform key / n_faces) integrated with their Duffy rule. */
+  int n_transforms;
+  int transform_signs[12];
+  const double* row_weights; /* [V] P1 orbit row weights applied to A and b (NULL: 1) */
+  const int* plane_mask;     /* [F] bit r: the face takes part in an image pair of transform r (NULL: none) */
 } bem_mesh_desc;
 
 /* Upload a prepared mesh: SoA geometry, locality/slot tables, singular pairs and Duffy rules, field sources.

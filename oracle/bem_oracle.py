@@ -18,6 +18,8 @@ It restates, in float64 NumPy, the BEAT CPU algorithm of the reference:
 * Burton-Miller system   ``src/blab/solvers/julia_local/src/BeatEngineCpuSolve.jl:45-71``
 * field evaluation       ``BeatEngineCore.jl:1240-1280``, ``BeatEngineCpuField.jl:1-45``
 * request driver         ``src/blab/solvers/julia_local/solver.jl:252-631,680-1020``
+* plane symmetry x / xy  ``BeatEngineCore.jl:317-439,917-945,952-979,1245-1280``,
+                         ``BeatEngineCpuAssembly.jl:61-100,411-445,488-527,571-633,713-886``
 
 Pinning (see tests/test_oracle_*.py): rule sums and monomial exactness, Duffy
 weight sums and tensor-rule agreement on smooth integrands, curl identities,
@@ -461,10 +463,12 @@ def _scatter(S, Ds, D, H, faces, tt, si, Sb, Dsb, Db, Hb):
             np.add.at(H, (r, c), Hb[:, i, j])
 
 
-def singular_pair_blocks(mesh: Mesh, k: float, cache: SingularCache, curls=None):
-    """Per-pair Duffy blocks (S, Ds, D, H) for every vertex-sharing pair (BeatEngineCpuAssembly.jl:151-218)."""
+def singular_pair_blocks(mesh: Mesh, k: float, cache: SingularCache, curls=None, trial_geometry=None):
+    """Per-pair Duffy blocks (S, Ds, D, H) for every vertex-sharing pair (BeatEngineCpuAssembly.jl:151-218).
+    ``trial_geometry`` = (face_vertices, normals, curls) of reflected trial elements for image pairs."""
     if curls is None:
         curls = surface_curls(mesh.face_vertices, mesh.normals)
+    fv_s, n_s, c_s = trial_geometry if trial_geometry is not None else (mesh.face_vertices, mesh.normals, curls)
     P = len(cache.test)
     Sb = np.zeros((P, 3), complex)
     Dsb = np.zeros((P, 3), complex)
@@ -480,7 +484,7 @@ def singular_pair_blocks(mesh: Mesh, k: float, cache: SingularCache, curls=None)
             idx = sel[c0 : c0 + 256]
             t, s = cache.test[idx], cache.trial[idx]
             x = np.einsum("ql,plc->pqc", bt, fv[t])  # (P,n,3)
-            y = np.einsum("ql,plc->pqc", bs, fv[s])
+            y = np.einsum("ql,plc->pqc", bs, fv_s[s])
             jac = (2.0 * mesh.areas[t]) * (2.0 * mesh.areas[s])
             # Duffy points are paired one-to-one: evaluate on the diagonal only.
             rv = y - x
@@ -490,7 +494,7 @@ def singular_pair_blocks(mesh: Mesh, k: float, cache: SingularCache, curls=None)
             inv = 1.0 / rs
             green = np.where(ok, np.exp(1j * k * rs) * (inv / FOUR_PI), 0.0)
             grad = green * (-inv + 1j * k)
-            nt, ns = mesh.normals[t], mesh.normals[s]
+            nt, ns = mesh.normals[t], n_s[s]
             trial_dot = (rv * ns[:, None, :]).sum(axis=2) * inv
             test_dot = -(rv * nt[:, None, :]).sum(axis=2) * inv
             ww = w[None, :] * jac[:, None]
@@ -502,7 +506,7 @@ def singular_pair_blocks(mesh: Mesh, k: float, cache: SingularCache, curls=None)
             GM = np.matmul((gw[:, :, None] * bt[None]).transpose(0, 2, 1), bs)
             Gs = gw.sum(axis=1)
             nn = (nt * ns).sum(axis=1)
-            cp = np.einsum("pic,pjc->pij", curls[t], curls[s])
+            cp = np.einsum("pic,pjc->pij", curls[t], c_s[s])
             Hb[idx] = cp * Gs[:, None, None] - (k * k) * nn[:, None, None] * GM
     return Sb, Dsb, Db, Hb
 
@@ -588,6 +592,210 @@ def evaluate_field(points, mesh: Mesh, pressure, q_neumann, k: float, sources=No
         dl = green * (1j * k - 1.0 / rs) * proj
         out[c0 : c0 + chunk] = dl @ pw - green @ qw
     return out[:, 0] if single else out
+
+
+# --------------------------------------------------------------------------------------
+# Plane symmetry x / xy (BeatEngineCore.jl:317-439,917-945,952-979,1245-1280;
+# BeatEngineCpuAssembly.jl:61-100,411-445,488-527,571-633,713-886; solver.jl:607-608,718-719)
+# --------------------------------------------------------------------------------------
+
+
+def symmetry_transforms(mode: str, include_identity: bool = True):
+    """[(name, signs (3,), determinant)] in the reference order (BeatEngineCore.jl:323-339)."""
+    mode = str(mode).strip().lower()
+    if mode not in ("off", "x", "xy"):
+        raise ValueError(f"Unsupported symmetry mode: {mode}. Expected off, x, or xy.")
+    out = [("identity", np.array([1.0, 1.0, 1.0]), 1.0)] if include_identity else []
+    if mode == "x":
+        out.append(("x", np.array([-1.0, 1.0, 1.0]), -1.0))
+    elif mode == "xy":
+        out += [("x", np.array([-1.0, 1.0, 1.0]), -1.0), ("y", np.array([1.0, -1.0, 1.0]), -1.0),
+                ("xy", np.array([-1.0, -1.0, 1.0]), 1.0)]
+    return out
+
+
+def _active_axes(mode):
+    return {"off": (), "x": (0,), "xy": (0, 1)}[str(mode).strip().lower()]
+
+
+def symmetry_plane_tolerance(vertices, absolute_floor=1e-9, relative_tolerance=1e-6):
+    """BeatEngineCore.jl:358-375: max(floor, |bbox diagonal| * rel)."""
+    v = np.asarray(vertices, float)
+    scale = float(np.linalg.norm(v.max(0) - v.min(0))) if len(v) else 0.0
+    return max(absolute_floor, scale * relative_tolerance)
+
+
+def snap_symmetry_planes(vertices, mode, tol=None):
+    """Coordinates within tol of an active plane become exactly 0 (BeatEngineCore.jl:377-403)."""
+    v = np.array(vertices, float)
+    tol = symmetry_plane_tolerance(v) if tol is None else tol
+    for a in _active_axes(mode):
+        v[np.abs(v[:, a]) <= tol, a] = 0.0
+    return v
+
+
+def validate_fundamental_domain(vertices, mode, tol=None):
+    """Error if a vertex lies on the negative side of an active plane (BeatEngineCore.jl:405-423)."""
+    v = np.asarray(vertices, float)
+    tol = symmetry_plane_tolerance(v) if tol is None else tol
+    for a in _active_axes(mode):
+        bad = np.nonzero(v[:, a] < -tol)[0]
+        if bad.size:
+            name = "XYZ"[a]
+            raise ValueError(f"Mesh is not in the positive {name} fundamental domain for {str(mode).upper()} "
+                             f"symmetry. Vertex {bad[0] + 1} has {name.lower()}={v[bad[0], a]} m.")
+
+
+def p1_orbit_weights(vertices, mode, tol=None):
+    """2 per active plane the vertex lies on (BeatEngineCore.jl:425-439)."""
+    v = np.asarray(vertices, float)
+    tol = symmetry_plane_tolerance(v) if tol is None else tol
+    w = np.ones(len(v))
+    for a in _active_axes(mode):
+        w[np.abs(v[:, a]) <= tol] *= 2.0
+    return w
+
+
+def geometric_adjacency_info(test_vertices, trial_vertices, tol):
+    """Shared vertices by position (BeatEngineCore.jl:481-507), 0-based locals, same ordering rules as
+    ``adjacency_info``."""
+    sa, sb = [], []
+    for i in range(3):
+        for j in range(3):
+            if np.linalg.norm(test_vertices[i] - trial_vertices[j]) <= tol:
+                sa.append(i)
+                sb.append(j)
+    if len(sa) == 3:
+        return "coincident", (sa[0], sa[1], sa[2]), (sb[0], sb[1], sb[2])
+    if len(sa) == 2:
+        if sb[1] < sb[0]:
+            sa = [sa[1], sa[0]]
+            sb = [sb[1], sb[0]]
+        return "edge", (sa[0], sa[1]), (sb[0], sb[1])
+    if len(sa) == 1:
+        return "vertex", (sa[0],), (sb[0],)
+    return "regular", (), ()
+
+
+def reflected_geometry(mesh: Mesh, signs, det, curls=None):
+    """Trial data of the image elements: vertices, normals and curls (BeatEngineCore.jl:341-356)."""
+    if curls is None:
+        curls = surface_curls(mesh.face_vertices, mesh.normals)
+    return mesh.face_vertices * signs, mesh.normals * signs, curls * (det * signs)
+
+
+def image_singular_cache(mesh: Mesh, order: int, signs, tol: float = 1e-8) -> SingularCache:
+    """Pairs (t, s) whose image R s touches t geometrically (BeatEngineCore.jl:917-945,
+    BeatEngineCpuAssembly.jl:488-527): candidates through rounded vertex keys, then the geometric
+    classification and remapped Duffy rule."""
+    base = {1: duffy_rule(order, "coincident"), 2: duffy_rule(order, "edge"), 3: duffy_rule(order, "vertex")}
+    fv = mesh.face_vertices
+    by_key = {}
+    for t in range(mesh.n_faces):
+        for p in fv[t]:
+            by_key.setdefault(tuple(np.rint(p / tol).astype(np.int64)), []).append(t)
+    cands, seen = [], set()
+    for s in range(mesh.n_faces):
+        for p in fv[s] * signs:
+            for t in by_key.get(tuple(np.rint(p / tol).astype(np.int64)), []):
+                if (t, s) not in seen:
+                    seen.add((t, s))
+                    cands.append((t, s))
+    rules, keys, tt, ss, rr = [], [], [], [], []
+    for t, s in cands:
+        kind, tv, sv = geometric_adjacency_info(fv[t], fv[s] * signs, tol)
+        if kind == "regular":
+            continue
+        key = orientation_key(kind, tv, sv)
+        if key not in keys:
+            keys.append(key)
+            rules.append(remapped_rule(base[key[0]], key))
+        tt.append(t)
+        ss.append(s)
+        rr.append(keys.index(key))
+    return SingularCache(np.asarray(tt, int), np.asarray(ss, int), np.asarray(rr, int), rules, keys)
+
+
+def assemble_operators_symmetric(mesh: Mesh, k: float, mode: str, singular_order: int = 4, rule=None,
+                                 tol: float | None = None) -> Operators:
+    """S, D, D*, H of the symmetric extension: the plain operators, plus for every image transform R all pairs
+    (t, R s) with the regular rule except the geometrically touching ones, which take their Duffy rule; then
+    the P1 orbit row weights (BeatEngineCpuAssembly.jl:636-886)."""
+    ops = assemble_operators(mesh, k, rule=rule, singular_order=singular_order)
+    S, D, Ds, H = ops.single_layer, ops.double_layer, ops.adjoint_double_layer, ops.hypersingular
+    pts, wts = rule if rule is not None else triangle_rule(4)
+    F = mesh.n_faces
+    faces = mesh.faces
+    curls = surface_curls(mesh.face_vertices, mesh.normals)
+    X = local_to_global(mesh.face_vertices, pts)
+    B = p1_values(pts)
+    W2 = np.outer(wts, wts)
+    for _, signs, det in symmetry_transforms(mode, include_identity=False):
+        fv_r, n_r, c_r = reflected_geometry(mesh, signs, det, curls)
+        X_r = X * signs
+        img = image_singular_cache(mesh, singular_order, signs)
+        touching = np.zeros((F, F), bool)
+        touching[img.test, img.trial] = True
+        chunk = max(1, int(2.0e6 // max(1, F * len(wts) ** 2)))
+        for t0 in range(0, F, chunk):
+            rows = np.arange(t0, min(F, t0 + chunk))
+            ti, si = np.nonzero(~touching[rows])
+            tt = rows[ti]
+            jac = 4.0 * mesh.areas[tt] * mesh.areas[si]
+            w = W2[None, :, :] * jac[:, None, None]
+            Sb, Dsb, Db, GMb, Gsb = _pair_blocks(X[tt], X_r[si], mesh.normals[tt], n_r[si], w, B, B, k)
+            nn = np.einsum("pc,pc->p", mesh.normals[tt], n_r[si])
+            cp = np.einsum("pic,pjc->pij", curls[tt], c_r[si])
+            Hb = cp * Gsb[:, None, None] - (k * k) * nn[:, None, None] * GMb
+            _scatter(S, Ds, D, H, faces, tt, si, Sb, Dsb, Db, Hb)
+        if len(img.test):
+            Sb, Dsb, Db, Hb = singular_pair_blocks(mesh, k, img, curls, trial_geometry=(fv_r, n_r, c_r))
+            _scatter(S, Ds, D, H, faces, img.test, img.trial, Sb, Dsb, Db, Hb)
+    wgt = p1_orbit_weights(mesh.vertices, mode, tol)[:, None]
+    return Operators(S * wgt, D * wgt, Ds * wgt, H * wgt)
+
+
+def identity_matrices_symmetric(mesh: Mesh, mode: str, rule=None, tol=None):
+    """Identity blocks with the orbit row weights (BeatEngineCore.jl:952-979)."""
+    M11, M10 = identity_matrices(mesh, rule)
+    wgt = p1_orbit_weights(mesh.vertices, mode, tol)[:, None]
+    return M11 * wgt, M10 * wgt
+
+
+def field_sources_symmetric(mesh: Mesh, mode: str, rule=None):
+    """Sources of every transform, element-major then transform (BeatEngineCore.jl:1245-1280)."""
+    pts, wts = rule if rule is not None else triangle_rule(4)
+    tfs = symmetry_transforms(mode)
+    X = local_to_global(mesh.face_vertices, pts)  # (F,Q,3)
+    F, Q, R = X.shape[0], X.shape[1], len(tfs)
+    signs = np.stack([t[1] for t in tfs])  # (R,3)
+    pts_all = X[:, None, :, :] * signs[None, :, None, :]  # (F,R,Q,3)
+    nrm_all = np.repeat((mesh.normals[:, None, :] * signs[None, :, :])[:, :, None, :], Q, axis=2)
+    return {
+        "points": pts_all.reshape(F * R * Q, 3),
+        "normals": nrm_all.reshape(F * R * Q, 3),
+        "weights": np.broadcast_to((wts[None, :] * (2.0 * mesh.areas)[:, None])[:, None, :], (F, R, Q)).reshape(-1),
+        "faces": np.repeat(mesh.faces, R * Q, axis=0),
+        "elements": np.repeat(np.arange(F), R * Q),
+        "basis": np.tile(p1_values(pts), (F * R, 1)),
+    }
+
+
+def solve_frequency_symmetric(mesh: Mesh, freq: float, q_unit, mode: str, sound_speed=343.0, rho=1.21,
+                              points=None):
+    """One frequency of the plain method on the fundamental domain of an x / xy symmetric model."""
+    omega = 2.0 * math.pi * freq
+    k = omega / sound_speed
+    ops = assemble_operators_symmetric(mesh, k, mode)
+    M11, M10 = identity_matrices_symmetric(mesh, mode)
+    lhs, rhs_op = burton_miller(ops, M11, M10, k)
+    q = 1j * rho * omega * np.asarray(q_unit, complex).reshape(mesh.n_faces, -1)
+    rhs = rhs_op @ q
+    p = solve_dense(lhs, rhs)
+    out = {"pressure": p, "q": q, "lhs": lhs, "rhs": rhs, "k": k}
+    if points is not None:
+        out["field"] = evaluate_field(points, mesh, p, q, k, sources=field_sources_symmetric(mesh, mode))
+    return out
 
 
 # --------------------------------------------------------------------------------------

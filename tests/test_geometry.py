@@ -158,3 +158,62 @@ def test_locality_order_permutes_the_operators():
     assert np.abs(internal["lhs"] - orig["lhs"][np.ix_(vp, vp)]).max() < 1e-12 * np.abs(orig["lhs"]).max()
     assert np.abs(internal["rhs"] - orig["rhs"][vp]).max() < 1e-12 * np.abs(orig["rhs"]).max()
     assert np.allclose(prep.unpermute_vertices(internal["pressure"]), orig["pressure"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("mode", ["x", "xy"])
+def test_image_pairs_match_oracle(mode):
+    """Symmetric preparation: image pairs (trial key s + r F), their orientation keys and Duffy rules equal the
+    oracle's image caches; masks cover every image pair; orbit weights as the oracle."""
+    V, F, T = syn.uv_sphere(6, 12)
+    v, f, t = syn.symmetric_part(V, F, T, mode)
+    mesh = G.BoundaryMesh(v, f, t)
+    prep = G.prepare(mesh, reorder=False, symmetry=mode)
+    om = O.Mesh(v, f, t, np.ones(len(f), int))
+    nF = len(f)
+    pr = prep.pairs
+    got = {}
+    for tt, key, r in zip(pr.test, pr.trial, pr.rule):
+        got[(int(tt), int(key))] = pr.keys[r]
+    expect = {}
+    base = O.build_singular_cache(om, 4)
+    for tt, ss, r in zip(base.test, base.trial, base.rule):
+        expect[(int(tt), int(ss))] = base.keys[r]
+    tfs = O.symmetry_transforms(mode)
+    for ri, (_, signs, _) in enumerate(tfs[1:], start=1):
+        img = O.image_singular_cache(om, 4, signs)
+        for tt, ss, r in zip(img.test, img.trial, img.rule):
+            expect[(int(tt), int(ss) + ri * nF)] = img.keys[r]
+    assert got == expect
+    assert any(key >= nF for (_, key) in got)
+    # rule points of image pairs equal the oracle's remapped rules
+    for (tt, key), k in got.items():
+        if key >= nF:
+            ri = key // nF
+            img = O.image_singular_cache(om, 4, tfs[ri][1])
+            tp, sp, w = img.rules[img.keys.index(k)]
+            r = pr.keys.index(k)
+            pts = pr.rule_pts[pr.rule_off[r]: pr.rule_off[r] + pr.rule_len[r]]
+            assert np.allclose(pts[:, :2], tp, atol=1e-15) and np.allclose(pts[:, 2:4], sp, atol=1e-15)
+            break
+    img_pairs = [(tt, key) for (tt, key) in got if key >= nF]
+    for tt, key in img_pairs:
+        bit = 1 << (key // nF)
+        assert prep.plane_mask[tt] & bit and prep.plane_mask[key % nF] & bit
+    assert np.array_equal(prep.row_weights, O.p1_orbit_weights(v, mode))
+    assert np.array_equal(prep.transforms, np.array([s for _, s, _ in tfs], dtype=np.int32))
+
+
+def test_symmetry_snap_and_validation():
+    V, F, T = syn.uv_sphere(6, 12)
+    v, f, t = syn.symmetric_part(V, F, T, "x")
+    v = v.copy()
+    v[v[:, 0] == 0.0, 0] = 3e-11  # within the plane tolerance
+    m = G.snap_symmetry_planes(G.BoundaryMesh(v, f, t), "x")
+    assert np.sum(m.vertices[:, 0] == 0.0) == np.sum(v[:, 0] == 3e-11)
+    G.validate_symmetry_domain(m, "x")
+    bad = G.BoundaryMesh(v * np.array([-1.0, 1.0, 1.0]), f[:, ::-1], t)
+    with pytest.raises(ValueError, match="positive X fundamental domain"):
+        G.validate_symmetry_domain(bad, "x")
+    assert G.symmetry_reduction_factor("xy") == 4 and G.symmetry_reduction_factor("off") == 1
+    with pytest.raises(ValueError):
+        G.normalized_symmetry_mode("z")

@@ -226,3 +226,68 @@ def test_ragged_multi_batch_and_graph_replay(gpu_device):
     fr = ref[0].field
     assert np.abs(20 * np.log10(np.abs(fm) / np.abs(fr))).max() < 0.05
     eng.close()
+
+
+@pytest.fixture(scope="module")
+def sym_models():
+    """Half (x) and quarter (xy) of a mirror-symmetric latitude/longitude sphere, with oracle solutions."""
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+    from oracle import bem_oracle as O
+
+    V, F, T = syn.uv_sphere(10, 20, 0.1)
+    pts = np.array([[0.3, 0.2, 1.5], [1.0, 0.5, -0.3], [0.2, 1.1, 0.4], [-0.7, -0.4, 0.9]])
+    out = {}
+    for mode in ("x", "xy"):
+        v, f, t = syn.symmetric_part(V, F, T, mode)
+        prep = G.prepare(G.BoundaryMesh(v, f, t), symmetry=mode)
+        m = prep.original
+        om = O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids)
+        ref = O.solve_frequency_symmetric(om, FREQ, np.ones((len(f), 1)), mode, C0, RHO, points=pts)
+        out[mode] = (prep, ref, pts)
+    return out
+
+
+@pytest.mark.parametrize("mode", ["x", "xy"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-5)])
+def test_symmetric_assembly_matches_oracle(gpu_device, sym_models, mode, precision, tol):
+    """Images of every trial element, image-singular Duffy pairs and orbit row weights: A and b of the
+    fundamental domain against the oracle's symmetric operators."""
+    prep, ref, _ = sym_models[mode]
+    eng = _engine(gpu_device, prep)
+    b = eng.batch(precision, 1, None, RHO, C0)
+    b.set_frequencies([FREQ])
+    with gpu_device.torch.cuda.stream(gpu_device.stream):
+        b._assemble()
+    gpu_device.sync()
+    N = prep.n_vertices
+    A = b.A0[0, :, :N].cpu().numpy().astype(np.complex128)
+    rhs = b.b[0, 0, :N].cpu().numpy()
+    vp = prep.vperm
+    print(mode, precision, "A rel", _rel(A, ref["lhs"][np.ix_(vp, vp)]), "b rel", _rel(rhs, ref["rhs"][vp, 0]))
+    assert _rel(A, ref["lhs"][np.ix_(vp, vp)]) < tol
+    assert _rel(rhs, ref["rhs"][vp, 0]) < tol
+    eng.close()
+
+
+@pytest.mark.parametrize("mode", ["x", "xy"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_symmetric_solve_and_field_match_oracle(gpu_device, sym_models, mode, precision):
+    prep, ref, pts = sym_models[mode]
+    eng = _engine(gpu_device, prep)
+    res, _ = eng.solve([FREQ], precision=precision, obs=pts, rho=RHO, sound_speed=C0)
+    p, u = res[0].pressure[0, :, 0], res[0].field[0, :, 0]
+    po, uo = ref["pressure"][:, 0], ref["field"][:, 0]
+    db = np.abs(20 * np.log10(np.abs(u) / np.abs(uo)))
+    print(mode, precision, "p rel", _rel(p, po), "field rel", _rel(u, uo), "max dB", db.max())
+    if precision == "fp64":
+        assert _rel(p, po) < 1e-10 and _rel(u, uo) < 1e-10
+    else:
+        assert _rel(p, po) < 1e-4 and db.max() < 0.05
+    # the reduced model also meets the pulsating-sphere closed form
+    from oracle.bem_oracle import pulsating_sphere_pressure
+
+    k = 2 * math.pi * FREQ / C0
+    pa = pulsating_sphere_pressure(np.linalg.norm(pts, axis=1), A_SPHERE, k, RHO, 2 * math.pi * FREQ)
+    assert np.abs(20 * np.log10(np.abs(u) / np.abs(pa))).max() < 0.2
+    eng.close()

@@ -184,3 +184,39 @@ def test_polar_points_follow_reference_conventions():
     pts, meta = D.spherical_observation({"spherical_sampling_enabled": True, "spherical_sampling_points": 1000})
     assert pts.shape == (1000, 3) and np.allclose(np.linalg.norm(pts, axis=1), 2.0, atol=1e-5)
     assert abs(pts[0, 2] - 2.0 * (1 - 1 / 1000)) < 1e-5  # z = 1 - (2i+1)/n ordering
+
+
+def test_symmetry_requests(tmp_path):
+    """x / xy symmetry: fundamental-domain meshes are accepted (planes snapped, domain validated), the engine
+    receives image tables, impedance carries the reduction factor (solver.jl:607-608); meshes crossing a
+    plane fail with the reference's message."""
+    V, F, T = syn.uv_sphere(6, 12, 0.1)
+    seen = {}
+
+    class Spy(FakeEngine):
+        def __init__(self, prep, qunit):
+            super().__init__(prep, qunit)
+            seen["prep"] = prep
+
+    imps = {}
+    for mode in ("off", "x", "xy"):
+        v, f, t = (V, F, T) if mode == "off" else syn.symmetric_part(V, F, T, mode)
+        path = tmp_path / f"s_{mode}.msh"
+        syn.write_gmsh22(path, v, f, t, scale=0.001)
+        req = _request(str(path), freqs=(800.0,), symmetry=mode)
+        ev = []
+        D.run_job(req, Spy, ev.append, batch=1)
+        assert ev[-1]["type"] == "completed", ev[-1]
+        prep = seen["prep"]
+        assert prep.symmetry == mode and len(prep.transforms) == {"off": 1, "x": 2, "xy": 4}[mode]
+        if mode != "off":
+            assert np.any(prep.pairs.trial >= prep.n_faces) and prep.row_weights.max() > 1
+        res = [e for e in ev if e["type"] == "result"][0]["result"]
+        imps[mode] = res["impedance"][0][0]
+    # the double returns p = 1 on every vertex: force = 10 * area of the radiator * factor, same on every model
+    assert imps["x"] == pytest.approx(imps["off"], rel=2e-3) and imps["xy"] == pytest.approx(imps["off"], rel=2e-3)
+    bad = tmp_path / "bad.msh"
+    syn.write_gmsh22(bad, V, F, T, scale=0.001)
+    ev = []
+    with pytest.raises(ValueError, match="positive X fundamental domain"):
+        D.run_job(_request(str(bad), freqs=(800.0,), symmetry="x"), FakeEngine, ev.append, batch=1)
