@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=16)
     ap.add_argument("--serving-rate", type=float, default=2.0)
+    ap.add_argument("--fanout", type=int, default=4,
+                    help="streams used by the batched LU and refinement (1 = single stream, e.g. for tracing)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the workload of the HBM-counter passes (copy calibration + one assembly)")
     args = ap.parse_args()
@@ -215,6 +217,8 @@ def main():
     prep, qunit, obs, freqs = horn_job()
     eng = Engine(prep, qunit)
     dev = eng.dev
+    _lib_mod = __import__("beat_hip._lib", fromlist=["check"])
+    _lib_mod.check(dev.lib.bem_ctx_set_fanout(dev.ctx, args.fanout), "bem_ctx_set_fanout")
     nb = max(1, min(args.batch, len(freqs)))
 
     # ---- mixed, batched, graph-captured -------------------------------------------------------

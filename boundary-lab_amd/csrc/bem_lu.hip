@@ -48,6 +48,25 @@ struct Mfma<double> {
 };
 
 
+// DPP lane moves for 32-bit and 64-bit values (two 32-bit halves with the same control); lanes without a
+// source keep their own value.
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, RM, 0xf, false));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)b, hi = (int)(b >> 32);
+  const int nlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xf, false);
+  const int nhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xf, false);
+  return __longlong_as_double(((long long)nhi << 32) | (unsigned)nlo);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_mov(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xf, false);
+}
+
 // ------------------------------------------------------------------------------------------------
 // GEMM: C[c_r0+i][c_c0+j] -= sum_k A[a_r0+i][a_c0+k] * B[b_r0+k][b_c0+j], all blocks of one matrix.
 // ------------------------------------------------------------------------------------------------
@@ -134,6 +153,182 @@ __global__ __launch_bounds__(256) void k_gemm_sub(MatBatch<T> mb, GemmShape g, c
           *cp = *cp - acc[ti][tj][v];
         }
       }
+}
+
+// Trailing update on v_mfma_f32_32x32x2f32 (FP32): a workgroup owns 128 rows x 64 complex columns of C, its four
+// waves 64 x 32 complex each as 2 x 2 tiles of 32 x 32 reals. Complex arithmetic without expanding B: for each
+// complex k every lane issues two MFMAs on the same loaded values, A.re against the B row (br, bi) and A.im
+// against (-bi, br), with the real/imaginary column parity of the lane selecting the entries. K is staged in two
+// stages of 16 complex; the next stage is fetched into registers while the current one is multiplied.
+// Output layout of the 32x32 MFMA: register v of lane l holds row (v/4)*8 + (l/32)*4 + v%4, column l%32.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm_tile32(MatBatch<float> mb,
+                                                                                              GemmShape g,
+                                                                                              const int* skip) {
+  constexpr int BM = 128, BN = 64, KC = 16;  // rows, complex columns, complex k per stage
+  const int mat = blockIdx.z;
+  if (skip && skip[mat]) return;
+  cplx<float>* A = mb.A + (size_t)mat * mb.strideA;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wy = w >> 1, wx = w & 1;
+  const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
+  const int half = lane >> 5, l32 = lane & 31;
+  __shared__ cplx<float> As[BM][KC + 1];
+  __shared__ cplx<float> Bs[KC][BN + 1];
+  // The accumulators start as the C tile itself and the MFMAs add (-A) B, so the epilogue is a plain store and
+  // the C reads travel together with the first stage of A and B. Tile rows: (v/4)*8 + half*4 + v%4, real
+  // column l32; 32-bit offsets from the matrix base.
+  float* Cb = reinterpret_cast<float*>(A);
+  const int lda2 = (int)(2 * lda);
+  const int rbase = bm + 64 * wy + half * 4;  // + 32 ti + (v/4)*8 + v%4
+  const int cbase = 2 * bn + 64 * wx + l32;   // + 32 tj (real column)
+  const int obase = (g.c_r0 + rbase) * lda2 + 2 * g.c_c0 + cbase;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int dr = 32 * ti + (v >> 2) * 8 + (v & 3);
+        const bool ok = rbase + dr < g.M && ((cbase + 32 * tj) >> 1) < g.Nc;
+        acc[ti][tj][v] = ok ? Cb[obase + dr * lda2 + 32 * tj] : 0.f;
+      }
+
+  // staging per stage: A = 128 x 16 complex (8 per thread, stored negated), B = 16 x 64 complex (4 per thread)
+  cplx<float> ra[8], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q, r = e >> 4, kk = e & 15;
+      ra[q] = (bm + r < g.M && k0 + kk < g.K) ? A[(size_t)(g.a_r0 + bm + r) * lda + g.a_c0 + k0 + kk]
+                                              : mk<float>(0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q, kk = e >> 6, j = e & 63;
+      rb[q] = (k0 + kk < g.K && bn + j < g.Nc) ? A[(size_t)(g.b_r0 + k0 + kk) * lda + g.b_c0 + bn + j]
+                                               : mk<float>(0.f, 0.f);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q;
+      As[e >> 4][e & 15] = mk<float>(-ra[q].re, -ra[q].im);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      Bs[e >> 6][e & 63] = rb[q];
+    }
+  };
+  const int pj = l32 & 1;  // real/imaginary column parity of this lane's B entries
+  auto multiply = [&]() {
+#pragma unroll 4
+    for (int st = 0; st < KC / 2; ++st) {
+      const int kc = 2 * st + half;
+      cplx<float> a[2];
+      float bre[2], bim[2];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) a[ti] = As[64 * wy + 32 * ti + l32][kc];
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) {
+        const cplx<float> b = Bs[kc][(64 * wx + 32 * tj + l32) >> 1];
+        bre[tj] = pj ? b.im : b.re;   // row (kc, re) of the real expansion: (br, bi)
+        bim[tj] = pj ? b.re : -b.im;  // row (kc, im): (-bi, br)
+      }
+      // four independent accumulators between dependent MFMAs
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ti].re, bre[tj], acc[ti][tj], 0, 0, 0);
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ti].im, bim[tj], acc[ti][tj], 0, 0, 0);
+    }
+  };
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+    if (k0 == 0) fetch(0);
+    stash();
+    __syncthreads();
+    if (k0 + KC < g.K) fetch(k0 + KC);  // next stage in flight during the multiply
+    multiply();
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int dr = 32 * ti + (v >> 2) * 8 + (v & 3);
+        if (rbase + dr < g.M && ((cbase + 32 * tj) >> 1) < g.Nc) Cb[obase + dr * lda2 + 32 * tj] = acc[ti][tj][v];
+      }
+}
+
+// Skinny variant for the sub-panel updates (Nc <= 16 columns, K <= 64): the K x Nc block of B sits in LDS,
+// four threads share a row of C (each a quarter of K, contiguous A loads), partial sums meet by DPP quad
+// swaps, and each thread writes a quarter of the row. Plain FMA: the update is bound by reading A once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_skinny(MatBatch<T> mb, GemmShape g, const int* skip) {
+  constexpr int NCMAX = 16, KMAX = BEM_NBO;
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x;
+  __shared__ cplx<T> Bs[KMAX][NCMAX + 1];
+  for (int e = tid; e < g.K * NCMAX; e += 256) {
+    const int kk = e / NCMAX, j = e % NCMAX;
+    Bs[kk][j] = j < g.Nc ? A[(size_t)(g.b_r0 + kk) * lda + g.b_c0 + j] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  const int row = blockIdx.x * 64 + (tid >> 2), part = tid & 3;
+  const int kq = (g.K + 3) / 4, ka = part * kq, kb = min(g.K, ka + kq);
+  T accr[NCMAX], acci[NCMAX];
+#pragma unroll
+  for (int j = 0; j < NCMAX; ++j) accr[j] = acci[j] = T(0);
+  // this thread's quarter of the row of A, loaded up front (independent loads, one latency)
+  constexpr int KQ = KMAX / 4;
+  T ar_[KQ], ai_[KQ];
+  const cplx<T>* Ar = A + (size_t)(g.a_r0 + min(row, g.M - 1)) * lda + g.a_c0;
+#pragma unroll
+  for (int i = 0; i < KQ; ++i) {
+    const cplx<T> a = (row < g.M && ka + i < kb) ? Ar[ka + i] : mk<T>(0, 0);
+    ar_[i] = a.re;
+    ai_[i] = a.im;
+  }
+#pragma unroll
+  for (int i = 0; i < KQ; ++i) {
+    if (ka + i < kb) {
+#pragma unroll
+      for (int j = 0; j < NCMAX; ++j) {
+        const cplx<T> b = Bs[ka + i][j];
+        accr[j] += ar_[i] * b.re - ai_[i] * b.im;
+        acci[j] += ar_[i] * b.im + ai_[i] * b.re;
+      }
+    }
+  }
+  // sum the four quarter-K partials of a row (lanes 4i..4i+3: quad_perm swaps 1 and 2)
+#pragma unroll
+  for (int j = 0; j < NCMAX; ++j) {
+    accr[j] += dpp_mov<0xb1, 0xf>(accr[j]);  // quad_perm [1,0,3,2]
+    acci[j] += dpp_mov<0xb1, 0xf>(acci[j]);
+    accr[j] += dpp_mov<0x4e, 0xf>(accr[j]);  // quad_perm [2,3,0,1]
+    acci[j] += dpp_mov<0x4e, 0xf>(acci[j]);
+  }
+  if (row < g.M) {
+    cplx<T>* Cr = A + (size_t)(g.c_r0 + row) * lda + g.c_c0;
+#pragma unroll
+    for (int j = 0; j < NCMAX; ++j)
+      if ((j >> 2) == part && j < g.Nc) Cr[j] = mk<T>(Cr[j].re - accr[j], Cr[j].im - acci[j]);
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -260,6 +455,47 @@ __device__ __forceinline__ void wave_max_key(unsigned (&k)[W]) {
   for (int i = 0; i < W; ++i) k[i] = (unsigned)__builtin_amdgcn_readlane((int)k[i], 63);
 }
 
+// maximum over the 64 lanes (non-NaN inputs), in every lane
+template <typename T>
+__device__ __forceinline__ T wave_max_dpp(T v) {
+  v = fmax(v, dpp_mov<0x111, 0xf>(v));
+  v = fmax(v, dpp_mov<0x112, 0xf>(v));
+  v = fmax(v, dpp_mov<0x114, 0xf>(v));
+  v = fmax(v, dpp_mov<0x118, 0xf>(v));
+  v = fmax(v, dpp_mov<0x142, 0xa>(v));
+  v = fmax(v, dpp_mov<0x143, 0xc>(v));
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+}
+
+// maximum / minimum over each group of 16 lanes, returned from lane 15 (every lane gets group 0's result)
+template <typename T>
+__device__ __forceinline__ T row16_max_dpp(T v) {
+  v = fmax(v, dpp_mov<0x111, 0xf>(v));
+  v = fmax(v, dpp_mov<0x112, 0xf>(v));
+  v = fmax(v, dpp_mov<0x114, 0xf>(v));
+  v = fmax(v, dpp_mov<0x118, 0xf>(v));
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 15), hi = __builtin_amdgcn_readlane((int)(b >> 32), 15);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+}
+__device__ __forceinline__ int row16_min_dpp(int v) {
+  v = min(v, dpp_mov<0x111, 0xf>(v));
+  v = min(v, dpp_mov<0x112, 0xf>(v));
+  v = min(v, dpp_mov<0x114, 0xf>(v));
+  v = min(v, dpp_mov<0x118, 0xf>(v));
+  return __builtin_amdgcn_readlane(v, 15);
+}
+
 // Composite of the row interchanges (c0+jj, piv[jj]), jj < nb, applied in order (LAPACK laswp order),
 // restricted to the rows it touches: afterwards row rows[q] holds the former row rows[src[q]]. Run by one
 // wave; rows [c0, c0+nb) come first, pivot rows beyond are appended in first-seen order. Returns the count.
@@ -318,10 +554,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   const long long lda = mb.lda;
   const int N = mb.N;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  constexpr int KW = PivKey<T>::W;
   // per-column exchange area, double-buffered by column parity so that one barrier per column suffices:
-  // every wave publishes its best key and that row's data; the owner of row c publishes row c
-  __shared__ unsigned s_key[2][16][KW];
+  // every wave publishes its largest magnitude, the first row holding it and that row's data; the owner of
+  // row c publishes row c
+  __shared__ T s_val[2][16];
+  __shared__ int s_row[2][16];
   __shared__ cplx<T> s_cand[2][16][NBI];
   __shared__ cplx<T> s_crow[2][NBI];
   __shared__ int s_pivs[NBI];
@@ -344,29 +581,41 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     if (jj < nb) {
       const int buf = jj & 1;
       const int c = c0 + jj;
-      unsigned key[KW];
-#pragma unroll
-      for (int i = 0; i < KW; ++i) key[i] = 0u;
+      // this thread's best candidate: strict > keeps the lowest of its rows among equal magnitudes
+      T bv = T(-1);
+      int bm = -1;
 #pragma unroll
       for (int m = 0; m < MR; ++m) {
         const int r = c0 + tid + 1024 * m;
         if (r >= c && r < N) {
-          unsigned kr[KW];
-          PivKey<T>::make(cabs1(P[m][jj]), r, kr);
-          if (key_less<KW>(key, kr))
-#pragma unroll
-            for (int i = 0; i < KW; ++i) key[i] = kr[i];
+          const T v = cabs1(P[m][jj]);
+          if (v > bv) {
+            bv = v;
+            bm = m;
+          }
         }
       }
-      wave_max_key<KW>(key);
-      const int wrow = (int)~key[KW - 1];  // this wave's best row (meaningless when key is 0)
-      if (lane == 0)
+      // wave maximum (DPP), then the first row holding it: a wave's rows are ordered by (m, lane)
+      const T wmax = wave_max_dpp(bv);
+      int wrow = 0x7fffffff;
+      if (wmax >= T(0)) {
 #pragma unroll
-        for (int i = 0; i < KW; ++i) s_key[buf][w][i] = key[i];
+        for (int m = 0; m < MR; ++m) {
+          const unsigned long long ball = __ballot(bm == m && bv == wmax);
+          if (ball) {
+            wrow = c0 + 64 * w + (int)__builtin_ctzll(ball) + 1024 * m;
+            break;
+          }
+        }
+      }
+      if (lane == 0) {
+        s_val[buf][w] = wmax;
+        s_row[buf][w] = wrow;
+      }
 #pragma unroll
       for (int m = 0; m < MR; ++m) {
         const int r = c0 + tid + 1024 * m;
-        if (r == wrow && r < N)
+        if (r == wrow)
 #pragma unroll
           for (int j = 0; j < NBI; ++j) s_cand[buf][w][j] = P[m][j];
         if (r == c)
@@ -374,27 +623,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
           for (int j = 0; j < NBI; ++j) s_crow[buf][j] = P[m][j];
       }
       __syncthreads();
-      // every lane reduces the 16 wave keys (LDS broadcast reads)
-      unsigned best[KW];
-      int gw = 0;
-#pragma unroll
-      for (int i = 0; i < KW; ++i) best[i] = s_key[buf][0][i];
-#pragma unroll
-      for (int q = 1; q < 16; ++q) {
-        unsigned kq[KW];
-#pragma unroll
-        for (int i = 0; i < KW; ++i) kq[i] = s_key[buf][q][i];
-        if (key_less<KW>(best, kq)) {
-#pragma unroll
-          for (int i = 0; i < KW; ++i) best[i] = kq[i];
-          gw = q;
-        }
-      }
-      bool any = false;
-#pragma unroll
-      for (int i = 0; i < KW; ++i) any = any || best[i] != 0u;
+      // across the 16 waves: largest magnitude, then the lowest row holding it (LAPACK i?amax order)
+      const T vq = s_val[buf][lane & 15];
+      const int rq = s_row[buf][lane & 15];
+      const T gmax = row16_max_dpp(vq);
+      const int pick = row16_min_dpp(vq == gmax && gmax >= T(0) ? rq : 0x7fffffff);
       // no candidate at all cannot happen for c < N; keep row c then (the pivot test records a breakdown)
-      const int pv = any ? (int)~best[KW - 1] : c;
+      const int pv = gmax >= T(0) ? pick : c;
+      const int gw = ((pv - c0) & 1023) >> 6;
       const cplx<T>* prow = pv == c ? s_crow[buf] : s_cand[buf][gw];  // pivot row (LDS broadcast reads)
       if (pv != c) {
 #pragma unroll
@@ -835,6 +1071,17 @@ template <typename T>
 static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape& g, const int* skip,
                               hipStream_t st) {
   if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
+  if (g.Nc <= 16 && g.K <= BEM_NBO) {
+    hipLaunchKernelGGL(k_gemm_skinny<T>, dim3((g.M + 63) / 64, batch), dim3(256), 0, st, mb, g, skip);
+    return hipGetLastError();
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (g.K <= BEM_NBO) {
+      hipLaunchKernelGGL(k_gemm_tile32, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st, mb, g,
+                         skip);
+      return hipGetLastError();
+    }
+  }
   dim3 grid((g.Nc + 63) / 64, (g.M + 63) / 64, batch);
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, mb, g, skip);
   return hipGetLastError();

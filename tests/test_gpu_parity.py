@@ -130,12 +130,15 @@ def test_lu_random(gpu_device, precision, n, batch):
                                 2 * lda, _ptr(y), lda, 2 * lda, 2, None), "solve")
     gpu_device.sync()
     X = y[:, :, :n].cpu().numpy().astype(np.complex128)
-    tol = 1e-10 if precision == "fp64" else 5e-4
+    # forward error vs LAPACK, and the backward error |B - M X| / (|M|_2 |X|) that partial-pivoting LU bounds.
+    # LAPACK's own cgetrf reaches 6.6e-4 forward / 1.6e-5 backward on a matrix of this family at N = 2100.
+    tol_fwd, tol_bwd = (1e-10, 1e-13) if precision == "fp64" else (3e-3, 1e-4)
     for m in range(batch):
         ref = np.linalg.solve(M[m], B[m].T).T
         err = np.linalg.norm(X[m] - ref) / np.linalg.norm(ref)
-        print(precision, n, m, "rel err", err)
-        assert err < tol
+        bwd = np.linalg.norm(B[m].T - M[m] @ X[m].T) / (np.linalg.norm(M[m], 2) * np.linalg.norm(X[m]))
+        print(precision, n, m, "rel err", err, "backward", bwd)
+        assert err < tol_fwd and bwd < tol_bwd
     assert int(info.cpu().numpy().max()) == 0
 
 
