@@ -377,11 +377,15 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
     const double dx = (s0[0] + sxi * se1[0] + set * se2[0]) - (t0[0] + txi * te1[0] + tet * te2[0]);
     const double dy = (s0[1] + sxi * se1[1] + set * se2[1]) - (t0[1] + txi * te1[1] + tet * te2[1]);
     const double dz = (s0[2] + sxi * se1[2] + set * se2[2]) - (t0[2] + txi * te1[2] + tet * te2[2]);
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r == 0.0) continue;
-    const double rinv = 1.0 / r;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 == 0.0) continue;
+    // 1/r from the hardware reciprocal square root and two Newton steps (full FP64 accuracy)
+    double rinv = __builtin_amdgcn_rsq(r2);
+    rinv = rinv * fma(-0.5 * r2 * rinv, rinv, 1.5);
+    rinv = rinv * fma(-0.5 * r2 * rinv, rinv, 1.5);
+    const double r = r2 * rinv;
     double sn, cs;
-    sincos(k * r, &sn, &cs);
+    sincos_phase(k * r, &sn, &cs);
     const double gr = cs * rinv * w, gi = sn * rinv * w;  // 4 pi G w
     const double hr = -gr * rinv - gi * k, hi = gr * k - gi * rinv;
     const double td = (dx * ns[0] + dy * ns[1] + dz * ns[2]) * rinv;

@@ -53,6 +53,40 @@ __device__ __forceinline__ T cabs1(cplx<T> a) {  // |re| + |im|, the LAPACK i?am
 //  float : hardware v_sin_f32 / v_cos_f32 take revolutions; the phase is formed as (k/2pi)*r and
 //          reduced to [0,1) with v_fract_f32 before the call.
 //  double: library sincos in radians.
+// sin and cos of a non-negative FP64 phase of moderate size (k r below 2^20 rad: element-scale distances):
+// quadrant reduction q = rint(x 2/pi) with pi/2 split in two doubles (fused multiply-adds keep it exact at these
+// magnitudes), then the Taylor series of sin and cos on |y| <= pi/4 to degree 15 / 16 (truncation below 2e-17).
+__device__ __forceinline__ void sincos_phase(double x, double* s, double* c) {
+  constexpr double kTwoOverPi = 0.6366197723675814;
+  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+  const double q = rint(x * kTwoOverPi);
+  double y = fma(-q, kPio2Hi, x);
+  y = fma(-q, kPio2Lo, y);
+  const double y2 = y * y;
+  // 1/n! coefficients, Horner form
+  double ps = -1.0 / 1307674368000.0;                 // -1/15!
+  ps = fma(ps, y2, 1.0 / 6227020800.0);               // +1/13!
+  ps = fma(ps, y2, -1.0 / 39916800.0);                // -1/11!
+  ps = fma(ps, y2, 1.0 / 362880.0);                   // +1/9!
+  ps = fma(ps, y2, -1.0 / 5040.0);                    // -1/7!
+  ps = fma(ps, y2, 1.0 / 120.0);                      // +1/5!
+  ps = fma(ps, y2, -1.0 / 6.0);                       // -1/3!
+  const double sy = fma(ps * y2, y, y);
+  double pc = 1.0 / 20922789888000.0;                 // +1/16!
+  pc = fma(pc, y2, -1.0 / 87178291200.0);             // -1/14!
+  pc = fma(pc, y2, 1.0 / 479001600.0);                // +1/12!
+  pc = fma(pc, y2, -1.0 / 3628800.0);                 // -1/10!
+  pc = fma(pc, y2, 1.0 / 40320.0);                    // +1/8!
+  pc = fma(pc, y2, -1.0 / 720.0);                     // -1/6!
+  pc = fma(pc, y2, 1.0 / 24.0);                       // +1/4!
+  pc = fma(pc, y2, -0.5);                             // -1/2!
+  const double cy = fma(pc, y2, 1.0);
+  const int iq = (int)q & 3;
+  const double ss = (iq & 1) ? cy : sy, cc = (iq & 1) ? sy : cy;
+  *s = (iq & 2) ? -ss : ss;
+  *c = ((iq + 1) & 2) ? -cc : cc;
+}
+
 template <typename T>
 struct Phase;
 template <>
@@ -69,9 +103,14 @@ template <>
 struct Phase<double> {
   __device__ __forceinline__ static void sincos_kr(double kappa_rev, double k, double r, double* s, double* c) {
     (void)kappa_rev;
-    sincos(k * r, s, c);
+    sincos_phase(k * r, s, c);
   }
-  __device__ __forceinline__ static double rsqrt_(double x) { return x > 0.0 ? 1.0 / sqrt(x) : 0.0; }
+  __device__ __forceinline__ static double rsqrt_(double x) {
+    if (!(x > 0.0)) return 0.0;
+    double y = __builtin_amdgcn_rsq(x);  // hardware estimate, two Newton steps to full FP64 accuracy
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return y * fma(-0.5 * x * y, y, 1.5);
+  }
 };
 
 // Wave-level reductions (wave64).
