@@ -85,9 +85,11 @@ SIGNATURES = {
     "bem_mesh_reserve": (_i, [_p, _i, _i, _i, _i, _i]),
     "bem_mesh_device_bytes": (_i, [_p, C.POINTER(_ll)]),
     "bem_assemble": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _ll, _ll, _p, _p, _ll, _ll, _p]),
-    "bem_lu_factor": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _ll, _p]),
-    "bem_lu_solve": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _ll, _p, _ll, _ll, _p, _ll, _ll, _i, _p]),
-    "bem_refine": (_i, [_p, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _ll, _ll, _p, _p, _p, _i, _i, _d, _p, _p, _p]),
+    "bem_lu_workspace_bytes": (_i, [_i, _i, _i, C.POINTER(_ll)]),
+    "bem_lu_factor": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _ll, _p, _p]),
+    "bem_lu_solve": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _ll, _ll, _p, _ll, _ll, _i, _p]),
+    "bem_refine": (_i, [_p, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _p, _ll, _ll, _p, _p, _p, _i, _i, _d, _p, _p,
+                        _p]),
     "bem_residual": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _ll, _ll, _p, _i, _p, _p]),
     "bem_growth": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _p]),
     "bem_field": (_i, [_p, _p, _i, _i, _p, _i, _p, _p, _ll, _ll, _p, _ll, _i, _p, _ll, _ll]),
@@ -136,3 +138,10 @@ def last_error(lib=None) -> str:
 def check(status: int, what: str = "", lib=None):
     if status != BEM_OK:
         raise BemError(f"{what} failed (status {status}): {last_error(lib)}")
+
+
+def lu_workspace_bytes(precision: int, n: int, batch: int) -> int:
+    """Bytes of the LU workspace (inverted diagonal blocks) for `batch` n x n factorizations."""
+    out = C.c_longlong(0)
+    check(load().bem_lu_workspace_bytes(precision, n, batch, C.byref(out)), "bem_lu_workspace_bytes")
+    return int(out.value)

@@ -213,6 +213,8 @@ class Batch:
             self.ipiv = torch.zeros((nf, N), dtype=torch.int32, **kw)
             self.perm = torch.zeros((nf, N), dtype=torch.int32, **kw)
             self.info = torch.zeros(nf, dtype=torch.int32, **kw)
+            wbytes = _lib.lu_workspace_bytes(self.prec, N, nf)
+            self.luw = torch.empty(wbytes // (8 if precision == "mixed" else 16), dtype=cdt, **kw)
             self.rel = torch.zeros(nf, dtype=torch.float64, **kw)
             self.iters = torch.zeros(nf, dtype=torch.int32, **kw)
             self.done = torch.zeros(nf, dtype=torch.int32, **kw)
@@ -240,15 +242,17 @@ class Batch:
         lib, dev = self.dev.lib, self.dev
         sA, sB, sP = self.N * self.lda, self.n_ch * self.ldb, self.N
         _lib.check(lib.bem_lu_factor(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A), self.lda, sA,
-                                     _ptr(self.ipiv), _ptr(self.perm), sP, _ptr(self.info)), "bem_lu_factor")
+                                     _ptr(self.ipiv), _ptr(self.perm), sP, _ptr(self.info), _ptr(self.luw)),
+                   "bem_lu_factor")
         if self.precision == "mixed":
             _lib.check(lib.bem_refine(dev.ctx, self.N, self.nf, _ptr(self.A), self.lda, sA, _ptr(self.perm), sP,
-                                      _ptr(self.A0), _ptr(self.b), self.ldb, sB, _ptr(self.x), _ptr(self.r),
+                                      _ptr(self.luw), _ptr(self.A0), _ptr(self.b), self.ldb, sB, _ptr(self.x), _ptr(self.r),
                                       _ptr(self.y), self.n_ch, self.max_iter, self.tol, _ptr(self.rel),
                                       _ptr(self.iters), _ptr(self.done)), "bem_refine")
         else:
             _lib.check(lib.bem_lu_solve(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A), self.lda, sA,
-                                        _ptr(self.perm), sP, _ptr(self.b), self.ldb, sB, _ptr(self.x), self.ldb, sB,
+                                        _ptr(self.perm), sP, _ptr(self.luw), _ptr(self.b), self.ldb, sB, _ptr(self.x),
+                                        self.ldb, sB,
                                         self.n_ch, None), "bem_lu_solve")
             _lib.check(lib.bem_residual(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A0), self.lda, sA,
                                         _ptr(self.b), _ptr(self.x), self.ldb, sB, _ptr(self.r), self.n_ch,

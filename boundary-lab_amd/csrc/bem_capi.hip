@@ -599,9 +599,18 @@ int bem_assemble(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k
   return fail(BEM_E_ARG, "unknown precision");
 }
 
+int bem_lu_workspace_bytes(int precision, int n, int batch, long long* bytes) {
+  if (!bytes || n <= 0 || batch <= 0) return fail(BEM_E_ARG, "invalid arguments");
+  const long long esz = precision == BEM_PREC_MIXED ? 8 : precision == BEM_PREC_FP64 ? 16 : 0;
+  if (!esz) return fail(BEM_E_ARG, "unknown precision");
+  *bytes = lu_workspace_elems(n) * esz * batch;
+  return BEM_OK;
+}
+
 int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long long lda, long long strideA,
-                  int* ipiv, int* perm, long long strideP, int* info) {
-  if (!c || !A || !ipiv || !perm || !info) return fail(BEM_E_ARG, "NULL argument");
+                  int* ipiv, int* perm, long long strideP, int* info, void* work) {
+  if (!c || !A || !ipiv || !perm || !info || !work) return fail(BEM_E_ARG, "NULL argument");
+  const long long sw = lu_workspace_elems(n);
   if (n <= 0 || batch <= 0 || lda < n || strideA < lda * n || strideP < n) return fail(BEM_E_ARG, "invalid sizes");
   if (n > 12288) return fail(BEM_E_ARG, "N above 12288 is not supported by the register panel");
   hipError_t e;
@@ -611,13 +620,13 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
     e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) {
       MatBatch<float> mb = {static_cast<cplx<float>*>(A) + (size_t)b0 * strideA, lda, strideA, n};
       return lu_factor_batch<float>(mb, nb, ipiv + (size_t)b0 * strideP, perm + (size_t)b0 * strideP, strideP,
-                                    info + b0, nullptr, st);
+                                    info + b0, nullptr, static_cast<cplx<float>*>(work) + (size_t)b0 * sw, st);
     });
   } else if (precision == BEM_PREC_FP64) {
     e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) {
       MatBatch<double> mb = {static_cast<cplx<double>*>(A) + (size_t)b0 * strideA, lda, strideA, n};
       return lu_factor_batch<double>(mb, nb, ipiv + (size_t)b0 * strideP, perm + (size_t)b0 * strideP, strideP,
-                                     info + b0, nullptr, st);
+                                     info + b0, nullptr, static_cast<cplx<double>*>(work) + (size_t)b0 * sw, st);
     });
   } else {
     return fail(BEM_E_ARG, "unknown precision");
@@ -627,20 +636,20 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
 }
 
 int bem_lu_solve(bem_ctx* c, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
-                 const int* perm, long long strideP, const void* b, long long ldb, long long strideB, void* y,
-                 long long ldy, long long strideY, int nrhs, const int* skip) {
-  if (!c || !LU || !perm || !b || !y) return fail(BEM_E_ARG, "NULL argument");
+                 const int* perm, long long strideP, const void* work, const void* b, long long ldb,
+                 long long strideB, void* y, long long ldy, long long strideY, int nrhs, const int* skip) {
+  if (!c || !LU || !perm || !b || !y || !work) return fail(BEM_E_ARG, "NULL argument");
   if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || n <= 0 || batch <= 0) return fail(BEM_E_ARG, "invalid sizes");
   ScopedTimer tm(c, BEM_TIMER_SOLVE, (long long)batch);
   hipError_t e;
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {(cplx<float>*)LU, lda, strideA, n};
     e = lu_solve_batch<float>(mb, batch, perm, strideP, (const cplx<double>*)b, ldb, strideB, (cplx<float>*)y, ldy,
-                              strideY, nrhs, skip, c->stream);
+                              strideY, nrhs, skip, (const cplx<float>*)work, c->stream);
   } else if (precision == BEM_PREC_FP64) {
     MatBatch<double> mb = {(cplx<double>*)LU, lda, strideA, n};
     e = lu_solve_batch<double>(mb, batch, perm, strideP, (const cplx<double>*)b, ldb, strideB, (cplx<double>*)y,
-                               ldy, strideY, nrhs, skip, c->stream);
+                               ldy, strideY, nrhs, skip, (const cplx<double>*)work, c->stream);
   } else {
     return fail(BEM_E_ARG, "unknown precision");
   }
@@ -649,10 +658,12 @@ int bem_lu_solve(bem_ctx* c, int precision, int n, int batch, const void* LU, lo
 }
 
 int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA, const int* perm,
-               long long strideP, const void* A0, const void* b, long long ldb, long long strideB, void* x,
-               void* work_r, void* work_y, int nrhs, int max_iter, double tol, double* rel, int* iters, int* done) {
-  if (!c || !LU || !perm || !A0 || !b || !x || !work_r || !work_y || !rel || !iters || !done)
+               long long strideP, const void* lu_work, const void* A0, const void* b, long long ldb,
+               long long strideB, void* x, void* work_r, void* work_y, int nrhs, int max_iter, double tol,
+               double* rel, int* iters, int* done) {
+  if (!c || !LU || !perm || !lu_work || !A0 || !b || !x || !work_r || !work_y || !rel || !iters || !done)
     return fail(BEM_E_ARG, "NULL argument");
+  const long long sw = lu_workspace_elems(n);
   if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || max_iter < 0) return fail(BEM_E_ARG, "invalid arguments");
   ScopedTimer tm(c, BEM_TIMER_SOLVE, (long long)batch);
   // matrices are independent: sub-batches refine on separate streams (each solve is a chain of short launches)
@@ -665,17 +676,18 @@ int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long
     cplx<double>* rr = (cplx<double>*)work_r + (size_t)b0 * strideB;
     cplx<float>* yy = (cplx<float>*)work_y + (size_t)b0 * strideB;
     int* dn = done + b0;
+    const cplx<float>* ww = (const cplx<float>*)lu_work + (size_t)b0 * sw;
     hipError_t e2;
     // done[] = 0 with a fill kernel (no memset nodes: this sequence may be captured into a graph)
     if ((e2 = fill_int(dn, nb, 0, st))) return e2;
-    if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, st)))
+    if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, ww, st)))
       return e2;
     if ((e2 = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 1, nullptr, nb, st))) return e2;
     for (int it = 0; it <= max_iter; ++it) {
       if ((e2 = residual_batch<float>(a0, nb, bb, xx, rr, ldb, strideB, nrhs, dn, st))) return e2;
       if ((e2 = resnorm_batch(rr, bb, ldb, strideB, n, nrhs, tol, rel + b0, dn, it, iters + b0, nb, st))) return e2;
       if (it == max_iter) break;
-      if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, dn, st)))
+      if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, dn, ww, st)))
         return e2;
       if ((e2 = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 0, dn, nb, st))) return e2;
     }

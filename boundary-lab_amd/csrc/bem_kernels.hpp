@@ -119,13 +119,15 @@ hipError_t launch_rhs(const RhsParams<T>&, int nf, hipStream_t);
 template <typename T>
 hipError_t launch_gather(const GatherParams<T>&, int nf, hipStream_t);
 
+// elements (cT) of LU workspace per matrix: inverted 64 x 64 diagonal blocks of L and U
+long long lu_workspace_elems(int N);
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, hipStream_t);
+                           const int* skip, cplx<T>* W, hipStream_t);
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>&, int batch, const int* perm, long long strideP, const cplx<double>* b,
                           long long ldb, long long strideB, cplx<T>* y, long long ldy, long long strideY, int nrhs,
-                          const int* skip, hipStream_t);
+                          const int* skip, const cplx<T>* W, hipStream_t);
 template <typename T>
 hipError_t residual_batch(const MatBatch<T>&, int batch, const cplx<double>* b, const cplx<double>* x,
                           cplx<double>* r, long long ldv, long long strideV, int nrhs, const int* skip, hipStream_t);

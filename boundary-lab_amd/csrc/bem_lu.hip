@@ -23,6 +23,9 @@
 #define NBO BEM_NBO
 #define MAX_CH BEM_MAX_CH
 
+static_assert(NBO == 64, "the inverted diagonal blocks of the LU workspace are one outer panel wide");
+long long lu_workspace_elems(int N) { return (long long)((N + 63) / 64) * 2 * 4096; }
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -163,9 +166,12 @@ __global__ __launch_bounds__(256) void k_gemm_sub(MatBatch<T> mb, GemmShape g, c
 // Output layout of the 32x32 MFMA: register v of lane l holds row (v/4)*8 + (l/32)*4 + v%4, column l%32.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm_tile32(MatBatch<float> mb,
-                                                                                              GemmShape g,
-                                                                                              const int* skip) {
+// ASSIGN = false: C -= A B (C starts in the accumulators, A staged negated). ASSIGN = true: C = A B with A read
+// from a separate operand array (Aop, leading dimension ldaop, stride strideAop per matrix): the triangular solve
+// with an inverted diagonal block, in place on B (a workgroup reads its whole B block before writing C).
+template <bool ASSIGN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm_tile32(
+    MatBatch<float> mb, GemmShape g, const int* skip, const cplx<float>* Aop, long long ldaop, long long strideAop) {
   constexpr int BM = 128, BN = 64, KC = 16;  // rows, complex columns, complex k per stage
   const int mat = blockIdx.z;
   if (skip && skip[mat]) return;
@@ -193,9 +199,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int dr = 32 * ti + (v >> 2) * 8 + (v & 3);
-        const bool ok = rbase + dr < g.M && ((cbase + 32 * tj) >> 1) < g.Nc;
+        const bool ok = !ASSIGN && rbase + dr < g.M && ((cbase + 32 * tj) >> 1) < g.Nc;
         acc[ti][tj][v] = ok ? Cb[obase + dr * lda2 + 32 * tj] : 0.f;
       }
+  const cplx<float>* Asrc = ASSIGN ? Aop + (size_t)mat * strideAop : A;
+  const long long ldas = ASSIGN ? ldaop : lda;
 
   // staging per stage: A = 128 x 16 complex (8 per thread, stored negated), B = 16 x 64 complex (4 per thread)
   cplx<float> ra[8], rb[4];
@@ -203,7 +211,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int e = tid + 256 * q, r = e >> 4, kk = e & 15;
-      ra[q] = (bm + r < g.M && k0 + kk < g.K) ? A[(size_t)(g.a_r0 + bm + r) * lda + g.a_c0 + k0 + kk]
+      ra[q] = (bm + r < g.M && k0 + kk < g.K) ? Asrc[(size_t)(g.a_r0 + bm + r) * ldas + g.a_c0 + k0 + kk]
                                               : mk<float>(0.f, 0.f);
     }
 #pragma unroll
@@ -217,7 +225,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int e = tid + 256 * q;
-      As[e >> 4][e & 15] = mk<float>(-ra[q].re, -ra[q].im);
+      As[e >> 4][e & 15] = ASSIGN ? ra[q] : mk<float>(-ra[q].re, -ra[q].im);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -766,6 +774,65 @@ __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, 
 }
 
 // ------------------------------------------------------------------------------------------------
+// Inverses of the 64 x 64 diagonal blocks of L (unit lower) and U (upper), stored per matrix in the LU
+// workspace W[mat][block][0 = L, 1 = U][64][64] (identity-padded past N). They turn the TRSM of each outer panel
+// into an MFMA product and the diagonal step of the triangular sweeps into a matrix-vector product.
+// One workgroup per (block, matrix); thread j computes column j by substitution in LDS.
+// ------------------------------------------------------------------------------------------------
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, long long strideW, int kb_first,
+                                                 const int* skip) {
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  const int kb = kb_first + blockIdx.x;
+  const int N = mb.N;
+  const int r0 = kb * 64, nr = min(64, N - r0);
+  if (nr <= 0) return;
+  const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x;
+  __shared__ cplx<T> Bk[64][65];
+  __shared__ cplx<T> X[64][65];
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    cplx<T> v = mk<T>(i == j ? T(1) : T(0), T(0));
+    if (i < nr && j < nr) {
+      if (LOWER ? (j < i) : (j >= i)) v = A[(size_t)(r0 + i) * lda + r0 + j];
+      else if (LOWER && i == j) v = mk<T>(T(1), T(0));
+      else v = mk<T>(T(0), T(0));
+    }
+    Bk[i][j] = v;
+    X[i][j] = mk<T>(T(0), T(0));
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int j = tid;
+    if (LOWER) {
+      // column j of L^-1: x_j = 1, x_i = -sum_{k=j}^{i-1} L_ik x_k
+      X[j][j] = mk<T>(T(1), T(0));
+      for (int i = j + 1; i < 64; ++i) {
+        cplx<T> acc = mk<T>(T(0), T(0));
+        for (int k = j; k < i; ++k) acc = cfma(acc, Bk[i][k], X[k][j]);
+        X[i][j] = mk<T>(-acc.re, -acc.im);
+      }
+    } else {
+      // column j of U^-1: x_j = 1 / U_jj, x_i = -(sum_{k=i+1}^{j} U_ik x_k) / U_ii
+      X[j][j] = cdiv(mk<T>(T(1), T(0)), Bk[j][j]);
+      for (int i = j - 1; i >= 0; --i) {
+        cplx<T> acc = mk<T>(T(0), T(0));
+        for (int k = i + 1; k <= j; ++k) acc = cfma(acc, Bk[i][k], X[k][j]);
+        const cplx<T> q = cdiv(acc, Bk[i][i]);
+        X[i][j] = mk<T>(-q.re, -q.im);
+      }
+    }
+  }
+  __syncthreads();
+  cplx<T>* Wb = W + (size_t)mat * strideW + ((size_t)kb * 2 + (LOWER ? 0 : 1)) * 4096;
+  for (int e = tid; e < 64 * 64; e += 256) Wb[e] = X[e >> 6][e & 63];
+}
+
+// ------------------------------------------------------------------------------------------------
 // Misc kernels: identity permutation, RHS permutation, triangular sweeps, residual, norms, update.
 // ------------------------------------------------------------------------------------------------
 
@@ -808,7 +875,8 @@ __global__ void k_permute_rhs(const cplx<double>* src, long long lds_, long long
 // Upper (backward, U with diagonal): the mirror image with blocks descending; kb = nblk solves the last block.
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, long long ldy, long long strideY,
-                                                  int nrhs, int kb, const int* skip) {
+                                                  int nrhs, int kb, const int* skip, const cplx<T>* W,
+                                                  long long strideW) {
   const int mat = blockIdx.y;
   if (skip && skip[mat]) return;
   const int N = mb.N;
@@ -816,22 +884,21 @@ __global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, l
   const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
   cplx<T>* Y = y + (size_t)mat * strideY;
   const long long lda = mb.lda;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x;
   const int rb = LOWER ? kb + 1 + (int)blockIdx.x : kb - 1 - (int)blockIdx.x;  // row block of this workgroup
   if (rb < 0 || rb >= nblk) return;
   const int r0 = rb * 64, nr = min(64, N - r0);
   __shared__ cplx<T> xk[MAX_CH][64];
-  __shared__ cplx<T> yb[MAX_CH][64];  // updated rows of the block solved by this workgroup
-  __shared__ cplx<T> Dt[64][65];      // its diagonal block, transposed (padded: conflict-free column reads)
+  __shared__ cplx<T> yb[MAX_CH][64];  // current values of the block solved by this workgroup
   const bool solve_here = LOWER ? (rb == kb + 1) : (rb == kb - 1);
   const bool updated = kb >= 0 && kb < nblk;
+  // 4 threads per row, 16 columns each
+  const int row = tid >> 2, part = tid & 3;
   if (updated) {
     const int c0 = kb * 64, nc = min(64, N - c0);
     if (tid < 64)
       for (int c = 0; c < nrhs; ++c) xk[c][tid] = tid < nc ? Y[(size_t)c * ldy + c0 + tid] : mk<T>(0, 0);
     __syncthreads();
-    // 4 threads per row, 16 columns each
-    const int row = tid >> 2, part = tid & 3;
     cplx<T> acc[MAX_CH];
 #pragma unroll
     for (int c = 0; c < MAX_CH; ++c) acc[c] = mk<T>(0, 0);
@@ -862,34 +929,42 @@ __global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, l
             yb[c][row] = nv;
           else
             *yp = nv;
+        } else if (part == 0 && solve_here) {
+          yb[c][row] = mk<T>(0, 0);  // rows past N meet zero columns of the padded inverse
         }
       }
     }
   }
-  // the workgroup holding the next block to solve finishes it (updated rows are handed over in LDS); the
-  // diagonal block is staged in LDS first so the serial column sweep waits on no global load
+  // the workgroup holding the next block to solve finishes it with the inverted diagonal block:
+  // y_blk = D^-1 y_blk, a 64 x 64 matrix-vector product (no serial substitution)
   if (!solve_here) return;
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int i = e >> 6, j = e & 63;  // coalesced along rows of A
-    Dt[j][i] = (i < nr && j < nr) ? A[(size_t)(r0 + i) * lda + r0 + j] : mk<T>(0, 0);
-  }
+  if (!updated && tid < 64)
+    for (int c = 0; c < nrhs; ++c) yb[c][tid] = tid < nr ? Y[(size_t)c * ldy + r0 + tid] : mk<T>(0, 0);
   __syncthreads();
-  if (w != 0) return;
-  for (int c = 0; c < nrhs; ++c) {
-    cplx<T> v = lane < nr ? (updated ? yb[c][lane] : Y[(size_t)c * ldy + r0 + lane]) : mk<T>(0, 0);
-    if (LOWER) {
-      for (int r = 0; r < nr; ++r) {
-        const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
-        if (lane > r && lane < nr) v = cfms(v, Dt[r][lane], xr);
-      }
-    } else {
-      for (int r = nr - 1; r >= 0; --r) {
-        if (lane == r) v = cdiv(v, Dt[r][r]);
-        const cplx<T> xr = mk<T>(__shfl(v.re, r, 64), __shfl(v.im, r, 64));
-        if (lane < r) v = cfms(v, Dt[r][lane], xr);
-      }
+  const cplx<T>* Wb = W + (size_t)mat * strideW + ((size_t)rb * 2 + (LOWER ? 0 : 1)) * 4096;
+  cplx<T> acc[MAX_CH];
+#pragma unroll
+  for (int c = 0; c < MAX_CH; ++c) acc[c] = mk<T>(0, 0);
+  {
+    const cplx<T>* Wr = Wb + (size_t)row * 64 + 16 * part;
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+      const cplx<T> wv = Wr[j];
+#pragma unroll
+      for (int c = 0; c < MAX_CH; ++c)
+        if (c < nrhs) acc[c] = cfma(acc[c], wv, yb[c][16 * part + j]);
     }
-    if (lane < nr) Y[(size_t)c * ldy + r0 + lane] = v;
+  }
+#pragma unroll
+  for (int c = 0; c < MAX_CH; ++c) {
+    if (c < nrhs) {
+      T ar = acc[c].re, ai = acc[c].im;
+      ar += __shfl_xor(ar, 1, 64);
+      ai += __shfl_xor(ai, 1, 64);
+      ar += __shfl_xor(ar, 2, 64);
+      ai += __shfl_xor(ai, 2, 64);
+      if (part == 0 && row < nr) Y[(size_t)c * ldy + r0 + row] = mk<T>(ar, ai);
+    }
   }
 }
 
@@ -1077,8 +1152,8 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
   }
   if constexpr (sizeof(T) == 4) {
     if (g.K <= BEM_NBO) {
-      hipLaunchKernelGGL(k_gemm_tile32, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st, mb, g,
-                         skip);
+      hipLaunchKernelGGL(k_gemm_tile32<false>, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st,
+                         mb, g, skip, nullptr, 0LL, 0LL);
       return hipGetLastError();
     }
   }
@@ -1105,8 +1180,9 @@ static hipError_t launch_trsm(const MatBatch<T>& mb, int batch, int r0, int nb, 
 
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, hipStream_t st) {
+                           const int* skip, cplx<T>* W, hipStream_t st) {
   const int N = mb.N;
+  const long long strideW = lu_workspace_elems(N);
   hipLaunchKernelGGL(k_iota, dim3((N + 255) / 256, batch), dim3(256), 0, st, perm, strideP, N, info);
   CHK(hipGetLastError());
   for (int k0 = 0; k0 < N; k0 += NBO) {
@@ -1122,6 +1198,9 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
       CHK(launch_panel<T>(mb, batch, c0, nb, k0, k1, ipiv, perm, strideP, info, skip, st));
       c0 += nb;
     }
+    // L11 of this outer panel is final (later interchanges touch rows >= k1 only): invert it
+    hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(1, batch), dim3(256), 0, st, mb, W, strideW, k0 / NBO, skip);
+    CHK(hipGetLastError());
     if (N > k1 - k0) {
       constexpr int CB = sizeof(T) == 4 ? 64 : 32;
       hipLaunchKernelGGL(k_laswp<T>, dim3((N - (k1 - k0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv, strideP,
@@ -1129,18 +1208,30 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
       CHK(hipGetLastError());
     }
     if (k1 < N) {
-      CHK(launch_trsm<T>(mb, batch, k0, k1 - k0, k1, N - k1, skip, st));
+      if constexpr (sizeof(T) == 4) {
+        // U12 = L11^-1 A12 as an MFMA product with the inverted block, in place
+        GemmShape t = {0, 0, k0, k1, k0, k1, k1 - k0, N - k1, k1 - k0};
+        hipLaunchKernelGGL(k_gemm_tile32<true>, dim3((N - k1 + 63) / 64, 1, batch), dim3(256), 0, st, mb, t, skip,
+                           (const cplx<float>*)W + (size_t)(k0 / NBO) * 2 * 4096, 64LL, strideW);
+        CHK(hipGetLastError());
+      } else {
+        CHK(launch_trsm<T>(mb, batch, k0, k1 - k0, k1, N - k1, skip, st));
+      }
       GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, N - k1, k1 - k0};
       CHK(launch_gemm<T>(mb, batch, g, skip, st));
     }
   }
+  // inverses of the diagonal blocks of U for the backward sweeps
+  hipLaunchKernelGGL((k_tri_inv<T, false>), dim3((N + 63) / 64, batch), dim3(256), 0, st, mb, W, strideW, 0, skip);
+  CHK(hipGetLastError());
   return hipSuccess;
 }
 
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, long long strideP,
                           const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
-                          long long strideY, int nrhs, const int* skip, hipStream_t st) {
+                          long long strideY, int nrhs, const int* skip, const cplx<T>* W, hipStream_t st) {
+  const long long strideW = lu_workspace_elems(mb.N);
   const int N = mb.N;
   hipLaunchKernelGGL(k_permute_rhs<T>, dim3((N + 255) / 256, nrhs, batch), dim3(256), 0, st, b, ldb, strideB, y,
                      ldy, strideY, perm, strideP, N, nrhs, skip);
@@ -1150,13 +1241,13 @@ hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, lon
   for (int kb = -1; kb < nblk - 1; ++kb) {
     const int rows_after = nblk - 1 - kb;
     hipLaunchKernelGGL((k_trsv_step<T, true>), dim3(rows_after, batch), dim3(256), 0, st, mb, y, ldy, strideY,
-                       nrhs, kb, skip);
+                       nrhs, kb, skip, W, strideW);
     CHK(hipGetLastError());
   }
   // backward sweep: step kb updates blocks < kb and solves block kb-1
   for (int kb = nblk; kb > 0; --kb) {
     hipLaunchKernelGGL((k_trsv_step<T, false>), dim3(kb, batch), dim3(256), 0, st, mb, y, ldy, strideY, nrhs, kb,
-                       skip);
+                       skip, W, strideW);
     CHK(hipGetLastError());
   }
   return hipSuccess;
@@ -1197,15 +1288,15 @@ hipError_t growth_batch(const MatBatch<T>& mb, int batch, double* umax, const do
 }
 
 template hipError_t lu_factor_batch<float>(const MatBatch<float>&, int, int*, int*, long long, int*, const int*,
-                                           hipStream_t);
+                                           cplx<float>*, hipStream_t);
 template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, int*, long long, int*, const int*,
-                                            hipStream_t);
+                                            cplx<double>*, hipStream_t);
 template hipError_t lu_solve_batch<float>(const MatBatch<float>&, int, const int*, long long, const cplx<double>*,
                                           long long, long long, cplx<float>*, long long, long long, int, const int*,
-                                          hipStream_t);
+                                          const cplx<float>*, hipStream_t);
 template hipError_t lu_solve_batch<double>(const MatBatch<double>&, int, const int*, long long,
                                            const cplx<double>*, long long, long long, cplx<double>*, long long,
-                                           long long, int, const int*, hipStream_t);
+                                           long long, int, const int*, const cplx<double>*, hipStream_t);
 template hipError_t residual_batch<float>(const MatBatch<float>&, int, const cplx<double>*, const cplx<double>*,
                                           cplx<double>*, long long, long long, int, const int*, hipStream_t);
 template hipError_t residual_batch<double>(const MatBatch<double>&, int, const cplx<double>*,

@@ -151,27 +151,34 @@ int bem_assemble(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const doub
                  int n_ch, const void* rhs_scale, void* A, long long lda, long long strideA, void* A0, void* b,
                  long long ldb, long long strideB, double* rowmax);
 
+/* Bytes of the LU workspace for `batch` N x N matrices: the inverted 64 x 64 diagonal blocks of L and U
+ * that bem_lu_factor writes and bem_lu_solve / bem_refine read (cT [batch][ceil(N/64)][2][64][64]). */
+int bem_lu_workspace_bytes(int precision, int n, int batch, long long* bytes);
+
 /* In-place LU with partial pivoting of `batch` N x N cT matrices (P A = L U, LAPACK getrf semantics;
  * ipiv[c] = row exchanged with row c, perm = resulting row permutation; info[m] = 0 or first zero pivot + 1;
- * int arrays of stride strideP). Replaces rocm_dense_lu! (JS/BeatEngineRocmOperators.jl:161-194). */
+ * int arrays of stride strideP). `work` (bem_lu_workspace_bytes) receives the inverted diagonal blocks and
+ * travels with the factors. Replaces rocm_dense_lu! (JS/BeatEngineRocmOperators.jl:161-194). */
 int bem_lu_factor(bem_ctx* ctx, int precision, int n, int batch, void* A, long long lda, long long strideA,
-                  int* ipiv, int* perm, long long strideP, int* info);
+                  int* ipiv, int* perm, long long strideP, int* info, void* work);
 
 /* y = U^-1 L^-1 P b for nrhs right-hand sides (b c128 [batch][nrhs][ldb]; y cT [batch][nrhs][ldy]).
- * `skip` (device int [batch], may be NULL) skips matrices whose flag is set.
+ * `work` is the workspace bem_lu_factor filled for these factors. `skip` (device int [batch], may be
+ * NULL) skips matrices whose flag is set.
  * Replaces solve_rocm_dense_factorization (JS/BeatEngineRocmOperators.jl:196-222). */
 int bem_lu_solve(bem_ctx* ctx, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
-                 const int* perm, long long strideP, const void* b, long long ldb, long long strideB, void* y,
-                 long long ldy, long long strideY, int nrhs, const int* skip);
+                 const int* perm, long long strideP, const void* work, const void* b, long long ldb,
+                 long long strideB, void* y, long long ldy, long long strideY, int nrhs, const int* skip);
 
 /* Mixed-precision solve with FP64 iterative refinement (no reference counterpart; the reference solves
  * in FP32 once, JS/BeatEngineRocmOperators.jl:132-159): x (c128) = A^-1 b where A0 is the c64 matrix as
  * assembled and LU its c64 factorization. Iterates r = b - A0 x (FP64), d = LU^-1 r, x += d until
  * ||r_c|| / ||b_c|| <= tol for every channel or max_iter corrections. rel[m] receives the final relative
- * residual, iters[m] the corrections applied. work_r: c128 like b; work_y: c64 [batch][nrhs][ldb];
+ * residual, iters[m] the corrections applied. lu_work: the factorization's workspace (bem_lu_factor);
+ * work_r: c128 like b; work_y: c64 [batch][nrhs][ldb];
  * done: device int [batch] scratch. Works under graph capture (fixed launch count, device-side exit). */
 int bem_refine(bem_ctx* ctx, int n, int batch, const void* LU, long long lda, long long strideA, const int* perm,
-               long long strideP, const void* A0, const void* b, long long ldb, long long strideB, void* x,
+               long long strideP, const void* lu_work, const void* A0, const void* b, long long ldb, long long strideB, void* x,
                void* work_r, void* work_y, int nrhs, int max_iter, double tol, double* rel, int* iters, int* done);
 
 /* FP64 relative residual ||b_c - A x_c|| / ||b_c|| (max over channels) of a cT matrix and c128 x, b.

@@ -66,3 +66,13 @@ def test_mesh_desc_layout_matches_header():
     body = text[text.index("typedef struct bem_mesh_desc {") : text.index("} bem_mesh_desc;")]
     names = re.findall(r"\b([a-z_0-9]+)(?:\[\d+\])?;", body)
     assert names == [f[0] for f in _lib.MeshDesc._fields_]
+
+
+def test_lu_workspace_size():
+    # one inverted 64 x 64 block of L and one of U per 64-row block, complex in the factor's precision
+    from beat_hip import _lib
+
+    assert _lib.lu_workspace_bytes(_lib.BEM_PREC_MIXED, 3000, 5) == 47 * 2 * 4096 * 8 * 5
+    assert _lib.lu_workspace_bytes(_lib.BEM_PREC_FP64, 64, 1) == 2 * 4096 * 16
+    with pytest.raises(RuntimeError):
+        _lib.lu_workspace_bytes(7, 64, 1)

@@ -121,13 +121,14 @@ def test_lu_random(gpu_device, precision, n, batch):
         ipiv = torch.zeros((batch, n), dtype=torch.int32, device=dev)
         perm = torch.zeros((batch, n), dtype=torch.int32, device=dev)
         info = torch.zeros(batch, dtype=torch.int32, device=dev)
+        prec = PRECISIONS[precision]
+        work = torch.empty(_lib.lu_workspace_bytes(prec, n, batch), dtype=torch.uint8, device=dev)
     gpu_device.sync()
     lib = gpu_device.lib
-    prec = PRECISIONS[precision]
     _lib.check(lib.bem_lu_factor(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(ipiv), _ptr(perm), n,
-                                 _ptr(info)), "factor")
-    _lib.check(lib.bem_lu_solve(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(perm), n, _ptr(b), lda,
-                                2 * lda, _ptr(y), lda, 2 * lda, 2, None), "solve")
+                                 _ptr(info), _ptr(work)), "factor")
+    _lib.check(lib.bem_lu_solve(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(perm), n, _ptr(work),
+                                _ptr(b), lda, 2 * lda, _ptr(y), lda, 2 * lda, 2, None), "solve")
     gpu_device.sync()
     X = y[:, :, :n].cpu().numpy().astype(np.complex128)
     # forward error vs LAPACK, and the backward error |B - M X| / (|M|_2 |X|) that partial-pivoting LU bounds.
