@@ -43,10 +43,7 @@ class MeshDesc(C.Structure):
         ("slot_count", _ip),
         ("slot_start", _u16p),
         ("slot_inc", _u16p),
-        ("slot_base", _ip),
-        ("strip_len", _i),
-        ("cs_ptr", _ip),
-        ("cs_slot", _ip),
+        ("slot_vertex", _ip),
         ("vstar_ptr", _ip),
         ("vstar_row", _ip),
         ("n_pairs", _i),

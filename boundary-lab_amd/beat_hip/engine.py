@@ -99,10 +99,7 @@ class DeviceMesh:
         d.slot_count = _np_ptr(arr("sc", s.slot_count, np.int32), C.c_int)
         d.slot_start = _np_ptr(arr("ss", s.slot_start, np.uint16), C.c_uint16)
         d.slot_inc = _np_ptr(arr("si", s.slot_inc, np.uint16), C.c_uint16)
-        d.slot_base = _np_ptr(arr("sb", s.slot_base, np.int32), C.c_int)
-        d.strip_len = s.strip_len
-        d.cs_ptr = _np_ptr(arr("csp", s.cs_ptr, np.int32), C.c_int)
-        d.cs_slot = _np_ptr(arr("css", s.cs_slot, np.int32), C.c_int)
+        d.slot_vertex = _np_ptr(arr("sv", s.slot_vertex, np.int32), C.c_int)
         d.vstar_ptr = _np_ptr(arr("vsp", s.vstar_ptr, np.int32), C.c_int)
         d.vstar_row = _np_ptr(arr("vsr", s.vstar_row, np.int32), C.c_int)
         p = prep.pairs

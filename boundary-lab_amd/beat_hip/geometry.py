@@ -11,9 +11,10 @@ Restates the reference's mesh / space / quadrature set-up in the layout the gfx9
 * mass blocks                BeatEngineCore.jl:731-749
 
 New for the HIP layout: trial elements are put in Morton (Z-curve) order of their centroids and cut into
-chunks of 64 (one lane each). Each chunk's distinct vertices are "slots"; the regular kernel writes one value
-per (test-element row, slot) and the row gather maps slots back to P1 columns (``cs_ptr/cs_slot``) and P1
-rows back to test-element rows (``vstar_ptr/vstar_row``).
+chunks of 64 (one lane each). Each chunk's distinct vertices are "slots"; the regular kernel adds each slot's
+value into that vertex's column of the test-element strip row (``slot_vertex``: the first chunk holding a vertex
+stores, later chunks accumulate), and the row gather sums the strip rows of each P1 row's star
+(``vstar_ptr/vstar_row``).
 """
 
 from __future__ import annotations
@@ -446,10 +447,7 @@ class SlotTables:
     slot_count: np.ndarray
     slot_start: np.ndarray  # (n_chunks, 196) uint16
     slot_inc: np.ndarray  # (n_chunks, 192) uint16
-    slot_base: np.ndarray
-    strip_len: int
-    cs_ptr: np.ndarray
-    cs_slot: np.ndarray
+    slot_vertex: np.ndarray  # (n_chunks, 192) int32: 2 * vertex + (an earlier chunk holds the vertex)
     vstar_ptr: np.ndarray
     vstar_row: np.ndarray
 
@@ -464,9 +462,8 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
     slot_count = np.zeros(n_chunks, dtype=np.int32)
     slot_start = np.zeros((n_chunks, 196), dtype=np.uint16)
     slot_inc = np.zeros((n_chunks, 192), dtype=np.uint16)
-    slot_base = np.zeros(n_chunks, dtype=np.int32)
-    slot_vertex = []
-    base = 0
+    slot_vertex = np.zeros((n_chunks, 192), dtype=np.int32)
+    seen = np.zeros(V, dtype=bool)
     for c in range(n_chunks):
         elems = order[c * 64 : (c + 1) * 64]
         valid = elems >= 0
@@ -479,15 +476,8 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
         cnt = np.bincount(inv, minlength=len(uniq))
         slot_start[c, 1 : len(uniq) + 1] = np.cumsum(cnt)
         slot_count[c] = len(uniq)
-        slot_base[c] = base
-        slot_vertex.append(uniq)
-        base += len(uniq)
-    strip_len = ((base + 63) // 64) * 64
-    sv = np.concatenate(slot_vertex)
-    gs = np.arange(len(sv))
-    o = np.lexsort((gs, sv))
-    cs_ptr = np.zeros(V + 1, dtype=np.int64)
-    np.add.at(cs_ptr, sv + 1, 1)
+        slot_vertex[c, : len(uniq)] = 2 * uniq + seen[uniq]
+        seen[uniq] = True
     rows = np.arange(3 * F)
     rv = mesh.faces.reshape(-1)
     o2 = np.lexsort((rows, rv))
@@ -498,10 +488,7 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
         slot_count=slot_count,
         slot_start=slot_start,
         slot_inc=slot_inc,
-        slot_base=slot_base,
-        strip_len=int(strip_len),
-        cs_ptr=np.cumsum(cs_ptr).astype(np.int32),
-        cs_slot=gs[o].astype(np.int32),
+        slot_vertex=slot_vertex,
         vstar_ptr=np.cumsum(vs_ptr).astype(np.int32),
         vstar_row=rows[o2].astype(np.int32),
     )

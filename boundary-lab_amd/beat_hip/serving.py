@@ -28,12 +28,14 @@ from . import driver as D
 from . import geometry as G
 
 
-def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, strip_len: int, batch: int, n_obs: int,
+def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, batch: int, n_obs: int,
                        precision: str = "mixed", n_ch: int = 1) -> int:
     """Device bytes a job holds while resident (matrices dominate: 2 x N^2 per frequency in the batch)."""
     cb = 8 if precision == "mixed" else 16
     lda = ((n_vertices + 15) // 16) * 16
-    matrices = batch * 2 * n_vertices * lda * cb
+    strip_len = ((n_vertices + 63) // 64) * 64  # strip rows hold one column per vertex
+    lu_work = batch * ((n_vertices + 63) // 64) * 2 * 4096 * cb  # inverted diagonal blocks
+    matrices = batch * 2 * n_vertices * lda * cb + lu_work
     vectors = batch * 8 * n_ch * lda * 16
     strip = min(batch, 4) * 3 * n_faces * strip_len * cb
     singular = batch * n_pairs * 12 * 16
@@ -100,9 +102,8 @@ class Scheduler:
             h.t_done = time.perf_counter()
             return h, init
         nb = max(1, min(self.batch, len(job.frequencies)))
-        h.footprint = estimate_footprint(job.mesh.n_vertices, job.mesh.n_faces, len(prep.pairs.test),
-                                         prep.slots.strip_len, nb, len(job.obs), job.precision,
-                                         len(job.channel_names))
+        h.footprint = estimate_footprint(job.mesh.n_vertices, job.mesh.n_faces, len(prep.pairs.test), nb,
+                                         len(job.obs), job.precision, len(job.channel_names))
         with self._lock:
             self._pending.append((h, job, prep))
             self._lock.notify_all()

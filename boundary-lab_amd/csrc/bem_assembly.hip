@@ -16,11 +16,12 @@
 //                          64. Each lane integrates its (t,s) pair with the 6x6 order-4 rule into a 3x3 block
 //                          (or picks up the FP64 singular block for adjacent pairs). Blocks go to LDS, then lanes
 //                          switch to "slot" ownership: each slot (a trial vertex of the chunk) sums the block
-//                          columns that touch it, in a fixed order, and stores one value per test-element row.
-//                          Every strip entry is written exactly once: no atomics, no colouring, deterministic.
+//                          columns that touch it, in a fixed order, into the vertex's column of the three
+//                          test-element strip rows (stored by its first chunk, accumulated by later ones).
+//                          Only this wave touches its strip rows: no atomics, no colouring, deterministic.
 //   k_rhs_part    (T)      one wave per test element: sum over radiating faces of the RHS-operator blocks times
 //                          the unit channel drives.
-//   k_gather      (T)      one block per P1 row i: A[i, j] = sum over (t,l) in star(i), slots of column j.
+//   k_gather      (T)      one block per P1 row i: A[i, j] = sum over (t,l) in star(i) of strip[3t+l][j].
 //
 // Regular-pair arithmetic follows BeatEngineCpuAssembly.jl:278-373 / BeatEngineRocmPairKernels.jl:100-150:
 //   G = e^{ikr}/(4 pi r), grad = G (ik - 1/r) r_hat, D uses the trial normal, D* uses -r.n_test,
@@ -258,6 +259,22 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
 
   for (int c = 0; c < p.n_chunks; ++c) {
     const int s = p.order[c * 64 + lane];
+    // Strip columns of vertices that an earlier chunk already stored are read-modify-written below; the
+    // first slot of each lane is loaded here, so the load overlaps the pair integrals of this chunk.
+    const int nsl = p.slot_count[c];
+    const int* sv = p.slot_vertex + (size_t)c * 192;
+    const int vc0 = lane < nsl ? sv[lane] : 0;
+    cplx<T> pre[NFW][3];
+#pragma unroll
+    for (int j = 0; j < NFW; ++j) {
+      pre[j][0] = pre[j][1] = pre[j][2] = mk<T>(0, 0);
+      if ((vc0 & 1) && j < nfw) {
+        const cplx<T>* rows = p.strip + (size_t)(f0 + j) * p.strip_fstride + (size_t)(3 * t) * p.L + (vc0 >> 1);
+        pre[j][0] = rows[0];
+        pre[j][1] = rows[p.L];
+        pre[j][2] = rows[2 * (size_t)p.L];
+      }
+    }
     cplx<T> C[NFW][9];
 #pragma unroll
     for (int j = 0; j < NFW; ++j)
@@ -305,12 +322,14 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) blk[j][lane][i] = C[j][i];
     __syncthreads();
-    const int nsl = p.slot_count[c];
     const uint16_t* st = p.slot_start + (size_t)c * SLOT_STRIDE;
     const uint16_t* inc = p.slot_inc + (size_t)c * 192;
-    const int base = p.slot_base[c];
     for (int kk = lane; kk < nsl; kk += 64) {
       const int e0 = st[kk], e1 = st[kk + 1];
+      const bool first_slot = kk == lane;
+      const int vcode = first_slot ? vc0 : sv[kk];
+      const int v = vcode >> 1;
+      const bool again = vcode & 1;  // an earlier chunk of this wave already stored this vertex column
 #pragma unroll
       for (int j = 0; j < NFW; ++j) {
         if (j < nfw) {
@@ -322,13 +341,23 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
             a1 = cadd(a1, blk[j][src][3 + m]);
             a2 = cadd(a2, blk[j][src][6 + m]);
           }
-          cplx<T>* rows = p.strip + (size_t)(f0 + j) * p.strip_fstride + (size_t)(3 * t) * p.L + base + kk;
+          cplx<T>* rows = p.strip + (size_t)(f0 + j) * p.strip_fstride + (size_t)(3 * t) * p.L + v;
+          if (again && first_slot) {
+            a0 = cadd(a0, pre[j][0]);
+            a1 = cadd(a1, pre[j][1]);
+            a2 = cadd(a2, pre[j][2]);
+          } else if (again) {
+            a0 = cadd(a0, rows[0]);
+            a1 = cadd(a1, rows[p.L]);
+            a2 = cadd(a2, rows[2 * (size_t)p.L]);
+          }
           rows[0] = a0;
           rows[p.L] = a1;
           rows[2 * (size_t)p.L] = a2;
         }
       }
     }
+    // the barrier orders this chunk's strip stores before the next chunk's read-modify-write of a shared vertex
     __syncthreads();
   }
 }
@@ -366,14 +395,24 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
     nt[c] = p.fnrm[3 * t + c];
     ns[c] = sg[c] * p.fnrm[3 * s + c];
   }
-  double Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
+  double Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
   double Dr[9], Di[9], Gr[9], Gi[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Dr[i] = Di[i] = Gr[i] = Gi[i] = 0.0;
   const int off = p.rule_off[ru], len = p.rule_len[ru];
   const double* rp = p.rpts + (size_t)off * 5;
-  for (int q = 0; q < len; ++q, rp += 5) {
-    const double txi = rp[0], tet = rp[1], sxi = rp[2], set = rp[3], w = rp[4];
+  // the next rule point is loaded while the current one is integrated
+  double nxt[5] = {0, 0, 0, 0, 0};
+  if (len > 0)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) nxt[i] = rp[i];
+  for (int q = 0; q < len; ++q) {
+    const double txi = nxt[0], tet = nxt[1], sxi = nxt[2], set = nxt[3], w = nxt[4];
+    if (q + 1 < len) {
+      rp += 5;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) nxt[i] = rp[i];
+    }
     const double dx = (s0[0] + sxi * se1[0] + set * se2[0]) - (t0[0] + txi * te1[0] + tet * te2[0]);
     const double dy = (s0[1] + sxi * se1[1] + set * se2[1]) - (t0[1] + txi * te1[1] + tet * te2[1]);
     const double dz = (s0[2] + sxi * se1[2] + set * se2[2]) - (t0[2] + txi * te1[2] + tet * te2[2]);
@@ -395,8 +434,6 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
     const double dgr = hr * td, dgi = hi * td;
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-      Sr[l] += bt[l] * gr;
-      Si[l] += bt[l] * gi;
       Ar[l] += bt[l] * hr * ad;
       Ai[l] += bt[l] * hi * ad;
 #pragma unroll
@@ -441,9 +478,12 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
   cplx<double>* ro = p.rhs + ((size_t)f * p.P + pi) * 3;
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
-    // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair
-    const double re = scale * (-Sr[l] + Ai[l] * kinv);
-    double im = scale * (-Si[l] - Ar[l] * kinv);
+    // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair. S_l = sum_m G_lm: the trial
+    // barycentric weights sum to one at every point.
+    const double Sr = Gr[3 * l] + Gr[3 * l + 1] + Gr[3 * l + 2];
+    const double Si = Gi[3 * l] + Gi[3 * l + 1] + Gi[3 * l + 2];
+    const double re = scale * (-Sr + Ai[l] * kinv);
+    double im = scale * (-Si - Ar[l] * kinv);
     if (self) im -= 0.5 * kinv * p.m10[l] * a2;
     ro[l] = mk<double>(re, im);
   }
@@ -569,7 +609,7 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Row gather: A[i][j] = sum_{(t,l) in star(i)} sum_{slot in cols(j)} strip[3t+l][slot]
+// Row gather: A[i][j] = sum_{(t,l) in star(i)} strip[3t+l][j]
 //             b[c][i] = scale_f * sum_{(t,l) in star(i)} rhs_part[3t+l][c]
 // ------------------------------------------------------------------------------------------------
 
@@ -590,13 +630,10 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
   cplx<T>* A0r = g.A0 ? g.A0 + (size_t)f * g.strideA + (size_t)i * g.lda : nullptr;
   const double wi = g.roww ? g.roww[i] : 1.0;  // P1 orbit row weight (plane symmetry)
   double rmax = 0.0;
+  // strip rows are vertex-indexed: row i of A is the plain sum of the strip rows of its star (coalesced)
   for (int j = tid; j < g.N; j += 256) {
     cplx<T> acc = mk<T>(0, 0);
-    const int e1 = g.cs_ptr[j + 1];
-    for (int e = g.cs_ptr[j]; e < e1; ++e) {
-      const int slot = g.cs_slot[e];
-      for (int r = 0; r < ns; ++r) acc = cadd(acc, strip[(size_t)rows[r] * g.L + slot]);
-    }
+    for (int r = 0; r < ns; ++r) acc = cadd(acc, strip[(size_t)rows[r] * g.L + j]);
     if (g.roww) acc = mk<T>(acc.re * T(wi), acc.im * T(wi));
     Ar[j] = acc;
     if (A0r) A0r[j] = acc;

@@ -46,6 +46,7 @@ struct bem_ctx {
   hipStream_t aux[BEM_AUX_STREAMS];
   hipEvent_t fork_ev, join_ev[BEM_AUX_STREAMS];
   int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
+  hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
 };
 
 // Run body(first, count, stream) on up to 1 + BEM_AUX_STREAMS contiguous sub-batches, forked from and joined
@@ -114,7 +115,7 @@ struct bem_mesh {
   float* srcw32;
   double* srcw64;
   int4* fv;
-  int *order, *slot_count, *slot_base, *cs_ptr, *cs_slot, *vstar_ptr, *vstar_row;
+  int *order, *slot_count, *slot_vertex, *vstar_ptr, *vstar_row;
   uint16_t *slot_start, *slot_inc;
   double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
   int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces, *sing_by_rule;
@@ -196,6 +197,11 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
       return fail(BEM_E_HIP, "auxiliary stream creation failed");
     }
   }
+  for (int i = 0; i < 2; ++i)
+    if (hipEventCreateWithFlags(&c->lu_ev[i], hipEventDisableTiming) != hipSuccess) {
+      delete c;
+      return fail(BEM_E_HIP, "hipEventCreate failed");
+    }
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
     c->own_stream = false;
@@ -218,6 +224,7 @@ int bem_ctx_destroy(bem_ctx* c) {
     (void)hipStreamDestroy(c->aux[i]);
     (void)hipEventDestroy(c->join_ev[i]);
   }
+  for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c->lu_ev[i]);
   (void)hipEventDestroy(c->fork_ev);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -283,7 +290,7 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   m->F = F;
   m->V = V;
   m->n_chunks = d->n_chunks;
-  m->L = d->strip_len;
+  m->L = (V + 63) / 64 * 64;  // strip rows hold one column per vertex
   m->P = d->n_pairs;
   m->n_rules = d->n_rules;
   m->n_src = d->n_src;
@@ -362,7 +369,6 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   int rc = BEM_OK;
   size_t total_rule_pts = 0;
   for (int r = 0; r < d->n_rules; ++r) total_rule_pts = std::max(total_rule_pts, (size_t)(d->rule_off[r] + d->rule_len[r]));
-  size_t cs_nnz = (size_t)d->cs_ptr[V];
   size_t vs_nnz = (size_t)d->vstar_ptr[V];
 #define UP(dst, src, n)                      \
   if ((rc = upload(m, &(dst), (src), (n)))) { \
@@ -380,9 +386,7 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   UP(m->slot_count, d->slot_count, (size_t)d->n_chunks);
   UP(m->slot_start, d->slot_start, (size_t)d->n_chunks * 196);
   UP(m->slot_inc, d->slot_inc, (size_t)d->n_chunks * 192);
-  UP(m->slot_base, d->slot_base, (size_t)d->n_chunks);
-  UP(m->cs_ptr, d->cs_ptr, (size_t)V + 1);
-  UP(m->cs_slot, d->cs_slot, cs_nnz);
+  UP(m->slot_vertex, d->slot_vertex, (size_t)d->n_chunks * 192);
   UP(m->vstar_ptr, d->vstar_ptr, (size_t)V + 1);
   UP(m->vstar_row, d->vstar_row, vs_nnz);
   UP(m->fvtx, d->face_vertices, (size_t)F * 9);
@@ -534,7 +538,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.slot_count = m->slot_count;
     p.slot_start = m->slot_start;
     p.slot_inc = m->slot_inc;
-    p.slot_base = m->slot_base;
+    p.slot_vertex = m->slot_vertex;
     p.sing_ptr = m->sing_ptr;
     p.sing_trial = m->sing_trial;
     p.sing_lhs = static_cast<const cplx<double>*>(m->sing_lhs.p) + (size_t)f0 * m->P * 9;
@@ -557,8 +561,6 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     gp.L = m->L;
     gp.vstar_ptr = m->vstar_ptr;
     gp.vstar_row = m->vstar_row;
-    gp.cs_ptr = m->cs_ptr;
-    gp.cs_slot = m->cs_slot;
     gp.rhs_part = static_cast<const cplx<double>*>(m->rhs_part.p) + (size_t)f0 * 3 * m->F * BEM_MAX_CHANNELS;
     gp.rhs_scale = static_cast<const cplx<double>*>(rhs_scale) + f0;
     gp.A = static_cast<cplx<T>*>(A) + (size_t)f0 * strideA;
@@ -610,26 +612,23 @@ int bem_lu_workspace_bytes(int precision, int n, int batch, long long* bytes) {
 int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long long lda, long long strideA,
                   int* ipiv, int* perm, long long strideP, int* info, void* work) {
   if (!c || !A || !ipiv || !perm || !info || !work) return fail(BEM_E_ARG, "NULL argument");
-  const long long sw = lu_workspace_elems(n);
   if (n <= 0 || batch <= 0 || lda < n || strideA < lda * n || strideP < n) return fail(BEM_E_ARG, "invalid sizes");
   if (n > 12288) return fail(BEM_E_ARG, "N above 12288 is not supported by the register panel");
-  hipError_t e;
+  if (precision != BEM_PREC_MIXED && precision != BEM_PREC_FP64) return fail(BEM_E_ARG, "unknown precision");
   ScopedTimer tm(c, BEM_TIMER_LU, (long long)batch);
-  // matrices are independent: sub-batches of at least 4 factor on separate streams
+  // One factorization of the whole batch with look-ahead: the context stream runs the panel chain, the first
+  // auxiliary stream the large trailing updates (fork and join with the context stream only, as everywhere
+  // else, so the sequence is capturable). Fan-out 1 runs everything on the context stream.
+  const LuLookahead la = {c->fanout >= 2 ? c->aux[0] : nullptr, c->lu_ev[0], c->lu_ev[1]};
+  hipError_t e;
   if (precision == BEM_PREC_MIXED) {
-    e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) {
-      MatBatch<float> mb = {static_cast<cplx<float>*>(A) + (size_t)b0 * strideA, lda, strideA, n};
-      return lu_factor_batch<float>(mb, nb, ipiv + (size_t)b0 * strideP, perm + (size_t)b0 * strideP, strideP,
-                                    info + b0, nullptr, static_cast<cplx<float>*>(work) + (size_t)b0 * sw, st);
-    });
-  } else if (precision == BEM_PREC_FP64) {
-    e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) {
-      MatBatch<double> mb = {static_cast<cplx<double>*>(A) + (size_t)b0 * strideA, lda, strideA, n};
-      return lu_factor_batch<double>(mb, nb, ipiv + (size_t)b0 * strideP, perm + (size_t)b0 * strideP, strideP,
-                                     info + b0, nullptr, static_cast<cplx<double>*>(work) + (size_t)b0 * sw, st);
-    });
+    MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
+    e = lu_factor_batch<float>(mb, batch, ipiv, perm, strideP, info, nullptr, static_cast<cplx<float>*>(work),
+                               c->stream, &la);
   } else {
-    return fail(BEM_E_ARG, "unknown precision");
+    MatBatch<double> mb = {static_cast<cplx<double>*>(A), lda, strideA, n};
+    e = lu_factor_batch<double>(mb, batch, ipiv, perm, strideP, info, nullptr, static_cast<cplx<double>*>(work),
+                                c->stream, &la);
   }
   if (e != hipSuccess) return hipfail(e, "lu_factor launch");
   return BEM_OK;

@@ -33,12 +33,12 @@ struct RegParams {
   const int* slot_count;        // [n_chunks]
   const uint16_t* slot_start;   // [n_chunks][BEM_SLOT_STRIDE]
   const uint16_t* slot_inc;     // [n_chunks][BEM_SLOT_INC]  (lane << 2) | m
-  const int* slot_base;         // [n_chunks]
+  const int* slot_vertex;       // [n_chunks][BEM_SLOT_INC]  2 * vertex + (earlier chunk holds the vertex)
   const int* sing_ptr;          // [F+1]
   const int* sing_trial;        // [P]
   const cplx<double>* sing_lhs; // [nf][P][9]
   const double* k;              // [nf]
-  cplx<T>* strip;               // [nf][3F][L]
+  cplx<T>* strip;               // [nf][3F][L], L = V padded to 64: one column per vertex
   long long strip_fstride;
   int L, F, n_chunks, P;
   int R;                        // transforms (identity + images); rec holds R*F records, image r at r*F
@@ -90,8 +90,6 @@ struct GatherParams {
   int L;
   const int* vstar_ptr;           // [N+1]
   const int* vstar_row;           // strip rows 3t+l
-  const int* cs_ptr;              // [N+1]
-  const int* cs_slot;
   const cplx<double>* rhs_part;   // [nf][3F][BEM_MAX_CH]
   const cplx<double>* rhs_scale;  // [nf]  (i rho omega)
   cplx<T>* A;                     // [nf] row-major N x lda
@@ -121,9 +119,15 @@ hipError_t launch_gather(const GatherParams<T>&, int nf, hipStream_t);
 
 // elements (cT) of LU workspace per matrix: inverted 64 x 64 diagonal blocks of L and U
 long long lu_workspace_elems(int N);
+// Second stream for the look-ahead factorization (stream null: everything on the main stream). `ready` and
+// `done` are events private to this factorization (recorded on the main / look-ahead stream).
+struct LuLookahead {
+  hipStream_t stream;
+  hipEvent_t ready, done;
+};
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, cplx<T>* W, hipStream_t);
+                           const int* skip, cplx<T>* W, hipStream_t, const LuLookahead* la);
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>&, int batch, const int* perm, long long strideP, const cplx<double>* b,
                           long long ldb, long long strideB, cplx<T>* y, long long ldy, long long strideY, int nrhs,

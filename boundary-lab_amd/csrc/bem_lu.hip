@@ -10,20 +10,26 @@
 // chosen with |re|+|im| (i?amax), ties to the lowest row; ipiv[c] is the row swapped with row c. A running
 // permutation perm (perm[i] = original row now at position i) is kept for O(N) RHS permutation.
 //
-// Right-looking at the outer level (panel width NBO = 64, trailing update C -= L21 U12 on MFMA), left-looking
-// inside the panel: sub-panels of NBI columns are factored by one 1024-thread workgroup that holds the
-// sub-panel in registers (k_lu_panel), after a top TRSM and an MFMA update with the sub-panels already done.
+// Three levels. Right-looking over trailing blocks of NBT = 128 columns (trailing update C -= L21 U12 with
+// K = 128 on MFMA). Each trailing block is two halves of NBO = 64: the second half is brought up to date with
+// the first by an MFMA product (K = 64) before it is factored. Left-looking inside a half: sub-panels of NBI
+// columns are factored by one 1024-thread workgroup that holds the sub-panel in registers (k_lu_panel), after
+// a top TRSM and an update with the sub-panels of the half already done. Row interchanges are applied inside
+// the trailing block by the panel kernel and outside it by k_laswp once the block is factored. The 64 x 64
+// diagonal blocks of L and U are inverted into a workspace; the U12 solves and the diagonal steps of the
+// triangular sweeps multiply by those inverses.
 //
 // Complex GEMM on real MFMA: C (M x 2Nc reals, interleaved) = A (M x 2K reals) * B'' where B'' is the 2x2 real
 // expansion [[br, bi], [-bi, br]] of each complex entry of B, formed on the fly from B in LDS. FP32 uses
-// v_mfma_f32_16x16x4f32 (FP32 operands, FP32 accumulate); FP64 uses v_mfma_f64_16x16x4f64.
+// v_mfma_f32_32x32x2f32 (k_gemm_tile32), FP64 v_mfma_f64_16x16x4f64 (k_gemm_sub).
 
 #include "bem_kernels.hpp"
 
 #define NBO BEM_NBO
 #define MAX_CH BEM_MAX_CH
+constexpr int NBT = 2 * NBO;  // trailing block: two halves
 
-static_assert(NBO == 64, "the inverted diagonal blocks of the LU workspace are one outer panel wide");
+static_assert(NBO == 64, "the inverted diagonal blocks of the LU workspace are one half block wide");
 long long lu_workspace_elems(int N) { return (long long)((N + 63) / 64) * 2 * 4096; }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -546,8 +552,8 @@ __device__ int compose_swaps(const int* piv, int c0, int nb, int* rows, int* src
 
 // ------------------------------------------------------------------------------------------------
 // Sub-panel factorization in registers: rows [c0, N), columns [c0, c0+nb), nb <= NBI. The row interchanges
-// are applied to the other columns of the outer panel [k0, k1) here; columns outside the outer panel are
-// permuted once per outer panel by k_laswp.
+// are applied to the other columns of the trailing block [k0, k1) here; columns outside the block are
+// permuted once per block by k_laswp.
 // ------------------------------------------------------------------------------------------------
 
 template <typename T, int NBI, int MR>
@@ -690,10 +696,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     if (lane == 0 && zero_col != 0 && info[mat] == 0) info[mat] = zero_col;
   }
   __syncthreads();
-  // interchanges on the other columns of the outer panel: one (column, row) pair per thread, all loads first
+  // interchanges on the other columns of the trailing block [k0, k1): one (column, row) pair per thread, all
+  // loads first
   const int na = s_na;
   const int ncols = (k1 - k0) - nb;
-  constexpr int PER = (2 * NBI * BEM_NBO + 1023) / 1024;
+  constexpr int PER = (2 * NBI * NBT + 1023) / 1024;
   T vr[PER], vi[PER];  // scalar arrays stay in registers
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
@@ -723,12 +730,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   }
 }
 
-// Row interchanges of one outer panel (pivots of rows [k0, k1)) on the columns outside it. Each workgroup
-// resolves the composite permutation in LDS, stages the touched rows of its column block in LDS and writes
-// them back permuted.
+// Row interchanges of the pivots of rows [p0, p1) (at most NBO of them) on the columns outside the trailing
+// block [k0, k1). Each workgroup resolves the composite permutation in LDS, stages the touched rows of its
+// column block in LDS and writes them back permuted.
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, long long strideP, int k0, int k1,
-                                               const int* skip) {
+__global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, long long strideP, int p0, int p1,
+                                               int k0, int k1, const int* skip) {
   constexpr int CB = sizeof(T) == 4 ? 64 : 32;  // columns per workgroup
   const int mat = blockIdx.y;
   if (skip && skip[mat]) return;
@@ -742,11 +749,11 @@ __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, 
   __shared__ int s_src[2 * BEM_NBO];
   __shared__ int s_na;
   __shared__ cplx<T> buf[2 * BEM_NBO][CB];
-  const int nb = k1 - k0;
-  if (tid < nb) s_piv[tid] = piv[k0 + tid];
+  const int np = p1 - p0, nb = k1 - k0;
+  if (tid < np) s_piv[tid] = piv[p0 + tid];
   __syncthreads();
   if (w == 0) {
-    const int na = compose_swaps(s_piv, k0, nb, s_rows, s_src, lane);
+    const int na = compose_swaps(s_piv, p0, np, s_rows, s_src, lane);
     if (lane == 0) s_na = na;
   }
   __syncthreads();
@@ -1151,11 +1158,9 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
     return hipGetLastError();
   }
   if constexpr (sizeof(T) == 4) {
-    if (g.K <= BEM_NBO) {
-      hipLaunchKernelGGL(k_gemm_tile32<false>, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st,
-                         mb, g, skip, nullptr, 0LL, 0LL);
-      return hipGetLastError();
-    }
+    hipLaunchKernelGGL(k_gemm_tile32<false>, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st,
+                       mb, g, skip, nullptr, 0LL, 0LL);
+    return hipGetLastError();
   }
   dim3 grid((g.Nc + 63) / 64, (g.M + 63) / 64, batch);
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, mb, g, skip);
@@ -1178,49 +1183,110 @@ static hipError_t launch_trsm(const MatBatch<T>& mb, int batch, int r0, int nb, 
     if (e_ != hipSuccess) return e_;      \
   } while (0)
 
+// X = L^-1 X for the rows [r0, r0 + 64) of half block r0 / 64 (L its unit-lower diagonal block) and the ncols
+// columns starting at col0. FP32: one MFMA product with the inverted diagonal block (k_gemm_tile32 in assign
+// mode, in place); FP64: substitution (k_trsm_lower).
+template <typename T>
+static hipError_t solve_upper_rows(const MatBatch<T>& mb, int batch, int r0, int col0, int ncols, cplx<T>* W,
+                                   long long strideW, const int* skip, hipStream_t st) {
+  const int nr = min(NBO, mb.N - r0);
+  if (ncols <= 0 || nr <= 0) return hipSuccess;
+  if constexpr (sizeof(T) == 4) {
+    GemmShape t = {0, 0, r0, col0, r0, col0, nr, ncols, nr};
+    hipLaunchKernelGGL(k_gemm_tile32<true>, dim3((ncols + 63) / 64, 1, batch), dim3(256), 0, st, mb, t, skip,
+                       (const cplx<float>*)W + (size_t)(r0 / NBO) * 2 * 4096, 64LL, strideW);
+    return hipGetLastError();
+  } else {
+    (void)W;
+    (void)strideW;
+    return launch_trsm<T>(mb, batch, r0, nr, col0, ncols, skip, st);
+  }
+}
+
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, cplx<T>* W, hipStream_t st) {
+                           const int* skip, cplx<T>* W, hipStream_t st, const LuLookahead* la) {
   const int N = mb.N;
   const long long strideW = lu_workspace_elems(N);
+  const bool look = la && la->stream;
   hipLaunchKernelGGL(k_iota, dim3((N + 255) / 256, batch), dim3(256), 0, st, perm, strideP, N, info);
   CHK(hipGetLastError());
-  for (int k0 = 0; k0 < N; k0 += NBO) {
-    const int k1 = min(N, k0 + NBO);
-    int c0 = k0;
-    while (c0 < k1) {
-      const int nb = min(panel_width<T>(N, c0), k1 - c0);
-      if (c0 > k0) {
-        CHK(launch_trsm<T>(mb, batch, k0, c0 - k0, c0, nb, skip, st));
-        GemmShape g = {c0, k0, k0, c0, c0, c0, N - c0, nb, c0 - k0};
+
+  // factor the trailing block [k0, k1) whose columns are up to date: two halves, sub-panels in registers
+  auto factor_block = [&](int k0, int k1) -> hipError_t {
+    for (int h0 = k0; h0 < k1; h0 += NBO) {
+      const int h1 = min(k1, h0 + NBO);
+      if (h0 > k0) {
+        // second half: rows [k0, h0) become U with the first half's L, the rows below take the K = 64 update
+        CHK(solve_upper_rows(mb, batch, k0, h0, h1 - h0, W, strideW, skip, st));
+        GemmShape g = {h0, k0, k0, h0, h0, h0, N - h0, h1 - h0, h0 - k0};
         CHK(launch_gemm<T>(mb, batch, g, skip, st));
       }
-      CHK(launch_panel<T>(mb, batch, c0, nb, k0, k1, ipiv, perm, strideP, info, skip, st));
-      c0 += nb;
-    }
-    // L11 of this outer panel is final (later interchanges touch rows >= k1 only): invert it
-    hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(1, batch), dim3(256), 0, st, mb, W, strideW, k0 / NBO, skip);
-    CHK(hipGetLastError());
-    if (N > k1 - k0) {
-      constexpr int CB = sizeof(T) == 4 ? 64 : 32;
-      hipLaunchKernelGGL(k_laswp<T>, dim3((N - (k1 - k0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv, strideP,
-                         k0, k1, skip);
+      int c0 = h0;
+      while (c0 < h1) {
+        const int nb = min(panel_width<T>(N, c0), h1 - c0);
+        if (c0 > h0) {
+          CHK(launch_trsm<T>(mb, batch, h0, c0 - h0, c0, nb, skip, st));
+          GemmShape g = {c0, h0, h0, c0, c0, c0, N - c0, nb, c0 - h0};
+          CHK(launch_gemm<T>(mb, batch, g, skip, st));
+        }
+        CHK(launch_panel<T>(mb, batch, c0, nb, k0, k1, ipiv, perm, strideP, info, skip, st));
+        c0 += nb;
+      }
+      // the diagonal block of this half is final (later interchanges touch rows >= h1 only): invert its L
+      hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(1, batch), dim3(256), 0, st, mb, W, strideW, h0 / NBO, skip);
       CHK(hipGetLastError());
     }
-    if (k1 < N) {
-      if constexpr (sizeof(T) == 4) {
-        // U12 = L11^-1 A12 as an MFMA product with the inverted block, in place
-        GemmShape t = {0, 0, k0, k1, k0, k1, k1 - k0, N - k1, k1 - k0};
-        hipLaunchKernelGGL(k_gemm_tile32<true>, dim3((N - k1 + 63) / 64, 1, batch), dim3(256), 0, st, mb, t, skip,
-                           (const cplx<float>*)W + (size_t)(k0 / NBO) * 2 * 4096, 64LL, strideW);
+    return hipSuccess;
+  };
+
+  // Look-ahead: once block j is factored and its U12 formed, the main stream updates only the next block's
+  // columns and factors that block (a chain of latency-bound launches), while the look-ahead stream applies
+  // block j to the columns beyond it (the large MFMA update). The next interchanges wait for that update.
+  bool pending = false;  // a look-ahead update is in flight
+  CHK(factor_block(0, min(N, NBT)));
+  for (int k0 = 0; k0 < N; k0 += NBT) {
+    const int k1 = min(N, k0 + NBT);
+    if (pending) {
+      CHK(hipStreamWaitEvent(st, la->done, 0));
+      pending = false;
+    }
+    // interchanges of the block's pivots on the columns outside it, one half at a time (in pivot order)
+    if (N > k1 - k0) {
+      constexpr int CB = sizeof(T) == 4 ? 64 : 32;
+      for (int h0 = k0; h0 < k1; h0 += NBO) {
+        hipLaunchKernelGGL(k_laswp<T>, dim3((N - (k1 - k0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv,
+                           strideP, h0, min(k1, h0 + NBO), k0, k1, skip);
         CHK(hipGetLastError());
-      } else {
-        CHK(launch_trsm<T>(mb, batch, k0, k1 - k0, k1, N - k1, skip, st));
       }
+    }
+    if (k1 >= N) break;
+    // U12 = L11^-1 A12 by halves
+    CHK(solve_upper_rows(mb, batch, k0, k1, N - k1, W, strideW, skip, st));
+    if (k1 - k0 > NBO) {
+      const int h0 = k0 + NBO;
+      GemmShape g = {h0, k0, k0, k1, h0, k1, k1 - h0, N - k1, NBO};
+      CHK(launch_gemm<T>(mb, batch, g, skip, st));
+      CHK(solve_upper_rows(mb, batch, h0, k1, N - k1, W, strideW, skip, st));
+    }
+    // trailing update with K = 128: columns [k1, k2) on the main stream, [k2, N) on the look-ahead stream
+    const int k2 = min(N, k1 + NBT);
+    if (look && k2 < N) {
+      CHK(hipEventRecord(la->ready, st));
+      CHK(hipStreamWaitEvent(la->stream, la->ready, 0));
+      GemmShape gr = {k1, k0, k0, k2, k1, k2, N - k1, N - k2, k1 - k0};
+      CHK(launch_gemm<T>(mb, batch, gr, skip, la->stream));
+      CHK(hipEventRecord(la->done, la->stream));
+      pending = true;
+      GemmShape gl = {k1, k0, k0, k1, k1, k1, N - k1, k2 - k1, k1 - k0};
+      CHK(launch_gemm<T>(mb, batch, gl, skip, st));
+    } else {
       GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, N - k1, k1 - k0};
       CHK(launch_gemm<T>(mb, batch, g, skip, st));
     }
+    CHK(factor_block(k1, k2));
   }
+  if (pending) CHK(hipStreamWaitEvent(st, la->done, 0));
   // inverses of the diagonal blocks of U for the backward sweeps
   hipLaunchKernelGGL((k_tri_inv<T, false>), dim3((N + 63) / 64, batch), dim3(256), 0, st, mb, W, strideW, 0, skip);
   CHK(hipGetLastError());
@@ -1288,9 +1354,9 @@ hipError_t growth_batch(const MatBatch<T>& mb, int batch, double* umax, const do
 }
 
 template hipError_t lu_factor_batch<float>(const MatBatch<float>&, int, int*, int*, long long, int*, const int*,
-                                           cplx<float>*, hipStream_t);
+                                           cplx<float>*, hipStream_t, const LuLookahead*);
 template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, int*, long long, int*, const int*,
-                                            cplx<double>*, hipStream_t);
+                                            cplx<double>*, hipStream_t, const LuLookahead*);
 template hipError_t lu_solve_batch<float>(const MatBatch<float>&, int, const int*, long long, const cplx<double>*,
                                           long long, long long, cplx<float>*, long long, long long, int, const int*,
                                           const cplx<float>*, hipStream_t);

@@ -93,10 +93,7 @@ typedef struct bem_mesh_desc {
   const int* slot_count;     /* [n_chunks] */
   const uint16_t* slot_start;/* [n_chunks][196] */
   const uint16_t* slot_inc;  /* [n_chunks][192], (lane << 2) | local column */
-  const int* slot_base;      /* [n_chunks] */
-  int strip_len;             /* L: slots per strip row (padded) */
-  const int* cs_ptr;         /* [V+1] column -> slots CSR */
-  const int* cs_slot;
+  const int* slot_vertex;    /* [n_chunks][192]: 2 * vertex + 1 if an earlier chunk already holds the vertex */
   const int* vstar_ptr;      /* [V+1] vertex -> strip rows (3t+l) CSR */
   const int* vstar_row;
   /* singular (vertex-sharing) pairs, grouped by test element */

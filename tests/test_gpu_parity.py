@@ -96,8 +96,11 @@ def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precisi
     eng.close()
 
 
-@pytest.mark.parametrize("precision,n,batch", [("fp64", 200, 2), ("mixed", 200, 3), ("fp64", 1500, 1),
-                                               ("mixed", 2100, 2)])
+# sizes: one partial half block (40), one trailing block with a partial second half (100), partial trailing
+# blocks (190, 200) and the register-panel thresholds (1500, 2100)
+@pytest.mark.parametrize("precision,n,batch", [("mixed", 40, 2), ("fp64", 100, 1), ("mixed", 100, 2),
+                                               ("mixed", 190, 2), ("fp64", 200, 2), ("mixed", 200, 3),
+                                               ("fp64", 1500, 1), ("mixed", 2100, 2)])
 def test_lu_random(gpu_device, precision, n, batch):
     import ctypes as C
 
@@ -141,6 +144,45 @@ def test_lu_random(gpu_device, precision, n, batch):
         print(precision, n, m, "rel err", err, "backward", bwd)
         assert err < tol_fwd and bwd < tol_bwd
     assert int(info.cpu().numpy().max()) == 0
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_lu_streams_do_not_change_the_factors(gpu_device, precision):
+    """Look-ahead (fan-out >= 2: a second stream for the trailing updates beyond the next block) computes every
+    entry with the same operations in the same order as one stream: bitwise equal factors, pivots and
+    workspace."""
+    from beat_hip import _lib
+    from beat_hip.engine import PRECISIONS, _ptr
+
+    torch = gpu_device.torch
+    n, batch = 700, 8
+    rng = np.random.default_rng(11)
+    lda = ((n + 15) // 16) * 16
+    M = rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))
+    cdt = torch.complex64 if precision == "mixed" else torch.complex128
+    prec = PRECISIONS[precision]
+    lib = gpu_device.lib
+    outs = []
+    try:
+        for fanout in (1, 4):
+            _lib.check(lib.bem_ctx_set_fanout(gpu_device.ctx, fanout), "fanout")
+            with torch.cuda.stream(gpu_device.stream):
+                A = torch.zeros((batch, n, lda), dtype=cdt, device=gpu_device.device)
+                A[:, :, :n] = torch.from_numpy(M).to(gpu_device.device).to(cdt)
+                ipiv = torch.zeros((batch, n), dtype=torch.int32, device=gpu_device.device)
+                perm = torch.zeros((batch, n), dtype=torch.int32, device=gpu_device.device)
+                info = torch.zeros(batch, dtype=torch.int32, device=gpu_device.device)
+                work = torch.zeros(_lib.lu_workspace_bytes(prec, n, batch), dtype=torch.uint8,
+                                   device=gpu_device.device)
+            gpu_device.sync()
+            _lib.check(lib.bem_lu_factor(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(ipiv),
+                                         _ptr(perm), n, _ptr(info), _ptr(work)), "factor")
+            gpu_device.sync()
+            outs.append((A.cpu().numpy(), ipiv.cpu().numpy(), perm.cpu().numpy(), work.cpu().numpy()))
+    finally:
+        _lib.check(lib.bem_ctx_set_fanout(gpu_device.ctx, 4), "fanout")
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
 
 
 def test_golden_fixtures(gpu_device):

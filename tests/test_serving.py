@@ -134,6 +134,6 @@ def test_backend_contract(tmp_path, monkeypatch):
 
 
 def test_footprint_grows_with_n_squared():
-    a = S.estimate_footprint(1000, 2000, 27000, 1600, 8, 146)
-    b = S.estimate_footprint(4000, 8000, 108000, 6400, 8, 146)
+    a = S.estimate_footprint(1000, 2000, 27000, 8, 146)
+    b = S.estimate_footprint(4000, 8000, 108000, 8, 146)
     assert 10 < b / a < 17
