@@ -12,6 +12,8 @@ src/blab/server.py:154-185). Here one process-wide scheduler owns the GPU:
 * Frequencies are batched per job (one launch set per batch); jobs whose mesh and drives are identical share
   one device mesh and are merged into the same batches.
 * Per-job latency (arrival -> last result) is recorded for p50/p99 reporting.
+* A completed job can leave the server's ``result.npz`` artifact (src/blab/server.py:477-511), and
+  ``health_payload`` is the body of the server's ``/health`` route (src/blab/server.py:548-568).
 """
 
 from __future__ import annotations
@@ -21,11 +23,95 @@ import queue
 import threading
 import time
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
+from . import contract as K
 from . import driver as D
 from . import geometry as G
+
+CAPABILITY_KEYS = (
+    "supports_spherical_sampling",
+    "supports_impedance",
+    "supports_burton_miller",
+    "supports_flat_target_normalization",
+    "supports_channel_resynthesis",
+    "supports_cancellation",
+    "supports_streaming",
+    "supports_symmetry",
+)
+
+
+def health_payload(solver_id: str = "beat_rocm") -> dict:
+    """``GET /health`` body for a server whose solver is this backend (src/blab/server.py:548-568): status,
+    solver and backend ids, label and the eight advertised capability flags of ``beat_rocm``
+    (src/blab/solvers/registry.py:59-71 over the SolverCapabilities defaults, src/blab/solvers/base.py:78-89)."""
+    caps = K.BEAT_ROCM_CAPABILITIES
+    return {
+        "status": "ok",
+        "solver": str(solver_id),
+        "backend": "beat_rocm",
+        "solver_label": "BEAT Engine (AMD ROCm)",
+        "capabilities": {key: bool(getattr(caps, key)) for key in CAPABILITY_KEYS},
+    }
+
+
+def result_artifact_bundle(init: dict, results: list[dict]) -> dict | None:
+    """Arrays of a job's ``result.npz`` from its ``initialized`` payload and ``result`` payloads, as the server
+    writes them (src/blab/server.py:477-511 over LiveSolveDataset, src/blab/live.py:37-87,175-206): results
+    keyed by frequency (a repeated frequency keeps the last result) in ascending order, Float32 rows, impedance
+    as [radiator][frequency]. The per-result SPL rows are the channel synthesis the solver already applied with
+    the request's channels (julia_local/solver.jl:546-588), which is what the server's re-synthesis with the same
+    channel configuration reproduces. None when nothing was solved."""
+    by_freq = {}
+    for r in results:
+        by_freq[float(r["freq_hz"])] = r
+    if not by_freq:
+        return None
+    ordered = [by_freq[k] for k in sorted(by_freq)]
+
+    def f32(v):
+        return np.asarray(v, dtype=np.float32)
+
+    def raw_or_norm(r, key):
+        return r.get(key + "_spl_db") if r.get(key + "_spl_db") is not None else r[key + "_spl_norm_db"]
+
+    freqs = f32([r["freq_hz"] for r in ordered])
+    impedance = np.stack([f32(r["impedance"]) for r in ordered], axis=1)
+    bundle = {
+        "freq_hz": freqs,
+        "polar_angle_deg": f32(init["polar_angle_deg"]),
+        "horizontal_spl_db": np.vstack([f32(raw_or_norm(r, "horizontal")) for r in ordered]),
+        "vertical_spl_db": np.vstack([f32(raw_or_norm(r, "vertical")) for r in ordered]),
+        "horizontal_spl_norm_db": np.vstack([f32(r["horizontal_spl_norm_db"]) for r in ordered]),
+        "vertical_spl_norm_db": np.vstack([f32(r["vertical_spl_norm_db"]) for r in ordered]),
+        "impedance_freq_hz": freqs,
+        "impedance_radiator_names": np.asarray(init["radiator_names"]).astype(str),
+        "impedance_real": impedance[:, :, 0],
+        "impedance_imag": impedance[:, :, 1],
+    }
+    sphere = init.get("sphere_metadata")
+    if sphere and all(r.get("sphere_spl_norm_db") is not None for r in ordered):
+        bundle.update(
+            sphere_r_distance_m=f32(sphere["r_distance_m"]),
+            sphere_theta_polar_rad=f32(sphere["theta_polar_rad"]),
+            sphere_phi_azimuth_rad=f32(sphere["phi_azimuth_rad"]),
+            sphere_spl_norm_db=np.vstack([f32(r["sphere_spl_norm_db"]) for r in ordered]),
+        )
+    return bundle
+
+
+def write_result_artifact(directory, init: dict, results: list[dict]) -> Path | None:
+    """Write ``<directory>/result.npz`` (compressed, as the server does); None when nothing was solved."""
+    bundle = result_artifact_bundle(init, results)
+    if bundle is None:
+        return None
+    directory = Path(directory)
+    directory.mkdir(parents=True, exist_ok=True)
+    path = directory / "result.npz"
+    np.savez_compressed(path, **bundle)
+    return path
 
 
 def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, batch: int, n_obs: int,
@@ -55,6 +141,9 @@ class JobHandle:
     t_admit: float | None = None
     n_results: int = 0
     footprint: int = 0
+    init: dict | None = None
+    results: list = field(default_factory=list)
+    artifact_path: Path | None = None
 
     def cancel(self):
         self.cancel_flag.set()
@@ -66,7 +155,8 @@ class JobHandle:
 
 class Scheduler:
     def __init__(self, hbm_budget_bytes: int | None = None, slots: int = 4, batch: int = 8,
-                 device_index: int = 0, engine_factory=None):
+                 device_index: int = 0, engine_factory=None, artifact_root=None):
+        self.artifact_root = None if artifact_root is None else Path(artifact_root)
         self.slots = int(slots)
         self.batch = int(batch)
         self.device_index = device_index
@@ -93,6 +183,7 @@ class Scheduler:
             jid = self._next_id
             self._next_id += 1
         h = JobHandle(jid, request, time.perf_counter())
+        h.init = init
         h.events.put({"type": "initialized", **init})
         try:
             job = D.build_job(request, parsed)
@@ -194,10 +285,15 @@ class Scheduler:
                                                timings, D.diagnostics_for(job, res, j, {"serving_slot": slot}))
                     solved = i0 + j + 1
                     h.n_results = solved
+                    h.results.append(payload)
                     h.events.put({"type": "result", "solved_count": solved, "total_count": len(freqs),
                                   "result": payload})
             b.close()
             terminal = {"type": "completed", "solved_count": len(freqs)}
+            if self.artifact_root is not None:
+                # the server's artifact, written before the terminal event (src/blab/server.py:446-456)
+                h.artifact_path = write_result_artifact(self.artifact_root / str(h.job_id), h.init, h.results)
+                terminal["artifact"] = f"/jobs/{h.job_id}/artifacts/result.npz"
         except Exception as exc:
             terminal = {"type": "failed", "error": f"{type(exc).__name__}: {exc}"}
         finally:

@@ -137,3 +137,74 @@ def test_footprint_grows_with_n_squared():
     a = S.estimate_footprint(1000, 2000, 27000, 8, 146)
     b = S.estimate_footprint(4000, 8000, 108000, 8, 146)
     assert 10 < b / a < 17
+
+
+ARTIFACT_KEYS = {"freq_hz", "polar_angle_deg", "horizontal_spl_db", "vertical_spl_db", "horizontal_spl_norm_db",
+                 "vertical_spl_norm_db", "impedance_freq_hz", "impedance_radiator_names", "impedance_real",
+                 "impedance_imag"}
+SPHERE_KEYS = {"sphere_r_distance_m", "sphere_theta_polar_rad", "sphere_phi_azimuth_rad", "sphere_spl_norm_db"}
+
+
+def test_result_artifact_written_on_completion(tmp_path):
+    # the server's result.npz (src/blab/server.py:477-511): ascending frequencies, Float32 rows taken from the
+    # result events, impedance as [radiator][frequency], balloon arrays when the sphere is sampled
+    sched = S.Scheduler(hbm_budget_bytes=1 << 40, slots=1, batch=2, engine_factory=FakeEngine,
+                        artifact_root=tmp_path / "jobs")
+    path = _mesh(tmp_path)
+    req = _req(path)
+    req["frequencies_hz"] = [900.0, 300.0, 1500.0, 600.0, 1200.0]
+    req["config"].update(spherical_sampling_enabled=True, spherical_sampling_points=16)
+    h, init = sched.submit(req)
+    ev = _drain(h)
+    assert ev[-1]["type"] == "completed"
+    assert ev[-1]["artifact"] == f"/jobs/{h.job_id}/artifacts/result.npz"
+    assert h.artifact_path == tmp_path / "jobs" / str(h.job_id) / "result.npz"
+    results = sorted((e["result"] for e in ev if e["type"] == "result"), key=lambda r: r["freq_hz"])
+    with np.load(h.artifact_path) as z:
+        assert set(z.files) == ARTIFACT_KEYS | SPHERE_KEYS
+        assert z["freq_hz"].dtype == np.float32
+        assert z["freq_hz"].tolist() == [300.0, 600.0, 900.0, 1200.0, 1500.0]
+        n_ang = len(init["polar_angle_deg"])
+        for key in ("horizontal_spl_db", "vertical_spl_db", "horizontal_spl_norm_db", "vertical_spl_norm_db"):
+            assert z[key].shape == (5, n_ang) and z[key].dtype == np.float32
+            for i, r in enumerate(results):
+                np.testing.assert_array_equal(z[key][i], np.asarray(r[key], np.float32))
+        n_rad = len(init["radiator_names"])
+        assert z["impedance_real"].shape == (n_rad, 5)
+        for i, r in enumerate(results):
+            imp = np.asarray(r["impedance"], np.float32)
+            np.testing.assert_array_equal(z["impedance_real"][:, i], imp[:, 0])
+            np.testing.assert_array_equal(z["impedance_imag"][:, i], imp[:, 1])
+        assert z["impedance_radiator_names"].tolist() == list(init["radiator_names"])
+        assert z["sphere_spl_norm_db"].shape == (5, 16)
+        np.testing.assert_array_equal(z["sphere_theta_polar_rad"],
+                                      np.asarray(init["sphere_metadata"]["theta_polar_rad"], np.float32))
+    # a cancelled job leaves no artifact, as the server returns before writing it
+    h2, _ = sched.submit(_req(path, nf=40))
+    h2.cancel()
+    assert _drain(h2)[-1]["type"] == "cancelled" and h2.artifact_path is None
+    sched.shutdown()
+
+
+def test_artifact_bundle_keeps_last_result_per_frequency_and_skips_balloon_without_sphere():
+    init = {"polar_angle_deg": [0.0, 90.0], "radiator_names": ["a", "b"], "sphere_metadata": None}
+
+    def res(f, v):
+        return {"freq_hz": f, "horizontal_spl_norm_db": [v, v], "vertical_spl_norm_db": [v, v],
+                "impedance": [[v, -v], [2 * v, 0.0]]}
+
+    b = S.result_artifact_bundle(init, [res(200.0, 1.0), res(100.0, 2.0), res(200.0, 3.0)])
+    assert set(b) == ARTIFACT_KEYS
+    assert b["freq_hz"].tolist() == [100.0, 200.0]
+    assert b["horizontal_spl_db"].tolist() == [[2.0, 2.0], [3.0, 3.0]]  # raw rows fall back to the normalised
+    assert b["impedance_real"].tolist() == [[2.0, 3.0], [4.0, 6.0]]
+    assert S.result_artifact_bundle(init, []) is None
+
+
+def test_health_payload_matches_server_schema():
+    h = S.health_payload()
+    assert set(h) == {"status", "solver", "backend", "solver_label", "capabilities"}
+    assert h["status"] == "ok" and h["backend"] == "beat_rocm" and h["solver_label"] == "BEAT Engine (AMD ROCm)"
+    assert set(h["capabilities"]) == set(S.CAPABILITY_KEYS) and len(S.CAPABILITY_KEYS) == 8
+    assert all(h["capabilities"].values())  # beat_rocm advertises every flag the route reports
+    assert json.loads(json.dumps(h)) == h
