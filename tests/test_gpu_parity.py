@@ -337,3 +337,65 @@ def test_symmetric_solve_and_field_match_oracle(gpu_device, sym_models, mode, pr
     pa = pulsating_sphere_pressure(np.linalg.norm(pts, axis=1), A_SPHERE, k, RHO, 2 * math.pi * FREQ)
     assert np.abs(20 * np.log10(np.abs(u) / np.abs(pa))).max() < 0.2
     eng.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-9), ("float32", 1e-4)])
+def test_system_exterior_request_matches_oracle(gpu_device, tmp_path, precision, tol):
+    """Compiled-system exterior request (one BEM surface, two excitation ports on disjoint moving boundaries)
+    through the coupled-worker wire: boundary pressure, field and impedance against the FP64 oracle."""
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+    from beat_hip import system as SY
+    from beat_hip.engine import Engine
+    from oracle import bem_oracle as O
+
+    V, F, _ = syn.icosphere(3, A_SPHERE)
+    T = np.where(V[F].mean(axis=1)[:, 2] > 0, 4, 9)
+    path = tmp_path / "s.msh"
+    syn.write_gmsh22(path, V, F, T, scale=1.0)
+    pts = [[0.0, 0.0, 2.0], [1.5, 0.0, 1.0], [0.0, -2.0, 0.3]]
+    req = {
+        "schema_version": 1,
+        "compiled_system": {
+            "meshes": [{"id": "m", "file": str(path), "purpose": "bem_surface", "scale_to_m": 1.0,
+                        "translation_m": [0.0, 0.0, 0.0]}],
+            "regions": [{"id": "air", "kind": "unbounded_air", "mesh_ids": ["m"], "sound_speed_m_per_s": C0,
+                         "density_kg_per_m3": RHO}],
+            "boundaries": [{"id": "top", "region_id": "air", "kind": "moving", "group": {"mesh_id": "m", "tag": 4}},
+                           {"id": "bot", "region_id": "air", "kind": "moving",
+                            "group": {"mesh_id": "m", "tag": 9}}],
+            "components": [{"id": "c1", "kind": "ideal_velocity_source", "boundary_ids": ["top"],
+                            "parameters": {"boundary_motion_weights": {"top": 0.5}}},
+                           {"id": "c2", "kind": "ideal_velocity_source", "boundary_ids": ["bot"]}],
+            "excitation_ports": [{"id": "e1", "component_id": "c1", "kind": "normal_velocity"},
+                                 {"id": "e2", "component_id": "c2", "kind": "normal_velocity"}],
+        },
+        "frequencies_hz": [FREQ],
+        "excitation_port_ids": ["e1", "e2"],
+        "outputs": [{"id": "far", "quantity": "exterior_pressure", "options": {"points_m": pts}},
+                    {"id": "p", "quantity": "bem_boundary_pressure"},
+                    {"id": "z", "quantity": "radiation_impedance"}],
+        "solver_options": {"precision": precision, "bem_backend": "rocm"},
+    }
+    events = []
+    SY.solve_system_request(req, lambda prep, q: Engine(prep, q, dev=gpu_device), events.append)
+    assert len(events) == 1
+    qs = {q["id"]: q for q in events[0]["result"]["quantities"]}
+
+    def arr(w):
+        return (np.asarray(w["real"]) + 1j * np.asarray(w["imag"])).reshape(w["shape"])
+
+    job = SY.parse_exterior_request(req)
+    m = job["mesh"]
+    drives = SY.unit_drives(m, job["excitations"])
+    freq = float(np.float32(FREQ)) if precision == "float32" else FREQ
+    c, rho = job["sound_speed"], job["density"]
+    ref = O.solve_frequency(O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids), freq, drives, c, rho,
+                            points=np.asarray(pts))
+    p = arr(qs["p"]["values"])
+    assert p.shape == (2, m.n_vertices)
+    assert _rel(p.T, ref["pressure"]) < tol
+    assert _rel(arr(qs["far"]["values"]).T, ref["field"]) < tol
+    z = arr(qs["z"]["values"])
+    for e, ex in enumerate(job["excitations"]):
+        assert abs(z[e] - SY.component_impedance(m, ref["pressure"][:, e], ex, 1)) < tol * abs(z[e]) * 10
