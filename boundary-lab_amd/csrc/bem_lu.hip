@@ -840,7 +840,7 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
 }
 
 // ------------------------------------------------------------------------------------------------
-// Misc kernels: identity permutation, RHS permutation, triangular sweeps, residual, norms, update.
+// Misc kernels: identity permutation, residual, norms, update.
 // ------------------------------------------------------------------------------------------------
 
 __global__ void k_fill_int(int* p, int n, int value) {
@@ -861,118 +861,186 @@ __global__ void k_iota(int* perm, long long strideP, int N, int* info) {
   if (i == 0) info[mat] = 0;
 }
 
-// y[c][i] = src[c][perm[i]]  (src is FP64 complex, y is T)
+// Whole solve of one matrix per workgroup (1024 threads): y = P b, then the forward sweep with unit L and the
+// backward sweep with U, by 64-row blocks. A block's rows take their dot products with every entry solved so far
+// (16 threads per row, 4 contiguous columns per thread and group of 64 columns, several groups of loads in flight),
+// then the inverted diagonal block finishes the block as a 64 x 64 matrix-vector product. The solution lives in
+// LDS when it fits (N <= CAP columns per channel), otherwise solved entries are staged through LDS in chunks.
+// One launch per solve: the sweeps are chains of dependent blocks, and one workgroup walks its chain with
+// barriers instead of launches, while the matrices of a batch stream their factors on separate CUs.
 template <typename T>
-__global__ void k_permute_rhs(const cplx<double>* src, long long lds_, long long strideS, cplx<T>* y,
-                              long long ldy, long long strideY, const int* perm, long long strideP, int N,
-                              int nrhs, const int* skip) {
-  const int mat = blockIdx.z;
-  if (skip && skip[mat]) return;
-  const int c = blockIdx.y;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N || c >= nrhs) return;
-  const cplx<double> v = src[(size_t)mat * strideS + (size_t)c * lds_ + perm[(size_t)mat * strideP + i]];
-  y[(size_t)mat * strideY + (size_t)c * ldy + i] = mk<T>(T(v.re), T(v.im));
+__device__ __forceinline__ void load4(const cplx<T>* p, cplx<T> (&a)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 x = reinterpret_cast<const v4*>(p)[0], y = reinterpret_cast<const v4*>(p)[1];
+    a[0] = mk<T>(x[0], x[1]);
+    a[1] = mk<T>(x[2], x[3]);
+    a[2] = mk<T>(y[0], y[1]);
+    a[3] = mk<T>(y[2], y[3]);
+  } else {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const v2 x = reinterpret_cast<const v2*>(p)[q];
+      a[q] = mk<T>(x[0], x[1]);
+    }
+  }
 }
 
-// One block step of a triangular sweep, many workgroups per matrix.
-// Lower (forward, unit L): blocks ascend; step `kb` applies the solved block kb to the rows of every later
-// block (one workgroup per 64-row block), and the workgroup owning block kb+1 then solves its diagonal block,
-// whose rows now carry every earlier contribution. kb = -1 only solves block 0.
-// Upper (backward, U with diagonal): the mirror image with blocks descending; kb = nblk solves the last block.
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, long long ldy, long long strideY,
-                                                  int nrhs, int kb, const int* skip, const cplx<T>* W,
-                                                  long long strideW) {
-  const int mat = blockIdx.y;
+template <typename T, int NR>
+__global__ __launch_bounds__(1024) void k_trsv_wg(MatBatch<T> mb, const cplx<double>* b, long long ldb,
+                                                  long long strideB, cplx<T>* y, long long ldy, long long strideY,
+                                                  const int* perm, long long strideP, const int* skip,
+                                                  const cplx<T>* W, long long strideW) {
+  constexpr int CAP = 32768 / (NR * (int)sizeof(cplx<T>));  // columns per channel in 32 KB of LDS
+  const int mat = blockIdx.x;
   if (skip && skip[mat]) return;
   const int N = mb.N;
   const int nblk = (N + 63) / 64;
+  const bool resident = N <= CAP;
   const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
-  cplx<T>* Y = y + (size_t)mat * strideY;
   const long long lda = mb.lda;
-  const int tid = threadIdx.x;
-  const int rb = LOWER ? kb + 1 + (int)blockIdx.x : kb - 1 - (int)blockIdx.x;  // row block of this workgroup
-  if (rb < 0 || rb >= nblk) return;
-  const int r0 = rb * 64, nr = min(64, N - r0);
-  __shared__ cplx<T> xk[MAX_CH][64];
-  __shared__ cplx<T> yb[MAX_CH][64];  // current values of the block solved by this workgroup
-  const bool solve_here = LOWER ? (rb == kb + 1) : (rb == kb - 1);
-  const bool updated = kb >= 0 && kb < nblk;
-  // 4 threads per row, 16 columns each
-  const int row = tid >> 2, part = tid & 3;
-  if (updated) {
-    const int c0 = kb * 64, nc = min(64, N - c0);
-    if (tid < 64)
-      for (int c = 0; c < nrhs; ++c) xk[c][tid] = tid < nc ? Y[(size_t)c * ldy + c0 + tid] : mk<T>(0, 0);
-    __syncthreads();
-    cplx<T> acc[MAX_CH];
-#pragma unroll
-    for (int c = 0; c < MAX_CH; ++c) acc[c] = mk<T>(0, 0);
-    if (row < nr) {
-      const cplx<T>* Ar = A + (size_t)(r0 + row) * lda + c0 + 16 * part;
-#pragma unroll 4
-      for (int j = 0; j < 16; ++j) {
-        if (16 * part + j < nc) {
-          const cplx<T> a = Ar[j];
-#pragma unroll
-          for (int c = 0; c < MAX_CH; ++c)
-            if (c < nrhs) acc[c] = cfma(acc[c], a, xk[c][16 * part + j]);
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < MAX_CH; ++c) {
-      if (c < nrhs) {
-        T ar = acc[c].re, ai = acc[c].im;
-        ar += __shfl_xor(ar, 1, 64);
-        ai += __shfl_xor(ai, 1, 64);
-        ar += __shfl_xor(ar, 2, 64);
-        ai += __shfl_xor(ai, 2, 64);
-        if (part == 0 && row < nr) {
-          cplx<T>* yp = Y + (size_t)c * ldy + r0 + row;
-          const cplx<T> nv = mk<T>(yp->re - ar, yp->im - ai);
-          if (solve_here)
-            yb[c][row] = nv;
-          else
-            *yp = nv;
-        } else if (part == 0 && solve_here) {
-          yb[c][row] = mk<T>(0, 0);  // rows past N meet zero columns of the padded inverse
-        }
-      }
-    }
-  }
-  // the workgroup holding the next block to solve finishes it with the inverted diagonal block:
-  // y_blk = D^-1 y_blk, a 64 x 64 matrix-vector product (no serial substitution)
-  if (!solve_here) return;
-  if (!updated && tid < 64)
-    for (int c = 0; c < nrhs; ++c) yb[c][tid] = tid < nr ? Y[(size_t)c * ldy + r0 + tid] : mk<T>(0, 0);
-  __syncthreads();
-  const cplx<T>* Wb = W + (size_t)mat * strideW + ((size_t)rb * 2 + (LOWER ? 0 : 1)) * 4096;
-  cplx<T> acc[MAX_CH];
-#pragma unroll
-  for (int c = 0; c < MAX_CH; ++c) acc[c] = mk<T>(0, 0);
+  cplx<T>* Y = y + (size_t)mat * strideY;
+  const int tid = threadIdx.x, row = tid >> 4, part = tid & 15;
+  __shared__ cplx<T> ys[NR][CAP];
+  __shared__ cplx<T> sb[NR][64];
+
+  // y = P b (FP64 right-hand sides rounded to T), into LDS when the solution fits there
   {
-    const cplx<T>* Wr = Wb + (size_t)row * 64 + 16 * part;
-#pragma unroll 4
-    for (int j = 0; j < 16; ++j) {
-      const cplx<T> wv = Wr[j];
+    const cplx<double>* B = b + (size_t)mat * strideB;
+    const int* pm = perm + (size_t)mat * strideP;
+    for (int i = tid; i < N; i += 1024) {
+      const int pi = pm[i];
 #pragma unroll
-      for (int c = 0; c < MAX_CH; ++c)
-        if (c < nrhs) acc[c] = cfma(acc[c], wv, yb[c][16 * part + j]);
+      for (int c = 0; c < NR; ++c) {
+        const cplx<double> v = B[(size_t)c * ldb + pi];
+        const cplx<T> t = mk<T>(T(v.re), T(v.im));
+        if (resident)
+          ys[c][i] = t;
+        else
+          Y[(size_t)c * ldy + i] = t;
+      }
     }
   }
+  __syncthreads();
+
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool lower = pass == 0;
+    for (int s = 0; s < nblk; ++s) {
+      const int kb = lower ? s : nblk - 1 - s;
+      const int r0 = kb * 64, nr = min(64, N - r0);
+      // inverted diagonal block entries of this thread, loaded before the dot products
+      const cplx<T>* Wr = W + (size_t)mat * strideW + ((size_t)kb * 2 + (lower ? 0 : 1)) * 4096 + row * 64 + 4 * part;
+      cplx<T> w[4];
+      load4(Wr, w);
+      // solved entries this block depends on: [0, r0) going forward, [r0 + nr, N) going backward
+      const int cb = lower ? 0 : r0 + nr, ce = lower ? r0 : N;
+      T accr[NR], acci[NR];
 #pragma unroll
-  for (int c = 0; c < MAX_CH; ++c) {
-    if (c < nrhs) {
-      T ar = acc[c].re, ai = acc[c].im;
-      ar += __shfl_xor(ar, 1, 64);
-      ai += __shfl_xor(ai, 1, 64);
-      ar += __shfl_xor(ar, 2, 64);
-      ai += __shfl_xor(ai, 2, 64);
-      if (part == 0 && row < nr) Y[(size_t)c * ldy + r0 + row] = mk<T>(ar, ai);
+      for (int c = 0; c < NR; ++c) accr[c] = acci[c] = T(0);
+      const cplx<T>* Ar = A + (size_t)(r0 + min(row, nr - 1)) * lda;
+      for (int c0 = cb; c0 < ce; c0 += resident ? (ce - cb) : CAP) {
+        const int len = resident ? ce - cb : min(CAP, ce - c0);
+        const int yo = resident ? c0 : 0;  // LDS column of global column c0
+        if (!resident) {
+          for (int e = tid; e < len; e += 1024)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) ys[c][e] = Y[(size_t)c * ldy + c0 + e];
+          __syncthreads();
+        }
+        if (row < nr) {
+          const cplx<T>* Ap = Ar + c0 + 4 * part;
+          const int ng = len >> 6;  // full groups of 64 columns
+          auto fma4 = [&](const cplx<T> (&a)[4], int col) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+              for (int c = 0; c < NR; ++c) {
+                const cplx<T> v = ys[c][col + q];
+                accr[c] += a[q].re * v.re - a[q].im * v.im;
+                acci[c] += a[q].re * v.im + a[q].im * v.re;
+              }
+          };
+          constexpr int UG = sizeof(T) == 4 ? 4 : 2;  // groups of loads in flight (FP64: register budget)
+          int g = 0;
+          for (; g + UG <= ng; g += UG) {
+            cplx<T> a[UG][4];
+#pragma unroll
+            for (int u = 0; u < UG; ++u) load4(Ap + 64 * (g + u), a[u]);
+#pragma unroll
+            for (int u = 0; u < UG; ++u) fma4(a[u], yo + 4 * part + 64 * (g + u));
+          }
+          for (; g < ng; ++g) {
+            cplx<T> a[4];
+            load4(Ap + 64 * g, a);
+            fma4(a, yo + 4 * part + 64 * g);
+          }
+          // tail of fewer than 64 columns
+          for (int jj = 64 * ng + part; jj < len; jj += 16) {
+            const cplx<T> a = Ar[c0 + jj];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+              const cplx<T> v = ys[c][yo + jj];
+              accr[c] += a.re * v.re - a.im * v.im;
+              acci[c] += a.re * v.im + a.im * v.re;
+            }
+          }
+        }
+        if (!resident) __syncthreads();
+      }
+      // sum the 16 partial dot products of a row (lanes 16r .. 16r+15 of one wave)
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+          accr[c] += __shfl_xor(accr[c], m, 64);
+          acci[c] += __shfl_xor(acci[c], m, 64);
+        }
+      if (part == 0)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          if (row < nr) {
+            const cplx<T> v = resident ? ys[c][r0 + row] : Y[(size_t)c * ldy + r0 + row];
+            sb[c][row] = mk<T>(v.re - accr[c], v.im - acci[c]);
+          } else {
+            sb[c][row] = mk<T>(0, 0);  // rows past N meet zero columns of the padded inverse
+          }
+        }
+      __syncthreads();
+      // the block's entries: inverted diagonal block times the updated right-hand side
+      T dr[NR], di[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) dr[c] = di[c] = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const cplx<T> v = sb[c][4 * part + q];
+          dr[c] += w[q].re * v.re - w[q].im * v.im;
+          di[c] += w[q].re * v.im + w[q].im * v.re;
+        }
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+          dr[c] += __shfl_xor(dr[c], m, 64);
+          di[c] += __shfl_xor(di[c], m, 64);
+        }
+      if (part == 0 && row < nr)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          if (resident)
+            ys[c][r0 + row] = mk<T>(dr[c], di[c]);
+          else
+            Y[(size_t)c * ldy + r0 + row] = mk<T>(dr[c], di[c]);
+        }
+      __syncthreads();
     }
   }
+  if (resident)
+    for (int i = tid; i < N; i += 1024)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) Y[(size_t)c * ldy + i] = ys[c][i];
 }
 
 // r = b - A0 x in FP64 (A0 in T, x and b in FP64); one wave per row, all RHS together.
@@ -1298,25 +1366,18 @@ hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, lon
                           const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
                           long long strideY, int nrhs, const int* skip, const cplx<T>* W, hipStream_t st) {
   const long long strideW = lu_workspace_elems(mb.N);
-  const int N = mb.N;
-  hipLaunchKernelGGL(k_permute_rhs<T>, dim3((N + 255) / 256, nrhs, batch), dim3(256), 0, st, b, ldb, strideB, y,
-                     ldy, strideY, perm, strideP, N, nrhs, skip);
-  CHK(hipGetLastError());
-  const int nblk = (N + 63) / 64;
-  // forward sweep: step kb updates blocks > kb and solves block kb+1
-  for (int kb = -1; kb < nblk - 1; ++kb) {
-    const int rows_after = nblk - 1 - kb;
-    hipLaunchKernelGGL((k_trsv_step<T, true>), dim3(rows_after, batch), dim3(256), 0, st, mb, y, ldy, strideY,
-                       nrhs, kb, skip, W, strideW);
-    CHK(hipGetLastError());
+#define TRSV(NR_)                                                                                             \
+  hipLaunchKernelGGL((k_trsv_wg<T, NR_>), dim3(batch), dim3(1024), 0, st, mb, b, ldb, strideB, y, ldy, strideY, \
+                     perm, strideP, skip, W, strideW)
+  switch (nrhs) {
+    case 1: TRSV(1); break;
+    case 2: TRSV(2); break;
+    case 3: TRSV(3); break;
+    case 4: TRSV(4); break;
+    default: return hipErrorInvalidValue;
   }
-  // backward sweep: step kb updates blocks < kb and solves block kb-1
-  for (int kb = nblk; kb > 0; --kb) {
-    hipLaunchKernelGGL((k_trsv_step<T, false>), dim3(kb, batch), dim3(256), 0, st, mb, y, ldy, strideY, nrhs, kb,
-                       skip, W, strideW);
-    CHK(hipGetLastError());
-  }
-  return hipSuccess;
+#undef TRSV
+  return hipGetLastError();
 }
 
 template <typename T>
