@@ -28,6 +28,8 @@ def build(verbose: bool = False) -> Path:
         "-fPIC",
         "-std=c++17",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # register-resident LU sub-panels (up to 2 x 16 complex per thread) stay in VGPRs instead of scratch
+        "-mllvm", "-amdgpu-promote-alloca-to-vector-limit=1024",
         "-I", str(HERE.parent / "include"),
     ]
     build_dir = HERE / "_build"

@@ -580,13 +580,19 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   __shared__ int s_src[2 * NBI];
   __shared__ int s_na;
 
-  cplx<T> P[MR][NBI];
+  // the sub-panel: separate real and imaginary arrays (scalar arrays stay in registers at every size used)
+  T Pr[MR][NBI], Pi[MR][NBI];
+  auto getP = [&](int m, int j) { return mk<T>(Pr[m][j], Pi[m][j]); };
+  auto setP = [&](int m, int j, cplx<T> v) {
+    Pr[m][j] = v.re;
+    Pi[m][j] = v.im;
+  };
 #pragma unroll
   for (int m = 0; m < MR; ++m) {
     const int r = c0 + tid + 1024 * m;
 #pragma unroll
     for (int j = 0; j < NBI; ++j)
-      P[m][j] = (r < N && j < nb) ? A[(size_t)r * lda + c0 + j] : mk<T>(0, 0);
+      setP(m, j, (r < N && j < nb) ? A[(size_t)r * lda + c0 + j] : mk<T>(0, 0));
   }
   int zero_col = 0;  // first breakdown column (1-based), tracked by thread 0
 
@@ -602,7 +608,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
       for (int m = 0; m < MR; ++m) {
         const int r = c0 + tid + 1024 * m;
         if (r >= c && r < N) {
-          const T v = cabs1(P[m][jj]);
+          const T v = cabs1(getP(m, jj));
           if (v > bv) {
             bv = v;
             bm = m;
@@ -631,10 +637,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
         const int r = c0 + tid + 1024 * m;
         if (r == wrow)
 #pragma unroll
-          for (int j = 0; j < NBI; ++j) s_cand[buf][w][j] = P[m][j];
+          for (int j = 0; j < NBI; ++j) s_cand[buf][w][j] = getP(m, j);
         if (r == c)
 #pragma unroll
-          for (int j = 0; j < NBI; ++j) s_crow[buf][j] = P[m][j];
+          for (int j = 0; j < NBI; ++j) s_crow[buf][j] = getP(m, j);
       }
       __syncthreads();
       // across the 16 waves: largest magnitude, then the lowest row holding it (LAPACK i?amax order)
@@ -652,10 +658,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
           const int r = c0 + tid + 1024 * m;
           if (r == c)
 #pragma unroll
-            for (int j = 0; j < NBI; ++j) P[m][j] = prow[j];
+            for (int j = 0; j < NBI; ++j) setP(m, j, prow[j]);
           if (r == pv)
 #pragma unroll
-            for (int j = 0; j < NBI; ++j) P[m][j] = s_crow[buf][j];
+            for (int j = 0; j < NBI; ++j) setP(m, j, s_crow[buf][j]);
         }
       }
       if (tid == 0) s_pivs[jj] = pv;
@@ -668,10 +674,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
         for (int m = 0; m < MR; ++m) {
           const int r = c0 + tid + 1024 * m;
           if (r > c && r < N) {
-            const cplx<T> l = cmul(P[m][jj], inv);
-            P[m][jj] = l;
+            const cplx<T> l = cmul(getP(m, jj), inv);
+            setP(m, jj, l);
 #pragma unroll
-            for (int j2 = jj + 1; j2 < NBI; ++j2) P[m][j2] = cfms(P[m][j2], l, prow[j2]);
+            for (int j2 = jj + 1; j2 < NBI; ++j2) setP(m, j2, cfms(getP(m, j2), l, prow[j2]));
           }
         }
       }
@@ -683,7 +689,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     if (r < N)
 #pragma unroll
       for (int j = 0; j < NBI; ++j)
-        if (j < nb) A[(size_t)r * lda + c0 + j] = P[m][j];
+        if (j < nb) A[(size_t)r * lda + c0 + j] = getP(m, j);
   }
   __syncthreads();
   if (w == 0) {
@@ -782,9 +788,11 @@ __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, 
 
 // ------------------------------------------------------------------------------------------------
 // Inverses of the 64 x 64 diagonal blocks of L (unit lower) and U (upper), stored per matrix in the LU
-// workspace W[mat][block][0 = L, 1 = U][64][64] (identity-padded past N). They turn the TRSM of each outer panel
-// into an MFMA product and the diagonal step of the triangular sweeps into a matrix-vector product.
-// One workgroup per (block, matrix); thread j computes column j by substitution in LDS.
+// workspace W[mat][block][0 = L, 1 = U][64][64] (identity-padded past N). They turn the U12 solves into MFMA
+// products and the diagonal steps of the triangular sweeps into matrix-vector products.
+// One workgroup per (block, matrix), block-recursive: the four 16 x 16 diagonal sub-blocks are inverted by
+// substitution (one thread per column), then pairs are joined at sizes 16 and 32 with
+//   [A 0; C B]^-1 = [A^-1 0; -B^-1 C A^-1  B^-1]     and     [A C; 0 B]^-1 = [A^-1  -A^-1 C B^-1; 0 B^-1].
 // ------------------------------------------------------------------------------------------------
 
 template <typename T, bool LOWER>
@@ -799,7 +807,8 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
   const cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
   const long long lda = mb.lda;
   const int tid = threadIdx.x;
-  __shared__ cplx<T> Bk[64][65];
+  __shared__ cplx<T> Bk[64][65];  // the triangle (identity past nr); joined blocks' off-diagonal parts are
+                                  // replaced by the intermediate products once read
   __shared__ cplx<T> X[64][65];
   for (int e = tid; e < 64 * 64; e += 256) {
     const int i = e >> 6, j = e & 63;
@@ -813,28 +822,76 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
     X[i][j] = mk<T>(T(0), T(0));
   }
   __syncthreads();
+  // 16 x 16 diagonal sub-blocks by substitution, one thread per column
   if (tid < 64) {
-    const int j = tid;
+    const int o = tid & 48, j = tid & 15;
     if (LOWER) {
-      // column j of L^-1: x_j = 1, x_i = -sum_{k=j}^{i-1} L_ik x_k
-      X[j][j] = mk<T>(T(1), T(0));
-      for (int i = j + 1; i < 64; ++i) {
+      X[o + j][o + j] = mk<T>(T(1), T(0));
+      for (int i = j + 1; i < 16; ++i) {
         cplx<T> acc = mk<T>(T(0), T(0));
-        for (int k = j; k < i; ++k) acc = cfma(acc, Bk[i][k], X[k][j]);
-        X[i][j] = mk<T>(-acc.re, -acc.im);
+        for (int k = j; k < i; ++k) acc = cfma(acc, Bk[o + i][o + k], X[o + k][o + j]);
+        X[o + i][o + j] = mk<T>(-acc.re, -acc.im);
       }
     } else {
-      // column j of U^-1: x_j = 1 / U_jj, x_i = -(sum_{k=i+1}^{j} U_ik x_k) / U_ii
-      X[j][j] = cdiv(mk<T>(T(1), T(0)), Bk[j][j]);
+      X[o + j][o + j] = cdiv(mk<T>(T(1), T(0)), Bk[o + j][o + j]);
       for (int i = j - 1; i >= 0; --i) {
         cplx<T> acc = mk<T>(T(0), T(0));
-        for (int k = i + 1; k <= j; ++k) acc = cfma(acc, Bk[i][k], X[k][j]);
-        const cplx<T> q = cdiv(acc, Bk[i][i]);
-        X[i][j] = mk<T>(-q.re, -q.im);
+        for (int k = i + 1; k <= j; ++k) acc = cfma(acc, Bk[o + i][o + k], X[o + k][o + j]);
+        const cplx<T> q = cdiv(acc, Bk[o + i][o + i]);
+        X[o + i][o + j] = mk<T>(-q.re, -q.im);
       }
     }
   }
   __syncthreads();
+  // join pairs of h x h inverses into 2h x 2h ones (h = 16: two pairs, h = 32: one pair)
+  for (int h = 16; h < 64; h <<= 1) {
+    const int E = 64 * h / 2;  // output entries over all pairs: (64 / 2h) pairs x h x h
+    cplx<T> t[4];
+    // T = C A^-1 (lower: C = L21) or C B^-1 (upper: C = U12), h x h per pair
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      t[u] = mk<T>(T(0), T(0));
+      if (e < E) {
+        const int p = e / (h * h), r = (e / h) % h, c = e % h, o = 2 * h * p;
+        cplx<T> acc = mk<T>(T(0), T(0));
+        for (int k = 0; k < h; ++k)
+          acc = LOWER ? cfma(acc, Bk[o + h + r][o + k], X[o + k][o + c])
+                      : cfma(acc, Bk[o + r][o + h + k], X[o + h + k][o + h + c]);
+        t[u] = acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      if (e < E) {
+        const int p = e / (h * h), r = (e / h) % h, c = e % h, o = 2 * h * p;
+        if (LOWER)
+          Bk[o + h + r][o + c] = t[u];
+        else
+          Bk[o + r][o + h + c] = t[u];
+      }
+    }
+    __syncthreads();
+    // lower: X21 = -B^-1 T ; upper: X12 = -A^-1 T
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      if (e < E) {
+        const int p = e / (h * h), r = (e / h) % h, c = e % h, o = 2 * h * p;
+        cplx<T> acc = mk<T>(T(0), T(0));
+        for (int k = 0; k < h; ++k)
+          acc = LOWER ? cfma(acc, X[o + h + r][o + h + k], Bk[o + h + k][o + c])
+                      : cfma(acc, X[o + r][o + k], Bk[o + k][o + h + c]);
+        if (LOWER)
+          X[o + h + r][o + c] = mk<T>(-acc.re, -acc.im);
+        else
+          X[o + r][o + h + c] = mk<T>(-acc.re, -acc.im);
+      }
+    }
+    __syncthreads();
+  }
   cplx<T>* Wb = W + (size_t)mat * strideW + ((size_t)kb * 2 + (LOWER ? 0 : 1)) * 4096;
   for (int e = tid; e < 64 * 64; e += 256) Wb[e] = X[e >> 6][e & 63];
 }
@@ -1193,7 +1250,7 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
   // (NBI, MR) pairs chosen so that the register-resident sub-panel does not spill at 128 VGPRs/lane
   if constexpr (sizeof(T) == 4) {
     if (mr <= 1) PANEL(16, 1);
-    else if (mr <= 2) PANEL(8, 2);
+    else if (mr <= 2) PANEL(16, 2);
     else if (mr <= 3) PANEL(8, 3);
     else if (mr <= 6) PANEL(4, 6);
     else if (mr <= 12) PANEL(2, 12);
@@ -1213,7 +1270,7 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
 template <typename T>
 int panel_width(int N, int c0) {
   const int mr = (N - c0 + 1023) / 1024;
-  if (sizeof(T) == 4) return mr <= 1 ? 16 : (mr <= 3 ? 8 : (mr <= 6 ? 4 : 2));
+  if (sizeof(T) == 4) return mr <= 2 ? 16 : (mr <= 3 ? 8 : (mr <= 6 ? 4 : 2));
   return mr <= 2 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
 }
 
