@@ -44,6 +44,7 @@ class MeshDesc(C.Structure):
         ("slot_start", _u16p),
         ("slot_inc", _u16p),
         ("slot_vertex", _ip),
+        ("slot_pend", _ip),
         ("vstar_ptr", _ip),
         ("vstar_row", _ip),
         ("n_pairs", _i),

@@ -12,9 +12,9 @@ Restates the reference's mesh / space / quadrature set-up in the layout the gfx9
 
 New for the HIP layout: trial elements are put in Morton (Z-curve) order of their centroids and cut into
 chunks of 64 (one lane each). Each chunk's distinct vertices are "slots"; the regular kernel adds each slot's
-value into that vertex's column of the test-element strip row (``slot_vertex``: the first chunk holding a vertex
-stores, later chunks accumulate), and the row gather sums the strip rows of each P1 row's star
-(``vstar_ptr/vstar_row``).
+value into that vertex's column of the test-element strip row (``slot_vertex``; a vertex shared by several chunks
+keeps its partial sums in LDS until its last chunk stores them, ``slot_pend``), and the row gather sums the strip
+rows of each P1 row's star (``vstar_ptr/vstar_row``).
 """
 
 from __future__ import annotations
@@ -447,12 +447,23 @@ class SlotTables:
     slot_count: np.ndarray
     slot_start: np.ndarray  # (n_chunks, 196) uint16
     slot_inc: np.ndarray  # (n_chunks, 192) uint16
-    slot_vertex: np.ndarray  # (n_chunks, 192) int32: 2 * vertex + (an earlier chunk holds the vertex)
+    slot_vertex: np.ndarray  # (n_chunks, 192) int32: 2 * vertex + (an earlier chunk stored the vertex column)
+    slot_pend: np.ndarray  # (n_chunks, 192) int32: LDS partial-sum slot | PEND_FROM_LDS | PEND_TO_STRIP
     vstar_ptr: np.ndarray
     vstar_row: np.ndarray
 
 
-def slot_tables(mesh: BoundaryMesh) -> SlotTables:
+PEND_SLOTS = 192  # LDS partial sums per wave in the regular kernel (BEM_PEND_SLOTS)
+PEND_NONE = 0xFFFF
+PEND_FROM_LDS = 1 << 16  # an earlier chunk left this vertex's partial sums in LDS
+PEND_TO_STRIP = 1 << 17  # the vertex's last chunk: the finished sums go to the strip
+
+
+def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
+    """Chunked trial order and per-chunk slot tables. A vertex that appears in several chunks keeps its partial
+    strip sums in one of ``pend_slots`` LDS slots from its first to its last chunk (interval allocation in chunk
+    order) and is stored once, by its last chunk; when no LDS slot is free it is stored by its first chunk and
+    read-modify-written by the later ones (``slot_vertex`` odd)."""
     F, V = mesh.n_faces, mesh.n_vertices
     centroids = mesh.face_vertices.mean(axis=1)
     perm = np.argsort(_morton(centroids), kind="stable")
@@ -463,7 +474,8 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
     slot_start = np.zeros((n_chunks, 196), dtype=np.uint16)
     slot_inc = np.zeros((n_chunks, 192), dtype=np.uint16)
     slot_vertex = np.zeros((n_chunks, 192), dtype=np.int32)
-    seen = np.zeros(V, dtype=bool)
+    slot_pend = np.full((n_chunks, 192), PEND_NONE | PEND_TO_STRIP, dtype=np.int32)
+    chunk_verts = []
     for c in range(n_chunks):
         elems = order[c * 64 : (c + 1) * 64]
         valid = elems >= 0
@@ -476,8 +488,30 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
         cnt = np.bincount(inv, minlength=len(uniq))
         slot_start[c, 1 : len(uniq) + 1] = np.cumsum(cnt)
         slot_count[c] = len(uniq)
-        slot_vertex[c, : len(uniq)] = 2 * uniq + seen[uniq]
-        seen[uniq] = True
+        chunk_verts.append(uniq)
+    first = np.full(V, -1, dtype=np.int64)
+    last = np.full(V, -1, dtype=np.int64)
+    for c, uniq in enumerate(chunk_verts):
+        first[uniq] = np.where(first[uniq] < 0, c, first[uniq])
+        last[uniq] = c
+    # LDS slots for multi-chunk vertices, allocated greedily in chunk order; a slot is reused from the chunk
+    # after its vertex's last one (within a chunk every live vertex has its own slot)
+    pend = np.full(V, -1, dtype=np.int64)
+    free = list(range(pend_slots - 1, -1, -1))
+    release = [[] for _ in range(n_chunks)]
+    for c, uniq in enumerate(chunk_verts):
+        for v in uniq:
+            if first[v] == c and last[v] > c and free:
+                pend[v] = free.pop()
+                release[last[v]].append(pend[v])
+        seen_before = first[uniq] < c
+        in_lds = pend[uniq] >= 0
+        code = np.where(in_lds, pend[uniq], PEND_NONE)
+        code = code | np.where(in_lds & seen_before, PEND_FROM_LDS, 0)
+        code = code | np.where(~in_lds | (last[uniq] == c), PEND_TO_STRIP, 0)
+        slot_pend[c, : len(uniq)] = code
+        slot_vertex[c, : len(uniq)] = 2 * uniq + (seen_before & ~in_lds)
+        free.extend(release[c])
     rows = np.arange(3 * F)
     rv = mesh.faces.reshape(-1)
     o2 = np.lexsort((rows, rv))
@@ -489,6 +523,7 @@ def slot_tables(mesh: BoundaryMesh) -> SlotTables:
         slot_start=slot_start,
         slot_inc=slot_inc,
         slot_vertex=slot_vertex,
+        slot_pend=slot_pend,
         vstar_ptr=np.cumsum(vs_ptr).astype(np.int32),
         vstar_row=rows[o2].astype(np.int32),
     )

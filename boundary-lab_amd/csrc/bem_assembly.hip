@@ -238,6 +238,7 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
   const int nfw = min(NFW, nf - f0);  // the last group may be partial: its spare slot repeats a wavenumber
   const int lane = threadIdx.x;
   __shared__ cplx<T> blk[NFW][64][9];
+  __shared__ cplx<T> pend[BEM_PEND_SLOTS][NFW][3];  // partial strip sums of vertices shared with later chunks
 
   T k[NFW], kappa[NFW];
 #pragma unroll
@@ -324,12 +325,15 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
     __syncthreads();
     const uint16_t* st = p.slot_start + (size_t)c * SLOT_STRIDE;
     const uint16_t* inc = p.slot_inc + (size_t)c * 192;
+    const int* sp = p.slot_pend + (size_t)c * 192;
     for (int kk = lane; kk < nsl; kk += 64) {
       const int e0 = st[kk], e1 = st[kk + 1];
       const bool first_slot = kk == lane;
       const int vcode = first_slot ? vc0 : sv[kk];
       const int v = vcode >> 1;
       const bool again = vcode & 1;  // an earlier chunk of this wave already stored this vertex column
+      const int pc = sp[kk];
+      const int pidx = pc & 0xFFFF;  // LDS slot of a vertex shared with other chunks (or none)
 #pragma unroll
       for (int j = 0; j < NFW; ++j) {
         if (j < nfw) {
@@ -342,7 +346,20 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
             a2 = cadd(a2, blk[j][src][6 + m]);
           }
           cplx<T>* rows = p.strip + (size_t)(f0 + j) * p.strip_fstride + (size_t)(3 * t) * p.L + v;
-          if (again && first_slot) {
+          if (pidx != BEM_PEND_NONE) {
+            // partial sums travel in LDS between the vertex's chunks; its last chunk stores them once
+            if (pc & BEM_PEND_FROM_LDS) {
+              a0 = cadd(a0, pend[pidx][j][0]);
+              a1 = cadd(a1, pend[pidx][j][1]);
+              a2 = cadd(a2, pend[pidx][j][2]);
+            }
+            if (!(pc & BEM_PEND_TO_STRIP)) {
+              pend[pidx][j][0] = a0;
+              pend[pidx][j][1] = a1;
+              pend[pidx][j][2] = a2;
+              continue;
+            }
+          } else if (again && first_slot) {
             a0 = cadd(a0, pre[j][0]);
             a1 = cadd(a1, pre[j][1]);
             a2 = cadd(a2, pre[j][2]);
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
         }
       }
     }
-    // the barrier orders this chunk's strip stores before the next chunk's read-modify-write of a shared vertex
+    // the barrier orders this chunk's strip and LDS partial-sum stores before the next chunk reads them
     __syncthreads();
   }
 }

@@ -115,7 +115,7 @@ struct bem_mesh {
   float* srcw32;
   double* srcw64;
   int4* fv;
-  int *order, *slot_count, *slot_vertex, *vstar_ptr, *vstar_row;
+  int *order, *slot_count, *slot_vertex, *slot_pend, *vstar_ptr, *vstar_row;
   uint16_t *slot_start, *slot_inc;
   double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
   int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces, *sing_by_rule;
@@ -283,6 +283,16 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
       return fail(BEM_E_ARG, "transform signs must be +1 or -1");
   for (int i = 0; i < d->n_pairs; ++i)
     if (d->sing_trial[i] < 0 || d->sing_trial[i] >= R * d->n_faces) return fail(BEM_E_ARG, "singular trial key out of range");
+  if (!d->slot_count || !d->slot_vertex || !d->slot_pend) return fail(BEM_E_ARG, "NULL slot table");
+  for (int ch = 0; ch < d->n_chunks; ++ch) {
+    if (d->slot_count[ch] < 0 || d->slot_count[ch] > BEM_SLOT_INC) return fail(BEM_E_ARG, "slot count out of range");
+    for (int k = 0; k < d->slot_count[ch]; ++k) {
+      const int v = d->slot_vertex[(size_t)ch * BEM_SLOT_INC + k] >> 1;
+      const int pd = d->slot_pend[(size_t)ch * BEM_SLOT_INC + k] & 0xFFFF;
+      if (v < 0 || v >= d->n_vertices) return fail(BEM_E_ARG, "slot vertex out of range");
+      if (pd != BEM_PEND_NONE && pd >= BEM_PEND_SLOTS) return fail(BEM_E_ARG, "LDS partial-sum slot out of range");
+    }
+  }
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   bem_mesh* m = new bem_mesh();
   m->ctx = c;
@@ -387,6 +397,7 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   UP(m->slot_start, d->slot_start, (size_t)d->n_chunks * 196);
   UP(m->slot_inc, d->slot_inc, (size_t)d->n_chunks * 192);
   UP(m->slot_vertex, d->slot_vertex, (size_t)d->n_chunks * 192);
+  UP(m->slot_pend, d->slot_pend, (size_t)d->n_chunks * 192);
   UP(m->vstar_ptr, d->vstar_ptr, (size_t)V + 1);
   UP(m->vstar_row, d->vstar_row, vs_nnz);
   UP(m->fvtx, d->face_vertices, (size_t)F * 9);
@@ -539,6 +550,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.slot_start = m->slot_start;
     p.slot_inc = m->slot_inc;
     p.slot_vertex = m->slot_vertex;
+    p.slot_pend = m->slot_pend;
     p.sing_ptr = m->sing_ptr;
     p.sing_trial = m->sing_trial;
     p.sing_lhs = static_cast<const cplx<double>*>(m->sing_lhs.p) + (size_t)f0 * m->P * 9;

@@ -5,6 +5,10 @@
 
 #define BEM_SLOT_STRIDE 196  // per-chunk slot_start entries (<= 192 slots + 1, padded)
 #define BEM_SLOT_INC 192     // per-chunk incidence entries (64 elements x 3 columns)
+#define BEM_PEND_SLOTS 192   // LDS partial strip sums per regular-assembly wave
+#define BEM_PEND_NONE 0xFFFF
+#define BEM_PEND_FROM_LDS (1 << 16)
+#define BEM_PEND_TO_STRIP (1 << 17)
 #define BEM_MAX_CH 4
 #define BEM_NBO 64           // outer LU panel width (trailing-update K)
 
@@ -33,7 +37,8 @@ struct RegParams {
   const int* slot_count;        // [n_chunks]
   const uint16_t* slot_start;   // [n_chunks][BEM_SLOT_STRIDE]
   const uint16_t* slot_inc;     // [n_chunks][BEM_SLOT_INC]  (lane << 2) | m
-  const int* slot_vertex;       // [n_chunks][BEM_SLOT_INC]  2 * vertex + (earlier chunk holds the vertex)
+  const int* slot_vertex;       // [n_chunks][BEM_SLOT_INC]  2 * vertex + (earlier chunk stored the column)
+  const int* slot_pend;         // [n_chunks][BEM_SLOT_INC]  LDS slot | BEM_PEND_FROM_LDS | BEM_PEND_TO_STRIP
   const int* sing_ptr;          // [F+1]
   const int* sing_trial;        // [P]
   const cplx<double>* sing_lhs; // [nf][P][9]

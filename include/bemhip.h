@@ -93,7 +93,9 @@ typedef struct bem_mesh_desc {
   const int* slot_count;     /* [n_chunks] */
   const uint16_t* slot_start;/* [n_chunks][196] */
   const uint16_t* slot_inc;  /* [n_chunks][192], (lane << 2) | local column */
-  const int* slot_vertex;    /* [n_chunks][192]: 2 * vertex + 1 if an earlier chunk already holds the vertex */
+  const int* slot_vertex;    /* [n_chunks][192]: 2 * vertex + 1 if an earlier chunk stored the vertex column */
+  const int* slot_pend;      /* [n_chunks][192]: LDS partial-sum slot (0xFFFF: none) | 1 << 16 (an earlier chunk
+                                left partial sums there) | 1 << 17 (store the sums to the strip) */
   const int* vstar_ptr;      /* [V+1] vertex -> strip rows (3t+l) CSR */
   const int* vstar_row;
   /* singular (vertex-sharing) pairs, grouped by test element */

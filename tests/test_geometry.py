@@ -80,8 +80,9 @@ def test_slot_tables_cover_every_incidence_once(sphere):
         for k in range(s.slot_count[c]):
             code_v = int(s.slot_vertex[c, k])
             vk, again = code_v >> 1, code_v & 1
-            # the first chunk holding a vertex stores its column, later chunks accumulate
-            assert again == (first[vk] > 0)
+            # a vertex without an LDS slot is stored by its first chunk and accumulated by later ones
+            in_lds = (int(s.slot_pend[c, k]) & 0xFFFF) != G.PEND_NONE
+            assert again == (first[vk] > 0 and not in_lds)
             first[vk] += 1
             for e in range(s.slot_start[c, k], s.slot_start[c, k + 1]):
                 code = int(s.slot_inc[c, e])
@@ -99,11 +100,15 @@ def test_slot_tables_cover_every_incidence_once(sphere):
     assert s.vstar_ptr[-1] == 3 * F
 
 
-def test_strip_store_and_gather_reproduce_dense_assembly(sphere):
-    # Emulate the regular kernel's strip stores (store on a vertex's first chunk, accumulate later) and the
-    # row gather with random element blocks: the result must equal the direct scatter of the blocks.
+@pytest.mark.parametrize("pend_slots", [G.PEND_SLOTS, 3, 0])
+def test_strip_store_and_gather_reproduce_dense_assembly(sphere, pend_slots):
+    # Emulate the regular kernel's strip stores with random element blocks: a multi-chunk vertex keeps its
+    # partial sums in an LDS slot until its last chunk stores them; without a free slot it is stored by its first
+    # chunk and read-modify-written later. Then the row gather. The result must equal the direct scatter, and no
+    # LDS slot may be shared by two live vertices.
     prep = G.prepare(sphere)
-    s, m = prep.slots, prep.mesh
+    m = prep.mesh
+    s = G.slot_tables(m, pend_slots=pend_slots)
     F, V = m.n_faces, m.n_vertices
     rng = np.random.default_rng(3)
     blocks = rng.standard_normal((F, F, 3, 3))  # [t, s, l, m]
@@ -112,15 +117,39 @@ def test_strip_store_and_gather_reproduce_dense_assembly(sphere):
         for l in range(3):
             np.add.at(dense[m.faces[t, l]], m.faces[:, :], blocks[t, :, l, :])
     strip = np.full((3 * F, V), np.nan)
+    lds = {}
+    owner = {}
+    used_lds = 0
     for c in range(len(s.slot_count)):
+        live = set()
         for k in range(s.slot_count[c]):
             vk, again = int(s.slot_vertex[c, k]) >> 1, int(s.slot_vertex[c, k]) & 1
+            pc = int(s.slot_pend[c, k])
+            pidx, from_lds, to_strip = pc & 0xFFFF, bool(pc & G.PEND_FROM_LDS), bool(pc & G.PEND_TO_STRIP)
             val = np.zeros((F, 3))
             for e in range(s.slot_start[c, k], s.slot_start[c, k + 1]):
                 code = int(s.slot_inc[c, e])
                 val += blocks[:, s.order[c * 64 + (code >> 2)], :, code & 3]
             val = val.reshape(3 * F)
-            strip[:, vk] = strip[:, vk] + val if again else val
+            if pidx != G.PEND_NONE:
+                assert pidx < pend_slots and pidx not in live
+                live.add(pidx)
+                used_lds += 1
+                if from_lds:
+                    assert owner[pidx] == vk
+                    val = val + lds[pidx]
+                else:
+                    assert not again
+                owner[pidx] = vk
+                if to_strip:
+                    assert np.all(np.isnan(strip[:, vk]))  # stored exactly once
+                    strip[:, vk] = val
+                else:
+                    lds[pidx] = val
+            else:
+                assert to_strip
+                strip[:, vk] = strip[:, vk] + val if again else val
+    assert (used_lds > 0) == (pend_slots > 0)
     A = np.zeros((V, V))
     for i in range(V):
         A[i] = strip[s.vstar_row[s.vstar_ptr[i] : s.vstar_ptr[i + 1]]].sum(axis=0)
