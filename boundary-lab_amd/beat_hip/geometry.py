@@ -578,11 +578,15 @@ class PreparedMesh:
 
 
 def locality_order(mesh: BoundaryMesh):
-    """Faces in Morton order of their centroids; vertices in order of first use by those faces."""
+    """Faces in Morton order of their centroids; vertices in order of last use by those faces. With vertices
+    numbered by last use, the vertices whose last chunk of 64 trial faces is chunk c are a contiguous range of
+    ids, so the strip stores of each chunk (made by a vertex's last chunk) are contiguous."""
     fperm = np.argsort(_morton(mesh.face_vertices.mean(axis=1)), kind="stable")
     flat = mesh.faces[fperm].reshape(-1)
-    ids, first = np.unique(flat, return_index=True)
-    used = ids[np.argsort(first, kind="stable")]
+    last = np.full(mesh.n_vertices, -1, dtype=np.int64)
+    np.maximum.at(last, flat, np.arange(len(flat)))
+    ids = np.nonzero(last >= 0)[0]
+    used = ids[np.argsort(last[ids], kind="stable")]
     unused = np.setdiff1d(np.arange(mesh.n_vertices), ids)
     vperm = np.concatenate([used, unused]).astype(np.int64)
     return fperm.astype(np.int64), vperm
