@@ -57,9 +57,10 @@ int bem_ctx_sync(bem_ctx* ctx);
 /* The hipStream_t used by the context. */
 int bem_ctx_stream(bem_ctx* ctx, void** stream);
 
-/* Batch fan-out (no reference counterpart; tuning). LU factorization and refinement split a batch of
- * independent matrices into up to `streams` contiguous groups (at least 4 matrices each) that run on separate
- * default-priority streams forked from and joined back into the context stream. Default 4; 1 disables. */
+/* Stream fan-out (no reference counterpart; tuning), on default-priority streams forked from and joined back
+ * into the context stream. Refinement splits a batch of independent matrices into up to `streams` contiguous
+ * groups (at least 4 matrices each); LU factorization uses a second stream for look-ahead when streams >= 2
+ * (same results bit for bit). Default 4; 1 runs everything on the context stream. */
 int bem_ctx_set_fanout(bem_ctx* ctx, int streams);
 
 /* Kernel timers (no reference counterpart; diagnostics for roofline measurement). While enabled, each timed
