@@ -32,20 +32,25 @@ for p in (ROOT, ROOT / "boundary-lab_amd"):
     sys.path.insert(0, str(p))
 
 METRIC = "freq-points/s @6k tris, mixed vs FP64 GPU (≤0.05 dB); p50/p99 job latency"
-# Algorithmic work of k_regular_lhs per quadrature-point pair (fixed from the kernel's arithmetic, FMA = 2):
-# geometry 9, rsqrt/r 2, phase 2 (+ sin, cos), Green and gradient 10, trial-normal projection 8,
-# inner contraction 24, outer contraction 12 (72 flops per test point / 6 trial points).
+# Algorithmic work of regular assembly per ordered quadrature-point pair (test point, trial point) and frequency
+# (fixed from the one-sided kernel's arithmetic, FMA = 2): geometry 9, rsqrt/r 2, phase 2 (+ sin, cos), Green and
+# gradient 10, trial-normal projection 8, inner contraction 24, outer contraction 12 (72 flops per test point /
+# 6 trial points). `achieved` counts this work over the (F^2 - adjacent) ordered element pairs.
 FLOPS_PER_QPAIR = 66
+# What k_regular_pair executes per unordered quadrature-point pair and frequency: both blocks of the pair from one
+# kernel evaluation, geometry/phase/Green shared (23), two normal projections (16), six contracted components
+# (inner 36, outer 18) = 93, i.e. 93 / (2 x 66) = 0.70 of the ordered-pair work.
+FLOPS_PER_UNORDERED_QPAIR = 93
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 PIPE_NOTE = ("compute-bound on the FP32 vector ALUs (no MFMA, no HBM limit); 157.3 TFLOP/s is MI355X's FP32 peak, "
              "shared by the matrix pipe and the packed-FP32 vector pipe; without packed FP32 the VALU issue ceiling "
              "is half of it")
-TRAFFIC_PROFILE = ROOT / "profiles" / "k_regular_lhs_horn6k_traffic.md"
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_regular_pair_horn6k_traffic.md"
 
 
 def measured_traffic():
-    """HBM bytes per k_regular_lhs<float> launch from the committed rocprofv3 counter summary (FETCH_SIZE +
+    """HBM bytes per k_regular_pair<2> launch from the committed rocprofv3 counter summary (FETCH_SIZE +
     WRITE_SIZE per dispatch, each scaled by the factor a plain copy of known size reads), or None."""
     try:
         vals = {}
@@ -61,7 +66,7 @@ def measured_traffic():
 def traffic_probe():
     """Workload for the two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE): a 1 GiB device-to-device copy
     (calibration: the copy kernel fetches and writes exactly 1 GiB) followed by one eager assembly of 4
-    frequencies of the horn job (one k_regular_lhs<float> launch)."""
+    frequencies of the horn job (one k_regular_pair<2> launch)."""
     import torch
 
     from beat_hip.engine import Engine
@@ -259,9 +264,10 @@ def main():
     freqs_per_launch = units / max(1, regions)
     qpairs = (F * F - len(prep.pairs.test)) * 36 * freqs_per_launch
     achieved = qpairs * FLOPS_PER_QPAIR / (avg_ms * 1e-3) / 1e12
+    executed = 0.5 * qpairs * FLOPS_PER_UNORDERED_QPAIR / (avg_ms * 1e-3) / 1e12
     traffic = measured_traffic()
-    log(f"bench: k_regular_lhs<float> avg {avg_ms:.3f} ms/launch ({freqs_per_launch:g} freqs) -> "
-        f"{achieved:.1f} TFLOP/s; stages (ms/job): "
+    log(f"bench: k_regular_pair<2> avg {avg_ms:.3f} ms/launch ({freqs_per_launch:g} freqs) -> "
+        f"{achieved:.1f} TFLOP/s of ordered-pair work ({executed:.1f} executed); stages (ms/job): "
         + ", ".join(f"{k} {v[0]:.1f}" for k, v in stages.items()))
 
     # ---- plain FP64 path: one frequency per launch ------------------------------------------------
@@ -321,9 +327,11 @@ def main():
         "max_relative_residual": max_res,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "k_regular_lhs<float>", "pipe": PIPE_NOTE, "avg_launch_ms": round(avg_ms, 4),
+                     "kernel": "k_regular_pair<2>", "pipe": PIPE_NOTE, "avg_launch_ms": round(avg_ms, 4),
                      "freqs_per_launch": freqs_per_launch, "flops_per_qpair": FLOPS_PER_QPAIR,
                      "qpairs_per_launch": int(qpairs),
+                     "executed_tflops": round(executed, 3),
+                     "executed_flops_per_unordered_qpair": FLOPS_PER_UNORDERED_QPAIR,
                      "frac_of_unpacked_fp32_valu": round(achieved / (PEAK_FP32_TFLOPS / 2), 4)},
         "stages_ms_per_job": {k: round(v[0], 3) for k, v in stages.items()},
         "cpu_baseline": cpu,

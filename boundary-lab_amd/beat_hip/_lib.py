@@ -45,6 +45,10 @@ class MeshDesc(C.Structure):
         ("slot_inc", _u16p),
         ("slot_vertex", _ip),
         ("slot_pend", _ip),
+        ("slot_prev", _ip),
+        ("elem_pos", _ip),
+        ("chunk_start", _ip),
+        ("vertex_last", _ip),
         ("vstar_ptr", _ip),
         ("vstar_row", _ip),
         ("n_pairs", _i),

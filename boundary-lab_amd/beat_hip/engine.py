@@ -101,6 +101,10 @@ class DeviceMesh:
         d.slot_inc = _np_ptr(arr("si", s.slot_inc, np.uint16), C.c_uint16)
         d.slot_vertex = _np_ptr(arr("sv", s.slot_vertex, np.int32), C.c_int)
         d.slot_pend = _np_ptr(arr("spd", s.slot_pend, np.int32), C.c_int)
+        d.slot_prev = _np_ptr(arr("spv", s.slot_prev, np.int32), C.c_int)
+        d.elem_pos = _np_ptr(arr("epos", s.elem_pos, np.int32), C.c_int)
+        d.chunk_start = _np_ptr(arr("cst", s.chunk_start, np.int32), C.c_int)
+        d.vertex_last = _np_ptr(arr("vlast", s.vertex_last, np.int32), C.c_int)
         d.vstar_ptr = _np_ptr(arr("vsp", s.vstar_ptr, np.int32), C.c_int)
         d.vstar_row = _np_ptr(arr("vsr", s.vstar_row, np.int32), C.c_int)
         p = prep.pairs

@@ -449,11 +449,15 @@ class SlotTables:
     slot_inc: np.ndarray  # (n_chunks, 192) uint16
     slot_vertex: np.ndarray  # (n_chunks, 192) int32: 2 * vertex + (an earlier chunk stored the vertex column)
     slot_pend: np.ndarray  # (n_chunks, 192) int32: LDS partial-sum slot | PEND_FROM_LDS | PEND_TO_STRIP
+    slot_prev: np.ndarray  # (n_chunks, 192) int32: previous chunk holding the slot's vertex, -1 if none
+    elem_pos: np.ndarray  # (F,) int32: position of each element in the trial order
+    chunk_start: np.ndarray  # (F,) int32: first chunk holding the element or an element sharing a vertex with it
+    vertex_last: np.ndarray  # (V,) int32: last chunk holding each vertex (-1: unused vertex)
     vstar_ptr: np.ndarray
     vstar_row: np.ndarray
 
 
-PEND_SLOTS = 192  # LDS partial sums per wave in the regular kernel (BEM_PEND_SLOTS)
+PEND_SLOTS = 96  # LDS partial-sum slots per wave in the regular kernels (BEM_PEND_SLOTS)
 PEND_NONE = 0xFFFF
 PEND_FROM_LDS = 1 << 16  # an earlier chunk left this vertex's partial sums in LDS
 PEND_TO_STRIP = 1 << 17  # the vertex's last chunk: the finished sums go to the strip
@@ -494,14 +498,36 @@ def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
     for c, uniq in enumerate(chunk_verts):
         first[uniq] = np.where(first[uniq] < 0, c, first[uniq])
         last[uniq] = c
-    # LDS slots for multi-chunk vertices, allocated greedily in chunk order; a slot is reused from the chunk
-    # after its vertex's last one (within a chunk every live vertex has its own slot)
+    # LDS slots for multi-chunk vertices. The live ranges [first, last] are intervals over chunks; keep the largest
+    # set that needs at most pend_slots slots at every chunk (scan by start, and when over capacity drop the
+    # interval that ends last), then colour it in chunk order. A slot is reused from the chunk after its
+    # vertex's last one (within a chunk every live vertex has its own slot).
+    import heapq
+
+    multi = np.nonzero((first >= 0) & (last > first))[0]
+    keep = np.zeros(V, dtype=bool)
+    active = []  # max-heap by last chunk of the kept intervals that may still cover the scan point
+    for v in multi[np.lexsort((last[multi], first[multi]))]:
+        keep[v] = True
+        heapq.heappush(active, (-int(last[v]), int(v)))
+        # kept intervals covering first[v]; over capacity: drop the one ending last
+        covering = [u for _, u in active if keep[u] and last[u] >= first[v]]
+        if len(covering) > pend_slots:
+            while active and (not keep[active[0][1]] or last[active[0][1]] < first[v]):
+                heapq.heappop(active)
+            _, u = heapq.heappop(active)
+            keep[u] = False
+        if len(active) > 4 * pend_slots:  # prune expired entries
+            active = [(e, u) for e, u in active if keep[u] and last[u] >= first[v]]
+            heapq.heapify(active)
     pend = np.full(V, -1, dtype=np.int64)
+    slot_prev = np.full((n_chunks, 192), -1, dtype=np.int32)
+    last_seen = np.full(V, -1, dtype=np.int64)
     free = list(range(pend_slots - 1, -1, -1))
     release = [[] for _ in range(n_chunks)]
     for c, uniq in enumerate(chunk_verts):
         for v in uniq:
-            if first[v] == c and last[v] > c and free:
+            if first[v] == c and keep[v]:
                 pend[v] = free.pop()
                 release[last[v]].append(pend[v])
         seen_before = first[uniq] < c
@@ -511,7 +537,17 @@ def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
         code = code | np.where(~in_lds | (last[uniq] == c), PEND_TO_STRIP, 0)
         slot_pend[c, : len(uniq)] = code
         slot_vertex[c, : len(uniq)] = 2 * uniq + (seen_before & ~in_lds)
+        slot_prev[c, : len(uniq)] = last_seen[uniq]
+        last_seen[uniq] = c
         free.extend(release[c])
+    # paired assembly: element positions, the first chunk a test element's wave must visit (its own or that of an
+    # element sharing a vertex with it), and each vertex's last chunk (the strip columns a wave stores)
+    elem_pos = np.empty(F, dtype=np.int64)
+    elem_pos[perm] = np.arange(F)
+    echunk = elem_pos // 64
+    vmin = np.full(V, n_chunks, dtype=np.int64)
+    np.minimum.at(vmin, mesh.faces.reshape(-1), np.repeat(echunk, 3))
+    chunk_start = np.minimum(echunk, vmin[mesh.faces].min(axis=1))
     rows = np.arange(3 * F)
     rv = mesh.faces.reshape(-1)
     o2 = np.lexsort((rows, rv))
@@ -524,6 +560,10 @@ def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
         slot_inc=slot_inc,
         slot_vertex=slot_vertex,
         slot_pend=slot_pend,
+        slot_prev=slot_prev,
+        elem_pos=elem_pos.astype(np.int32),
+        chunk_start=chunk_start.astype(np.int32),
+        vertex_last=last.astype(np.int32),
         vstar_ptr=np.cumsum(vs_ptr).astype(np.int32),
         vstar_row=rows[o2].astype(np.int32),
     )

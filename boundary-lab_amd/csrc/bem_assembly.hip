@@ -229,6 +229,107 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
   }
 }
 
+// Both blocks of an unordered pair {t, s} from one evaluation of the kernel at the 36 point pairs: G is symmetric,
+// so the (s, t) block differs from the (t, s) block only in its double-layer part, which takes the derivative
+// along the other element's normal. C receives the (t, s) block, Cp[l][m] the (s, t) block's entry (m, l).
+template <int NFW>
+__device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, const float (&nt)[3],
+                                                 const float (&ct)[9], float a2t, const ElemRec<float>& se,
+                                                 const float (&k)[NFW], const float (&kappa)[NFW],
+                                                 cplx<float> (&C)[NFW][9], cplx<float> (&Cp)[NFW][9]) {
+  float y[18];
+#pragma unroll
+  for (int i = 0; i < 18; ++i) y[i] = se.q[i];
+  const float nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
+  const float ntx = nt[0], nty = nt[1], ntz = nt[2];
+  float DMr[NFW][9], DMi[NFW][9], GMr[NFW][9], GMi[NFW][9], EMr[NFW][9], EMi[NFW][9];
+#pragma unroll
+  for (int j = 0; j < NFW; ++j)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) DMr[j][i] = DMi[j][i] = GMr[j][i] = GMi[j][i] = EMr[j][i] = EMi[j][i] = 0.f;
+#pragma unroll 1
+  for (int a = 0; a < 6; ++a) {
+    const float xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
+    float xDr[NFW][6], xDi[NFW][6], xGr[NFW][6], xGi[NFW][6], xEr[NFW][6], xEi[NFW][6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const float dx = y[3 * b + 0] - xa;
+      const float dy = y[3 * b + 1] - ya;
+      const float dz = y[3 * b + 2] - za;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float rinv = Phase<float>::rsqrt_(r2);
+      const float r = r2 * rinv;
+      const float td = (dx * nsx + dy * nsy + dz * nsz) * rinv;   // (s, t): trial normal n_s
+      const float te = -(dx * ntx + dy * nty + dz * ntz) * rinv;  // (t, s) seen from s: trial normal n_t
+#pragma unroll
+      for (int j = 0; j < NFW; ++j) {
+        float sn, cs;
+        Phase<float>::sincos_kr(kappa[j], k[j], r, &sn, &cs);
+        const float gr = cs * rinv, gi = sn * rinv;  // 4 pi G
+        const float hr = -gr * rinv - gi * k[j];     // 4 pi G (ik - 1/r)
+        const float hi = gr * k[j] - gi * rinv;
+        xDr[j][b] = hr * td;
+        xDi[j][b] = hi * td;
+        xEr[j][b] = hr * te;
+        xEi[j][b] = hi * te;
+        xGr[j][b] = gr;
+        xGi[j][b] = gi;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NFW; ++j) {
+      float tDr[3], tDi[3], tGr[3], tGi[3], tEr[3], tEi[3];
+      trial_contract(xDr[j], tDr);
+      trial_contract(xDi[j], tDi);
+      trial_contract(xGr[j], tGr);
+      trial_contract(xGi[j], tGi);
+      trial_contract(xEr[j], tEr);
+      trial_contract(xEi[j], tEi);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const float wa = float(c_phiw.v[l * 6 + a]);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          DMr[j][3 * l + m] += wa * tDr[m];
+          DMi[j][3 * l + m] += wa * tDi[m];
+          GMr[j][3 * l + m] += wa * tGr[m];
+          GMi[j][3 * l + m] += wa * tGi[m];
+          EMr[j][3 * l + m] += wa * tEr[m];
+          EMi[j][3 * l + m] += wa * tEi[m];
+        }
+      }
+    }
+  }
+  const float nn = ntx * nsx + nty * nsy + ntz * nsz;
+  const float scale = a2t * se.a2 * float(BEM_INV_FOUR_PI);
+  float cp[9];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      cp[3 * l + m] = ct[3 * l + 0] * se.curl[3 * m + 0] + ct[3 * l + 1] * se.curl[3 * m + 1] +
+                      ct[3 * l + 2] * se.curl[3 * m + 2];
+#pragma unroll
+  for (int j = 0; j < NFW; ++j) {
+    float gsr = 0.f, gsi = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      gsr += GMr[j][i];
+      gsi += GMi[j][i];
+    }
+    const float k2nn = k[j] * k[j] * nn;
+    const float kinv = 1.f / k[j];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const float Hr = cp[i] * gsr - k2nn * GMr[j][i];
+      const float Hi = cp[i] * gsi - k2nn * GMi[j][i];
+      // -D + (i/k) H for both orientations; H is the same (transposed) for both
+      C[j][i] = mk<float>(scale * (-DMr[j][i] - Hi * kinv), scale * (-DMi[j][i] + Hr * kinv));
+      Cp[j][i] = mk<float>(scale * (-EMr[j][i] - Hi * kinv), scale * (-EMi[j][i] + Hr * kinv));
+    }
+  }
+}
+
 // One wave per (test element t, group of NFW wavenumbers); FP32 shares the pair geometry across NFW = 2
 // wavenumbers, FP64 runs one wavenumber per wave (NFW = 1).
 template <typename T, int NFW, bool SYM>
@@ -376,6 +477,147 @@ __global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
     }
     // the barrier orders this chunk's strip and LDS partial-sum stores before the next chunk reads them
     __syncthreads();
+  }
+}
+
+// Paired FP32 assembly (no symmetry images): the wave of test element t evaluates each unordered pair once.
+// It visits the trial chunks from chunk_start[t] (its own chunk, or the first chunk holding an element that
+// shares a vertex with t); a trial element s after t in the trial order that shares no vertex with t gives the
+// (t, s) block to t's row strip and the transposed (s, t) block to t's column strip (one kernel evaluation for
+// both, regular_pair_f32); an element sharing a vertex gives its FP64 singular block to the row strip; earlier
+// non-adjacent elements are left to their own waves. Row-strip columns are stored for vertices whose last chunk
+// is at or after chunk_start[t], column-strip columns for those at or after t's own chunk; the gather masks the
+// others. Partial sums of shared vertices travel in LDS (or are read-modify-written) exactly as in k_regular_lhs,
+// with "an earlier chunk of this wave" judged from the vertex's previous chunk.
+template <int NFW>
+__global__ __launch_bounds__(64) void k_regular_pair(RegParams<float> p, int nf) {
+  const int t = blockIdx.x;
+  const int f0 = blockIdx.y * NFW;
+  const int nfw = min(NFW, nf - f0);
+  const int lane = threadIdx.x;
+  __shared__ cplx<float> blk[NFW][64][9];
+  __shared__ cplx<float> pend[BEM_PEND_SLOTS][NFW][6];  // row-strip sums [0..2], column-strip sums [3..5]
+
+  float k[NFW], kappa[NFW];
+#pragma unroll
+  for (int j = 0; j < NFW; ++j) {
+    const double kd = p.k[f0 + min(j, nfw - 1)];
+    k[j] = float(kd);
+    kappa[j] = float(kd / (2.0 * BEM_PI));
+  }
+  const ElemRec<float>* te = p.rec + t;
+  float nt[3], ct[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) nt[i] = te->n[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) ct[i] = te->curl[i];
+  const float a2t = te->a2;
+  const int4 tv = p.fv[t];
+  const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
+  const int tpos = p.elem_pos[t];
+  const int c_row = p.chunk_start[t], c_col = tpos >> 6;
+
+  // per-slot sums of the chunk's blocks in blk into one strip (w = 0 row strip, 1 column strip)
+  auto store_slots = [&](int c, int w, int c_first) {
+    const int nsl = p.slot_count[c];
+    const uint16_t* st = p.slot_start + (size_t)c * SLOT_STRIDE;
+    const uint16_t* inc = p.slot_inc + (size_t)c * 192;
+    const int* sv = p.slot_vertex + (size_t)c * 192;
+    const int* sp = p.slot_pend + (size_t)c * 192;
+    const int* spv = p.slot_prev + (size_t)c * 192;
+    cplx<float>* strip = w == 0 ? p.strip : p.strip2;
+    for (int kk = lane; kk < nsl; kk += 64) {
+      const int e0 = st[kk], e1 = st[kk + 1];
+      const int vcode = sv[kk];
+      const int v = vcode >> 1;
+      const bool seen = spv[kk] >= c_first;  // an earlier chunk of this wave held the vertex
+      const bool again = (vcode & 1) && seen;
+      const int pc = sp[kk];
+      const int pidx = pc & 0xFFFF;
+#pragma unroll
+      for (int j = 0; j < NFW; ++j) {
+        if (j < nfw) {
+          cplx<float> a0 = mk<float>(0, 0), a1 = mk<float>(0, 0), a2 = mk<float>(0, 0);
+          for (int e = e0; e < e1; ++e) {
+            const int code = inc[e];
+            const int src = code >> 2, m = code & 3;
+            a0 = cadd(a0, blk[j][src][m]);
+            a1 = cadd(a1, blk[j][src][3 + m]);
+            a2 = cadd(a2, blk[j][src][6 + m]);
+          }
+          cplx<float>* rows = strip + (size_t)(f0 + j) * p.strip_fstride + (size_t)(3 * t) * p.L + v;
+          if (pidx != BEM_PEND_NONE) {
+            if ((pc & BEM_PEND_FROM_LDS) && seen) {
+              a0 = cadd(a0, pend[pidx][j][3 * w + 0]);
+              a1 = cadd(a1, pend[pidx][j][3 * w + 1]);
+              a2 = cadd(a2, pend[pidx][j][3 * w + 2]);
+            }
+            if (!(pc & BEM_PEND_TO_STRIP)) {
+              pend[pidx][j][3 * w + 0] = a0;
+              pend[pidx][j][3 * w + 1] = a1;
+              pend[pidx][j][3 * w + 2] = a2;
+              continue;
+            }
+          } else if (again) {
+            a0 = cadd(a0, rows[0]);
+            a1 = cadd(a1, rows[p.L]);
+            a2 = cadd(a2, rows[2 * (size_t)p.L]);
+          }
+          rows[0] = a0;
+          rows[p.L] = a1;
+          rows[2 * (size_t)p.L] = a2;
+        }
+      }
+    }
+  };
+
+  for (int c = c_row; c < p.n_chunks; ++c) {
+    const int spos = c * 64 + lane;
+    const int s = p.order[spos];
+    cplx<float> C[NFW][9], Cp[NFW][9];
+#pragma unroll
+    for (int j = 0; j < NFW; ++j)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) C[j][i] = Cp[j][i] = mk<float>(0, 0);
+    if (s >= 0) {
+      const int4 sv = p.fv[s];
+      const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
+                       sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
+      if (adj) {
+        int qs = -1;
+        for (int q = sp0; q < sp1; ++q)
+          if (p.sing_trial[q] == s) {
+            qs = q;
+            break;
+          }
+        if (qs >= 0) {
+#pragma unroll
+          for (int j = 0; j < NFW; ++j) {
+            const cplx<double>* sl = p.sing_lhs + ((size_t)(f0 + min(j, nfw - 1)) * p.P + qs) * 9;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) C[j][i] = mk<float>(float(sl[i].re), float(sl[i].im));
+          }
+        }
+      } else if (spos > tpos) {
+        regular_pair_f32<NFW>(te->q, nt, ct, a2t, p.rec[s], k, kappa, C, Cp);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NFW; ++j)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) blk[j][lane][i] = C[j][i];
+    __syncthreads();
+    store_slots(c, 0, c_row);
+    __syncthreads();  // orders blk reads before the column blocks overwrite it
+    if (c >= c_col) {
+#pragma unroll
+      for (int j = 0; j < NFW; ++j)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) blk[j][lane][i] = Cp[j][i];
+      __syncthreads();
+      store_slots(c, 1, c_col);
+      __syncthreads();
+    }
   }
 }
 
@@ -672,9 +914,128 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
   }
 }
 
+// Paired gather, one 64 x 64 tile of A per workgroup: A[i][j] = sum over the row strips of star(i) at column j
+// (stored when j's last chunk is at or after the element's chunk_start) + sum over the column strips of star(j)
+// at column i (stored when i's last chunk is at or after the element's own chunk). Wave w owns tile rows
+// [16w, 16w+16) for the row strips and tile columns [16w, 16w+16) for the column strips, walks the flat star
+// entries of those 16 vertices eight at a time and adds each 64-column strip segment into the LDS tile. The
+// segments are buffer loads with one descriptor per strip row: a lane whose column was not stored gets an
+// offset past the descriptor's range, which reads zero without touching memory, so the eight loads of a batch
+// are issued back to back with no branch between them. Row maxima of |A| (|re|+|im|) meet across tiles by an
+// atomic max on the FP64 bit pattern (non-negative, order-independent).
+typedef unsigned int gather_u2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ gather_u2 strip_segment(const cplx<float>* row, int len, int off) {
+  __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)row, (short)0, len * 8, 0x00020000);
+  return __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+}
+
+// Adds the strip segments of star entries [qa, qb) into the LDS tile: entry q's strip row (column) is read at
+// byte offset `off` per lane when the lane's vertex_last `vl` reaches the entry's first stored chunk (field
+// `thr_field` of the entry), and added to tile element `(key - base, lane)` (TRANSPOSED: `(lane, key - base)`).
+// One vector load brings 64 entries into the lanes; each is then broadcast by readlane, so the descriptors are
+// scalar and NB segment loads are in flight back to back.
+template <int NB, bool TRANSPOSED>
+__device__ __forceinline__ void add_star_segments(cplx<float> (*tile)[65], const cplx<float>* strip, int len,
+                                                  const int4* ent, int qa, int qb, int base, int lane, int vl,
+                                                  int off, int thr_field) {
+  constexpr int OOB = 0x40000000;  // byte offset past every strip row (at most 2^24 vertices per mesh)
+  for (int q0 = qa; q0 < qb; q0 += 64) {
+    const int4 my = ent[min(q0 + lane, qb - 1)];
+    const int mthr = thr_field == 1 ? my.y : my.z;
+    const int n = min(64, qb - q0);
+    for (int u0 = 0; u0 < n; u0 += NB) {
+      gather_u2 v[NB];
+      int key[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int src = min(u0 + u, n - 1);
+        const int r = __builtin_amdgcn_readlane(my.x, src);
+        const int thr = __builtin_amdgcn_readlane(mthr, src);
+        key[u] = u0 + u < n ? __builtin_amdgcn_readlane(my.w, src) - base : -1;
+        v[u] = strip_segment(strip + (size_t)r * len, len, (u0 + u < n && vl >= thr) ? off : OOB);
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u)
+        if (key[u] >= 0) {
+          cplx<float>& a = TRANSPOSED ? tile[lane][key[u]] : tile[key[u]][lane];
+          a = mk<float>(a.re + __uint_as_float(v[u].x), a.im + __uint_as_float(v[u].y));
+        }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_pair(GatherParams<T> g) {
+  static_assert(sizeof(T) == 4, "the paired strips are single precision");
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64, f = blockIdx.z;
+  // the wave index is uniform: keeping it scalar keeps the entry walk and the descriptors in SGPRs
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  __shared__ cplx<T> tile[64][65];
+  const cplx<T>* rs = g.strip + (size_t)f * g.strip_fstride;
+  const cplx<T>* cs = g.strip2 + (size_t)f * g.strip_fstride;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) tile[16 * w + r][lane] = mk<T>(0, 0);
+  // part 1: row strips of the star of rows [i0 + 16w, i0 + 16w + 16), columns j0 + lane
+  {
+    const int j = j0 + lane;
+    const int jl = j < g.N ? g.vertex_last[j] : -1;
+    const int qa = g.vstar_ptr[min(i0 + 16 * w, g.N)], qb = g.vstar_ptr[min(i0 + 16 * w + 16, g.N)];
+    add_star_segments<8, false>(tile, rs, g.L, g.vstar_ent, qa, qb, i0, lane, jl, j * 8, 1);
+  }
+  __syncthreads();
+  // part 2: column strips of the star of columns [j0 + 16w, j0 + 16w + 16), rows i0 + lane, added transposed
+  {
+    const int i = i0 + lane;
+    const int il = i < g.N ? g.vertex_last[i] : -1;
+    const int qa = g.vstar_ptr[min(j0 + 16 * w, g.N)], qb = g.vstar_ptr[min(j0 + 16 * w + 16, g.N)];
+    add_star_segments<8, true>(tile, cs, g.L, g.vstar_ent, qa, qb, j0, lane, il, i * 8, 2);
+  }
+  __syncthreads();
+  // store rows (one wave per row at a time) and fold the row maxima
+  for (int ii = w; ii < 64; ii += 4) {
+    const int i = i0 + ii, j = j0 + lane;
+    double v = 0.0;
+    if (i < g.N && j < g.N) {
+      const cplx<T> a = tile[ii][lane];
+      g.A[(size_t)f * g.strideA + (size_t)i * g.lda + j] = a;
+      if (g.A0) g.A0[(size_t)f * g.strideA + (size_t)i * g.lda + j] = a;
+      v = (double)(fabs(a.re) + fabs(a.im));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if (lane == 0 && i < g.N)
+      atomicMax(reinterpret_cast<unsigned long long*>(g.rowmax + (size_t)f * g.N + i),
+                (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// b[c][i] = scale_f * w_i * sum over (t, l) in star(i) of rhs_part[3t+l][c]; rowmax[f][i] = 0 (before the paired
+// gather's atomic maxima)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_rhs(GatherParams<T> g) {
+  const int i = blockIdx.x * 256 + threadIdx.x, f = blockIdx.y;
+  if (i >= g.N) return;
+  g.rowmax[(size_t)f * g.N + i] = 0.0;
+  const double wi = g.roww ? g.roww[i] : 1.0;
+  const cplx<double>* rp = g.rhs_part + (size_t)f * 3 * g.F * MAX_CH;
+  for (int c = 0; c < g.n_ch; ++c) {
+    cplx<double> acc = mk<double>(0, 0);
+    for (int q = g.vstar_ptr[i]; q < g.vstar_ptr[i + 1]; ++q) acc = cadd(acc, rp[(size_t)g.vstar_row[q] * MAX_CH + c]);
+    acc = mk<double>(acc.re * wi, acc.im * wi);
+    g.b[(size_t)f * g.strideB + (size_t)c * g.ldb + i] = cmul(acc, g.rhs_scale[f]);
+  }
+}
+
 template <typename T>
 hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
   constexpr int NFW = sizeof(T) == 4 ? 2 : 1;
+  if constexpr (sizeof(T) == 4) {
+    if (p.strip2) {
+      hipLaunchKernelGGL((k_regular_pair<NFW>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
+      return hipGetLastError();
+    }
+  }
   if (p.R > 1)
     hipLaunchKernelGGL((k_regular_lhs<T, NFW, true>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
   else
@@ -697,6 +1058,16 @@ hipError_t launch_rhs(const RhsParams<T>& p, int nf, hipStream_t st) {
 }
 template <typename T>
 hipError_t launch_gather(const GatherParams<T>& g, int nf, hipStream_t st) {
+  if constexpr (sizeof(T) == 4) {
+    if (g.strip2) {
+      hipLaunchKernelGGL(k_gather_rhs<T>, dim3((g.N + 255) / 256, nf), dim3(256), 0, st, g);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      const int nt = (g.N + 63) / 64;
+      hipLaunchKernelGGL(k_gather_pair<T>, dim3(nt, nt, nf), dim3(256), 0, st, g);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL(k_gather<T>, dim3(g.N, nf), dim3(256), 0, st, g);
   return hipGetLastError();
 }

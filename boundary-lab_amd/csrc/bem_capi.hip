@@ -115,7 +115,9 @@ struct bem_mesh {
   float* srcw32;
   double* srcw64;
   int4* fv;
-  int *order, *slot_count, *slot_vertex, *slot_pend, *vstar_ptr, *vstar_row;
+  int *order, *slot_count, *slot_vertex, *slot_pend, *slot_prev, *elem_pos, *chunk_start, *vertex_last, *vstar_ptr,
+      *vstar_row;
+  int4* vstar_ent;  // per star entry: strip row, first chunk of its row / column strip, the star's vertex
   uint16_t *slot_start, *slot_inc;
   double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
   int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces, *sing_by_rule;
@@ -275,6 +277,7 @@ int bem_ctx_stream(bem_ctx* c, void** s) {
 int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   if (!c || !d || !out) return fail(BEM_E_ARG, "NULL argument");
   if (d->n_faces <= 0 || d->n_vertices <= 0) return fail(BEM_E_ARG, "empty mesh");
+  if (d->n_vertices > (1 << 24)) return fail(BEM_E_ARG, "more than 2^24 vertices");
   if (d->n_chunks != (d->n_faces + 63) / 64) return fail(BEM_E_ARG, "n_chunks must be ceil(F/64)");
   const int R = d->n_transforms <= 0 ? 1 : d->n_transforms;
   if (R != 1 && R != 2 && R != 4) return fail(BEM_E_ARG, "n_transforms must be 1, 2 or 4");
@@ -283,7 +286,14 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
       return fail(BEM_E_ARG, "transform signs must be +1 or -1");
   for (int i = 0; i < d->n_pairs; ++i)
     if (d->sing_trial[i] < 0 || d->sing_trial[i] >= R * d->n_faces) return fail(BEM_E_ARG, "singular trial key out of range");
-  if (!d->slot_count || !d->slot_vertex || !d->slot_pend) return fail(BEM_E_ARG, "NULL slot table");
+  if (!d->slot_count || !d->slot_vertex || !d->slot_pend || !d->slot_prev || !d->elem_pos || !d->chunk_start ||
+      !d->vertex_last)
+    return fail(BEM_E_ARG, "NULL slot table");
+  for (int t = 0; t < d->n_faces; ++t) {
+    if (d->elem_pos[t] < 0 || d->elem_pos[t] >= d->n_faces || d->order[d->elem_pos[t]] != t)
+      return fail(BEM_E_ARG, "elem_pos does not invert order");
+    if (d->chunk_start[t] < 0 || d->chunk_start[t] > d->elem_pos[t] / 64) return fail(BEM_E_ARG, "chunk_start out of range");
+  }
   for (int ch = 0; ch < d->n_chunks; ++ch) {
     if (d->slot_count[ch] < 0 || d->slot_count[ch] > BEM_SLOT_INC) return fail(BEM_E_ARG, "slot count out of range");
     for (int k = 0; k < d->slot_count[ch]; ++k) {
@@ -291,6 +301,8 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
       const int pd = d->slot_pend[(size_t)ch * BEM_SLOT_INC + k] & 0xFFFF;
       if (v < 0 || v >= d->n_vertices) return fail(BEM_E_ARG, "slot vertex out of range");
       if (pd != BEM_PEND_NONE && pd >= BEM_PEND_SLOTS) return fail(BEM_E_ARG, "LDS partial-sum slot out of range");
+      const int pv = d->slot_prev[(size_t)ch * BEM_SLOT_INC + k];
+      if (pv < -1 || pv >= ch) return fail(BEM_E_ARG, "slot_prev out of range");
     }
   }
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
@@ -398,8 +410,21 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   UP(m->slot_inc, d->slot_inc, (size_t)d->n_chunks * 192);
   UP(m->slot_vertex, d->slot_vertex, (size_t)d->n_chunks * 192);
   UP(m->slot_pend, d->slot_pend, (size_t)d->n_chunks * 192);
+  UP(m->slot_prev, d->slot_prev, (size_t)d->n_chunks * 192);
+  UP(m->elem_pos, d->elem_pos, (size_t)F);
+  UP(m->chunk_start, d->chunk_start, (size_t)F);
+  UP(m->vertex_last, d->vertex_last, (size_t)V);
   UP(m->vstar_ptr, d->vstar_ptr, (size_t)V + 1);
   UP(m->vstar_row, d->vstar_row, vs_nnz);
+  {
+    std::vector<int4> ent(vs_nnz);
+    for (int v = 0; v < V; ++v)
+      for (int q = d->vstar_ptr[v]; q < d->vstar_ptr[v + 1]; ++q) {
+        const int t = d->vstar_row[q] / 3;
+        ent[q] = make_int4(d->vstar_row[q], d->chunk_start[t], d->elem_pos[t] / 64, v);
+      }
+    UP(m->vstar_ent, ent.data(), vs_nnz);
+  }
   UP(m->fvtx, d->face_vertices, (size_t)F * 9);
   UP(m->fnrm, d->normals, (size_t)F * 3);
   UP(m->fcurl, d->curls, (size_t)F * 9);
@@ -455,7 +480,9 @@ static int reserve_impl(bem_mesh* m, int precision, int nf, int strip_nf, int n_
   const size_t cb = precision == BEM_PREC_MIXED ? 8 : 16;
   strip_nf = std::max(1, std::min(strip_nf, nf));
   int rc;
-  if ((rc = dev_alloc(m, m->strip, (size_t)strip_nf * 3 * m->F * (size_t)m->L * cb))) return rc;
+  // FP32 without images assembles pairs once: a row strip and a column strip per frequency
+  const size_t nstrips = (precision == BEM_PREC_MIXED && m->R == 1) ? 2 : 1;
+  if ((rc = dev_alloc(m, m->strip, nstrips * strip_nf * 3 * m->F * (size_t)m->L * cb))) return rc;
   m->strip_nf = strip_nf;
   if ((rc = dev_alloc(m, m->sing_lhs, (size_t)nf * std::max(1, m->P) * 9 * 16))) return rc;
   if ((rc = dev_alloc(m, m->sing_rhs, (size_t)nf * std::max(1, m->P) * 3 * 16))) return rc;
@@ -551,11 +578,16 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.slot_inc = m->slot_inc;
     p.slot_vertex = m->slot_vertex;
     p.slot_pend = m->slot_pend;
+    p.slot_prev = m->slot_prev;
+    p.elem_pos = m->elem_pos;
+    p.chunk_start = m->chunk_start;
     p.sing_ptr = m->sing_ptr;
     p.sing_trial = m->sing_trial;
     p.sing_lhs = static_cast<const cplx<double>*>(m->sing_lhs.p) + (size_t)f0 * m->P * 9;
     p.k = k + f0;
     p.strip = static_cast<cplx<T>*>(m->strip.p);
+    // paired assembly (FP32, no images): the column strip follows the row strips of the group
+    p.strip2 = (sizeof(T) == 4 && m->R == 1) ? p.strip + (size_t)m->strip_nf * fstride : nullptr;
     p.strip_fstride = fstride;
     p.L = m->L;
     p.F = m->F;
@@ -569,6 +601,11 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     }
     GatherParams<T> gp;
     gp.strip = p.strip;
+    gp.strip2 = p.strip2;
+    gp.elem_pos = m->elem_pos;
+    gp.chunk_start = m->chunk_start;
+    gp.vertex_last = m->vertex_last;
+    gp.vstar_ent = m->vstar_ent;
     gp.strip_fstride = fstride;
     gp.L = m->L;
     gp.vstar_ptr = m->vstar_ptr;

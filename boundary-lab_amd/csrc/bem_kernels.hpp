@@ -5,7 +5,7 @@
 
 #define BEM_SLOT_STRIDE 196  // per-chunk slot_start entries (<= 192 slots + 1, padded)
 #define BEM_SLOT_INC 192     // per-chunk incidence entries (64 elements x 3 columns)
-#define BEM_PEND_SLOTS 192   // LDS partial strip sums per regular-assembly wave
+#define BEM_PEND_SLOTS 96    // LDS partial strip sums per regular-assembly wave
 #define BEM_PEND_NONE 0xFFFF
 #define BEM_PEND_FROM_LDS (1 << 16)
 #define BEM_PEND_TO_STRIP (1 << 17)
@@ -39,6 +39,10 @@ struct RegParams {
   const uint16_t* slot_inc;     // [n_chunks][BEM_SLOT_INC]  (lane << 2) | m
   const int* slot_vertex;       // [n_chunks][BEM_SLOT_INC]  2 * vertex + (earlier chunk stored the column)
   const int* slot_pend;         // [n_chunks][BEM_SLOT_INC]  LDS slot | BEM_PEND_FROM_LDS | BEM_PEND_TO_STRIP
+  const int* slot_prev;         // [n_chunks][BEM_SLOT_INC]  previous chunk holding the vertex (-1: none)
+  const int* elem_pos;          // [F] position in the trial order
+  const int* chunk_start;       // [F] first chunk with the element or an element sharing a vertex
+  cplx<T>* strip2;              // paired assembly: column strip [nf][3F][L] (transposed (s, t) blocks)
   const int* sing_ptr;          // [F+1]
   const int* sing_trial;        // [P]
   const cplx<double>* sing_lhs; // [nf][P][9]
@@ -105,6 +109,13 @@ struct GatherParams {
   double* rowmax;                 // [nf][N]
   int N, F, n_ch;
   const double* roww;             // [N] P1 orbit row weights (null: 1)
+  // paired assembly (strip2 non-null): the column strip and the validity of strip columns
+  const cplx<T>* strip2;          // [nf][3F][L]
+  const int* elem_pos;            // [F]
+  const int* chunk_start;         // [F]
+  const int* vertex_last;         // [N]
+  const int4* vstar_ent;          // per star entry: {strip row, first chunk of its row strip, of its column
+                                  // strip, the star's vertex}
 };
 
 template <typename T>

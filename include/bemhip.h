@@ -97,6 +97,10 @@ typedef struct bem_mesh_desc {
   const int* slot_vertex;    /* [n_chunks][192]: 2 * vertex + 1 if an earlier chunk stored the vertex column */
   const int* slot_pend;      /* [n_chunks][192]: LDS partial-sum slot (0xFFFF: none) | 1 << 16 (an earlier chunk
                                 left partial sums there) | 1 << 17 (store the sums to the strip) */
+  const int* slot_prev;      /* [n_chunks][192]: previous chunk holding the slot's vertex, -1 if none */
+  const int* elem_pos;       /* [F] position of each element in the trial order */
+  const int* chunk_start;    /* [F] first chunk holding the element or an element sharing a vertex with it */
+  const int* vertex_last;    /* [V] last chunk holding each vertex, -1 if unused */
   const int* vstar_ptr;      /* [V+1] vertex -> strip rows (3t+l) CSR */
   const int* vstar_row;
   /* singular (vertex-sharing) pairs, grouped by test element */

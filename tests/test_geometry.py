@@ -273,3 +273,82 @@ def test_symmetry_snap_and_validation():
     assert G.symmetry_reduction_factor("xy") == 4 and G.symmetry_reduction_factor("off") == 1
     with pytest.raises(ValueError):
         G.normalized_symmetry_mode("z")
+
+
+@pytest.mark.parametrize("pend_slots", [G.PEND_SLOTS, 2])
+def test_paired_strips_reproduce_dense_assembly(pend_slots):
+    # Emulate the paired regular assembly: the wave of test element t visits chunks from chunk_start[t]; a trial
+    # element s after t that shares no vertex with t gives the (t, s) block to t's row strip and the transposed
+    # (s, t) block to t's column strip; an element sharing a vertex gives its own block to the row strip only;
+    # earlier non-adjacent elements give nothing. Partial sums follow the LDS / read-modify-write rules with the
+    # previous-chunk test; the gather masks columns by the vertex's last chunk. Must equal the dense scatter.
+    V, F, T = syn.icosphere(2, 0.1)  # 320 faces: 5 chunks
+    m = G.prepare(G.BoundaryMesh(V, F, T)).mesh
+    s = G.slot_tables(m, pend_slots=pend_slots)
+    F, V = m.n_faces, m.n_vertices
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((F, F, 3, 3))  # block (t, s)[l][m] for every ordered pair
+    dense = np.zeros((V, V))
+    for t in range(F):
+        for l in range(3):
+            np.add.at(dense[m.faces[t, l]], m.faces[:, :], B[t, :, l, :])
+    pos = s.elem_pos
+    fv = [set(f) for f in m.faces.tolist()]
+    RS = np.full((3 * F, V), np.nan)
+    CS = np.full((3 * F, V), np.nan)
+    nch = len(s.slot_count)
+    for t in range(F):
+        c0 = int(s.chunk_start[t])
+        assert c0 <= pos[t] // 64
+        lds = {}
+        for c in range(c0, nch):
+            blk = np.zeros((64, 3, 3))
+            blkp = np.zeros((64, 3, 3))
+            for lane in range(64):
+                sidx = s.order[c * 64 + lane]
+                if sidx < 0:
+                    continue
+                if fv[t] & fv[sidx]:
+                    blk[lane] = B[t, sidx]
+                    assert pos[sidx] // 64 >= c0
+                elif pos[sidx] > pos[t]:
+                    blk[lane] = B[t, sidx]
+                    blkp[lane] = B[sidx, t].T
+            for k in range(s.slot_count[c]):
+                vk, again = int(s.slot_vertex[c, k]) >> 1, bool(int(s.slot_vertex[c, k]) & 1)
+                pc, prev = int(s.slot_pend[c, k]), int(s.slot_prev[c, k])
+                pidx = pc & 0xFFFF
+                vals = []
+                for bb in (blk, blkp):
+                    a = np.zeros(3)
+                    for e in range(s.slot_start[c, k], s.slot_start[c, k + 1]):
+                        code = int(s.slot_inc[c, e])
+                        a += bb[code >> 2, :, code & 3]
+                    vals.append(a)
+                for w, (strip, a) in enumerate(zip((RS, CS), vals)):
+                    # the column strip only from t's own chunk on (no later element sits in an earlier chunk)
+                    cw = c0 if w == 0 else int(pos[t] // 64)
+                    if c < cw:
+                        continue
+                    from_lds = bool(pc & G.PEND_FROM_LDS) and prev >= cw
+                    again_w = bool(int(s.slot_vertex[c, k]) & 1) and prev >= cw
+                    rows = 3 * t + np.arange(3)
+                    if pidx != G.PEND_NONE:
+                        if from_lds:
+                            a = a + lds[(pidx, w)]
+                        if pc & G.PEND_TO_STRIP:
+                            strip[rows, vk] = a
+                        else:
+                            lds[(pidx, w)] = a
+                    else:
+                        strip[rows, vk] = strip[rows, vk] + a if again_w else a
+    A = np.zeros((V, V))
+    cs_row = np.repeat(s.chunk_start, 3)
+    cs_col = np.repeat(s.elem_pos // 64, 3)
+    for i in range(V):
+        for r in s.vstar_row[s.vstar_ptr[i] : s.vstar_ptr[i + 1]]:
+            ok = s.vertex_last >= cs_row[r]
+            A[i, ok] += RS[r, ok]
+            ok = s.vertex_last >= cs_col[r]
+            A[ok, i] += CS[r, ok]
+    assert np.allclose(A, dense, rtol=1e-12, atol=1e-12)
