@@ -16,6 +16,8 @@ BEM_OK = 0
 BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_MAX_CHANNELS = 4
+BEM_LU_FP32 = 0
+BEM_LU_SPLIT_BF16 = 1
 TIMER_REGULAR, TIMER_SINGULAR, TIMER_GATHER, TIMER_RHS, TIMER_LU, TIMER_FIELD, TIMER_SOLVE = range(7)
 TIMER_NAMES = ("regular", "singular", "gather", "rhs", "lu", "field", "solve")
 
@@ -80,6 +82,7 @@ SIGNATURES = {
     "bem_ctx_sync": (_i, [_p]),
     "bem_ctx_stream": (_i, [_p, C.POINTER(_p)]),
     "bem_ctx_set_fanout": (_i, [_p, _i]),
+    "bem_ctx_set_lu_products": (_i, [_p, _i]),
     "bem_ctx_timers": (_i, [_p, _i]),
     "bem_ctx_timer_read": (_i, [_p, _i, C.POINTER(_d), C.POINTER(_ll), C.POINTER(_ll)]),
     "bem_mesh_create": (_i, [_p, C.POINTER(MeshDesc), C.POINTER(_p)]),

@@ -243,6 +243,9 @@ class Batch:
     def _factor_solve(self):
         lib, dev = self.dev.lib, self.dev
         sA, sB, sP = self.N * self.lda, self.n_ch * self.ldb, self.N
+        # mixed: the large LU updates as split-bf16 products; the factors only precondition the FP64 refinement
+        _lib.check(lib.bem_ctx_set_lu_products(dev.ctx, _lib.BEM_LU_SPLIT_BF16 if self.precision == "mixed"
+                                               else _lib.BEM_LU_FP32), "bem_ctx_set_lu_products")
         _lib.check(lib.bem_lu_factor(dev.ctx, self.prec, self.N, self.nf, _ptr(self.A), self.lda, sA,
                                      _ptr(self.ipiv), _ptr(self.perm), sP, _ptr(self.info), _ptr(self.luw)),
                    "bem_lu_factor")

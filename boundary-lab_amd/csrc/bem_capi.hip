@@ -47,6 +47,7 @@ struct bem_ctx {
   hipEvent_t fork_ev, join_ev[BEM_AUX_STREAMS];
   int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
   hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
+  int lu_products;      // BEM_LU_FP32 or BEM_LU_SPLIT_BF16 for the large updates of FP32 factorizations
 };
 
 // Run body(first, count, stream) on up to 1 + BEM_AUX_STREAMS contiguous sub-batches, forked from and joined
@@ -182,6 +183,7 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   c->graph = nullptr;
   c->timing = false;
   c->fanout = 1 + BEM_AUX_STREAMS;
+  c->lu_products = BEM_LU_FP32;
   for (int i = 0; i < BEM_TIMER_SLOTS; ++i) {
     c->timer_ms[i] = 0.0;
     c->timer_count[i] = 0;
@@ -245,6 +247,13 @@ int bem_ctx_set_fanout(bem_ctx* c, int streams) {
   if (streams < 1 || streams > 1 + BEM_AUX_STREAMS) return fail(BEM_E_ARG, "fan-out must be 1..4 streams");
   if (c->capturing) return fail(BEM_E_STATE, "cannot change the fan-out during graph capture");
   c->fanout = streams;
+  return BEM_OK;
+}
+
+int bem_ctx_set_lu_products(bem_ctx* c, int mode) {
+  if (!c) return fail(BEM_E_ARG, "ctx is NULL");
+  if (mode != BEM_LU_FP32 && mode != BEM_LU_SPLIT_BF16) return fail(BEM_E_ARG, "unknown LU product mode");
+  c->lu_products = mode;
   return BEM_OK;
 }
 
@@ -673,11 +682,11 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
     e = lu_factor_batch<float>(mb, batch, ipiv, perm, strideP, info, nullptr, static_cast<cplx<float>*>(work),
-                               c->stream, &la);
+                               c->stream, &la, c->lu_products == BEM_LU_SPLIT_BF16);
   } else {
     MatBatch<double> mb = {static_cast<cplx<double>*>(A), lda, strideA, n};
     e = lu_factor_batch<double>(mb, batch, ipiv, perm, strideP, info, nullptr, static_cast<cplx<double>*>(work),
-                                c->stream, &la);
+                                c->stream, &la, false);
   }
   if (e != hipSuccess) return hipfail(e, "lu_factor launch");
   return BEM_OK;

@@ -143,7 +143,7 @@ struct LuLookahead {
 };
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, cplx<T>* W, hipStream_t, const LuLookahead* la);
+                           const int* skip, cplx<T>* W, hipStream_t, const LuLookahead* la, bool split_bf16);
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>&, int batch, const int* perm, long long strideP, const cplx<double>* b,
                           long long ldb, long long strideB, cplx<T>* y, long long ldy, long long strideY, int nrhs,

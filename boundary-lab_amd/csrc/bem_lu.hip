@@ -286,6 +286,149 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
 }
 
+// Trailing update C -= A B with every FP32 product taken as three bf16 products on v_mfma_f32_32x32x16_bf16:
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (round to nearest even, v_cvt_pk_bf16_f32), and
+// a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32. The dropped a_lo b_lo term and the rounding of
+// the lo parts leave a relative error near 2^-16 per product: more than FP32's, but the factors only precondition
+// the FP64 refinement that follows, which converges in the same number of steps (the LU of a Burton-Miller
+// matrix has a condition number of tens to hundreds). A workgroup owns 128 rows x 64 complex columns of C, each
+// of its four waves 64 rows x 32 complex columns as two 32-row blocks with separate real and imaginary
+// accumulators. Complex arithmetic as a real product with K doubled: the A operand is A itself ((re, im)
+// interleaved along k), the B operand for the real part is (br, -bi) per k and for the imaginary part (bi, br),
+// both formed from the staged (br, bi) pairs by a sign flip or a 16-bit rotation of each dword. A is staged
+// negated, split into hi and lo rows in LDS; B transposed (one row per column of C) and split the same way.
+// Operand layout of the 32x32x16 MFMA: lane l holds row (or column) l%32 and k = 8(l/32) .. 8(l/32)+7.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, unsigned& lo) {
+  const f32x2 v = {re, im};
+  const bf16x2 h = __builtin_convertvector(v, bf16x2);
+  const f32x2 hf = __builtin_convertvector(h, f32x2);
+  const bf16x2 l = __builtin_convertvector(v - hf, bf16x2);
+  hi = __builtin_bit_cast(unsigned, h);
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
+    MatBatch<float> mb, GemmShape g, const int* skip) {
+  constexpr int BM = 128, BN = 64, KC = 16;  // rows, complex columns, complex k per stage
+  constexpr int RW = KC + 4;                 // LDS row: KC (re, im) bf16 pairs + 16 B of padding, in dwords
+  const int mat = blockIdx.z;
+  if (skip && skip[mat]) return;
+  cplx<float>* A = mb.A + (size_t)mat * mb.strideA;
+  const long long lda = mb.lda;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wy = w >> 1, wx = w & 1;
+  const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
+  const int half = lane >> 5, l32 = lane & 31;
+  __shared__ __attribute__((aligned(16))) unsigned Ah[BM][RW], Al[BM][RW], Bh[BN][RW], Bl[BN][RW];
+
+  // accumulators start as the C tile: block ti, register v holds row 32 ti + (v/4)*8 + half*4 + v%4 of the
+  // wave's 64 rows and complex column l32 of its 32
+  const int rbase = bm + 64 * wy + half * 4;
+  const int col = bn + 32 * wx + l32;
+  f32x16 acr[2], aci[2];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int r = rbase + 32 * ti + (v >> 2) * 8 + (v & 3);
+      cplx<float> c = mk<float>(0.f, 0.f);
+      if (r < g.M && col < g.Nc) c = A[(size_t)(g.c_r0 + r) * lda + g.c_c0 + col];
+      acr[ti][v] = c.re;
+      aci[ti][v] = c.im;
+    }
+
+  // staging per stage: A = 128 x 16 complex (8 per thread), B = 16 x 64 complex (4 per thread)
+  cplx<float> ra[8], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q, r = e >> 4, kk = e & 15;
+      ra[q] = (bm + r < g.M && k0 + kk < g.K) ? A[(size_t)(g.a_r0 + bm + r) * lda + g.a_c0 + k0 + kk]
+                                              : mk<float>(0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q, kk = e >> 6, j = e & 63;
+      rb[q] = (k0 + kk < g.K && bn + j < g.Nc) ? A[(size_t)(g.b_r0 + k0 + kk) * lda + g.b_c0 + bn + j]
+                                               : mk<float>(0.f, 0.f);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q;
+      unsigned hi, lo;
+      split_bf16(-ra[q].re, -ra[q].im, hi, lo);
+      Ah[e >> 4][e & 15] = hi;
+      Al[e >> 4][e & 15] = lo;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      unsigned hi, lo;
+      split_bf16(rb[q].re, rb[q].im, hi, lo);
+      Bh[e & 63][e >> 6] = hi;
+      Bl[e & 63][e >> 6] = lo;
+    }
+  };
+  auto multiply = [&]() {
+#pragma unroll
+    for (int kg = 0; kg < KC / 8; ++kg) {
+      const int d0 = 8 * kg + 4 * half;  // first complex k of this lane's eight bf16
+      u32x4 ah[2], al[2];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+        const int r = 64 * wy + 32 * ti + l32;
+        ah[ti] = *reinterpret_cast<const u32x4*>(&Ah[r][d0]);
+        al[ti] = *reinterpret_cast<const u32x4*>(&Al[r][d0]);
+      }
+      const int cl = 32 * wx + l32;
+      const u32x4 bh = *reinterpret_cast<const u32x4*>(&Bh[cl][d0]);
+      const u32x4 bl = *reinterpret_cast<const u32x4*>(&Bl[cl][d0]);
+      u32x4 brh, brl, bih, bil;  // (br, -bi) and (bi, br) per complex k
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        brh[i] = bh[i] ^ 0x80000000u;
+        brl[i] = bl[i] ^ 0x80000000u;
+        bih[i] = (bh[i] >> 16) | (bh[i] << 16);
+        bil[i] = (bl[i] >> 16) | (bl[i] << 16);
+      }
+      const bf16x8 Brh = __builtin_bit_cast(bf16x8, brh), Brl = __builtin_bit_cast(bf16x8, brl);
+      const bf16x8 Bih = __builtin_bit_cast(bf16x8, bih), Bil = __builtin_bit_cast(bf16x8, bil);
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+        const bf16x8 Ahh = __builtin_bit_cast(bf16x8, ah[ti]), All = __builtin_bit_cast(bf16x8, al[ti]);
+        acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brh, acr[ti], 0, 0, 0);
+        aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bih, aci[ti], 0, 0, 0);
+        acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brl, acr[ti], 0, 0, 0);
+        aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bil, aci[ti], 0, 0, 0);
+        acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Brh, acr[ti], 0, 0, 0);
+        aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Bih, aci[ti], 0, 0, 0);
+      }
+    }
+  };
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+    if (k0 == 0) fetch(0);
+    stash();
+    __syncthreads();
+    if (k0 + KC < g.K) fetch(k0 + KC);  // next stage in flight during the multiply
+    multiply();
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int r = rbase + 32 * ti + (v >> 2) * 8 + (v & 3);
+      if (r < g.M && col < g.Nc) A[(size_t)(g.c_r0 + r) * lda + g.c_c0 + col] = mk<float>(acr[ti][v], aci[ti][v]);
+    }
+}
+
 // Skinny variant for the sub-panel updates (Nc <= 16 columns, K <= 64): the K x Nc block of B sits in LDS,
 // four threads share a row of C (each a quarter of K, contiguous A loads), partial sums meet by DPP quad
 // swaps, and each thread writes a quarter of the row. Plain FMA: the update is bound by reading A once.
@@ -1274,15 +1417,21 @@ int panel_width(int N, int c0) {
   return mr <= 2 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
 }
 
+// split: FP32 products as three bf16 products (k_gemm_bf16x3) for the large updates
 template <typename T>
 static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape& g, const int* skip,
-                              hipStream_t st) {
+                              hipStream_t st, bool split = false) {
   if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
   if (g.Nc <= 16 && g.K <= BEM_NBO) {
     hipLaunchKernelGGL(k_gemm_skinny<T>, dim3((g.M + 63) / 64, batch), dim3(256), 0, st, mb, g, skip);
     return hipGetLastError();
   }
   if constexpr (sizeof(T) == 4) {
+    if (split) {
+      hipLaunchKernelGGL(k_gemm_bf16x3, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st, mb, g,
+                         skip);
+      return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_gemm_tile32<false>, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st,
                        mb, g, skip, nullptr, 0LL, 0LL);
     return hipGetLastError();
@@ -1330,8 +1479,9 @@ static hipError_t solve_upper_rows(const MatBatch<T>& mb, int batch, int r0, int
 
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, cplx<T>* W, hipStream_t st, const LuLookahead* la) {
+                           const int* skip, cplx<T>* W, hipStream_t st, const LuLookahead* la, bool split_bf16) {
   const int N = mb.N;
+  const bool split = sizeof(T) == 4 && split_bf16;  // three bf16 products per FP32 product in the large updates
   const long long strideW = lu_workspace_elems(N);
   const bool look = la && la->stream;
   hipLaunchKernelGGL(k_iota, dim3((N + 255) / 256, batch), dim3(256), 0, st, perm, strideP, N, info);
@@ -1345,7 +1495,7 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
         // second half: rows [k0, h0) become U with the first half's L, the rows below take the K = 64 update
         CHK(solve_upper_rows(mb, batch, k0, h0, h1 - h0, W, strideW, skip, st));
         GemmShape g = {h0, k0, k0, h0, h0, h0, N - h0, h1 - h0, h0 - k0};
-        CHK(launch_gemm<T>(mb, batch, g, skip, st));
+        CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
       }
       int c0 = h0;
       while (c0 < h1) {
@@ -1391,7 +1541,7 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
     if (k1 - k0 > NBO) {
       const int h0 = k0 + NBO;
       GemmShape g = {h0, k0, k0, k1, h0, k1, k1 - h0, N - k1, NBO};
-      CHK(launch_gemm<T>(mb, batch, g, skip, st));
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
       CHK(solve_upper_rows(mb, batch, h0, k1, N - k1, W, strideW, skip, st));
     }
     // trailing update with K = 128: columns [k1, k2) on the main stream, [k2, N) on the look-ahead stream
@@ -1400,14 +1550,14 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
       CHK(hipEventRecord(la->ready, st));
       CHK(hipStreamWaitEvent(la->stream, la->ready, 0));
       GemmShape gr = {k1, k0, k0, k2, k1, k2, N - k1, N - k2, k1 - k0};
-      CHK(launch_gemm<T>(mb, batch, gr, skip, la->stream));
+      CHK(launch_gemm<T>(mb, batch, gr, skip, la->stream, split));
       CHK(hipEventRecord(la->done, la->stream));
       pending = true;
       GemmShape gl = {k1, k0, k0, k1, k1, k1, N - k1, k2 - k1, k1 - k0};
-      CHK(launch_gemm<T>(mb, batch, gl, skip, st));
+      CHK(launch_gemm<T>(mb, batch, gl, skip, st, split));
     } else {
       GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, N - k1, k1 - k0};
-      CHK(launch_gemm<T>(mb, batch, g, skip, st));
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
     }
     CHK(factor_block(k1, k2));
   }
@@ -1472,9 +1622,9 @@ hipError_t growth_batch(const MatBatch<T>& mb, int batch, double* umax, const do
 }
 
 template hipError_t lu_factor_batch<float>(const MatBatch<float>&, int, int*, int*, long long, int*, const int*,
-                                           cplx<float>*, hipStream_t, const LuLookahead*);
+                                           cplx<float>*, hipStream_t, const LuLookahead*, bool);
 template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, int*, long long, int*, const int*,
-                                            cplx<double>*, hipStream_t, const LuLookahead*);
+                                            cplx<double>*, hipStream_t, const LuLookahead*, bool);
 template hipError_t lu_solve_batch<float>(const MatBatch<float>&, int, const int*, long long, const cplx<double>*,
                                           long long, long long, cplx<float>*, long long, long long, int, const int*,
                                           const cplx<float>*, hipStream_t);

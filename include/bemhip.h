@@ -63,6 +63,15 @@ int bem_ctx_stream(bem_ctx* ctx, void** stream);
  * (same results bit for bit). Default 4; 1 runs everything on the context stream. */
 int bem_ctx_set_fanout(bem_ctx* ctx, int streams);
 
+/* Products of the large LU updates of BEM_PREC_MIXED factorizations on this context (no reference counterpart:
+ * the reference factors in its working precision). BEM_LU_FP32 (default): FP32 MFMA. BEM_LU_SPLIT_BF16: each
+ * FP32 product as three bf16 MFMA products (x = hi + lo, a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32
+ * accumulation), relative error near 2^-16 per product, for factors that precondition the FP64 refinement of
+ * bem_refine. The sub-panel updates, panels and triangular solves stay FP32. Ignored for BEM_PREC_FP64. */
+#define BEM_LU_FP32 0
+#define BEM_LU_SPLIT_BF16 1
+int bem_ctx_set_lu_products(bem_ctx* ctx, int mode);
+
 /* Kernel timers (no reference counterpart; diagnostics for roofline measurement). While enabled, each timed
  * launch group is bracketed by HIP events on the context's stream and waited for (not during capture).
  * bem_ctx_timer_read returns accumulated milliseconds, accumulated work units (frequencies or matrices) and the

@@ -128,6 +128,7 @@ def test_lu_random(gpu_device, precision, n, batch):
         work = torch.empty(_lib.lu_workspace_bytes(prec, n, batch), dtype=torch.uint8, device=dev)
     gpu_device.sync()
     lib = gpu_device.lib
+    _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, _lib.BEM_LU_FP32), "products")
     _lib.check(lib.bem_lu_factor(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(ipiv), _ptr(perm), n,
                                  _ptr(info), _ptr(work)), "factor")
     _lib.check(lib.bem_lu_solve(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(perm), n, _ptr(work),
@@ -144,6 +145,72 @@ def test_lu_random(gpu_device, precision, n, batch):
         print(precision, n, m, "rel err", err, "backward", bwd)
         assert err < tol_fwd and bwd < tol_bwd
     assert int(info.cpu().numpy().max()) == 0
+
+
+# sizes: partial 128-row and 64-column tiles of the split update (700), the mr = 2 and mr = 3 panels (2100)
+@pytest.mark.parametrize("n,batch", [(700, 3), (2100, 2)])
+def test_lu_split_bf16_products_precondition_refinement(gpu_device, n, batch):
+    """Split-bf16 trailing updates (three bf16 MFMA products per FP32 product): the factors of a matrix with a
+    condition number in the tens have a backward error near 2^-16 sqrt(N), and FP64 refinement with them reaches
+    a relative residual of 1e-10 in as many steps as with FP32 factors (at most three here)."""
+    from beat_hip import _lib
+    from beat_hip.engine import PRECISIONS, _ptr
+
+    torch = gpu_device.torch
+    rng = np.random.default_rng(5 + n)
+    lda = ((n + 15) // 16) * 16
+    M = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))) / np.sqrt(2 * n)
+    M += (1.5 + 0.5j) * np.eye(n)[None]
+    B = rng.standard_normal((batch, 1, n)) + 1j * rng.standard_normal((batch, 1, n))
+    prec = PRECISIONS["mixed"]
+    lib, dev = gpu_device.lib, gpu_device.device
+    results = {}
+    try:
+        for mode in (_lib.BEM_LU_FP32, _lib.BEM_LU_SPLIT_BF16):
+            _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, mode), "products")
+            with torch.cuda.stream(gpu_device.stream):
+                A = torch.zeros((batch, n, lda), dtype=torch.complex64, device=dev)
+                A[:, :, :n] = torch.from_numpy(M).to(dev).to(torch.complex64)
+                ipiv = torch.zeros((batch, n), dtype=torch.int32, device=dev)
+                perm = torch.zeros((batch, n), dtype=torch.int32, device=dev)
+                info = torch.zeros(batch, dtype=torch.int32, device=dev)
+                work = torch.empty(_lib.lu_workspace_bytes(prec, n, batch), dtype=torch.uint8, device=dev)
+                b = torch.zeros((batch, 1, lda), dtype=torch.complex128, device=dev)
+                y = torch.zeros((batch, 1, lda), dtype=torch.complex64, device=dev)
+            gpu_device.sync()
+            _lib.check(lib.bem_lu_factor(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(ipiv),
+                                         _ptr(perm), n, _ptr(info), _ptr(work)), "factor")
+            gpu_device.sync()
+            assert int(info.cpu().numpy().max()) == 0
+
+            def solve(rhs):
+                b[:, :, :n] = torch.from_numpy(rhs).to(dev)
+                gpu_device.sync()
+                _lib.check(lib.bem_lu_solve(gpu_device.ctx, prec, n, batch, _ptr(A), lda, n * lda, _ptr(perm), n,
+                                            _ptr(work), _ptr(b), lda, lda, _ptr(y), lda, lda, 1, None), "solve")
+                gpu_device.sync()
+                return y[:, :, :n].cpu().numpy().astype(np.complex128)
+
+            X = solve(B)
+            bwd = [np.linalg.norm(B[m, 0] - M[m] @ X[m, 0]) / (np.linalg.norm(M[m], 2) * np.linalg.norm(X[m, 0]))
+                   for m in range(batch)]
+            hist = []
+            for it in range(4):
+                R = B - np.einsum("mij,mj->mi", M, X[:, 0])[:, None]
+                rel = max(np.linalg.norm(R[m]) / np.linalg.norm(B[m]) for m in range(batch))
+                hist.append(rel)
+                if rel < 1e-10:
+                    break
+                X = X + solve(R)
+            results[mode] = (max(bwd), hist, A.cpu().numpy())
+            print("products", mode, "n", n, "backward", max(bwd), "residuals", hist)
+    finally:
+        _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, _lib.BEM_LU_FP32), "products")
+    bwd32, hist32, f32 = results[_lib.BEM_LU_FP32]
+    bwd16, hist16, f16 = results[_lib.BEM_LU_SPLIT_BF16]
+    assert not np.array_equal(f32, f16)  # the split products are in use
+    assert bwd32 < 1e-5 and bwd16 < 1e-4
+    assert hist16[-1] < 1e-10 and len(hist16) <= 4 and len(hist16) <= len(hist32) + 1
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
