@@ -50,7 +50,7 @@ TRAFFIC_PROFILE = ROOT / "profiles" / "k_regular_pair_horn6k_traffic.md"
 
 
 def measured_traffic():
-    """HBM bytes per k_regular_pair<2> launch from the committed rocprofv3 counter summary (FETCH_SIZE +
+    """HBM bytes per k_regular_pair<1> launch from the committed rocprofv3 counter summary (FETCH_SIZE +
     WRITE_SIZE per dispatch, each scaled by the factor a plain copy of known size reads), or None."""
     try:
         vals = {}
@@ -66,7 +66,7 @@ def measured_traffic():
 def traffic_probe():
     """Workload for the two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE): a 1 GiB device-to-device copy
     (calibration: the copy kernel fetches and writes exactly 1 GiB) followed by one eager assembly of 4
-    frequencies of the horn job (one k_regular_pair<2> launch)."""
+    frequencies of the horn job (one k_regular_pair<1> launch)."""
     import torch
 
     from beat_hip.engine import Engine
@@ -266,7 +266,7 @@ def main():
     achieved = qpairs * FLOPS_PER_QPAIR / (avg_ms * 1e-3) / 1e12
     executed = 0.5 * qpairs * FLOPS_PER_UNORDERED_QPAIR / (avg_ms * 1e-3) / 1e12
     traffic = measured_traffic()
-    log(f"bench: k_regular_pair<2> avg {avg_ms:.3f} ms/launch ({freqs_per_launch:g} freqs) -> "
+    log(f"bench: k_regular_pair<1> avg {avg_ms:.3f} ms/launch ({freqs_per_launch:g} freqs) -> "
         f"{achieved:.1f} TFLOP/s of ordered-pair work ({executed:.1f} executed); stages (ms/job): "
         + ", ".join(f"{k} {v[0]:.1f}" for k, v in stages.items()))
 
@@ -327,7 +327,7 @@ def main():
         "max_relative_residual": max_res,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "k_regular_pair<2>", "pipe": PIPE_NOTE, "avg_launch_ms": round(avg_ms, 4),
+                     "kernel": "k_regular_pair<1>", "pipe": PIPE_NOTE, "avg_launch_ms": round(avg_ms, 4),
                      "freqs_per_launch": freqs_per_launch, "flops_per_qpair": FLOPS_PER_QPAIR,
                      "qpairs_per_launch": int(qpairs),
                      "executed_tflops": round(executed, 3),

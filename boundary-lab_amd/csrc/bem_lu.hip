@@ -326,21 +326,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int half = lane >> 5, l32 = lane & 31;
   __shared__ __attribute__((aligned(16))) unsigned Ah[BM][RW], Al[BM][RW], Bh[BN][RW], Bl[BN][RW];
 
-  // accumulators start as the C tile: block ti, register v holds row 32 ti + (v/4)*8 + half*4 + v%4 of the
-  // wave's 64 rows and complex column l32 of its 32
+  // accumulators: block ti, register v holds row 32 ti + (v/4)*8 + half*4 + v%4 of the wave's 64 rows and
+  // complex column l32 of its 32. They start at zero and C is added in the epilogue: a C tile read before the
+  // first product stalls the workgroup on HBM latency (measured 10 % slower on the 2,874-row updates).
   const int rbase = bm + 64 * wy + half * 4;
   const int col = bn + 32 * wx + l32;
   f32x16 acr[2], aci[2];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int r = rbase + 32 * ti + (v >> 2) * 8 + (v & 3);
-      cplx<float> c = mk<float>(0.f, 0.f);
-      if (r < g.M && col < g.Nc) c = A[(size_t)(g.c_r0 + r) * lda + g.c_c0 + col];
-      acr[ti][v] = c.re;
-      aci[ti][v] = c.im;
-    }
+    for (int v = 0; v < 16; ++v) acr[ti][v] = aci[ti][v] = 0.f;
 
   // staging per stage: A = 128 x 16 complex (8 per thread), B = 16 x 64 complex (4 per thread)
   cplx<float> ra[8], rb[4];
@@ -425,7 +420,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int r = rbase + 32 * ti + (v >> 2) * 8 + (v & 3);
-      if (r < g.M && col < g.Nc) A[(size_t)(g.c_r0 + r) * lda + g.c_c0 + col] = mk<float>(acr[ti][v], aci[ti][v]);
+      if (r < g.M && col < g.Nc) {
+        cplx<float>* cp = A + (size_t)(g.c_r0 + r) * lda + g.c_c0 + col;
+        const cplx<float> c = *cp;
+        *cp = mk<float>(c.re + acr[ti][v], c.im + aci[ti][v]);
+      }
     }
 }
 
