@@ -38,9 +38,11 @@ METRIC = "freq-points/s @6k tris, mixed vs FP64 GPU (≤0.05 dB); p50/p99 job la
 # 6 trial points). `achieved` counts this work over the (F^2 - adjacent) ordered element pairs.
 FLOPS_PER_QPAIR = 66
 # What k_regular_pair executes per unordered quadrature-point pair and frequency: both blocks of the pair from one
-# kernel evaluation, geometry/phase/Green shared (23), two normal projections (16), six contracted components
-# (inner 36, outer 18) = 93, i.e. 93 / (2 x 66) = 0.70 of the ordered-pair work.
-FLOPS_PER_UNORDERED_QPAIR = 93
+# kernel evaluation. Difference vector 3, |r|^2 5, r and 1/r^2 2, phase 1, Green's function 2, the double-layer
+# factor 4 pi G (ik - 1/r) / r 6, its product with the trial point's n_t offset 2 (the n_s offset is constant
+# over the test point's trial points and scales the contracted sum, 1), six contracted quantities: inner 19,
+# outer 18.5 = 59, i.e. 59 / (2 x 66) = 0.45 of the ordered-pair work.
+FLOPS_PER_UNORDERED_QPAIR = 59
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 PIPE_NOTE = ("compute-bound on the FP32 vector ALUs (no MFMA, no HBM limit); 157.3 TFLOP/s is MI355X's FP32 peak, "

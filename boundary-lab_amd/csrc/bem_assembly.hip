@@ -45,6 +45,13 @@ struct RuleWeights {
   }
 };
 __constant__ RuleWeights c_phiw = RuleWeights();
+struct RuleWeightsF {
+  float v[18];
+  constexpr RuleWeightsF() : v() {
+    for (int i = 0; i < 18; ++i) v[i] = float(RuleWeights().v[i]);
+  }
+};
+__constant__ RuleWeightsF c_phiw_f = RuleWeightsF();
 
 // 3x3 LHS block of a regular (t,s) pair, scaled and combined: C += J/(4pi) (-D + (i/k) H).
 // UNROLL_A = 6 fully unrolls the test-point loop (FP32: test points live in SGPRs); UNROLL_A = 1 keeps it
@@ -242,6 +249,15 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
   for (int i = 0; i < 18; ++i) y[i] = se.q[i];
   const float nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
   const float ntx = nt[0], nty = nt[1], ntz = nt[2];
+  // Both elements are flat: (y_b - x_a).n_s = (y_0 - x_a).n_s depends on the test point only and
+  // -(y_b - x_a).n_t = (x_0 - y_b).n_t on the trial point only. The double-layer kernels of the two blocks are
+  // u (y_b - x_a).n / r with u = 4 pi G (ik - 1/r) / r; the n_s factor of a test point multiplies its trial
+  // contraction once, the n_t factors of the six trial points are formed once per pair.
+  const float cs_plane = y[0] * nsx + y[1] * nsy + y[2] * nsz;
+  const float ct_plane = xq[0] * ntx + xq[1] * nty + xq[2] * ntz;
+  float pt[6];
+#pragma unroll
+  for (int b = 0; b < 6; ++b) pt[b] = ct_plane - (y[3 * b] * ntx + y[3 * b + 1] * nty + y[3 * b + 2] * ntz);
   float DMr[NFW][9], DMi[NFW][9], GMr[NFW][9], GMi[NFW][9], EMr[NFW][9], EMi[NFW][9];
 #pragma unroll
   for (int j = 0; j < NFW; ++j)
@@ -250,6 +266,7 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
 #pragma unroll 1
   for (int a = 0; a < 6; ++a) {
     const float xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
+    const float ps = cs_plane - (xa * nsx + ya * nsy + za * nsz);  // (y_b - x_a).n_s for every b
     float xDr[NFW][6], xDi[NFW][6], xGr[NFW][6], xGi[NFW][6], xEr[NFW][6], xEi[NFW][6];
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -257,21 +274,21 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
       const float dy = y[3 * b + 1] - ya;
       const float dz = y[3 * b + 2] - za;
       const float r2 = dx * dx + dy * dy + dz * dz;
-      const float rinv = Phase<float>::rsqrt_(r2);
+      // no guard against r = 0: the points of two elements without a shared vertex are distinct
+      const float rinv = __builtin_amdgcn_rsqf(r2);
       const float r = r2 * rinv;
-      const float td = (dx * nsx + dy * nsy + dz * nsz) * rinv;   // (s, t): trial normal n_s
-      const float te = -(dx * ntx + dy * nty + dz * ntz) * rinv;  // (t, s) seen from s: trial normal n_t
+      const float rinv2 = rinv * rinv;
 #pragma unroll
       for (int j = 0; j < NFW; ++j) {
         float sn, cs;
         Phase<float>::sincos_kr(kappa[j], k[j], r, &sn, &cs);
-        const float gr = cs * rinv, gi = sn * rinv;  // 4 pi G
-        const float hr = -gr * rinv - gi * k[j];     // 4 pi G (ik - 1/r)
-        const float hi = gr * k[j] - gi * rinv;
-        xDr[j][b] = hr * td;
-        xDi[j][b] = hi * td;
-        xEr[j][b] = hr * te;
-        xEi[j][b] = hi * te;
+        const float gr = cs * rinv, gi = sn * rinv;       // 4 pi G
+        const float ur = -rinv2 * fmaf(sn, k[j], gr);     // 4 pi G (ik - 1/r) / r
+        const float ui = rinv2 * fmaf(cs, k[j], -gi);
+        xDr[j][b] = ur;
+        xDi[j][b] = ui;
+        xEr[j][b] = ur * pt[b];
+        xEi[j][b] = ui * pt[b];
         xGr[j][b] = gr;
         xGi[j][b] = gi;
       }
@@ -287,11 +304,12 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
       trial_contract(xEi[j], tEi);
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        const float wa = float(c_phiw.v[l * 6 + a]);
+        const float wa = c_phiw_f.v[l * 6 + a];
+        const float wd = wa * ps;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          DMr[j][3 * l + m] += wa * tDr[m];
-          DMi[j][3 * l + m] += wa * tDi[m];
+          DMr[j][3 * l + m] += wd * tDr[m];
+          DMi[j][3 * l + m] += wd * tDi[m];
           GMr[j][3 * l + m] += wa * tGr[m];
           GMi[j][3 * l + m] += wa * tGi[m];
           EMr[j][3 * l + m] += wa * tEr[m];
