@@ -10,11 +10,12 @@
 // chosen with |re|+|im| (i?amax), ties to the lowest row; ipiv[c] is the row swapped with row c. A running
 // permutation perm (perm[i] = original row now at position i) is kept for O(N) RHS permutation.
 //
-// Three levels. Right-looking over trailing blocks of NBT = 128 columns (trailing update C -= L21 U12 with
-// K = 128 on MFMA). Each trailing block is two halves of NBO = 64: the second half is brought up to date with
-// the first by an MFMA product (K = 64) before it is factored. Left-looking inside a half: sub-panels of NBI
-// columns are factored by one 1024-thread workgroup that holds the sub-panel in registers (k_lu_panel), after
-// a top TRSM and an update with the sub-panels of the half already done. Row interchanges are applied inside
+// Three levels. Right-looking over blocks of NBT = 128 columns whose trailing updates C -= L21 U12 are applied
+// to the matrix beyond a pair of blocks at once (K = 256 on MFMA). Each block is two halves of NBO = 64: the
+// second half is brought up to date with the first by an MFMA product (K = 64) before it is factored.
+// Left-looking inside a half: sub-panels of NBI columns are factored by one 1024-thread workgroup that holds
+// the sub-panel in registers (k_lu_panel), after a top TRSM and an update with the sub-panels of the half
+// already done. Row interchanges are applied inside
 // the trailing block by the panel kernel and outside it by k_laswp once the block is factored. The 64 x 64
 // diagonal blocks of L and U are inverted into a workspace; the U12 solves and the diagonal steps of the
 // triangular sweeps multiply by those inverses.
@@ -1514,51 +1515,82 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
     return hipSuccess;
   };
 
-  // Look-ahead: once block j is factored and its U12 formed, the main stream updates only the next block's
-  // columns and factors that block (a chain of latency-bound launches), while the look-ahead stream applies
-  // block j to the columns beyond it (the large MFMA update). The next interchanges wait for that update.
+  // interchanges of a block's pivots on the columns outside it, one half at a time (in pivot order)
+  auto laswp_block = [&](int b0, int b1) -> hipError_t {
+    if (N > b1 - b0) {
+      constexpr int CB = sizeof(T) == 4 ? 64 : 32;
+      for (int h0 = b0; h0 < b1; h0 += NBO) {
+        hipLaunchKernelGGL(k_laswp<T>, dim3((N - (b1 - b0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv,
+                           strideP, h0, min(b1, h0 + NBO), b0, b1, skip);
+        CHK(hipGetLastError());
+      }
+    }
+    return hipSuccess;
+  };
+  // U12 = L11^-1 A12 for the rows of block [b0, b1) and the columns [c, N), by halves
+  auto solve_u12 = [&](int b0, int b1, int c) -> hipError_t {
+    CHK(solve_upper_rows(mb, batch, b0, c, N - c, W, strideW, skip, st));
+    if (b1 - b0 > NBO) {
+      const int h0 = b0 + NBO;
+      GemmShape g = {h0, b0, b0, c, h0, c, b1 - h0, N - c, NBO};
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+      CHK(solve_upper_rows(mb, batch, h0, c, N - c, W, strideW, skip, st));
+    }
+    return hipSuccess;
+  };
+
+  // Blocks go in pairs P = [k0, k1), Q = [k1, k2) whose trailing updates are applied together (K = 256), so the
+  // trailing matrix is read and written once per pair instead of once per block. P updates only Q's columns
+  // before Q is factored; P's update of Q's rows beyond Q waits until Q's interchanges have been applied to
+  // them (every row of the columns beyond Q is then still without P's update, so the interchanges commute with
+  // the deferred update, which uses P's equally interchanged L rows). Look-ahead: the pair's update of the next
+  // block's columns runs on the main stream, which then factors that block (a chain of latency-bound
+  // launches), while the look-ahead stream updates the columns beyond it; the next interchanges wait for it.
   bool pending = false;  // a look-ahead update is in flight
   CHK(factor_block(0, min(N, NBT)));
-  for (int k0 = 0; k0 < N; k0 += NBT) {
-    const int k1 = min(N, k0 + NBT);
+  int k0 = 0;
+  while (true) {
+    const int k1 = min(N, k0 + NBT);  // block P, factored
     if (pending) {
       CHK(hipStreamWaitEvent(st, la->done, 0));
       pending = false;
     }
-    // interchanges of the block's pivots on the columns outside it, one half at a time (in pivot order)
-    if (N > k1 - k0) {
-      constexpr int CB = sizeof(T) == 4 ? 64 : 32;
-      for (int h0 = k0; h0 < k1; h0 += NBO) {
-        hipLaunchKernelGGL(k_laswp<T>, dim3((N - (k1 - k0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv,
-                           strideP, h0, min(k1, h0 + NBO), k0, k1, skip);
-        CHK(hipGetLastError());
-      }
-    }
+    CHK(laswp_block(k0, k1));
     if (k1 >= N) break;
-    // U12 = L11^-1 A12 by halves
-    CHK(solve_upper_rows(mb, batch, k0, k1, N - k1, W, strideW, skip, st));
-    if (k1 - k0 > NBO) {
-      const int h0 = k0 + NBO;
-      GemmShape g = {h0, k0, k0, k1, h0, k1, k1 - h0, N - k1, NBO};
-      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
-      CHK(solve_upper_rows(mb, batch, h0, k1, N - k1, W, strideW, skip, st));
-    }
-    // trailing update with K = 128: columns [k1, k2) on the main stream, [k2, N) on the look-ahead stream
-    const int k2 = min(N, k1 + NBT);
-    if (look && k2 < N) {
-      CHK(hipEventRecord(la->ready, st));
-      CHK(hipStreamWaitEvent(la->stream, la->ready, 0));
-      GemmShape gr = {k1, k0, k0, k2, k1, k2, N - k1, N - k2, k1 - k0};
-      CHK(launch_gemm<T>(mb, batch, gr, skip, la->stream, split));
-      CHK(hipEventRecord(la->done, la->stream));
-      pending = true;
-      GemmShape gl = {k1, k0, k0, k1, k1, k1, N - k1, k2 - k1, k1 - k0};
-      CHK(launch_gemm<T>(mb, batch, gl, skip, st, split));
-    } else {
-      GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, N - k1, k1 - k0};
+    CHK(solve_u12(k0, k1, k1));
+    const int k2 = min(N, k1 + NBT);  // block Q
+    {
+      GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, k2 - k1, k1 - k0};  // P's update of Q's columns
       CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
     }
     CHK(factor_block(k1, k2));
+    if (k2 >= N) {  // Q is the last block: its interchanges at the top of the loop, then done
+      k0 = k1;
+      continue;
+    }
+    CHK(laswp_block(k1, k2));
+    {
+      GemmShape g = {k1, k0, k0, k2, k1, k2, k2 - k1, N - k2, k1 - k0};  // P's update of Q's rows beyond Q
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+    }
+    CHK(solve_u12(k1, k2, k2));
+    // P's and Q's updates of the rows and columns beyond Q in one pass, K = k2 - k0
+    const int k3 = min(N, k2 + NBT);
+    if (look && k3 < N) {
+      CHK(hipEventRecord(la->ready, st));
+      CHK(hipStreamWaitEvent(la->stream, la->ready, 0));
+      GemmShape gr = {k2, k0, k0, k3, k2, k3, N - k2, N - k3, k2 - k0};
+      CHK(launch_gemm<T>(mb, batch, gr, skip, la->stream, split));
+      CHK(hipEventRecord(la->done, la->stream));
+      pending = true;
+      GemmShape gl = {k2, k0, k0, k2, k2, k2, N - k2, k3 - k2, k2 - k0};
+      CHK(launch_gemm<T>(mb, batch, gl, skip, st, split));
+    } else {
+      GemmShape g = {k2, k0, k0, k2, k2, k2, N - k2, N - k2, k2 - k0};
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+    }
+    CHK(factor_block(k2, k3));
+    k0 = k2;
   }
   if (pending) CHK(hipStreamWaitEvent(st, la->done, 0));
   // inverses of the diagonal blocks of U for the backward sweeps
