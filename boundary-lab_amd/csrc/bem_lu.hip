@@ -292,9 +292,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32. The dropped a_lo b_lo term and the rounding of
 // the lo parts leave a relative error near 2^-16 per product: more than FP32's, but the factors only precondition
 // the FP64 refinement that follows, which converges in the same number of steps (the LU of a Burton-Miller
-// matrix has a condition number of tens to hundreds). A workgroup owns 128 rows x 64 complex columns of C, each
-// of its four waves 64 rows x 32 complex columns as two 32-row blocks with separate real and imaginary
-// accumulators. Complex arithmetic as a real product with K doubled: the A operand is A itself ((re, im)
+// matrix has a condition number of tens to hundreds). Each wave owns 64 rows x 32 complex columns of C as two
+// 32-row blocks with separate real and imaginary accumulators. Complex arithmetic as a real product with K doubled: the A operand is A itself ((re, im)
 // interleaved along k), the B operand for the real part is (br, -bi) per k and for the imaginary part (bi, br),
 // both formed from the staged (br, bi) pairs by a sign flip or a 16-bit rotation of each dword. A is staged
 // negated, split into hi and lo rows in LDS; B transposed (one row per column of C) and split the same way.
@@ -313,16 +312,24 @@ __device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, uns
   lo = __builtin_bit_cast(unsigned, l);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
+// WY x WX waves of 64 rows x 32 complex columns each: a tile of 64 WY rows x 32 WX complex columns. The large
+// updates use 4 x 4 (256 x 128, 1024 threads, one workgroup per CU): every staged A and B element feeds twice
+// as many products as with 2 x 2 (128 x 64), which cut a 2,746-row K = 256 update over 40 matrices from 3.31
+// to 2.56 ms; updates of one 128-column block keep 2 x 2 (more workgroups).
+template <int WY, int WX>
+__global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
     MatBatch<float> mb, GemmShape g, const int* skip) {
-  constexpr int BM = 128, BN = 64, KC = 16;  // rows, complex columns, complex k per stage
-  constexpr int RW = KC + 4;                 // LDS row: KC (re, im) bf16 pairs + 16 B of padding, in dwords
+  constexpr int BM = 64 * WY, BN = 32 * WX, KC = 16;  // rows, complex columns, complex k per stage
+  constexpr int NT = 64 * WY * WX;                     // threads
+  constexpr int QA = BM * KC / NT, QB = KC * BN / NT;  // staged A and B elements per thread and stage
+  constexpr int RW = KC + 4;  // LDS row: KC (re, im) bf16 pairs + 16 B of padding, in dwords
+  static_assert(QA * NT == BM * KC && QB * NT == KC * BN, "tile and thread count do not match");
   const int mat = blockIdx.z;
   if (skip && skip[mat]) return;
   cplx<float>* A = mb.A + (size_t)mat * mb.strideA;
   const long long lda = mb.lda;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wy = w >> 1, wx = w & 1;
+  const int wy = w / WX, wx = w % WX;
   const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
   const int half = lane >> 5, l32 = lane & 31;
   __shared__ __attribute__((aligned(16))) unsigned Ah[BM][RW], Al[BM][RW], Bh[BN][RW], Bl[BN][RW];
@@ -338,38 +345,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int v = 0; v < 16; ++v) acr[ti][v] = aci[ti][v] = 0.f;
 
-  // staging per stage: A = 128 x 16 complex (8 per thread), B = 16 x 64 complex (4 per thread)
-  cplx<float> ra[8], rb[4];
+  // staging per stage: A = BM x 16 complex, B = 16 x BN complex
+  cplx<float> ra[QA], rb[QB];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int e = tid + 256 * q, r = e >> 4, kk = e & 15;
+    for (int q = 0; q < QA; ++q) {
+      const int e = tid + NT * q, r = e >> 4, kk = e & 15;
       ra[q] = (bm + r < g.M && k0 + kk < g.K) ? A[(size_t)(g.a_r0 + bm + r) * lda + g.a_c0 + k0 + kk]
                                               : mk<float>(0.f, 0.f);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + 256 * q, kk = e >> 6, j = e & 63;
+    for (int q = 0; q < QB; ++q) {
+      const int e = tid + NT * q, kk = e / BN, j = e % BN;
       rb[q] = (k0 + kk < g.K && bn + j < g.Nc) ? A[(size_t)(g.b_r0 + k0 + kk) * lda + g.b_c0 + bn + j]
                                                : mk<float>(0.f, 0.f);
     }
   };
   auto stash = [&]() {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int e = tid + 256 * q;
+    for (int q = 0; q < QA; ++q) {
+      const int e = tid + NT * q;
       unsigned hi, lo;
       split_bf16(-ra[q].re, -ra[q].im, hi, lo);
       Ah[e >> 4][e & 15] = hi;
       Al[e >> 4][e & 15] = lo;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + 256 * q;
+    for (int q = 0; q < QB; ++q) {
+      const int e = tid + NT * q;
       unsigned hi, lo;
       split_bf16(rb[q].re, rb[q].im, hi, lo);
-      Bh[e & 63][e >> 6] = hi;
-      Bl[e & 63][e >> 6] = lo;
+      Bh[e % BN][e / BN] = hi;
+      Bl[e % BN][e / BN] = lo;
     }
   };
   auto multiply = [&]() {
@@ -1428,8 +1435,12 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
   }
   if constexpr (sizeof(T) == 4) {
     if (split) {
-      hipLaunchKernelGGL(k_gemm_bf16x3, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st, mb, g,
-                         skip);
+      if (g.M >= 512 && g.Nc >= 256)
+        hipLaunchKernelGGL((k_gemm_bf16x3<4, 4>), dim3((g.Nc + 127) / 128, (g.M + 255) / 256, batch), dim3(1024), 0,
+                           st, mb, g, skip);
+      else
+        hipLaunchKernelGGL((k_gemm_bf16x3<2, 2>), dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0,
+                           st, mb, g, skip);
       return hipGetLastError();
     }
     hipLaunchKernelGGL(k_gemm_tile32<false>, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st,

@@ -213,11 +213,12 @@ def test_lu_split_bf16_products_precondition_refinement(gpu_device, n, batch):
     assert hist16[-1] < 1e-10 and len(hist16) <= 4 and len(hist16) <= len(hist32) + 1
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_lu_streams_do_not_change_the_factors(gpu_device, precision):
+@pytest.mark.parametrize("precision,products", [("mixed", "fp32"), ("mixed", "split_bf16"), ("fp64", "fp32")])
+def test_lu_streams_do_not_change_the_factors(gpu_device, precision, products):
     """Look-ahead (fan-out >= 2: a second stream for the trailing updates beyond the next block) computes every
     entry with the same operations in the same order as one stream: bitwise equal factors, pivots and
-    workspace."""
+    workspace. With split-bf16 products the look-ahead splits an update between the 256 x 128 and the 128 x 64
+    tile kernels, which form every entry with the same products in the same order."""
     from beat_hip import _lib
     from beat_hip.engine import PRECISIONS, _ptr
 
@@ -230,7 +231,9 @@ def test_lu_streams_do_not_change_the_factors(gpu_device, precision):
     prec = PRECISIONS[precision]
     lib = gpu_device.lib
     outs = []
+    mode = _lib.BEM_LU_SPLIT_BF16 if products == "split_bf16" else _lib.BEM_LU_FP32
     try:
+        _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, mode), "products")
         for fanout in (1, 4):
             _lib.check(lib.bem_ctx_set_fanout(gpu_device.ctx, fanout), "fanout")
             with torch.cuda.stream(gpu_device.stream):
@@ -248,6 +251,7 @@ def test_lu_streams_do_not_change_the_factors(gpu_device, precision):
             outs.append((A.cpu().numpy(), ipiv.cpu().numpy(), perm.cpu().numpy(), work.cpu().numpy()))
     finally:
         _lib.check(lib.bem_ctx_set_fanout(gpu_device.ctx, 4), "fanout")
+        _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, _lib.BEM_LU_FP32), "products")
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
 
