@@ -223,7 +223,7 @@ def test_lu_streams_do_not_change_the_factors(gpu_device, precision, products):
     from beat_hip.engine import PRECISIONS, _ptr
 
     torch = gpu_device.torch
-    n, batch = 700, 8
+    n, batch = 1100, 8  # the first pair's update spans 844 rows x 716 columns beyond the next block
     rng = np.random.default_rng(11)
     lda = ((n + 15) // 16) * 16
     M = rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))
