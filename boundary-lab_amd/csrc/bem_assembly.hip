@@ -649,36 +649,57 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // ------------------------------------------------------------------------------------------------
 
 
-// One lane per (pair, frequency) item; items run pair-major over pairs sorted by Duffy rule, frequency fastest,
-// so the lanes of a wave share one rule (uniform trip count, broadcast rule-point loads) and need no
-// cross-lane reduction: each lane integrates its pair at its wavenumber over all Duffy points in FP64.
+// One lane per (pair, group of NF frequencies) item; items run pair-major over pairs sorted by Duffy rule,
+// frequency group fastest, so the lanes of a wave share one rule (uniform trip count, broadcast rule-point loads)
+// and need no cross-lane reduction: each lane integrates its pair at its NF wavenumbers over all Duffy points in
+// FP64. The point geometry (r, 1/r, the normal projections, the basis products) is formed once for the NF
+// wavenumbers: NF = 2 halves that part of the work at the same occupancy (two waves per SIMD).
+template <int NF>
 __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
+  const int ngr = (nf + NF - 1) / NF;  // frequency groups per pair
   const long long item = (long long)blockIdx.x * 64 + threadIdx.x;
-  if (item >= (long long)p.P * nf) return;
-  const int pi = p.by_rule[item / nf];
-  const int f = (int)(item % nf);
+  if (item >= (long long)p.P * ngr) return;
+  const int pi = p.by_rule[item / ngr];
+  const int f0 = (int)(item % ngr) * NF;
+  const int nfv = min(NF, nf - f0);  // valid frequencies of this lane
   const int t = p.test[pi], key = p.trial[pi], ru = p.rule[pi];
   // trial key = s + r F: element s reflected by transform r (r = 0: itself)
   const int s = key % p.F, tr = key / p.F;
   const double sg[3] = {double(p.sgn[3 * tr]), double(p.sgn[3 * tr + 1]), double(p.sgn[3 * tr + 2])};
   const double det = sg[0] * sg[1] * sg[2];
-  const double k = p.k[f];
-  double t0[3], te1[3], te2[3], s0[3], se1[3], se2[3], nt[3], ns[3];
+  double k[NF];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) k[j] = p.k[f0 + min(j, nfv - 1)];
+  // y - x = d0 + sxi se1 + set se2 - txi te1 - tet te2; both elements are flat, so (y - x).n_s = pa0 + txi pa1
+  // + tet pa2 and (y - x).n_t = pb0 + sxi pb1 + set pb2 (the in-plane edges of an element are normal to its n)
+  double d0[3], te1[3], te2[3], se1[3], se2[3];
+  double pa0 = 0, pa1 = 0, pa2 = 0, pb0 = 0, pb1 = 0, pb2 = 0, nn = 0;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    t0[c] = p.fvtx[9 * t + c];
-    te1[c] = p.fvtx[9 * t + 3 + c] - t0[c];
-    te2[c] = p.fvtx[9 * t + 6 + c] - t0[c];
-    s0[c] = sg[c] * p.fvtx[9 * s + c];
-    se1[c] = sg[c] * p.fvtx[9 * s + 3 + c] - s0[c];
-    se2[c] = sg[c] * p.fvtx[9 * s + 6 + c] - s0[c];
-    nt[c] = p.fnrm[3 * t + c];
-    ns[c] = sg[c] * p.fnrm[3 * s + c];
+    const double t0 = p.fvtx[9 * t + c], s0 = sg[c] * p.fvtx[9 * s + c];
+    te1[c] = p.fvtx[9 * t + 3 + c] - t0;
+    te2[c] = p.fvtx[9 * t + 6 + c] - t0;
+    se1[c] = sg[c] * p.fvtx[9 * s + 3 + c] - s0;
+    se2[c] = sg[c] * p.fvtx[9 * s + 6 + c] - s0;
+    d0[c] = s0 - t0;
+    const double ntc = p.fnrm[3 * t + c], nsc = sg[c] * p.fnrm[3 * s + c];
+    pa0 += d0[c] * nsc;
+    pa1 -= te1[c] * nsc;
+    pa2 -= te2[c] * nsc;
+    pb0 += d0[c] * ntc;
+    pb1 += se1[c] * ntc;
+    pb2 += se2[c] * ntc;
+    nn += ntc * nsc;
   }
-  double Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
-  double Dr[9], Di[9], Gr[9], Gi[9];
+  double Ar[NF][3], Ai[NF][3];
+  double Dr[NF][9], Di[NF][9], Gr[NF][9], Gi[NF][9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) Dr[i] = Di[i] = Gr[i] = Gi[i] = 0.0;
+  for (int j = 0; j < NF; ++j) {
+#pragma unroll
+    for (int l = 0; l < 3; ++l) Ar[j][l] = Ai[j][l] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Dr[j][i] = Di[j][i] = Gr[j][i] = Gi[j][i] = 0.0;
+  }
   const int off = p.rule_off[ru], len = p.rule_len[ru];
   const double* rp = p.rpts + (size_t)off * 5;
   // the next rule point is loaded while the current one is integrated
@@ -693,79 +714,89 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
 #pragma unroll
       for (int i = 0; i < 5; ++i) nxt[i] = rp[i];
     }
-    const double dx = (s0[0] + sxi * se1[0] + set * se2[0]) - (t0[0] + txi * te1[0] + tet * te2[0]);
-    const double dy = (s0[1] + sxi * se1[1] + set * se2[1]) - (t0[1] + txi * te1[1] + tet * te2[1]);
-    const double dz = (s0[2] + sxi * se1[2] + set * se2[2]) - (t0[2] + txi * te1[2] + tet * te2[2]);
-    const double r2 = dx * dx + dy * dy + dz * dz;
+    double d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[c] = d0[c] + sxi * se1[c] + set * se2[c] - txi * te1[c] - tet * te2[c];
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     if (r2 == 0.0) continue;
     // 1/r from the hardware reciprocal square root and two Newton steps (full FP64 accuracy)
     double rinv = __builtin_amdgcn_rsq(r2);
     rinv = rinv * fma(-0.5 * r2 * rinv, rinv, 1.5);
     rinv = rinv * fma(-0.5 * r2 * rinv, rinv, 1.5);
     const double r = r2 * rinv;
-    double sn, cs;
-    sincos_phase(k * r, &sn, &cs);
-    const double gr = cs * rinv * w, gi = sn * rinv * w;  // 4 pi G w
-    const double hr = -gr * rinv - gi * k, hi = gr * k - gi * rinv;
-    const double td = (dx * ns[0] + dy * ns[1] + dz * ns[2]) * rinv;
-    const double ad = -(dx * nt[0] + dy * nt[1] + dz * nt[2]) * rinv;
+    const double td = (pa0 + txi * pa1 + tet * pa2) * rinv;
+    const double ad = -(pb0 + sxi * pb1 + set * pb2) * rinv;
     const double bt[3] = {1.0 - txi - tet, txi, tet};
     const double bs[3] = {1.0 - sxi - set, sxi, set};
-    const double dgr = hr * td, dgi = hi * td;
 #pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      Ar[l] += bt[l] * hr * ad;
-      Ai[l] += bt[l] * hi * ad;
+    for (int j = 0; j < NF; ++j) {
+      double sn, cs;
+      sincos_phase(k[j] * r, &sn, &cs);
+      const double gr = cs * rinv * w, gi = sn * rinv * w;  // 4 pi G w
+      const double hr = -gr * rinv - gi * k[j], hi = gr * k[j] - gi * rinv;
+      const double dgr = hr * td, dgi = hi * td;
+      const double har = hr * ad, hai = hi * ad;
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const double bb = bt[l] * bs[m];
-        Dr[3 * l + m] += bb * dgr;
-        Di[3 * l + m] += bb * dgi;
-        Gr[3 * l + m] += bb * gr;
-        Gi[3 * l + m] += bb * gi;
+      for (int l = 0; l < 3; ++l) {
+        Ar[j][l] += bt[l] * har;
+        Ai[j][l] += bt[l] * hai;
       }
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const double bb = bt[l] * bs[m];
+          Dr[j][3 * l + m] += bb * dgr;
+          Di[j][3 * l + m] += bb * dgi;
+          Gr[j][3 * l + m] += bb * gr;
+          Gi[j][3 * l + m] += bb * gi;
+        }
     }
   }
   const double scale = p.fa2[t] * p.fa2[s] * BEM_INV_FOUR_PI;
-  const double kinv = 1.0 / k;
-  double gsr = 0, gsi = 0;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    gsr += Gr[i];
-    gsi += Gi[i];
-  }
-  const double nn = nt[0] * ns[0] + nt[1] * ns[1] + nt[2] * ns[2];
   const double* ctp = p.fcurl + 9 * t;
   const double* csp = p.fcurl + 9 * s;
-  cplx<double>* out = p.lhs + ((size_t)f * p.P + pi) * 9;
   const bool self = (t == key);  // the coincident pair of an element with itself (not with an image)
   const double a2 = p.fa2[t];
   const double cs0 = det * sg[0], cs1 = det * sg[1], cs2 = det * sg[2];  // reflected curls: det * signs
 #pragma unroll
-  for (int l = 0; l < 3; ++l) {
+  for (int j = 0; j < NF; ++j) {
+    if (j >= nfv) break;
+    const int f = f0 + j;
+    const double kinv = 1.0 / k[j];
+    double gsr = 0, gsi = 0;
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double cp = ctp[3 * l] * cs0 * csp[3 * m] + ctp[3 * l + 1] * cs1 * csp[3 * m + 1] +
-                        ctp[3 * l + 2] * cs2 * csp[3 * m + 2];
-      const double Hr = cp * gsr - k * k * nn * Gr[3 * l + m];
-      const double Hi = cp * gsi - k * k * nn * Gi[3 * l + m];
-      double re = scale * (-Dr[3 * l + m] - Hi * kinv);
-      const double im = scale * (-Di[3 * l + m] + Hr * kinv);
-      if (self) re += 0.5 * p.m11[3 * l + m] * a2;
-      out[3 * l + m] = mk<double>(re, im);
+    for (int i = 0; i < 9; ++i) {
+      gsr += Gr[j][i];
+      gsi += Gi[j][i];
     }
-  }
-  cplx<double>* ro = p.rhs + ((size_t)f * p.P + pi) * 3;
+    cplx<double>* out = p.lhs + ((size_t)f * p.P + pi) * 9;
 #pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair. S_l = sum_m G_lm: the trial
-    // barycentric weights sum to one at every point.
-    const double Sr = Gr[3 * l] + Gr[3 * l + 1] + Gr[3 * l + 2];
-    const double Si = Gi[3 * l] + Gi[3 * l + 1] + Gi[3 * l + 2];
-    const double re = scale * (-Sr + Ai[l] * kinv);
-    double im = scale * (-Si - Ar[l] * kinv);
-    if (self) im -= 0.5 * kinv * p.m10[l] * a2;
-    ro[l] = mk<double>(re, im);
+    for (int l = 0; l < 3; ++l) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double cp = ctp[3 * l] * cs0 * csp[3 * m] + ctp[3 * l + 1] * cs1 * csp[3 * m + 1] +
+                          ctp[3 * l + 2] * cs2 * csp[3 * m + 2];
+        const double Hr = cp * gsr - k[j] * k[j] * nn * Gr[j][3 * l + m];
+        const double Hi = cp * gsi - k[j] * k[j] * nn * Gi[j][3 * l + m];
+        double re = scale * (-Dr[j][3 * l + m] - Hi * kinv);
+        const double im = scale * (-Di[j][3 * l + m] + Hr * kinv);
+        if (self) re += 0.5 * p.m11[3 * l + m] * a2;
+        out[3 * l + m] = mk<double>(re, im);
+      }
+    }
+    cplx<double>* ro = p.rhs + ((size_t)f * p.P + pi) * 3;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair. S_l = sum_m G_lm: the trial
+      // barycentric weights sum to one at every point.
+      const double Sr = Gr[j][3 * l] + Gr[j][3 * l + 1] + Gr[j][3 * l + 2];
+      const double Si = Gi[j][3 * l] + Gi[j][3 * l + 1] + Gi[j][3 * l + 2];
+      const double re = scale * (-Sr + Ai[j][l] * kinv);
+      double im = scale * (-Si - Ar[j][l] * kinv);
+      if (self) im -= 0.5 * kinv * p.m10[l] * a2;
+      ro[l] = mk<double>(re, im);
+    }
   }
 }
 
@@ -1068,8 +1099,13 @@ hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
 }
 hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
   if (p.P == 0) return hipSuccess;
-  const long long items = (long long)p.P * nf;
-  hipLaunchKernelGGL(k_singular, dim3((unsigned)((items + 63) / 64)), dim3(64), 0, st, p, nf);
+  if (nf >= 2) {
+    const long long items = (long long)p.P * ((nf + 1) / 2);
+    hipLaunchKernelGGL(k_singular<2>, dim3((unsigned)((items + 63) / 64)), dim3(64), 0, st, p, nf);
+  } else {
+    const long long items = (long long)p.P * nf;
+    hipLaunchKernelGGL(k_singular<1>, dim3((unsigned)((items + 63) / 64)), dim3(64), 0, st, p, nf);
+  }
   return hipGetLastError();
 }
 template <typename T>
