@@ -43,11 +43,17 @@ FLOPS_PER_QPAIR = 66
 # over the test point's trial points and scales the contracted sum, 1), six contracted quantities: inner 19,
 # outer 18.5 = 59, i.e. 59 / (2 x 66) = 0.45 of the ordered-pair work.
 FLOPS_PER_UNORDERED_QPAIR = 59
+# Issue slots of k_regular_pair's compiled point-pair loop: 43 VALU instructions, three of them quarter-rate
+# transcendentals (v_sin, v_cos, v_rsq) that occupy four slots each. A SIMD issues one wave64 instruction per
+# four clocks; MI355X has 256 CUs x 4 SIMDs at 2.4 GHz.
+VALU_SLOTS_PER_QPAIR = 52
+VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 PIPE_NOTE = ("compute-bound on the FP32 vector ALUs (no MFMA, no HBM limit); 157.3 TFLOP/s is MI355X's FP32 peak, "
-             "shared by the matrix pipe and the packed-FP32 vector pipe; without packed FP32 the VALU issue ceiling "
-             "is half of it")
+             "shared by the matrix pipe and the packed-FP32 vector pipe; achieved counts ordered point pairs, the "
+             "kernel evaluates each unordered pair once (executed_tflops); valu_issue_frac is the share of the "
+             "VALU issue slots the compiled loop needs")
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_regular_pair_horn6k_traffic.md"
 
 
@@ -334,7 +340,8 @@ def main():
                      "qpairs_per_launch": int(qpairs),
                      "executed_tflops": round(executed, 3),
                      "executed_flops_per_unordered_qpair": FLOPS_PER_UNORDERED_QPAIR,
-                     "frac_of_unpacked_fp32_valu": round(achieved / (PEAK_FP32_TFLOPS / 2), 4)},
+                     "valu_issue_frac": round(0.5 * qpairs * VALU_SLOTS_PER_QPAIR / 64 / VALU_WAVE_ISSUE_PER_S
+                                              / (avg_ms * 1e-3), 4)},
         "stages_ms_per_job": {k: round(v[0], 3) for k, v in stages.items()},
         "cpu_baseline": cpu,
         "serving": serving,
