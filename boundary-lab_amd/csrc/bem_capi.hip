@@ -48,6 +48,7 @@ struct bem_ctx {
   int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
   hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
   int lu_products;      // BEM_LU_FP32 or BEM_LU_SPLIT_BF16 for the large updates of FP32 factorizations
+  int num_cus;          // compute units of the device
 };
 
 // Run body(first, count, stream) on up to 1 + BEM_AUX_STREAMS contiguous sub-batches, forked from and joined
@@ -184,6 +185,10 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   c->timing = false;
   c->fanout = 1 + BEM_AUX_STREAMS;
   c->lu_products = BEM_LU_FP32;
+  c->num_cus = 0;
+  if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      c->num_cus <= 0)
+    c->num_cus = 256;
   for (int i = 0; i < BEM_TIMER_SLOTS; ++i) {
     c->timer_ms[i] = 0.0;
     c->timer_count[i] = 0;
@@ -677,7 +682,9 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
   // One factorization of the whole batch with look-ahead: the context stream runs the panel chain, the first
   // auxiliary stream the large trailing updates (fork and join with the context stream only, as everywhere
   // else, so the sequence is capturable). Fan-out 1 runs everything on the context stream.
-  const LuLookahead la = {c->fanout >= 2 ? c->aux[0] : nullptr, c->lu_ev[0], c->lu_ev[1]};
+  // the look-ahead update keeps one CU per matrix free for the panel chain that runs beside it
+  const LuLookahead la = {c->fanout >= 2 ? c->aux[0] : nullptr, c->lu_ev[0], c->lu_ev[1],
+                          std::max(c->num_cus / 2, c->num_cus - batch)};
   hipError_t e;
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};

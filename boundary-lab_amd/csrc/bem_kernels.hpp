@@ -140,6 +140,7 @@ long long lu_workspace_elems(int N);
 struct LuLookahead {
   hipStream_t stream;
   hipEvent_t ready, done;
+  int gemm_wgs;  // cap on the workgroups of the look-ahead update (0: none)
 };
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,

@@ -317,20 +317,18 @@ __device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, uns
 // as many products as with 2 x 2 (128 x 64), which cut a 2,746-row K = 256 update over 40 matrices from 3.31
 // to 2.56 ms; updates of one 128-column block keep 2 x 2 (more workgroups).
 template <int WY, int WX>
-__global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
-    MatBatch<float> mb, GemmShape g, const int* skip) {
+__device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, const GemmShape& g, int mat, int tx,
+                                                 int ty) {
   constexpr int BM = 64 * WY, BN = 32 * WX, KC = 16;  // rows, complex columns, complex k per stage
   constexpr int NT = 64 * WY * WX;                     // threads
   constexpr int QA = BM * KC / NT, QB = KC * BN / NT;  // staged A and B elements per thread and stage
   constexpr int RW = KC + 4;  // LDS row: KC (re, im) bf16 pairs + 16 B of padding, in dwords
   static_assert(QA * NT == BM * KC && QB * NT == KC * BN, "tile and thread count do not match");
-  const int mat = blockIdx.z;
-  if (skip && skip[mat]) return;
   cplx<float>* A = mb.A + (size_t)mat * mb.strideA;
   const long long lda = mb.lda;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wy = w / WX, wx = w % WX;
-  const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
+  const int bm = ty * BM, bn = tx * BN;
   const int half = lane >> 5, l32 = lane & 31;
   __shared__ __attribute__((aligned(16))) unsigned Ah[BM][RW], Al[BM][RW], Bh[BN][RW], Bl[BN][RW];
 
@@ -434,6 +432,18 @@ __global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2)
         *cp = mk<float>(c.re + acr[ti][v], c.im + aci[ti][v]);
       }
     }
+}
+
+// Tiles in the order x (columns) fastest, then rows, then matrices; a launch with fewer workgroups than tiles
+// walks them persistently, so a large update can leave CUs free for a concurrent panel chain.
+template <int WY, int WX>
+__global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
+    MatBatch<float> mb, GemmShape g, const int* skip, int nx, int ny, int ntiles) {
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tx = tile % nx, ty = (tile / nx) % ny, mat = tile / (nx * ny);
+    if (skip && skip[mat]) continue;
+    gemm_bf16x3_tile<WY, WX>(mb, g, mat, tx, ty);
+  }
 }
 
 // Skinny variant for the sub-panel updates (Nc <= 16 columns, K <= 64): the K x Nc block of B sits in LDS,
@@ -886,12 +896,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   }
 }
 
-// Row interchanges of the pivots of rows [p0, p1) (at most NBO of them) on the columns outside the trailing
-// block [k0, k1). Each workgroup resolves the composite permutation in LDS, stages the touched rows of its
-// column block in LDS and writes them back permuted.
+// Row interchanges of the pivots of rows [p0, p1) (at most NBO of them) on the columns [c_lo, c_hi) outside the
+// trailing block [k0, k1). Each workgroup resolves the composite permutation in LDS, stages the touched rows of
+// its column block in LDS and writes them back permuted.
 template <typename T>
 __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, long long strideP, int p0, int p1,
-                                               int k0, int k1, const int* skip) {
+                                               int k0, int k1, int c_lo, int c_hi, const int* skip) {
   constexpr int CB = sizeof(T) == 4 ? 64 : 32;  // columns per workgroup
   const int mat = blockIdx.y;
   if (skip && skip[mat]) return;
@@ -914,14 +924,17 @@ __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, 
   }
   __syncthreads();
   const int na = s_na;
-  // column block: columns [0, k0) then [k1, N)
-  const int outside = N - nb;
+  // column block: columns [c_lo, k0) then [k1, c_hi) (the block clipped to the window)
+  const int b0 = min(max(k0, c_lo), c_hi), b1 = min(max(k1, c_lo), c_hi);
+  const int outside = (c_hi - c_lo) - (b1 - b0);
   const int cb0 = blockIdx.x * CB;
+  (void)nb;
+  (void)N;
   for (int e = tid; e < na * CB; e += 256) {
     const int q = e / CB, jc = e % CB;
     const int cc = cb0 + jc;
     if (cc < outside) {
-      const int col = cc < k0 ? cc : cc + nb;
+      const int col = c_lo + cc < b0 ? c_lo + cc : c_lo + cc + (b1 - b0);
       buf[q][jc] = A[(size_t)s_rows[s_src[q]] * lda + col];
     }
   }
@@ -930,7 +943,7 @@ __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, 
     const int q = e / CB, jc = e % CB;
     const int cc = cb0 + jc;
     if (cc < outside && s_src[q] != q) {
-      const int col = cc < k0 ? cc : cc + nb;
+      const int col = c_lo + cc < b0 ? c_lo + cc : c_lo + cc + (b1 - b0);
       A[(size_t)s_rows[q] * lda + col] = buf[q][jc];
     }
   }
@@ -1424,10 +1437,11 @@ int panel_width(int N, int c0) {
   return mr <= 2 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
 }
 
-// split: FP32 products as three bf16 products (k_gemm_bf16x3) for the large updates
+// split: FP32 products as three bf16 products (k_gemm_bf16x3) for the large updates; max_wgs > 0 caps the
+// workgroups of a large split update (persistent tiles)
 template <typename T>
 static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape& g, const int* skip,
-                              hipStream_t st, bool split = false) {
+                              hipStream_t st, bool split = false, int max_wgs = 0) {
   if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
   if (g.Nc <= 16 && g.K <= BEM_NBO) {
     hipLaunchKernelGGL(k_gemm_skinny<T>, dim3((g.M + 63) / 64, batch), dim3(256), 0, st, mb, g, skip);
@@ -1435,12 +1449,14 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
   }
   if constexpr (sizeof(T) == 4) {
     if (split) {
-      if (g.M >= 512 && g.Nc >= 256)
-        hipLaunchKernelGGL((k_gemm_bf16x3<4, 4>), dim3((g.Nc + 127) / 128, (g.M + 255) / 256, batch), dim3(1024), 0,
-                           st, mb, g, skip);
-      else
-        hipLaunchKernelGGL((k_gemm_bf16x3<2, 2>), dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0,
-                           st, mb, g, skip);
+      if (g.M >= 512 && g.Nc >= 256) {
+        const int nx = (g.Nc + 127) / 128, ny = (g.M + 255) / 256, nt = nx * ny * batch;
+        const int nwg = max_wgs > 0 ? min(nt, max_wgs) : nt;
+        hipLaunchKernelGGL((k_gemm_bf16x3<4, 4>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
+      } else {
+        const int nx = (g.Nc + 63) / 64, ny = (g.M + 127) / 128, nt = nx * ny * batch;
+        hipLaunchKernelGGL((k_gemm_bf16x3<2, 2>), dim3(nt), dim3(256), 0, st, mb, g, skip, nx, ny, nt);
+      }
       return hipGetLastError();
     }
     hipLaunchKernelGGL(k_gemm_tile32<false>, dim3((g.Nc + 63) / 64, (g.M + 127) / 128, batch), dim3(256), 0, st,
@@ -1526,26 +1542,29 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
     return hipSuccess;
   };
 
-  // interchanges of a block's pivots on the columns outside it, one half at a time (in pivot order)
-  auto laswp_block = [&](int b0, int b1) -> hipError_t {
-    if (N > b1 - b0) {
+  // interchanges of a block's pivots on the columns [c_lo, c_hi) outside it, one half at a time (in pivot order)
+  auto laswp_cols = [&](int b0, int b1, int c_lo, int c_hi, hipStream_t s) -> hipError_t {
+    const int bl = min(max(b0, c_lo), c_hi), bh = min(max(b1, c_lo), c_hi);
+    const int ncols = (c_hi - c_lo) - (bh - bl);
+    if (ncols > 0) {
       constexpr int CB = sizeof(T) == 4 ? 64 : 32;
       for (int h0 = b0; h0 < b1; h0 += NBO) {
-        hipLaunchKernelGGL(k_laswp<T>, dim3((N - (b1 - b0) + CB - 1) / CB, batch), dim3(256), 0, st, mb, ipiv,
-                           strideP, h0, min(b1, h0 + NBO), b0, b1, skip);
+        hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + CB - 1) / CB, batch), dim3(256), 0, s, mb, ipiv, strideP, h0,
+                           min(b1, h0 + NBO), b0, b1, c_lo, c_hi, skip);
         CHK(hipGetLastError());
       }
     }
     return hipSuccess;
   };
-  // U12 = L11^-1 A12 for the rows of block [b0, b1) and the columns [c, N), by halves
-  auto solve_u12 = [&](int b0, int b1, int c) -> hipError_t {
-    CHK(solve_upper_rows(mb, batch, b0, c, N - c, W, strideW, skip, st));
+  // U12 = L11^-1 A12 for the rows of block [b0, b1) and the columns [c, ce), by halves
+  auto solve_u12 = [&](int b0, int b1, int c, int ce, hipStream_t s) -> hipError_t {
+    if (ce <= c) return hipSuccess;
+    CHK(solve_upper_rows(mb, batch, b0, c, ce - c, W, strideW, skip, s));
     if (b1 - b0 > NBO) {
       const int h0 = b0 + NBO;
-      GemmShape g = {h0, b0, b0, c, h0, c, b1 - h0, N - c, NBO};
-      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
-      CHK(solve_upper_rows(mb, batch, h0, c, N - c, W, strideW, skip, st));
+      GemmShape g = {h0, b0, b0, c, h0, c, b1 - h0, ce - c, NBO};
+      CHK(launch_gemm<T>(mb, batch, g, skip, s, split));
+      CHK(solve_upper_rows(mb, batch, h0, c, ce - c, W, strideW, skip, s));
     }
     return hipSuccess;
   };
@@ -1554,22 +1573,32 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
   // trailing matrix is read and written once per pair instead of once per block. P updates only Q's columns
   // before Q is factored; P's update of Q's rows beyond Q waits until Q's interchanges have been applied to
   // them (every row of the columns beyond Q is then still without P's update, so the interchanges commute with
-  // the deferred update, which uses P's equally interchanged L rows). Look-ahead: the pair's update of the next
-  // block's columns runs on the main stream, which then factors that block (a chain of latency-bound
-  // launches), while the look-ahead stream updates the columns beyond it; the next interchanges wait for it.
-  bool pending = false;  // a look-ahead update is in flight
+  // the deferred update, which uses P's equally interchanged L rows). Look-ahead (a second stream): everything
+  // on the columns beyond the next block R = [k2, k3) -- the pair's interchanges there, its U12 rows there and
+  // the K = 256 update there -- runs on the look-ahead stream, on at most la->gemm_wgs workgroups for the
+  // update, beside the chain on the main stream (Q, then R, factored with their columns). The main stream waits
+  // for it before it touches those columns again, at the next pair.
+  bool pending = false;  // look-ahead work on the columns beyond the next block is in flight
   CHK(factor_block(0, min(N, NBT)));
   int k0 = 0;
   while (true) {
-    const int k1 = min(N, k0 + NBT);  // block P, factored
+    const int k1 = min(N, k0 + NBT), k2 = min(N, k1 + NBT), k3 = min(N, k2 + NBT);  // P factored
     if (pending) {
       CHK(hipStreamWaitEvent(st, la->done, 0));
       pending = false;
     }
-    CHK(laswp_block(k0, k1));
+    const bool far = look && k3 < N;  // the columns [k3, N) go to the look-ahead stream
+    const int ce = far ? k3 : N;
+    hipStream_t as = far ? la->stream : st;
+    if (far) {  // P factored on the main stream: its interchanges and U12 rows on the far columns
+      CHK(hipEventRecord(la->ready, st));
+      CHK(hipStreamWaitEvent(as, la->ready, 0));
+      CHK(laswp_cols(k0, k1, k3, N, as));
+      CHK(solve_u12(k0, k1, k3, N, as));
+    }
+    CHK(laswp_cols(k0, k1, 0, ce, st));
     if (k1 >= N) break;
-    CHK(solve_u12(k0, k1, k1));
-    const int k2 = min(N, k1 + NBT);  // block Q
+    CHK(solve_u12(k0, k1, k1, ce, st));
     {
       GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, k2 - k1, k1 - k0};  // P's update of Q's columns
       CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
@@ -1579,26 +1608,30 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
       k0 = k1;
       continue;
     }
-    CHK(laswp_block(k1, k2));
+    CHK(laswp_cols(k1, k2, 0, ce, st));
+    if (far) {  // Q's interchanges, P's update of Q's rows and Q's U12 rows on the far columns
+      CHK(hipEventRecord(la->ready, st));
+      CHK(hipStreamWaitEvent(as, la->ready, 0));
+      CHK(laswp_cols(k1, k2, k3, N, as));
+      GemmShape g = {k1, k0, k0, k3, k1, k3, k2 - k1, N - k3, k1 - k0};
+      CHK(launch_gemm<T>(mb, batch, g, skip, as, split));
+      CHK(solve_u12(k1, k2, k3, N, as));
+    }
     {
-      GemmShape g = {k1, k0, k0, k2, k1, k2, k2 - k1, N - k2, k1 - k0};  // P's update of Q's rows beyond Q
+      GemmShape g = {k1, k0, k0, k2, k1, k2, k2 - k1, ce - k2, k1 - k0};  // P's update of Q's rows beyond Q
       CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
     }
-    CHK(solve_u12(k1, k2, k2));
+    CHK(solve_u12(k1, k2, k2, ce, st));
     // P's and Q's updates of the rows and columns beyond Q in one pass, K = k2 - k0
-    const int k3 = min(N, k2 + NBT);
-    if (look && k3 < N) {
-      CHK(hipEventRecord(la->ready, st));
-      CHK(hipStreamWaitEvent(la->stream, la->ready, 0));
-      GemmShape gr = {k2, k0, k0, k3, k2, k3, N - k2, N - k3, k2 - k0};
-      CHK(launch_gemm<T>(mb, batch, gr, skip, la->stream, split));
-      CHK(hipEventRecord(la->done, la->stream));
-      pending = true;
-      GemmShape gl = {k2, k0, k0, k2, k2, k2, N - k2, k3 - k2, k2 - k0};
-      CHK(launch_gemm<T>(mb, batch, gl, skip, st, split));
-    } else {
-      GemmShape g = {k2, k0, k0, k2, k2, k2, N - k2, N - k2, k2 - k0};
+    {
+      GemmShape g = {k2, k0, k0, k2, k2, k2, N - k2, ce - k2, k2 - k0};
       CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+    }
+    if (far) {
+      GemmShape gr = {k2, k0, k0, k3, k2, k3, N - k2, N - k3, k2 - k0};
+      CHK(launch_gemm<T>(mb, batch, gr, skip, as, split, la->gemm_wgs));
+      CHK(hipEventRecord(la->done, as));
+      pending = true;
     }
     CHK(factor_block(k2, k3));
     k0 = k2;
