@@ -716,7 +716,7 @@ __device__ int compose_swaps(const int* piv, int c0, int nb, int* rows, int* src
 // permuted once per block by k_laswp.
 // ------------------------------------------------------------------------------------------------
 
-template <typename T, int NBI, int MR>
+template <typename T, int NBI, int MR, int MRL = 0>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lu_panel(
     MatBatch<T> mb, int c0, int nb, int k0, int k1, int* ipiv, int* perm, long long strideP, int* info,
     const int* skip) {
@@ -740,12 +740,24 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   __shared__ int s_src[2 * NBI];
   __shared__ int s_na;
 
-  // the sub-panel: separate real and imaginary arrays (scalar arrays stay in registers at every size used)
-  T Pr[MR][NBI], Pi[MR][NBI];
-  auto getP = [&](int m, int j) { return mk<T>(Pr[m][j], Pi[m][j]); };
+  // the sub-panel: separate real and imaginary arrays (scalar arrays stay in registers at every size used); the
+  // last MRL row sets live in LDS instead (one column of a thread's row at [j][tid]: conflict-free), which lets a
+  // 16-column sub-panel cover 3,072 rows without spilling
+  constexpr int MRR = MR - MRL;
+  T Pr[MRR > 0 ? MRR : 1][NBI], Pi[MRR > 0 ? MRR : 1][NBI];
+  __shared__ T Xr[MRL > 0 ? MRL * NBI : 1][MRL > 0 ? 1024 : 1], Xi[MRL > 0 ? MRL * NBI : 1][MRL > 0 ? 1024 : 1];
+  auto getP = [&](int m, int j) {
+    if (m < MRR) return mk<T>(Pr[m][j], Pi[m][j]);
+    return mk<T>(Xr[(m - MRR) * NBI + j][tid], Xi[(m - MRR) * NBI + j][tid]);
+  };
   auto setP = [&](int m, int j, cplx<T> v) {
-    Pr[m][j] = v.re;
-    Pi[m][j] = v.im;
+    if (m < MRR) {
+      Pr[m][j] = v.re;
+      Pi[m][j] = v.im;
+    } else {
+      Xr[(m - MRR) * NBI + j][tid] = v.re;
+      Xi[(m - MRR) * NBI + j][tid] = v.im;
+    }
   };
 #pragma unroll
   for (int m = 0; m < MR; ++m) {
@@ -1407,14 +1419,15 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
                                int* perm, long long strideP, int* info, const int* skip, hipStream_t st) {
   const int H = mb.N - c0;
   const int mr = (H + 1023) / 1024;
-#define PANEL(NBI_, MR_)                                                                                    \
-  hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, k0, k1, ipiv, \
-                     perm, strideP, info, skip)
-  // (NBI, MR) pairs chosen so that the register-resident sub-panel does not spill at 128 VGPRs/lane
+#define PANEL(NBI_, MR_, ...)                                                                                 \
+  hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_, ##__VA_ARGS__>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, k0, \
+                     k1, ipiv, perm, strideP, info, skip)
+  // (NBI, MR) pairs chosen so that the register-resident sub-panel does not spill at 128 VGPRs/lane; (16, 3) keeps
+  // its third row set in LDS (128 KB)
   if constexpr (sizeof(T) == 4) {
     if (mr <= 1) PANEL(16, 1);
     else if (mr <= 2) PANEL(16, 2);
-    else if (mr <= 3) PANEL(8, 3);
+    else if (mr <= 3) PANEL(16, 3, 1);
     else if (mr <= 6) PANEL(4, 6);
     else if (mr <= 12) PANEL(2, 12);
     else return hipErrorInvalidValue;
@@ -1433,7 +1446,7 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
 template <typename T>
 int panel_width(int N, int c0) {
   const int mr = (N - c0 + 1023) / 1024;
-  if (sizeof(T) == 4) return mr <= 2 ? 16 : (mr <= 3 ? 8 : (mr <= 6 ? 4 : 2));
+  if (sizeof(T) == 4) return mr <= 3 ? 16 : (mr <= 6 ? 4 : 2);
   return mr <= 2 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
 }
 
