@@ -518,6 +518,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int lane = threadIdx.x;
   __shared__ cplx<float> blk[NFW][64][9];
   __shared__ cplx<float> pend[BEM_PEND_SLOTS][NFW][6];  // row-strip sums [0..2], column-strip sums [3..5]
+  __shared__ uint32_t tab[128];                          // the chunk's slot byte table
 
   float k[NFW], kappa[NFW];
 #pragma unroll
@@ -538,17 +539,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int tpos = p.elem_pos[t];
   const int c_row = p.chunk_start[t], c_col = tpos >> 6;
 
-  // per-slot sums of the chunk's blocks in blk into one strip (w = 0 row strip, 1 column strip)
+  // per-slot sums of the chunk's blocks in blk into one strip (w = 0 row strip, 1 column strip); the slot
+  // incidence ranges and codes come from the chunk's byte table in LDS (tb), staged before the chunk's pairs
+  const uint8_t* tb = reinterpret_cast<const uint8_t*>(tab);
   auto store_slots = [&](int c, int w, int c_first) {
     const int nsl = p.slot_count[c];
-    const uint16_t* st = p.slot_start + (size_t)c * SLOT_STRIDE;
-    const uint16_t* inc = p.slot_inc + (size_t)c * 192;
     const int* sv = p.slot_vertex + (size_t)c * 192;
     const int* sp = p.slot_pend + (size_t)c * 192;
     const int* spv = p.slot_prev + (size_t)c * 192;
     cplx<float>* strip = w == 0 ? p.strip : p.strip2;
     for (int kk = lane; kk < nsl; kk += 64) {
-      const int e0 = st[kk], e1 = st[kk + 1];
+      const int e0 = tb[kk], e1 = tb[kk + 1];
       const int vcode = sv[kk];
       const int v = vcode >> 1;
       const bool seen = spv[kk] >= c_first;  // an earlier chunk of this wave held the vertex
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (j < nfw) {
           cplx<float> a0 = mk<float>(0, 0), a1 = mk<float>(0, 0), a2 = mk<float>(0, 0);
           for (int e = e0; e < e1; ++e) {
-            const int code = inc[e];
+            const int code = tb[256 + e];
             const int src = code >> 2, m = code & 3;
             a0 = cadd(a0, blk[j][src][m]);
             a1 = cadd(a1, blk[j][src][3 + m]);
@@ -595,6 +596,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   for (int c = c_row; c < p.n_chunks; ++c) {
     const int spos = c * 64 + lane;
     const int s = p.order[spos];
+    // the chunk's slot byte table, loaded now and stored to LDS after the pairs: its latency hides behind them
+    const uint2 tw = reinterpret_cast<const uint2*>(p.slot_bytes + (size_t)c * 128)[lane];
     cplx<float> C[NFW][9], Cp[NFW][9];
 #pragma unroll
     for (int j = 0; j < NFW; ++j)
@@ -623,6 +626,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         regular_pair_f32<NFW>(te->q, nt, ct, a2t, p.rec[s], k, kappa, C, Cp);
       }
     }
+    tab[2 * lane] = tw.x;
+    tab[2 * lane + 1] = tw.y;
 #pragma unroll
     for (int j = 0; j < NFW; ++j)
 #pragma unroll

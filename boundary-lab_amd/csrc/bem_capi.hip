@@ -119,7 +119,8 @@ struct bem_mesh {
   int4* fv;
   int *order, *slot_count, *slot_vertex, *slot_pend, *slot_prev, *elem_pos, *chunk_start, *vertex_last, *vstar_ptr,
       *vstar_row;
-  int4* vstar_ent;  // per star entry: strip row, first chunk of its row / column strip, the star's vertex
+  int4* vstar_ent;       // per star entry: strip row, first chunk of its row / column strip, the star's vertex
+  uint32_t* slot_bytes;  // per chunk: 256 bytes of incidence starts, 256 bytes of incidence codes
   uint16_t *slot_start, *slot_inc;
   double *fvtx, *fnrm, *fcurl, *fa2, *rpts;
   int *sing_test, *sing_trial, *sing_rule, *sing_ptr, *rule_off, *rule_len, *src_faces, *sing_by_rule;
@@ -310,6 +311,14 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   }
   for (int ch = 0; ch < d->n_chunks; ++ch) {
     if (d->slot_count[ch] < 0 || d->slot_count[ch] > BEM_SLOT_INC) return fail(BEM_E_ARG, "slot count out of range");
+    // incidence ranges non-decreasing within the chunk's 192 entries; codes (lane << 2 | corner) of 64 lanes
+    const uint16_t* st = d->slot_start + (size_t)ch * BEM_SLOT_STRIDE;
+    for (int k = 0; k < d->slot_count[ch]; ++k)
+      if (st[k] > st[k + 1] || st[k + 1] > BEM_SLOT_INC) return fail(BEM_E_ARG, "slot incidence range out of order");
+    for (int e = 0; e < st[d->slot_count[ch]]; ++e) {
+      const int code = d->slot_inc[(size_t)ch * BEM_SLOT_INC + e];
+      if ((code >> 2) >= 64 || (code & 3) == 3) return fail(BEM_E_ARG, "slot incidence code out of range");
+    }
     for (int k = 0; k < d->slot_count[ch]; ++k) {
       const int v = d->slot_vertex[(size_t)ch * BEM_SLOT_INC + k] >> 1;
       const int pd = d->slot_pend[(size_t)ch * BEM_SLOT_INC + k] & 0xFFFF;
@@ -425,6 +434,19 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   UP(m->slot_vertex, d->slot_vertex, (size_t)d->n_chunks * 192);
   UP(m->slot_pend, d->slot_pend, (size_t)d->n_chunks * 192);
   UP(m->slot_prev, d->slot_prev, (size_t)d->n_chunks * 192);
+  {
+    // per chunk 512 bytes: incidence starts of the slots (bytes 0..192) and incidence codes (bytes 256..447), the
+    // paired kernel's LDS copy of the chunk's slot table
+    std::vector<uint32_t> sb((size_t)d->n_chunks * 128, 0u);
+    uint8_t* b = reinterpret_cast<uint8_t*>(sb.data());
+    for (int ch = 0; ch < d->n_chunks; ++ch) {
+      const uint16_t* st = d->slot_start + (size_t)ch * BEM_SLOT_STRIDE;
+      for (int k = 0; k <= d->slot_count[ch]; ++k) b[(size_t)ch * 512 + k] = (uint8_t)st[k];
+      for (int e = 0; e < st[d->slot_count[ch]]; ++e)
+        b[(size_t)ch * 512 + 256 + e] = (uint8_t)d->slot_inc[(size_t)ch * BEM_SLOT_INC + e];
+    }
+    UP(m->slot_bytes, sb.data(), sb.size());
+  }
   UP(m->elem_pos, d->elem_pos, (size_t)F);
   UP(m->chunk_start, d->chunk_start, (size_t)F);
   UP(m->vertex_last, d->vertex_last, (size_t)V);
@@ -590,6 +612,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.slot_count = m->slot_count;
     p.slot_start = m->slot_start;
     p.slot_inc = m->slot_inc;
+    p.slot_bytes = m->slot_bytes;
     p.slot_vertex = m->slot_vertex;
     p.slot_pend = m->slot_pend;
     p.slot_prev = m->slot_prev;

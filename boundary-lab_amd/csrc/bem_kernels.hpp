@@ -37,6 +37,7 @@ struct RegParams {
   const int* slot_count;        // [n_chunks]
   const uint16_t* slot_start;   // [n_chunks][BEM_SLOT_STRIDE]
   const uint16_t* slot_inc;     // [n_chunks][BEM_SLOT_INC]  (lane << 2) | m
+  const uint32_t* slot_bytes;   // [n_chunks][128] bytes: slot starts at 0.., incidence codes at 256..
   const int* slot_vertex;       // [n_chunks][BEM_SLOT_INC]  2 * vertex + (earlier chunk stored the column)
   const int* slot_pend;         // [n_chunks][BEM_SLOT_INC]  LDS slot | BEM_PEND_FROM_LDS | BEM_PEND_TO_STRIP
   const int* slot_prev;         // [n_chunks][BEM_SLOT_INC]  previous chunk holding the vertex (-1: none)
