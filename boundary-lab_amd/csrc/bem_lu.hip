@@ -1603,13 +1603,15 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
     const bool far = look && k3 < N;  // the columns [k3, N) go to the look-ahead stream
     const int ce = far ? k3 : N;
     hipStream_t as = far ? la->stream : st;
-    if (far) {  // P factored on the main stream: its interchanges and U12 rows on the far columns
+    if (far) {  // P factored on the main stream: its interchanges and U12 rows on the far columns, and its
+                // interchanges on the L columns of the blocks before it (read by nothing until the solves)
       CHK(hipEventRecord(la->ready, st));
       CHK(hipStreamWaitEvent(as, la->ready, 0));
       CHK(laswp_cols(k0, k1, k3, N, as));
       CHK(solve_u12(k0, k1, k3, N, as));
+      CHK(laswp_cols(k0, k1, 0, k0, as));
     }
-    CHK(laswp_cols(k0, k1, 0, ce, st));
+    CHK(laswp_cols(k0, k1, far ? k1 : 0, ce, st));
     if (k1 >= N) break;
     CHK(solve_u12(k0, k1, k1, ce, st));
     {
@@ -1621,14 +1623,16 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
       k0 = k1;
       continue;
     }
-    CHK(laswp_cols(k1, k2, 0, ce, st));
-    if (far) {  // Q's interchanges, P's update of Q's rows and Q's U12 rows on the far columns
+    CHK(laswp_cols(k1, k2, far ? k0 : 0, ce, st));
+    if (far) {  // Q's interchanges, P's update of Q's rows and Q's U12 rows on the far columns; Q's
+                // interchanges on the L columns before P
       CHK(hipEventRecord(la->ready, st));
       CHK(hipStreamWaitEvent(as, la->ready, 0));
       CHK(laswp_cols(k1, k2, k3, N, as));
       GemmShape g = {k1, k0, k0, k3, k1, k3, k2 - k1, N - k3, k1 - k0};
       CHK(launch_gemm<T>(mb, batch, g, skip, as, split));
       CHK(solve_u12(k1, k2, k3, N, as));
+      CHK(laswp_cols(k1, k2, 0, k0, as));
     }
     {
       GemmShape g = {k1, k0, k0, k2, k1, k2, k2 - k1, ce - k2, k1 - k0};  // P's update of Q's rows beyond Q
