@@ -256,6 +256,38 @@ def test_lu_streams_do_not_change_the_factors(gpu_device, precision, products):
         assert np.array_equal(a, b)
 
 
+def _corrupt_slots(s, what):
+    s.slot_start = s.slot_start.copy()
+    s.slot_inc = s.slot_inc.copy()
+    s.slot_pend = s.slot_pend.copy()
+    if what == "start":  # an incidence range running backwards
+        s.slot_start[0, 1], s.slot_start[0, 2] = s.slot_start[0, 2], s.slot_start[0, 1]
+    elif what == "lane":  # an incidence code naming lane 64
+        s.slot_inc[0, 0] = 64 << 2
+    elif what == "corner":  # corner 3 of a triangle
+        s.slot_inc[0, 0] = (s.slot_inc[0, 0] & ~3) | 3
+    elif what == "pend":  # an LDS partial-sum slot past the 96 the kernels hold
+        s.slot_pend[0, 0] = 96
+
+
+@pytest.mark.parametrize("what,message", [("start", "incidence range"), ("lane", "incidence code"),
+                                          ("corner", "incidence code"), ("pend", "partial-sum slot")])
+def test_mesh_upload_rejects_bad_slot_tables(gpu_device, sphere, what, message):
+    """The slot tables index LDS arrays inside the assembly kernels; mesh upload refuses tables that would send
+    an index past them (the error names the table) instead of launching on them."""
+    import copy
+    import dataclasses
+
+    from beat_hip import _lib
+    from beat_hip.engine import DeviceMesh
+
+    slots = copy.copy(sphere.slots)
+    _corrupt_slots(slots, what)
+    prep = dataclasses.replace(sphere, slots=slots)
+    with pytest.raises(_lib.BemError, match=message):
+        DeviceMesh(gpu_device, prep, np.arange(prep.n_faces, dtype=np.int32))
+
+
 def test_golden_fixtures(gpu_device):
     """Committed oracle fixtures: sphere (1,280 triangles) pressure and field; two disjoint tetrahedra (ragged
     chunk of 8 elements, two unwelded parts) operators recombined into the Burton-Miller LHS."""
