@@ -1422,21 +1422,21 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
 #define PANEL(NBI_, MR_, ...)                                                                                 \
   hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_, ##__VA_ARGS__>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, k0, \
                      k1, ipiv, perm, strideP, info, skip)
-  // (NBI, MR) pairs chosen so that the register-resident sub-panel does not spill at 128 VGPRs/lane; (16, 3) keeps
-  // its third row set in LDS (128 KB)
+  // (NBI, MR, MRL): the sub-panel's register rows stay within 128 VGPRs/lane without spilling; the last MRL row
+  // sets per thread live in 128 KB of LDS, which doubles the width wherever more than two row sets are needed
   if constexpr (sizeof(T) == 4) {
     if (mr <= 1) PANEL(16, 1);
     else if (mr <= 2) PANEL(16, 2);
     else if (mr <= 3) PANEL(16, 3, 1);
-    else if (mr <= 6) PANEL(4, 6);
-    else if (mr <= 12) PANEL(2, 12);
+    else if (mr <= 6) PANEL(8, 6, 2);
+    else if (mr <= 12) PANEL(4, 12, 4);
     else return hipErrorInvalidValue;
   } else {
     if (mr <= 1) PANEL(8, 1);
     else if (mr <= 2) PANEL(8, 2);
-    else if (mr <= 3) PANEL(4, 3);
-    else if (mr <= 6) PANEL(2, 6);
-    else if (mr <= 12) PANEL(1, 12);
+    else if (mr <= 3) PANEL(8, 3, 1);
+    else if (mr <= 6) PANEL(4, 6, 2);
+    else if (mr <= 12) PANEL(2, 12, 4);
     else return hipErrorInvalidValue;
   }
 #undef PANEL
@@ -1446,8 +1446,8 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
 template <typename T>
 int panel_width(int N, int c0) {
   const int mr = (N - c0 + 1023) / 1024;
-  if (sizeof(T) == 4) return mr <= 3 ? 16 : (mr <= 6 ? 4 : 2);
-  return mr <= 2 ? 8 : (mr <= 3 ? 4 : (mr <= 6 ? 2 : 1));
+  if (sizeof(T) == 4) return mr <= 3 ? 16 : (mr <= 6 ? 8 : 4);
+  return mr <= 3 ? 8 : (mr <= 6 ? 4 : 2);
 }
 
 // split: FP32 products as three bf16 products (k_gemm_bf16x3) for the large updates; max_wgs > 0 caps the

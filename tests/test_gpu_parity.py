@@ -97,10 +97,12 @@ def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precisi
 
 
 # sizes: one partial half block (40), one trailing block with a partial second half (100), partial trailing
-# blocks (190, 200) and the register-panel thresholds (1500, 2100)
+# blocks (190, 200) and the register-panel thresholds (1500, 2100), and the panels with row sets in LDS (3,500 to
+# 6,500 rows: three to seven row sets per thread)
 @pytest.mark.parametrize("precision,n,batch", [("mixed", 40, 2), ("fp64", 100, 1), ("mixed", 100, 2),
                                                ("mixed", 190, 2), ("fp64", 200, 2), ("mixed", 200, 3),
-                                               ("fp64", 1500, 1), ("mixed", 2100, 2)])
+                                               ("fp64", 1500, 1), ("mixed", 2100, 2), ("fp64", 3500, 1),
+                                               ("mixed", 4500, 1), ("mixed", 6500, 1), ("fp64", 6500, 1)])
 def test_lu_random(gpu_device, precision, n, batch):
     import ctypes as C
 
@@ -136,8 +138,9 @@ def test_lu_random(gpu_device, precision, n, batch):
     gpu_device.sync()
     X = y[:, :, :n].cpu().numpy().astype(np.complex128)
     # forward error vs LAPACK, and the backward error |B - M X| / (|M|_2 |X|) that partial-pivoting LU bounds.
-    # LAPACK's own cgetrf reaches 6.6e-4 forward / 1.6e-5 backward on a matrix of this family at N = 2100.
-    tol_fwd, tol_bwd = (1e-10, 1e-13) if precision == "fp64" else (3e-3, 1e-4)
+    # LAPACK's own cgetrf reaches 6.6e-4 forward / 1.6e-5 backward on a matrix of this family at N = 2100, and
+    # 1.1e-2 / 4.7e-5 at N = 6,500 (the forward error grows with the condition number).
+    tol_fwd, tol_bwd = (1e-10, 1e-13) if precision == "fp64" else ((3e-3 if n <= 2100 else 5e-2), 1e-4)
     for m in range(batch):
         ref = np.linalg.solve(M[m], B[m].T).T
         err = np.linalg.norm(X[m] - ref) / np.linalg.norm(ref)
