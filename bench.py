@@ -208,6 +208,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=16)
     ap.add_argument("--serving-rate", type=float, default=2.0)
+    ap.add_argument("--serving-batch", type=int, default=32, help="frequencies per device batch in the serving loop")
     ap.add_argument("--fanout", type=int, default=4,
                     help="streams used by the batched LU and refinement (1 = single stream, e.g. for tracing)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -307,7 +308,7 @@ def main():
     serving = None
     if args.serving_jobs > 0:
         log(f"bench: serving stream, {args.serving_jobs} jobs at {args.serving_rate}/s")
-        serving = serving_sample(args.serving_jobs, args.serving_rate, 2024, batch=8, slots=4)
+        serving = serving_sample(args.serving_jobs, args.serving_rate, 2024, batch=args.serving_batch, slots=4)
         log(f"bench: serving p50 {serving['p50_latency_s']:.2f} s p99 {serving['p99_latency_s']:.2f} s")
 
     out = {

@@ -18,6 +18,7 @@ from typing import Callable, Iterator
 import numpy as np
 
 from . import contract as K
+from .driver import DEFAULT_FREQUENCY_BATCH
 from .serving import shared_scheduler
 
 
@@ -79,7 +80,8 @@ class BeatHipRocmBackend:
     beat_engine_backend = "rocm"
     capabilities = K.BEAT_ROCM_CAPABILITIES
 
-    def __init__(self, *, slots: int = 4, batch: int = 8, hbm_budget_bytes: int | None = None, **_ignored):
+    def __init__(self, *, slots: int = 4, batch: int = DEFAULT_FREQUENCY_BATCH, hbm_budget_bytes: int | None = None,
+                 **_ignored):
         self._sched_kw = dict(slots=slots, batch=batch, hbm_budget_bytes=hbm_budget_bytes)
 
     @property

@@ -154,7 +154,7 @@ class JobHandle:
 
 
 class Scheduler:
-    def __init__(self, hbm_budget_bytes: int | None = None, slots: int = 4, batch: int = 8,
+    def __init__(self, hbm_budget_bytes: int | None = None, slots: int = 4, batch: int = D.DEFAULT_FREQUENCY_BATCH,
                  device_index: int = 0, engine_factory=None, artifact_root=None):
         self.artifact_root = None if artifact_root is None else Path(artifact_root)
         self.slots = int(slots)

@@ -25,6 +25,7 @@ import numpy as np
 
 from . import geometry as G
 from . import _lib
+from .driver import DEFAULT_FREQUENCY_BATCH
 
 SCHEMA_VERSION = 1
 
@@ -330,7 +331,8 @@ def frequency_result(job, freq, pressure, field, timings, first) -> dict:
     }
 
 
-def solve_system_request(request, engine_factory, emit, batch: int = 8, event_mode: bool = True):
+def solve_system_request(request, engine_factory, emit, batch: int = DEFAULT_FREQUENCY_BATCH,
+                         event_mode: bool = True):
     """Run a compiled-system exterior request; emits one ``result`` per frequency (wrapped as an event in
     worker mode, bare in one-shot mode) and returns (cancelled, solved_count) (coupled_solver.jl:267-569).
     engine_factory(prepared_mesh, qunit) -> beat_hip.engine.Engine."""
