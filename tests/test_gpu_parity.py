@@ -268,7 +268,7 @@ def _corrupt_slots(s, what):
     elif what == "lane":  # an incidence code naming lane 64
         s.slot_inc[0, 0] = 64 << 2
     elif what == "corner":  # corner 3 of a triangle
-        s.slot_inc[0, 0] = (s.slot_inc[0, 0] & ~3) | 3
+        s.slot_inc[0, 0] = int(s.slot_inc[0, 0]) | 3
     elif what == "pend":  # an LDS partial-sum slot past the 96 the kernels hold
         s.slot_pend[0, 0] = 96
 
