@@ -146,7 +146,10 @@ __device__ __forceinline__ void trial_contract(const float (&x)[6], float (&t)[3
   t[2] = base + float(kSpA) * x[3] + float(kSpB) * x[5];
 }
 
-// FP32 block for NFW wavenumbers at once: the pair geometry (r, 1/r, r.n_s) is shared by the frequencies.
+// FP32 block for NFW wavenumbers at once: the pair geometry (r, 1/r) is shared by the frequencies. The trial
+// element is flat, so (y_b - x_a).n_s is the same for the six trial points of a test point: the double-layer
+// kernel is u (y_b - x_a).n_s with u = 4 pi G (ik - 1/r) / r, and the n_s offset scales the trial-contracted sum
+// once per test point. The points of a regular pair (no shared vertex, images included) are distinct.
 template <int NFW>
 __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, const float (&nt)[3],
                                                   const float (&ct)[9], float a2t, const ElemRec<float>& se,
@@ -156,14 +159,16 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
 #pragma unroll
   for (int i = 0; i < 18; ++i) y[i] = se.q[i];
   const float nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
+  const float cs_plane = y[0] * nsx + y[1] * nsy + y[2] * nsz;
   float DMr[NFW][9], DMi[NFW][9], GMr[NFW][9], GMi[NFW][9];
 #pragma unroll
   for (int j = 0; j < NFW; ++j)
 #pragma unroll
     for (int i = 0; i < 9; ++i) DMr[j][i] = DMi[j][i] = GMr[j][i] = GMi[j][i] = 0.f;
-#pragma unroll 1  // rolled: 185 VGPRs (two waves per SIMD) and no spills with two wavenumbers
+#pragma unroll 1
   for (int a = 0; a < 6; ++a) {
     const float xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
+    const float ps = cs_plane - (xa * nsx + ya * nsy + za * nsz);  // (y_b - x_a).n_s for every b
     float xDr[NFW][6], xDi[NFW][6], xGr[NFW][6], xGi[NFW][6];
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -171,18 +176,16 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
       const float dy = y[3 * b + 1] - ya;
       const float dz = y[3 * b + 2] - za;
       const float r2 = dx * dx + dy * dy + dz * dz;
-      const float rinv = Phase<float>::rsqrt_(r2);
+      const float rinv = __builtin_amdgcn_rsqf(r2);
       const float r = r2 * rinv;
-      const float td = (dx * nsx + dy * nsy + dz * nsz) * rinv;
+      const float rinv2 = rinv * rinv;
 #pragma unroll
       for (int j = 0; j < NFW; ++j) {
         float sn, cs;
         Phase<float>::sincos_kr(kappa[j], k[j], r, &sn, &cs);
         const float gr = cs * rinv, gi = sn * rinv;  // 4 pi G
-        const float hr = -gr * rinv - gi * k[j];     // 4 pi G (ik - 1/r)
-        const float hi = gr * k[j] - gi * rinv;
-        xDr[j][b] = hr * td;
-        xDi[j][b] = hi * td;
+        xDr[j][b] = -rinv2 * fmaf(sn, k[j], gr);      // 4 pi G (ik - 1/r) / r
+        xDi[j][b] = rinv2 * fmaf(cs, k[j], -gi);
         xGr[j][b] = gr;
         xGi[j][b] = gi;
       }
@@ -196,11 +199,12 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
       trial_contract(xGi[j], tGi);
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        const float wa = float(c_phiw.v[l * 6 + a]);
+        const float wa = c_phiw_f.v[l * 6 + a];
+        const float wd = wa * ps;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          DMr[j][3 * l + m] += wa * tDr[m];
-          DMi[j][3 * l + m] += wa * tDi[m];
+          DMr[j][3 * l + m] += wd * tDr[m];
+          DMi[j][3 * l + m] += wd * tDi[m];
           GMr[j][3 * l + m] += wa * tGr[m];
           GMi[j][3 * l + m] += wa * tGi[m];
         }
@@ -348,10 +352,11 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
   }
 }
 
-// One wave per (test element t, group of NFW wavenumbers); FP32 shares the pair geometry across NFW = 2
-// wavenumbers, FP64 runs one wavenumber per wave (NFW = 1).
+// One wave per (test element t, group of NFW wavenumbers that share the pair geometry); both precisions launch
+// NFW = 1: FP32 then needs at most 128 VGPRs (149 with the image loop) and 7 KB of LDS, four (three) waves per SIMD.
 template <typename T, int NFW, bool SYM>
-__global__ __launch_bounds__(64) void k_regular_lhs(RegParams<T> p, int nf) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (SYM ? 3 : 4) : 1))) void
+k_regular_lhs(RegParams<T> p, int nf) {
   const int t = blockIdx.x;
   const int f0 = blockIdx.y * NFW;
   const int nfw = min(NFW, nf - f0);  // the last group may be partial: its spare slot repeats a wavenumber
@@ -1086,7 +1091,7 @@ __global__ __launch_bounds__(256) void k_gather_rhs(GatherParams<T> g) {
 
 template <typename T>
 hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
-  constexpr int NFW = sizeof(T) == 4 ? 2 : 1;
+  constexpr int NFW = 1;  // one wavenumber per wave: four FP32 waves per SIMD (see k_regular_pair)
   if constexpr (sizeof(T) == 4) {
     if (p.strip2) {
       // one wavenumber per wave: 9 KB of LDS and at most 128 VGPRs give four waves per SIMD, against two
