@@ -940,12 +940,13 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
   const int i = blockIdx.x;
   const int f = blockIdx.y;
   const int tid = threadIdx.x;
-  __shared__ int rows[32];  // vertex valence <= 32 is checked at mesh upload
+  __shared__ int rows[64];  // the first 64 strip rows of the star; the rest (valence above 64) come from memory
   __shared__ double red[256];
   const int s0 = g.vstar_ptr[i], s1 = g.vstar_ptr[i + 1];
-  const int ns = min(s1 - s0, 32);
-  if (tid < ns) rows[tid] = g.vstar_row[s0 + tid];
+  const int ns = s1 - s0;
+  if (tid < min(ns, 64)) rows[tid] = g.vstar_row[s0 + tid];
   __syncthreads();
+  auto row = [&](int r) { return r < 64 ? rows[r] : g.vstar_row[s0 + r]; };
   const cplx<T>* strip = g.strip + (size_t)f * g.strip_fstride;
   cplx<T>* Ar = g.A + (size_t)f * g.strideA + (size_t)i * g.lda;
   cplx<T>* A0r = g.A0 ? g.A0 + (size_t)f * g.strideA + (size_t)i * g.lda : nullptr;
@@ -954,7 +955,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
   // strip rows are vertex-indexed: row i of A is the plain sum of the strip rows of its star (coalesced)
   for (int j = tid; j < g.N; j += 256) {
     cplx<T> acc = mk<T>(0, 0);
-    for (int r = 0; r < ns; ++r) acc = cadd(acc, strip[(size_t)rows[r] * g.L + j]);
+    for (int r = 0; r < ns; ++r) acc = cadd(acc, strip[(size_t)row(r) * g.L + j]);
     if (g.roww) acc = mk<T>(acc.re * T(wi), acc.im * T(wi));
     Ar[j] = acc;
     if (A0r) A0r[j] = acc;
@@ -970,7 +971,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams<T> g) {
   if (tid < g.n_ch) {
     const cplx<double>* rp = g.rhs_part + (size_t)f * 3 * g.F * MAX_CH;
     cplx<double> acc = mk<double>(0, 0);
-    for (int r = 0; r < ns; ++r) acc = cadd(acc, rp[(size_t)rows[r] * MAX_CH + tid]);
+    for (int r = 0; r < ns; ++r) acc = cadd(acc, rp[(size_t)row(r) * MAX_CH + tid]);
     acc = mk<double>(acc.re * wi, acc.im * wi);
     g.b[(size_t)f * g.strideB + (size_t)tid * g.ldb + i] = cmul(acc, g.rhs_scale[f]);
   }

@@ -486,11 +486,6 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   if (R > 1 && d->row_weights) UP(m->roww, d->row_weights, (size_t)V);
   if (R > 1 && d->plane_mask) UP(m->pmask, d->plane_mask, (size_t)F);
 #undef UP
-  for (int v = 0; v < V; ++v)
-    if (d->vstar_ptr[v + 1] - d->vstar_ptr[v] > 32) {
-      bem_mesh_destroy(m);
-      return fail(BEM_E_ARG, "vertex valence above 32 is not supported");
-    }
   *out = m;
   return BEM_OK;
 }

@@ -65,6 +65,45 @@ def test_assembly_matches_oracle(gpu_device, sphere, oracle_sphere, precision, t
     eng.close()
 
 
+@pytest.fixture(scope="module")
+def fan_sphere():
+    """Latitude/longitude sphere whose two poles are shared by 80 triangles each (1,120 triangles): star rows
+    beyond the 64 the row gather keeps in LDS, and vertices that span two 64-element chunks."""
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+    from oracle import bem_oracle as O
+
+    V, F, T = syn.uv_sphere(8, 80, A_SPHERE)
+    prep = G.prepare(G.BoundaryMesh(V, F, T))
+    m = prep.original
+    ref = O.solve_frequency(O.Mesh(m.vertices, m.faces, m.tags, m.mesh_ids), FREQ, np.ones((m.n_faces, 1)), C0, RHO)
+    return prep, ref
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-5)])
+def test_high_valence_vertices_match_oracle(gpu_device, fan_sphere, precision, tol):
+    prep, ref = fan_sphere
+    assert np.bincount(prep.original.faces.ravel()).max() == 80
+    eng = _engine(gpu_device, prep)
+    b = eng.batch(precision, 1, None, RHO, C0)
+    b.set_frequencies([FREQ])
+    with gpu_device.torch.cuda.stream(gpu_device.stream):
+        b._assemble()
+    gpu_device.sync()
+    N = prep.n_vertices
+    A = b.A0[0, :, :N].cpu().numpy().astype(np.complex128)
+    rhs = b.b[0, 0, :N].cpu().numpy()
+    vp = prep.vperm
+    print(precision, "A rel", _rel(A, ref["lhs"][np.ix_(vp, vp)]), "b rel", _rel(rhs, ref["rhs"][vp, 0]))
+    assert _rel(A, ref["lhs"][np.ix_(vp, vp)]) < tol
+    assert _rel(rhs, ref["rhs"][vp, 0]) < tol
+    res, _ = eng.solve([FREQ], precision=precision, rho=RHO, sound_speed=C0)
+    p = res[0].pressure[0, :, 0]
+    assert res[0].info[0] == 0
+    assert _rel(p, ref["pressure"][:, 0]) < (1e-10 if precision == "fp64" else 1e-4)
+    eng.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precision):
     eng = _engine(gpu_device, sphere)
