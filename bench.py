@@ -114,6 +114,85 @@ def horn_job():
     return prep, qunit, obs, freqs
 
 
+def scale_job():
+    """Two-way loudspeaker-like box (~20,000 triangles): horn throat and two piston cones as three velocity
+    sources, 24 log-spaced frequencies 200 Hz-4 kHz, far field on a 5,000-point Fibonacci sphere at 2 m."""
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+    from beat_hip import driver as D
+
+    V, F, T = syn.loudspeaker_box()
+    mesh = G.BoundaryMesh(V, F, T)
+    prep = G.prepare(mesh)
+    qunit = np.stack([(mesh.tags == tag) for tag in (2, 3, 4)], axis=1).astype(np.complex128)
+    obs = D.spherical_observation({"spherical_sampling_enabled": True, "spherical_sampling_points": 5000,
+                                   "distance": 2.0})[0].astype(np.float64)
+    freqs = np.logspace(np.log10(200.0), np.log10(4000.0), 24)
+    return prep, qunit, obs, freqs
+
+
+def run_scale(args):
+    """Mixed batched path on the loudspeaker job (all 24 frequencies in one captured batch), checked against the
+    plain FP64 path at the lowest and highest frequency. Prints one JSON line."""
+    import torch
+
+    from beat_hip.engine import Engine
+
+    log("bench: building the 20k-triangle two-way loudspeaker job")
+    prep, qunit, obs, freqs = scale_job()
+    eng = Engine(prep, qunit)
+    nb = len(freqs)
+    b = eng.batch("mixed", nb, obs, 1.205, 343.0)
+    b.capture()
+    for _ in range(args.warmup):
+        b.run(freqs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = b.run(freqs)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    b.close()
+    value = len(freqs) * args.steps / t
+    log(f"bench: scale mixed {value:.2f} freq-points/s ({1e3 * t / args.steps:.0f} ms per 24-frequency job)")
+    out = {"metric": METRIC, "value": round(value, 4), "unit": "freq-points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * t / args.steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+           "config": {"workload": "two-way loudspeaker box ~20k tris, 3 sources, 24 log freqs 200 Hz-4 kHz, mixed "
+                                  "precision batched (hipGraph), 5,000-pt Fibonacci sphere at 2 m",
+                      "triangles": int(prep.n_faces), "p1_dofs": int(prep.n_vertices), "frequencies": nb,
+                      "batch": nb, "channels": int(qunit.shape[1]), "precision": "mixed"},
+           "max_refinement_iterations": int(r.iters.max()), "max_relative_residual": float(r.residual.max())}
+    from beat_hip import _lib
+
+    be = eng.batch("mixed", nb, obs, 1.205, 343.0)
+    be.run(freqs)
+    eng.dev.timers(True)
+    be.run(freqs)
+    out["stages_ms_per_job"] = {name: round(eng.dev.timer(slot)[0], 3) for slot, name in enumerate(_lib.TIMER_NAMES)}
+    eng.dev.timers(False)
+    be.close()
+    if not args.no_fp64:
+        b1 = eng.batch("fp64", 1, obs, 1.205, 343.0)
+        b1.run(freqs[:1])
+        fm, fr = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j in (0, nb - 1):
+            fr.append(b1.run(freqs[j : j + 1]).field[0])
+            fm.append(r.field[j])
+        torch.cuda.synchronize()
+        out["fp64_gpu_value"] = round(2 / (time.perf_counter() - t0), 4)
+        out["mixed_over_fp64"] = round(value / out["fp64_gpu_value"], 3)
+        b1.close()
+        out["max_field_error_db"], out["max_field_error_deg"] = field_errors(fm, fr)
+        out["fp64_check_freqs_hz"] = [round(float(freqs[0]), 2), round(float(freqs[-1]), 2)]
+        log(f"bench: scale field error vs FP64 {out['max_field_error_db']:.4f} dB / "
+            f"{out['max_field_error_deg']:.3f} deg")
+    eng.close()
+    print(json.dumps(out))
+
+
 def field_errors(fields_mixed, fields_ref):
     """Max far-field |dB| and phase error of mixed results against the FP64 path, over all frequencies."""
     db, ph = [], []
@@ -213,6 +292,9 @@ def main():
                     help="streams used by the batched LU and refinement (1 = single stream, e.g. for tracing)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the workload of the HBM-counter passes (copy calibration + one assembly)")
+    ap.add_argument("--workload", choices=["horn", "scale"], default="horn",
+                    help="horn: the 6k-triangle horn-in-box job (headline); scale: the 20k-triangle two-way "
+                         "loudspeaker job")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build runs on a single MI355X); got --gpus {args.gpus}",
@@ -221,6 +303,9 @@ def main():
 
     if args.traffic_probe:
         traffic_probe()
+        return
+    if args.workload == "scale":
+        run_scale(args)
         return
 
     import torch
