@@ -449,9 +449,9 @@ __global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2)
 // Skinny variant for the sub-panel updates (Nc <= 16 columns, K <= 64): the K x Nc block of B sits in LDS,
 // four threads share a row of C (each a quarter of K, contiguous A loads), partial sums meet by DPP quad
 // swaps, and each thread writes a quarter of the row. Plain FMA: the update is bound by reading A once.
-template <typename T>
+template <typename T, int NCMAX>
 __global__ __launch_bounds__(256) void k_gemm_skinny(MatBatch<T> mb, GemmShape g, const int* skip) {
-  constexpr int NCMAX = 16, KMAX = BEM_NBO;
+  constexpr int KMAX = BEM_NBO;
   const int mat = blockIdx.y;
   if (skip && skip[mat]) return;
   cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
@@ -1275,6 +1275,155 @@ __global__ __launch_bounds__(1024) void k_trsv_wg(MatBatch<T> mb, const cplx<dou
       for (int c = 0; c < NR; ++c) Y[(size_t)c * ldy + i] = ys[c][i];
 }
 
+// Blocked sweeps over many workgroups, one launch per 64-row block (for large N, where one workgroup per matrix
+// leaves most CUs idle and is bound by one CU's bandwidth). y = P b first (k_perm_rhs), then, for block k of
+// the forward sweep (unit L) or the backward sweep (U), every workgroup of the launch forms the block's
+// solution x_k = W_k y_k (W_k the inverted diagonal block; y_k is final, since the launches of the earlier
+// blocks have subtracted their parts) and subtracts A[i][block k] x_k from its tile of TSW_ROWS rows still to be
+// solved (after block k going forward, before it going backward). The launch's workgroups all read y_k, so
+// none may overwrite it: workgroup 0 of the next launch writes the previous block's solution, recomputed from
+// its y (which no later launch of the sweep changes), and the sweep's last launch, a single workgroup with
+// no rows left to update, writes its own block too.
+constexpr int TSW_ROWS = 256;
+// one workgroup per matrix solves when N <= TSW_MIN_N and at least TSW_MIN_BATCH matrices share the launch
+constexpr int TSW_MIN_N = 4096, TSW_MIN_BATCH = 8;
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_perm_rhs(const cplx<double>* b, long long ldb, long long strideB,
+                                                  cplx<T>* y, long long ldy, long long strideY, const int* perm,
+                                                  long long strideP, int N, const int* skip) {
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const int pi = perm[(size_t)mat * strideP + i];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    const cplx<double> v = b[(size_t)mat * strideB + (size_t)c * ldb + pi];
+    y[(size_t)mat * strideY + (size_t)c * ldy + i] = mk<T>(T(v.re), T(v.im));
+  }
+}
+
+// x = W_kb y_kb for the block's rows (rows past N read as zero) into xs; 4 threads per row of W, 16 columns each.
+template <typename T, int NR>
+__device__ __forceinline__ void sweep_block_solution(const cplx<T>* Wb, const cplx<T>* Y, long long ldy, int r0,
+                                                     int nr, cplx<T> (*sb)[64], cplx<T> (*xs)[64]) {
+  const int tid = threadIdx.x;
+  if (tid < 64)
+#pragma unroll
+    for (int c = 0; c < NR; ++c) sb[c][tid] = tid < nr ? Y[(size_t)c * ldy + r0 + tid] : mk<T>(0, 0);
+  __syncthreads();
+  const int row = tid >> 2, part = tid & 3;
+  T dr[NR], di[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) dr[c] = di[c] = T(0);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    cplx<T> w[4];
+    load4(Wb + row * 64 + 16 * part + 4 * u, w);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const cplx<T> v = sb[c][16 * part + 4 * u + q];
+        dr[c] += w[q].re * v.re - w[q].im * v.im;
+        di[c] += w[q].re * v.im + w[q].im * v.re;
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < NR; ++c)
+#pragma unroll
+    for (int m = 1; m < 4; m <<= 1) {
+      dr[c] += __shfl_xor(dr[c], m, 64);
+      di[c] += __shfl_xor(di[c], m, 64);
+    }
+  if (part == 0)
+#pragma unroll
+    for (int c = 0; c < NR; ++c) xs[c][row] = row < nr ? mk<T>(dr[c], di[c]) : mk<T>(0, 0);
+  __syncthreads();
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_trsv_step(MatBatch<T> mb, cplx<T>* y, long long ldy, long long strideY,
+                                                   const int* skip, const cplx<T>* W, long long strideW, int kb,
+                                                   int kb_prev, int lower) {
+  const int mat = blockIdx.y;
+  if (skip && skip[mat]) return;
+  const int N = mb.N, tid = threadIdx.x;
+  const int r0 = kb * 64, nr = min(64, N - r0);
+  const int half = lower ? 0 : 1;  // which inverted diagonal block: L (0) or U (1)
+  const cplx<T>* Wm = W + (size_t)mat * strideW;
+  cplx<T>* Y = y + (size_t)mat * strideY;
+  __shared__ cplx<T> sb[NR][64];
+  __shared__ cplx<T> xs[NR][64];
+  __shared__ cplx<T> xp[NR][64];
+  // rows this launch updates: [ua, ub)
+  const int ua = lower ? r0 + nr : 0, ub = lower ? N : r0;
+  if (blockIdx.x == 0 && kb_prev >= 0) {
+    // the previous block's solution, from its right-hand side that no workgroup of this launch reads
+    const int p0 = kb_prev * 64, np = min(64, N - p0);
+    sweep_block_solution<T, NR>(Wm + ((size_t)kb_prev * 2 + half) * 4096, Y, ldy, p0, np, sb, xp);
+    if (tid < np)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) Y[(size_t)c * ldy + p0 + tid] = xp[c][tid];
+    __syncthreads();  // sb is reused below
+  }
+  sweep_block_solution<T, NR>(Wm + ((size_t)kb * 2 + half) * 4096, Y, ldy, r0, nr, sb, xs);
+  if (ua >= ub) {
+    // last block of the sweep: nothing left to update, the single workgroup writes the block's solution
+    if (tid < nr)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) Y[(size_t)c * ldy + r0 + tid] = xs[c][tid];
+    return;
+  }
+  const int ta = ua + blockIdx.x * TSW_ROWS, tb = min(ub, ta + TSW_ROWS);
+  const int row = tid >> 4, part = tid & 15;
+  const bool full = nr == 64;  // a partial last block reads only its columns below N
+  for (int i0 = ta; i0 < tb; i0 += 64) {
+    cplx<T> a[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = min(i0 + row + 16 * u, tb - 1);
+      const cplx<T>* Ap = mb.A + (size_t)mat * mb.strideA + (size_t)i * mb.lda + r0 + 4 * part;
+      if (full) {
+        load4(Ap, a[u]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[u][q] = 4 * part + q < nr ? Ap[q] : mk<T>(0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      T accr[NR], acci[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) accr[c] = acci[c] = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const cplx<T> v = xs[c][4 * part + q];
+          accr[c] += a[u][q].re * v.re - a[u][q].im * v.im;
+          acci[c] += a[u][q].re * v.im + a[u][q].im * v.re;
+        }
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+          accr[c] += __shfl_xor(accr[c], m, 64);
+          acci[c] += __shfl_xor(acci[c], m, 64);
+        }
+      const int i = i0 + row + 16 * u;
+      if (part == 0 && i < tb)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          cplx<T>* yi = Y + (size_t)c * ldy + i;
+          const cplx<T> v = *yi;
+          *yi = mk<T>(v.re - accr[c], v.im - acci[c]);
+        }
+    }
+  }
+}
+
 // r = b - A0 x in FP64 (A0 in T, x and b in FP64); one wave per row, all RHS together.
 template <typename T>
 __global__ __launch_bounds__(256) void k_residual(MatBatch<T> mb, const cplx<double>* b, const cplx<double>* x,
@@ -1457,7 +1606,14 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
                               hipStream_t st, bool split = false, int max_wgs = 0) {
   if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
   if (g.Nc <= 16 && g.K <= BEM_NBO) {
-    hipLaunchKernelGGL(k_gemm_skinny<T>, dim3((g.M + 63) / 64, batch), dim3(256), 0, st, mb, g, skip);
+    // as many accumulator columns as the update has (the narrow panels of large N update 4 or 8 columns)
+    const dim3 grid((g.M + 63) / 64, batch);
+    if (g.Nc <= 4)
+      hipLaunchKernelGGL((k_gemm_skinny<T, 4>), grid, dim3(256), 0, st, mb, g, skip);
+    else if (g.Nc <= 8)
+      hipLaunchKernelGGL((k_gemm_skinny<T, 8>), grid, dim3(256), 0, st, mb, g, skip);
+    else
+      hipLaunchKernelGGL((k_gemm_skinny<T, 16>), grid, dim3(256), 0, st, mb, g, skip);
     return hipGetLastError();
   }
   if constexpr (sizeof(T) == 4) {
@@ -1660,23 +1816,47 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
   return hipSuccess;
 }
 
+// One workgroup per matrix walks the whole chain while the matrices fill the CUs (small N or large batches);
+// otherwise the blocked sweeps spread each block's update over TSW_ROWS-row tiles, one launch per block.
+template <typename T, int NR>
+static hipError_t solve_sweeps(const MatBatch<T>& mb, int batch, const int* perm, long long strideP,
+                               const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
+                               long long strideY, const int* skip, const cplx<T>* W, long long strideW,
+                               hipStream_t st) {
+  const int N = mb.N, nblk = (N + 63) / 64;
+  if (N <= TSW_MIN_N && batch >= TSW_MIN_BATCH) {
+    hipLaunchKernelGGL((k_trsv_wg<T, NR>), dim3(batch), dim3(1024), 0, st, mb, b, ldb, strideB, y, ldy, strideY, perm,
+                       strideP, skip, W, strideW);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((k_perm_rhs<T, NR>), dim3((N + 255) / 256, batch), dim3(256), 0, st, b, ldb, strideB, y, ldy,
+                     strideY, perm, strideP, N, skip);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int lower = pass == 0;
+    for (int s = 0; s < nblk; ++s) {
+      const int kb = lower ? s : nblk - 1 - s;
+      const int prev = s == 0 ? -1 : (lower ? kb - 1 : kb + 1);
+      const int nupd = lower ? max(0, N - (kb + 1) * 64) : kb * 64;
+      const int tiles = max(1, (nupd + TSW_ROWS - 1) / TSW_ROWS);
+      hipLaunchKernelGGL((k_trsv_step<T, NR>), dim3(tiles, batch), dim3(256), 0, st, mb, y, ldy, strideY, skip, W,
+                         strideW, kb, prev, lower);
+    }
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, long long strideP,
                           const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
                           long long strideY, int nrhs, const int* skip, const cplx<T>* W, hipStream_t st) {
   const long long strideW = lu_workspace_elems(mb.N);
-#define TRSV(NR_)                                                                                             \
-  hipLaunchKernelGGL((k_trsv_wg<T, NR_>), dim3(batch), dim3(1024), 0, st, mb, b, ldb, strideB, y, ldy, strideY, \
-                     perm, strideP, skip, W, strideW)
   switch (nrhs) {
-    case 1: TRSV(1); break;
-    case 2: TRSV(2); break;
-    case 3: TRSV(3); break;
-    case 4: TRSV(4); break;
+    case 1: return solve_sweeps<T, 1>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
+    case 2: return solve_sweeps<T, 2>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
+    case 3: return solve_sweeps<T, 3>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
+    case 4: return solve_sweeps<T, 4>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
     default: return hipErrorInvalidValue;
   }
-#undef TRSV
-  return hipGetLastError();
 }
 
 template <typename T>

@@ -137,11 +137,13 @@ def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precisi
 
 # sizes: one partial half block (40), one trailing block with a partial second half (100), partial trailing
 # blocks (190, 200) and the register-panel thresholds (1500, 2100), and the panels with row sets in LDS (3,500 to
-# 6,500 rows: three to seven row sets per thread)
+# 6,500 rows: three to seven row sets per thread); batches of 8 at N <= 4096 take the one-workgroup-per-matrix
+# solve, the others the blocked multi-workgroup sweeps
 @pytest.mark.parametrize("precision,n,batch", [("mixed", 40, 2), ("fp64", 100, 1), ("mixed", 100, 2),
                                                ("mixed", 190, 2), ("fp64", 200, 2), ("mixed", 200, 3),
                                                ("fp64", 1500, 1), ("mixed", 2100, 2), ("fp64", 3500, 1),
-                                               ("mixed", 4500, 1), ("mixed", 6500, 1), ("fp64", 6500, 1)])
+                                               ("mixed", 4500, 1), ("mixed", 6500, 1), ("fp64", 6500, 1),
+                                               ("fp64", 300, 8), ("mixed", 1100, 8)])
 def test_lu_random(gpu_device, precision, n, batch):
     import ctypes as C
 
