@@ -591,6 +591,9 @@ def run_job(request, engine_factory, emit, batch: int | None = None):
         freqs = job.frequencies.astype(np.float64)
         nb = int(batch or get_value(job.config, "frequency_batch", DEFAULT_FREQUENCY_BATCH))
         nb = max(1, min(nb, len(freqs)))
+        fit = getattr(eng, "fit_batch", None)  # device engines shrink the batch to the memory available
+        if fit is not None:
+            nb = fit(job.precision, nb, 0 if job.obs is None else len(job.obs))
         b = eng.batch(job.precision, nb, job.obs, job.rho, job.sound_speed)
         ref = eng.batch("fp64", nb, job.obs, job.rho, job.sound_speed) if (job.fp64_check and job.precision == "mixed") \
             else None

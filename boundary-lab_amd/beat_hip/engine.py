@@ -34,6 +34,22 @@ def _np_ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, batch: int, n_obs: int,
+                       precision: str = "mixed", n_ch: int = 1) -> int:
+    """Device bytes a job holds while resident (matrices dominate: 2 x N^2 per frequency in the batch)."""
+    cb = 8 if precision == "mixed" else 16
+    lda = ((n_vertices + 15) // 16) * 16
+    strip_len = ((n_vertices + 63) // 64) * 64  # strip rows hold one column per vertex
+    lu_work = batch * ((n_vertices + 63) // 64) * 2 * 4096 * cb  # inverted diagonal blocks
+    matrices = batch * 2 * n_vertices * lda * cb + lu_work
+    vectors = batch * 8 * n_ch * lda * 16
+    strip = min(batch, 4) * 3 * n_faces * strip_len * cb * (2 if precision == "mixed" else 1)  # row + column strips
+    singular = batch * n_pairs * 12 * 16
+    mesh = n_faces * (2 * 128 + 6 * 6 * 12 + 200) + n_pairs * 16
+    field_ws = batch * n_ch * 6 * n_faces * 2 * cb + batch * n_obs * n_ch * 16 * 64
+    return int(matrices + vectors + strip + singular + mesh + field_ws)
+
+
 class Device:
     """One HIP context on one GPU with its own (default-priority) stream."""
 
@@ -227,7 +243,12 @@ class Batch:
             if self.n_obs:
                 self.obs = torch.from_numpy(np.ascontiguousarray(obs, dtype=np.float64)).to(dev.device)
                 self.field = torch.zeros((nf, self.n_ch, self.n_obs), dtype=torch.complex128, **kw)
-        dmesh.reserve(precision, nf, min(strip_nf, nf), self.n_obs, self.n_ch)
+        # strip buffers (row and column strips of strip_nf frequencies) within the memory still free
+        free, _ = torch.cuda.mem_get_info(dev.device)
+        strip_len = ((N + 63) // 64) * 64
+        per_freq = 2 * 3 * F * strip_len * (8 if precision == "mixed" else 16)
+        strip_nf = max(1, min(strip_nf, nf, int(0.9 * free) // per_freq))
+        dmesh.reserve(precision, nf, strip_nf, self.n_obs, self.n_ch)
         self.graph = None
 
     # -- launch sequence ------------------------------------------------------------------------
@@ -369,6 +390,20 @@ class Engine:
 
     def batch(self, precision, nf, obs=None, rho=1.21, sound_speed=343.0, **kw) -> Batch:
         return Batch(self.dmesh, precision, nf, self.qunit, obs, rho, sound_speed, **kw)
+
+    def fit_batch(self, precision: str, nf: int, n_obs: int, share: float = 0.9) -> int:
+        """Largest frequency batch <= nf whose estimated device footprint fits in `share` of the memory still
+        available to this process (free device memory plus what torch's allocator holds unused)."""
+        torch = self.dev.torch
+        free, _ = torch.cuda.mem_get_info(self.dev.device)
+        avail = free + torch.cuda.memory_reserved(self.dev.device) - torch.cuda.memory_allocated(self.dev.device)
+        p = self.prep
+        n_ch = self.qunit.shape[1]
+        nf = max(1, int(nf))
+        while nf > 1 and estimate_footprint(p.n_vertices, p.n_faces, len(p.pairs.test), nf, n_obs, precision,
+                                            n_ch) > share * avail:
+            nf -= 1
+        return nf
 
     def solve(self, freqs, precision="mixed", batch=None, obs=None, rho=1.21, sound_speed=343.0, graph=False,
               **kw):

@@ -30,6 +30,7 @@ import numpy as np
 from . import contract as K
 from . import driver as D
 from . import geometry as G
+from .engine import estimate_footprint
 
 CAPABILITY_KEYS = (
     "supports_spherical_sampling",
@@ -112,22 +113,6 @@ def write_result_artifact(directory, init: dict, results: list[dict]) -> Path | 
     path = directory / "result.npz"
     np.savez_compressed(path, **bundle)
     return path
-
-
-def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, batch: int, n_obs: int,
-                       precision: str = "mixed", n_ch: int = 1) -> int:
-    """Device bytes a job holds while resident (matrices dominate: 2 x N^2 per frequency in the batch)."""
-    cb = 8 if precision == "mixed" else 16
-    lda = ((n_vertices + 15) // 16) * 16
-    strip_len = ((n_vertices + 63) // 64) * 64  # strip rows hold one column per vertex
-    lu_work = batch * ((n_vertices + 63) // 64) * 2 * 4096 * cb  # inverted diagonal blocks
-    matrices = batch * 2 * n_vertices * lda * cb + lu_work
-    vectors = batch * 8 * n_ch * lda * 16
-    strip = min(batch, 4) * 3 * n_faces * strip_len * cb * (2 if precision == "mixed" else 1)  # row + column strips
-    singular = batch * n_pairs * 12 * 16
-    mesh = n_faces * (2 * 128 + 6 * 6 * 12 + 200) + n_pairs * 16
-    field_ws = batch * n_ch * 6 * n_faces * 2 * cb + batch * n_obs * n_ch * 16 * 64
-    return int(matrices + vectors + strip + singular + mesh + field_ws)
 
 
 @dataclass

@@ -694,7 +694,7 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
                   int* ipiv, int* perm, long long strideP, int* info, void* work) {
   if (!c || !A || !ipiv || !perm || !info || !work) return fail(BEM_E_ARG, "NULL argument");
   if (n <= 0 || batch <= 0 || lda < n || strideA < lda * n || strideP < n) return fail(BEM_E_ARG, "invalid sizes");
-  if (n > 12288) return fail(BEM_E_ARG, "N above 12288 is not supported by the register panel");
+  if (n > BEM_LU_MAX_N) return fail(BEM_E_ARG, "N above 24576 is not supported by the register panel");
   if (precision != BEM_PREC_MIXED && precision != BEM_PREC_FP64) return fail(BEM_E_ARG, "unknown precision");
   ScopedTimer tm(c, BEM_TIMER_LU, (long long)batch);
   // One factorization of the whole batch with look-ahead: the context stream runs the panel chain, the first

@@ -1579,6 +1579,7 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
     else if (mr <= 3) PANEL(16, 3, 1);
     else if (mr <= 6) PANEL(8, 6, 2);
     else if (mr <= 12) PANEL(4, 12, 4);
+    else if (mr <= 24) PANEL(2, 24, 8);
     else return hipErrorInvalidValue;
   } else {
     if (mr <= 1) PANEL(8, 1);
@@ -1586,6 +1587,7 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
     else if (mr <= 3) PANEL(8, 3, 1);
     else if (mr <= 6) PANEL(4, 6, 2);
     else if (mr <= 12) PANEL(2, 12, 4);
+    else if (mr <= 24) PANEL(1, 24, 8);
     else return hipErrorInvalidValue;
   }
 #undef PANEL
@@ -1595,8 +1597,8 @@ static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb,
 template <typename T>
 int panel_width(int N, int c0) {
   const int mr = (N - c0 + 1023) / 1024;
-  if (sizeof(T) == 4) return mr <= 3 ? 16 : (mr <= 6 ? 8 : 4);
-  return mr <= 3 ? 8 : (mr <= 6 ? 4 : 2);
+  if (sizeof(T) == 4) return mr <= 3 ? 16 : (mr <= 6 ? 8 : (mr <= 12 ? 4 : 2));
+  return mr <= 3 ? 8 : (mr <= 6 ? 4 : (mr <= 12 ? 2 : 1));
 }
 
 // split: FP32 products as three bf16 products (k_gemm_bf16x3) for the large updates; max_wgs > 0 caps the

@@ -36,6 +36,7 @@ extern "C" {
 #define BEM_PREC_MIXED 1 /* FP32 regular assembly + FP64 singular, FP32 LU (FP32 MFMA), FP64 refinement */
 
 #define BEM_MAX_CHANNELS 4
+#define BEM_LU_MAX_N 24576 /* largest N bem_lu_factor accepts (24 row sets of 1024 per panel thread) */
 
 typedef struct bem_ctx bem_ctx;
 typedef struct bem_mesh bem_mesh;
@@ -171,7 +172,7 @@ int bem_lu_workspace_bytes(int precision, int n, int batch, long long* bytes);
 /* In-place LU with partial pivoting of `batch` N x N cT matrices (P A = L U, LAPACK getrf semantics;
  * ipiv[c] = row exchanged with row c, perm = resulting row permutation; info[m] = 0 or first zero pivot + 1;
  * int arrays of stride strideP). `work` (bem_lu_workspace_bytes) receives the inverted diagonal blocks and
- * travels with the factors. Replaces rocm_dense_lu! (JS/BeatEngineRocmOperators.jl:161-194). */
+ * travels with the factors. N <= BEM_LU_MAX_N. Replaces rocm_dense_lu! (JS/BeatEngineRocmOperators.jl:161-194). */
 int bem_lu_factor(bem_ctx* ctx, int precision, int n, int batch, void* A, long long lda, long long strideA,
                   int* ipiv, int* perm, long long strideP, int* info, void* work);
 

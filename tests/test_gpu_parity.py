@@ -138,12 +138,14 @@ def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precisi
 # sizes: one partial half block (40), one trailing block with a partial second half (100), partial trailing
 # blocks (190, 200) and the register-panel thresholds (1500, 2100), and the panels with row sets in LDS (3,500 to
 # 6,500 rows: three to seven row sets per thread); batches of 8 at N <= 4096 take the one-workgroup-per-matrix
-# solve, the others the blocked multi-workgroup sweeps
+# solve, the others the blocked multi-workgroup sweeps; above 12,288 rows the panels hold 24 row sets per thread
+# (checked by the backward error only: a CPU solve of that size takes minutes)
 @pytest.mark.parametrize("precision,n,batch", [("mixed", 40, 2), ("fp64", 100, 1), ("mixed", 100, 2),
                                                ("mixed", 190, 2), ("fp64", 200, 2), ("mixed", 200, 3),
                                                ("fp64", 1500, 1), ("mixed", 2100, 2), ("fp64", 3500, 1),
                                                ("mixed", 4500, 1), ("mixed", 6500, 1), ("fp64", 6500, 1),
-                                               ("fp64", 300, 8), ("mixed", 1100, 8)])
+                                               ("fp64", 300, 8), ("mixed", 1100, 8), ("fp64", 12400, 1),
+                                               ("mixed", 13000, 1)])
 def test_lu_random(gpu_device, precision, n, batch):
     import ctypes as C
 
@@ -183,9 +185,13 @@ def test_lu_random(gpu_device, precision, n, batch):
     # 1.1e-2 / 4.7e-5 at N = 6,500 (the forward error grows with the condition number).
     tol_fwd, tol_bwd = (1e-10, 1e-13) if precision == "fp64" else ((3e-3 if n <= 2100 else 5e-2), 1e-4)
     for m in range(batch):
-        ref = np.linalg.solve(M[m], B[m].T).T
-        err = np.linalg.norm(X[m] - ref) / np.linalg.norm(ref)
-        bwd = np.linalg.norm(B[m].T - M[m] @ X[m].T) / (np.linalg.norm(M[m], 2) * np.linalg.norm(X[m]))
+        err = 0.0
+        if n <= 8000:
+            ref = np.linalg.solve(M[m], B[m].T).T
+            err = np.linalg.norm(X[m] - ref) / np.linalg.norm(ref)
+        # for large n, |M|_2 of an iid complex Gaussian matrix is close to 2 |M|_F / sqrt(n) (no CPU SVD)
+        mnorm = np.linalg.norm(M[m], 2) if n <= 8000 else np.linalg.norm(M[m]) / np.sqrt(n) * 2
+        bwd = np.linalg.norm(B[m].T - M[m] @ X[m].T) / (mnorm * np.linalg.norm(X[m]))
         print(precision, n, m, "rel err", err, "backward", bwd)
         assert err < tol_fwd and bwd < tol_bwd
     assert int(info.cpu().numpy().max()) == 0
