@@ -141,6 +141,9 @@ def run_scale(args):
     log("bench: building the 20k-triangle two-way loudspeaker job")
     prep, qunit, obs, freqs = scale_job()
     eng = Engine(prep, qunit)
+    from beat_hip import _lib
+
+    _lib.check(eng.dev.lib.bem_ctx_set_fanout(eng.dev.ctx, args.fanout), "bem_ctx_set_fanout")
     nb = len(freqs)
     b = eng.batch("mixed", nb, obs, 1.205, 343.0)
     b.capture()
@@ -163,8 +166,6 @@ def run_scale(args):
                       "triangles": int(prep.n_faces), "p1_dofs": int(prep.n_vertices), "frequencies": nb,
                       "batch": nb, "channels": int(qunit.shape[1]), "precision": "mixed"},
            "max_refinement_iterations": int(r.iters.max()), "max_relative_residual": float(r.residual.max())}
-    from beat_hip import _lib
-
     be = eng.batch("mixed", nb, obs, 1.205, 343.0)
     be.run(freqs)
     eng.dev.timers(True)
