@@ -49,6 +49,13 @@ def test_scale_mixed_matches_fp64_and_is_linear(gpu_device, loudspeaker):
     assert mixed[0].residual.max() <= 1e-10
     assert db.max() < 0.05 and ph.max() < 0.5
 
+    # the job driver's batch sizing: a batch that cannot fit the device shrinks; a small one is kept
+    eng2 = Engine(prep, q3, dev=gpu_device)
+    big = eng2.fit_batch("mixed", 100000, len(obs))
+    assert 1 <= big < 100000 and eng2.fit_batch("mixed", 2, len(obs)) == 2
+    print("largest fitting mixed batch", big, flush=True)
+    eng2.close()
+
     eng1 = Engine(prep, q3.sum(axis=1, keepdims=True), dev=gpu_device)
     summed, _ = eng1.solve(FREQS, precision="mixed", obs=obs, rho=RHO, sound_speed=C0)
     eng1.close()
