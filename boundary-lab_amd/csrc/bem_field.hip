@@ -52,7 +52,9 @@ __global__ void k_field_sources(const int4* fv, const T* src_w, const cplx<doubl
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_field(FieldParams<T> p) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // w through readfirstlane: the wave's source range is then provably uniform and the source records and
+  // weights come in through scalar loads
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int o = blockIdx.x * 64 + lane;
   const int slice = blockIdx.y, f = blockIdx.z;
   const double kd = p.k[f];
