@@ -13,6 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(os.environ.get("BEMHIP_LIBRARY", Path(__file__).with_name("libbemhip.so")))
 
 BEM_OK = 0
+BEM_E_ARG, BEM_E_HIP, BEM_E_NOMEM, BEM_E_STATE, BEM_E_SINGULAR, BEM_E_ACCURACY = 1, 2, 3, 4, 5, 6
 BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_MAX_CHANNELS = 4
@@ -85,7 +86,12 @@ SIGNATURES = {
     "bem_ctx_set_lu_products": (_i, [_p, _i]),
     "bem_ctx_timers": (_i, [_p, _i]),
     "bem_ctx_timer_read": (_i, [_p, _i, C.POINTER(_d), C.POINTER(_ll), C.POINTER(_ll)]),
+    "bem_mesh_validate": (_i, [C.POINTER(MeshDesc)]),
     "bem_mesh_create": (_i, [_p, C.POINTER(MeshDesc), C.POINTER(_p)]),
+    "bem_mesh_live_graphs": (_i, [_p, C.POINTER(_i)]),
+    "bem_assemble_rhs": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _ll, _ll]),
+    "bem_neumann": (_i, [_p, _i, _i, _i, _p, _p, _p]),
+    "bem_solve_status": (_i, [_p, _i, _i, _p, _p, _ll, _ll, _i, _p, _d, _p, _d, _ip]),
     "bem_mesh_destroy": (_i, [_p]),
     "bem_mesh_reserve": (_i, [_p, _i, _i, _i, _i, _i]),
     "bem_mesh_device_bytes": (_i, [_p, C.POINTER(_ll)]),
@@ -106,7 +112,9 @@ SIGNATURES = {
 
 
 class BemError(RuntimeError):
-    pass
+    def __init__(self, message: str, status: int = 0):
+        super().__init__(message)
+        self.status = status
 
 
 _LIB = None
@@ -142,7 +150,7 @@ def last_error(lib=None) -> str:
 
 def check(status: int, what: str = "", lib=None):
     if status != BEM_OK:
-        raise BemError(f"{what} failed (status {status}): {last_error(lib)}")
+        raise BemError(f"{what} failed (status {status}): {last_error(lib)}", status)
 
 
 def lu_workspace_bytes(precision: int, n: int, batch: int) -> int:

@@ -573,11 +573,146 @@ def cancelled(job: Job) -> bool:
 # 0.26 / 1.33 s at 32).
 DEFAULT_FREQUENCY_BATCH = 32
 
-def run_job(request, engine_factory, emit, batch: int | None = None):
+
+class JobFailed(RuntimeError):
+    """A solve the engine could not stand behind (LU breakdown, non-finite values, a residual or pivot growth
+    above its limit after the FP64 retry). Reported as the job's `failed` event, as the reference reports a
+    singular factorization (julia_local/solver.jl:1033)."""
+
+
+def status_text(bits: int) -> str:
+    from .engine import STATUS_BREAKDOWN, STATUS_GROWTH, STATUS_NONFINITE, STATUS_RESIDUAL
+
+    parts = []
+    if bits & STATUS_BREAKDOWN:
+        parts.append("LU breakdown (zero pivot)")
+    if bits & STATUS_NONFINITE:
+        parts.append("non-finite solution")
+    if bits & STATUS_RESIDUAL:
+        parts.append("relative residual above the tolerance")
+    if bits & STATUS_GROWTH:
+        parts.append("pivot growth above the limit")
+    return ", ".join(parts) or "ok"
+
+
+def plan_batch(job: Job, eng, batch: int | None = None) -> int:
+    """Frequencies per device batch: the request's `frequency_batch` (default DEFAULT_FREQUENCY_BATCH), at most the
+    job's frequency count, shrunk until every batch the job holds at once fits the device memory (the FP64 check
+    batch of `fp64_check` included)."""
+    n = len(job.frequencies)
+    nb = int(batch or get_value(job.config, "frequency_batch", DEFAULT_FREQUENCY_BATCH))
+    nb = max(1, min(nb, n))
+    fit = getattr(eng, "fit_batch", None)  # device engines shrink the batch to the memory available
+    if fit is not None:
+        companions = ("fp64",) if (job.fp64_check and job.precision == "mixed") else ()
+        nb = fit(job.precision, nb, 0 if job.obs is None else len(job.obs), companions=companions)
+    return nb
+
+
+@dataclass
+class RawResult:
+    """One solved frequency before post-processing: boundary pressure (V, C), field (n_obs, C), the batch result
+    and index its diagnostics come from, extra diagnostics, per-frequency timings."""
+    index: int
+    freq: float
+    pressure: np.ndarray
+    field: np.ndarray
+    res: object
+    j: int
+    extra: dict
+    timings: dict
+
+
+def solve_raw(job: Job, eng, nb: int, freqs=None, graph: bool = True):
+    """Generator over RawResult of every frequency (the job's, or `freqs`) in order (solver.jl:805-997).
+
+    Frequencies run in device batches of nb. A job of several batches captures the batch's launch sequence as
+    HIP graphs after its first (eager) batch and replays them for the rest. Each batch's solves are checked on
+    the device (bem_solve_status): a mixed-precision frequency whose LU broke down, whose solution is not finite,
+    whose refinement missed its tolerance or whose pivot growth is above the limit is solved again on the plain
+    FP64 path (one bounded retry, as the reference retries a corrupt rocSOLVER result,
+    JS/BeatEngineRocmOperators.jl:123-130); if that fails too, or an FP64 job fails its checks, JobFailed is
+    raised. Closing the generator (e.g. on cancellation) releases the batches."""
+    freqs = job.frequencies.astype(np.float64) if freqs is None else np.asarray(freqs, dtype=np.float64)
+    n_obs = 0 if job.obs is None else len(job.obs)
+    b = eng.batch(job.precision, nb, job.obs, job.rho, job.sound_speed)
+    ref = eng.batch("fp64", nb, job.obs, job.rho, job.sound_speed) if (job.fp64_check and job.precision == "mixed") \
+        else None
+    retry = None
+    try:
+        for bi, i0 in enumerate(range(0, len(freqs), nb)):
+            if graph and bi == 1 and getattr(b, "capture", None) is not None:
+                b.capture(warm=False)
+            fr = freqs[i0 : i0 + nb]
+            res = b.run(fr)
+            rref = ref.run(fr) if ref is not None else None
+            n = len(fr)
+            flags = getattr(res, "flags", None)
+            for j in range(n):
+                bits = 0 if flags is None else int(flags[j])
+                p_j, f_j, r_j, jj, extra = res.pressure[j], res.field[j], res, j, {}
+                if bits:
+                    freq = float(np.float32(fr[j]))
+                    if job.precision != "mixed":
+                        raise JobFailed(f"Dense solve failed at {freq:g} Hz: {status_text(bits)}.")
+                    if retry is None:
+                        retry = _retry_batch(eng, b, job, n_obs)
+                    rr = retry.run(fr[j : j + 1])
+                    rbits = 0 if getattr(rr, "flags", None) is None else int(rr.flags[0])
+                    if rbits:
+                        raise JobFailed(f"Dense solve failed at {freq:g} Hz: {status_text(bits)} in the mixed "
+                                        f"solve, {status_text(rbits)} in the FP64 retry.")
+                    p_j, f_j, r_j, jj = rr.pressure[0], rr.field[0], rr, 0
+                    extra = {"fp64_retry": True, "fp64_retry_reason": status_text(bits), "precision": "fp64"}
+                if rref is not None:
+                    extra.update(fp64_errors(p_j, f_j, rref.pressure[j], rref.field[j]))
+                timings = {"assembly_s": res.t_assembly / n, "solve_s": res.t_solve / n, "field_s": res.t_field / n}
+                yield RawResult(i0 + j, float(fr[j]), p_j, f_j, r_j, jj, extra, timings)
+    finally:
+        for x in (b, ref, retry):
+            if x is not None:
+                x.close()
+
+
+def payload_for(job: Job, raw: RawResult, freq: float, extra_diagnostics=None) -> dict:
+    """The `result` payload of one solved frequency for `job` (solver.jl:921-996)."""
+    extra = dict(raw.extra)
+    if extra_diagnostics:
+        extra.update(extra_diagnostics)
+    return result_payload(job, freq, raw.pressure, raw.field, raw.timings, diagnostics_for(job, raw.res, raw.j, extra))
+
+
+def solve_results(job: Job, eng, nb: int, graph: bool = True, extra_diagnostics=None):
+    """Generator over (index, payload) of every frequency of the job in request order (see solve_raw)."""
+    gen = solve_raw(job, eng, nb, graph=graph)
+    try:
+        for raw in gen:
+            yield raw.index, payload_for(job, raw, float(job.frequencies[raw.index]), extra_diagnostics)
+    finally:
+        gen.close()
+
+
+def _retry_batch(eng, b, job: Job, n_obs: int):
+    """FP64 batch of one frequency for retries. Its workspace may need to grow the mesh's, which captured graphs
+    prevent: the main batch then drops its graphs and continues eagerly."""
+    from . import _lib
+
+    try:
+        return eng.batch("fp64", 1, job.obs, job.rho, job.sound_speed)
+    except _lib.BemError as exc:
+        if exc.status != _lib.BEM_E_STATE:
+            raise
+        b.close()
+        return eng.batch("fp64", 1, job.obs, job.rho, job.sound_speed)
+
+
+def run_job(request, engine_factory, emit, batch: int | None = None, graph: bool = True):
     """Drive one request to completion, emitting event dicts through ``emit`` (solver.jl:680-1020).
 
-    engine_factory(prepared_mesh, qunit) -> beat_hip.engine.Engine
-    """
+    engine_factory(prepared_mesh, qunit) -> beat_hip.engine.Engine. Cancellation (the request's cancel_path) is
+    checked before every frequency's result, as the reference checks it before every frequency
+    (solver.jl:805-809); device work already enqueued for the current batch completes first. A failed solve
+    ends the job with a `failed` event (solver.jl:1033)."""
     init, parsed = parse_request(request)
     emit({"type": "initialized", **init})
     emit({"type": "status", "message": f"BEAT Engine loading {len(parsed['mesh_inputs'])} mesh"
@@ -586,41 +721,30 @@ def run_job(request, engine_factory, emit, batch: int | None = None):
     prep = G.prepare(job.mesh, symmetry=job.symmetry)
     emit({"type": "status", "message": f"Initializing BEAT Engine using native HIP assembly and GPU solve "
           f"({job.precision}, {job.mesh.n_faces} triangles, {job.mesh.n_vertices} P1 dofs)..."})
+    if cancelled(job):
+        emit({"type": "cancelled", "solved_count": 0})
+        return
     eng = engine_factory(prep, job.qunit)
+    total = len(job.frequencies)
+    solved = 0
     try:
-        freqs = job.frequencies.astype(np.float64)
-        nb = int(batch or get_value(job.config, "frequency_batch", DEFAULT_FREQUENCY_BATCH))
-        nb = max(1, min(nb, len(freqs)))
-        fit = getattr(eng, "fit_batch", None)  # device engines shrink the batch to the memory available
-        if fit is not None:
-            nb = fit(job.precision, nb, 0 if job.obs is None else len(job.obs))
-        b = eng.batch(job.precision, nb, job.obs, job.rho, job.sound_speed)
-        ref = eng.batch("fp64", nb, job.obs, job.rho, job.sound_speed) if (job.fp64_check and job.precision == "mixed") \
-            else None
-        solved = 0
-        for i0 in range(0, len(freqs), nb):
-            if cancelled(job):
-                emit({"type": "cancelled", "solved_count": solved})
-                return
-            fr = freqs[i0 : i0 + nb]
-            res = b.run(fr)
-            rref = ref.run(fr) if ref is not None else None
-            n = len(fr)
-            for j in range(n):
-                timings = {"assembly_s": res.t_assembly / n, "solve_s": res.t_solve / n, "field_s": res.t_field / n}
-                extra = None
-                if rref is not None:
-                    extra = fp64_errors(res.pressure[j], res.field[j], rref.pressure[j], rref.field[j])
-                payload = result_payload(job, float(job.frequencies[i0 + j]), res.pressure[j], res.field[j], timings,
-                                         diagnostics_for(job, res, j, extra))
-                solved = i0 + j + 1
-                emit({"type": "result", "solved_count": solved, "total_count": len(freqs), "result": payload})
-        b.close()
-        if ref is not None:
-            ref.close()
+        nb = plan_batch(job, eng, batch)
+        gen = solve_results(job, eng, nb, graph=graph)
+        try:
+            for i, payload in gen:
+                if cancelled(job):
+                    emit({"type": "cancelled", "solved_count": solved})
+                    return
+                solved = i + 1
+                emit({"type": "result", "solved_count": solved, "total_count": total, "result": payload})
+        except JobFailed as exc:
+            emit({"type": "failed", "error": str(exc)})
+            return
+        finally:
+            gen.close()
     finally:
         eng.close()
-    emit({"type": "completed", "solved_count": len(job.frequencies)})
+    emit({"type": "completed", "solved_count": total})
 
 
 def now():

@@ -507,19 +507,26 @@ def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
     multi = np.nonzero((first >= 0) & (last > first))[0]
     keep = np.zeros(V, dtype=bool)
     active = []  # max-heap by last chunk of the kept intervals that may still cover the scan point
+    # every kept interval starts at or before the scan point (the scan runs by start), so the kept intervals
+    # covering it are those that end at or after it: `live` counts them, `ends[c]` the kept ones ending at c
+    ends = np.zeros(n_chunks + 1, dtype=np.int64)
+    live, scan = 0, 0
     for v in multi[np.lexsort((last[multi], first[multi]))]:
+        fv, lv = int(first[v]), int(last[v])
+        while scan < fv:
+            live -= int(ends[scan])
+            scan += 1
         keep[v] = True
-        heapq.heappush(active, (-int(last[v]), int(v)))
-        # kept intervals covering first[v]; over capacity: drop the one ending last
-        covering = [u for _, u in active if keep[u] and last[u] >= first[v]]
-        if len(covering) > pend_slots:
-            while active and (not keep[active[0][1]] or last[active[0][1]] < first[v]):
+        heapq.heappush(active, (-lv, int(v)))
+        live += 1
+        ends[lv] += 1
+        if live > pend_slots:  # over capacity: drop the covering interval that ends last
+            while not keep[active[0][1]] or last[active[0][1]] < fv:
                 heapq.heappop(active)
             _, u = heapq.heappop(active)
             keep[u] = False
-        if len(active) > 4 * pend_slots:  # prune expired entries
-            active = [(e, u) for e, u in active if keep[u] and last[u] >= first[v]]
-            heapq.heapify(active)
+            live -= 1
+            ends[last[u]] -= 1
     pend = np.full(V, -1, dtype=np.int64)
     slot_prev = np.full((n_chunks, 192), -1, dtype=np.int32)
     last_seen = np.full(V, -1, dtype=np.int64)

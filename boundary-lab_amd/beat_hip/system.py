@@ -24,7 +24,7 @@ import time
 import numpy as np
 
 from . import geometry as G
-from . import _lib
+from . import driver as D
 from .driver import DEFAULT_FREQUENCY_BATCH
 
 SCHEMA_VERSION = 1
@@ -345,31 +345,33 @@ def solve_system_request(request, engine_factory, emit, batch: int = DEFAULT_FRE
     obs = np.concatenate(job["point_sets"], axis=0) if job["point_sets"] else None
     freqs = job["frequencies"]
     nb = max(1, min(int(batch), len(freqs))) if freqs else 1
-    # channels in groups of at most BEM_MAX_CHANNELS excitations, one engine each
-    groups = [list(range(i, min(i + _lib.BEM_MAX_CHANNELS, len(exc))))
-              for i in range(0, len(exc), _lib.BEM_MAX_CHANNELS)]
-    engines = [engine_factory(prep, qunit[:, g]) for g in groups]
+    # one engine for every excitation: the system is assembled and factored once per frequency, the
+    # right-hand sides, solves and field run per group of at most four excitations on the same factors
+    eng = engine_factory(prep, qunit)
+    fit = getattr(eng, "fit_batch", None)
+    if fit is not None:
+        nb = fit(job["engine_precision"], nb, 0 if obs is None else len(obs))
     solved = 0
+    b = None
     try:
-        batches = [e.batch(job["engine_precision"], nb, obs, job["density"], job["sound_speed"]) for e in engines]
+        b = eng.batch(job["engine_precision"], nb, obs, job["density"], job["sound_speed"])
         for i0 in range(0, len(freqs), nb):
             if _cancelled(job):
                 return True, solved
             fr = np.asarray(freqs[i0 : i0 + nb], dtype=np.float64)
-            results = [b.run(fr) for b in batches]
+            r = b.run(fr)
+            flags = getattr(r, "flags", None)
             n = len(fr)
             for j in range(n):
-                pressure = np.concatenate([r.pressure[j] for r in results], axis=1)
-                field = None if obs is None else np.concatenate([r.field[j] for r in results], axis=1)
-                timings = {"assembly_s": sum(r.t_assembly for r in results) / n,
-                           "solve_s": sum(r.t_solve for r in results) / n,
-                           "field_s": sum(r.t_field for r in results) / n}
-                res = frequency_result(job, freqs[i0 + j], pressure, field, timings, first=(i0 + j == 0))
+                if flags is not None and int(flags[j]):
+                    raise RuntimeError(f"Dense solve failed at {fr[j]:g} Hz: {D.status_text(int(flags[j]))}.")
+                timings = {"assembly_s": r.t_assembly / n, "solve_s": r.t_solve / n, "field_s": r.t_field / n}
+                res = frequency_result(job, freqs[i0 + j], r.pressure[j], None if obs is None else r.field[j],
+                                       timings, first=(i0 + j == 0))
                 emit({"type": "result", "result": res} if event_mode else res)
                 solved = i0 + j + 1
-        for b in batches:
-            b.close()
     finally:
-        for e in engines:
-            e.close()
+        if b is not None:
+            b.close()
+        eng.close()
     return _cancelled(job), solved

@@ -1079,7 +1079,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gather_rhs(GatherParams<T> g) {
   const int i = blockIdx.x * 256 + threadIdx.x, f = blockIdx.y;
   if (i >= g.N) return;
-  g.rowmax[(size_t)f * g.N + i] = 0.0;
+  if (g.rowmax) g.rowmax[(size_t)f * g.N + i] = 0.0;
   const double wi = g.roww ? g.roww[i] : 1.0;
   const cplx<double>* rp = g.rhs_part + (size_t)f * 3 * g.F * MAX_CH;
   for (int c = 0; c < g.n_ch; ++c) {
@@ -1143,6 +1143,15 @@ hipError_t launch_gather(const GatherParams<T>& g, int nf, hipStream_t st) {
   return hipGetLastError();
 }
 
+// b only (the RHS of a further channel group; A, A0 and rowmax untouched when g.rowmax is null)
+template <typename T>
+hipError_t launch_gather_rhs(const GatherParams<T>& g, int nf, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_rhs<T>, dim3((g.N + 255) / 256, nf), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+template hipError_t launch_gather_rhs<float>(const GatherParams<float>&, int, hipStream_t);
+template hipError_t launch_gather_rhs<double>(const GatherParams<double>&, int, hipStream_t);
 template hipError_t launch_regular<float>(const RegParams<float>&, int, hipStream_t);
 template hipError_t launch_regular<double>(const RegParams<double>&, int, hipStream_t);
 template hipError_t launch_rhs<float>(const RhsParams<float>&, int, hipStream_t);

@@ -49,6 +49,10 @@ struct bem_ctx {
   hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
   int lu_products;      // BEM_LU_FP32 or BEM_LU_SPLIT_BF16 for the large updates of FP32 factorizations
   int num_cus;          // compute units of the device
+  std::vector<bem_mesh*> capture_meshes;  // meshes used by the sequence being captured
+  // bem_solve_status: device flags and their host copy
+  int* status_dev = nullptr;
+  int status_cap = 0;
 };
 
 // Run body(first, count, stream) on up to 1 + BEM_AUX_STREAMS contiguous sub-batches, forked from and joined
@@ -129,11 +133,26 @@ struct bem_mesh {
   // workspace
   DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial;
   int strip_nf = 0;
+  // captured graphs whose nodes address this mesh's buffers; while any exists the workspace may not move
+  int live_graphs = 0;
+  bool closed = false;  // bem_mesh_destroy was called while graphs were live
+  // the singular RHS blocks in sing_rhs belong to these wavenumbers (bem_assemble_rhs reuses them)
+  const double* sing_k = nullptr;
+  int sing_nf = 0;
+};
+
+// Executable graph handle: the instantiated graph and the meshes its nodes use.
+struct bem_graph {
+  hipGraphExec_t exec;
+  std::vector<bem_mesh*> meshes;
 };
 
 static int dev_alloc(bem_mesh* m, DevBuf& b, size_t bytes) {
   if (b.bytes >= bytes && b.p) return BEM_OK;
   if (m->ctx->capturing) return fail(BEM_E_STATE, "device allocation requested during graph capture; call bem_mesh_reserve first");
+  if (m->live_graphs > 0)
+    return fail(BEM_E_STATE, "the mesh workspace must grow, but a captured graph still uses it; destroy the graph "
+                             "first or reserve the largest shape before capturing");
   if (b.p) {
     (void)hipFree(b.p);
     b.p = nullptr;
@@ -238,6 +257,7 @@ int bem_ctx_destroy(bem_ctx* c) {
   (void)hipEventDestroy(c->fork_ev);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
+  if (c->status_dev) (void)hipFree(c->status_dev);
   delete c;
   return BEM_OK;
 }
@@ -289,45 +309,132 @@ int bem_ctx_stream(bem_ctx* c, void** s) {
   return BEM_OK;
 }
 
-int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
-  if (!c || !d || !out) return fail(BEM_E_ARG, "NULL argument");
-  if (d->n_faces <= 0 || d->n_vertices <= 0) return fail(BEM_E_ARG, "empty mesh");
-  if (d->n_vertices > (1 << 24)) return fail(BEM_E_ARG, "more than 2^24 vertices");
-  if (d->n_chunks != (d->n_faces + 63) / 64) return fail(BEM_E_ARG, "n_chunks must be ceil(F/64)");
+}  // extern "C"
+
+// Every index table of a mesh description is checked here, on the host, before anything is uploaded: the
+// kernels index device memory with these values and do no bounds checks of their own. Returns an empty string
+// when the description is consistent, else what is wrong.
+static std::string desc_problem(const bem_mesh_desc* d) {
+  const long long F = d->n_faces, V = d->n_vertices;
+  if (F <= 0 || V <= 0) return "empty mesh";
+  if (F > (1 << 24) || V > (1 << 24)) return "more than 2^24 faces or vertices";
+  if (!d->faces || !d->face_vertices || !d->normals || !d->curls || !d->area2 || !d->qpoints)
+    return "NULL geometry array";
+  if (!d->order || !d->slot_count || !d->slot_start || !d->slot_inc || !d->slot_vertex || !d->slot_pend ||
+      !d->slot_prev || !d->elem_pos || !d->chunk_start || !d->vertex_last || !d->vstar_ptr || !d->vstar_row)
+    return "NULL slot table";
+  if (d->n_chunks != (d->n_faces + 63) / 64) return "n_chunks must be ceil(F/64)";
   const int R = d->n_transforms <= 0 ? 1 : d->n_transforms;
-  if (R != 1 && R != 2 && R != 4) return fail(BEM_E_ARG, "n_transforms must be 1, 2 or 4");
+  if (R != 1 && R != 2 && R != 4) return "n_transforms must be 1, 2 or 4";
   for (int i = 0; i < 3 * R; ++i)
     if (d->n_transforms > 0 && d->transform_signs[i] != 1 && d->transform_signs[i] != -1)
-      return fail(BEM_E_ARG, "transform signs must be +1 or -1");
-  for (int i = 0; i < d->n_pairs; ++i)
-    if (d->sing_trial[i] < 0 || d->sing_trial[i] >= R * d->n_faces) return fail(BEM_E_ARG, "singular trial key out of range");
-  if (!d->slot_count || !d->slot_vertex || !d->slot_pend || !d->slot_prev || !d->elem_pos || !d->chunk_start ||
-      !d->vertex_last)
-    return fail(BEM_E_ARG, "NULL slot table");
-  for (int t = 0; t < d->n_faces; ++t) {
-    if (d->elem_pos[t] < 0 || d->elem_pos[t] >= d->n_faces || d->order[d->elem_pos[t]] != t)
-      return fail(BEM_E_ARG, "elem_pos does not invert order");
-    if (d->chunk_start[t] < 0 || d->chunk_start[t] > d->elem_pos[t] / 64) return fail(BEM_E_ARG, "chunk_start out of range");
+      return "transform signs must be +1 or -1";
+  if (d->n_transforms > 0)
+    for (int i = 0; i < 3; ++i)
+      if (d->transform_signs[i] != 1) return "transform 0 must be the identity";
+  for (long long i = 0; i < 3 * F; ++i)
+    if (d->faces[i] < 0 || d->faces[i] >= V) return "face vertex id out of range";
+  for (long long t = 0; t < F; ++t) {
+    const int* f = d->faces + 3 * t;
+    if (f[0] == f[1] || f[0] == f[2] || f[1] == f[2]) return "face with a repeated vertex";
+    if (!(d->area2[t] > 0.0)) return "face with non-positive area";
   }
+  // trial order: a permutation of the faces followed by -1 padding, inverted by elem_pos
+  const long long nord = 64LL * d->n_chunks;
+  for (long long i = 0; i < nord; ++i) {
+    const int e = d->order[i];
+    if (i < F ? (e < 0 || e >= F) : e != -1) return "order must be a permutation of the faces padded with -1";
+  }
+  for (long long t = 0; t < F; ++t) {
+    if (d->elem_pos[t] < 0 || d->elem_pos[t] >= F || d->order[d->elem_pos[t]] != t) return "elem_pos does not invert order";
+    if (d->chunk_start[t] < 0 || d->chunk_start[t] > d->elem_pos[t] / 64) return "chunk_start out of range";
+  }
+  for (long long v = 0; v < V; ++v)
+    if (d->vertex_last[v] < -1 || d->vertex_last[v] >= d->n_chunks) return "vertex_last out of range";
   for (int ch = 0; ch < d->n_chunks; ++ch) {
-    if (d->slot_count[ch] < 0 || d->slot_count[ch] > BEM_SLOT_INC) return fail(BEM_E_ARG, "slot count out of range");
+    if (d->slot_count[ch] < 0 || d->slot_count[ch] > BEM_SLOT_INC) return "slot count out of range";
     // incidence ranges non-decreasing within the chunk's 192 entries; codes (lane << 2 | corner) of 64 lanes
     const uint16_t* st = d->slot_start + (size_t)ch * BEM_SLOT_STRIDE;
+    if (st[0] != 0) return "slot incidence ranges must start at 0";
     for (int k = 0; k < d->slot_count[ch]; ++k)
-      if (st[k] > st[k + 1] || st[k + 1] > BEM_SLOT_INC) return fail(BEM_E_ARG, "slot incidence range out of order");
+      if (st[k] > st[k + 1] || st[k + 1] > BEM_SLOT_INC) return "slot incidence range out of order";
     for (int e = 0; e < st[d->slot_count[ch]]; ++e) {
       const int code = d->slot_inc[(size_t)ch * BEM_SLOT_INC + e];
-      if ((code >> 2) >= 64 || (code & 3) == 3) return fail(BEM_E_ARG, "slot incidence code out of range");
+      if ((code >> 2) >= 64 || (code & 3) == 3) return "slot incidence code out of range";
     }
     for (int k = 0; k < d->slot_count[ch]; ++k) {
       const int v = d->slot_vertex[(size_t)ch * BEM_SLOT_INC + k] >> 1;
       const int pd = d->slot_pend[(size_t)ch * BEM_SLOT_INC + k] & 0xFFFF;
-      if (v < 0 || v >= d->n_vertices) return fail(BEM_E_ARG, "slot vertex out of range");
-      if (pd != BEM_PEND_NONE && pd >= BEM_PEND_SLOTS) return fail(BEM_E_ARG, "LDS partial-sum slot out of range");
+      if (v < 0 || v >= V) return "slot vertex out of range";
+      if (pd != BEM_PEND_NONE && pd >= BEM_PEND_SLOTS) return "LDS partial-sum slot out of range";
       const int pv = d->slot_prev[(size_t)ch * BEM_SLOT_INC + k];
-      if (pv < -1 || pv >= ch) return fail(BEM_E_ARG, "slot_prev out of range");
+      if (pv < -1 || pv >= ch) return "slot_prev out of range";
     }
   }
+  // vertex -> strip-row CSR: every strip row 3t+l once, under the vertex of corner l of face t
+  if (d->vstar_ptr[0] != 0 || d->vstar_ptr[V] != 3 * F) return "vstar_ptr must run from 0 to 3F";
+  for (long long v = 0; v < V; ++v) {
+    if (d->vstar_ptr[v + 1] < d->vstar_ptr[v]) return "vstar_ptr not monotone";
+    for (int q = d->vstar_ptr[v]; q < d->vstar_ptr[v + 1]; ++q) {
+      const int r = d->vstar_row[q];
+      if (r < 0 || r >= 3 * F || d->faces[r] != v) return "vstar_row entry out of range or not at its vertex";
+    }
+  }
+  // singular pairs grouped by test element, trial keys s + r F, Duffy rules within the rule table
+  const long long P = d->n_pairs;
+  if (P < 0 || d->n_rules < 0) return "negative pair or rule count";
+  if (P > 0 && (!d->sing_test || !d->sing_trial || !d->sing_rule)) return "NULL singular pair table";
+  if (!d->sing_ptr || d->sing_ptr[0] != 0 || d->sing_ptr[F] != P) return "sing_ptr must run from 0 to n_pairs";
+  for (long long t = 0; t < F; ++t) {
+    if (d->sing_ptr[t + 1] < d->sing_ptr[t]) return "sing_ptr not monotone";
+    for (int q = d->sing_ptr[t]; q < d->sing_ptr[t + 1]; ++q)
+      if (d->sing_test[q] != t) return "singular pairs must be grouped by test element";
+  }
+  for (long long i = 0; i < P; ++i) {
+    if (d->sing_trial[i] < 0 || d->sing_trial[i] >= R * F) return "singular trial key out of range";
+    if (d->sing_rule[i] < 0 || d->sing_rule[i] >= d->n_rules) return "singular rule index out of range";
+  }
+  if (d->n_rules > 0 && (!d->rule_off || !d->rule_len || !d->rule_pts)) return "NULL rule table";
+  for (int r = 0; r < d->n_rules; ++r)
+    if (d->rule_off[r] < 0 || d->rule_len[r] <= 0 || (long long)d->rule_off[r] + d->rule_len[r] > (1LL << 26))
+      return "rule offset or length out of range";
+  if (d->n_src < 0 || (d->n_src > 0 && !d->src_faces)) return "invalid radiating-face list";
+  for (int i = 0; i < d->n_src; ++i)
+    if (d->src_faces[i] < 0 || d->src_faces[i] >= F) return "radiating face out of range";
+  return std::string();
+}
+
+extern "C" {
+
+int bem_mesh_validate(const bem_mesh_desc* d) {
+  if (!d) return fail(BEM_E_ARG, "NULL argument");
+  try {
+    const std::string why = desc_problem(d);
+    if (!why.empty()) return fail(BEM_E_ARG, "invalid mesh description: " + why);
+  } catch (const std::exception& ex) {
+    return fail(BEM_E_NOMEM, std::string("mesh validation: ") + ex.what());
+  }
+  return BEM_OK;
+}
+
+}  // extern "C"
+
+static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out);
+
+extern "C" int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
+  if (!c || !d || !out) return fail(BEM_E_ARG, "NULL argument");
+  *out = nullptr;
+  int rc = bem_mesh_validate(d);
+  if (rc) return rc;
+  try {
+    return mesh_create_impl(c, d, out);
+  } catch (const std::exception& ex) {
+    return fail(BEM_E_NOMEM, std::string("bem_mesh_create: ") + ex.what());
+  }
+}
+
+static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
+  const int R = d->n_transforms <= 0 ? 1 : d->n_transforms;
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   bem_mesh* m = new bem_mesh();
   m->ctx = c;
@@ -490,13 +597,31 @@ int bem_mesh_create(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) {
   return BEM_OK;
 }
 
-int bem_mesh_destroy(bem_mesh* m) {
-  if (!m) return BEM_OK;
+static void mesh_free(bem_mesh* m) {
+  (void)hipSetDevice(m->ctx->device);
   for (auto& b : m->owned)
     if (b.p) (void)hipFree(b.p);
   for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial})
     if (b->p) (void)hipFree(b->p);
   delete m;
+}
+
+// A mesh used by a captured graph is freed only when the last such graph is destroyed: the graph's kernel nodes
+// hold its table and workspace addresses.
+static void note_capture_use(bem_ctx* c, bem_mesh* m) {
+  if (c->capturing && std::find(c->capture_meshes.begin(), c->capture_meshes.end(), m) == c->capture_meshes.end())
+    c->capture_meshes.push_back(m);
+}
+
+extern "C" {
+
+int bem_mesh_destroy(bem_mesh* m) {
+  if (!m) return BEM_OK;
+  if (m->live_graphs > 0) {
+    m->closed = true;  // freed by bem_graph_destroy of the last graph that uses it
+    return BEM_OK;
+  }
+  mesh_free(m);
   return BEM_OK;
 }
 
@@ -549,6 +674,9 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
                          long long ldb, long long strideB, double* rowmax, int precision) {
   int rc;
   if ((rc = reserve_impl(m, precision, nf, m->strip_nf > 0 ? m->strip_nf : std::min(nf, 4), 0, n_ch))) return rc;
+  note_capture_use(c, m);
+  m->sing_k = k;
+  m->sing_nf = nf;
   hipStream_t st = c->stream;
   // singular blocks for all frequencies
   SingParams sp;
@@ -680,6 +808,109 @@ int bem_assemble(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k
     return assemble_impl<double>(c, m, nf, k, qunit, n_ch, rhs_scale, A, lda, strideA, A0, b, ldb, strideB,
                                  rowmax, precision);
   return fail(BEM_E_ARG, "unknown precision");
+}
+
+int bem_assemble_rhs(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, const void* qunit, int n_ch,
+                     const void* rhs_scale, void* b, long long ldb, long long strideB) {
+  if (!c || !m || !k || !qunit || !rhs_scale || !b) return fail(BEM_E_ARG, "NULL argument");
+  if (nf <= 0 || n_ch < 1 || n_ch > BEM_MAX_CHANNELS) return fail(BEM_E_ARG, "invalid nf or n_ch");
+  if (ldb < m->V || strideB < ldb * n_ch) return fail(BEM_E_ARG, "leading dimension or stride too small");
+  if (precision != BEM_PREC_MIXED && precision != BEM_PREC_FP64) return fail(BEM_E_ARG, "unknown precision");
+  if (m->sing_k != k || m->sing_nf < nf)
+    return fail(BEM_E_STATE, "bem_assemble_rhs reuses the singular blocks of the last bem_assemble on this mesh, "
+                             "which was called with other wavenumbers");
+  note_capture_use(c, m);
+  hipStream_t st = c->stream;
+  hipError_t e;
+  auto run = [&](auto tag) -> hipError_t {
+    using T = decltype(tag);
+    RhsParams<T> rp;
+    rp.rec = sizeof(T) == 4 ? (const ElemRec<T>*)m->rec32 : (const ElemRec<T>*)m->rec64;
+    rp.fv = m->fv;
+    rp.src = m->src_faces;
+    rp.qunit = static_cast<const cplx<double>*>(qunit);
+    rp.sing_ptr = m->sing_ptr;
+    rp.sing_trial = m->sing_trial;
+    rp.sing_rhs = static_cast<const cplx<double>*>(m->sing_rhs.p);
+    rp.k = k;
+    rp.out = static_cast<cplx<double>*>(m->rhs_part.p);
+    rp.F = m->F;
+    rp.P = m->P;
+    rp.n_src = m->n_src;
+    rp.n_ch = n_ch;
+    rp.R = m->R;
+    rp.pmask = m->pmask;
+    hipError_t e2;
+    {
+      ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
+      if ((e2 = launch_rhs<T>(rp, nf, st)) != hipSuccess) return e2;
+    }
+    GatherParams<T> gp{};
+    gp.vstar_ptr = m->vstar_ptr;
+    gp.vstar_row = m->vstar_row;
+    gp.rhs_part = rp.out;
+    gp.rhs_scale = static_cast<const cplx<double>*>(rhs_scale);
+    gp.b = static_cast<cplx<double>*>(b);
+    gp.ldb = ldb;
+    gp.strideB = strideB;
+    gp.rowmax = nullptr;
+    gp.N = m->V;
+    gp.F = m->F;
+    gp.n_ch = n_ch;
+    gp.roww = m->roww;
+    ScopedTimer tm(c, BEM_TIMER_GATHER, (long long)nf);
+    return launch_gather_rhs<T>(gp, nf, st);
+  };
+  e = precision == BEM_PREC_MIXED ? run(float()) : run(double());
+  if (e != hipSuccess) return hipfail(e, "assemble_rhs launch");
+  return BEM_OK;
+}
+
+int bem_neumann(bem_ctx* c, int nf, int n_ch, int n_faces, const void* rhs_scale, const void* qunit, void* q) {
+  if (!c || !rhs_scale || !qunit || !q) return fail(BEM_E_ARG, "NULL argument");
+  if (nf <= 0 || n_ch <= 0 || n_faces <= 0) return fail(BEM_E_ARG, "invalid sizes");
+  hipError_t e = neumann_batch(static_cast<const cplx<double>*>(rhs_scale), static_cast<const cplx<double>*>(qunit),
+                               static_cast<cplx<double>*>(q), n_ch, n_faces, nf, c->stream);
+  if (e != hipSuccess) return hipfail(e, "neumann launch");
+  return BEM_OK;
+}
+
+int bem_solve_status(bem_ctx* c, int n, int batch, const int* info, const void* x, long long ldx, long long strideX,
+                     int nrhs, const double* rel, double tol, const double* growth, double growth_max, int* flags) {
+  if (!c || !info || !x || !flags) return fail(BEM_E_ARG, "NULL argument");
+  if (n <= 0 || batch <= 0 || nrhs < 1 || ldx < n || strideX < ldx * nrhs) return fail(BEM_E_ARG, "invalid sizes");
+  if (c->capturing) return fail(BEM_E_STATE, "bem_solve_status synchronizes and cannot be captured");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  if (c->status_cap < batch) {
+    if (c->status_dev) (void)hipFree(c->status_dev);
+    c->status_dev = nullptr;
+    c->status_cap = 0;
+    hipError_t e = hipMalloc(&c->status_dev, (size_t)batch * sizeof(int));
+    if (e != hipSuccess) return fail(BEM_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    c->status_cap = batch;
+  }
+  HIPCHK(solve_status_batch(info, static_cast<const cplx<double>*>(x), ldx, strideX, n, nrhs, rel, tol, growth,
+                            growth_max, c->status_dev, batch, c->stream),
+         "solve_status launch");
+  HIPCHK(hipMemcpyAsync(flags, c->status_dev, (size_t)batch * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+         "hipMemcpyAsync");
+  HIPCHK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  int any = 0;
+  for (int i = 0; i < batch; ++i) any |= flags[i];
+  if (any & 3) {
+    int first = 0;
+    while (!(flags[first] & 3)) ++first;
+    return fail(BEM_E_SINGULAR, "matrix " + std::to_string(first) +
+                                    ((flags[first] & 1) ? ": LU breakdown (zero pivot)" : ": non-finite solution"));
+  }
+  if (any & 12) {
+    int first = 0;
+    while (!(flags[first] & 12)) ++first;
+    return fail(BEM_E_ACCURACY, "matrix " + std::to_string(first) +
+                                    ((flags[first] & 4) ? ": relative residual above the tolerance"
+                                                        : ": pivot growth above the limit"));
+  }
+  return BEM_OK;
 }
 
 int bem_lu_workspace_bytes(int precision, int n, int batch, long long* bytes) {
@@ -828,6 +1059,7 @@ int bem_field(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, i
   int rc;
   if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * 6 * m->F * 2 * cb))) return rc;
   if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
+  note_capture_use(c, m);
   hipError_t e;
   ScopedTimer tm(c, BEM_TIMER_FIELD, (long long)nf);
   if (precision == BEM_PREC_MIXED) {
@@ -848,31 +1080,54 @@ int bem_graph_begin(bem_ctx* c) {
   if (c->capturing) return fail(BEM_E_STATE, "already capturing");
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   c->capturing = true;
+  c->capture_meshes.clear();
   return BEM_OK;
 }
 
 int bem_graph_end(bem_ctx* c, void** graph_exec) {
   if (!c || !graph_exec) return fail(BEM_E_ARG, "NULL argument");
   if (!c->capturing) return fail(BEM_E_STATE, "not capturing");
+  *graph_exec = nullptr;
   hipGraph_t g = nullptr;
   c->capturing = false;
+  std::vector<bem_mesh*> used;
+  used.swap(c->capture_meshes);
   HIPCHK(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
   hipGraphExec_t ge = nullptr;
   hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
-  *graph_exec = ge;
+  bem_graph* h = nullptr;
+  try {
+    h = new bem_graph{ge, used};
+  } catch (const std::exception&) {
+    (void)hipGraphExecDestroy(ge);
+    return fail(BEM_E_NOMEM, "graph handle allocation failed");
+  }
+  for (bem_mesh* m : h->meshes) ++m->live_graphs;
+  *graph_exec = h;
   return BEM_OK;
 }
 
 int bem_graph_launch(bem_ctx* c, void* graph_exec) {
   if (!c || !graph_exec) return fail(BEM_E_ARG, "NULL argument");
-  HIPCHK(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), c->stream), "hipGraphLaunch");
+  HIPCHK(hipGraphLaunch(static_cast<bem_graph*>(graph_exec)->exec, c->stream), "hipGraphLaunch");
   return BEM_OK;
 }
 
 int bem_graph_destroy(void* graph_exec) {
-  if (graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec));
+  if (!graph_exec) return BEM_OK;
+  bem_graph* h = static_cast<bem_graph*>(graph_exec);
+  (void)hipGraphExecDestroy(h->exec);
+  for (bem_mesh* m : h->meshes)
+    if (--m->live_graphs == 0 && m->closed) mesh_free(m);
+  delete h;
+  return BEM_OK;
+}
+
+int bem_mesh_live_graphs(bem_mesh* m, int* count) {
+  if (!m || !count) return fail(BEM_E_ARG, "NULL argument");
+  *count = m->live_graphs;
   return BEM_OK;
 }
 

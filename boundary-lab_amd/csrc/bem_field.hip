@@ -163,6 +163,19 @@ hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, con
   return hipGetLastError();
 }
 
+// Neumann data of every channel: q[f][c][j] = scale[f] * qunit[c][j] (scale = i rho omega per frequency).
+__global__ void k_neumann(const cplx<double>* scale, const cplx<double>* qunit, cplx<double>* q, int n_ch, int F) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y, f = blockIdx.z;
+  if (j >= F) return;
+  q[((size_t)f * n_ch + c) * F + j] = cmul(scale[f], qunit[(size_t)c * F + j]);
+}
+
+hipError_t neumann_batch(const cplx<double>* scale, const cplx<double>* qunit, cplx<double>* q, int n_ch, int F,
+                         int nf, hipStream_t st) {
+  hipLaunchKernelGGL(k_neumann, dim3((F + 255) / 256, n_ch, nf), dim3(256), 0, st, scale, qunit, q, n_ch, F);
+  return hipGetLastError();
+}
+
 template hipError_t field_batch<float>(const SrcRec<float>*, const int4*, const float*, const double*, int, int, int,
                                        const double*, const cplx<double>*, long long, long long,
                                        const cplx<double>*, long long, long long, int, int, cplx<float>*,

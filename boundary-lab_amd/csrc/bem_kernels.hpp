@@ -133,6 +133,13 @@ template <typename T>
 hipError_t launch_rhs(const RhsParams<T>&, int nf, hipStream_t);
 template <typename T>
 hipError_t launch_gather(const GatherParams<T>&, int nf, hipStream_t);
+template <typename T>
+hipError_t launch_gather_rhs(const GatherParams<T>&, int nf, hipStream_t);
+hipError_t neumann_batch(const cplx<double>* scale, const cplx<double>* qunit, cplx<double>* q, int n_ch, int F,
+                         int nf, hipStream_t);
+hipError_t solve_status_batch(const int* info, const cplx<double>* x, long long ldx, long long strideX, int N,
+                              int nrhs, const double* rel, double tol, const double* growth, double gmax, int* flags,
+                              int batch, hipStream_t);
 
 // elements (cT) of LU workspace per matrix: inverted 64 x 64 diagonal blocks of L and U
 long long lu_workspace_elems(int N);

@@ -1895,6 +1895,42 @@ hipError_t growth_batch(const MatBatch<T>& mb, int batch, double* umax, const do
   return hipGetLastError();
 }
 
+// Outcome of a batch of solves, one workgroup per matrix: flags[m] bit 0 = LU breakdown (info[m] != 0), bit 1 =
+// a non-finite solution entry, bit 2 = relative residual above tol (or NaN), bit 3 = pivot growth above gmax (or
+// NaN). rel and growth may be null (not checked).
+__global__ __launch_bounds__(256) void k_solve_status(const int* info, const cplx<double>* x, long long ldx,
+                                                      long long strideX, int N, int nrhs, const double* rel,
+                                                      double tol, const double* growth, double gmax, int* flags) {
+  const int mat = blockIdx.x, tid = threadIdx.x;
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  int local = 0;
+  for (int c = 0; c < nrhs; ++c) {
+    const cplx<double>* xc = x + (size_t)mat * strideX + (size_t)c * ldx;
+    for (int i = tid; i < N; i += 256)
+      if (!isfinite(xc[i].re) || !isfinite(xc[i].im)) local = 1;
+  }
+  if (local) atomicOr(&bad, 1);
+  __syncthreads();
+  if (tid == 0) {
+    int f = 0;
+    if (info[mat] != 0) f |= 1;
+    if (bad) f |= 2;
+    if (rel && !(rel[mat] <= tol)) f |= 4;
+    if (growth && !(growth[mat] <= gmax)) f |= 8;
+    flags[mat] = f;
+  }
+}
+
+hipError_t solve_status_batch(const int* info, const cplx<double>* x, long long ldx, long long strideX, int N,
+                              int nrhs, const double* rel, double tol, const double* growth, double gmax, int* flags,
+                              int batch, hipStream_t st) {
+  hipLaunchKernelGGL(k_solve_status, dim3(batch), dim3(256), 0, st, info, x, ldx, strideX, N, nrhs, rel, tol, growth,
+                     gmax, flags);
+  return hipGetLastError();
+}
+
 template hipError_t lu_factor_batch<float>(const MatBatch<float>&, int, int*, int*, long long, int*, const int*,
                                            cplx<float>*, hipStream_t, const LuLookahead*, bool);
 template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, int*, long long, int*, const int*,

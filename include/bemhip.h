@@ -31,6 +31,8 @@ extern "C" {
 #define BEM_E_HIP 2      /* HIP runtime error */
 #define BEM_E_NOMEM 3    /* device allocation failed */
 #define BEM_E_STATE 4    /* operation not allowed in the current state (e.g. allocation during capture) */
+#define BEM_E_SINGULAR 5 /* LU breakdown (zero pivot) or a non-finite solution (bem_solve_status) */
+#define BEM_E_ACCURACY 6 /* refinement short of its tolerance or pivot growth above the limit (bem_solve_status) */
 
 #define BEM_PREC_FP64 0  /* FP64 assembly, FP64 LU, FP64 field */
 #define BEM_PREC_MIXED 1 /* FP32 regular assembly + FP64 singular, FP32 LU (FP32 MFMA), FP64 refinement */
@@ -140,13 +142,28 @@ typedef struct bem_mesh_desc {
   const int* plane_mask;     /* [F] bit r: the face takes part in an image pair of transform r (NULL: none) */
 } bem_mesh_desc;
 
+/* Check every index table of a mesh description on the host (no device needed): face vertex ids < V, the trial
+ * order a permutation of the faces padded with -1 and inverted by elem_pos, slot tables, vertex_last, the
+ * vertex -> strip-row CSR (each strip row 3t+l exactly once, under the vertex of corner l of face t), singular
+ * pairs grouped by test element with trial keys < n_transforms * F and rule indices < n_rules, rule ranges and
+ * radiating faces. BEM_E_ARG with the reason when anything is out of range; bem_mesh_create runs it first, so a
+ * malformed description never reaches a kernel. Part of build_rocm_regular_assembly_cache's input contract
+ * (JS/BeatEngineRocmRegular.jl:134-205), which the reference trusts unchecked. */
+int bem_mesh_validate(const bem_mesh_desc* desc);
+
 /* Upload a prepared mesh: SoA geometry, locality/slot tables, singular pairs and Duffy rules, field sources.
  * Replaces build_rocm_regular_assembly_cache (JS/BeatEngineRocmRegular.jl:134-205),
  * build_rocm_singular_correction_cache (JS/BeatEngineRocmSingular.jl:1-55),
  * build_rocm_field_evaluation_cache (JS/BeatEngineRocmField.jl:1-39) and the identity cache
  * (JS/BeatEngineRocmOperators.jl:23-32). Synchronous. */
 int bem_mesh_create(bem_ctx* ctx, const bem_mesh_desc* desc, bem_mesh** out);
+/* Release the mesh. While captured graphs still use it (see bem_graph_end) the release is deferred until the
+ * last of them is destroyed. */
 int bem_mesh_destroy(bem_mesh* mesh);
+/* Number of live captured graphs whose nodes use this mesh's tables or workspace. While it is non-zero the mesh's
+ * workspace cannot grow: bem_mesh_reserve, bem_assemble or bem_field asking for more returns BEM_E_STATE instead
+ * of moving buffers the graphs address. */
+int bem_mesh_live_graphs(bem_mesh* mesh, int* count);
 /* Reserve library workspace for up to `nf` frequencies per call at `precision` (device allocations happen
  * here, never inside a captured graph). `strip_nf` bounds how many frequencies share one strip buffer. */
 int bem_mesh_reserve(bem_mesh* mesh, int precision, int nf, int strip_nf, int n_obs, int n_ch);
@@ -164,6 +181,20 @@ int bem_mesh_device_bytes(bem_mesh* mesh, long long* bytes);
 int bem_assemble(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const double* k, const void* qunit,
                  int n_ch, const void* rhs_scale, void* A, long long lda, long long strideA, void* A0, void* b,
                  long long ldb, long long strideB, double* rowmax);
+
+/* Right-hand sides only, for a further group of up to BEM_MAX_CHANNELS channels of the same frequencies: b as in
+ * bem_assemble for these unit drives, A untouched. Reuses the singular RHS blocks of the last bem_assemble on this
+ * mesh, which must have been called with the same device wavenumber array k (BEM_E_STATE otherwise). The
+ * reference re-forms and re-factors the system per channel (julia_local/solver.jl:889-909 calling
+ * JS/BeatEngineRocmOperators.jl:224-251); here the factors of one bem_assemble serve every channel group and only
+ * _rocm_burton_miller_rhs (JS/BeatEngineRocmOperators.jl:110-121) runs per group. */
+int bem_assemble_rhs(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const double* k, const void* qunit,
+                     int n_ch, const void* rhs_scale, void* b, long long ldb, long long strideB);
+
+/* Neumann data q[f][c][j] = rhs_scale[f] * qunit[c][j] (c128; rhs_scale = i rho omega per frequency, qunit the
+ * unit channel drives [n_ch][n_faces]) on the device. Replaces the q = i rho omega v_n of pressure_for_drives
+ * (julia_local/solver.jl:434-470, :456). */
+int bem_neumann(bem_ctx* ctx, int nf, int n_ch, int n_faces, const void* rhs_scale, const void* qunit, void* q);
 
 /* Bytes of the LU workspace for `batch` N x N matrices: the inverted 64 x 64 diagonal blocks of L and U
  * that bem_lu_factor writes and bem_lu_solve / bem_refine read (cT [batch][ceil(N/64)][2][64][64]). */
@@ -206,6 +237,18 @@ int bem_residual(bem_ctx* ctx, int precision, int n, int batch, const void* A, l
 int bem_growth(bem_ctx* ctx, int precision, int n, int batch, const void* LU, long long lda, long long strideA,
                const double* amax, double* umax, double* growth);
 
+/* Outcome of a batch of solves, made visible to the host (synchronizes the context stream; not capturable).
+ * flags (HOST int [batch]) receives per matrix bit 0: info[m] != 0 (LU breakdown), bit 1: a non-finite entry of
+ * x (c128 [batch][nrhs][ldx], stride strideX), bit 2: rel[m] > tol or NaN, bit 3: growth[m] > growth_max or NaN
+ * (rel, growth: device double [batch], may be NULL to skip that check). Returns BEM_E_SINGULAR if any matrix has
+ * bit 0 or 1, else BEM_E_ACCURACY if any has bit 2 or 3, else BEM_OK. The reference's counterpart is the
+ * rocSOLVER info check (_is_corrupt_rocsolver_info, JS/BeatEngineRocmOperators.jl:123-130) and lu!'s
+ * SingularException (JS/BeatEngineCpuSolve.jl:63), which its worker turns into a `failed` event
+ * (julia_local/solver.jl:1033). */
+int bem_solve_status(bem_ctx* ctx, int n, int batch, const int* info, const void* x, long long ldx,
+                     long long strideX, int nrhs, const double* rel, double tol, const double* growth,
+                     double growth_max, int* flags);
+
 /* Field at n_obs points (obs: device FP64 [n_obs][3]) for nf frequencies and n_ch channels:
  *   u = sum_m dG/dn p_w - G q_w     (p: c128 [nf][n_ch][ldp] boundary pressure; q: c128 [nf][n_ch][F]
  *   Neumann data including i rho omega; out: c128 [nf][n_ch][ldo]).
@@ -215,7 +258,9 @@ int bem_field(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const double*
               void* out, long long ldo, long long strideO);
 
 /* Stream capture into a HIP graph on the context's stream (frequency batching under one graph launch).
- * bem_graph_end returns an executable graph handle usable with bem_graph_launch. */
+ * bem_graph_end returns an executable graph handle usable with bem_graph_launch. The handle records the meshes
+ * the captured sequence used: until bem_graph_destroy, their workspace cannot grow (bem_mesh_live_graphs) and
+ * bem_mesh_destroy defers their release. Replaces the per-frequency launch loop of julia_local/solver.jl:805-997. */
 int bem_graph_begin(bem_ctx* ctx);
 int bem_graph_end(bem_ctx* ctx, void** graph_exec);
 int bem_graph_launch(bem_ctx* ctx, void* graph_exec);

@@ -5,6 +5,14 @@ Fixtures (NumPy .npz, no pickles):
   boundary pressure (642), field at a 100-point Fibonacci sphere of radius 2 m, LHS rows 0..7 and RHS.
 * tetra_pair_ops.npz -- two disjoint tetrahedron surfaces (8 triangles, 8 vertices), k = 5:
   the four Galerkin operators S, D, D*, H in full.
+* horn6k_fp64.npz -- the 6,000-triangle horn-in-box mesh (vertices, faces, tags as generated by
+  synthetic.horn_in_box(), seed 1234), unit throat velocity, rho = 1.205, c = 343, at 100 Hz, 946 Hz (the 21st of
+  the 40 log-spaced frequencies 100 Hz - 8 kHz) and 8 kHz: LHS times a seeded complex vector (A v), the RHS b,
+  boundary pressure and the field at the 2 x 73 polar-arc points at 2 m (5 degree step). FP64 C oracle
+  (oracle/beat_cpu.c, double) with LAPACK zgetrf/zgetrs.
+* sphere1280_40f.npz -- pulsating sphere, 1,280 triangles, all 40 log-spaced frequencies 100 Hz - 8 kHz: boundary
+  pressure and field at the polar arcs plus a 200-point Fibonacci sphere at 2 m, stored as complex64 (the mixed
+  path is compared at 0.05 dB / 0.5 degrees). FP64 C oracle.
 The mesh generators are the product's own seeded synthetic ones; the oracle is the only solver used.
 """
 
@@ -27,6 +35,63 @@ def tetra_pair():
     return V, F, np.ones(len(F), int)
 
 
+HORN_FREQS = (100.0, float(np.logspace(2.0, np.log10(8000.0), 40)[20]), 8000.0)
+
+
+def polar_points():
+    from beat_hip import driver as D
+
+    _, hor, ver, _ = D.polar_observation_points({"step_size": 5.0})
+    return np.concatenate([hor, ver]).astype(np.float64)
+
+
+def horn6k():
+    from oracle import beat_cpu as B
+
+    V, F, T = syn.horn_in_box()
+    m = O.Mesh(V, F, T, np.ones(len(F), int))
+    cm = B.CpuMesh(m, "double")
+    qunit = (T == 2).astype(complex)[:, None]
+    obs = polar_points()
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(len(V)) + 1j * rng.standard_normal(len(V))
+    out = {"vertices": V, "faces": F.astype(np.int32), "tags": T.astype(np.int32), "points": obs, "v": v,
+           "freqs": np.asarray(HORN_FREQS)}
+    Av, b, p, fld = [], [], [], []
+    for f in HORN_FREQS:
+        k = 2 * np.pi * f / 343.0
+        ops = cm.assemble(k)
+        lhs, rhs_op = O.burton_miller(ops, cm.M11, cm.M10, k)
+        q = 1j * 1.205 * 2 * np.pi * f * qunit[:, 0]
+        Av.append(lhs @ v)
+        b.append(rhs_op @ q)
+        sol = cm.solve_frequency(f, qunit, 343.0, 1.205, obs)
+        p.append(sol["pressure"][:, 0])
+        fld.append(sol["field"][:, 0])
+        print(f"horn6k {f:.1f} Hz done", flush=True)
+    out.update(Av=np.array(Av), b=np.array(b), pressure=np.array(p), field=np.array(fld))
+    np.savez_compressed(HERE / "horn6k_fp64.npz", **out)
+
+
+def sphere40():
+    from oracle import beat_cpu as B
+
+    V, F, T = syn.icosphere(3, 0.1)
+    m = O.Mesh(V, F, T, np.ones(len(F), int))
+    cm = B.CpuMesh(m, "double")
+    freqs = np.logspace(2.0, np.log10(8000.0), 40)
+    sph = O.fibonacci_observation({"spherical_sampling_enabled": True, "spherical_sampling_points": 200,
+                                   "distance": 2.0})[0].astype(np.float64)
+    obs = np.concatenate([polar_points(), sph])
+    p, fld = [], []
+    for f in freqs:
+        sol = cm.solve_frequency(float(f), np.ones((len(F), 1)), 343.0, 1.205, obs)
+        p.append(sol["pressure"][:, 0])
+        fld.append(sol["field"][:, 0])
+    np.savez_compressed(HERE / "sphere1280_40f.npz", vertices=V, faces=F.astype(np.int32), points=obs, freqs=freqs,
+                        pressure=np.array(p, np.complex64), field=np.array(fld, np.complex64))
+
+
 def main():
     V, F, T = syn.icosphere(3, 0.1)
     m = O.Mesh(V, F, T, np.ones(len(F), int))
@@ -40,7 +105,13 @@ def main():
     ops = O.assemble_operators(m, 5.0)
     np.savez_compressed(HERE / "tetra_pair_ops.npz", vertices=V, faces=F, k=5.0, S=ops.single_layer,
                         D=ops.double_layer, Ds=ops.adjoint_double_layer, H=ops.hypersingular)
+    sphere40()
+    horn6k()
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[name]()
+    else:
+        main()

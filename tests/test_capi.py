@@ -4,6 +4,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -76,3 +77,60 @@ def test_lu_workspace_size():
     assert _lib.lu_workspace_bytes(_lib.BEM_PREC_FP64, 64, 1) == 2 * 4096 * 16
     with pytest.raises(RuntimeError):
         _lib.lu_workspace_bytes(7, 64, 1)
+
+
+@pytest.fixture(scope="module")
+def desc_parts():
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+
+    V, F, T = syn.icosphere(2, 0.1)
+    prep = G.prepare(G.BoundaryMesh(V, F, T))
+    return prep
+
+
+def _validate(prep, mutate=None, src=(0, 1)):
+    from beat_hip import _lib
+    from beat_hip.engine import mesh_desc
+
+    import copy
+
+    d, keep = mesh_desc(copy.deepcopy(prep), np.asarray(src))  # mutations must not reach the shared mesh
+    if mutate is not None:
+        mutate(d, keep)
+    lib = _lib.load()
+    rc = lib.bem_mesh_validate(ctypes.byref(d))
+    return rc, _lib.last_error(lib)
+
+
+def test_valid_mesh_description_passes(desc_parts):
+    from beat_hip import _lib
+
+    rc, msg = _validate(desc_parts)
+    assert rc == _lib.BEM_OK, msg
+
+
+@pytest.mark.parametrize("name,mutate,expect", [
+    ("faces", lambda d, k: k["faces"].__setitem__((5, 1), 10 ** 6), "face vertex id"),
+    ("faces_neg", lambda d, k: k["faces"].__setitem__((0, 0), -1), "face vertex id"),
+    ("vstar_row", lambda d, k: k["vsr"].__setitem__(3, 10 ** 7), "vstar_row"),
+    ("vstar_row_swap", lambda d, k: k["vsr"].__setitem__(slice(0, 2), k["vsr"][[1, 0]] + 3), "vstar_row"),
+    ("vstar_ptr_neg", lambda d, k: k["vsp"].__setitem__(-1, -5), "vstar_ptr"),
+    ("vstar_ptr_order", lambda d, k: k["vsp"].__setitem__(4, k["vsp"][6]), "vstar"),
+    ("order_pad", lambda d, k: k["order"].__setitem__(-1, 3), "order"),
+    ("sing_rule", lambda d, k: k["sr"].__setitem__(7, 99), "rule index"),
+    ("sing_test", lambda d, k: k["st"].__setitem__(7, k["st"][7] + 1), "grouped by test"),
+    ("sing_ptr", lambda d, k: k["sp"].__setitem__(-1, k["sp"][-1] + 1), "sing_ptr"),
+    ("sing_trial", lambda d, k: k["str"].__setitem__(2, 10 ** 6), "trial key"),
+    ("rule_len", lambda d, k: k["rl"].__setitem__(0, 0), "rule offset"),
+    ("src", lambda d, k: k["src"].__setitem__(0, 10 ** 6), "radiating face"),
+    ("vertex_last", lambda d, k: k["vlast"].__setitem__(0, 999), "vertex_last"),
+    ("area", lambda d, k: k["a2"].__setitem__(3, 0.0), "area"),
+])
+def test_malformed_mesh_description_rejected(desc_parts, name, mutate, expect):
+    # every index table the kernels dereference is checked on the host: BEM_E_ARG before any upload or launch
+    from beat_hip import _lib
+
+    rc, msg = _validate(desc_parts, mutate)
+    assert rc == _lib.BEM_E_ARG
+    assert expect in msg, msg

@@ -60,7 +60,7 @@ def _drain(h):
 
 def test_jobs_overlap_and_complete(tmp_path):
     SlowEngine.peak = 0
-    sched = S.Scheduler(hbm_budget_bytes=1 << 40, slots=3, batch=2, engine_factory=SlowEngine)
+    sched = S.Scheduler(hbm_budget_bytes=1 << 40, slots=3, batch=2, engine_factory=SlowEngine, merge=False)
     path = _mesh(tmp_path)
     handles = [sched.submit(_req(path, nf=6))[0] for _ in range(5)]
     results = [_drain(h) for h in handles]
@@ -74,7 +74,7 @@ def test_jobs_overlap_and_complete(tmp_path):
 
 
 def test_admission_by_footprint_serialises_big_jobs(tmp_path):
-    sched = S.Scheduler(hbm_budget_bytes=1, slots=4, batch=2, engine_factory=SlowEngine)
+    sched = S.Scheduler(hbm_budget_bytes=1, slots=4, batch=2, engine_factory=SlowEngine, merge=False)
     path = _mesh(tmp_path)
     handles = [sched.submit(_req(path, nf=3))[0] for _ in range(3)]
     for h in handles:
@@ -208,3 +208,84 @@ def test_health_payload_matches_server_schema():
     assert set(h["capabilities"]) == set(S.CAPABILITY_KEYS) and len(S.CAPABILITY_KEYS) == 8
     assert all(h["capabilities"].values())  # beat_rocm advertises every flag the route reports
     assert json.loads(json.dumps(h)) == h
+
+
+class RecordingEngine(SlowEngine):
+    """Slow double that records the frequency lists of its device batches."""
+
+    batches = []
+
+    def batch(self, precision, nf, obs, rho, c, **kw):
+        inner = super().batch(precision, nf, obs, rho, c, **kw)
+        rec = RecordingEngine.batches
+
+        class B:
+            def run(self, freqs):
+                rec.append(list(np.round(freqs, 3)))
+                return inner.run(freqs)
+
+            def close(self):
+                pass
+
+        return B()
+
+
+def test_identical_meshes_merge_into_shared_batches(tmp_path):
+    # while one job holds the only slot, three requests for the same mesh and drives queue up; they are admitted
+    # together and their frequencies share device batches, each job still receiving exactly its own results
+    RecordingEngine.batches = []
+    sched = S.Scheduler(hbm_budget_bytes=1 << 40, slots=1, batch=8, engine_factory=RecordingEngine)
+    other = _mesh(tmp_path, 2, "other.msh")
+    same = _mesh(tmp_path, 1, "same.msh")
+    h0 = sched.submit(_req(other, nf=200))[0]  # 25 slow batches hold the slot for about half a second
+    time.sleep(0.1)
+    reqs = [_req(same, nf=3), _req(same, nf=4), _req(same, nf=2)]
+    reqs[1]["frequencies_hz"] = [250.0, 750.0, 1250.0, 1750.0]
+    hs = [sched.submit(r)[0] for r in reqs]
+    ev = [_drain(h) for h in [h0] + hs]
+    sched.shutdown()
+    assert sched.merged_jobs == 2 and [h.group_size for h in hs] == [3, 3, 3]
+    for r, e in zip(reqs, ev[1:]):
+        res = [x["result"] for x in e if x["type"] == "result"]
+        assert e[-1]["type"] == "completed"
+        assert [x["freq_hz"] for x in res] == [float(np.float32(f)) for f in r["frequencies_hz"]]
+        assert [x["solved_count"] for x in e if x["type"] == "result"] == list(range(1, len(res) + 1))
+    # the 9 frequencies of the merged jobs ran as 8 + 1, not 3 + 4 + 2
+    assert sorted(len(b) for b in RecordingEngine.batches[-2:]) == [1, 8]
+
+
+def test_bypass_is_bounded(tmp_path):
+    # a large job that does not fit beside running small ones may be passed at most max_bypass times
+    small = _mesh(tmp_path, 1, "small.msh")
+    big = _mesh(tmp_path, 3, "big.msh")
+    from beat_hip import driver as D
+    from beat_hip import geometry as G
+
+    def fp(path, nf):
+        init, parsed = D.parse_request(_req(path, nf))
+        job = D.build_job(_req(path, nf), parsed)
+        prep = G.prepare(job.mesh)
+        return S.Scheduler._footprint(None, job, prep, min(2, nf))
+
+    budget = fp(big, 4) + fp(small, 4) // 2  # the big job fits alone, not beside a small one
+    sched = S.Scheduler(hbm_budget_bytes=budget, slots=3, batch=2, engine_factory=SlowEngine, merge=False,
+                        max_bypass=2)
+    first = sched.submit(_req(small, nf=12))[0]
+    time.sleep(0.15)
+    hb = sched.submit(_req(big, nf=4))[0]
+    time.sleep(0.15)
+    smalls = [sched.submit(_req(small, nf=12))[0] for _ in range(6)]
+    for h in [first, hb] + smalls:
+        assert _drain(h)[-1]["type"] == "completed"
+    sched.shutdown()
+    passed = sum(1 for h in smalls if h.t_admit < hb.t_admit)
+    assert hb.bypassed <= 2 and passed <= 2
+
+
+def test_arrival_time_is_the_submission(tmp_path):
+    sched = S.Scheduler(hbm_budget_bytes=1 << 40, slots=1, batch=4, engine_factory=FakeEngine)
+    t0 = time.perf_counter()
+    h, _ = sched.submit(_req(_mesh(tmp_path), nf=2), arrival=t0 - 5.0)
+    _drain(h)
+    sched.shutdown()
+    assert h.latency >= 5.0 and h.t_ready is not None and h.t_admit >= h.t_ready

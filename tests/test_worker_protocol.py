@@ -220,3 +220,113 @@ def test_symmetry_requests(tmp_path):
     ev = []
     with pytest.raises(ValueError, match="positive X fundamental domain"):
         D.run_job(_request(str(bad), freqs=(800.0,), symmetry="x"), FakeEngine, ev.append, batch=1)
+
+
+class FlagEngine(FakeEngine):
+    """Test double whose solves report device status bits: `bad[precision]` maps a frequency to its bits."""
+
+    bad = {}
+    calls = []
+
+    def batch(self, precision, nf, obs, rho, c, **kw):
+        eng = self
+        base = FakeBatch(self.prep, self.qunit, obs, rho, c, nf)
+
+        class B:
+            def run(self, freqs):
+                FlagEngine.calls.append((precision, list(freqs)))
+                r = base.run(freqs)
+                r.flags = np.array([FlagEngine.bad.get(precision, {}).get(float(f), 0) for f in freqs], np.int32)
+                if precision == "fp64":
+                    r.pressure = r.pressure * 2.0  # tell the retry's result apart
+                return r
+
+            def close(self):
+                pass
+
+        return B()
+
+
+def test_fp64_job_with_lu_breakdown_fails(mesh_file):
+    FlagEngine.bad = {"fp64": {1000.0: 1}}
+    ev = []
+    req = _request(mesh_file, freqs=(500.0, 1000.0, 1500.0), precision="fp64")
+    D.run_job(req, FlagEngine, ev.append, batch=3)
+    # frequencies before the failing one keep their results, as in the reference's per-frequency loop
+    assert [e["type"] for e in ev if e["type"] != "status"] == ["initialized", "result", "failed"]
+    assert "LU breakdown" in ev[-1]["error"] and "1000" in ev[-1]["error"]
+
+
+def test_mixed_failure_retried_in_fp64(mesh_file):
+    # a mixed solve that misses its refinement tolerance is solved again on the FP64 path (bounded retry)
+    FlagEngine.bad = {"mixed": {1000.0: 4}}
+    FlagEngine.calls = []
+    ev = []
+    D.run_job(_request(mesh_file, freqs=(500.0, 1000.0, 1500.0)), FlagEngine, ev.append, batch=3)
+    res = [e for e in ev if e["type"] == "result"]
+    assert ev[-1]["type"] == "completed" and [r["solved_count"] for r in res] == [1, 2, 3]
+    d = res[1]["result"]["diagnostics"]
+    assert d["fp64_retry"] is True and "residual" in d["fp64_retry_reason"] and d["precision"] == "fp64"
+    assert "fp64_retry" not in res[0]["result"]["diagnostics"]
+    assert ("fp64", [1000.0]) in FlagEngine.calls
+
+
+def test_mixed_failure_and_failed_retry_is_failed(mesh_file):
+    FlagEngine.bad = {"mixed": {1500.0: 2}, "fp64": {1500.0: 2}}
+    ev = []
+    D.run_job(_request(mesh_file, freqs=(500.0, 1000.0, 1500.0)), FlagEngine, ev.append, batch=3)
+    types = [e["type"] for e in ev if e["type"] != "status"]
+    assert types == ["initialized", "result", "result", "failed"]
+    assert "non-finite" in ev[-1]["error"]
+
+
+def test_cancellation_is_checked_before_every_frequency(mesh_file, tmp_path):
+    # one device batch of 6 frequencies; the cancel file appears after the second result
+    cancel = tmp_path / "cancel"
+    req = _request(mesh_file, freqs=(300.0, 400.0, 500.0, 600.0, 700.0, 800.0), cancel_path=None)
+    req["cancel_path"] = str(cancel)
+    ev = []
+
+    def emit(e):
+        ev.append(e)
+        if e["type"] == "result" and e["solved_count"] == 2:
+            cancel.write_text("1")
+
+    D.run_job(req, FakeEngine, emit, batch=6)
+    assert ev[-1] == {"type": "cancelled", "solved_count": 2}
+    assert sum(e["type"] == "result" for e in ev) == 2
+
+
+def test_more_than_four_channels(mesh_file, tmp_path):
+    # six radiators on six channels: the reference loops over every channel (solver.jl:889-920)
+    V, F, T = syn.icosphere(1, 0.1)
+    T = (np.arange(len(F)) % 6 + 2).astype(int)
+    path = tmp_path / "six.msh"
+    syn.write_gmsh22(path, V, F, T, scale=0.001)
+    rads = [{"name": f"r{i}", "tag": i + 2, "channel": f"c{i}"} for i in range(6)]
+    req = _request(str(path), radiators=rads)
+    ev = []
+    D.run_job(req, FakeEngine, ev.append, batch=2)
+    res = [e["result"] for e in ev if e["type"] == "result"]
+    assert ev[-1]["type"] == "completed" and len(res[0]["channel_names"]) == 6
+    assert len(res[0]["horizontal_pressure"]["real"]) == 6
+
+
+def test_fit_batch_counts_the_fp64_companion():
+    from types import SimpleNamespace
+
+    from beat_hip.engine import Engine, estimate_footprint
+
+    V, F, T = syn.icosphere(3, 0.1)
+    from beat_hip import geometry as G
+
+    prep = G.prepare(G.BoundaryMesh(V, F, T))
+    fake = SimpleNamespace(prep=prep, qunit=np.ones((len(F), 1)))
+    one = estimate_footprint(prep.n_vertices, prep.n_faces, len(prep.pairs.test), 16, 146, "mixed", 1)
+    avail = int(one / 0.9) + 1
+    assert Engine.fit_batch(fake, "mixed", 16, 146, avail=avail) == 16
+    with_ref = Engine.fit_batch(fake, "mixed", 16, 146, companions=("fp64",), avail=avail)
+    assert with_ref < 16
+    need = sum(estimate_footprint(prep.n_vertices, prep.n_faces, len(prep.pairs.test), with_ref, 146, p, 1)
+               for p in ("mixed", "fp64"))
+    assert need <= 0.9 * avail
