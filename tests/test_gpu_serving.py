@@ -15,9 +15,10 @@ pytestmark = pytest.mark.gpu
 def _requests(tmp_path):
     from beat_hip import synthetic as syn
 
-    specs = [("sphere", syn.icosphere(3, 0.1), [1, 1], [300.0, 900.0, 1700.0, 2600.0, 4100.0, 6000.0]),
+    # the icosphere's faces all carry the radiator tag 2; the horn's throat disk is tag 2
+    specs = [("sphere", syn.icosphere(3, 0.1), [2, 2], [300.0, 900.0, 1700.0, 2600.0, 4100.0, 6000.0]),
              ("horn", syn.horn_in_box(h=0.05), [2], [150.0, 800.0, 2000.0, 5000.0, 7000.0]),
-             ("small", syn.icosphere(2, 0.1), [1], [500.0, 2500.0, 3300.0])]
+             ("small", syn.icosphere(2, 0.1), [2], [500.0, 2500.0, 3300.0])]
     reqs = []
     for name, (V, F, T), tags, freqs in specs:
         path = tmp_path / f"{name}.msh"
