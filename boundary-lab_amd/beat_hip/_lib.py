@@ -17,6 +17,7 @@ BEM_E_ARG, BEM_E_HIP, BEM_E_NOMEM, BEM_E_STATE, BEM_E_SINGULAR, BEM_E_ACCURACY =
 BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_MAX_CHANNELS = 4
+BEM_NOALLOC_SCRATCH_BYTES = 64 << 20
 BEM_LU_FP32 = 0
 BEM_LU_SPLIT_BF16 = 1
 TIMER_REGULAR, TIMER_SINGULAR, TIMER_GATHER, TIMER_RHS, TIMER_LU, TIMER_FIELD, TIMER_SOLVE = range(7)
@@ -101,6 +102,7 @@ SIGNATURES = {
     "bem_lu_solve": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _ll, _ll, _p, _ll, _ll, _i, _p]),
     "bem_refine": (_i, [_p, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _p, _ll, _ll, _p, _p, _p, _i, _i, _d, _p, _p,
                         _p]),
+    "bem_refine_noalloc": (_i, [_p, _i, _i, _p, _ll, _ll, _p, _ll, _p, _p, _p, _ll, _ll, _p, _i, _i, _d, _p, _p]),
     "bem_residual": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _ll, _ll, _p, _i, _p, _p]),
     "bem_growth": (_i, [_p, _i, _i, _i, _p, _ll, _ll, _p, _p, _p]),
     "bem_field": (_i, [_p, _p, _i, _i, _p, _i, _p, _p, _ll, _ll, _p, _ll, _i, _p, _ll, _ll]),

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,31 @@ static int hipfail(hipError_t e, const char* what) {
 
 #define BEM_TIMER_SLOTS 8
 #define BEM_AUX_STREAMS 3
+
+// Static scratch of bem_refine_noalloc: one copy per device, held until the process ends. Host code never uses
+// the array's name as an address: each device's copy is resolved once, with that device current, by
+// hipGetSymbolAddress in bem_ctx_create, and that address is what the kernels receive.
+static __device__ __align__(256) unsigned char scratch[BEM_NOALLOC_SCRATCH_BYTES];
+#define BEM_MAX_DEVICES 64
+static void* g_scratch_ptr[BEM_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int resolve_scratch(int device) {
+  if (device < 0 || device >= BEM_MAX_DEVICES) return fail(BEM_E_ARG, "device index above the library's limit");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return BEM_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return hipfail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = p;
+  return BEM_OK;
+}
+
+static void* scratch_ptr(int device) {
+  if (device < 0 || device >= BEM_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 struct bem_ctx {
   int device;
@@ -198,6 +224,11 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   HIPCHK(hipGetDeviceCount(&n), "hipGetDeviceCount");
   if (device < 0 || device >= n) return fail(BEM_E_ARG, "device index out of range");
   HIPCHK(hipSetDevice(device), "hipSetDevice");
+  // the static scratch of bem_refine_noalloc: this device's copy, resolved with the device current
+  {
+    const int rc = resolve_scratch(device);
+    if (rc) return rc;
+  }
   bem_ctx* c = new bem_ctx();
   c->device = device;
   c->capturing = false;
@@ -970,14 +1001,13 @@ int bem_lu_solve(bem_ctx* c, int precision, int n, int batch, const void* LU, lo
   return BEM_OK;
 }
 
-int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA, const int* perm,
-               long long strideP, const void* lu_work, const void* A0, const void* b, long long ldb,
-               long long strideB, void* x, void* work_r, void* work_y, int nrhs, int max_iter, double tol,
-               double* rel, int* iters, int* done) {
-  if (!c || !LU || !perm || !lu_work || !A0 || !b || !x || !work_r || !work_y || !rel || !iters || !done)
-    return fail(BEM_E_ARG, "NULL argument");
+}  // extern "C"
+
+static int refine_impl(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA,
+                       const int* perm, long long strideP, const void* lu_work, const void* A0, const void* b,
+                       long long ldb, long long strideB, void* x, void* work_r, void* work_y, int nrhs, int max_iter,
+                       double tol, double* rel, int* iters, int* done) {
   const long long sw = lu_workspace_elems(n);
-  if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || max_iter < 0) return fail(BEM_E_ARG, "invalid arguments");
   ScopedTimer tm(c, BEM_TIMER_SOLVE, (long long)batch);
   // matrices are independent: sub-batches refine on separate streams (each solve is a chain of short launches)
   hipError_t e = fan_out(c, batch, 4, [&](int b0, int nb, hipStream_t st) -> hipError_t {
@@ -1008,6 +1038,52 @@ int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long
   });
   if (e != hipSuccess) return hipfail(e, "refine");
   return BEM_OK;
+}
+
+static int refine_args_ok(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA,
+                          const int* perm, long long strideP, const void* lu_work, const void* A0, const void* b,
+                          long long ldb, long long strideB, const void* x, int nrhs, int max_iter, const double* rel,
+                          const int* iters) {
+  if (!c || !LU || !perm || !lu_work || !A0 || !b || !x || !rel || !iters) return fail(BEM_E_ARG, "NULL argument");
+  if (n <= 0 || batch <= 0 || lda < n || strideA < lda * n || strideP < n || ldb < n || strideB < ldb * nrhs)
+    return fail(BEM_E_ARG, "invalid sizes");
+  if (nrhs < 1 || nrhs > BEM_MAX_CHANNELS || max_iter < 0) return fail(BEM_E_ARG, "invalid arguments");
+  return BEM_OK;
+}
+
+extern "C" {
+
+int bem_refine(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA, const int* perm,
+               long long strideP, const void* lu_work, const void* A0, const void* b, long long ldb,
+               long long strideB, void* x, void* work_r, void* work_y, int nrhs, int max_iter, double tol,
+               double* rel, int* iters, int* done) {
+  int rc = refine_args_ok(c, n, batch, LU, lda, strideA, perm, strideP, lu_work, A0, b, ldb, strideB, x, nrhs,
+                          max_iter, rel, iters);
+  if (rc) return rc;
+  if (!work_r || !work_y || !done) return fail(BEM_E_ARG, "NULL argument");
+  return refine_impl(c, n, batch, LU, lda, strideA, perm, strideP, lu_work, A0, b, ldb, strideB, x, work_r, work_y,
+                     nrhs, max_iter, tol, rel, iters, done);
+}
+
+int bem_refine_noalloc(bem_ctx* c, int n, int batch, const void* LU, long long lda, long long strideA,
+                       const int* perm, long long strideP, const void* lu_work, const void* A0, const void* b,
+                       long long ldb, long long strideB, void* x, int nrhs, int max_iter, double tol, double* rel,
+                       int* iters) {
+  int rc = refine_args_ok(c, n, batch, LU, lda, strideA, perm, strideP, lu_work, A0, b, ldb, strideB, x, nrhs,
+                          max_iter, rel, iters);
+  if (rc) return rc;
+  // the residuals (c128), corrections (c64) and done flags in the device's static scratch, laid out with b's
+  // strides: [batch][strideB] each, 256-byte aligned
+  const size_t r_bytes = ((size_t)batch * strideB * 16 + 255) / 256 * 256;
+  const size_t y_bytes = ((size_t)batch * strideB * 8 + 255) / 256 * 256;
+  const size_t need = r_bytes + y_bytes + (size_t)batch * sizeof(int);
+  if (need > (size_t)BEM_NOALLOC_SCRATCH_BYTES)
+    return fail(BEM_E_ARG, "bem_refine_noalloc needs " + std::to_string(need) + " bytes of scratch; the static "
+                           "scratch holds " + std::to_string((long long)BEM_NOALLOC_SCRATCH_BYTES));
+  char* base = static_cast<char*>(scratch_ptr(c->device));
+  if (!base) return fail(BEM_E_STATE, "the static scratch of this device was not resolved");
+  return refine_impl(c, n, batch, LU, lda, strideA, perm, strideP, lu_work, A0, b, ldb, strideB, x, base,
+                     base + r_bytes, nrhs, max_iter, tol, rel, iters, reinterpret_cast<int*>(base + r_bytes + y_bytes));
 }
 
 int bem_residual(bem_ctx* c, int precision, int n, int batch, const void* A, long long lda, long long strideA,

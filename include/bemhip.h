@@ -226,6 +226,19 @@ int bem_refine(bem_ctx* ctx, int n, int batch, const void* LU, long long lda, lo
                long long strideP, const void* lu_work, const void* A0, const void* b, long long ldb, long long strideB, void* x,
                void* work_r, void* work_y, int nrhs, int max_iter, double tol, double* rel, int* iters, int* done);
 
+/* bem_refine without workspace arguments, for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. The residuals,
+ * corrections and done flags live in a static per-device scratch of BEM_NOALLOC_SCRATCH_BYTES (64 MiB, held by
+ * each device the library runs on until the process ends): batch * strideB * 24 bytes (+ 4 bytes per matrix,
+ * 256-byte aligned parts) must fit, else BEM_E_ARG. Calls no allocating routine. Every call on a device shares
+ * that device's scratch, so calls that overlap on one device (other streams, other threads) race; the scratch
+ * keeps the last call's contents and each call initialises what it reads. Same results as bem_refine. */
+#define BEM_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int bem_refine_noalloc(bem_ctx* ctx, int n, int batch, const void* LU, long long lda, long long strideA,
+                       const int* perm, long long strideP, const void* lu_work, const void* A0, const void* b,
+                       long long ldb, long long strideB, void* x, int nrhs, int max_iter, double tol, double* rel,
+                       int* iters);
+
 /* FP64 relative residual ||b_c - A x_c|| / ||b_c|| (max over channels) of a cT matrix and c128 x, b.
  * rel: device double [batch]. work_r: c128 like b. scratch: device int [2*batch]. */
 int bem_residual(bem_ctx* ctx, int precision, int n, int batch, const void* A, long long lda, long long strideA,
