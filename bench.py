@@ -4,15 +4,19 @@ Workload (synthetic, seeded): exponential horn (throat 25 mm, mouth 300 mm, leng
 0.4 x 0.6 x 0.3 m box, 6,000 triangles / 3,002 P1 unknowns, +-5 % vertex jitter (seed 1234), unit normal velocity
 on the throat disk, 40 log-spaced frequencies 100 Hz - 8 kHz, polar arcs (2 x 73 points) at 2 m.
 
-One step = solving all 40 frequencies of that job on the device with the mesh resident: mixed precision
-(FP32 assembly + FP64 singular pairs, FP32-MFMA LU, FP64 refinement to 1e-10), frequencies batched and the
-launch sequence captured once as a HIP graph and replayed. value = 40 * K / (time of K steps).
-The plain FP64 path (FP64 assembly/LU/field, one frequency per launch) is timed once beside it, and every
-mixed result is compared with its FP64 counterpart (far-field dB / phase error).
+One step = one request of that job through the worker's job driver (driver.run_job, the path a `beat_rocm`
+request takes): from the request dict to the last `result` event, including the Gmsh read, mesh preparation,
+upload, device batches (HIP graphs for jobs of several batches), result post-processing and the JSON encoding of
+every event, on a warm worker (kernels loaded). Mixed precision: FP32 assembly + FP64 singular pairs, split-bf16
+MFMA LU, FP64 refinement to 1e-10. value = 40 * K / (wall time of K requests).
+The plain FP64 path (FP64 assembly/LU/field, one frequency per launch) is timed the same way, end to end, and
+every mixed result is compared with its FP64 counterpart (far-field dB / phase error). The steady-state rate of
+the mixed batch alone (mesh resident, graph replay) is reported beside it.
 
 Also reported: the roofline of the dominant kernel (regular-pair assembly) from HIP events on the library's
 stream, the CPU baseline (C/OpenMP restatement of the BEAT CPU algorithm, FP32, LAPACK cgetrf) on a bounded
-sample, and the serving loop's p50/p99 job latency on a short mixed-size Poisson stream.
+sample, and the serving loop on the seeded Poisson stream of mixed-size jobs (200 jobs at 2 jobs/s by default):
+throughput and p50/p99 job latency from each job's scheduled arrival to its last result.
 """
 
 from __future__ import annotations
@@ -50,10 +54,10 @@ VALU_SLOTS_PER_QPAIR = 52
 VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-PIPE_NOTE = ("compute-bound on the FP32 vector ALUs (no MFMA, no HBM limit); 157.3 TFLOP/s is MI355X's FP32 peak, "
-             "shared by the matrix pipe and the packed-FP32 vector pipe; achieved counts ordered point pairs, the "
-             "kernel evaluates each unordered pair once (executed_tflops); valu_issue_frac is the share of the "
-             "VALU issue slots the compiled loop needs")
+PIPE_NOTE = ("bound by the FP32 vector ALUs (VALU issue, no MFMA, HBM far from its limit); 157.3 TFLOP/s is MI355X's "
+             "FP32 vector peak (packed FP32); achieved counts ordered point pairs, the kernel evaluates each "
+             "unordered pair once (executed_tflops); valu_issue_frac is the share of the VALU issue slots the "
+             "compiled loop needs")
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_regular_pair_horn6k_traffic.md"
 
 
@@ -204,6 +208,16 @@ def field_errors(fields_mixed, fields_ref):
     return float(max(db)), float(max(ph))
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(prep, qunit, obs, freqs, budget_s=20.0):
     """BEAT CPU algorithm restated (FP32, C/OpenMP + LAPACK) on a bounded sample of the same job."""
     from oracle import beat_cpu as B
@@ -221,45 +235,119 @@ def cpu_baseline(prep, qunit, obs, freqs, budget_s=20.0):
             break
     el = time.perf_counter() - t0
     return {"value": done / el, "unit": "freq-points/s", "cores": B.threads(), "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{done} of the 40 frequencies ({', '.join(f'{f:.0f} Hz' for f in sample[:done])}) of the "
-                      f"6k-triangle horn job, BEAT CPU algorithm restated in C/OpenMP FP32 + LAPACK cgetrf"}
+                      f"6k-triangle horn job, BEAT CPU algorithm restated in C/OpenMP FP32 + LAPACK cgetrf, "
+                      f"{B.threads()} threads on {cpu_model()}"}
 
 
-def serving_sample(n_jobs, rate, seed, batch, slots):
-    """Seeded Poisson stream of mixed-size jobs through the serving scheduler; returns latency stats."""
-    import tempfile
+# ---------------------------------------------------------------------------------------------------------------
+# end-to-end jobs through the worker's driver
+# ---------------------------------------------------------------------------------------------------------------
 
-    from beat_hip import serving as S
+
+def horn_request(path, precision, freqs):
+    cfg = {"mesh_file": str(path), "scale_factor": 1.0, "rho": 1.205, "sound_speed": 343.0, "step_size": 5.0,
+           "precision": precision, "radiators": [{"name": "throat", "tag": 2, "channel": "main"}]}
+    if precision == "fp64":
+        cfg["frequency_batch"] = 1  # the plain method: one frequency per launch
+    return {"schema_version": 2, "config": cfg, "frequencies_hz": [float(f) for f in freqs],
+            "beat_engine_backend": "rocm"}
+
+
+def run_request(req):
+    """One request through driver.run_job on the worker's shared device; returns (wall s, result events). Every
+    event is JSON-encoded as the worker writes it."""
+    from beat_hip import driver as D
+    from beat_hip import worker as W
+
+    lines, results = [], []
+
+    def emit(ev):
+        lines.append(json.dumps(ev, separators=(",", ":")))
+        if ev["type"] == "result":
+            results.append(ev["result"])
+        elif ev["type"] == "failed":
+            raise RuntimeError(ev["error"])
+
+    t0 = time.perf_counter()
+    D.run_job(req, W.engine_factory, emit)
+    return time.perf_counter() - t0, results
+
+
+def wire_field(result):
+    """Complex far field of a result event (horizontal then vertical arcs, channel 0) from its Float32 rows."""
+    rows = []
+    for key in ("horizontal_pressure", "vertical_pressure"):
+        w = result[key]
+        rows.append(np.asarray(w["real"][0], np.float64) + 1j * np.asarray(w["imag"][0], np.float64))
+    return np.concatenate(rows)
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# serving stream
+# ---------------------------------------------------------------------------------------------------------------
+
+
+def _serving_mesh(args):
+    j, kind, n_tri, seed, path = args
     from beat_hip import synthetic as syn
 
+    V, F, T = syn.mesh_for_triangle_count(kind, n_tri, seed=seed)
+    syn.write_gmsh22(path, V, F, T, scale=1.0)
+    return j, len(F), sorted(set(T.tolist()))
+
+
+def serving_stream(n_jobs, rate, seed, tmp):
+    """The seeded mixed-size stream: Poisson arrivals at `rate` jobs/s, mesh sizes log-uniform in 1,000-15,000
+    triangles over the sphere / horn-in-box / box-with-pistons generators, 8-60 frequencies log-uniform in
+    100 Hz - 10 kHz, 1-3 velocity sources. Meshes are generated up front in a process pool (host only)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
     rng = np.random.default_rng(seed)
-    tmp = Path(tempfile.mkdtemp(prefix="beat-serve-"))
-    jobs = []
+    specs, jobs = [], []
     for j in range(n_jobs):
-        n_tri = float(np.exp(rng.uniform(np.log(1000), np.log(15000))))
+        n_tri = int(np.exp(rng.uniform(np.log(1000), np.log(15000))))
         kind = ["sphere", "horn", "pistons"][j % 3]
-        V, F, T = syn.mesh_for_triangle_count(kind, int(n_tri), seed=seed + j)
-        path = tmp / f"job{j}.msh"
-        syn.write_gmsh22(path, V, F, T, scale=1.0)
-        tags = sorted(set(T.tolist()) - {1}) or [2]
-        nsrc = min(len(tags), int(rng.integers(1, 4)))
-        rads = [{"name": f"r{i}", "tag": int(tags[i]), "channel": f"c{i}"} for i in range(nsrc)]
-        nf = int(rng.integers(8, 61))
-        fr = np.sort(np.exp(rng.uniform(np.log(100), np.log(10000), nf))).tolist()
-        req = {"schema_version": 2, "frequencies_hz": fr,
-               "config": {"mesh_file": str(path), "scale_factor": 1.0, "radiators": rads, "step_size": 5.0}}
-        jobs.append((len(F), nf, req))
+        specs.append((j, kind, n_tri, seed + j, str(Path(tmp) / f"job{j}.msh")))
+        jobs.append({"nsrc": int(rng.integers(1, 4)), "nf": int(rng.integers(8, 61))})
+    for jb in jobs:
+        jb["freqs"] = np.sort(np.exp(rng.uniform(np.log(100), np.log(10000), jb["nf"]))).tolist()
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    workers = min(16, os.cpu_count() or 1)
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as pool:
+        made = {j: (nf, tags) for j, nf, tags in pool.map(_serving_mesh, specs)}
+    stream = []
+    for (j, kind, n_tri, _, path), jb, ta in zip(specs, jobs, arrivals):
+        n_faces, tags = made[j]
+        rtags = [t for t in tags if t != 1] or [2]
+        nsrc = min(len(rtags), jb["nsrc"])
+        rads = [{"name": f"r{i}", "tag": int(rtags[i]), "channel": f"c{i}"} for i in range(nsrc)]
+        req = {"schema_version": 2, "frequencies_hz": jb["freqs"],
+               "config": {"mesh_file": path, "scale_factor": 1.0, "radiators": rads, "step_size": 5.0}}
+        stream.append({"arrival": float(ta), "request": req, "triangles": n_faces, "kind": kind,
+                       "frequencies": jb["nf"], "sources": nsrc})
+    return stream
+
+
+def serving_sample(stream, rate, seed, batch, slots):
+    """Run the stream through the serving scheduler: each request is submitted at its scheduled arrival and
+    stamped with it, so host preparation and queueing count in its latency."""
+    from beat_hip import serving as S
+
     sched = S.Scheduler(slots=slots, batch=batch)
     handles = []
     t0 = time.perf_counter()
-    for (nt, nf, req), ta in zip(jobs, arrivals):
+    for job in stream:
         now = time.perf_counter() - t0
-        if ta > now:
-            time.sleep(ta - now)
-        h, _ = sched.submit(req)
+        if job["arrival"] > now:
+            time.sleep(job["arrival"] - now)
+        h, _ = sched.submit(job["request"], arrival=t0 + job["arrival"])
         handles.append(h)
-    total_f = 0
+        if len(handles) % 50 == 0:
+            log(f"bench: serving, {len(handles)} jobs submitted")
+    total_f, failed = 0, []
     for h in handles:
         while True:
             ev = h.events.get()
@@ -267,15 +355,21 @@ def serving_sample(n_jobs, rate, seed, batch, slots):
                 total_f += 1
             if ev["type"] in ("completed", "failed", "cancelled"):
                 if ev["type"] == "failed":
-                    raise RuntimeError(ev["error"])
+                    failed.append(ev["error"])
                 break
     wall = time.perf_counter() - t0
     sched.shutdown()
+    if failed:
+        raise RuntimeError(f"{len(failed)} serving jobs failed, first: {failed[0]}")
     lat = [h.latency for h in handles]
-    return {"jobs": n_jobs, "rate_jobs_per_s": rate, "freq_points": total_f, "wall_s": wall,
+    queue = [h.t_admit - h.arrival for h in handles]
+    return {"jobs": len(stream), "rate_jobs_per_s": rate, "seed": seed, "freq_points": total_f, "wall_s": wall,
             "throughput_freq_points_per_s": total_f / wall, "p50_latency_s": S.percentile(lat, 50),
-            "p99_latency_s": S.percentile(lat, 99), "tri_min": min(j[0] for j in jobs),
-            "tri_max": max(j[0] for j in jobs)}
+            "p99_latency_s": S.percentile(lat, 99), "mean_latency_s": float(np.mean(lat)),
+            "p50_wait_to_admission_s": S.percentile(queue, 50), "p99_wait_to_admission_s": S.percentile(queue, 99),
+            "peak_running": sched.peak_running, "merged_jobs": sched.merged_jobs, "slots": slots,
+            "frequency_batch": batch, "tri_min": min(j["triangles"] for j in stream),
+            "tri_max": max(j["triangles"] for j in stream)}
 
 
 def main():
@@ -286,9 +380,10 @@ def main():
     ap.add_argument("--batch", type=int, default=40)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-jobs", type=int, default=16)
+    ap.add_argument("--serving-jobs", type=int, default=200)
     ap.add_argument("--serving-rate", type=float, default=2.0)
-    ap.add_argument("--serving-batch", type=int, default=32, help="frequencies per device batch in the serving loop")
+    ap.add_argument("--serving-batch", type=int, default=40, help="frequencies per device batch in the serving loop")
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--fanout", type=int, default=4,
                     help="streams used by the batched LU and refinement (1 = single stream, e.g. for tracing)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -309,42 +404,103 @@ def main():
         run_scale(args)
         return
 
+    import tempfile
+
+    tmp = Path(tempfile.mkdtemp(prefix="beat-bench-"))
+    stream = None
+    if args.serving_jobs > 0:  # host-only mesh generation, before anything touches the GPU
+        log(f"bench: generating the serving stream ({args.serving_jobs} meshes)")
+        stream = serving_stream(args.serving_jobs, args.serving_rate, 2024, tmp)
+
     import torch
 
+    from beat_hip import _lib
+    from beat_hip import synthetic as syn
+    from beat_hip import worker as W
     from beat_hip.engine import Engine
 
     log("bench: building 6k-triangle horn-in-box job")
     prep, qunit, obs, freqs = horn_job()
-    eng = Engine(prep, qunit)
-    dev = eng.dev
-    _lib_mod = __import__("beat_hip._lib", fromlist=["check"])
-    _lib_mod.check(dev.lib.bem_ctx_set_fanout(dev.ctx, args.fanout), "bem_ctx_set_fanout")
-    nb = max(1, min(args.batch, len(freqs)))
+    V, F, T = syn.horn_in_box()
+    mesh_path = tmp / "horn6k.msh"
+    syn.write_gmsh22(mesh_path, V, F, T, scale=1.0)
+    dev = W.shared_device()
+    _lib.check(dev.lib.bem_ctx_set_fanout(dev.ctx, args.fanout), "bem_ctx_set_fanout")
+    F = prep.n_faces
 
-    # ---- mixed, batched, graph-captured -------------------------------------------------------
-    log(f"bench: mixed batched path, batch={nb}, capturing graph")
-    b = eng.batch("mixed", nb, obs, 1.205, 343.0)
-    b.capture()
+    # ---- warm worker: one small job of each precision loads the kernels --------------------------------------
+    Vs, Fs, Ts = syn.icosphere(2, 0.1)
+    warm_path = tmp / "warm.msh"
+    syn.write_gmsh22(warm_path, Vs, Fs, Ts, scale=1.0)
+    for precision in ("mixed", "fp64"):
+        warm = horn_request(warm_path, precision, [500.0, 1500.0, 2500.0])
+        warm["config"]["radiators"] = [{"name": "s", "tag": int(Ts[0]), "channel": "main"}]
+        run_request(warm)
+
+    # ---- headline: mixed precision, end to end through driver.run_job ------------------------------------------
+    req = horn_request(mesh_path, "mixed", freqs)
+    log(f"bench: mixed end to end, {args.warmup} warmup + {args.steps} timed requests")
     for _ in range(args.warmup):
-        [b.run(freqs[i : i + nb]) for i in range(0, len(freqs), nb)]
+        run_request(req)
     torch.cuda.synchronize()
+    walls, mixed_results = [], None
     t0 = time.perf_counter()
-    mixed = None
     for _ in range(args.steps):
-        mixed = [b.run(freqs[i : i + nb]) for i in range(0, len(freqs), nb)]
+        w, mixed_results = run_request(req)
+        walls.append(w)
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     ms_per_step = 1e3 * t_mixed / args.steps
     value = len(freqs) * args.steps / t_mixed
-    log(f"bench: mixed {value:.2f} freq-points/s ({ms_per_step:.1f} ms per 40-frequency job)")
-    max_iters = int(max(int(r.iters.max()) for r in mixed))
-    max_res = float(max(float(r.residual.max()) for r in mixed))
+    log(f"bench: mixed end to end {value:.2f} freq-points/s ({ms_per_step:.1f} ms per 40-frequency request)")
+
+    # ---- plain FP64 path, end to end, one frequency per launch ------------------------------------------------
+    fp64_value, db_err, ph_err, fp64_walls = None, None, None, []
+    if not args.no_fp64:
+        fsteps = max(1, min(args.steps, 2))
+        log(f"bench: plain FP64 end to end, {fsteps} requests")
+        req64 = horn_request(mesh_path, "fp64", freqs)
+        for _ in range(fsteps):
+            w, fp64_results = run_request(req64)
+            fp64_walls.append(w)
+        fp64_value = len(freqs) * fsteps / sum(fp64_walls)
+        db_err, ph_err = field_errors([wire_field(r) for r in mixed_results], [wire_field(r) for r in fp64_results])
+        log(f"bench: fp64 end to end {fp64_value:.2f} freq-points/s; mixed/fp64 = {value / fp64_value:.2f}x; "
+            f"max field error {db_err:.4f} dB / {ph_err:.3f} deg")
+    max_iters = max(int(r["diagnostics"]["refinement_iterations"]) for r in mixed_results)
+    max_res = max(float(r["diagnostics"]["relative_residual"]) for r in mixed_results)
+
+    # ---- steady state: the mixed batch alone, mesh resident, graph replay ---------------------------------------
+    eng = Engine(prep, qunit, dev=dev)
+    nb = max(1, min(args.batch, len(freqs)))
+    b = eng.batch("mixed", nb, obs, 1.205, 343.0)
+    b.capture()
+    for _ in range(max(1, args.warmup)):
+        [b.run(freqs[i : i + nb]) for i in range(0, len(freqs), nb)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        [b.run(freqs[i : i + nb]) for i in range(0, len(freqs), nb)]
+    torch.cuda.synchronize()
+    t_ss = time.perf_counter() - t0
+    steady_value = len(freqs) * args.steps / t_ss
     b.close()
+    fp64_steady = None
+    if not args.no_fp64:
+        b1 = eng.batch("fp64", 1, obs, 1.205, 343.0)
+        b1.capture()
+        b1.run(freqs[:1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(len(freqs)):
+            b1.run(freqs[i : i + 1])
+        torch.cuda.synchronize()
+        fp64_steady = len(freqs) / (time.perf_counter() - t0)
+        b1.close()
+    log(f"bench: steady state (graph replay, mesh resident): mixed {steady_value:.2f}"
+        + ("" if fp64_steady is None else f", fp64 {fp64_steady:.2f}") + " freq-points/s")
 
-    # ---- per-stage times of one eager job and the roofline of the regular assembly kernel -----------
-    # HIP events on the library's stream bracket every assembly launch (one region per kernel launch)
-    from beat_hip import _lib
-
+    # ---- per-stage times of one eager job and the roofline of the regular assembly kernel -----------------------
     be = eng.batch("mixed", nb, obs, 1.205, 343.0)
     be.run(freqs[:nb])
     dev.timers(True)
@@ -355,7 +511,6 @@ def main():
     be.close()
     t_ms, units, regions = stages["regular"]
     avg_ms = t_ms / max(1, regions)
-    F = prep.n_faces
     freqs_per_launch = units / max(1, regions)
     qpairs = (F * F - len(prep.pairs.test)) * 36 * freqs_per_launch
     achieved = qpairs * FLOPS_PER_QPAIR / (avg_ms * 1e-3) / 1e12
@@ -365,25 +520,6 @@ def main():
         f"{achieved:.1f} TFLOP/s of ordered-pair work ({executed:.1f} executed); stages (ms/job): "
         + ", ".join(f"{k} {v[0]:.1f}" for k, v in stages.items()))
 
-    # ---- plain FP64 path: one frequency per launch ------------------------------------------------
-    fp64_value, db_err, ph_err = None, None, None
-    if not args.no_fp64:
-        log("bench: plain FP64 path, one frequency per launch")
-        b1 = eng.batch("fp64", 1, obs, 1.205, 343.0)
-        b1.run(freqs[:1])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ref = [b1.run(freqs[i : i + 1]) for i in range(len(freqs))]
-        torch.cuda.synchronize()
-        t_fp64 = time.perf_counter() - t0
-        b1.close()
-        fp64_value = len(freqs) / t_fp64
-        fm = [r.field[j] for r in mixed for j in range(len(r.freqs))]
-        fr = [r.field[0] for r in ref]
-        db_err, ph_err = field_errors(fm, fr)
-        log(f"bench: fp64 {fp64_value:.2f} freq-points/s; mixed/fp64 = {value / fp64_value:.2f}x; "
-            f"max field error {db_err:.4f} dB / {ph_err:.3f} deg")
-
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline sample")
@@ -392,10 +528,11 @@ def main():
     eng.close()
 
     serving = None
-    if args.serving_jobs > 0:
-        log(f"bench: serving stream, {args.serving_jobs} jobs at {args.serving_rate}/s")
-        serving = serving_sample(args.serving_jobs, args.serving_rate, 2024, batch=args.serving_batch, slots=4)
-        log(f"bench: serving p50 {serving['p50_latency_s']:.2f} s p99 {serving['p99_latency_s']:.2f} s")
+    if stream is not None:
+        log(f"bench: serving stream, {len(stream)} jobs at {args.serving_rate}/s")
+        serving = serving_sample(stream, args.serving_rate, 2024, batch=args.serving_batch, slots=args.serving_slots)
+        log(f"bench: serving p50 {serving['p50_latency_s']:.2f} s p99 {serving['p99_latency_s']:.2f} s, "
+            f"{serving['throughput_freq_points_per_s']:.1f} freq-points/s")
 
     out = {
         "metric": METRIC,
@@ -410,17 +547,22 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64",
         "data": "synthetic",
-        "config": {"workload": "horn-in-box 6000 tris / 3002 P1 dofs, 40 log freqs 100 Hz-8 kHz, mixed precision "
-                               "batched (hipGraph), polar 2x73 pts at 2 m", "triangles": int(F),
-                   "p1_dofs": int(prep.n_vertices), "frequencies": int(len(freqs)), "batch": nb,
+        "config": {"workload": "horn-in-box 6000 tris / 3002 P1 dofs, 40 log freqs 100 Hz-8 kHz, mixed precision, "
+                               "end to end per request through the worker's job driver (Gmsh read, preparation, "
+                               "upload, device batches, result events), polar 2x73 pts at 2 m",
+                   "triangles": int(F), "p1_dofs": int(prep.n_vertices), "frequencies": int(len(freqs)),
                    "precision": "mixed"},
+        "request_latency_s": {"mean": round(float(np.mean(walls)), 4), "max": round(float(np.max(walls)), 4)},
         "fp64_gpu_value": None if fp64_value is None else round(fp64_value, 4),
         "mixed_over_fp64": None if fp64_value is None else round(value / fp64_value, 3),
+        "steady_state_value": round(steady_value, 4),
+        "steady_state_fp64_value": None if fp64_steady is None else round(fp64_steady, 4),
+        "steady_state_mixed_over_fp64": None if fp64_steady is None else round(steady_value / fp64_steady, 3),
         "max_field_error_db": db_err,
         "max_field_error_deg": ph_err,
         "max_refinement_iterations": max_iters,
         "max_relative_residual": max_res,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": "k_regular_pair<1>", "pipe": PIPE_NOTE, "avg_launch_ms": round(avg_ms, 4),
                      "freqs_per_launch": freqs_per_launch, "flops_per_qpair": FLOPS_PER_QPAIR,

@@ -570,8 +570,16 @@ def cancelled(job: Job) -> bool:
 
 # Frequencies per device batch: the LU's latency-bound panel chain is shared by the batch, so larger batches
 # factor more matrices per chain (serving stream of 16 mixed-size jobs: p50 / p99 latency 0.36 / 1.80 s at 8,
-# 0.26 / 1.33 s at 32).
-DEFAULT_FREQUENCY_BATCH = 32
+# 0.26 / 1.33 s at 32); 40 holds the whole 40-frequency sweep in one batch.
+DEFAULT_FREQUENCY_BATCH = 40
+
+
+def balanced_batch(n: int, nb_max: int) -> int:
+    """Batch size for n frequencies in batches of at most nb_max, as even as possible (a padded last batch
+    costs as much device work as a full one): 40 frequencies at most 32 per batch run as 2 x 20, not 32 + 8."""
+    n, nb_max = max(1, int(n)), max(1, int(nb_max))
+    batches = -(-n // nb_max)
+    return -(-n // batches)
 
 
 class JobFailed(RuntimeError):
@@ -606,7 +614,7 @@ def plan_batch(job: Job, eng, batch: int | None = None) -> int:
     if fit is not None:
         companions = ("fp64",) if (job.fp64_check and job.precision == "mixed") else ()
         nb = fit(job.precision, nb, 0 if job.obs is None else len(job.obs), companions=companions)
-    return nb
+    return balanced_batch(n, nb)
 
 
 @dataclass

@@ -72,6 +72,10 @@ def read_gmsh22(path, scale: float = 1.0, translation=(0.0, 0.0, 0.0)):
         es, ee = stripped.index("$Elements"), stripped.index("$EndElements")
     except ValueError as exc:
         raise ValueError("Only Gmsh 2.2 ASCII meshes with Nodes and Elements sections are supported.") from exc
+    fast = _gmsh22_blocks(lines[ns + 2 : ne], lines[es + 2 : ee], scale)
+    if fast is not None:
+        v, faces, tags = fast
+        return v + np.asarray(translation, dtype=np.float64), faces, tags
     node_index = {}
     coords = []
     for line in lines[ns + 2 : ne]:
@@ -94,6 +98,47 @@ def read_gmsh22(path, scale: float = 1.0, translation=(0.0, 0.0, 0.0)):
         tags.append(int(parts[3]))
     v = np.asarray(coords, dtype=np.float64).reshape(-1, 3) + np.asarray(translation, dtype=np.float64)
     return v, np.asarray(faces, dtype=np.int64).reshape(-1, 3), np.asarray(tags, dtype=np.int64)
+
+
+def _uniform_block(lines, dtype):
+    """The whitespace-separated numbers of `lines` as a (len(lines), k) array when every line holds k of them,
+    else None."""
+    if not lines:
+        return None
+    k = len(lines[0].split())
+    flat = np.fromstring(" ".join(lines), dtype=dtype, sep=" ")  # noqa: NPY201 (text mode)
+    if k == 0 or flat.size != k * len(lines):
+        return None
+    return flat.reshape(len(lines), k)
+
+
+def _gmsh22_blocks(node_lines, elem_lines, scale):
+    """Vectorised read of the Nodes and Elements blocks (the same semantics as the line loop of read_gmsh22:
+    triangles only, element nodes are the last three tokens, tag is the first tag token). None when a block has
+    lines of different lengths or an element names an unknown node; the caller then reads line by line."""
+    import warnings
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", DeprecationWarning)
+        nodes = _uniform_block(node_lines, np.float64)
+        elems = _uniform_block(elem_lines, np.int64) if elem_lines else np.zeros((0, 8), np.int64)
+    if nodes is None or elems is None or nodes.shape[1] != 4 or elems.shape[1] < 8:
+        return None
+    ids = nodes[:, 0].astype(np.int64)
+    if not np.array_equal(ids, nodes[:, 0]):
+        return None
+    tri = elems[elems[:, 1] == 2]
+    order = np.argsort(ids, kind="stable")
+    sorted_ids = ids[order]
+    if len(sorted_ids) > 1 and (np.diff(sorted_ids) == 0).any():
+        return None  # repeated node ids: the line loop keeps the last definition
+    ref = tri[:, -3:]
+    pos = np.searchsorted(sorted_ids, ref)
+    pos_c = np.minimum(pos, len(sorted_ids) - 1)
+    if len(ref) and not (sorted_ids[pos_c] == ref).all():
+        return None
+    faces = order[pos_c].reshape(-1, 3)
+    return nodes[:, 1:4] * scale, faces.astype(np.int64), tri[:, 3].astype(np.int64)
 
 
 def combine(parts) -> BoundaryMesh:
@@ -309,14 +354,26 @@ def _topological_candidates(mesh: BoundaryMesh):
     return t, s, match
 
 
+def _first_two_set_bits():
+    """For every 9-bit mask: the positions of its first and second set bit in scan order (0 when absent)."""
+    lut = np.zeros((512, 2), dtype=np.int64)
+    for mask in range(512):
+        bits = [b for b in range(9) if (mask >> b) & 1] + [0, 0]
+        lut[mask] = bits[:2]
+    return lut
+
+
+_FIRST_TWO = _first_two_set_bits()
+
+
 def _pair_codes(match):
     """Orientation codes of vertex-sharing pairs (BeatEngineCore.jl:453-507,820-844): kind 1 coincident,
     2 edge (test a, b; trial a, b after ordering the trial indices), 3 vertex (test a, trial a)."""
     nshared = match.sum(axis=(1, 2))
     kinds = np.where(nshared >= 3, 1, np.where(nshared == 2, 2, 3))
     # edge orientation: list matches in (i, j) scan order, then sort the pair by trial index j
-    flat = match.reshape(len(match), 9)
-    order_ij = np.argsort(~flat, axis=1, kind="stable")[:, :2]  # first two matches in scan order
+    mask = match.reshape(len(match), 9).astype(np.int64) @ (1 << np.arange(9, dtype=np.int64))
+    order_ij = _FIRST_TWO[mask]  # first two matches in scan order
     ia, ja = order_ij[:, 0] // 3, order_ij[:, 0] % 3
     ib, jb = order_ij[:, 1] // 3, order_ij[:, 1] % 3
     swap = jb < ja

@@ -13,7 +13,9 @@ src/blab/server.py:154-185). Here one process-wide scheduler owns the GPU:
   ``max_bypass`` times nothing behind it is admitted until it is (no starvation of large meshes).
 * Each admitted job runs on its own slot (HIP stream + context, one stream per slot so that the slots map onto
   the device's hardware queues), so kernels of different jobs overlap on the device; its mesh, singular tables
-  and factor buffers stay resident between its frequency batches.
+  and factor buffers stay resident between its frequency batches. Slots launch their batches eagerly: a stream
+  capture in one slot thread was invalidated by the allocations and copies the other slots make meanwhile
+  (the worker path, one device thread, captures its multi-batch jobs).
 * Jobs whose mesh, drives, observation points, medium and precision are identical (``mesh_key``) and that are
   waiting at the same time are merged: one device mesh, their frequencies concatenated into shared device
   batches (fuller batches amortise the LU's latency-bound panel chain), each job's results routed back to it.
@@ -159,7 +161,7 @@ class _Pending:
 class Scheduler:
     def __init__(self, hbm_budget_bytes: int | None = None, slots: int = 4, batch: int = D.DEFAULT_FREQUENCY_BATCH,
                  device_index: int = 0, engine_factory=None, artifact_root=None, prep_workers: int = 2,
-                 max_bypass: int = 4, merge: bool = True, fanout: int = 1, graph: bool = True):
+                 max_bypass: int = 4, merge: bool = True, fanout: int = 1, graph: bool = False):
         from concurrent.futures import ThreadPoolExecutor
 
         self.artifact_root = None if artifact_root is None else Path(artifact_root)
@@ -229,6 +231,7 @@ class Scheduler:
             budget = self.budget() if self._engine_factory is None else (self._budget or 1 << 62)
             while nb > 1 and self._footprint(job, prep, nb) > budget:
                 nb -= 1
+            nb = D.balanced_batch(len(job.frequencies), nb)
             h.footprint = self._footprint(job, prep, nb)
             h.key = mesh_key(job, prep) if self.merge else None
         except Exception as exc:
@@ -297,7 +300,7 @@ class Scheduler:
                     while nb > lead.nb and (self._footprint(lead.job, lead.prep, nb) > budget or (
                             self._running and self._used + self._footprint(lead.job, lead.prep, nb) > budget)):
                         nb -= 1
-                    lead.nb = nb
+                    lead.nb = D.balanced_batch(total, nb)
                     lead.h.footprint = self._footprint(lead.job, lead.prep, nb)
                 slot = free_slots.pop(0)
                 self._used += lead.h.footprint

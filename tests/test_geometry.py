@@ -352,3 +352,34 @@ def test_paired_strips_reproduce_dense_assembly(pend_slots):
             ok = s.vertex_last >= cs_col[r]
             A[ok, i] += CS[r, ok]
     assert np.allclose(A, dense, rtol=1e-12, atol=1e-12)
+
+
+def test_gmsh_block_reader_matches_line_reader(tmp_path):
+    # the vectorised reader and the line loop give the same mesh; files it cannot take as uniform blocks (mixed
+    # element lines, sparse node ids with unknown references) fall back to the loop with the same result
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+
+    V, F, T = syn.horn_in_box(h=0.05)
+    p = tmp_path / "h.msh"
+    syn.write_gmsh22(p, V, F, T, scale=0.001)
+    fast = G.read_gmsh22(p, 0.001, (0.1, 0.0, -0.2))
+    assert G._gmsh22_blocks is not None
+    mixed = tmp_path / "mixed.msh"
+    mixed.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n5\n10 0 0 0\n20 1 0 0\n30 0 1 0\n40 0 0 1\n"
+                     "50 9 9 9\n$EndNodes\n$Elements\n5\n1 15 2 0 0 50\n2 1 2 7 7 10 20\n3 2 2 3 3 10 30 20\n"
+                     "4 2 3 4 4 9 10 20 40\n5 2 2 5 5 20 30 40\n$EndElements\n")
+    vm, fm, tm = G.read_gmsh22(mixed)
+    assert fm.tolist() == [[0, 2, 1], [0, 1, 3], [1, 2, 3]] and tm.tolist() == [3, 4, 5]
+    assert np.allclose(vm[4], [9, 9, 9])
+    orig = G._gmsh22_blocks
+    try:
+        G._gmsh22_blocks = lambda *a: None
+        loop = G.read_gmsh22(p, 0.001, (0.1, 0.0, -0.2))
+        loop_mixed = G.read_gmsh22(mixed)
+    finally:
+        G._gmsh22_blocks = orig
+    for a, b in zip(fast, loop):
+        assert np.array_equal(a, b)
+    for a, b in zip((vm, fm, tm), loop_mixed):
+        assert np.array_equal(a, b)

@@ -250,8 +250,8 @@ def test_identical_meshes_merge_into_shared_batches(tmp_path):
         assert e[-1]["type"] == "completed"
         assert [x["freq_hz"] for x in res] == [float(np.float32(f)) for f in r["frequencies_hz"]]
         assert [x["solved_count"] for x in e if x["type"] == "result"] == list(range(1, len(res) + 1))
-    # the 9 frequencies of the merged jobs ran as 8 + 1, not 3 + 4 + 2
-    assert sorted(len(b) for b in RecordingEngine.batches[-2:]) == [1, 8]
+    # the 9 frequencies of the merged jobs ran in two balanced batches (5 + 4), not as 3 + 4 + 2
+    assert sorted(len(b) for b in RecordingEngine.batches[-2:]) == [4, 5]
 
 
 def test_bypass_is_bounded(tmp_path):

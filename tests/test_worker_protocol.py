@@ -330,3 +330,8 @@ def test_fit_batch_counts_the_fp64_companion():
     need = sum(estimate_footprint(prep.n_vertices, prep.n_faces, len(prep.pairs.test), with_ref, 146, p, 1)
                for p in ("mixed", "fp64"))
     assert need <= 0.9 * avail
+
+
+def test_balanced_batches():
+    assert D.balanced_batch(40, 32) == 20 and D.balanced_batch(40, 40) == 40 and D.balanced_batch(41, 40) == 21
+    assert D.balanced_batch(7, 3) == 3 and D.balanced_batch(1, 8) == 1 and D.balanced_batch(60, 40) == 30
