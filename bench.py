@@ -75,28 +75,34 @@ def measured_traffic():
         return None
 
 
-def traffic_probe():
-    """Workload for the two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE): a 1 GiB device-to-device copy
-    (calibration: the copy kernel fetches and writes exactly 1 GiB) followed by one eager assembly of 4
-    frequencies of the horn job (one k_regular_pair<1> launch)."""
+def traffic_probe(workload="horn"):
+    """Workload of the rocprofv3 counter passes: a 1 GiB device-to-device copy (calibration: the copy kernel
+    fetches and writes exactly 1 GiB), then one eager mixed-precision job on one stream (every kernel of the path:
+    assembly, split-bf16 LU, refinement, field) and, for the horn job, one plain FP64 frequency. The first
+    k_regular_pair<1> launch of the horn job is the one-launch unit of the committed traffic summary."""
     import torch
 
+    from beat_hip import _lib
     from beat_hip.engine import Engine
 
-    prep, qunit, obs, freqs = horn_job()
+    prep, qunit, obs, freqs = horn_job() if workload == "horn" else scale_job()
     eng = Engine(prep, qunit)
+    _lib.check(eng.dev.lib.bem_ctx_set_fanout(eng.dev.ctx, 1), "bem_ctx_set_fanout")
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    b = eng.batch("mixed", 4, obs, 1.205, 343.0)
-    b.set_frequencies(freqs[-4:])
-    with torch.cuda.stream(eng.dev.stream):
-        b._assemble()
-    eng.dev.sync()
+    b = eng.batch("mixed", len(freqs), obs, 1.205, 343.0)
+    r = b.run(freqs)
     b.close()
+    if workload == "horn":
+        b1 = eng.batch("fp64", 1, obs, 1.205, 343.0)
+        b1.run(freqs[20:21])
+        b1.close()
+    eng.dev.sync()
     eng.close()
-    log("probe: copy 1 GiB, then one 4-frequency assembly of the 6k-triangle horn job")
+    log(f"probe: copy 1 GiB, then one eager mixed {workload} job of {len(freqs)} frequencies "
+        f"(max residual {r.residual.max():.1e})" + (" and one FP64 frequency" if workload == "horn" else ""))
 
 
 def log(msg):
@@ -387,7 +393,8 @@ def main():
     ap.add_argument("--fanout", type=int, default=4,
                     help="streams used by the batched LU and refinement (1 = single stream, e.g. for tracing)")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="only run the workload of the HBM-counter passes (copy calibration + one assembly)")
+                    help="only run the workload of the counter passes (copy calibration + one eager job of the "
+                         "chosen --workload)")
     ap.add_argument("--workload", choices=["horn", "scale"], default="horn",
                     help="horn: the 6k-triangle horn-in-box job (headline); scale: the 20k-triangle two-way "
                          "loudspeaker job")
@@ -398,7 +405,7 @@ def main():
         sys.exit(2)
 
     if args.traffic_probe:
-        traffic_probe()
+        traffic_probe(args.workload)
         return
     if args.workload == "scale":
         run_scale(args)
