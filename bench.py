@@ -105,6 +105,46 @@ def traffic_probe(workload="horn"):
         f"(max residual {r.residual.max():.1e})" + (" and one FP64 frequency" if workload == "horn" else ""))
 
 
+def field_probe():
+    """Workload of the field-kernel counter passes at the field shape of the 20k-triangle loudspeaker job: its
+    mesh resident, 24 wavenumbers x 3 sources x the 5,000-point Fibonacci sphere at 2 m, boundary pressure of
+    random values (the field kernels' work does not depend on them), one bem_field call after the 1 GiB copy
+    calibration. Each step prints before it runs."""
+    import ctypes as C
+
+    import torch
+
+    from beat_hip import _lib
+    from beat_hip.engine import PRECISIONS, DeviceMesh, Device, _lda, _ptr
+
+    log("probe: building the 20k-triangle loudspeaker mesh")
+    prep, qunit, obs, freqs = scale_job()
+    dev = Device(0)
+    log("probe: uploading the mesh")
+    src = np.nonzero(np.any(prep.permute_faces(qunit) != 0, axis=1))[0]
+    dm = DeviceMesh(dev, prep, src)
+    nf, nch, F, V = len(freqs), qunit.shape[1], prep.n_faces, prep.n_vertices
+    ld = _lda(V)
+    rng = np.random.default_rng(3)
+    with torch.cuda.stream(dev.stream):
+        k = torch.from_numpy(2 * np.pi * freqs / 343.0).to(dev.device)
+        p = torch.from_numpy(rng.standard_normal((nf, nch, ld)) + 1j * rng.standard_normal((nf, nch, ld))).to(dev.device)
+        q = torch.from_numpy(rng.standard_normal((nf, nch, F)) + 1j * rng.standard_normal((nf, nch, F))).to(dev.device)
+        ob = torch.from_numpy(np.ascontiguousarray(obs)).to(dev.device)
+        out = torch.zeros((nf, nch, len(obs)), dtype=torch.complex128, device=dev.device)
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev.device)
+        b = torch.empty_like(a)
+        b.copy_(a)
+    dev.sync()
+    log(f"probe: bem_field, {nf} wavenumbers x {nch} sources x {len(obs)} points, {6 * F} sources")
+    _lib.check(dev.lib.bem_field(dev.ctx, dm.handle, PRECISIONS["mixed"], nf, _ptr(k), len(obs), _ptr(ob), _ptr(p),
+                                 ld, nch * ld, _ptr(q), nch * F, nch, _ptr(out), len(obs), nch * len(obs)), "bem_field")
+    dev.sync()
+    log("probe: field done")
+    dm.close()
+    dev.close()
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -395,6 +435,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the workload of the counter passes (copy calibration + one eager job of the "
                          "chosen --workload)")
+    ap.add_argument("--probe", choices=["job", "field"], default="job",
+                    help="--traffic-probe workload: one eager job of --workload, or the field evaluation of the "
+                         "20k-triangle job alone")
     ap.add_argument("--workload", choices=["horn", "scale"], default="horn",
                     help="horn: the 6k-triangle horn-in-box job (headline); scale: the 20k-triangle two-way "
                          "loudspeaker job")
@@ -405,7 +448,7 @@ def main():
         sys.exit(2)
 
     if args.traffic_probe:
-        traffic_probe(args.workload)
+        field_probe() if args.probe == "field" else traffic_probe(args.workload)
         return
     if args.workload == "scale":
         run_scale(args)

@@ -422,7 +422,11 @@ def box_with_pistons(h: float = 0.02, seed: int = 99, jitter: float = 0.05, widt
 def mesh_for_triangle_count(kind: str, target: int, seed: int = 0):
     """Generate ``kind`` ('sphere'|'horn'|'pistons') near ``target`` triangles (edge length by area scaling)."""
     if kind == "sphere":
+        # icospheres have 20 x 4^n triangles: the level nearest the target in log scale, one level lower when
+        # that one exceeds the target by more than 30 % (a 15,000 target gives 5,120, not 20,480)
         n = max(1, int(round(math.log(max(target, 20) / 20.0, 4))))
+        if n > 1 and 20 * 4 ** n > 1.3 * target:
+            n -= 1
         return icosphere(n, 0.1)
     gen = {"horn": horn_in_box, "pistons": box_with_pistons}[kind]
     h = 0.03
