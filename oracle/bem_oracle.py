@@ -99,19 +99,6 @@ def load_gmsh22(path: str, scale: float = 1.0, translation=(0.0, 0.0, 0.0)):
     return v, np.asarray(faces, dtype=np.int64).reshape(-1, 3), np.asarray(tags, dtype=np.int64)
 
 
-def combine_meshes(parts):
-    """Concatenate meshes without welding (solver.jl:252-269). parts = [(V, F, tags), ...]."""
-    vs, fs, ts, ids = [], [], [], []
-    offset = 0
-    for mesh_id, (v, f, t) in enumerate(parts, start=1):
-        vs.append(v)
-        fs.append(f + offset)
-        ts.append(t)
-        ids.append(np.full(len(f), mesh_id, dtype=np.int64))
-        offset += len(v)
-    return Mesh(np.concatenate(vs), np.concatenate(fs), np.concatenate(ts), np.concatenate(ids))
-
-
 # --------------------------------------------------------------------------------------
 # Quadrature (BeatEngineCore.jl:517-655)
 # --------------------------------------------------------------------------------------
@@ -843,19 +830,6 @@ def fibonacci_observation(config):
     phi = np.mod(np.arctan2(y, x), np.float32(2 * np.pi)).astype(np.float32)
     return np.stack([x, y, zz], axis=1), {"r_distance_m": np.full(n, d, np.float32), "theta_polar_rad": theta,
                                            "phi_azimuth_rad": phi}
-
-
-def pressure_to_spl(p):
-    return (20.0 * np.log10(np.abs(p) / 20e-6)).astype(np.float32)
-
-
-def neumann_data(mesh: Mesh, radiator_tags, drives, rho: float, omega: float):
-    """q_e = i rho omega drive on radiator faces (solver.jl:434-458). radiator_tags=[(tag, mesh_id)]."""
-    q = np.zeros(mesh.n_faces, complex)
-    for (tag, mesh_id), drive in zip(radiator_tags, drives):
-        sel = (mesh.tags == tag) & (mesh.mesh_ids == mesh_id)
-        q[sel] = 1j * rho * omega * drive
-    return q
 
 
 def solve_frequency(mesh: Mesh, freq: float, q_unit, sound_speed=343.0, rho=1.21, singular_cache=None,
