@@ -75,6 +75,12 @@ class MeshDesc(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """bem_tuning: launch-shape choices of a context (see include/bemhip.h)."""
+
+    _fields_ = [("split_tile", _i), ("trsv_wg_min_batch", _i), ("lookahead_wgs", _i), ("field_slice_budget", _i)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "bem_version": (_i, []),
@@ -85,6 +91,8 @@ SIGNATURES = {
     "bem_ctx_stream": (_i, [_p, C.POINTER(_p)]),
     "bem_ctx_set_fanout": (_i, [_p, _i]),
     "bem_ctx_set_lu_products": (_i, [_p, _i]),
+    "bem_ctx_set_tuning": (_i, [_p, C.c_void_p]),
+    "bem_ctx_get_tuning": (_i, [_p, C.c_void_p]),
     "bem_ctx_timers": (_i, [_p, _i]),
     "bem_ctx_timer_read": (_i, [_p, _i, C.POINTER(_d), C.POINTER(_ll), C.POINTER(_ll)]),
     "bem_mesh_validate": (_i, [C.POINTER(MeshDesc)]),

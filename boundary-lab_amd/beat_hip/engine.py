@@ -220,7 +220,9 @@ class Batch:
     solves and field run per group of at most four channels on the same factors."""
 
     def __init__(self, dmesh: DeviceMesh, precision: str, nf: int, qunit: np.ndarray, obs: np.ndarray | None,
-                 rho: float, sound_speed: float, tol: float = 1e-10, max_iter: int = 4, strip_nf: int = 4):
+                 rho: float, sound_speed: float, tol: float = 1e-10, max_iter: int = 4, strip_nf: int | None = None):
+        from . import tuning as TU
+
         dev = dmesh.dev
         torch = dev.torch
         self.dmesh, self.dev, self.precision, self.nf = dmesh, dev, precision, nf
@@ -228,6 +230,12 @@ class Batch:
         self.rho, self.c = float(rho), float(sound_speed)
         self.tol, self.max_iter = float(tol), int(max_iter)
         N, F = dmesh.V, dmesh.F
+        # launch shapes of the tuner's table for this shape (library defaults where it has none)
+        tab = TU.table()
+        self.lu_choice = tab.lu_choice(precision, N, nf)
+        self.asm_choice = tab.assembly_choice(precision, F)
+        if strip_nf is None:
+            strip_nf = self.asm_choice.strip_nf
         qunit = np.asarray(qunit, dtype=np.complex128).reshape(F, -1)
         self.n_ch = qunit.shape[1]
         if self.n_ch < 1:
@@ -272,6 +280,7 @@ class Batch:
         strip_len = ((N + 63) // 64) * 64
         per_freq = 2 * 3 * F * strip_len * (8 if precision == "mixed" else 16)
         strip_nf = max(1, min(strip_nf, nf, int(0.9 * free) // per_freq))
+        self.strip_nf = strip_nf
         dmesh.reserve(precision, nf, strip_nf, self.n_obs, min(self.n_ch, G))
         self.graphs = None
 
@@ -279,6 +288,9 @@ class Batch:
     def _assemble(self):
         lib, dev, m = self.dev.lib, self.dev, self.dmesh
         sB = self.n_ch * self.ldb
+        # the mesh's strip grouping is whatever the last reservation set (another batch on the same mesh, e.g. a
+        # one-frequency retry, may have changed it): this batch's own reservation, host-only once buffers fit
+        m.reserve(self.precision, self.nf, self.strip_nf, self.n_obs, min(self.n_ch, _lib.BEM_MAX_CHANNELS))
         _lib.check(lib.bem_neumann(dev.ctx, self.nf, self.n_ch, self.F, _ptr(self.rhs_scale), _ptr(self.qunit),
                                    _ptr(self.q)), "bem_neumann")
         c0, nc = self.groups[0]
@@ -334,12 +346,18 @@ class Batch:
     def _stages(self):
         return (self._assemble, self._factor_solve, self._field)
 
+    def _apply_tuning(self):
+        c = self.lu_choice
+        t = _lib.Tuning(c.split_tile, c.trsv_wg_min_batch, c.lookahead_wgs, self.asm_choice.field_slice_budget)
+        _lib.check(self.dev.lib.bem_ctx_set_tuning(self.dev.ctx, C.byref(t)), "bem_ctx_set_tuning")
+
     def capture(self, warm: bool = True):
         """Capture the launch sequence once, one graph per stage (assembly, factor + solve, field) so that
         replays keep per-stage timings; later batches replay them with new wavenumbers. `warm` runs the sequence
         eagerly first (kernel code objects loaded before the capture)."""
         torch = self.dev.torch
         graphs = []
+        self._apply_tuning()
         try:
             with torch.cuda.stream(self.dev.stream):
                 if warm:
@@ -381,6 +399,8 @@ class Batch:
         """Enqueue the whole sequence for up to nf frequencies (no wait); returns the stage events."""
         torch = self.dev.torch
         self.set_frequencies(freqs)
+        if self.graphs is None:
+            self._apply_tuning()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         with torch.cuda.stream(self.dev.stream):
             ev[0].record(self.dev.stream)

@@ -76,6 +76,7 @@ struct bem_ctx {
   int lu_products;      // BEM_LU_FP32 or BEM_LU_SPLIT_BF16 for the large updates of FP32 factorizations
   int num_cus;          // compute units of the device
   std::vector<bem_mesh*> capture_meshes;  // meshes used by the sequence being captured
+  bem_tuning tune;                        // launch-shape choices (bem_ctx_set_tuning)
   // bem_solve_status: device flags and their host copy
   int* status_dev = nullptr;
   int status_cap = 0;
@@ -236,6 +237,7 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   c->timing = false;
   c->fanout = 1 + BEM_AUX_STREAMS;
   c->lu_products = BEM_LU_FP32;
+  c->tune = bem_tuning{0, BEM_TSW_MIN_BATCH_DEFAULT, 0, 0};
   c->num_cus = 0;
   if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       c->num_cus <= 0)
@@ -311,6 +313,23 @@ int bem_ctx_set_lu_products(bem_ctx* c, int mode) {
   if (!c) return fail(BEM_E_ARG, "ctx is NULL");
   if (mode != BEM_LU_FP32 && mode != BEM_LU_SPLIT_BF16) return fail(BEM_E_ARG, "unknown LU product mode");
   c->lu_products = mode;
+  return BEM_OK;
+}
+
+int bem_ctx_set_tuning(bem_ctx* c, const bem_tuning* t) {
+  if (!c || !t) return fail(BEM_E_ARG, "NULL argument");
+  if (t->split_tile < 0 || t->split_tile > 1) return fail(BEM_E_ARG, "split_tile must be 0 or 1");
+  if (t->trsv_wg_min_batch < 0) return fail(BEM_E_ARG, "trsv_wg_min_batch must be >= 0");
+  if (t->lookahead_wgs < -1) return fail(BEM_E_ARG, "lookahead_wgs must be >= -1");
+  if (t->field_slice_budget < 0) return fail(BEM_E_ARG, "field_slice_budget must be >= 0");
+  if (c->capturing) return fail(BEM_E_STATE, "cannot change the tuning during graph capture");
+  c->tune = *t;
+  return BEM_OK;
+}
+
+int bem_ctx_get_tuning(bem_ctx* c, bem_tuning* t) {
+  if (!c || !t) return fail(BEM_E_ARG, "NULL argument");
+  *t = c->tune;
   return BEM_OK;
 }
 
@@ -658,7 +677,8 @@ int bem_mesh_destroy(bem_mesh* m) {
 
 static int field_slices(const bem_mesh* m, int n_obs, int nf) {
   const int obs_blocks = (n_obs + 63) / 64;
-  int s = 2048 / std::max(1, obs_blocks * nf);
+  const int budget = m->ctx->tune.field_slice_budget > 0 ? m->ctx->tune.field_slice_budget : 2048;
+  int s = budget / std::max(1, obs_blocks * nf);
   s = std::max(1, std::min(s, (m->M + 255) / 256));
   return s;
 }
@@ -684,7 +704,8 @@ static int reserve_impl(bem_mesh* m, int precision, int nf, int strip_nf, int n_
 
 int bem_mesh_reserve(bem_mesh* m, int precision, int nf, int strip_nf, int n_obs, int n_ch) {
   if (!m || nf <= 0 || n_ch < 1 || n_ch > BEM_MAX_CHANNELS) return fail(BEM_E_ARG, "invalid reserve arguments");
-  HIPCHK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  // during capture nothing may be allocated (dev_alloc refuses): only the strip grouping is set
+  if (!m->ctx->capturing) HIPCHK(hipSetDevice(m->ctx->device), "hipSetDevice");
   return reserve_impl(m, precision, nf, strip_nf, n_obs, n_ch);
 }
 
@@ -963,8 +984,10 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
   // auxiliary stream the large trailing updates (fork and join with the context stream only, as everywhere
   // else, so the sequence is capturable). Fan-out 1 runs everything on the context stream.
   // the look-ahead update keeps one CU per matrix free for the panel chain that runs beside it
-  const LuLookahead la = {c->fanout >= 2 ? c->aux[0] : nullptr, c->lu_ev[0], c->lu_ev[1],
-                          std::max(c->num_cus / 2, c->num_cus - batch)};
+  const int la_wgs = c->tune.lookahead_wgs > 0 ? c->tune.lookahead_wgs
+                     : c->tune.lookahead_wgs < 0 ? 0 : std::max(c->num_cus / 2, c->num_cus - batch);
+  const LuLookahead la = {c->fanout >= 2 ? c->aux[0] : nullptr, c->lu_ev[0], c->lu_ev[1], la_wgs,
+                          c->tune.split_tile};
   hipError_t e;
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
@@ -989,11 +1012,12 @@ int bem_lu_solve(bem_ctx* c, int precision, int n, int batch, const void* LU, lo
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {(cplx<float>*)LU, lda, strideA, n};
     e = lu_solve_batch<float>(mb, batch, perm, strideP, (const cplx<double>*)b, ldb, strideB, (cplx<float>*)y, ldy,
-                              strideY, nrhs, skip, (const cplx<float>*)work, c->stream);
+                              strideY, nrhs, skip, (const cplx<float>*)work, c->stream, c->tune.trsv_wg_min_batch);
   } else if (precision == BEM_PREC_FP64) {
     MatBatch<double> mb = {(cplx<double>*)LU, lda, strideA, n};
     e = lu_solve_batch<double>(mb, batch, perm, strideP, (const cplx<double>*)b, ldb, strideB, (cplx<double>*)y,
-                               ldy, strideY, nrhs, skip, (const cplx<double>*)work, c->stream);
+                               ldy, strideY, nrhs, skip, (const cplx<double>*)work, c->stream,
+                               c->tune.trsv_wg_min_batch);
   } else {
     return fail(BEM_E_ARG, "unknown precision");
   }
@@ -1023,14 +1047,16 @@ static int refine_impl(bem_ctx* c, int n, int batch, const void* LU, long long l
     hipError_t e2;
     // done[] = 0 with a fill kernel (no memset nodes: this sequence may be captured into a graph)
     if ((e2 = fill_int(dn, nb, 0, st))) return e2;
-    if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, ww, st)))
+    if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, bb, ldb, strideB, yy, ldb, strideB, nrhs, nullptr, ww, st,
+                                    c->tune.trsv_wg_min_batch)))
       return e2;
     if ((e2 = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 1, nullptr, nb, st))) return e2;
     for (int it = 0; it <= max_iter; ++it) {
       if ((e2 = residual_batch<float>(a0, nb, bb, xx, rr, ldb, strideB, nrhs, dn, st))) return e2;
       if ((e2 = resnorm_batch(rr, bb, ldb, strideB, n, nrhs, tol, rel + b0, dn, it, iters + b0, nb, st))) return e2;
       if (it == max_iter) break;
-      if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, dn, ww, st)))
+      if ((e2 = lu_solve_batch<float>(lu, nb, pm, strideP, rr, ldb, strideB, yy, ldb, strideB, nrhs, dn, ww, st,
+                                      c->tune.trsv_wg_min_batch)))
         return e2;
       if ((e2 = update_batch<float>(xx, yy, ldb, ldb, strideB, strideB, n, nrhs, 0, dn, nb, st))) return e2;
     }

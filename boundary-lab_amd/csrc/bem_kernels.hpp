@@ -148,15 +148,19 @@ long long lu_workspace_elems(int N);
 struct LuLookahead {
   hipStream_t stream;
   hipEvent_t ready, done;
-  int gemm_wgs;  // cap on the workgroups of the look-ahead update (0: none)
+  int gemm_wgs;     // cap on the workgroups of the look-ahead update (0: none)
+  int split_tile;   // 0: 256 x 128 tiles of 16 waves for the large split-bf16 updates, 1: 128 x 64 tiles everywhere
 };
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,
                            const int* skip, cplx<T>* W, hipStream_t, const LuLookahead* la, bool split_bf16);
+// Triangular solves: one workgroup per matrix when N <= 4096 and batch >= wg_min_batch (0: never), else the
+// blocked multi-workgroup sweeps.
+constexpr int BEM_TSW_MIN_BATCH_DEFAULT = 8;
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>&, int batch, const int* perm, long long strideP, const cplx<double>* b,
                           long long ldb, long long strideB, cplx<T>* y, long long ldy, long long strideY, int nrhs,
-                          const int* skip, const cplx<T>* W, hipStream_t);
+                          const int* skip, const cplx<T>* W, hipStream_t, int wg_min_batch = BEM_TSW_MIN_BATCH_DEFAULT);
 template <typename T>
 hipError_t residual_batch(const MatBatch<T>&, int batch, const cplx<double>* b, const cplx<double>* x,
                           cplx<double>* r, long long ldv, long long strideV, int nrhs, const int* skip, hipStream_t);

@@ -1286,7 +1286,7 @@ __global__ __launch_bounds__(1024) void k_trsv_wg(MatBatch<T> mb, const cplx<dou
 // no rows left to update, writes its own block too.
 constexpr int TSW_ROWS = 256;
 // one workgroup per matrix solves when N <= TSW_MIN_N and at least TSW_MIN_BATCH matrices share the launch
-constexpr int TSW_MIN_N = 4096, TSW_MIN_BATCH = 8;
+constexpr int TSW_MIN_N = 4096;
 
 template <typename T, int NR>
 __global__ __launch_bounds__(256) void k_perm_rhs(const cplx<double>* b, long long ldb, long long strideB,
@@ -1605,7 +1605,7 @@ int panel_width(int N, int c0) {
 // workgroups of a large split update (persistent tiles)
 template <typename T>
 static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape& g, const int* skip,
-                              hipStream_t st, bool split = false, int max_wgs = 0) {
+                              hipStream_t st, bool split = false, int max_wgs = 0, int split_tile = 0) {
   if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
   if (g.Nc <= 16 && g.K <= BEM_NBO) {
     // as many accumulator columns as the update has (the narrow panels of large N update 4 or 8 columns)
@@ -1620,7 +1620,7 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
   }
   if constexpr (sizeof(T) == 4) {
     if (split) {
-      if (g.M >= 512 && g.Nc >= 256) {
+      if (split_tile == 0 && g.M >= 512 && g.Nc >= 256) {
         const int nx = (g.Nc + 127) / 128, ny = (g.M + 255) / 256, nt = nx * ny * batch;
         const int nwg = max_wgs > 0 ? min(nt, max_wgs) : nt;
         hipLaunchKernelGGL((k_gemm_bf16x3<4, 4>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
@@ -1678,6 +1678,7 @@ static hipError_t solve_upper_rows(const MatBatch<T>& mb, int batch, int r0, int
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* perm, long long strideP, int* info,
                            const int* skip, cplx<T>* W, hipStream_t st, const LuLookahead* la, bool split_bf16) {
+  const int tile = la ? la->split_tile : 0;
   const int N = mb.N;
   const bool split = sizeof(T) == 4 && split_bf16;  // three bf16 products per FP32 product in the large updates
   const long long strideW = lu_workspace_elems(N);
@@ -1693,7 +1694,7 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
         // second half: rows [k0, h0) become U with the first half's L, the rows below take the K = 64 update
         CHK(solve_upper_rows(mb, batch, k0, h0, h1 - h0, W, strideW, skip, st));
         GemmShape g = {h0, k0, k0, h0, h0, h0, N - h0, h1 - h0, h0 - k0};
-        CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+        CHK(launch_gemm<T>(mb, batch, g, skip, st, split, 0, tile));
       }
       int c0 = h0;
       while (c0 < h1) {
@@ -1734,7 +1735,7 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
     if (b1 - b0 > NBO) {
       const int h0 = b0 + NBO;
       GemmShape g = {h0, b0, b0, c, h0, c, b1 - h0, ce - c, NBO};
-      CHK(launch_gemm<T>(mb, batch, g, skip, s, split));
+      CHK(launch_gemm<T>(mb, batch, g, skip, s, split, 0, tile));
       CHK(solve_upper_rows(mb, batch, h0, c, ce - c, W, strideW, skip, s));
     }
     return hipSuccess;
@@ -1774,7 +1775,7 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
     CHK(solve_u12(k0, k1, k1, ce, st));
     {
       GemmShape g = {k1, k0, k0, k1, k1, k1, N - k1, k2 - k1, k1 - k0};  // P's update of Q's columns
-      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split, 0, tile));
     }
     CHK(factor_block(k1, k2));
     if (k2 >= N) {  // Q is the last block: its interchanges at the top of the loop, then done
@@ -1788,23 +1789,23 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
       CHK(hipStreamWaitEvent(as, la->ready, 0));
       CHK(laswp_cols(k1, k2, k3, N, as));
       GemmShape g = {k1, k0, k0, k3, k1, k3, k2 - k1, N - k3, k1 - k0};
-      CHK(launch_gemm<T>(mb, batch, g, skip, as, split));
+      CHK(launch_gemm<T>(mb, batch, g, skip, as, split, 0, tile));
       CHK(solve_u12(k1, k2, k3, N, as));
       CHK(laswp_cols(k1, k2, 0, k0, as));
     }
     {
       GemmShape g = {k1, k0, k0, k2, k1, k2, k2 - k1, ce - k2, k1 - k0};  // P's update of Q's rows beyond Q
-      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split, 0, tile));
     }
     CHK(solve_u12(k1, k2, k2, ce, st));
     // P's and Q's updates of the rows and columns beyond Q in one pass, K = k2 - k0
     {
       GemmShape g = {k2, k0, k0, k2, k2, k2, N - k2, ce - k2, k2 - k0};
-      CHK(launch_gemm<T>(mb, batch, g, skip, st, split));
+      CHK(launch_gemm<T>(mb, batch, g, skip, st, split, 0, tile));
     }
     if (far) {
       GemmShape gr = {k2, k0, k0, k3, k2, k3, N - k2, N - k3, k2 - k0};
-      CHK(launch_gemm<T>(mb, batch, gr, skip, as, split, la->gemm_wgs));
+      CHK(launch_gemm<T>(mb, batch, gr, skip, as, split, la->gemm_wgs, tile));
       CHK(hipEventRecord(la->done, as));
       pending = true;
     }
@@ -1824,9 +1825,9 @@ template <typename T, int NR>
 static hipError_t solve_sweeps(const MatBatch<T>& mb, int batch, const int* perm, long long strideP,
                                const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
                                long long strideY, const int* skip, const cplx<T>* W, long long strideW,
-                               hipStream_t st) {
+                               hipStream_t st, int wg_min_batch) {
   const int N = mb.N, nblk = (N + 63) / 64;
-  if (N <= TSW_MIN_N && batch >= TSW_MIN_BATCH) {
+  if (N <= TSW_MIN_N && wg_min_batch > 0 && batch >= wg_min_batch) {
     hipLaunchKernelGGL((k_trsv_wg<T, NR>), dim3(batch), dim3(1024), 0, st, mb, b, ldb, strideB, y, ldy, strideY, perm,
                        strideP, skip, W, strideW);
     return hipGetLastError();
@@ -1850,13 +1851,14 @@ static hipError_t solve_sweeps(const MatBatch<T>& mb, int batch, const int* perm
 template <typename T>
 hipError_t lu_solve_batch(const MatBatch<T>& mb, int batch, const int* perm, long long strideP,
                           const cplx<double>* b, long long ldb, long long strideB, cplx<T>* y, long long ldy,
-                          long long strideY, int nrhs, const int* skip, const cplx<T>* W, hipStream_t st) {
+                          long long strideY, int nrhs, const int* skip, const cplx<T>* W, hipStream_t st,
+                          int wg_min_batch) {
   const long long strideW = lu_workspace_elems(mb.N);
   switch (nrhs) {
-    case 1: return solve_sweeps<T, 1>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
-    case 2: return solve_sweeps<T, 2>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
-    case 3: return solve_sweeps<T, 3>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
-    case 4: return solve_sweeps<T, 4>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st);
+    case 1: return solve_sweeps<T, 1>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st, wg_min_batch);
+    case 2: return solve_sweeps<T, 2>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st, wg_min_batch);
+    case 3: return solve_sweeps<T, 3>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st, wg_min_batch);
+    case 4: return solve_sweeps<T, 4>(mb, batch, perm, strideP, b, ldb, strideB, y, ldy, strideY, skip, W, strideW, st, wg_min_batch);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1937,10 +1939,10 @@ template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, 
                                             cplx<double>*, hipStream_t, const LuLookahead*, bool);
 template hipError_t lu_solve_batch<float>(const MatBatch<float>&, int, const int*, long long, const cplx<double>*,
                                           long long, long long, cplx<float>*, long long, long long, int, const int*,
-                                          const cplx<float>*, hipStream_t);
+                                          const cplx<float>*, hipStream_t, int);
 template hipError_t lu_solve_batch<double>(const MatBatch<double>&, int, const int*, long long,
                                            const cplx<double>*, long long, long long, cplx<double>*, long long,
-                                           long long, int, const int*, const cplx<double>*, hipStream_t);
+                                           long long, int, const int*, const cplx<double>*, hipStream_t, int);
 template hipError_t residual_batch<float>(const MatBatch<float>&, int, const cplx<double>*, const cplx<double>*,
                                           cplx<double>*, long long, long long, int, const int*, hipStream_t);
 template hipError_t residual_batch<double>(const MatBatch<double>&, int, const cplx<double>*,

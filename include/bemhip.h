@@ -75,6 +75,25 @@ int bem_ctx_set_fanout(bem_ctx* ctx, int streams);
 #define BEM_LU_SPLIT_BF16 1
 int bem_ctx_set_lu_products(bem_ctx* ctx, int mode);
 
+/* Launch-shape choices (no reference counterpart: tuning), per context; the project's tuner (beat_hip/autotune.py)
+ * times candidates on the device and the engine applies its table per (precision, N, batch). Captured graphs keep
+ * the shapes of the tuning in force when they were captured.
+ *  split_tile:         0: the large split-bf16 LU updates (M >= 512, N >= 256) on 256 x 128 tiles of 16 waves,
+ *                      smaller ones on 128 x 64 tiles of 4 waves; 1: 128 x 64 tiles everywhere.
+ *  trsv_wg_min_batch:  triangular solves of N <= 4096 run one workgroup per matrix when at least this many matrices
+ *                      share the launch (0: never; default 8), else as blocked multi-workgroup sweeps.
+ *  lookahead_wgs:      workgroups of the look-ahead trailing update: 0 = max(CUs / 2, CUs - batch) (default),
+ *                      > 0 that many, -1 uncapped.
+ *  field_slice_budget: source slices x observation blocks x frequencies per field launch (0: 2048). */
+typedef struct bem_tuning {
+  int split_tile;
+  int trsv_wg_min_batch;
+  int lookahead_wgs;
+  int field_slice_budget;
+} bem_tuning;
+int bem_ctx_set_tuning(bem_ctx* ctx, const bem_tuning* tuning);
+int bem_ctx_get_tuning(bem_ctx* ctx, bem_tuning* tuning);
+
 /* Kernel timers (no reference counterpart; diagnostics for roofline measurement). While enabled, each timed
  * launch group is bracketed by HIP events on the context's stream and waited for (not during capture).
  * bem_ctx_timer_read returns accumulated milliseconds, accumulated work units (frequencies or matrices) and the
