@@ -45,7 +45,8 @@ def log(msg):
 
 def _time(dev, fn, reps=3):
     torch = dev.torch
-    fn()
+    with torch.cuda.stream(dev.stream):
+        fn()
     dev.sync()
     out = []
     for _ in range(reps):
@@ -104,7 +105,8 @@ def lu_workload(dev, precision, n, batch, seed=0):
     keep = (A0, A, b, x, r, y, ipiv, perm, info, w, rel, it, done, scratch)
 
     def run():
-        A.copy_(A0)
+        with torch.cuda.stream(dev.stream):  # the copy precedes the factorization on the library's stream
+            A.copy_(A0)
         _lib.check(lib.bem_lu_factor(dev.ctx, prec, n, batch, _ptr(A), lda, sA, _ptr(ipiv), _ptr(perm), n,
                                      _ptr(info), _ptr(w)), "bem_lu_factor")
         if precision == "mixed":
