@@ -391,6 +391,12 @@ def _build_pairs(F, t, key, codes, order):
     o = np.lexsort((key, t))
     t, key, codes = t[o], key[o], codes[o]
     uniq, rule_of_pair = np.unique(codes, return_inverse=True)
+    return _pairs_with_rules(F, t, key, rule_of_pair, uniq, order)
+
+
+def _pairs_with_rules(F, t, key, rule_of_pair, uniq, order):
+    """SingularPairs from pairs already sorted by (test, trial key), rule_of_pair indexing the ascending distinct
+    orientation codes `uniq`: one remapped Duffy rule per code."""
     bases = {k: duffy_base(order, k) for k in (1, 2, 3)}
     pts, offs, lens_r, keys = [], [], [], []
     off = 0
@@ -425,16 +431,44 @@ def _build_pairs(F, t, key, codes, order):
     )
 
 
-def singular_pairs(mesh: BoundaryMesh, order: int = 4, transforms=None, tol: float = GEOMETRY_TOLERANCE
-                   ) -> SingularPairs:
+def _native_vertex_pairs(mesh: BoundaryMesh):
+    """(test, trial, rule rank, distinct codes) of the vertex-sharing pairs from the library's host builder."""
+    import ctypes as C
+
+    from . import _lib
+
+    faces = np.ascontiguousarray(mesh.faces, dtype=np.int32)
+    deg = np.bincount(faces.reshape(-1), minlength=mesh.n_vertices)
+    cap = int(deg[faces].sum())  # each face's candidates are at most the incidences of its three vertices
+    t = np.empty(cap, np.int32)
+    s = np.empty(cap, np.int32)
+    rule = np.empty(cap, np.int32)
+    codes = np.empty(64, np.int32)
+    nc, n = C.c_int(), C.c_longlong()
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    _lib.check(_lib.load().bem_host_vertex_pairs(mesh.n_faces, mesh.n_vertices, ip(faces), cap, ip(t), ip(s), ip(rule),
+                                                  ip(codes), C.byref(nc), C.byref(n)), "bem_host_vertex_pairs")
+    n = int(n.value)
+    return t[:n].astype(np.int64), s[:n].astype(np.int64), rule[:n].astype(np.int64), codes[: nc.value].astype(np.int64)
+
+
+def singular_pairs(mesh: BoundaryMesh, order: int = 4, transforms=None, tol: float = GEOMETRY_TOLERANCE,
+                   native: bool = True) -> SingularPairs:
     """All (test, trial) pairs sharing at least one vertex id (incl. t == s), with remapped Duffy rules.
 
     With plane symmetry, ``transforms`` lists the sign vectors of the identity and the image transforms; the
     image pairs (t, s) whose reflected trial element R_r s touches t geometrically (BeatEngineCore.jl:917-945,
     BeatEngineRocmRegular.jl:103-132) are added with trial key ``s + r * F``."""
     F = mesh.n_faces
-    t, s, match = _topological_candidates(mesh)
-    ts, keys, codes = [t], [s], [_pair_codes(match)]
+    images = [r for r in range(1, len(transforms or ()))]
+    if native:
+        t, s, rule, uniq = _native_vertex_pairs(mesh)
+        if not images:
+            return _pairs_with_rules(F, t, s, rule, uniq, order)
+        ts, keys, codes = [t], [s], [uniq[rule]]
+    else:
+        t, s, match = _topological_candidates(mesh)
+        ts, keys, codes = [t], [s], [_pair_codes(match)]
     for r, signs in enumerate(transforms or ()):
         if r == 0:
             continue
@@ -520,7 +554,31 @@ PEND_FROM_LDS = 1 << 16  # an earlier chunk left this vertex's partial sums in L
 PEND_TO_STRIP = 1 << 17  # the vertex's last chunk: the finished sums go to the strip
 
 
-def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
+def _native_slot_tables(mesh: BoundaryMesh, order: np.ndarray, pend_slots: int) -> SlotTables:
+    import ctypes as C
+
+    from . import _lib
+
+    F, V = mesh.n_faces, mesh.n_vertices
+    n_chunks = (F + 63) // 64
+    faces = np.ascontiguousarray(mesh.faces, dtype=np.int32)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    out = dict(slot_count=np.empty(n_chunks, np.int32), slot_start=np.empty((n_chunks, 196), np.uint16),
+               slot_inc=np.empty((n_chunks, 192), np.uint16), slot_vertex=np.empty((n_chunks, 192), np.int32),
+               slot_pend=np.empty((n_chunks, 192), np.int32), slot_prev=np.empty((n_chunks, 192), np.int32),
+               elem_pos=np.empty(F, np.int32), chunk_start=np.empty(F, np.int32), vertex_last=np.empty(V, np.int32),
+               vstar_ptr=np.empty(V + 1, np.int32), vstar_row=np.empty(3 * F, np.int32))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    up = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint16))  # noqa: E731
+    _lib.check(_lib.load().bem_host_slot_tables(
+        F, V, ip(faces), ip(order), pend_slots, ip(out["slot_count"]), up(out["slot_start"]), up(out["slot_inc"]),
+        ip(out["slot_vertex"]), ip(out["slot_pend"]), ip(out["slot_prev"]), ip(out["elem_pos"]),
+        ip(out["chunk_start"]), ip(out["vertex_last"]), ip(out["vstar_ptr"]), ip(out["vstar_row"])),
+        "bem_host_slot_tables")
+    return SlotTables(order=order, **out)
+
+
+def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS, native: bool = True) -> SlotTables:
     """Chunked trial order and per-chunk slot tables. A vertex that appears in several chunks keeps its partial
     strip sums in one of ``pend_slots`` LDS slots from its first to its last chunk (interval allocation in chunk
     order) and is stored once, by its last chunk; when no LDS slot is free it is stored by its first chunk and
@@ -531,6 +589,8 @@ def slot_tables(mesh: BoundaryMesh, pend_slots: int = PEND_SLOTS) -> SlotTables:
     n_chunks = (F + 63) // 64
     order = np.full(n_chunks * 64, -1, dtype=np.int32)
     order[:F] = perm
+    if native:
+        return _native_slot_tables(mesh, order, pend_slots)
     slot_count = np.zeros(n_chunks, dtype=np.int32)
     slot_start = np.zeros((n_chunks, 196), dtype=np.uint16)
     slot_inc = np.zeros((n_chunks, 192), dtype=np.uint16)

@@ -19,6 +19,7 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+int bem_fail_message(int code, const std::string& msg) { return fail(code, msg); }
 static int hipfail(hipError_t e, const char* what) {
   return fail(BEM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }

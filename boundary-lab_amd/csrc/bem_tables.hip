@@ -1,0 +1,233 @@
+// Host-side mesh tables (no device code): vertex-sharing element pairs with their Duffy orientation codes, and
+// the chunked slot tables of the regular assembly. Built per job before upload; these were the slowest part of
+// a request's host work.
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "../../include/bemhip.h"
+#include "bem_kernels.hpp"
+
+int bem_fail_message(int code, const std::string& msg);  // bem_capi.hip: records the thread's last error
+
+namespace {
+
+// Orientation code of a vertex-sharing pair (t, s) from the 3 x 3 vertex match matrix m[i][j] (i: test corner,
+// j: trial corner): 100000 coincident; 200000 + 1000 ia + 100 ib + 10 ja + jb for an edge, the two matches taken
+// in (i, j) scan order and ordered by trial corner; 300000 + 1000 ia + 10 ja for a shared vertex.
+int pair_code(const int* ft, const int* fs) {
+  int n = 0, first = -1, second = -1;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (ft[i] == fs[j]) {
+        if (first < 0)
+          first = 3 * i + j;
+        else if (second < 0)
+          second = 3 * i + j;
+        ++n;
+      }
+  if (n >= 3) return 100000;
+  int ia = first / 3, ja = first % 3;
+  if (n == 2) {
+    int ib = second / 3, jb = second % 3;
+    if (jb < ja) {
+      std::swap(ia, ib);
+      std::swap(ja, jb);
+    }
+    return 200000 + ia * 1000 + ib * 100 + ja * 10 + jb;
+  }
+  return 300000 + ia * 1000 + ja * 10;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bem_host_vertex_pairs(int F, int V, const int* faces, long long capacity, int* test, int* trial, int* rule,
+                          int* codes, int* n_codes, long long* n_pairs) {
+  if (F <= 0 || V <= 0 || !faces || !test || !trial || !rule || !codes || !n_codes || !n_pairs)
+    return bem_fail_message(BEM_E_ARG, "bem_host_vertex_pairs: invalid arguments");
+  try {
+    for (long long i = 0; i < 3LL * F; ++i)
+      if (faces[i] < 0 || faces[i] >= V) return bem_fail_message(BEM_E_ARG, "face vertex id out of range");
+    // vertex -> incident faces (CSR, faces ascending)
+    std::vector<int> ptr(V + 1, 0), inc(3 * (size_t)F);
+    for (long long i = 0; i < 3LL * F; ++i) ++ptr[faces[i] + 1];
+    for (int v = 0; v < V; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int t = 0; t < F; ++t)
+      for (int c = 0; c < 3; ++c) inc[fill[faces[3 * t + c]]++] = t;
+    // pairs grouped by test element, trial ascending; codes mapped to their rank among the distinct codes
+    std::vector<int> cand;
+    long long n = 0;
+    std::vector<int> code_of;  // per pair, before ranking
+    code_of.reserve((size_t)std::min<long long>(capacity, 1LL << 26));
+    for (int t = 0; t < F; ++t) {
+      cand.clear();
+      for (int c = 0; c < 3; ++c) {
+        const int v = faces[3 * t + c];
+        cand.insert(cand.end(), inc.begin() + ptr[v], inc.begin() + ptr[v + 1]);
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      if (n + (long long)cand.size() > capacity) return bem_fail_message(BEM_E_ARG, "pair capacity too small");
+      for (int s : cand) {
+        test[n] = t;
+        trial[n] = s;
+        code_of.push_back(pair_code(faces + 3 * t, faces + 3 * s));
+        ++n;
+      }
+    }
+    std::vector<int> uniq(code_of);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    if ((int)uniq.size() > 64) return bem_fail_message(BEM_E_ARG, "more than 64 orientation codes");
+    for (size_t i = 0; i < uniq.size(); ++i) codes[i] = uniq[i];
+    *n_codes = (int)uniq.size();
+    for (long long p = 0; p < n; ++p)
+      rule[p] = (int)(std::lower_bound(uniq.begin(), uniq.end(), code_of[p]) - uniq.begin());
+    *n_pairs = n;
+  } catch (const std::exception& ex) {
+    return bem_fail_message(BEM_E_NOMEM, std::string("bem_host_vertex_pairs: ") + ex.what());
+  }
+  return BEM_OK;
+}
+
+int bem_host_slot_tables(int F, int V, const int* faces, const int* order, int pend_slots, int* slot_count,
+                         uint16_t* slot_start, uint16_t* slot_inc, int* slot_vertex, int* slot_pend, int* slot_prev,
+                         int* elem_pos, int* chunk_start, int* vertex_last, int* vstar_ptr, int* vstar_row) {
+  if (F <= 0 || V <= 0 || !faces || !order || pend_slots < 0 || pend_slots > BEM_PEND_SLOTS || !slot_count ||
+      !slot_start || !slot_inc || !slot_vertex || !slot_pend || !slot_prev || !elem_pos || !chunk_start ||
+      !vertex_last || !vstar_ptr || !vstar_row)
+    return bem_fail_message(BEM_E_ARG, "bem_host_slot_tables: invalid arguments");
+  try {
+    const int n_chunks = (F + 63) / 64;
+    for (long long i = 0; i < 3LL * F; ++i)
+      if (faces[i] < 0 || faces[i] >= V) return bem_fail_message(BEM_E_ARG, "face vertex id out of range");
+    for (long long i = 0; i < 64LL * n_chunks; ++i)
+      if (i < F ? (order[i] < 0 || order[i] >= F) : order[i] != -1)
+        return bem_fail_message(BEM_E_ARG, "order must be a permutation of the faces padded with -1");
+    std::memset(slot_count, 0, sizeof(int) * n_chunks);
+    std::memset(slot_start, 0, sizeof(uint16_t) * (size_t)n_chunks * BEM_SLOT_STRIDE);
+    std::memset(slot_inc, 0, sizeof(uint16_t) * (size_t)n_chunks * BEM_SLOT_INC);
+    std::memset(slot_vertex, 0, sizeof(int) * (size_t)n_chunks * BEM_SLOT_INC);
+    for (size_t i = 0; i < (size_t)n_chunks * BEM_SLOT_INC; ++i) {
+      slot_pend[i] = BEM_PEND_NONE | BEM_PEND_TO_STRIP;
+      slot_prev[i] = -1;
+    }
+    // per chunk: its distinct vertices ascending, and the incidence codes (lane << 2 | corner) sorted by vertex,
+    // then code
+    std::vector<std::vector<int>> chunk_verts(n_chunks);
+    std::vector<std::pair<int, int>> vc;  // (vertex, code)
+    for (int c = 0; c < n_chunks; ++c) {
+      vc.clear();
+      for (int lane = 0; lane < 64; ++lane) {
+        const int e = order[64 * c + lane];
+        if (e < 0) continue;
+        for (int k = 0; k < 3; ++k) vc.emplace_back(faces[3 * e + k], (lane << 2) | k);
+      }
+      std::sort(vc.begin(), vc.end());
+      std::vector<int>& uv = chunk_verts[c];
+      uint16_t* st = slot_start + (size_t)c * BEM_SLOT_STRIDE;
+      uint16_t* si = slot_inc + (size_t)c * BEM_SLOT_INC;
+      for (size_t i = 0; i < vc.size(); ++i) {
+        si[i] = (uint16_t)vc[i].second;
+        if (uv.empty() || uv.back() != vc[i].first) uv.push_back(vc[i].first);
+        st[uv.size()] = (uint16_t)(i + 1);
+      }
+      slot_count[c] = (int)uv.size();
+    }
+    std::vector<long long> first(V, -1), last(V, -1);
+    for (int c = 0; c < n_chunks; ++c)
+      for (int v : chunk_verts[c]) {
+        if (first[v] < 0) first[v] = c;
+        last[v] = c;
+      }
+    // LDS slots for vertices used by several chunks: scan the live ranges [first, last] by start (then end, then
+    // vertex), keep as many as fit pend_slots at every chunk, dropping the covering range that ends last (ties:
+    // the lowest vertex id) when over capacity
+    std::vector<int> multi;
+    for (int v = 0; v < V; ++v)
+      if (first[v] >= 0 && last[v] > first[v]) multi.push_back(v);
+    std::stable_sort(multi.begin(), multi.end(), [&](int a, int b) {
+      return first[a] != first[b] ? first[a] < first[b] : last[a] < last[b];
+    });
+    std::vector<char> keep(V, 0);
+    auto worse = [&](const std::pair<long long, int>& a, const std::pair<long long, int>& b) {
+      // max-heap on last chunk; on ties the lower vertex id is popped first
+      return a.first != b.first ? a.first < b.first : a.second > b.second;
+    };
+    std::priority_queue<std::pair<long long, int>, std::vector<std::pair<long long, int>>, decltype(worse)> active(
+        worse);
+    std::vector<long long> ends(n_chunks + 1, 0);
+    long long live = 0, scan = 0;
+    for (int v : multi) {
+      const long long fv = first[v], lv = last[v];
+      while (scan < fv) live -= ends[scan++];
+      keep[v] = 1;
+      active.emplace(lv, v);
+      ++live;
+      ++ends[lv];
+      if (live > pend_slots) {
+        while (!keep[active.top().second] || last[active.top().second] < fv) active.pop();
+        const int u = active.top().second;
+        active.pop();
+        keep[u] = 0;
+        --live;
+        --ends[last[u]];
+      }
+    }
+    std::vector<long long> pend(V, -1), last_seen(V, -1);
+    std::vector<int> freelist;
+    for (int i = pend_slots - 1; i >= 0; --i) freelist.push_back(i);
+    std::vector<std::vector<int>> release(n_chunks);
+    for (int c = 0; c < n_chunks; ++c) {
+      const std::vector<int>& uv = chunk_verts[c];
+      for (int v : uv)
+        if (first[v] == c && keep[v]) {
+          pend[v] = freelist.back();
+          freelist.pop_back();
+          release[last[v]].push_back((int)pend[v]);
+        }
+      for (size_t k = 0; k < uv.size(); ++k) {
+        const int v = uv[k];
+        const bool seen = first[v] < c, in_lds = pend[v] >= 0;
+        int code = in_lds ? (int)pend[v] : BEM_PEND_NONE;
+        if (in_lds && seen) code |= BEM_PEND_FROM_LDS;
+        if (!in_lds || last[v] == c) code |= BEM_PEND_TO_STRIP;
+        const size_t o = (size_t)c * BEM_SLOT_INC + k;
+        slot_pend[o] = code;
+        slot_vertex[o] = 2 * v + ((seen && !in_lds) ? 1 : 0);
+        slot_prev[o] = (int)last_seen[v];
+        last_seen[v] = c;
+      }
+      freelist.insert(freelist.end(), release[c].begin(), release[c].end());
+    }
+    // element positions, the first chunk a test element's wave visits, each vertex's last chunk
+    for (int i = 0; i < F; ++i) elem_pos[order[i]] = i;
+    std::vector<int> vmin(V, n_chunks);
+    for (int t = 0; t < F; ++t)
+      for (int k = 0; k < 3; ++k) vmin[faces[3 * t + k]] = std::min(vmin[faces[3 * t + k]], elem_pos[t] / 64);
+    for (int t = 0; t < F; ++t) {
+      int m = elem_pos[t] / 64;
+      for (int k = 0; k < 3; ++k) m = std::min(m, vmin[faces[3 * t + k]]);
+      chunk_start[t] = m;
+    }
+    for (int v = 0; v < V; ++v) vertex_last[v] = (int)last[v];
+    // vertex -> strip rows 3t + k, rows ascending
+    std::memset(vstar_ptr, 0, sizeof(int) * ((size_t)V + 1));
+    for (long long r = 0; r < 3LL * F; ++r) ++vstar_ptr[faces[r] + 1];
+    for (int v = 0; v < V; ++v) vstar_ptr[v + 1] += vstar_ptr[v];
+    std::vector<int> pos(vstar_ptr, vstar_ptr + V);
+    for (long long r = 0; r < 3LL * F; ++r) vstar_row[pos[faces[r]]++] = (int)r;
+  } catch (const std::exception& ex) {
+    return bem_fail_message(BEM_E_NOMEM, std::string("bem_host_slot_tables: ") + ex.what());
+  }
+  return BEM_OK;
+}
+
+}  // extern "C"

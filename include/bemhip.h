@@ -170,6 +170,24 @@ typedef struct bem_mesh_desc {
  * (JS/BeatEngineRocmRegular.jl:134-205), which the reference trusts unchecked. */
 int bem_mesh_validate(const bem_mesh_desc* desc);
 
+/* Host-side tables (no device needed; HOST arrays). bem_host_vertex_pairs lists every (test, trial) element pair
+ * sharing at least one vertex id, self pairs included, grouped by test element with trials ascending (at most
+ * `capacity`), and classifies it by its Duffy orientation code: 100000 coincident; 200000 + 1000 ia + 100 ib +
+ * 10 ja + jb for a shared edge (test corners ia, ib and trial corners ja < jb of the two matches, taken in
+ * test-corner-major scan order); 300000 + 1000 ia + 10 ja for a shared vertex. codes (int[64]) receives the
+ * distinct codes ascending, rule[p] the rank of pair p's code among them. Replaces adjacency_info /
+ * singular_orientation_key / build_singular_correction_cache (JS/BeatEngineCore.jl:441-507,784-915). */
+int bem_host_vertex_pairs(int n_faces, int n_vertices, const int* faces, long long capacity, int* test, int* trial,
+                          int* rule, int* codes, int* n_codes, long long* n_pairs);
+/* The regular assembly's chunk tables for a trial order (`order`: n_chunks*64 face ids padded with -1) and
+ * `pend_slots` LDS partial-sum slots per wave: the bem_mesh_desc fields slot_count .. vstar_row (sizes as there).
+ * No reference counterpart: it replaces the greedy colouring of the reference's pair kernels
+ * (_beat_cpu_element_color_groups, JS/BeatEngineCpuAssembly.jl:102-130; JS/BeatEngineRocmRegular.jl:134-205). */
+int bem_host_slot_tables(int n_faces, int n_vertices, const int* faces, const int* order, int pend_slots,
+                         int* slot_count, uint16_t* slot_start, uint16_t* slot_inc, int* slot_vertex, int* slot_pend,
+                         int* slot_prev, int* elem_pos, int* chunk_start, int* vertex_last, int* vstar_ptr,
+                         int* vstar_row);
+
 /* Upload a prepared mesh: SoA geometry, locality/slot tables, singular pairs and Duffy rules, field sources.
  * Replaces build_rocm_regular_assembly_cache (JS/BeatEngineRocmRegular.jl:134-205),
  * build_rocm_singular_correction_cache (JS/BeatEngineRocmSingular.jl:1-55),

@@ -383,3 +383,57 @@ def test_gmsh_block_reader_matches_line_reader(tmp_path):
         assert np.array_equal(a, b)
     for a, b in zip((vm, fm, tm), loop_mixed):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["horn", "sphere", "fan", "pistons", "ragged", "x-sym"])
+def test_native_tables_equal_numpy_tables(case):
+    # the library's host builders (bem_host_vertex_pairs, bem_host_slot_tables) produce exactly the tables of the
+    # NumPy restatement: slot tables, pair lists, Duffy rule indices, rule points and orientation keys
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+
+    V, F, T = {"horn": lambda: syn.horn_in_box(), "sphere": lambda: syn.icosphere(3, 0.1),
+               "fan": lambda: syn.uv_sphere(8, 80, 0.1), "pistons": lambda: syn.box_with_pistons(h=0.05),
+               "ragged": lambda: syn.horn_in_box(h=0.05), "x-sym": lambda: syn.uv_sphere(10, 20, 0.1)}[case]()
+    mesh = G.BoundaryMesh(V, F, T)
+    sym = "off"
+    if case == "x-sym":  # the half x >= 0 of a mirror-symmetric sphere
+        keep = (mesh.face_vertices[:, :, 0] >= -1e-12).all(axis=1)
+        used = np.unique(mesh.faces[keep])
+        remap = -np.ones(len(V), int)
+        remap[used] = np.arange(len(used))
+        mesh = G.snap_symmetry_planes(G.BoundaryMesh(V[used], remap[mesh.faces[keep]], T[keep]), "x")
+        sym = "x"
+    a = G.prepare(mesh, symmetry=sym)
+    fp, vp = a.fperm, a.vperm
+    w = a.mesh
+    b_slots = G.slot_tables(w, native=False)
+    for f in b_slots.__dataclass_fields__:
+        assert np.array_equal(np.asarray(getattr(a.slots, f)), np.asarray(getattr(b_slots, f))), f
+    signs = G.SYMMETRY_SIGNS[sym]
+    b_pairs = G.singular_pairs(w, 4, transforms=signs if len(signs) > 1 else None, native=False)
+    for f in ("test", "trial", "rule", "ptr", "rule_off", "rule_len", "rule_pts"):
+        assert np.array_equal(getattr(a.pairs, f), getattr(b_pairs, f)), f
+    assert a.pairs.keys == b_pairs.keys
+
+
+def test_native_table_builders_reject_bad_input():
+    import ctypes as C
+
+    from beat_hip import _lib
+
+    lib = _lib.load()
+    faces = np.array([[0, 1, 2], [0, 2, 5]], np.int32)
+    out = [np.zeros(64, np.int32) for _ in range(4)]
+    n, nc = C.c_longlong(), C.c_int()
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    assert lib.bem_host_vertex_pairs(2, 4, ip(faces), 64, *[ip(o) for o in out], C.byref(nc), C.byref(n)) == \
+        _lib.BEM_E_ARG  # vertex 5 >= V
+    faces[1, 2] = 3
+    assert lib.bem_host_vertex_pairs(2, 4, ip(faces), 2, *[ip(o) for o in out], C.byref(nc), C.byref(n)) == \
+        _lib.BEM_E_ARG  # capacity too small (4 pairs)
+    assert lib.bem_host_vertex_pairs(2, 4, ip(faces), 64, *[ip(o) for o in out], C.byref(nc), C.byref(n)) == 0
+    # two self pairs (coincident) and the shared edge in both directions, whose orientations differ:
+    # (0, 1): test corners 0, 2 meet trial corners 0, 1 -> 200201; (1, 0): test 0, 1 meet trial 0, 2 -> 200102
+    assert n.value == 4 and nc.value == 3
+    assert out[3][:3].tolist() == [100000, 200102, 200201]
