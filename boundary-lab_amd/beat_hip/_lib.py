@@ -137,6 +137,10 @@ def load(path: Path | None = None):
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
+    # torch ships its own HIP and HSA runtimes under the same sonames as the system's: imported first, its runtime
+    # is the one this library binds to. Loaded the other way round, the process would hold two runtimes and the
+    # second finds no device.
+    import torch  # noqa: F401
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise BemError(

@@ -134,3 +134,16 @@ def test_malformed_mesh_description_rejected(desc_parts, name, mutate, expect):
     rc, msg = _validate(desc_parts, mutate)
     assert rc == _lib.BEM_E_ARG
     assert expect in msg, msg
+
+
+def test_library_binds_to_torchs_hip_runtime():
+    # loading the library imports torch first, so that one HIP runtime serves both (a fresh process that only
+    # prepares a mesh must not pull in a second runtime before torch)
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path[:0] = ['{root}', '{root}/boundary-lab_amd']; "
+            "from beat_hip import _lib; assert 'torch' not in sys.modules; _lib.load(); "
+            "assert 'torch' in sys.modules; print('ok')").format(root=ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr
