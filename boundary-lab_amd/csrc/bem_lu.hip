@@ -302,6 +302,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int gather_u2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, unsigned& lo) {
   const f32x2 v = {re, im};
@@ -343,20 +344,33 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
 #pragma unroll
     for (int v = 0; v < 16; ++v) acr[ti][v] = aci[ti][v] = 0.f;
 
-  // staging per stage: A = BM x 16 complex, B = 16 x BN complex
+  // staging per stage: A = BM x 16 complex, B = 16 x BN complex, through buffer descriptors with scalar bases at
+  // the tile's first A row and first B row: 32-bit byte offsets (at most 256 rows x lda x 8 B) instead of 64-bit
+  // addresses, and an element outside the update gets an offset past the descriptor's range, which loads zero
+  // without a branch
+  constexpr int OOB = 0x7fffffff;
+  const unsigned a_bytes = (unsigned)min((long long)BM * lda * 8, (long long)0x7ffffff0);
+  const unsigned b_bytes = (unsigned)min((long long)g.K * lda * 8, (long long)0x7ffffff0);
+  const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(A + (size_t)(g.a_r0 + bm) * lda + g.a_c0), (short)0, (int)a_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(A + (size_t)g.b_r0 * lda + g.b_c0 + bn), (short)0, (int)b_bytes, 0x00020000);
+  const int ldab = (int)lda * 8;
   cplx<float> ra[QA], rb[QB];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int e = tid + NT * q, r = e >> 4, kk = e & 15;
-      ra[q] = (bm + r < g.M && k0 + kk < g.K) ? A[(size_t)(g.a_r0 + bm + r) * lda + g.a_c0 + k0 + kk]
-                                              : mk<float>(0.f, 0.f);
+      const int off = (bm + r < g.M && k0 + kk < g.K) ? r * ldab + (k0 + kk) * 8 : OOB;
+      const gather_u2 v = __builtin_amdgcn_raw_buffer_load_b64(rsa, off, 0, 0);
+      ra[q] = mk<float>(__uint_as_float(v.x), __uint_as_float(v.y));
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int e = tid + NT * q, kk = e / BN, j = e % BN;
-      rb[q] = (k0 + kk < g.K && bn + j < g.Nc) ? A[(size_t)(g.b_r0 + k0 + kk) * lda + g.b_c0 + bn + j]
-                                               : mk<float>(0.f, 0.f);
+      const int off = (k0 + kk < g.K && bn + j < g.Nc) ? (k0 + kk) * ldab + j * 8 : OOB;
+      const gather_u2 v = __builtin_amdgcn_raw_buffer_load_b64(rsb, off, 0, 0);
+      rb[q] = mk<float>(__uint_as_float(v.x), __uint_as_float(v.y));
     }
   };
   auto stash = [&]() {
@@ -381,13 +395,6 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
 #pragma unroll
     for (int kg = 0; kg < KC / 8; ++kg) {
       const int d0 = 8 * kg + 4 * half;  // first complex k of this lane's eight bf16
-      u32x4 ah[2], al[2];
-#pragma unroll
-      for (int ti = 0; ti < 2; ++ti) {
-        const int r = 64 * wy + 32 * ti + l32;
-        ah[ti] = *reinterpret_cast<const u32x4*>(&Ah[r][d0]);
-        al[ti] = *reinterpret_cast<const u32x4*>(&Al[r][d0]);
-      }
       const int cl = 32 * wx + l32;
       const u32x4 bh = *reinterpret_cast<const u32x4*>(&Bh[cl][d0]);
       const u32x4 bl = *reinterpret_cast<const u32x4*>(&Bl[cl][d0]);
@@ -401,9 +408,12 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
       }
       const bf16x8 Brh = __builtin_bit_cast(bf16x8, brh), Brl = __builtin_bit_cast(bf16x8, brl);
       const bf16x8 Bih = __builtin_bit_cast(bf16x8, bih), Bil = __builtin_bit_cast(bf16x8, bil);
+      // one 32-row block's A operands at a time (8 registers live instead of 16)
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti) {
-        const bf16x8 Ahh = __builtin_bit_cast(bf16x8, ah[ti]), All = __builtin_bit_cast(bf16x8, al[ti]);
+        const int r = 64 * wy + 32 * ti + l32;
+        const bf16x8 Ahh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Ah[r][d0]));
+        const bf16x8 All = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Al[r][d0]));
         acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brh, acr[ti], 0, 0, 0);
         aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bih, aci[ti], 0, 0, 0);
         acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brl, acr[ti], 0, 0, 0);
