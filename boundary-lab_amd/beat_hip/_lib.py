@@ -20,6 +20,7 @@ BEM_MAX_CHANNELS = 4
 BEM_NOALLOC_SCRATCH_BYTES = 64 << 20
 BEM_LU_FP32 = 0
 BEM_LU_SPLIT_BF16 = 1
+BEM_LU_MAX_N = 24576
 TIMER_REGULAR, TIMER_SINGULAR, TIMER_GATHER, TIMER_RHS, TIMER_LU, TIMER_FIELD, TIMER_SOLVE = range(7)
 TIMER_NAMES = ("regular", "singular", "gather", "rhs", "lu", "field", "solve")
 
@@ -72,6 +73,7 @@ class MeshDesc(C.Structure):
         ("transform_signs", _i * 12),
         ("row_weights", _dp),
         ("plane_mask", _ip),
+        ("n_qp", _i),
     ]
 
 
@@ -120,6 +122,13 @@ SIGNATURES = {
     "bem_graph_end": (_i, [_p, C.POINTER(_p)]),
     "bem_graph_launch": (_i, [_p, _p]),
     "bem_graph_destroy": (_i, [_p]),
+    "bem_colset_create": (_i, [_p, _p, _i, _ip, _i, _ip, _ip, _dp, C.POINTER(_p)]),
+    "bem_colset_destroy": (_i, [_p]),
+    "bem_rhs_columns": (_i, [_p, _p, _p, _i, _i, _p, _p, _p, _ll, _ll]),
+    "bem_sparse_create": (_i, [_p, _i, _i, _ll, _ip, _ip, _dp, C.POINTER(_p)]),
+    "bem_sparse_destroy": (_i, [_p]),
+    "bem_sparse_scatter": (_i, [_p, _p, _i, _d, _d, _i, _p, _ll]),
+    "bem_dense_block": (_i, [_p, _i, _i, _i, _p, _ll, _d, _d, _i, _p, _ll]),
 }
 
 

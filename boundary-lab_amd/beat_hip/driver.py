@@ -437,6 +437,13 @@ class Job:
     obs: np.ndarray
     cancel_path: str | None
     fp64_check: bool
+    regular_order: int = 4
+    singular_order: int = 4
+
+    def prepare(self) -> G.PreparedMesh:
+        """The device-ready tables of the job's mesh under its rules."""
+        return G.prepare(self.mesh, symmetry=self.symmetry, regular_order=self.regular_order,
+                         singular_order=self.singular_order)
 
     @property
     def n_horizontal(self):
@@ -473,8 +480,10 @@ def parse_request(request) -> tuple[dict, dict]:
 def build_job(request, parsed) -> Job:
     config = parsed["config"]
     regular_quadrature_mode_from_config(config)
-    if int(get_value(config, "quadrature_order", 4)) != 4 or int(get_value(config, "singular_order", 4)) != 4:
-        raise ValueError("This engine implements the order-4 regular rule and order-4 Duffy rules only.")
+    regular_order = int(get_value(config, "quadrature_order", 4))
+    singular_order = int(get_value(config, "singular_order", 4))
+    if not 1 <= singular_order <= 4:
+        raise ValueError("Duffy 1D Gauss order must be between 1 and 4.")
     mesh = load_combined_mesh(parsed["mesh_inputs"])
     # x / xy symmetry: the mesh is the fundamental domain (solver.jl:718-719)
     mesh = G.snap_symmetry_planes(mesh, parsed["symmetry"])
@@ -495,7 +504,8 @@ def build_job(request, parsed) -> Job:
         ref_angle=float(get_value(config, "flat_target_reference_angle_deg", 0.0)),
         symmetry=parsed["symmetry"], precision=precision, qunit=qunit,
         obs=np.concatenate(pts).astype(np.float64), cancel_path=get_value(request, "cancel_path"),
-        fp64_check=bool(get_value(config, "fp64_check", False)),
+        fp64_check=bool(get_value(config, "fp64_check", False)), regular_order=regular_order,
+        singular_order=singular_order,
     )
 
 
@@ -540,8 +550,8 @@ def diagnostics_for(job: Job, res, i, extra=None) -> dict:
         "regular_assembly_mode": "hip_row_strip_slots",
         "blas_threads": 0,
         "regular_quadrature_mode": "fixed",
-        "regular_quadrature_order": 4,
-        "regular_quadrature_base_order": 4,
+        "regular_quadrature_order": job.regular_order,
+        "regular_quadrature_base_order": job.regular_order,
         "precision": job.precision,
         "relative_residual": float(res.residual[i]),
         "pivot_growth": float(res.growth[i]),
@@ -726,7 +736,7 @@ def run_job(request, engine_factory, emit, batch: int | None = None, graph: bool
     emit({"type": "status", "message": f"BEAT Engine loading {len(parsed['mesh_inputs'])} mesh"
           f"{'' if len(parsed['mesh_inputs']) == 1 else 'es'} (HIP engine)"})
     job = build_job(request, parsed)
-    prep = G.prepare(job.mesh, symmetry=job.symmetry)
+    prep = job.prepare()
     emit({"type": "status", "message": f"Initializing BEAT Engine using native HIP assembly and GPU solve "
           f"({job.precision}, {job.mesh.n_faces} triangles, {job.mesh.n_vertices} P1 dofs)..."})
     if cancelled(job):

@@ -105,7 +105,11 @@ def mesh_desc(prep: PreparedMesh, src_faces) -> tuple[_lib.MeshDesc, dict]:
     d.normals = _np_ptr(arr("n", m.normals, np.float64), C.c_double)
     d.curls = _np_ptr(arr("curl", prep.curls.reshape(F, 9), np.float64), C.c_double)
     d.area2 = _np_ptr(arr("a2", 2.0 * m.areas, np.float64), C.c_double)
-    d.qpoints = _np_ptr(arr("qp", prep.qpoints.reshape(F, 18), np.float64), C.c_double)
+    nq = prep.qpoints.shape[1]
+    qp18 = np.zeros((F, 18))
+    qp18[:, : 3 * nq] = prep.qpoints.reshape(F, 3 * nq)
+    d.qpoints = _np_ptr(arr("qp", qp18, np.float64), C.c_double)
+    d.n_qp = nq
     s = prep.slots
     d.n_chunks = len(s.slot_count)
     d.order = _np_ptr(arr("order", s.order, np.int32), C.c_int)

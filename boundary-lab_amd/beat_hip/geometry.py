@@ -278,10 +278,26 @@ def remap_edge(p, va, vb):
     return o[None, :] + p[:, :1] * (a - o)[None, :] + p[:, 1:2] * (b - o)[None, :]
 
 
-def mass_blocks():
-    b = p1_basis(RULE_POINTS)
-    m11 = np.einsum("qi,qj,q->ij", b, b, RULE_WEIGHTS)
-    m10 = b.T @ RULE_WEIGHTS
+def regular_rule(order: int = 4):
+    """Points (n, 2) and weights (n,) of the regular triangle rule of ``order`` (BeatEngineCore.jl:517-549): the
+    centroid rule for order <= 1, the six-point rule for order 4, the three-point rule (1/6, 1/6), (2/3, 1/6),
+    (1/6, 2/3) with weights 1/6 for any other order."""
+    order = int(order)
+    if order <= 1:
+        return np.array([[1.0 / 3.0, 1.0 / 3.0]]), np.array([0.5])
+    if order == 4:
+        return RULE_POINTS, RULE_WEIGHTS
+    return (np.array([[1.0 / 6.0, 1.0 / 6.0], [2.0 / 3.0, 1.0 / 6.0], [1.0 / 6.0, 2.0 / 3.0]]),
+            np.full(3, 1.0 / 6.0))
+
+
+def mass_blocks(order: int = 4):
+    """P1 x P1 and P1 x DP0 identity blocks per unit 2A under the regular rule: the reference assembles its
+    identity operators with that rule (assemble_l2_identity_matrix, BeatEngineCore.jl:1138-1180)."""
+    pts, wts = regular_rule(order)
+    b = p1_basis(pts)
+    m11 = np.einsum("qi,qj,q->ij", b, b, wts)
+    m10 = b.T @ wts
     return m11, m10
 
 
@@ -304,9 +320,9 @@ def surface_curls(face_vertices, normals):
     return np.cross(normals[:, None, :], grads)
 
 
-def quadrature_points(face_vertices):
-    """(F,6,3) order-4 rule points."""
-    return np.einsum("ql,flc->fqc", p1_basis(RULE_POINTS), face_vertices)
+def quadrature_points(face_vertices, order: int = 4):
+    """(F, n, 3) points of the regular rule of ``order``."""
+    return np.einsum("ql,flc->fqc", p1_basis(regular_rule(order)[0]), face_vertices)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -718,6 +734,8 @@ class PreparedMesh:
     transforms: np.ndarray = None  # (R,3) sign vectors, identity first
     row_weights: np.ndarray = None  # (V,) P1 orbit weights (internal order)
     plane_mask: np.ndarray = None  # (F,) bit r: face takes part in an image pair of transform r
+    regular_order: int = 4  # order of the regular rule (qpoints holds its points)
+    singular_order: int = 4
 
     @property
     def n_faces(self):
@@ -835,10 +853,14 @@ def plane_masks(pairs: SingularPairs, n_faces: int, n_transforms: int) -> np.nda
     return mask
 
 
-def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True, symmetry: str = "off"
-            ) -> PreparedMesh:
+def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True, symmetry: str = "off",
+            regular_order: int = 4) -> PreparedMesh:
     """Device-ready tables. With ``symmetry`` x / xy the mesh is the fundamental domain (already snapped and
-    validated, as the request driver does); image pairs join the singular pairs with trial key s + r F."""
+    validated, as the request driver does); image pairs join the singular pairs with trial key s + r F.
+    ``regular_order`` picks the regular rule (4: six points, 2: three, 1: the centroid) and ``singular_order``
+    the Gauss order of the Duffy rules (1..4)."""
+    if not 1 <= int(singular_order) <= 4:
+        raise ValueError("Duffy 1D Gauss order must be between 1 and 4.")
     symmetry = normalized_symmetry_mode(symmetry)
     if reorder:
         fperm, vperm = locality_order(mesh)
@@ -848,10 +870,10 @@ def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True, s
     vinv[vperm] = np.arange(len(vperm))
     work = BoundaryMesh(mesh.vertices[vperm], vinv[mesh.faces[fperm]], mesh.tags[fperm], mesh.mesh_ids[fperm])
     curls = surface_curls(work.face_vertices, work.normals)
-    qp = quadrature_points(work.face_vertices)
-    m11, m10 = mass_blocks()
+    qp = quadrature_points(work.face_vertices, regular_order)
+    m11, m10 = mass_blocks(regular_order)
     signs = SYMMETRY_SIGNS[symmetry]
     pairs = singular_pairs(work, singular_order, transforms=signs if len(signs) > 1 else None)
     return PreparedMesh(work, curls, qp, pairs, slot_tables(work), m11, m10, fperm, vperm, mesh, symmetry,
                         np.asarray(signs, dtype=np.int32), p1_orbit_weights(work, symmetry),
-                        plane_masks(pairs, work.n_faces, len(signs)))
+                        plane_masks(pairs, work.n_faces, len(signs)), int(regular_order), int(singular_order))

@@ -225,7 +225,7 @@ class Scheduler:
     def _prepare(self, h: JobHandle, request: dict, parsed: dict):
         try:
             job = D.build_job(request, parsed)
-            prep = G.prepare(job.mesh, symmetry=job.symmetry)
+            prep = job.prepare()
             # frequencies per device batch: the configured batch, shrunk until the job alone fits the budget
             nb = max(1, min(self.batch, len(job.frequencies)))
             budget = self.budget() if self._engine_factory is None else (self._budget or 1 << 62)
@@ -408,12 +408,13 @@ def percentile(values, q):
 
 def mesh_key(job: D.Job, prep: G.PreparedMesh | None = None) -> str:
     """Identity of everything the device solve of a job depends on besides its frequencies: mesh geometry and
-    faces, unit drives, symmetry, observation points, medium, precision and the FP64 check. Jobs with equal keys
+    faces, unit drives, symmetry, observation points, medium, precision, rules and the FP64 check. Jobs with equal keys
     can share one device mesh and one stream of frequency batches."""
     h = hashlib.sha1()
     for a in (job.mesh.vertices, job.mesh.faces, job.qunit, job.obs):
         a = np.ascontiguousarray(a)
         h.update(str((a.dtype, a.shape)).encode())
         h.update(a.tobytes())
-    h.update(repr((job.symmetry, job.precision, job.rho, job.sound_speed, bool(job.fp64_check))).encode())
+    h.update(repr((job.symmetry, job.precision, job.rho, job.sound_speed, bool(job.fp64_check), job.regular_order,
+                   job.singular_order)).encode())
     return h.hexdigest()

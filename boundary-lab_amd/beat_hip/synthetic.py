@@ -468,3 +468,84 @@ def write_gmsh22(path, V, F, T, scale: float = 1.0):
         for i, (f, t) in enumerate(zip(F, T), start=1):
             out.write("%d 2 2 %d %d %d %d %d\n" % (i, t, t, f[0] + 1, f[1] + 1, f[2] + 1))
         out.write("$EndElements\n")
+
+
+# ---------------------------------------------------------------------------------------------
+# Tetrahedral volumes for the coupled FEM-BEM path
+# ---------------------------------------------------------------------------------------------
+
+TAG_VOLUME, TAG_INNER, TAG_OUTER = 1, 2, 3
+
+
+def prism_layers_to_tets(faces: np.ndarray, n_surface: int, n_layers: int) -> np.ndarray:
+    """Tetrahedra of the prisms between consecutive copies of a triangulated surface (vertex v of layer j is
+    j * n_surface + v). Each prism (a, b, c) with a < b < c splits into (a, b, c, c'), (a, b, b', c'),
+    (a, a', b', c'): the diagonal of every side quad joins the lower-numbered bottom vertex to the top copy of the
+    higher-numbered one, so neighbouring prisms agree on it and the mesh is conforming."""
+    s = np.sort(np.asarray(faces, dtype=np.int64), axis=1)
+    a, b, c = s[:, 0], s[:, 1], s[:, 2]
+    out = []
+    for j in range(n_layers):
+        lo, hi = j * n_surface, (j + 1) * n_surface
+        out += [np.stack([a + lo, b + lo, c + lo, c + hi], 1), np.stack([a + lo, b + lo, b + hi, c + hi], 1),
+                np.stack([a + lo, a + hi, b + hi, c + hi], 1)]
+    return np.concatenate(out)
+
+
+def spherical_shell(subdivisions: int = 2, inner: float = 0.1, outer: float = 0.15, layers: int = 2):
+    """Air between two concentric icospheres as a tetrahedral volume (tag TAG_VOLUME), its inner sphere faces
+    (TAG_INNER) and outer sphere faces (TAG_OUTER) as boundary triangles, and the outer sphere as a BEM surface
+    with outward normals (tag TAG_OUTER, same vertex positions). Radii of the layers grow linearly.
+    Returns (VolumeMesh, (V, F, T) of the BEM surface)."""
+    from .fem import VolumeMesh
+
+    V, F, _ = icosphere(subdivisions, 1.0)
+    nv = len(V)
+    radii = np.linspace(inner, outer, layers + 1)
+    verts = np.concatenate([V * r for r in radii])
+    tets = prism_layers_to_tets(F, nv, layers)
+    bfaces = np.concatenate([F, F + layers * nv])
+    btags = np.concatenate([np.full(len(F), TAG_INNER), np.full(len(F), TAG_OUTER)])
+    names = {(3, TAG_VOLUME): "Volume", (2, TAG_INNER): "Radiator", (2, TAG_OUTER): "Interface"}
+    vm = VolumeMesh(verts, tets, np.full(len(tets), TAG_VOLUME), bfaces, btags, names)
+    return vm, (V * outer, F.copy(), np.full(len(F), TAG_OUTER))
+
+
+def structured_cube(n: int = 4, size: float = 1.0):
+    """The cube [0, size]^3 cut into n^3 hexahedra of six tetrahedra each (Kuhn split along the main diagonal,
+    conforming), boundary faces tagged 1..6 by side (x-, x+, y-, y+, z-, z+), volume tag 1."""
+    from .fem import VolumeMesh
+
+    g = np.linspace(0.0, size, n + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    verts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+
+    def vid(i, j, k):
+        return (i * (n + 1) + j) * (n + 1) + k
+
+    tets = []
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                base = np.array([i, j, k])
+                for p in perms:
+                    path = [base.copy()]
+                    cur = base.copy()
+                    for ax in p:
+                        cur = cur.copy()
+                        cur[ax] += 1
+                        path.append(cur)
+                    tets.append([vid(*q) for q in path])
+    tets = np.array(tets, dtype=np.int64)
+    # boundary faces: tetrahedral faces on the cube's sides
+    f = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]], tets[:, [1, 2, 3]]])
+    key = np.sort(f, 1)
+    uniq, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    bf = uniq[cnt == 1]
+    c = verts[bf].mean(1)
+    tag = np.zeros(len(bf), np.int64)
+    eps = 1e-12 * size
+    for t, (ax, val) in enumerate([(0, 0.0), (0, size), (1, 0.0), (1, size), (2, 0.0), (2, size)]):
+        tag[np.abs(c[:, ax] - val) <= eps] = t + 1
+    return VolumeMesh(verts, tets, np.ones(len(tets), np.int64), bf, tag, {(3, 1): "Volume"})

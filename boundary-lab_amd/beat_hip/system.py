@@ -10,9 +10,10 @@ BEM surface is solved once for every excitation port (one channel each), then th
 ``radiation_impedance`` per component.
 
 Precision ``float32`` runs the mixed path (FP32 LU with FP64 refinement; results are rounded to complex64 on the
-wire as the reference's Float32 solve reports them), ``float64`` the plain FP64 path. Coupled FEM-BEM and pure
-interior systems (bounded regions) and the retained-cache ``bem_field`` operation are not served here: they fail
-with an error naming the unsupported part.
+wire as the reference's Float32 solve reports them), ``float64`` the plain FP64 path. Systems with bounded regions
+and one unbounded region are handed to the coupled FEM-BEM solve (``beat_hip.coupled_system``); pure interior
+systems (no unbounded region) and the retained-cache ``bem_field`` operation are not served: they fail with an
+error naming the unsupported part.
 """
 
 from __future__ import annotations
@@ -198,7 +199,8 @@ def parse_exterior_request(request):
     if len(unbounded) != 1:
         raise ValueError("Coupled backend currently requires exactly one unbounded region.")
     if bounded:
-        raise ValueError("Coupled FEM-BEM system solves are not served by the HIP exterior backend.")
+        raise ValueError("A system with bounded regions is a coupled FEM-BEM solve (beat_hip.coupled_system), not "
+                         "an exterior one.")
     region = unbounded[0]
     opts = _options(request)
     precision = str(opts.get("precision", "float32")).lower()
@@ -214,10 +216,10 @@ def parse_exterior_request(request):
         raise ValueError(f"Unsupported regular quadrature mode: {mode}. Expected fixed or wavelength.")
     if mode != "fixed":
         raise ValueError("Wavelength-driven regular quadrature is currently available only on the CPU backend.")
-    for key in ("quadrature_order", "singular_order"):
-        if int(opts.get(key, 4)) != 4:
-            raise ValueError(f"The HIP exterior backend integrates with order 4 rules; {key}={opts[key]} "
-                             "is not supported.")
+    regular_order = int(opts.get("quadrature_order", 4))
+    singular_order = int(opts.get("singular_order", 4))
+    if not 1 <= singular_order <= 4:
+        raise ValueError("Duffy 1D Gauss order must be between 1 and 4.")
     symmetry = G.normalized_symmetry_mode(opts.get("symmetry", "off"))
     c = float(region["sound_speed_m_per_s"])
     rho = float(region["density_kg_per_m3"])
@@ -262,6 +264,8 @@ def parse_exterior_request(request):
         "mesh_setup_s": mesh_setup_s,
         "frequencies": [float(fdt(f)) for f in request.get("frequencies_hz", [])],
         "cancel_path": request.get("cancel_path"),
+        "regular_order": regular_order,
+        "singular_order": singular_order,
     }
 
 
@@ -312,7 +316,7 @@ def frequency_result(job, freq, pressure, field, timings, first) -> dict:
         "bounded_region_count": 0,
         "interface_count": 0,
         "regular_quadrature_mode": "fixed",
-        "regular_quadrature_order": 4,
+        "regular_quadrature_order": job["regular_order"],
         "blas_threads": 1,
         "timings": {
             "assembly_s": timings["assembly_s"],
@@ -332,15 +336,24 @@ def frequency_result(job, freq, pressure, field, timings, first) -> dict:
 
 
 def solve_system_request(request, engine_factory, emit, batch: int = DEFAULT_FREQUENCY_BATCH,
-                         event_mode: bool = True):
-    """Run a compiled-system exterior request; emits one ``result`` per frequency (wrapped as an event in
-    worker mode, bare in one-shot mode) and returns (cancelled, solved_count) (coupled_solver.jl:267-569).
-    engine_factory(prepared_mesh, qunit) -> beat_hip.engine.Engine."""
+                         event_mode: bool = True, coupled_factory=None):
+    """Run a compiled-system request; emits one ``result`` per frequency (wrapped as an event in worker mode,
+    bare in one-shot mode) and returns (cancelled, solved_count). Exterior systems (coupled_solver.jl:267-569):
+    engine_factory(prepared_mesh, qunit) -> beat_hip.engine.Engine. Systems with bounded regions and one
+    unbounded region go to the coupled FEM-BEM solve (beat_hip.coupled_system; coupled_factory(model, **options)
+    -> beat_hip.coupled.CoupledSolver, default a solver on device 0)."""
+    regions = (request.get("compiled_system") or {}).get("regions", [])
+    kinds = [str(r.get("kind")) for r in regions]
+    if "bounded_air" in kinds and "unbounded_air" in kinds:
+        from .coupled_system import solve_coupled_request
+
+        return solve_coupled_request(request, emit, event_mode=event_mode, solver_factory=coupled_factory)
     job = parse_exterior_request(request)
     if _cancelled(job):
         return True, 0
     mesh, exc = job["mesh"], job["excitations"]
-    prep = G.prepare(mesh, symmetry=job["symmetry"])
+    prep = G.prepare(mesh, symmetry=job["symmetry"], regular_order=job["regular_order"],
+                     singular_order=job["singular_order"])
     qunit = unit_drives(mesh, exc)
     obs = np.concatenate(job["point_sets"], axis=0) if job["point_sets"] else None
     freqs = job["frequencies"]

@@ -1,6 +1,7 @@
 """NDJSON worker for compiled-system solves: the stdin/stdout protocol of the reference's coupled worker
 (src/blab/solvers/julia_local/coupled_solver.jl:2515-2575, consumed by src/blab/solvers/coupled_backend.py:158-200),
-served for exterior systems by the HIP engine (``beat_hip.system``).
+served by the HIP engine: exterior systems (``beat_hip.system``) and coupled FEM-BEM systems
+(``beat_hip.coupled_system``).
 
 * ``--worker``: print ``{"type":"ready"}``, then for every stdin line ``{"request": "<path>", "operation": "solve"}``
   print ``{"type":"result","result":{...}}`` per frequency and one terminal ``completed`` / ``cancelled``
@@ -17,12 +18,20 @@ import sys
 from .worker import _emit, engine_factory
 
 
+def coupled_factory(model, **options):
+    from .coupled import CoupledSolver
+    from .worker import shared_device
+
+    return CoupledSolver(model, dev=shared_device(), **options)
+
+
 def solve_path(path: str, emit=_emit):
     from .system import solve_system_request
 
     with open(path, "r", encoding="utf-8") as handle:
         request = json.load(handle)
-    cancelled, solved = solve_system_request(request, engine_factory, emit, event_mode=True)
+    cancelled, solved = solve_system_request(request, engine_factory, emit, event_mode=True,
+                                             coupled_factory=coupled_factory)
     emit({"type": "cancelled" if cancelled else "completed", "solved_count": solved})
 
 
@@ -54,7 +63,7 @@ def main(argv=None) -> int:
         from .system import solve_system_request
 
         request = json.loads(sys.stdin.read())
-        solve_system_request(request, engine_factory, _emit, event_mode=False)
+        solve_system_request(request, engine_factory, _emit, event_mode=False, coupled_factory=coupled_factory)
         return 0
     except Exception as exc:
         sys.stderr.write(f"{type(exc).__name__}: {exc}\n")

@@ -10,7 +10,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "beat_hip" / "libbemhip.so"
-SOURCES = ["bem_assembly.hip", "bem_lu.hip", "bem_field.hip", "bem_capi.hip", "bem_tables.hip"]
+SOURCES = ["bem_assembly.hip", "bem_lu.hip", "bem_field.hip", "bem_capi.hip", "bem_tables.hip", "bem_coupled.hip"]
 ARCH = os.environ.get("BEM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -36,42 +36,62 @@
 
 
 
-// phi_l(a) * w_a and psi_m(b) * w_b for the six rule points (index l*6 + a).
+// phi_l(a) * w_a for the NQ points of a regular rule (index l * NQ + a), read from constant memory by the rolled
+// test-point loops.
+template <int NQ>
 struct RuleWeights {
-  double v[18];
+  double v[3 * NQ];
   constexpr RuleWeights() : v() {
     for (int l = 0; l < 3; ++l)
-      for (int a = 0; a < 6; ++a) v[l * 6 + a] = rule_phi(l, a) * rule_w(a);
+      for (int a = 0; a < NQ; ++a) v[l * NQ + a] = qphi<NQ>(l, a) * QRule<NQ>::w(a);
   }
 };
-__constant__ RuleWeights c_phiw = RuleWeights();
+template <int NQ>
 struct RuleWeightsF {
-  float v[18];
+  float v[3 * NQ];
   constexpr RuleWeightsF() : v() {
-    for (int i = 0; i < 18; ++i) v[i] = float(RuleWeights().v[i]);
+    for (int i = 0; i < 3 * NQ; ++i) v[i] = float(RuleWeights<NQ>().v[i]);
   }
 };
-__constant__ RuleWeightsF c_phiw_f = RuleWeightsF();
+__constant__ RuleWeights<6> c_phiw6 = RuleWeights<6>();
+__constant__ RuleWeights<3> c_phiw3 = RuleWeights<3>();
+__constant__ RuleWeights<1> c_phiw1 = RuleWeights<1>();
+__constant__ RuleWeightsF<6> c_phiw6_f = RuleWeightsF<6>();
+__constant__ RuleWeightsF<3> c_phiw3_f = RuleWeightsF<3>();
+__constant__ RuleWeightsF<1> c_phiw1_f = RuleWeightsF<1>();
+template <int NQ>
+__device__ __forceinline__ double phiw(int i) {
+  if constexpr (NQ == 6) return c_phiw6.v[i];
+  else if constexpr (NQ == 3) return c_phiw3.v[i];
+  else return c_phiw1.v[i];
+}
+template <int NQ>
+__device__ __forceinline__ float phiw_f(int i) {
+  if constexpr (NQ == 6) return c_phiw6_f.v[i];
+  else if constexpr (NQ == 3) return c_phiw3_f.v[i];
+  else return c_phiw1_f.v[i];
+}
 
 // 3x3 LHS block of a regular (t,s) pair, scaled and combined: C += J/(4pi) (-D + (i/k) H).
-// UNROLL_A = 6 fully unrolls the test-point loop (FP32: test points live in SGPRs); UNROLL_A = 1 keeps it
+// UNROLL_A = NQ fully unrolls the test-point loop (FP32: test points live in SGPRs); UNROLL_A = 1 keeps it
 // rolled with wave-uniform scalar loads of the test point (FP64: halves register pressure, no spills).
-template <typename T, int UNROLL_A>
+// NQ: points of the regular rule (6, 3 or 1).
+template <typename T, int UNROLL_A, int NQ>
 __device__ __forceinline__ void regular_block(const T* __restrict__ xq, const T (&nt)[3], const T (&ct)[9], T a2t,
                                               const ElemRec<T>& se, T k, T kappa, cplx<T> (&C)[9]) {
-  T y[18];
+  T y[3 * NQ];
 #pragma unroll
-  for (int i = 0; i < 18; ++i) y[i] = se.q[i];
+  for (int i = 0; i < 3 * NQ; ++i) y[i] = se.q[i];
   const T nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
   T DMr[9], DMi[9], GMr[9], GMi[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) DMr[i] = DMi[i] = GMr[i] = GMi[i] = T(0);
 #pragma unroll UNROLL_A
-  for (int a = 0; a < 6; ++a) {
+  for (int a = 0; a < NQ; ++a) {
     const T xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
     T tDr[3] = {0, 0, 0}, tDi[3] = {0, 0, 0}, tGr[3] = {0, 0, 0}, tGi[3] = {0, 0, 0};
 #pragma unroll
-    for (int b = 0; b < 6; ++b) {
+    for (int b = 0; b < NQ; ++b) {
       const T dx = y[3 * b + 0] - xa;
       const T dy = y[3 * b + 1] - ya;
       const T dz = y[3 * b + 2] - za;
@@ -87,7 +107,7 @@ __device__ __forceinline__ void regular_block(const T* __restrict__ xq, const T 
       const T dgr = hr * td, dgi = hi * td;
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        const T cw = T(rule_phi(m, b) * rule_w(b));
+        const T cw = T(qphi<NQ>(m, b) * QRule<NQ>::w(b));
         tDr[m] += cw * dgr;
         tDi[m] += cw * dgi;
         tGr[m] += cw * gr;
@@ -96,7 +116,7 @@ __device__ __forceinline__ void regular_block(const T* __restrict__ xq, const T 
     }
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-      const T wa = T(c_phiw.v[l * 6 + a]);
+      const T wa = T(phiw<NQ>(l * NQ + a));
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         DMr[3 * l + m] += wa * tDr[m];
@@ -145,19 +165,34 @@ __device__ __forceinline__ void trial_contract(const float (&x)[6], float (&t)[3
   t[1] = base + float(kSpA) * x[2] + float(kSpB) * x[4];
   t[2] = base + float(kSpA) * x[3] + float(kSpB) * x[5];
 }
+// Trial contraction of the NQ-point rule: the orbit form above for NQ = 6, plain weighted sums otherwise.
+template <int NQ>
+__device__ __forceinline__ void trial_contract_n(const float (&x)[NQ], float (&t)[3]) {
+  if constexpr (NQ == 6) {
+    trial_contract(x, t);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      float acc = 0.f;
+#pragma unroll
+      for (int b = 0; b < NQ; ++b) acc += float(qphi<NQ>(m, b) * QRule<NQ>::w(b)) * x[b];
+      t[m] = acc;
+    }
+  }
+}
 
 // FP32 block for NFW wavenumbers at once: the pair geometry (r, 1/r) is shared by the frequencies. The trial
 // element is flat, so (y_b - x_a).n_s is the same for the six trial points of a test point: the double-layer
 // kernel is u (y_b - x_a).n_s with u = 4 pi G (ik - 1/r) / r, and the n_s offset scales the trial-contracted sum
 // once per test point. The points of a regular pair (no shared vertex, images included) are distinct.
-template <int NFW>
+template <int NFW, int NQ>
 __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, const float (&nt)[3],
                                                   const float (&ct)[9], float a2t, const ElemRec<float>& se,
                                                   const float (&k)[NFW], const float (&kappa)[NFW],
                                                   cplx<float> (&C)[NFW][9]) {
-  float y[18];
+  float y[3 * NQ];
 #pragma unroll
-  for (int i = 0; i < 18; ++i) y[i] = se.q[i];
+  for (int i = 0; i < 3 * NQ; ++i) y[i] = se.q[i];
   const float nsx = se.n[0], nsy = se.n[1], nsz = se.n[2];
   const float cs_plane = y[0] * nsx + y[1] * nsy + y[2] * nsz;
   float DMr[NFW][9], DMi[NFW][9], GMr[NFW][9], GMi[NFW][9];
@@ -166,12 +201,12 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
 #pragma unroll
     for (int i = 0; i < 9; ++i) DMr[j][i] = DMi[j][i] = GMr[j][i] = GMi[j][i] = 0.f;
 #pragma unroll 1
-  for (int a = 0; a < 6; ++a) {
+  for (int a = 0; a < NQ; ++a) {
     const float xa = xq[3 * a + 0], ya = xq[3 * a + 1], za = xq[3 * a + 2];
     const float ps = cs_plane - (xa * nsx + ya * nsy + za * nsz);  // (y_b - x_a).n_s for every b
-    float xDr[NFW][6], xDi[NFW][6], xGr[NFW][6], xGi[NFW][6];
+    float xDr[NFW][NQ], xDi[NFW][NQ], xGr[NFW][NQ], xGi[NFW][NQ];
 #pragma unroll
-    for (int b = 0; b < 6; ++b) {
+    for (int b = 0; b < NQ; ++b) {
       const float dx = y[3 * b + 0] - xa;
       const float dy = y[3 * b + 1] - ya;
       const float dz = y[3 * b + 2] - za;
@@ -193,13 +228,13 @@ __device__ __forceinline__ void regular_block_f32(const float* __restrict__ xq, 
 #pragma unroll
     for (int j = 0; j < NFW; ++j) {
       float tDr[3], tDi[3], tGr[3], tGi[3];
-      trial_contract(xDr[j], tDr);
-      trial_contract(xDi[j], tDi);
-      trial_contract(xGr[j], tGr);
-      trial_contract(xGi[j], tGi);
+      trial_contract_n<NQ>(xDr[j], tDr);
+      trial_contract_n<NQ>(xDi[j], tDi);
+      trial_contract_n<NQ>(xGr[j], tGr);
+      trial_contract_n<NQ>(xGi[j], tGi);
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        const float wa = c_phiw_f.v[l * 6 + a];
+        const float wa = phiw_f<NQ>(l * NQ + a);
         const float wd = wa * ps;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -308,7 +343,7 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
       trial_contract(xEi[j], tEi);
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        const float wa = c_phiw_f.v[l * 6 + a];
+        const float wa = phiw_f<6>(l * 6 + a);
         const float wd = wa * ps;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -354,7 +389,7 @@ __device__ __forceinline__ void regular_pair_f32(const float* __restrict__ xq, c
 
 // One wave per (test element t, group of NFW wavenumbers that share the pair geometry); both precisions launch
 // NFW = 1: FP32 then needs at most 128 VGPRs (149 with the image loop) and 7 KB of LDS, four (three) waves per SIMD.
-template <typename T, int NFW, bool SYM>
+template <typename T, int NFW, bool SYM, int NQ>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (SYM ? 3 : 4) : 1))) void
 k_regular_lhs(RegParams<T> p, int nf) {
   const int t = blockIdx.x;
@@ -434,10 +469,10 @@ k_regular_lhs(RegParams<T> p, int nf) {
         } else if (r != 0 || !adj) {
           const ElemRec<T>& se = p.rec[(size_t)r * p.F + s];
           if constexpr (sizeof(T) == 4) {
-            regular_block_f32<NFW>(te->q, nt, ct, a2t, se, k, kappa, C);
+            regular_block_f32<NFW, NQ>(te->q, nt, ct, a2t, se, k, kappa, C);
           } else {
             static_assert(NFW == 1, "FP64 runs one wavenumber per wave");
-            regular_block<T, 1>(te->q, nt, ct, a2t, se, k[0], kappa[0], C[0]);
+            regular_block<T, 1, NQ>(te->q, nt, ct, a2t, se, k[0], kappa[0], C[0]);
           }
         }
       }
@@ -815,35 +850,23 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
 // ------------------------------------------------------------------------------------------------
 
 
-template <typename T, bool SYM>
-__global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
-  constexpr int RHS_UNROLL_A = sizeof(T) == 4 ? 6 : 1;  // FP64: rolled test-point loop, no spills
-  const int t = blockIdx.x;
-  const int f = blockIdx.y;
-  const int lane = threadIdx.x;
-  const double kd = p.k[f];
-  const T k = T(kd);
-  const T kappa = T(kd / (2.0 * BEM_PI));
-  const ElemRec<T>* te = p.rec + t;
-  const int4 tv = p.fv[t];
-  const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
-  double accr[3][MAX_CH], acci[3][MAX_CH];
+// RHS-operator block of the pair (t, s) with the images of s: br + i bi [l] = sum over r of B_{t, s_r}[l], each
+// regular pair from the NQ-point rule in T arithmetic, each singular pair (adjacent, or an image touching t) from
+// its FP64 Duffy block of frequency f.
+template <typename T, bool SYM, int NQ>
+__device__ __forceinline__ void rhs_pair_block(const RhsParams<T>& p, const ElemRec<T>* te, int4 tv, int sp0,
+                                               int sp1, int mt, int s, int f, double kd, T k, T kappa,
+                                               double (&br)[3], double (&bi)[3]) {
+  constexpr int RHS_UNROLL_A = sizeof(T) == 4 ? NQ : 1;  // FP64: rolled test-point loop, no spills
+  const int4 sv = p.fv[s];
+  const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
+                   sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
+  const int ms = p.pmask ? (mt & p.pmask[s]) : 0;
 #pragma unroll
-  for (int l = 0; l < 3; ++l)
-#pragma unroll
-    for (int c = 0; c < MAX_CH; ++c) accr[l][c] = acci[l][c] = 0.0;
-
-  const int mt = p.pmask ? p.pmask[t] : 0;
-  for (int j = lane; j < p.n_src; j += 64) {
-    const int s = p.src[j];
-    const int4 sv = p.fv[s];
-    const bool adj = sv.x == tv.x || sv.x == tv.y || sv.x == tv.z || sv.y == tv.x || sv.y == tv.y ||
-                     sv.y == tv.z || sv.z == tv.x || sv.z == tv.y || sv.z == tv.z;
-    const int ms = p.pmask ? (mt & p.pmask[s]) : 0;
-    double br[3] = {0, 0, 0}, bi[3] = {0, 0, 0};
-    // the radiating face and its images (r = 0: itself); singular pairs take their FP64 Duffy block
-    const int R = SYM ? p.R : 1;
-    for (int r = 0; r < R; ++r) {
+  for (int l = 0; l < 3; ++l) br[l] = bi[l] = 0.0;
+  // the radiating face and its images (r = 0: itself); singular pairs take their FP64 Duffy block
+  const int R = SYM ? p.R : 1;
+  for (int r = 0; r < R; ++r) {
     int qs = -1;
     if (r == 0 ? adj : ((ms >> r) & 1)) {
       const int key = s + r * p.F;
@@ -858,22 +881,22 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
       T Sr[3] = {0, 0, 0}, Si[3] = {0, 0, 0}, Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
       const T ntx = te->n[0], nty = te->n[1], ntz = te->n[2];
 #pragma unroll RHS_UNROLL_A
-      for (int a = 0; a < 6; ++a) {
+      for (int a = 0; a < NQ; ++a) {
         T sgr = 0, sgi = 0, sar = 0, sai = 0;
 #pragma unroll
-        for (int b = 0; b < 6; ++b) {
+        for (int b = 0; b < NQ; ++b) {
           const T dx = se.q[3 * b] - te->q[3 * a];
           const T dy = se.q[3 * b + 1] - te->q[3 * a + 1];
           const T dz = se.q[3 * b + 2] - te->q[3 * a + 2];
           const T r2 = dx * dx + dy * dy + dz * dz;
           const T rinv = Phase<T>::rsqrt_(r2);
-          const T r = r2 * rinv;
+          const T rr = r2 * rinv;
           T sn, cs;
-          Phase<T>::sincos_kr(kappa, k, r, &sn, &cs);
+          Phase<T>::sincos_kr(kappa, k, rr, &sn, &cs);
           const T gr = cs * rinv, gi = sn * rinv;
           const T hr = -gr * rinv - gi * k, hi = gr * k - gi * rinv;
           const T ad = -(dx * ntx + dy * nty + dz * ntz) * rinv;
-          const T wb = T(rule_w(b));
+          const T wb = T(QRule<NQ>::w(b));
           sgr += wb * gr;
           sgi += wb * gi;
           sar += wb * hr * ad;
@@ -881,7 +904,7 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
         }
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
-          const T wa = T(rule_phi(l, a) * rule_w(a));
+          const T wa = T(qphi<NQ>(l, a) * QRule<NQ>::w(a));
           Sr[l] += wa * sgr;
           Si[l] += wa * sgi;
           Ar[l] += wa * sar;
@@ -902,7 +925,31 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
         bi[l] += sr[(size_t)qs * 3 + l].im;
       }
     }
-    }
+  }
+}
+
+template <typename T, bool SYM, int NQ>
+__global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
+  const int t = blockIdx.x;
+  const int f = blockIdx.y;
+  const int lane = threadIdx.x;
+  const double kd = p.k[f];
+  const T k = T(kd);
+  const T kappa = T(kd / (2.0 * BEM_PI));
+  const ElemRec<T>* te = p.rec + t;
+  const int4 tv = p.fv[t];
+  const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
+  double accr[3][MAX_CH], acci[3][MAX_CH];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < MAX_CH; ++c) accr[l][c] = acci[l][c] = 0.0;
+
+  const int mt = p.pmask ? p.pmask[t] : 0;
+  for (int j = lane; j < p.n_src; j += 64) {
+    const int s = p.src[j];
+    double br[3], bi[3];
+    rhs_pair_block<T, SYM, NQ>(p, te, tv, sp0, sp1, mt, s, f, kd, k, kappa, br, bi);
 #pragma unroll
     for (int c = 0; c < MAX_CH; ++c) {
       if (c < p.n_ch) {
@@ -926,6 +973,31 @@ __global__ __launch_bounds__(64) void k_rhs_part(RhsParams<T> p) {
     cplx<double>* o = p.out + ((size_t)f * 3 * p.F + 3 * t) * MAX_CH;
     for (int l = 0; l < 3; ++l)
       for (int c = 0; c < MAX_CH; ++c) o[l * MAX_CH + c] = mk<double>(accr[l][c], acci[l][c]);
+  }
+}
+
+// RHS-operator columns of listed source faces, one wave per (test element t, frequency):
+// out[f][3t+l][j] = B_{t, src[j]}[l] for j < n_src (lanes along j, coalesced stores). The coupled FEM-BEM
+// system needs the operator applied to many sparse Neumann columns (interface fluxes, piston motions); the
+// per-element blocks are formed once and combined per column by k_cols_gather.
+template <typename T, bool SYM, int NQ>
+__global__ __launch_bounds__(64) void k_rhs_cols(RhsParams<T> p) {
+  const int t = blockIdx.x;
+  const int f = blockIdx.y;
+  const int lane = threadIdx.x;
+  const double kd = p.k[f];
+  const T k = T(kd);
+  const T kappa = T(kd / (2.0 * BEM_PI));
+  const ElemRec<T>* te = p.rec + t;
+  const int4 tv = p.fv[t];
+  const int sp0 = p.sing_ptr[t], sp1 = p.sing_ptr[t + 1];
+  const int mt = p.pmask ? p.pmask[t] : 0;
+  cplx<double>* o = p.out + ((size_t)f * 3 * p.F + 3 * (size_t)t) * p.n_src;
+  for (int j = lane; j < p.n_src; j += 64) {
+    double br[3], bi[3];
+    rhs_pair_block<T, SYM, NQ>(p, te, tv, sp0, sp1, mt, p.src[j], f, kd, k, kappa, br, bi);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) o[(size_t)l * p.n_src + j] = mk<double>(br[l], bi[l]);
   }
 }
 
@@ -1094,7 +1166,7 @@ template <typename T>
 hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
   constexpr int NFW = 1;  // one wavenumber per wave: four FP32 waves per SIMD (see k_regular_pair)
   if constexpr (sizeof(T) == 4) {
-    if (p.strip2) {
+    if (p.strip2 && p.nq == 6) {
       // one wavenumber per wave: 9 KB of LDS and at most 128 VGPRs give four waves per SIMD, against two
       // (255 VGPRs, 18 KB) when a wave shares the pair geometry between two wavenumbers: 13 % faster on the
       // 6,000-triangle horn job
@@ -1102,10 +1174,23 @@ hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
       return hipGetLastError();
     }
   }
-  if (p.R > 1)
-    hipLaunchKernelGGL((k_regular_lhs<T, NFW, true>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
-  else
-    hipLaunchKernelGGL((k_regular_lhs<T, NFW, false>), dim3(p.F, (nf + NFW - 1) / NFW), dim3(64), 0, st, p, nf);
+  const dim3 grid(p.F, (nf + NFW - 1) / NFW);
+  if (p.nq == 6) {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_regular_lhs<T, NFW, true, 6>), grid, dim3(64), 0, st, p, nf);
+    else
+      hipLaunchKernelGGL((k_regular_lhs<T, NFW, false, 6>), grid, dim3(64), 0, st, p, nf);
+  } else if (p.nq == 3) {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_regular_lhs<T, NFW, true, 3>), grid, dim3(64), 0, st, p, nf);
+    else
+      hipLaunchKernelGGL((k_regular_lhs<T, NFW, false, 3>), grid, dim3(64), 0, st, p, nf);
+  } else {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_regular_lhs<T, NFW, true, 1>), grid, dim3(64), 0, st, p, nf);
+    else
+      hipLaunchKernelGGL((k_regular_lhs<T, NFW, false, 1>), grid, dim3(64), 0, st, p, nf);
+  }
   return hipGetLastError();
 }
 hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
@@ -1121,10 +1206,44 @@ hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
 }
 template <typename T>
 hipError_t launch_rhs(const RhsParams<T>& p, int nf, hipStream_t st) {
-  if (p.R > 1)
-    hipLaunchKernelGGL((k_rhs_part<T, true>), dim3(p.F, nf), dim3(64), 0, st, p);
-  else
-    hipLaunchKernelGGL((k_rhs_part<T, false>), dim3(p.F, nf), dim3(64), 0, st, p);
+  const dim3 grid(p.F, nf);
+  if (p.nq == 6) {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_rhs_part<T, true, 6>), grid, dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_rhs_part<T, false, 6>), grid, dim3(64), 0, st, p);
+  } else if (p.nq == 3) {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_rhs_part<T, true, 3>), grid, dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_rhs_part<T, false, 3>), grid, dim3(64), 0, st, p);
+  } else {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_rhs_part<T, true, 1>), grid, dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_rhs_part<T, false, 1>), grid, dim3(64), 0, st, p);
+  }
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_rhs_cols(const RhsParams<T>& p, int nf, hipStream_t st) {
+  const dim3 grid(p.F, nf);
+  if (p.nq == 6) {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_rhs_cols<T, true, 6>), grid, dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_rhs_cols<T, false, 6>), grid, dim3(64), 0, st, p);
+  } else if (p.nq == 3) {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_rhs_cols<T, true, 3>), grid, dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_rhs_cols<T, false, 3>), grid, dim3(64), 0, st, p);
+  } else {
+    if (p.R > 1)
+      hipLaunchKernelGGL((k_rhs_cols<T, true, 1>), grid, dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL((k_rhs_cols<T, false, 1>), grid, dim3(64), 0, st, p);
+  }
   return hipGetLastError();
 }
 template <typename T>
@@ -1155,6 +1274,8 @@ template hipError_t launch_gather_rhs<double>(const GatherParams<double>&, int, 
 template hipError_t launch_regular<float>(const RegParams<float>&, int, hipStream_t);
 template hipError_t launch_regular<double>(const RegParams<double>&, int, hipStream_t);
 template hipError_t launch_rhs<float>(const RhsParams<float>&, int, hipStream_t);
+template hipError_t launch_rhs_cols<float>(const RhsParams<float>&, int, hipStream_t);
+template hipError_t launch_rhs_cols<double>(const RhsParams<double>&, int, hipStream_t);
 template hipError_t launch_rhs<double>(const RhsParams<double>&, int, hipStream_t);
 template hipError_t launch_gather<float>(const GatherParams<float>&, int, hipStream_t);
 template hipError_t launch_gather<double>(const GatherParams<double>&, int, hipStream_t);

@@ -140,6 +140,7 @@ struct bem_mesh {
   int R;        // symmetry transforms (identity first): element records and field sources exist per transform
   int sgn[12];  // their sign vectors
   double m11[9], m10[3];
+  int nq;  // points of the regular rule per face (6, 3 or 1)
   std::vector<DevBuf> owned;
   // device arrays
   ElemRec<float>* rec32;
@@ -159,7 +160,7 @@ struct bem_mesh {
   double* roww;  // [V] orbit row weights or null
   int* pmask;    // [F] image-pair participation bits or null
   // workspace
-  DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial;
+  DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial, cols_part;
   int strip_nf = 0;
   // captured graphs whose nodes address this mesh's buffers; while any exists the workspace may not move
   int live_graphs = 0;
@@ -449,6 +450,8 @@ static std::string desc_problem(const bem_mesh_desc* d) {
   for (int r = 0; r < d->n_rules; ++r)
     if (d->rule_off[r] < 0 || d->rule_len[r] <= 0 || (long long)d->rule_off[r] + d->rule_len[r] > (1LL << 26))
       return "rule offset or length out of range";
+  if (d->n_qp != 0 && d->n_qp != 6 && d->n_qp != 3 && d->n_qp != 1)
+    return "n_qp must be 6 (order 4), 3 (order 2) or 1 (order 1)";
   if (d->n_src < 0 || (d->n_src > 0 && !d->src_faces)) return "invalid radiating-face list";
   for (int i = 0; i < d->n_src; ++i)
     if (d->src_faces[i] < 0 || d->src_faces[i] >= F) return "radiating face out of range";
@@ -498,22 +501,24 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
   m->n_rules = d->n_rules;
   m->n_src = d->n_src;
   m->R = R;
-  m->M = 6 * F * R;
+  const int nq = d->n_qp ? d->n_qp : 6;
+  m->nq = nq;
+  m->M = nq * F * R;
   for (int i = 0; i < 12; ++i) m->sgn[i] = i < 3 * R && d->n_transforms > 0 ? d->transform_signs[i] : 1;
   std::memcpy(m->m11, d->m11, sizeof(m->m11));
   std::memcpy(m->m10, d->m10, sizeof(m->m10));
   // host-side records: transform r of element t at r*F + t (reflected rule points and normal, curls times
-  // det * signs: BeatEngineCore.jl:341-356); field sources transform-major, 6 per element
+  // det * signs: BeatEngineCore.jl:341-356); field sources transform-major, nq per element
   std::vector<ElemRec<float>> r32((size_t)F * R);
   std::vector<ElemRec<double>> r64((size_t)F * R);
-  std::vector<SrcRec<float>> s32(6 * (size_t)F * R);
-  std::vector<SrcRec<double>> s64(6 * (size_t)F * R);
-  std::vector<float> w32(6 * (size_t)F);
-  std::vector<double> w64(6 * (size_t)F);
+  std::vector<SrcRec<float>> s32(nq * (size_t)F * R);
+  std::vector<SrcRec<double>> s64(nq * (size_t)F * R);
+  std::vector<float> w32(nq * (size_t)F);
+  std::vector<double> w64(nq * (size_t)F);
   std::vector<int4> fv(F);
   for (int t = 0; t < F; ++t) {
     for (int i = 0; i < 18; ++i) {
-      r64[t].q[i] = d->qpoints[18 * (size_t)t + i];
+      r64[t].q[i] = i < 3 * nq ? d->qpoints[18 * (size_t)t + i] : 0.0;
       r32[t].q[i] = (float)r64[t].q[i];
     }
     for (int i = 0; i < 3; ++i) {
@@ -529,15 +534,15 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
     r64[t].pad = 0;
     r32[t].pad = 0;
     fv[t] = make_int4(d->faces[3 * (size_t)t], d->faces[3 * (size_t)t + 1], d->faces[3 * (size_t)t + 2], 0);
-    for (int q = 0; q < 6; ++q) {
-      const size_t mm = 6 * (size_t)t + q;
+    for (int q = 0; q < nq; ++q) {
+      const size_t mm = nq * (size_t)t + q;
       for (int i = 0; i < 3; ++i) {
         s64[mm].y[i] = r64[t].q[3 * q + i];
         s64[mm].n[i] = r64[t].n[i];
         s32[mm].y[i] = (float)s64[mm].y[i];
         s32[mm].n[i] = (float)s64[mm].n[i];
       }
-      w64[mm] = rule_w(q) * d->area2[t];
+      w64[mm] = qrule_w(nq, q) * d->area2[t];
       w32[mm] = (float)w64[mm];
     }
   }
@@ -547,7 +552,7 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
     for (int t = 0; t < F; ++t) {
       ElemRec<double>& e = r64[(size_t)r * F + t];
       e = r64[t];
-      for (int q = 0; q < 6; ++q)
+      for (int q = 0; q < nq; ++q)
         for (int i = 0; i < 3; ++i) e.q[3 * q + i] *= sg[i];
       for (int i = 0; i < 3; ++i) e.n[i] *= sg[i];
       for (int l = 0; l < 3; ++l)
@@ -558,8 +563,8 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
       for (int i = 0; i < 9; ++i) ef.curl[i] = (float)e.curl[i];
       ef.a2 = (float)e.a2;
       ef.pad = 0;
-      for (int q = 0; q < 6; ++q) {
-        const size_t mm = (size_t)r * 6 * F + 6 * (size_t)t + q;
+      for (int q = 0; q < nq; ++q) {
+        const size_t mm = (size_t)r * nq * F + nq * (size_t)t + q;
         for (int i = 0; i < 3; ++i) {
           s64[mm].y[i] = e.q[3 * q + i];
           s64[mm].n[i] = e.n[i];
@@ -652,7 +657,7 @@ static void mesh_free(bem_mesh* m) {
   (void)hipSetDevice(m->ctx->device);
   for (auto& b : m->owned)
     if (b.p) (void)hipFree(b.p);
-  for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial})
+  for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial, &m->cols_part})
     if (b->p) (void)hipFree(b->p);
   delete m;
 }
@@ -696,7 +701,7 @@ static int reserve_impl(bem_mesh* m, int precision, int nf, int strip_nf, int n_
   if ((rc = dev_alloc(m, m->sing_rhs, (size_t)nf * std::max(1, m->P) * 3 * 16))) return rc;
   if ((rc = dev_alloc(m, m->rhs_part, (size_t)nf * 3 * m->F * BEM_MAX_CHANNELS * 16))) return rc;
   if (n_obs > 0) {
-    if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * 6 * m->F * 2 * cb))) return rc;
+    if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->nq * m->F * 2 * cb))) return rc;
     const int ns = field_slices(m, n_obs, nf);
     if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
   }
@@ -714,7 +719,8 @@ int bem_mesh_device_bytes(bem_mesh* m, long long* bytes) {
   if (!m || !bytes) return fail(BEM_E_ARG, "NULL argument");
   size_t s = 0;
   for (auto& b : m->owned) s += b.bytes;
-  for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial}) s += b->bytes;
+  for (DevBuf* b : {&m->strip, &m->sing_lhs, &m->sing_rhs, &m->rhs_part, &m->wsrc, &m->partial, &m->cols_part})
+    s += b->bytes;
   *bytes = (long long)s;
   return BEM_OK;
 }
@@ -773,6 +779,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   rp.n_ch = n_ch;
   rp.R = m->R;
   rp.pmask = m->pmask;
+  rp.nq = m->nq;
   {
     ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
     HIPCHK(launch_rhs<T>(rp, nf, st), "k_rhs_part launch");
@@ -799,8 +806,9 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.sing_lhs = static_cast<const cplx<double>*>(m->sing_lhs.p) + (size_t)f0 * m->P * 9;
     p.k = k + f0;
     p.strip = static_cast<cplx<T>*>(m->strip.p);
-    // paired assembly (FP32, no images): the column strip follows the row strips of the group
-    p.strip2 = (sizeof(T) == 4 && m->R == 1) ? p.strip + (size_t)m->strip_nf * fstride : nullptr;
+    // paired assembly (FP32, no images, order-4 rule): the column strip follows the row strips of the group
+    p.strip2 = (sizeof(T) == 4 && m->R == 1 && m->nq == 6) ? p.strip + (size_t)m->strip_nf * fstride : nullptr;
+    p.nq = m->nq;
     p.strip_fstride = fstride;
     p.L = m->L;
     p.F = m->F;
@@ -893,6 +901,7 @@ int bem_assemble_rhs(bem_ctx* c, bem_mesh* m, int precision, int nf, const doubl
     rp.n_ch = n_ch;
     rp.R = m->R;
     rp.pmask = m->pmask;
+    rp.nq = m->nq;
     hipError_t e2;
     {
       ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
@@ -916,6 +925,212 @@ int bem_assemble_rhs(bem_ctx* c, bem_mesh* m, int precision, int nf, const doubl
   };
   e = precision == BEM_PREC_MIXED ? run(float()) : run(double());
   if (e != hipSuccess) return hipfail(e, "assemble_rhs launch");
+  return BEM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// Coupled FEM-BEM system blocks
+// ------------------------------------------------------------------------------------------------
+
+struct bem_colset {
+  bem_mesh* mesh;
+  int n_src = 0, n_cols = 0;
+  long long nnz = 0;
+  int* src = nullptr;              // [n_src] face ids
+  int* col_ptr = nullptr;          // [n_cols + 1]
+  int* col_idx = nullptr;          // [nnz] positions in src
+  cplx<double>* col_w = nullptr;   // [nnz]
+};
+
+struct bem_sparse {
+  int n_rows = 0, n_cols = 0;
+  long long nnz = 0;
+  int* rows = nullptr;
+  int* cols = nullptr;
+  cplx<double>* vals = nullptr;
+};
+
+template <typename U>
+static int raw_upload(U** dst, const U* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return BEM_OK;
+  hipError_t e = hipMalloc((void**)dst, count * sizeof(U));
+  if (e != hipSuccess) return fail(BEM_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  e = hipMemcpy(*dst, src, count * sizeof(U), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hipfail(e, "upload");
+  return BEM_OK;
+}
+
+static void colset_free(bem_colset* cs) {
+  if (!cs) return;
+  for (void* p : {(void*)cs->src, (void*)cs->col_ptr, (void*)cs->col_idx, (void*)cs->col_w})
+    if (p) (void)hipFree(p);
+  delete cs;
+}
+
+static void sparse_free(bem_sparse* sp) {
+  if (!sp) return;
+  for (void* p : {(void*)sp->rows, (void*)sp->cols, (void*)sp->vals})
+    if (p) (void)hipFree(p);
+  delete sp;
+}
+
+extern "C" {
+
+int bem_colset_create(bem_ctx* c, bem_mesh* m, int n_src, const int* src, int n_cols, const int* col_ptr,
+                      const int* col_idx, const double* col_w, bem_colset** out) {
+  if (!c || !m || !out || (n_src > 0 && !src) || n_cols < 0 || n_src < 0 || !col_ptr)
+    return fail(BEM_E_ARG, "NULL argument or negative size");
+  *out = nullptr;
+  if (col_ptr[0] != 0) return fail(BEM_E_ARG, "col_ptr must start at 0");
+  for (int j = 0; j < n_cols; ++j)
+    if (col_ptr[j + 1] < col_ptr[j]) return fail(BEM_E_ARG, "col_ptr not monotone");
+  const long long nnz = col_ptr[n_cols];
+  if (nnz > 0 && (!col_idx || !col_w)) return fail(BEM_E_ARG, "NULL column entries");
+  for (int j = 0; j < n_src; ++j)
+    if (src[j] < 0 || src[j] >= m->F) return fail(BEM_E_ARG, "column source face out of range");
+  for (long long e = 0; e < nnz; ++e)
+    if (col_idx[e] < 0 || col_idx[e] >= n_src) return fail(BEM_E_ARG, "column entry outside the source list");
+  if (c->capturing) return fail(BEM_E_STATE, "bem_colset_create allocates; not during graph capture");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  bem_colset* cs = new bem_colset();
+  cs->mesh = m;
+  cs->n_src = n_src;
+  cs->n_cols = n_cols;
+  cs->nnz = nnz;
+  int rc;
+  if ((rc = raw_upload(&cs->src, src, (size_t)n_src)) || (rc = raw_upload(&cs->col_ptr, col_ptr, (size_t)n_cols + 1)) ||
+      (rc = raw_upload(&cs->col_idx, col_idx, (size_t)nnz)) ||
+      (rc = raw_upload(&cs->col_w, reinterpret_cast<const cplx<double>*>(col_w), (size_t)nnz))) {
+    colset_free(cs);
+    return rc;
+  }
+  *out = cs;
+  return BEM_OK;
+}
+
+int bem_colset_destroy(bem_colset* cs) {
+  colset_free(cs);
+  return BEM_OK;
+}
+
+int bem_rhs_columns(bem_ctx* c, bem_mesh* m, const bem_colset* cs, int precision, int nf, const double* k,
+                    const void* alpha, void* out, long long ldo, long long strideO) {
+  if (!c || !m || !cs || !k || !alpha || !out) return fail(BEM_E_ARG, "NULL argument");
+  if (cs->mesh != m) return fail(BEM_E_ARG, "the column set belongs to another mesh");
+  if (nf <= 0) return fail(BEM_E_ARG, "invalid nf");
+  if (precision != BEM_PREC_MIXED && precision != BEM_PREC_FP64) return fail(BEM_E_ARG, "unknown precision");
+  if (ldo < cs->n_cols || strideO < ldo * m->V) return fail(BEM_E_ARG, "leading dimension or stride too small");
+  if (m->sing_k != k || m->sing_nf < nf)
+    return fail(BEM_E_STATE, "bem_rhs_columns reuses the singular blocks of the last bem_assemble on this mesh, "
+                             "which was called with other wavenumbers");
+  if (cs->n_cols == 0) return BEM_OK;
+  int rc;
+  if ((rc = dev_alloc(m, m->cols_part, std::max<size_t>(1, (size_t)nf * 3 * m->F * cs->n_src * 16)))) return rc;
+  note_capture_use(c, m);
+  hipStream_t st = c->stream;
+  auto run = [&](auto tag) -> hipError_t {
+    using T = decltype(tag);
+    hipError_t e2;
+    if (cs->n_src > 0) {
+      RhsParams<T> rp{};
+      rp.rec = sizeof(T) == 4 ? (const ElemRec<T>*)m->rec32 : (const ElemRec<T>*)m->rec64;
+      rp.fv = m->fv;
+      rp.src = cs->src;
+      rp.sing_ptr = m->sing_ptr;
+      rp.sing_trial = m->sing_trial;
+      rp.sing_rhs = static_cast<const cplx<double>*>(m->sing_rhs.p);
+      rp.k = k;
+      rp.out = static_cast<cplx<double>*>(m->cols_part.p);
+      rp.F = m->F;
+      rp.P = m->P;
+      rp.n_src = cs->n_src;
+      rp.n_ch = 0;
+      rp.R = m->R;
+      rp.pmask = m->pmask;
+      rp.nq = m->nq;
+      ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
+      if ((e2 = launch_rhs_cols<T>(rp, nf, st)) != hipSuccess) return e2;
+    }
+    ScopedTimer tm(c, BEM_TIMER_GATHER, (long long)nf);
+    return cols_gather<T>(static_cast<const cplx<double>*>(m->cols_part.p), m->F, cs->n_src, m->vstar_ptr,
+                          m->vstar_row, m->roww, m->V, cs->n_cols, cs->col_ptr, cs->col_idx, cs->col_w,
+                          static_cast<const cplx<double>*>(alpha), static_cast<cplx<T>*>(out), ldo, strideO, nf, st);
+  };
+  hipError_t e = precision == BEM_PREC_MIXED ? run(float()) : run(double());
+  if (e != hipSuccess) return hipfail(e, "rhs_columns launch");
+  return BEM_OK;
+}
+
+int bem_sparse_create(bem_ctx* c, int n_rows, int n_cols, long long nnz, const int* rows, const int* cols,
+                      const double* vals, bem_sparse** out) {
+  if (!c || !out || n_rows < 0 || n_cols < 0 || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals)))
+    return fail(BEM_E_ARG, "NULL argument or negative size");
+  *out = nullptr;
+  std::vector<long long> keys((size_t)nnz);
+  for (long long e = 0; e < nnz; ++e) {
+    if (rows[e] < 0 || rows[e] >= n_rows || cols[e] < 0 || cols[e] >= n_cols)
+      return fail(BEM_E_ARG, "sparse entry outside the block");
+    keys[(size_t)e] = (long long)rows[e] * n_cols + cols[e];
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+    return fail(BEM_E_ARG, "sparse entries must be unique (sum duplicates first)");
+  if (c->capturing) return fail(BEM_E_STATE, "bem_sparse_create allocates; not during graph capture");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  bem_sparse* sp = new bem_sparse();
+  sp->n_rows = n_rows;
+  sp->n_cols = n_cols;
+  sp->nnz = nnz;
+  int rc;
+  if ((rc = raw_upload(&sp->rows, rows, (size_t)nnz)) || (rc = raw_upload(&sp->cols, cols, (size_t)nnz)) ||
+      (rc = raw_upload(&sp->vals, reinterpret_cast<const cplx<double>*>(vals), (size_t)nnz))) {
+    sparse_free(sp);
+    return rc;
+  }
+  *out = sp;
+  return BEM_OK;
+}
+
+int bem_sparse_destroy(bem_sparse* sp) {
+  sparse_free(sp);
+  return BEM_OK;
+}
+
+int bem_sparse_scatter(bem_ctx* c, const bem_sparse* sp, int precision, double alpha_re, double alpha_im, int add,
+                       void* out, long long ldo) {
+  if (!c || !sp || !out) return fail(BEM_E_ARG, "NULL argument");
+  if (ldo < sp->n_cols) return fail(BEM_E_ARG, "leading dimension smaller than the block");
+  hipError_t e;
+  const cplx<double> a = mk<double>(alpha_re, alpha_im);
+  if (precision == BEM_PREC_MIXED)
+    e = scatter_coo<float>(sp->nnz, sp->rows, sp->cols, sp->vals, a, add, static_cast<cplx<float>*>(out), ldo,
+                           c->stream);
+  else if (precision == BEM_PREC_FP64)
+    e = scatter_coo<double>(sp->nnz, sp->rows, sp->cols, sp->vals, a, add, static_cast<cplx<double>*>(out), ldo,
+                            c->stream);
+  else
+    return fail(BEM_E_ARG, "unknown precision");
+  if (e != hipSuccess) return hipfail(e, "sparse scatter launch");
+  return BEM_OK;
+}
+
+int bem_dense_block(bem_ctx* c, int precision, int m, int n, const void* src, long long lds, double alpha_re,
+                    double alpha_im, int add, void* out, long long ldo) {
+  if (!c || !out || m < 0 || n < 0) return fail(BEM_E_ARG, "NULL argument or negative size");
+  if (ldo < n || (src && lds < n)) return fail(BEM_E_ARG, "leading dimension smaller than the block");
+  hipError_t e;
+  const cplx<double> a = mk<double>(alpha_re, alpha_im);
+  const cplx<double>* s = static_cast<const cplx<double>*>(src);
+  if (precision == BEM_PREC_MIXED)
+    e = dense_block<float>(m, n, s, lds, a, add, static_cast<cplx<float>*>(out), ldo, c->stream);
+  else if (precision == BEM_PREC_FP64)
+    e = dense_block<double>(m, n, s, lds, a, add, static_cast<cplx<double>*>(out), ldo, c->stream);
+  else
+    return fail(BEM_E_ARG, "unknown precision");
+  if (e != hipSuccess) return hipfail(e, "dense block launch");
   return BEM_OK;
 }
 
@@ -1160,19 +1375,19 @@ int bem_field(bem_ctx* c, bem_mesh* m, int precision, int nf, const double* k, i
   const int ns = field_slices(m, n_obs, nf);
   const size_t cb = precision == BEM_PREC_MIXED ? 8 : 16;
   int rc;
-  if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * 6 * m->F * 2 * cb))) return rc;
+  if ((rc = dev_alloc(m, m->wsrc, (size_t)nf * n_ch * m->nq * m->F * 2 * cb))) return rc;
   if ((rc = dev_alloc(m, m->partial, (size_t)nf * ns * n_ch * n_obs * 16))) return rc;
   note_capture_use(c, m);
   hipError_t e;
   ScopedTimer tm(c, BEM_TIMER_FIELD, (long long)nf);
   if (precision == BEM_PREC_MIXED) {
-    e = field_batch<float>(m->src32, m->fv, m->srcw32, obs, m->M, 6 * m->F, n_obs, k, (const cplx<double>*)p, ldp,
-                           strideP, (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<float>*)m->wsrc.p,
-                           (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, c->stream);
+    e = field_batch<float>(m->src32, m->fv, m->srcw32, obs, m->M, m->nq * m->F, n_obs, k, (const cplx<double>*)p,
+                           ldp, strideP, (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<float>*)m->wsrc.p,
+                           (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, m->nq, c->stream);
   } else {
-    e = field_batch<double>(m->src64, m->fv, m->srcw64, obs, m->M, 6 * m->F, n_obs, k, (const cplx<double>*)p, ldp, strideP,
-                            (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<double>*)m->wsrc.p,
-                            (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, c->stream);
+    e = field_batch<double>(m->src64, m->fv, m->srcw64, obs, m->M, m->nq * m->F, n_obs, k, (const cplx<double>*)p,
+                            ldp, strideP, (const cplx<double>*)q, m->F, strideQ, n_ch, nf, (cplx<double>*)m->wsrc.p,
+                            (cplx<double>*)m->partial.p, ns, (cplx<double>*)out, ldo, strideO, m->nq, c->stream);
   }
   if (e != hipSuccess) return hipfail(e, "field launch");
   return BEM_OK;

@@ -146,3 +146,38 @@ __host__ __device__ constexpr double rule_w(int q) {
 __host__ __device__ constexpr double rule_phi(int l, int q) {
   return l == 0 ? 1.0 - rule_xi(q) - rule_eta(q) : (l == 1 ? rule_xi(q) : rule_eta(q));
 }
+
+// The regular triangle rules a mesh can be integrated with, by point count (BeatEngineCore.jl:517-549):
+// NQ = 6 the order-4 rule above, NQ = 3 the order-2 rule (points (1/6, 1/6), (2/3, 1/6), (1/6, 2/3), weights 1/6),
+// NQ = 1 the centroid rule (weight 1/2). Weights sum to 1/2, the reference triangle's area.
+template <int NQ>
+struct QRule;
+template <>
+struct QRule<6> {
+  __host__ __device__ static constexpr double xi(int q) { return rule_xi(q); }
+  __host__ __device__ static constexpr double eta(int q) { return rule_eta(q); }
+  __host__ __device__ static constexpr double w(int q) { return rule_w(q); }
+};
+template <>
+struct QRule<3> {
+  __host__ __device__ static constexpr double xi(int q) { return q == 1 ? 2.0 / 3.0 : 1.0 / 6.0; }
+  __host__ __device__ static constexpr double eta(int q) { return q == 2 ? 2.0 / 3.0 : 1.0 / 6.0; }
+  __host__ __device__ static constexpr double w(int q) { return (void)q, 1.0 / 6.0; }
+};
+template <>
+struct QRule<1> {
+  __host__ __device__ static constexpr double xi(int q) { return (void)q, 1.0 / 3.0; }
+  __host__ __device__ static constexpr double eta(int q) { return (void)q, 1.0 / 3.0; }
+  __host__ __device__ static constexpr double w(int q) { return (void)q, 0.5; }
+};
+template <int NQ>
+__host__ __device__ constexpr double qphi(int l, int q) {
+  return l == 0 ? 1.0 - QRule<NQ>::xi(q) - QRule<NQ>::eta(q) : (l == 1 ? QRule<NQ>::xi(q) : QRule<NQ>::eta(q));
+}
+// w_q and phi_l(q) of the NQ-point rule at run time (q < NQ).
+__host__ __device__ inline double qrule_w(int nq, int q) {
+  return nq == 6 ? QRule<6>::w(q) : nq == 3 ? QRule<3>::w(q) : QRule<1>::w(q);
+}
+__host__ __device__ inline double qrule_phi(int nq, int l, int q) {
+  return nq == 6 ? qphi<6>(l, q) : nq == 3 ? qphi<3>(l, q) : qphi<1>(l, q);
+}

@@ -3,7 +3,7 @@
 // Replaces src/blab/solvers/julia_local/src/BeatEngineRocmField.jl:70-202 (weighted-source and
 // thread-per-point field kernels) and BeatEngineCpuField.jl:1-45:
 //   u(x) = sum_m [ dG/dn_y(x, y_m) p_w(m) - G(x, y_m) q_w(m) ],
-//   p_w(m) = w_m sum_l phi_l(q) p[face_l], q_w(m) = w_m q[face], y_m the order-4 rule points.
+//   p_w(m) = w_m sum_l phi_l(q) p[face_l], q_w(m) = w_m q[face], y_m the regular rule points (nq per face).
 // With x / xy plane symmetry the sources of every image transform follow those of the fundamental domain
 // (transform-major) and share its weighted values (BeatEngineCore.jl:1245-1280).
 // Design: the transfer matrix is generated on the fly (VALU bound, never stored). A block holds 64 observation
@@ -32,14 +32,14 @@ struct FieldParams {
 template <typename T>
 __global__ void k_field_sources(const int4* fv, const T* src_w, const cplx<double>* p, long long ldp,
                                 long long strideP, const cplx<double>* q, long long ldq, long long strideQ,
-                                cplx<T>* wsrc, int M, int n_ch) {
+                                cplx<T>* wsrc, int M, int n_ch, int nq) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y, f = blockIdx.z;
   if (m >= M) return;
-  const int face = m / 6, qp = m % 6;
+  const int face = m / nq, qp = m % nq;  // nq rule points per face
   const int4 v = fv[face];
   const cplx<double>* pp = p + (size_t)f * strideP + (size_t)c * ldp;
-  const double b0 = rule_phi(0, qp), b1 = rule_phi(1, qp), b2 = rule_phi(2, qp);
+  const double b0 = qrule_phi(nq, 0, qp), b1 = qrule_phi(nq, 1, qp), b2 = qrule_phi(nq, 2, qp);
   const double w = double(src_w[m]);
   const cplx<double> p0 = pp[v.x], p1 = pp[v.y], p2 = pp[v.z];
   const double pr = (b0 * p0.re + b1 * p1.re + b2 * p2.re) * w;
@@ -135,10 +135,10 @@ hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, con
                        int n_obs, const double* k, const cplx<double>* p, long long ldp, long long strideP,
                        const cplx<double>* q, long long ldq, long long strideQ, int n_ch, int nf, cplx<T>* wsrc,
                        cplx<double>* partial, int n_slices, cplx<double>* out, long long ldo, long long strideO,
-                       hipStream_t st) {
-  // weighted sources of the fundamental domain (M0 = 6F); image sources (M = R * M0) reuse them
+                       int nq, hipStream_t st) {
+  // weighted sources of the fundamental domain (M0 = nq F); image sources (M = R * M0) reuse them
   hipLaunchKernelGGL(k_field_sources<T>, dim3((M0 + 255) / 256, n_ch, nf), dim3(256), 0, st, fv, src_w, p, ldp,
-                     strideP, q, ldq, strideQ, wsrc, M0, n_ch);
+                     strideP, q, ldq, strideQ, wsrc, M0, n_ch, nq);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   FieldParams<T> fp;
@@ -179,8 +179,8 @@ hipError_t neumann_batch(const cplx<double>* scale, const cplx<double>* qunit, c
 template hipError_t field_batch<float>(const SrcRec<float>*, const int4*, const float*, const double*, int, int, int,
                                        const double*, const cplx<double>*, long long, long long,
                                        const cplx<double>*, long long, long long, int, int, cplx<float>*,
-                                       cplx<double>*, int, cplx<double>*, long long, long long, hipStream_t);
+                                       cplx<double>*, int, cplx<double>*, long long, long long, int, hipStream_t);
 template hipError_t field_batch<double>(const SrcRec<double>*, const int4*, const double*, const double*, int, int, int,
                                         const double*, const cplx<double>*, long long, long long,
                                         const cplx<double>*, long long, long long, int, int, cplx<double>*,
-                                        cplx<double>*, int, cplx<double>*, long long, long long, hipStream_t);
+                                        cplx<double>*, int, cplx<double>*, long long, long long, int, hipStream_t);

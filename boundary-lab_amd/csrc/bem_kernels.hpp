@@ -15,14 +15,14 @@
 // Per-element record used by the regular kernels (32 T values, 128 B in FP32).
 template <typename T>
 struct ElemRec {
-  T q[18];    // six order-4 quadrature points (x,y,z)
+  T q[18];    // the regular rule's points (x,y,z): six (order 4), three (order 2) or one (order 1)
   T n[3];     // unit normal
   T curl[9];  // P1 surface curls, 3 vectors
   T a2;       // 2 * area
   T pad;
 };
 
-// Field source point (order-4 rule point of a face) with its face normal.
+// Field source point (regular rule point of a face) with its face normal.
 template <typename T>
 struct SrcRec {
   T y[3];
@@ -53,6 +53,7 @@ struct RegParams {
   int L, F, n_chunks, P;
   int R;                        // transforms (identity + images); rec holds R*F records, image r at r*F
   const int* pmask;             // [F] bit r: face in an image pair of transform r (null: no image pairs)
+  int nq;                       // points of the regular rule (6, 3 or 1); the paired kernel needs 6
 };
 
 struct SingParams {
@@ -91,6 +92,7 @@ struct RhsParams {
   int F, P, n_src, n_ch;
   int R;                         // transforms; rec holds R*F records
   const int* pmask;              // [F] image-pair participation bits (null: none)
+  int nq;                        // points of the regular rule (6, 3 or 1)
 };
 
 template <typename T>
@@ -131,6 +133,26 @@ hipError_t launch_regular(const RegParams<T>&, int nf, hipStream_t);
 hipError_t launch_singular(const SingParams&, int nf, hipStream_t);
 template <typename T>
 hipError_t launch_rhs(const RhsParams<T>&, int nf, hipStream_t);
+// p.out is [nf][3F][p.n_src]: the RHS-operator block of every (test element, listed source face)
+template <typename T>
+hipError_t launch_rhs_cols(const RhsParams<T>&, int nf, hipStream_t);
+
+// Coupled-system blocks (bem_coupled.hip).
+// out[f] (cT, row-major, ldo) row i < N, column c < n_cols: alpha[f] * w_i * sum over strip rows r of star(i)
+// and entries e of column c of col_w[e] * part[f][r][col_idx[e]] (part: [nf][3F][n_src] from launch_rhs_cols).
+template <typename T>
+hipError_t cols_gather(const cplx<double>* part, int F, int n_src, const int* vstar_ptr, const int* vstar_row,
+                       const double* roww, int N, int n_cols, const int* col_ptr, const int* col_idx,
+                       const cplx<double>* col_w, const cplx<double>* alpha, cplx<T>* out, long long ldo,
+                       long long strideO, int nf, hipStream_t st);
+// out[rows[e] * ldo + cols[e]] (=, or += when add) alpha * vals[e] for e < nnz (entries unique per call).
+template <typename T>
+hipError_t scatter_coo(long long nnz, const int* rows, const int* cols, const cplx<double>* vals,
+                       cplx<double> alpha, int add, cplx<T>* out, long long ldo, hipStream_t st);
+// out[i * ldo + j] (=, or += when add) alpha * src[i * lds + j] (src null: alpha) for i < m, j < n.
+template <typename T>
+hipError_t dense_block(int m, int n, const cplx<double>* src, long long lds, cplx<double> alpha, int add,
+                       cplx<T>* out, long long ldo, hipStream_t st);
 template <typename T>
 hipError_t launch_gather(const GatherParams<T>&, int nf, hipStream_t);
 template <typename T>
@@ -179,4 +201,4 @@ hipError_t field_batch(const SrcRec<T>* src, const int4* fv, const T* src_w, con
                        const double* k, const cplx<double>* p, long long ldp, long long strideP,
                        const cplx<double>* q, long long ldq, long long strideQ, int n_ch, int nf, cplx<T>* wsrc,
                        cplx<double>* partial, int n_slices, cplx<double>* out, long long ldo, long long strideO,
-                       hipStream_t st);
+                       int nq, hipStream_t st);

@@ -42,6 +42,8 @@ extern "C" {
 
 typedef struct bem_ctx bem_ctx;
 typedef struct bem_mesh bem_mesh;
+typedef struct bem_colset bem_colset;
+typedef struct bem_sparse bem_sparse;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int bem_version(void);
@@ -118,7 +120,7 @@ typedef struct bem_mesh_desc {
   const double* normals;     /* [F][3] unit normals, cross(v2-v1, v3-v1) */
   const double* curls;       /* [F][9] P1 surface curls */
   const double* area2;       /* [F] 2*area */
-  const double* qpoints;     /* [F][18] order-4 rule points */
+  const double* qpoints;     /* [F][18] regular rule points (n_qp of them) */
   /* regular-assembly tables: trial elements in locality order, chunks of 64, slot tables */
   int n_chunks;
   const int* order;          /* [n_chunks*64], -1 padding */
@@ -159,6 +161,10 @@ typedef struct bem_mesh_desc {
   int transform_signs[12];
   const double* row_weights; /* [V] P1 orbit row weights applied to A and b (NULL: 1) */
   const int* plane_mask;     /* [F] bit r: the face takes part in an image pair of transform r (NULL: none) */
+  /* Points of the regular triangle rule (triangle_rule, JS/BeatEngineCore.jl:517-549): 6 (order 4; 0 means 6),
+   * 3 (orders 2 and 3) or 1 (order 1). qpoints holds that many points per face (the rest of its 18 values
+   * unused) and m11 / m10 must be the identity blocks under the same rule. */
+  int n_qp;
 } bem_mesh_desc;
 
 /* Check every index table of a mesh description on the host (no device needed): face vertex ids < V, the trial
@@ -232,6 +238,40 @@ int bem_assemble_rhs(bem_ctx* ctx, bem_mesh* mesh, int precision, int nf, const 
  * unit channel drives [n_ch][n_faces]) on the device. Replaces the q = i rho omega v_n of pressure_for_drives
  * (julia_local/solver.jl:434-470, :456). */
 int bem_neumann(bem_ctx* ctx, int nf, int n_ch, int n_faces, const void* rhs_scale, const void* qunit, void* q);
+
+/* ---- Coupled FEM-BEM system blocks (the coupled matrix is row-major cT; callers offset `out` to a block) ----
+ *
+ * A column set: sparse Neumann columns on the mesh's faces (HOST arrays, validated and uploaded once). Column
+ * c has entries e in [col_ptr[c], col_ptr[c+1]) with face src[col_idx[e]] and weight col_w[e] (c128). The
+ * interface block uses the interface flux basis (orientation / 3 on the faces around each interface vertex,
+ * assemble_interface_operators, JS/BeatEngineCoupled.jl:721-773), the motion block the piston normal
+ * velocities (assemble_transducer_operators, :775-914). Destroy it before its mesh. */
+int bem_colset_create(bem_ctx* ctx, bem_mesh* mesh, int n_src, const int* src, int n_cols, const int* col_ptr,
+                      const int* col_idx, const double* col_w, bem_colset** out);
+int bem_colset_destroy(bem_colset* cs);
+/* out[f][i][c] = alpha[f] * (B W)[i][c] for the N = n_vertices P1 rows and the set's columns, where B is the
+ * Burton-Miller RHS operator -S - (i/k)(D* + 1/2 M10) at wavenumber k[f] (device double [nf]; the same pointer
+ * and frequencies as the last bem_assemble on this mesh, whose singular blocks it reuses) and W the column set
+ * (alpha: device c128 [nf]; out: device cT, row-major with ldo >= n_cols, strideO between frequencies).
+ * Replaces bem_rhs_operator * bem_flux / * bem_motion_flux of _accelerator_coupled_bem_blocks
+ * (JS/BeatEngineCoupled.jl:1264-1406) without forming the operator. */
+int bem_rhs_columns(bem_ctx* ctx, bem_mesh* mesh, const bem_colset* cs, int precision, int nf, const double* k,
+                    const void* alpha, void* out, long long ldo, long long strideO);
+/* A sparse block (HOST COO arrays of n_rows x n_cols entries, c128 values; entries must be unique), uploaded once.
+ * Replaces build_rocm_sparse_scatter_cache (JS/BeatEngineRocmOperators.jl:40-54). */
+int bem_sparse_create(bem_ctx* ctx, int n_rows, int n_cols, long long nnz, const int* rows, const int* cols,
+                      const double* vals, bem_sparse** out);
+int bem_sparse_destroy(bem_sparse* sp);
+/* out[rows[e] * ldo + cols[e]] = alpha * vals[e] (add != 0: +=) into the block at `out` (device cT, at least
+ * n_rows x n_cols with leading dimension ldo). Replaces scatter_rocm_sparse_to_dense!
+ * (JS/BeatEngineRocmOperators.jl:56-100). */
+int bem_sparse_scatter(bem_ctx* ctx, const bem_sparse* sp, int precision, double alpha_re, double alpha_im, int add,
+                       void* out, long long ldo);
+/* out[i * ldo + j] = alpha * src[i * lds + j] (add != 0: +=) for i < m, j < n; src (device c128) may be NULL,
+ * which writes alpha (a fill). Places the dense Schur complement and the diagonal transducer blocks, and clears
+ * the coupled matrix, on the library's stream (no memset nodes). */
+int bem_dense_block(bem_ctx* ctx, int precision, int m, int n, const void* src, long long lds, double alpha_re,
+                    double alpha_im, int add, void* out, long long ldo);
 
 /* Bytes of the LU workspace for `batch` N x N matrices: the inverted 64 x 64 diagonal blocks of L and U
  * that bem_lu_factor writes and bem_lu_solve / bem_refine read (cT [batch][ceil(N/64)][2][64][64]). */

@@ -833,23 +833,26 @@ def fibonacci_observation(config):
 
 
 def solve_frequency(mesh: Mesh, freq: float, q_unit, sound_speed=343.0, rho=1.21, singular_cache=None,
-                    identity=None, points=None):
-    """One frequency of the plain method in float64: operators, BM system, dense LU, field.
+                    identity=None, points=None, regular_order: int = 4, singular_order: int = 4):
+    """One frequency of the plain method in float64: operators, BM system, dense LU, field. The regular rule
+    (triangle_rule(regular_order)) serves the regular pairs, the identity blocks and the field sources, as in
+    the reference (solver.jl:729-735); singular pairs take Duffy rules of singular_order.
 
     q_unit: (F, C) Neumann drives without the i rho omega factor (columns = channels).
-    Returns dict(pressure (V,C), q (F,C), field (N,C) if points given, lhs, rhs).
+    Returns dict(pressure (V,C), q (F,C), field (N,C) if points given, lhs, rhs, rhs_op).
     """
     omega = 2.0 * math.pi * freq
     k = omega / sound_speed
-    ops = assemble_operators(mesh, k, singular_cache=singular_cache)
-    M11, M10 = identity if identity is not None else identity_matrices(mesh)
+    rule = triangle_rule(regular_order)
+    ops = assemble_operators(mesh, k, rule=rule, singular_order=singular_order, singular_cache=singular_cache)
+    M11, M10 = identity if identity is not None else identity_matrices(mesh, rule)
     lhs, rhs_op = burton_miller(ops, M11, M10, k)
     q = 1j * rho * omega * np.asarray(q_unit, complex).reshape(mesh.n_faces, -1)
     rhs = rhs_op @ q
     p = solve_dense(lhs, rhs)
-    out = {"pressure": p, "q": q, "lhs": lhs, "rhs": rhs, "k": k}
+    out = {"pressure": p, "q": q, "lhs": lhs, "rhs": rhs, "rhs_op": rhs_op, "k": k}
     if points is not None:
-        out["field"] = evaluate_field(points, mesh, p, q, k)
+        out["field"] = evaluate_field(points, mesh, p, q, k, sources=field_sources(mesh, rule))
     return out
 
 

@@ -161,11 +161,11 @@ def test_results_carry_every_output(tmp_path, precision, cdt):
 @pytest.mark.parametrize("mutate,message", [
     (lambda r: r["compiled_system"]["regions"].append(
         {"id": "in", "name": "in", "kind": "bounded_air", "mesh_ids": [], "sound_speed_m_per_s": 343.0,
-         "density_kg_per_m3": 1.2}), "Coupled FEM-BEM"),
+         "density_kg_per_m3": 1.2}), "shared sound speed and density"),
     (lambda r: r["solver_options"].update(precision="float16"), "float32 or float64"),
     (lambda r: r["solver_options"].update(bem_backend="cuda"), "serves bem_backend 'rocm'"),
     (lambda r: r["solver_options"].update(regular_quadrature_mode="wavelength"), "only on the CPU backend"),
-    (lambda r: r["solver_options"].update(quadrature_order=2), "order 4"),
+    (lambda r: r["solver_options"].update(singular_order=5), "between 1 and 4"),
     (lambda r: r["outputs"].append({"id": "x", "quantity": "interior_pressure"}), "Unsupported exterior output"),
     (lambda r: r["outputs"].append({"id": "y", "quantity": "exterior_pressure", "options": {}}), "points_m"),
     (lambda r: r["compiled_system"]["components"][1].update(boundary_ids=[]), "no moving boundaries"),
