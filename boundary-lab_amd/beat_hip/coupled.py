@@ -616,7 +616,7 @@ class CoupledSolver:
         from .engine import Engine
 
         iro = 1j * self.model.density * 2.0 * math.pi * float(freq)
-        eng = Engine(self.prep, self.prep.permute_faces(q / iro), dev=self.dev)
+        eng = Engine(self.prep, q / iro, dev=self.dev)  # q in the caller's face order, as Engine takes it
         try:
             b = eng.batch(self.engine_precision, 1, None, self.model.density, self.model.sound_speed)
             r = b.run([float(freq)])

@@ -45,7 +45,7 @@ def test_lower_order_rules_match_oracle(gpu_device, models, orders, precision, t
 
     prep, ref, qu, pts = models[orders]
     print(f"rules {orders} {precision}", flush=True)
-    eng = Engine(prep, prep.permute_faces(qu), dev=gpu_device)
+    eng = Engine(prep, qu, dev=gpu_device)  # drives in the caller's face order; the engine permutes them
     try:
         b = eng.batch(precision, 1, pts, RHO, C0)
         b.set_frequencies([FREQ])
