@@ -244,6 +244,39 @@ def run_scale(args):
     print(json.dumps(out))
 
 
+def coupled_job(level, layers=3):
+    """Coupled FEM-BEM workload: air between spheres of 0.10 and 0.15 m (icosphere level `level`, `layers` prism
+    layers of tetrahedra) radiating through the outer sphere; the inner sphere is an electrodynamic rigid piston
+    along z (6 ohm, 0.5 mH, Bl 7, 15 g, 0.5 mm/N, 1 kg/s) driven at 2.83 V, and a separate BEM sphere of 0.05 m
+    (icosphere level - 2) at (0.32, 0, 0.05) is a unit-velocity source; bulk loss 0.01; order-2 regular and Duffy
+    rules (the coupled production defaults); 8 log-spaced frequencies 100 Hz - 2 kHz, 500 Fibonacci points at 2 m."""
+    import scipy.sparse as sp
+
+    from beat_hip import fem as FE
+    from beat_hip import geometry as G
+    from beat_hip import synthetic as syn
+    from beat_hip.coupled import CoupledModel, Excitation
+
+    vm, (V, F, T) = syn.spherical_shell(level, 0.1, 0.15, layers)
+    V2, F2, _ = syn.icosphere(max(1, level - 2), 0.05)
+    V = np.concatenate([V, V2 + np.array([0.32, 0.0, 0.05])])
+    F = np.concatenate([F, F2 + (len(V) - len(V2))])
+    T = np.concatenate([T, np.full(len(F2), 5)])
+    bm = G.BoundaryMesh(V, F, T)
+    im = FE.build_conforming_interface_map(vm, bm, syn.TAG_OUTER, syn.TAG_OUTER)
+    td = FE.Transducer("driver", [syn.TAG_INNER], [1.0], [], [], np.array([0.0, 0.0, 1.0]), 1.0, 1, 6.0, 5e-4, 7.0,
+                       0.015, 5e-4, 1.0)
+    sel = np.nonzero(T == 5)[0]
+    pv = sp.csr_matrix((np.ones(len(sel)), (sel, np.zeros(len(sel), np.int64))), shape=(len(F), 1))
+    model = CoupledModel(vm, bm, im, 1.21, 343.0, [td], pv, np.full(vm.n_vertices, 0.01), [])
+    exc = [Excitation("voltage", (), (), 0, 1, 2.83), Excitation("normal_velocity", (), (), 1, 0, 1.0)]
+    freqs = np.geomspace(100.0, 2000.0, 8)
+    i = np.arange(500) + 0.5
+    th, ph = np.arccos(1.0 - 2.0 * i / 500), math.pi * (1.0 + 5.0 ** 0.5) * i
+    pts = 2.0 * np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    return model, exc, freqs, pts
+
+
 def field_errors(fields_mixed, fields_ref):
     """Max far-field |dB| and phase error of mixed results against the FP64 path, over all frequencies."""
     db, ph = [], []
@@ -418,6 +451,105 @@ def serving_sample(stream, rate, seed, batch, slots):
             "tri_max": max(j["triangles"] for j in stream)}
 
 
+def run_coupled(args):
+    """Coupled FEM-BEM solve per frequency on the device (beat_hip.coupled), float32 production precision (FP32 LU
+    + FP64 refinement) with the requested static condensation left to the solver's cost choice, on the default
+    workload (icosphere level 3: 2,568 FEM / 684 BEM vertices) and a large one (level 4: 10,248 / 2,724);
+    float64 and forced host condensation timed beside; the oracle (FP64, monolithic, dense LAPACK) timed on one
+    frequency of the default workload as the CPU baseline and used as the parity check. Prints one JSON line."""
+    import torch
+
+    from beat_hip.coupled import CoupledSolver
+    from beat_hip.engine import Device
+
+    dev = Device(0)
+    out = {"metric": "coupled FEM-BEM freq-points/s (float32 production path vs FP64 oracle)", "n_gpus": 1,
+           "unit": "freq-points/s", "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32+fp64", "data": "synthetic", "steps": args.steps, "warmup": args.warmup}
+
+    def timed(model, exc, freqs, pts, **opts):
+        s = CoupledSolver(model, dev=dev, **opts)
+        try:
+            for _ in range(args.warmup):
+                s.solve(float(freqs[0]), exc, pts)
+            torch.cuda.synchronize()
+            stages = {}
+            iters = 0
+            t0 = time.perf_counter()
+            res = None
+            for _ in range(args.steps):
+                for f in freqs:
+                    sols, field, _, tim = s.solve(float(f), exc, pts)
+                    res = (f, sols, field) if res is None else res
+                    iters = max(iters, int(tim.pop("refinement_iterations", 0)))
+                    for k, v in tim.items():
+                        stages[k] = stages.get(k, 0.0) + v
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            n = args.steps * len(freqs)
+            return dict(value=n / el, ms_per_freq=1e3 * el / n, n=s.n, formulation="condensed" if s.condensed
+                        else "monolithic", max_refinement_iterations=iters,
+                        stages_ms={k: round(1e3 * v / n, 3) for k, v in stages.items()}), res
+        finally:
+            s.close()
+
+    model, exc, freqs, pts = coupled_job(3)
+    log(f"bench: coupled default workload ({model.fem_mesh.n_vertices} FEM / {model.bem_mesh.n_vertices} BEM vertices)")
+    r32, (f0, sols, field) = timed(model, exc, freqs, pts, precision="float32", static_condensation=True)
+    r64, _ = timed(model, exc, freqs, pts, precision="float64", static_condensation=True)
+    rcond, _ = timed(model, exc, freqs[:2], pts, precision="float32", static_condensation=True, formulation="condensed")
+    out.update(value=round(r32["value"], 4), ms_per_step=round(r32["ms_per_freq"] * len(freqs), 3))
+    out["config"] = {"workload": "coupled FEM-BEM shell (level-3 icosphere, 3 layers), electrodynamic piston + BEM "
+                                 "source, 8 log freqs 100 Hz-2 kHz, order-2 rules, 500 field points",
+                     "fem_vertices": int(model.fem_mesh.n_vertices), "tetrahedra": int(len(model.fem_mesh.tetrahedra)),
+                     "bem_vertices": int(model.bem_mesh.n_vertices), "interface_dofs": int(model.interface_map.count),
+                     "frequencies": len(freqs), "excitations": len(exc)}
+    out["float32"], out["float64"], out["float32_host_condensed"] = r32, r64, rcond
+    # oracle: CPU baseline on one frequency and parity of that frequency
+    from oracle import bem_oracle as O
+    from oracle import coupled_oracle as CO
+
+    if not args.no_cpu:
+        fm, bm, im = model.fem_mesh, model.bem_mesh, model.interface_map
+        om = O.Mesh(bm.vertices, bm.faces, bm.tags, np.zeros(bm.n_faces, np.int64))
+        imo = dict(fem_vertices=im.fem_vertex_indices, bem_vertices=im.fem_to_bem_vertex_indices,
+                   fem_faces=im.fem_face_indices, bem_faces=im.bem_face_indices, sign=im.normal_sign)
+        td = model.transducers[0]
+        pist = [dict(fem_tags=td.fem_tags, fem_signs=td.fem_signs, bem_tags=[], bem_signs=[], axis=td.motion_axis,
+                     completion=1.0, re=td.re_ohm, le=td.le_h, bl=td.bl_n_per_a, mmd=td.mmd_kg, cms=td.cms_m_per_n,
+                     rms=td.rms_n_s_per_m)]
+        t0 = time.perf_counter()
+        ref = CO.solve_coupled(fm.vertices, fm.tetrahedra, fm.boundary_faces, fm.boundary_tags, om, imo, float(f0),
+                               343.0, 1.21, [{"kind": "voltage", "transducer": 1, "amplitude": 2.83},
+                                             {"kind": "normal_velocity", "bem_source": 1}], pistons=pist,
+                               prescribed=model.prescribed_velocity.toarray(), loss_by_vertex=model.bulk_loss_by_vertex,
+                               regular_order=2, singular_order=2, points=pts)
+        el = time.perf_counter() - t0
+        cores = os.cpu_count()
+        out["cpu_baseline"] = {"value": round(1.0 / el, 5), "unit": "freq-points/s", "cores": cores, "kind": "port",
+                               "cpu_model": cpu_model(),
+                               "sample": f"1 of the 8 frequencies ({f0:.0f} Hz), FP64 oracle: per-element FEM loops, "
+                                         "dense BEM operators (NumPy), monolithic dense LAPACK solve, field"}
+        with np.errstate(divide="ignore", invalid="ignore"):
+            db = np.abs(20 * np.log10(np.abs(field) / np.abs(ref["field"])))
+            ph = np.degrees(np.abs(np.angle(field / ref["field"])))
+        out["parity_vs_oracle"] = {"freq_hz": round(float(f0), 2), "max_field_error_db": float(np.nanmax(db)),
+                                   "max_field_error_deg": float(np.nanmax(ph)),
+                                   "diaphragm_velocity_rel": float(abs(sols[0].diaphragm_velocity[0]
+                                                                       - ref["diaphragm_velocity"][0, 0])
+                                                                   / abs(ref["diaphragm_velocity"][0, 0]))}
+    if not args.no_fp64:  # the large workload (GPU only)
+        model, exc, freqs, pts = coupled_job(4)
+        log(f"bench: coupled large workload ({model.fem_mesh.n_vertices} FEM / {model.bem_mesh.n_vertices} BEM)")
+        big, _ = timed(model, exc, freqs[::2], pts, precision="float32", static_condensation=True)
+        bigc, _ = timed(model, exc, freqs[:1], pts, precision="float32", static_condensation=True,
+                        formulation="condensed")
+        out["large"] = {"fem_vertices": int(model.fem_mesh.n_vertices), "bem_vertices": int(model.bem_mesh.n_vertices),
+                        "interface_dofs": int(model.interface_map.count), "auto": big, "host_condensed": bigc}
+    dev.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,9 +570,9 @@ def main():
     ap.add_argument("--probe", choices=["job", "field"], default="job",
                     help="--traffic-probe workload: one eager job of --workload, or the field evaluation of the "
                          "20k-triangle job alone")
-    ap.add_argument("--workload", choices=["horn", "scale"], default="horn",
+    ap.add_argument("--workload", choices=["horn", "scale", "coupled"], default="horn",
                     help="horn: the 6k-triangle horn-in-box job (headline); scale: the 20k-triangle two-way "
-                         "loudspeaker job")
+                         "loudspeaker job; coupled: the FEM-BEM shell with a piston")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build runs on a single MI355X); got --gpus {args.gpus}",
@@ -452,6 +584,9 @@ def main():
         return
     if args.workload == "scale":
         run_scale(args)
+        return
+    if args.workload == "coupled":
+        run_coupled(args)
         return
 
     import tempfile

@@ -129,6 +129,7 @@ SIGNATURES = {
     "bem_sparse_destroy": (_i, [_p]),
     "bem_sparse_scatter": (_i, [_p, _p, _i, _d, _d, _i, _p, _ll]),
     "bem_dense_block": (_i, [_p, _i, _i, _i, _p, _ll, _d, _d, _i, _p, _ll]),
+    "bem_row_equilibrate": (_i, [_p, _i, _i, _i, _p, _ll, _p]),
 }
 
 

@@ -23,8 +23,9 @@ What runs where: FEM matrices, the interface and piston operators, the Schur com
 FEM interior, in complex128 whatever the request's precision, as the reference factors A_II in ComplexF64) and
 the reduction / recovery of right-hand sides stay on the host; the BEM block (bem_assemble), the BEM interface
 and piston blocks (bem_rhs_columns), the placement of every sparse and dense block (bem_sparse_scatter,
-bem_dense_block), the LU (bem_lu_factor) and the solve -- FP32 LU with FP64 iterative refinement against the
-assembled matrix for float32 requests, FP64 LU and solve for float64 -- and the far field run on the device.
+bem_dense_block), an exact power-of-two row equilibration (bem_row_equilibrate), the LU (bem_lu_factor) and the
+solve -- FP32 LU with FP64 iterative refinement against the assembled matrix for float32 requests, FP64 LU and
+solve for float64 -- and the far field run on the device.
 The device matrix is row-major with BEM rows and columns in the mesh's locality order; solutions are returned
 in the caller's numbering.
 """
@@ -131,6 +132,22 @@ def schur_complement(A: sp.spmatrix, retained: np.ndarray, block: int = 256):
     return S, parts
 
 
+def monolithic_is_cheaper(n_fem: int, n_retained: int, n_mono: int) -> bool:
+    """Choose the dense monolithic system on the device over host condensation when it fits the dense LU and its
+    estimated time is lower. Host condensation is one SuperLU factorization of the interior plus one sparse
+    triangular solve pair per retained vertex: measured about 2.6e6 nonzeros in L + U for a 7,700-vertex
+    tetrahedral interior (fill growing like n^(4/3)) at 6 GFLOP/s of complex solves. The device LU of the
+    monolithic system (one matrix, panel-chain bound) is estimated at 0.35 s x (n / 15,500)^3, and is never
+    below 10 ms."""
+    if n_mono > _lib.BEM_LU_MAX_N:
+        return False
+    n_int = max(0, n_fem - n_retained)
+    nnz_lu = 2.6e6 * (n_int / 7700.0) ** (4.0 / 3.0)
+    host = 8.0 * nnz_lu * n_retained / 6e9
+    device = max(0.01, 0.35 * (n_mono / 15500.0) ** 3)
+    return device < host
+
+
 def _colset_arrays(mat: sp.spmatrix, finv: np.ndarray):
     """Column-set tables (internal face numbering) of an (F x n_cols) sparse matrix of Neumann columns."""
     m = sp.csc_matrix(mat, dtype=np.complex128)
@@ -148,16 +165,22 @@ class CoupledSolver:
 
     def __init__(self, model: CoupledModel, precision: str = "float32", static_condensation: bool = True,
                  validation_diagnostics: bool = False, regular_order: int = 2, singular_order: int = 2,
-                 symmetry: str = "off", dev: Device | None = None):
+                 symmetry: str = "off", dev: Device | None = None, formulation: str = "auto"):
+        """formulation: "auto" follows static_condensation unless condensation was requested and the
+        monolithic system fits the dense LU and is estimated cheaper (monolithic_is_cheaper); "condensed" /
+        "monolithic" force one."""
         if precision not in ("float32", "float64"):
             raise ValueError("Unsupported coupled precision. Expected float32 or float64.")
         if static_condensation and validation_diagnostics:
             raise ValueError("FEM static condensation cannot be combined with full-matrix validation diagnostics.")
+        if formulation not in ("auto", "condensed", "monolithic"):
+            raise ValueError("formulation must be auto, condensed or monolithic")
         self.model = m = model
         self.precision = precision
         self.engine_precision = "mixed" if precision == "float32" else "fp64"
         self.prec = PRECISIONS[self.engine_precision]
-        self.condensed = bool(static_condensation)
+        self.condensation_requested = bool(static_condensation)
+        self.condensed = bool(static_condensation) if formulation == "auto" else formulation == "condensed"
         self.validation = bool(validation_diagnostics)
         self.symmetry = G.normalized_symmetry_mode(symmetry)
         self.regular_order, self.singular_order = int(regular_order), int(singular_order)
@@ -191,6 +214,10 @@ class CoupledSolver:
         self.nt = len(m.transducers)
         tfem = np.unique(self.tops["fem_surface"].nonzero()[0])
         self.retained = np.unique(np.concatenate([m.interface_map.fem_vertex_indices, tfem])).astype(np.int64)
+        n_mono = fm.n_vertices + bm.n_vertices + m.interface_map.count + 2 * self.nt
+        if formulation == "auto" and self.condensed and monolithic_is_cheaper(fm.n_vertices, len(self.retained),
+                                                                            n_mono):
+            self.condensed = False
         if self.condensed:
             outside = np.ones(fm.n_vertices, bool)
             outside[self.retained] = False
@@ -308,6 +335,7 @@ class CoupledSolver:
             wbytes = _lib.lu_workspace_bytes(self.prec, n, 1)
             self.luw = torch.empty(wbytes // (8 if self.engine_precision == "mixed" else 16), dtype=cdt, **kw)
             self.done = torch.zeros(1, dtype=torch.int32, **kw)
+            self.rscale = torch.ones(n, dtype=torch.float64, **kw)
             self.scratch = torch.zeros(2, dtype=torch.int32, **kw)
         self._rhs_cap = 0
 
@@ -424,6 +452,9 @@ class CoupledSolver:
                 diag = self._sparse(np.concatenate([mech, mech, elec, elec]), np.concatenate([mech, elec, mech, elec]),
                                     np.concatenate([zm, -bl, bl, ze]))
                 self._scatter(diag)
+            # exact power-of-two row scaling: FEM, BEM, trace and lumped rows differ in scale by orders of magnitude
+            _lib.check(lib.bem_row_equilibrate(dev.ctx, self.prec, self.n, self.n, _ptr(self.Cm), self.ldc,
+                                               _ptr(self.rscale)), "bem_row_equilibrate")
             self.C0.copy_(self.Cm)
             ev[3].record(dev.stream)
         # right-hand sides (host), as the reference builds them (BeatEngineCoupled.jl:2360-2420, 2285-2310)
@@ -470,6 +501,7 @@ class CoupledSolver:
             rhs[:, : self.nF] = fem_rhs.T
         rhs[:, self.bem_off : self.bem_off + self.V] = bem_rhs.T
         rhs[:, self.elec_off : self.elec_off + self.nt] = elec_rhs.T
+        rhs[:, : self.n] *= self.rscale.cpu().numpy()[None, :]  # the rows' equilibration factors
         t_rhs = time.perf_counter() - t0
         self._rhs_buffers(n_exc)
         ng = (n_exc + 3) // 4
@@ -524,7 +556,9 @@ class CoupledSolver:
 
                 raise RuntimeError(f"Coupled dense solve failed at {freq:g} Hz: {status_text(int(flags[0]))}.")
         X = self.x[:n_exc, : self.n].cpu().numpy().T  # (n, n_exc)
-        rel = self.rel.cpu().numpy()[:ng]
+        rel = self.rel.cpu().numpy()[:ng]  # relative residual of the row-equilibrated system
+        if self.engine_precision == "mixed":
+            tim["refinement_iterations"] = int(self.iters.cpu().numpy()[:ng].max())
         # recovery
         t0 = time.perf_counter()
         fem_p = np.zeros((self.n_fem, n_exc), np.complex128)

@@ -516,7 +516,7 @@ def frequency_result(job, freq, sols, field, ys, tim, first, solver) -> dict:
                                             cdt, metadata={"points_m": pts,
                                                            "observation_domains": opts.get("observation_domains", [])}))
             offset += n
-    static = job["static_condensation"]
+    static = bool(getattr(solver, "condensed", job["static_condensation"]))
     t = {
         "assembly_s": tim.get("assembly_s", 0.0), "solve_s": tim.get("solve_s", 0.0), "field_s": tim.get("field_s", 0.0),
         "mesh_setup_s": job["mesh_setup_s"] if first else 0.0,
@@ -562,7 +562,8 @@ def frequency_result(job, freq, sols, field, ys, tim, first, solver) -> dict:
     }
     if job["validation_diagnostics"]:
         diagnostics["all_bem_replay_error"] = max(s.all_bem_replay_error for s in sols)
-    diagnostics["fem_interior_residual"] = max(s.fem_interior_residual for s in sols) if static else None
+    diagnostics["fem_interior_residual"] = (max(s.fem_interior_residual for s in sols)
+                                            if static and sols[0].fem_interior_residual is not None else None)
     return {"schema_version": 1, "freq_hz": float(freq), "excitation_port_ids": list(job["port_ids"]),
             "quantities": quantities, "diagnostics": diagnostics}
 

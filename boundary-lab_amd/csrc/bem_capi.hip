@@ -1134,6 +1134,20 @@ int bem_dense_block(bem_ctx* c, int precision, int m, int n, const void* src, lo
   return BEM_OK;
 }
 
+int bem_row_equilibrate(bem_ctx* c, int precision, int n, int ncols, void* A, long long lda, double* scale) {
+  if (!c || !A || !scale) return fail(BEM_E_ARG, "NULL argument");
+  if (n < 0 || ncols < 0 || lda < ncols) return fail(BEM_E_ARG, "invalid sizes");
+  hipError_t e;
+  if (precision == BEM_PREC_MIXED)
+    e = row_equilibrate<float>(static_cast<cplx<float>*>(A), lda, n, ncols, scale, c->stream);
+  else if (precision == BEM_PREC_FP64)
+    e = row_equilibrate<double>(static_cast<cplx<double>*>(A), lda, n, ncols, scale, c->stream);
+  else
+    return fail(BEM_E_ARG, "unknown precision");
+  if (e != hipSuccess) return hipfail(e, "row equilibration launch");
+  return BEM_OK;
+}
+
 int bem_neumann(bem_ctx* c, int nf, int n_ch, int n_faces, const void* rhs_scale, const void* qunit, void* q) {
   if (!c || !rhs_scale || !qunit || !q) return fail(BEM_E_ARG, "NULL argument");
   if (nf <= 0 || n_ch <= 0 || n_faces <= 0) return fail(BEM_E_ARG, "invalid sizes");

@@ -76,6 +76,39 @@ __global__ __launch_bounds__(256) void k_dense_block(int m, int n, const cplx<do
     *o = mk<T>(T(v.re), T(v.im));
 }
 
+// Row equilibration by powers of two: row i is multiplied by s_i = 2^-e_i with max_j (|re| + |im|) of the row in
+// [2^(e_i - 1), 2^e_i), so the largest entry lands in [1/2, 1) and no entry is rounded. scale[i] = s_i (1 for an all-
+// zero row). One workgroup per row; the maximum is reduced in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_equilibrate(cplx<T>* A, long long lda, int ncols, double* scale) {
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  __shared__ double red[256];
+  cplx<T>* row = A + (size_t)i * lda;
+  double m = 0.0;
+  for (int j = tid; j < ncols; j += 256) m = fmax(m, fabs((double)row[j].re) + fabs((double)row[j].im));
+  red[tid] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+    __syncthreads();
+  }
+  const double mx = red[0];
+  int e = 0;
+  if (mx > 0.0 && isfinite(mx)) (void)frexp(mx, &e);
+  const double s = ldexp(1.0, -e);
+  const T st = T(s);
+  for (int j = tid; j < ncols; j += 256) row[j] = mk<T>(row[j].re * st, row[j].im * st);
+  if (tid == 0) scale[i] = s;
+}
+
+template <typename T>
+hipError_t row_equilibrate(cplx<T>* A, long long lda, int n, int ncols, double* scale, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_equilibrate<T>, dim3(n), dim3(256), 0, st, A, lda, ncols, scale);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t cols_gather(const cplx<double>* part, int F, int n_src, const int* vstar_ptr, const int* vstar_row,
                        const double* roww, int N, int n_cols, const int* col_ptr, const int* col_idx,
@@ -115,6 +148,8 @@ template hipError_t scatter_coo<float>(long long, const int*, const int*, const 
                                        cplx<float>*, long long, hipStream_t);
 template hipError_t scatter_coo<double>(long long, const int*, const int*, const cplx<double>*, cplx<double>, int,
                                         cplx<double>*, long long, hipStream_t);
+template hipError_t row_equilibrate<float>(cplx<float>*, long long, int, int, double*, hipStream_t);
+template hipError_t row_equilibrate<double>(cplx<double>*, long long, int, int, double*, hipStream_t);
 template hipError_t dense_block<float>(int, int, const cplx<double>*, long long, cplx<double>, int, cplx<float>*,
                                        long long, hipStream_t);
 template hipError_t dense_block<double>(int, int, const cplx<double>*, long long, cplx<double>, int, cplx<double>*,

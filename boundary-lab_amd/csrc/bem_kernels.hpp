@@ -149,6 +149,10 @@ hipError_t cols_gather(const cplx<double>* part, int F, int n_src, const int* vs
 template <typename T>
 hipError_t scatter_coo(long long nnz, const int* rows, const int* cols, const cplx<double>* vals,
                        cplx<double> alpha, int add, cplx<T>* out, long long ldo, hipStream_t st);
+// Rows i < n of A (ncols columns) scaled by powers of two so that max |re|+|im| lies in [1/2, 1); scale[i] receives
+// the factor.
+template <typename T>
+hipError_t row_equilibrate(cplx<T>* A, long long lda, int n, int ncols, double* scale, hipStream_t st);
 // out[i * ldo + j] (=, or += when add) alpha * src[i * lds + j] (src null: alpha) for i < m, j < n.
 template <typename T>
 hipError_t dense_block(int m, int n, const cplx<double>* src, long long lds, cplx<double> alpha, int add,

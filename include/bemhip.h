@@ -272,6 +272,13 @@ int bem_sparse_scatter(bem_ctx* ctx, const bem_sparse* sp, int precision, double
  * the coupled matrix, on the library's stream (no memset nodes). */
 int bem_dense_block(bem_ctx* ctx, int precision, int m, int n, const void* src, long long lds, double alpha_re,
                     double alpha_im, int add, void* out, long long ldo);
+/* Scale rows i < n of A (ncols columns, device cT, leading dimension lda) by powers of two so that each row's
+ * largest |re| + |im| lies in [1/2, 1) -- exact, no entry is rounded; scale (device double [n]) receives the
+ * factors (1 for a zero row). The solution of the scaled system with right-hand-side rows multiplied by the same
+ * factors is unchanged. No reference counterpart: the reference's single-precision dense LU runs unscaled;
+ * here it lets the FP32 factors of the coupled matrix, whose FEM, BEM, trace and lumped rows differ in scale by
+ * orders of magnitude, precondition the FP64 refinement in a few iterations. */
+int bem_row_equilibrate(bem_ctx* ctx, int precision, int n, int ncols, void* A, long long lda, double* scale);
 
 /* Bytes of the LU workspace for `batch` N x N matrices: the inverted 64 x 64 diagonal blocks of L and U
  * that bem_lu_factor writes and bem_lu_solve / bem_refine read (cT [batch][ceil(N/64)][2][64][64]). */
