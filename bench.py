@@ -471,24 +471,25 @@ def run_coupled(args):
         s = CoupledSolver(model, dev=dev, **opts)
         try:
             for _ in range(args.warmup):
-                s.solve(float(freqs[0]), exc, pts)
+                s.solve_batch(freqs[: s.batch_capacity()], exc, pts)
             torch.cuda.synchronize()
             stages = {}
             iters = 0
             t0 = time.perf_counter()
             res = None
+            nb = s.batch_capacity()
             for _ in range(args.steps):
-                for f in freqs:
-                    sols, field, _, tim = s.solve(float(f), exc, pts)
-                    res = (f, sols, field) if res is None else res
-                    iters = max(iters, int(tim.pop("refinement_iterations", 0)))
-                    for k, v in tim.items():
-                        stages[k] = stages.get(k, 0.0) + v
+                for i in range(0, len(freqs), nb):
+                    for j, (sols, field, _, tim) in enumerate(s.solve_batch(freqs[i : i + nb], exc, pts)):
+                        res = (freqs[i + j], sols, field) if res is None else res
+                        iters = max(iters, int(tim.pop("refinement_iterations", 0)))
+                        for k, v in tim.items():
+                            stages[k] = stages.get(k, 0.0) + v
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             n = args.steps * len(freqs)
             return dict(value=n / el, ms_per_freq=1e3 * el / n, n=s.n, formulation="condensed" if s.condensed
-                        else "monolithic", max_refinement_iterations=iters,
+                        else "monolithic", batch=nb, max_refinement_iterations=iters,
                         stages_ms={k: round(1e3 * v / n, 3) for k, v in stages.items()}), res
         finally:
             s.close()

@@ -316,46 +316,62 @@ class CoupledSolver:
             self.cs_pres, _ = self._colset(self.pvel)
 
     def _alloc(self):
+        self._cap_nb = 0
+        self._rhs_cap = (0, 0)
+
+    def _ensure(self, nb: int):
+        """Device buffers for batches of nb frequencies (grown on demand)."""
+        if nb <= self._cap_nb:
+            return
         torch = self.dev.torch
         cdt = torch.complex64 if self.engine_precision == "mixed" else torch.complex128
         n, ldc, V, F = self.n, self.ldc, self.V, self.F
         kw = dict(device=self.dev.device)
         with torch.cuda.stream(self.dev.stream):
-            self.Cm = torch.empty((n, ldc), dtype=cdt, **kw)
-            self.C0 = torch.empty((n, ldc), dtype=cdt, **kw)  # the assembled matrix, for residuals
-            self.k = torch.zeros(1, dtype=torch.float64, **kw)
-            self.scal = torch.zeros(4, dtype=torch.complex128, **kw)  # i rho w, -1, -i rho w, 1
+            self.Cm = self.C0 = None
+            self.Cm = torch.empty((nb, n, ldc), dtype=cdt, **kw)
+            self.C0 = torch.empty((nb, n, ldc), dtype=cdt, **kw)  # the assembled matrices, for residuals
+            self.k = torch.zeros(nb, dtype=torch.float64, **kw)
+            self.scal = torch.zeros((4, nb), dtype=torch.complex128, **kw)  # i rho w, -1, -i rho w, 1 per frequency
             self.qzero = torch.zeros((1, F), dtype=torch.complex128, **kw)
-            self.bdummy = torch.zeros((1, 1, _lda(V)), dtype=torch.complex128, **kw)
-            self.rowmax = torch.zeros(V, dtype=torch.float64, **kw)
-            self.P = torch.zeros((V, max(1, self.n_pres)), dtype=torch.complex128, **kw)
-            self.ipiv = torch.zeros(n, dtype=torch.int32, **kw)
-            self.perm = torch.zeros(n, dtype=torch.int32, **kw)
-            self.info = torch.zeros(1, dtype=torch.int32, **kw)
-            wbytes = _lib.lu_workspace_bytes(self.prec, n, 1)
+            self.bdummy = torch.zeros((nb, 1, _lda(V)), dtype=torch.complex128, **kw)
+            self.rowmax = torch.zeros((nb, V), dtype=torch.float64, **kw)
+            self.P = torch.zeros((nb, V, max(1, self.n_pres)), dtype=torch.complex128, **kw)
+            self.ipiv = torch.zeros((nb, n), dtype=torch.int32, **kw)
+            self.perm = torch.zeros((nb, n), dtype=torch.int32, **kw)
+            self.info = torch.zeros(nb, dtype=torch.int32, **kw)
+            wbytes = _lib.lu_workspace_bytes(self.prec, n, nb)
             self.luw = torch.empty(wbytes // (8 if self.engine_precision == "mixed" else 16), dtype=cdt, **kw)
-            self.done = torch.zeros(1, dtype=torch.int32, **kw)
-            self.rscale = torch.ones(n, dtype=torch.float64, **kw)
-            self.scratch = torch.zeros(2, dtype=torch.int32, **kw)
-        self._rhs_cap = 0
+            self.done = torch.zeros(nb, dtype=torch.int32, **kw)
+            self.scratch = torch.zeros(2 * nb, dtype=torch.int32, **kw)
+            self.rscale = torch.ones((nb, n), dtype=torch.float64, **kw)
+        self._cap_nb = nb
+        self._rhs_cap = (0, 0)
 
-    def _rhs_buffers(self, n_exc: int):
-        torch = self.dev.torch
+    def _rhs_buffers(self, nb: int, n_exc: int):
         ng = (n_exc + 3) // 4
-        if ng * 4 <= self._rhs_cap:
+        if self._rhs_cap == (nb, ng):
             return
+        torch = self.dev.torch
         cdt = torch.complex64 if self.engine_precision == "mixed" else torch.complex128
         kw = dict(device=self.dev.device)
         with torch.cuda.stream(self.dev.stream):
-            self.b = torch.zeros((ng * 4, self.ldc), dtype=torch.complex128, **kw)
-            self.x = torch.zeros((ng * 4, self.ldc), dtype=torch.complex128, **kw)
-            self.r = torch.zeros((ng * 4, self.ldc), dtype=torch.complex128, **kw)
-            self.y = torch.zeros((ng * 4, self.ldc), dtype=cdt, **kw)
-            self.rel = torch.zeros(ng, dtype=torch.float64, **kw)
-            self.iters = torch.zeros(ng, dtype=torch.int32, **kw)
-        self._rhs_cap = ng * 4
+            self.b = torch.zeros((nb, ng * 4, self.ldc), dtype=torch.complex128, **kw)
+            self.x = torch.zeros((nb, ng * 4, self.ldc), dtype=torch.complex128, **kw)
+            self.r = torch.zeros((nb, ng * 4, self.ldc), dtype=torch.complex128, **kw)
+            self.y = torch.zeros((nb, ng * 4, self.ldc), dtype=cdt, **kw)
+            self.rel = torch.zeros((ng, nb), dtype=torch.float64, **kw)
+            self.iters = torch.zeros((ng, nb), dtype=torch.int32, **kw)
+        self._rhs_cap = (nb, ng)
 
-    # -- per frequency ----------------------------------------------------------------------------------
+    def batch_capacity(self, share: float = 0.5) -> int:
+        """Frequencies per device batch that fit in `share` of the free device memory (at most 8)."""
+        torch = self.dev.torch
+        free, _ = torch.cuda.mem_get_info(self.dev.device)
+        cb = 8 if self.engine_precision == "mixed" else 16
+        per = 2 * self.n * self.ldc * cb + self.n * 64 * 16
+        return int(max(1, min(8, share * free // per)))
+
     def fem_system(self, freq: float):
         omega = 2.0 * math.pi * float(freq)
         k = omega / self.model.sound_speed
@@ -366,105 +382,272 @@ class CoupledSolver:
             A = A - (1j * self.model.density * omega * y) * mat
         return A.tocsr(), ys
 
-    def _ptr_at(self, t, row, col):
-        return C.c_void_p(t.data_ptr() + (row * self.ldc + col) * t.element_size())
+    def _ptr_at(self, t, f, row, col):
+        return C.c_void_p(t.data_ptr() + ((f * self.n + row) * self.ldc + col) * t.element_size())
 
-    def _scatter(self, h, alpha=1.0):
+    def _scatter(self, h, f, alpha=1.0):
         a = complex(alpha)
-        _lib.check(self.dev.lib.bem_sparse_scatter(self.dev.ctx, h, self.prec, a.real, a.imag, 1, _ptr(self.Cm),
-                                                   self.ldc), "bem_sparse_scatter")
+        _lib.check(self.dev.lib.bem_sparse_scatter(self.dev.ctx, h, self.prec, a.real, a.imag, 1,
+                                                   self._ptr_at(self.Cm, f, 0, 0), self.ldc), "bem_sparse_scatter")
 
     def solve(self, freq: float, excitations, points=None):
-        """Assemble, factor and solve the coupled system at one frequency for every excitation; returns
-        (solutions, exterior pressures at `points` or None, wall admittances, timings)."""
+        """One frequency: (solutions, exterior pressures at `points` or None, wall admittances, timings)."""
+        return self.solve_batch([freq], excitations, points)[0]
+
+    def solve_batch(self, freqs, excitations, points=None):
+        """Assemble, factor and solve the coupled systems of several frequencies as one device batch, for every
+        excitation; returns per frequency (solutions, exterior pressures at `points` or None, wall admittances,
+        timings; the batch's device times divided evenly)."""
         torch, lib, dev = self.dev.torch, self.dev.lib, self.dev
         exc = list(excitations)
         if not exc:
             raise ValueError("At least one coupled excitation is required.")
+        freqs = [float(f) for f in freqs]
+        nb = len(freqs)
+        self._ensure(nb)
         m = self.model
         rho = m.density
-        omega = 2.0 * math.pi * float(freq)
-        k = omega / m.sound_speed
-        tim = {}
-        t0 = time.perf_counter()
-        A_fem, ys = self.fem_system(freq)
-        tim["fem_system_s"] = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        parts = None
-        if self.condensed:
-            S, parts = schur_complement(A_fem, self.retained)
-        tim["fem_condensation_s"] = time.perf_counter() - t0
-        if parts is not None:
-            tim.update({"fem_condensation_" + k2: v for k2, v in parts["timings"].items()})
-        iro = 1j * rho * omega
+        omegas = [2.0 * math.pi * f for f in freqs]
+        ks = [w / m.sound_speed for w in omegas]
+        iros = [1j * rho * w for w in omegas]
+        tim = {"fem_system_s": 0.0, "fem_condensation_s": 0.0}
+        host = []
+        for f in freqs:
+            t0 = time.perf_counter()
+            A_fem, ys = self.fem_system(f)
+            tim["fem_system_s"] += time.perf_counter() - t0
+            t0 = time.perf_counter()
+            S = parts = None
+            if self.condensed:
+                S, parts = schur_complement(A_fem, self.retained)
+                for k2, v in parts["timings"].items():
+                    tim["fem_condensation_" + k2] = tim.get("fem_condensation_" + k2, 0.0) + v
+            tim["fem_condensation_s"] += time.perf_counter() - t0
+            host.append((A_fem, ys, S, parts))
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        sc_host = np.array([iros, [-1.0] * nb, [-x for x in iros], [1.0] * nb], dtype=np.complex128)
         with torch.cuda.stream(dev.stream):
             ev[0].record(dev.stream)
-            self.k.copy_(torch.tensor([k], dtype=torch.float64))
-            self.scal.copy_(torch.tensor([iro, -1.0, -iro, 1.0], dtype=torch.complex128))
-            _lib.check(lib.bem_dense_block(dev.ctx, self.prec, self.n, self.n, None, 0, 0.0, 0.0, 0,
+            self.k[:nb].copy_(torch.tensor(ks, dtype=torch.float64))
+            self.scal[:, :nb].copy_(torch.from_numpy(sc_host))
+            _lib.check(lib.bem_dense_block(dev.ctx, self.prec, nb * self.n, self.n, None, 0, 0.0, 0.0, 0,
                                            _ptr(self.Cm), self.ldc), "bem_dense_block")
-            # BEM block, interface and piston columns, prescribed right-hand sides
-            sc = _ptr(self.scal)
-            _lib.check(lib.bem_assemble(dev.ctx, self.dmesh.handle, self.prec, 1, _ptr(self.k), _ptr(self.qzero), 1, sc,
-                                        self._ptr_at(self.Cm, self.bem_off, self.bem_off), self.ldc,
-                                        self.ldc * self.n, None, _ptr(self.bdummy), self.bdummy.shape[-1],
-                                        self.bdummy.shape[-1], _ptr(self.rowmax)), "bem_assemble")
+            sA = self.n * self.ldc
+            ldb = self.bdummy.shape[-1]
+            _lib.check(lib.bem_assemble(dev.ctx, self.dmesh.handle, self.prec, nb, _ptr(self.k), _ptr(self.qzero), 1,
+                                        _ptr(self.scal), self._ptr_at(self.Cm, 0, self.bem_off, self.bem_off), self.ldc,
+                                        sA, None, _ptr(self.bdummy), ldb, ldb, _ptr(self.rowmax)), "bem_assemble")
             ev[1].record(dev.stream)
-            scal_p = lambda i: C.c_void_p(self.scal.data_ptr() + 16 * i)  # noqa: E731
-            _lib.check(lib.bem_rhs_columns(dev.ctx, self.dmesh.handle, self.cs_if, self.prec, 1, _ptr(self.k),
-                                           scal_p(1), self._ptr_at(self.Cm, self.bem_off, self.flux_off), self.ldc,
-                                           self.ldc * self.n), "bem_rhs_columns")
+            row = lambda i: C.c_void_p(self.scal.data_ptr() + 16 * i * self.scal.shape[1])  # noqa: E731
+            _lib.check(lib.bem_rhs_columns(dev.ctx, self.dmesh.handle, self.cs_if, self.prec, nb, _ptr(self.k), row(1),
+                                           self._ptr_at(self.Cm, 0, self.bem_off, self.flux_off), self.ldc, sA),
+                       "bem_rhs_columns")
             if self.nt:
-                _lib.check(lib.bem_rhs_columns(dev.ctx, self.dmesh.handle, self.cs_motion, self.prec, 1, _ptr(self.k),
-                                               scal_p(2), self._ptr_at(self.Cm, self.bem_off, self.mech_off),
-                                               self.ldc, self.ldc * self.n), "bem_rhs_columns")
+                _lib.check(lib.bem_rhs_columns(dev.ctx, self.dmesh.handle, self.cs_motion, self.prec, nb, _ptr(self.k),
+                                               row(2), self._ptr_at(self.Cm, 0, self.bem_off, self.mech_off), self.ldc,
+                                               sA), "bem_rhs_columns")
             if self.n_pres:
-                _lib.check(lib.bem_rhs_columns(dev.ctx, self.dmesh.handle, self.cs_pres, _lib.BEM_PREC_FP64, 1,
-                                               _ptr(self.k), scal_p(0), _ptr(self.P), self.n_pres,
-                                               self.V * self.n_pres), "bem_rhs_columns")
+                _lib.check(lib.bem_rhs_columns(dev.ctx, self.dmesh.handle, self.cs_pres, _lib.BEM_PREC_FP64, nb,
+                                               _ptr(self.k), row(0), _ptr(self.P), self.n_pres, self.V * self.n_pres),
+                           "bem_rhs_columns")
             ev[2].record(dev.stream)
-            # FEM block: the Schur complement (condensed) or the sparse dynamic stiffness (monolithic)
-            if self.condensed:
-                S_dev = torch.from_numpy(np.ascontiguousarray(S)).to(dev.device, non_blocking=False)
-                _lib.check(lib.bem_dense_block(dev.ctx, self.prec, self.nF, self.nF, _ptr(S_dev), self.nF, 1.0, 0.0,
-                                               1, _ptr(self.Cm), self.ldc), "bem_dense_block")
-            else:
-                self._scatter(self.sp_fem[0], 1.0)
-                self._scatter(self.sp_fem[1], -k * k)
-                self._scatter(self.sp_fem[2], -1j * k * k)
-                for h, y in zip(self.sp_walls, ys):
-                    self._scatter(h, -iro * y)
-            self._scatter(self.sp_fem_load)
-            self._scatter(self.sp_fem_trace)
-            self._scatter(self.sp_bem_trace)
-            diag = None
-            if self.nt:
-                self._scatter(self.sp_fem_motion, -iro)
-                self._scatter(self.sp_fem_force)
-                self._scatter(self.sp_bem_force)
-                tds = m.transducers
-                mech = self.mech_off + np.arange(self.nt)
-                elec = self.elec_off + np.arange(self.nt)
-                zm = np.array([complex(t.rms_n_s_per_m, -omega * t.mmd_kg + 1.0 / (omega * t.cms_m_per_n)) for t in tds])
-                ze = np.array([complex(t.re_ohm, -omega * t.le_h) for t in tds])
-                bl = np.array([t.bl_n_per_a for t in tds], dtype=np.complex128)
-                diag = self._sparse(np.concatenate([mech, mech, elec, elec]), np.concatenate([mech, elec, mech, elec]),
-                                    np.concatenate([zm, -bl, bl, ze]))
-                self._scatter(diag)
+            S_dev = []
+            diags = []
+            for j in range(nb):
+                A_fem, ys, S, parts = host[j]
+                k, iro = ks[j], iros[j]
+                if self.condensed:
+                    S_dev.append(torch.from_numpy(np.ascontiguousarray(S)).to(dev.device))
+                    _lib.check(lib.bem_dense_block(dev.ctx, self.prec, self.nF, self.nF, _ptr(S_dev[-1]), self.nF, 1.0,
+                                                   0.0, 1, self._ptr_at(self.Cm, j, 0, 0), self.ldc), "bem_dense_block")
+                else:
+                    self._scatter(self.sp_fem[0], j, 1.0)
+                    self._scatter(self.sp_fem[1], j, -k * k)
+                    self._scatter(self.sp_fem[2], j, -1j * k * k)
+                    for h, y in zip(self.sp_walls, ys):
+                        self._scatter(h, j, -iro * y)
+                self._scatter(self.sp_fem_load, j)
+                self._scatter(self.sp_fem_trace, j)
+                self._scatter(self.sp_bem_trace, j)
+                if self.nt:
+                    self._scatter(self.sp_fem_motion, j, -iro)
+                    self._scatter(self.sp_fem_force, j)
+                    self._scatter(self.sp_bem_force, j)
+                    w = omegas[j]
+                    tds = m.transducers
+                    mech = self.mech_off + np.arange(self.nt)
+                    elec = self.elec_off + np.arange(self.nt)
+                    zm = np.array([complex(t.rms_n_s_per_m, -w * t.mmd_kg + 1.0 / (w * t.cms_m_per_n)) for t in tds])
+                    ze = np.array([complex(t.re_ohm, -w * t.le_h) for t in tds])
+                    bl = np.array([t.bl_n_per_a for t in tds], dtype=np.complex128)
+                    diags.append(self._sparse(np.concatenate([mech, mech, elec, elec]),
+                                              np.concatenate([mech, elec, mech, elec]),
+                                              np.concatenate([zm, -bl, bl, ze])))
+                    self._scatter(diags[-1], j)
             # exact power-of-two row scaling: FEM, BEM, trace and lumped rows differ in scale by orders of magnitude
-            _lib.check(lib.bem_row_equilibrate(dev.ctx, self.prec, self.n, self.n, _ptr(self.Cm), self.ldc,
+            _lib.check(lib.bem_row_equilibrate(dev.ctx, self.prec, nb * self.n, self.n, _ptr(self.Cm), self.ldc,
                                                _ptr(self.rscale)), "bem_row_equilibrate")
-            self.C0.copy_(self.Cm)
+            self.C0[:nb].copy_(self.Cm[:nb])
             ev[3].record(dev.stream)
         # right-hand sides (host), as the reference builds them (BeatEngineCoupled.jl:2360-2420, 2285-2310)
+        n_exc = len(exc)
+        ng = (n_exc + 3) // 4
+        dev.stream.synchronize()
+        P_all = self.P[:nb].cpu().numpy() if self.n_pres else None
+        rs_all = self.rscale[:nb].cpu().numpy()
+        rhs = np.zeros((nb, ng * 4, self.ldc), np.complex128)
+        fem_rhs_all, pres_q_all, t_rhs = [], [], []
+        for j in range(nb):
+            t0 = time.perf_counter()
+            fem_rhs, bem_rhs, pres_q, elec_rhs = self._excitation_rhs(exc, omegas[j], iros[j],
+                                                                      None if P_all is None else P_all[j])
+            parts = host[j][3]
+            if self.condensed:
+                g = fem_rhs[self.retained]
+                if parts["lu"] is not None:
+                    g = g - parts["A_GI"] @ parts["lu"].solve(np.ascontiguousarray(fem_rhs[parts["interior"]]))
+                rhs[j, :n_exc, : self.nF] = g.T
+            else:
+                rhs[j, :n_exc, : self.nF] = fem_rhs.T
+            rhs[j, :n_exc, self.bem_off : self.bem_off + self.V] = bem_rhs.T
+            rhs[j, :n_exc, self.elec_off : self.elec_off + self.nt] = elec_rhs.T
+            rhs[j, :, : self.n] *= rs_all[j][None, :]  # the rows' equilibration factors
+            fem_rhs_all.append(fem_rhs)
+            pres_q_all.append(pres_q)
+            t_rhs.append(time.perf_counter() - t0)
+        self._rhs_buffers(nb, n_exc)
+        sA, sB = self.n * self.ldc, ng * 4 * self.ldc
+        with torch.cuda.stream(dev.stream):
+            self.b.copy_(torch.from_numpy(rhs))
+            self.x.zero_()
+            _lib.check(lib.bem_ctx_set_lu_products(dev.ctx, _lib.BEM_LU_SPLIT_BF16 if self.prec == _lib.BEM_PREC_MIXED
+                                                   else _lib.BEM_LU_FP32), "bem_ctx_set_lu_products")
+            c = TU.table().lu_choice(self.engine_precision, self.n, nb)
+            t = _lib.Tuning(c.split_tile, c.trsv_wg_min_batch, c.lookahead_wgs, 2048)
+            _lib.check(lib.bem_ctx_set_tuning(dev.ctx, C.byref(t)), "bem_ctx_set_tuning")
+            _lib.check(lib.bem_lu_factor(dev.ctx, self.prec, self.n, nb, _ptr(self.Cm), self.ldc, sA, _ptr(self.ipiv),
+                                         _ptr(self.perm), self.n, _ptr(self.info), _ptr(self.luw)), "bem_lu_factor")
+            for gi in range(ng):
+                nc = min(4, n_exc - 4 * gi)
+                grp = lambda t_: C.c_void_p(t_.data_ptr() + 4 * gi * self.ldc * t_.element_size())  # noqa: E731
+                relp = C.c_void_p(self.rel.data_ptr() + 8 * gi * nb)
+                if self.engine_precision == "mixed":
+                    _lib.check(lib.bem_refine(dev.ctx, self.n, nb, _ptr(self.Cm), self.ldc, sA, _ptr(self.perm),
+                                              self.n, _ptr(self.luw), _ptr(self.C0), grp(self.b), self.ldc, sB,
+                                              grp(self.x), grp(self.r), grp(self.y), nc, REFINE_MAX_ITER, REFINE_TOL,
+                                              relp, C.c_void_p(self.iters.data_ptr() + 4 * gi * nb),
+                                              _ptr(self.done)), "bem_refine")
+                else:
+                    _lib.check(lib.bem_lu_solve(dev.ctx, self.prec, self.n, nb, _ptr(self.Cm), self.ldc, sA,
+                                                _ptr(self.perm), self.n, _ptr(self.luw), grp(self.b), self.ldc, sB,
+                                                grp(self.x), self.ldc, sB, nc, None), "bem_lu_solve")
+                    _lib.check(lib.bem_residual(dev.ctx, self.prec, self.n, nb, _ptr(self.C0), self.ldc, sA,
+                                                grp(self.b), grp(self.x), self.ldc, sB, grp(self.r), nc, relp,
+                                                _ptr(self.scratch)), "bem_residual")
+            ev[4].record(dev.stream)
+        dev.stream.synchronize()
+        dt = {"bem_operator_s": ev[0].elapsed_time(ev[1]) / 1e3 / nb,
+              "bem_matrix_s": ev[1].elapsed_time(ev[2]) / 1e3 / nb,
+              "block_assembly_s": ev[2].elapsed_time(ev[3]) / 1e3 / nb,
+              "coupled_factorization_s": ev[3].elapsed_time(ev[4]) / 1e3 / nb}
+        tol = REFINE_TOL if self.engine_precision == "mixed" else FP64_RESIDUAL_LIMIT
+        for gi in range(ng):
+            nc = min(4, n_exc - 4 * gi)
+            flags = np.zeros(nb, np.int32)
+            rc = lib.bem_solve_status(dev.ctx, self.n, nb, _ptr(self.info),
+                                      C.c_void_p(self.x.data_ptr() + 4 * gi * self.ldc * 16), self.ldc, sB, nc,
+                                      C.c_void_p(self.rel.data_ptr() + 8 * gi * nb), tol, None, 0.0,
+                                      _np_ptr(flags, C.c_int))
+            if rc not in (_lib.BEM_OK, _lib.BEM_E_SINGULAR, _lib.BEM_E_ACCURACY):
+                _lib.check(rc, "bem_solve_status")
+            bad = np.nonzero(flags)[0]
+            if len(bad):
+                from .driver import status_text
+
+                j = int(bad[0])
+                raise RuntimeError(f"Coupled dense solve failed at {freqs[j]:g} Hz: {status_text(int(flags[j]))}.")
+        Xall = self.x[:nb, :n_exc, : self.n].cpu().numpy()  # (nb, n_exc, n)
+        rel = self.rel.cpu().numpy()  # (ng, nb): relative residual of the row-equilibrated systems
+        iters = self.iters.cpu().numpy() if self.engine_precision == "mixed" else None
+        qs = []
+        out = []
+        for j in range(nb):
+            X = Xall[j].T
+            parts = host[j][3]
+            fem_rhs = fem_rhs_all[j]
+            t0 = time.perf_counter()
+            fem_p = np.zeros((self.n_fem, n_exc), np.complex128)
+            interior_res = None
+            if self.condensed:
+                pG = X[: self.nF]
+                fem_p[self.retained] = pG
+                interior_res = np.zeros(n_exc)
+                if parts["lu"] is not None:
+                    I = parts["interior"]
+                    f_I = fem_rhs[I]
+                    u_I = parts["lu"].solve(np.ascontiguousarray(f_I - parts["A_IG"] @ pG))
+                    fem_p[I] = u_I
+                    t_int = parts["A_II"] @ u_I
+                    t_ret = parts["A_IG"] @ pG
+                    res = t_int + t_ret - f_I
+                    interior_res = np.array([
+                        np.linalg.norm(res[:, e]) / max(np.linalg.norm(t_int[:, e]), np.linalg.norm(t_ret[:, e]),
+                                                       np.linalg.norm(f_I[:, e]), np.finfo(float).eps)
+                        for e in range(n_exc)])
+            else:
+                fem_p[:] = X[: self.nF]
+            t_rec = time.perf_counter() - t0
+            bem_p = X[self.bem_off + self.vinv]  # caller numbering (V, n_exc)
+            flux = X[self.flux_off : self.flux_off + self.n_if]
+            u = X[self.mech_off : self.mech_off + self.nt]
+            cur = X[self.elec_off : self.elec_off + self.nt]
+            q = self.iops.bem_flux @ flux + iros[j] * (self.tops["bem_normal_velocity"] @ u) + pres_q_all[j]
+            qs.append(q)
+            sols = []
+            for e in range(n_exc):
+                pc, fc = self._interface_errors(fem_p[:, e], bem_p[:, e], flux[:, e], q[:, e])
+                sols.append(CoupledSolution(
+                    fem_pressure=fem_p[:, e], bem_pressure=bem_p[:, e], interface_flux=flux[:, e],
+                    bem_neumann=q[:, e], diaphragm_velocity=u[:, e], voice_coil_current=cur[:, e],
+                    pressure_continuity_error=pc, flux_conservation_error=fc,
+                    fem_interior_residual=None if interior_res is None else float(interior_res[e]),
+                    relative_residual=float(rel[e // 4, j]),
+                    fem_rhs_condensation_s=t_rhs[j] if self.condensed else None,
+                    fem_reconstruction_s=t_rec if self.condensed else None))
+            tj = dict(dt)
+            tj["fem_system_s"] = tim["fem_system_s"] / nb
+            tj["fem_condensation_s"] = tim["fem_condensation_s"] / nb
+            for k2 in ("partition_s", "factorization_s", "schur_extraction_s"):
+                if "fem_condensation_" + k2 in tim:
+                    tj["fem_condensation_" + k2] = tim["fem_condensation_" + k2] / nb
+            if iters is not None:
+                tj["refinement_iterations"] = int(iters[:, j].max())
+            out.append([sols, None, host[j][1], tj])
+        if points is not None and len(points):
+            t0 = time.perf_counter()
+            fields = self._field(np.asarray(points, np.float64), nb, n_exc, qs)
+            tf = (time.perf_counter() - t0) / nb
+            for j in range(nb):
+                out[j][1] = fields[j]
+                out[j][3]["field_s"] = tf
+        if self.validation:
+            for j in range(nb):
+                self._replay(freqs[j], out[j][0], qs[j])
+        for h in diags:
+            lib.bem_sparse_destroy(h)
+        ids = {id(h) for h in diags}
+        self._handles = [h for h in self._handles if id(h[1]) not in ids]
+        return [tuple(o) for o in out]
+
+    def _excitation_rhs(self, exc, omega, iro, P):
+        """FEM, BEM (internal vertex order), prescribed Neumann and coil right-hand sides of every excitation
+        (solve_coupled_excitations, BeatEngineCoupled.jl:2360-2420)."""
+        m = self.model
         n_exc = len(exc)
         fem_rhs = np.zeros((self.n_fem, n_exc), np.complex128)
         bem_rhs = np.zeros((self.V, n_exc), np.complex128)
         pres_q = np.zeros((self.F, n_exc), np.complex128)
         elec_rhs = np.zeros((self.nt, n_exc), np.complex128)
-        dev.stream.synchronize()
-        P = self.P.cpu().numpy() if self.n_pres else None
         for e, x in enumerate(exc):
             amp = complex(x.amplitude)
             if x.kind == "normal_velocity":
@@ -474,11 +657,11 @@ class CoupledSolver:
                 if not all(np.isfinite(w) and w > 0.0 for w in wts):
                     raise ValueError("Prescribed FEM boundary weights must be finite and greater than zero.")
                 for tag, w in zip(tags, wts):
-                    fem_rhs[:, e] += FE.prescribed_velocity_load(m.fem_mesh, tag, rho, omega, amp * w)
+                    fem_rhs[:, e] += FE.prescribed_velocity_load(m.fem_mesh, tag, m.density, omega, amp * w)
                 if x.bem_source_index > 0:
                     if x.bem_source_index > self.n_pres:
                         raise ValueError(f"Prescribed BEM source index {x.bem_source_index} is unavailable.")
-                    bem_rhs[:, e] = amp * P[:, x.bem_source_index - 1]  # internal vertex order
+                    bem_rhs[:, e] = amp * P[:, x.bem_source_index - 1]
                     pres_q[:, e] = amp * iro * self.pvel[:, x.bem_source_index - 1].toarray().reshape(-1)
                 if not tags and x.bem_source_index == 0:
                     raise ValueError("A prescribed-velocity excitation must own at least one FEM or BEM moving "
@@ -489,124 +672,7 @@ class CoupledSolver:
                 elec_rhs[x.transducer_index - 1, e] = amp
             else:
                 raise ValueError(f"Unsupported coupled excitation kind: {x.kind}.")
-        t0 = time.perf_counter()
-        rhs = np.zeros((n_exc, self.ldc), np.complex128)
-        if self.condensed:
-            f_I = fem_rhs[parts["interior"]]
-            g = fem_rhs[self.retained]
-            if parts["lu"] is not None:
-                g = g - parts["A_GI"] @ parts["lu"].solve(np.ascontiguousarray(f_I))
-            rhs[:, : self.nF] = g.T
-        else:
-            rhs[:, : self.nF] = fem_rhs.T
-        rhs[:, self.bem_off : self.bem_off + self.V] = bem_rhs.T
-        rhs[:, self.elec_off : self.elec_off + self.nt] = elec_rhs.T
-        rhs[:, : self.n] *= self.rscale.cpu().numpy()[None, :]  # the rows' equilibration factors
-        t_rhs = time.perf_counter() - t0
-        self._rhs_buffers(n_exc)
-        ng = (n_exc + 3) // 4
-        with torch.cuda.stream(dev.stream):
-            self.b.zero_()
-            self.b[:n_exc].copy_(torch.from_numpy(rhs))
-            _lib.check(lib.bem_ctx_set_lu_products(dev.ctx, _lib.BEM_LU_SPLIT_BF16 if self.prec == _lib.BEM_PREC_MIXED
-                                                   else _lib.BEM_LU_FP32), "bem_ctx_set_lu_products")
-            c = TU.table().lu_choice(self.engine_precision, self.n, 1)
-            t = _lib.Tuning(c.split_tile, c.trsv_wg_min_batch, c.lookahead_wgs, 2048)
-            _lib.check(lib.bem_ctx_set_tuning(dev.ctx, C.byref(t)), "bem_ctx_set_tuning")
-            _lib.check(lib.bem_lu_factor(dev.ctx, self.prec, self.n, 1, _ptr(self.Cm), self.ldc, self.ldc * self.n,
-                                         _ptr(self.ipiv), _ptr(self.perm), self.n, _ptr(self.info), _ptr(self.luw)),
-                       "bem_lu_factor")
-            for gi in range(ng):
-                nc = min(4, n_exc - 4 * gi)
-                row = lambda t_: C.c_void_p(t_.data_ptr() + 4 * gi * self.ldc * t_.element_size())  # noqa: E731
-                relp = C.c_void_p(self.rel.data_ptr() + 8 * gi)
-                sB = nc * self.ldc
-                if self.engine_precision == "mixed":
-                    _lib.check(lib.bem_refine(dev.ctx, self.n, 1, _ptr(self.Cm), self.ldc, self.ldc * self.n,
-                                              _ptr(self.perm), self.n, _ptr(self.luw), _ptr(self.C0), row(self.b),
-                                              self.ldc, sB, row(self.x), row(self.r), row(self.y), nc,
-                                              REFINE_MAX_ITER, REFINE_TOL, relp,
-                                              C.c_void_p(self.iters.data_ptr() + 4 * gi), _ptr(self.done)),
-                               "bem_refine")
-                else:
-                    _lib.check(lib.bem_lu_solve(dev.ctx, self.prec, self.n, 1, _ptr(self.Cm), self.ldc,
-                                                self.ldc * self.n, _ptr(self.perm), self.n, _ptr(self.luw), row(self.b),
-                                                self.ldc, sB, row(self.x), self.ldc, sB, nc, None), "bem_lu_solve")
-                    _lib.check(lib.bem_residual(dev.ctx, self.prec, self.n, 1, _ptr(self.C0), self.ldc,
-                                                self.ldc * self.n, row(self.b), row(self.x), self.ldc, sB, row(self.r),
-                                                nc, relp, _ptr(self.scratch)), "bem_residual")
-            ev[4].record(dev.stream)
-        dev.stream.synchronize()
-        tim["bem_operator_s"] = ev[0].elapsed_time(ev[1]) / 1e3
-        tim["bem_matrix_s"] = ev[1].elapsed_time(ev[2]) / 1e3
-        tim["block_assembly_s"] = ev[2].elapsed_time(ev[3]) / 1e3
-        tim["coupled_factorization_s"] = ev[3].elapsed_time(ev[4]) / 1e3
-        tol = REFINE_TOL if self.engine_precision == "mixed" else FP64_RESIDUAL_LIMIT
-        for gi in range(ng):
-            nc = min(4, n_exc - 4 * gi)
-            flags = np.zeros(1, np.int32)
-            relp = C.c_void_p(self.rel.data_ptr() + 8 * gi)
-            rc = lib.bem_solve_status(dev.ctx, self.n, 1, _ptr(self.info),
-                                      C.c_void_p(self.x.data_ptr() + 4 * gi * self.ldc * 16), self.ldc,
-                                      nc * self.ldc, nc, relp, tol, None, 0.0, _np_ptr(flags, C.c_int))
-            if rc not in (_lib.BEM_OK, _lib.BEM_E_SINGULAR, _lib.BEM_E_ACCURACY):
-                _lib.check(rc, "bem_solve_status")
-            if flags[0]:
-                from .driver import status_text
-
-                raise RuntimeError(f"Coupled dense solve failed at {freq:g} Hz: {status_text(int(flags[0]))}.")
-        X = self.x[:n_exc, : self.n].cpu().numpy().T  # (n, n_exc)
-        rel = self.rel.cpu().numpy()[:ng]  # relative residual of the row-equilibrated system
-        if self.engine_precision == "mixed":
-            tim["refinement_iterations"] = int(self.iters.cpu().numpy()[:ng].max())
-        # recovery
-        t0 = time.perf_counter()
-        fem_p = np.zeros((self.n_fem, n_exc), np.complex128)
-        interior_res = None
-        if self.condensed:
-            pG = X[: self.nF]
-            fem_p[self.retained] = pG
-            if parts["lu"] is not None:
-                I = parts["interior"]
-                f_I = fem_rhs[I]
-                u_I = parts["lu"].solve(np.ascontiguousarray(f_I - parts["A_IG"] @ pG))
-                fem_p[I] = u_I
-                t_int = parts["A_II"] @ u_I
-                t_ret = parts["A_IG"] @ pG
-                res = t_int + t_ret - f_I
-                interior_res = np.array([
-                    np.linalg.norm(res[:, e]) / max(np.linalg.norm(t_int[:, e]), np.linalg.norm(t_ret[:, e]),
-                                                   np.linalg.norm(f_I[:, e]), np.finfo(float).eps)
-                    for e in range(n_exc)])
-            else:
-                interior_res = np.zeros(n_exc)
-        else:
-            fem_p[:] = X[: self.nF]
-        t_rec = time.perf_counter() - t0
-        bem_p = X[self.bem_off + self.vinv]  # caller numbering (V, n_exc)
-        flux = X[self.flux_off : self.flux_off + self.n_if]
-        u = X[self.mech_off : self.mech_off + self.nt]
-        cur = X[self.elec_off : self.elec_off + self.nt]
-        q = (self.iops.bem_flux @ flux + iro * (self.tops["bem_normal_velocity"] @ u) + pres_q)  # (F, n_exc)
-        field = None
-        if points is not None and len(points):
-            field = self._field(k, np.asarray(points, np.float64), X, q)
-        sols = []
-        for e in range(n_exc):
-            pc, fc = self._interface_errors(fem_p[:, e], bem_p[:, e], flux[:, e], q[:, e])
-            sols.append(CoupledSolution(
-                fem_pressure=fem_p[:, e], bem_pressure=bem_p[:, e], interface_flux=flux[:, e], bem_neumann=q[:, e],
-                diaphragm_velocity=u[:, e], voice_coil_current=cur[:, e], pressure_continuity_error=pc,
-                flux_conservation_error=fc, fem_interior_residual=None if interior_res is None else float(interior_res[e]),
-                relative_residual=float(rel[e // 4]),
-                fem_rhs_condensation_s=t_rhs if self.condensed else None,
-                fem_reconstruction_s=t_rec if self.condensed else None))
-        if self.validation:
-            self._replay(freq, sols, q)
-        if diag is not None:
-            lib.bem_sparse_destroy(diag)
-            self._handles = [h for h in self._handles if h[1] is not diag]
-        return sols, field, ys, tim
+        return fem_rhs, bem_rhs, pres_q, elec_rhs
 
     def _interface_errors(self, p_fem, p_bem, flux, q):
         """Pressure continuity and integrated-flux conservation on the interface
@@ -624,25 +690,26 @@ class CoupledSolver:
         fscale = max(abs(fem_int), abs(bem_int), np.finfo(float).eps)
         return pc, float(abs(fem_int - bem_int) / fscale)
 
-    def _field(self, k, points, X, q):
-        """Exterior pressure from the BEM trace and the coupled Neumann data (bem_field, 4 excitations per call)."""
+    def _field(self, points, nb, n_exc, qs):
+        """Exterior pressure of every frequency of the batch from the BEM trace and the coupled Neumann data
+        (bem_field over the batch, 4 excitations per call). Returns (n_obs, n_exc) per frequency."""
         torch, lib, dev = self.dev.torch, self.dev.lib, self.dev
-        n_exc = q.shape[1]
         n_obs = len(points)
-        q_int = np.ascontiguousarray(q[self.prep.fperm].T)  # (n_exc, F) internal face order
+        ng = (n_exc + 3) // 4
+        q_int = np.stack([np.ascontiguousarray(q[self.prep.fperm].T) for q in qs])  # (nb, n_exc, F) internal order
         with torch.cuda.stream(dev.stream):
             obs = torch.from_numpy(np.ascontiguousarray(points)).to(dev.device)
             qd = torch.from_numpy(q_int).to(dev.device)
-            out = torch.zeros((n_exc, n_obs), dtype=torch.complex128, device=dev.device)
+            out = torch.zeros((nb, n_exc, n_obs), dtype=torch.complex128, device=dev.device)
             for c0 in range(0, n_exc, 4):
                 nc = min(4, n_exc - c0)
                 p = C.c_void_p(self.x.data_ptr() + (c0 * self.ldc + self.bem_off) * 16)
-                _lib.check(lib.bem_field(dev.ctx, self.dmesh.handle, self.prec, 1, _ptr(self.k), n_obs, _ptr(obs), p,
-                                         self.ldc, nc * self.ldc, C.c_void_p(qd.data_ptr() + c0 * self.F * 16),
-                                         nc * self.F, nc, C.c_void_p(out.data_ptr() + c0 * n_obs * 16), n_obs,
-                                         nc * n_obs), "bem_field")
+                _lib.check(lib.bem_field(dev.ctx, self.dmesh.handle, self.prec, nb, _ptr(self.k), n_obs, _ptr(obs), p,
+                                         self.ldc, ng * 4 * self.ldc, C.c_void_p(qd.data_ptr() + c0 * self.F * 16),
+                                         n_exc * self.F, nc, C.c_void_p(out.data_ptr() + c0 * n_obs * 16), n_obs,
+                                         n_exc * n_obs), "bem_field")
         dev.stream.synchronize()
-        return out.cpu().numpy().T  # (n_obs, n_exc)
+        return [o.T for o in out.cpu().numpy()]
 
     def _replay(self, freq, sols, q):
         """BEM-only replay error of the validation diagnostics (BeatEngineCoupled.jl:2236-2244): the BEM block

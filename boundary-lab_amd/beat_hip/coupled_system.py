@@ -596,19 +596,32 @@ def solve_coupled_request(request, emit, event_mode: bool = True, dev=None, solv
     else:
         solver = solver_factory(job["model"], **opts)
     solved = 0
+    freqs = job["frequencies"]
+    # frequencies go to the device in batches (one batched assembly, LU and refinement per batch)
+    nb = max(1, int(solver.batch_capacity())) if hasattr(solver, "batch_capacity") else 1
     try:
-        for i, f in enumerate(job["frequencies"]):
+        i = 0
+        while i < len(freqs):
             if _cancelled(job):
                 return True, solved
+            chunk = freqs[i : i + nb]
             t0 = time.perf_counter()
-            sols, field, ys, tim = solver.solve(f, job["excitations"], pts)
-            tim.setdefault("assembly_s", tim.get("fem_system_s", 0.0) + tim.get("fem_condensation_s", 0.0)
-                           + tim.get("bem_operator_s", 0.0) + tim.get("bem_matrix_s", 0.0)
-                           + tim.get("block_assembly_s", 0.0))
-            tim.setdefault("solve_s", max(0.0, time.perf_counter() - t0 - tim["assembly_s"]))
-            res = frequency_result(job, f, sols, field, ys, tim, first=(i == 0), solver=solver)
-            emit({"type": "result", "result": res} if event_mode else res)
-            solved = i + 1
+            if hasattr(solver, "solve_batch"):
+                results = solver.solve_batch(chunk, job["excitations"], pts)
+            else:
+                results = [solver.solve(f, job["excitations"], pts) for f in chunk]
+            wall = (time.perf_counter() - t0) / len(chunk)
+            for j, (sols, field, ys, tim) in enumerate(results):
+                if _cancelled(job):
+                    return True, solved
+                tim.setdefault("assembly_s", tim.get("fem_system_s", 0.0) + tim.get("fem_condensation_s", 0.0)
+                               + tim.get("bem_operator_s", 0.0) + tim.get("bem_matrix_s", 0.0)
+                               + tim.get("block_assembly_s", 0.0))
+                tim.setdefault("solve_s", max(0.0, wall - tim["assembly_s"] - tim.get("field_s", 0.0)))
+                res = frequency_result(job, chunk[j], sols, field, ys, tim, first=(i + j == 0), solver=solver)
+                emit({"type": "result", "result": res} if event_mode else res)
+                solved = i + j + 1
+            i += len(chunk)
     finally:
         solver.close()
     return _cancelled(job), solved

@@ -160,3 +160,27 @@ def test_reference_fixture_matches_oracle(gpu_device, fixture_model):
             err = _rel(getattr(s, key), ref[key][:, e])
             print(f"  excitation {e} {key}: {err:.1e}")
             assert err < 1e-9
+
+
+def test_batched_frequencies_match_single(gpu_device, tmp_path):
+    """Three frequencies solved as one device batch equal the same frequencies solved one at a time (the batched
+    LU may take other launch shapes: agreement to the solve's accuracy, 1e-10 FP64 / refinement tolerance)."""
+    from beat_hip import coupled_system as CS
+    from beat_hip.coupled import CoupledSolver
+
+    req = CC.coupled_request(tmp_path, "piston", precision="float64", freqs=(300.0,))
+    job = CS.parse_coupled_request(req)
+    for precision, tol in (("float64", 1e-10), ("float32", 1e-8)):
+        print(f"batched vs single, {precision}", flush=True)
+        s = CoupledSolver(job["model"], precision=precision, dev=gpu_device)
+        try:
+            freqs = [250.0, 700.0, 1300.0]
+            batch = s.solve_batch(freqs, job["excitations"], CC.observation_points())
+            for f, (sols, field, _, _) in zip(freqs, batch):
+                one, field1, _, _ = s.solve(f, job["excitations"], CC.observation_points())
+                for a, b in zip(sols, one):
+                    assert _rel(a.fem_pressure, b.fem_pressure) < tol
+                    assert _rel(a.diaphragm_velocity, b.diaphragm_velocity) < tol
+                assert _rel(field, field1) < tol
+        finally:
+            s.close()
