@@ -48,7 +48,10 @@ from . import tuning as TU
 from .engine import PRECISIONS, Device, DeviceMesh, _lda, _np_ptr, _ptr
 
 REFINE_TOL = 1e-10
-REFINE_MAX_ITER = 30  # the coupled matrix mixes FEM, BEM and lumped rows of different scales
+# bem_refine launches max_iter correction rounds whatever the convergence (its kernels exit on the device once a
+# matrix has converged): row-equilibrated coupled systems converge in 2-3, so a small cap keeps the idle rounds'
+# launch overhead low; a system that does not converge within it is reported by the solve status
+REFINE_MAX_ITER = 10
 FP64_RESIDUAL_LIMIT = 1e-8
 
 
