@@ -78,7 +78,8 @@ def beat_backend_from_request(request):
 
 
 def regular_quadrature_mode_from_config(config):
-    """solver.jl:93-101 for a GPU backend: only fixed order-4 quadrature."""
+    """solver.jl:93-101 for a GPU backend: fixed quadrature (the rule of quadrature_order for every frequency);
+    the wavelength-driven selection is the CPU backend's only, as in the reference."""
     mode = str(get_value(config, "regular_quadrature_mode", get_value(config, "quadrature_mode", "fixed")))
     mode = mode.strip().lower()
     if mode not in ("fixed", "wavelength"):

@@ -335,3 +335,26 @@ def test_fit_batch_counts_the_fp64_companion():
 def test_balanced_batches():
     assert D.balanced_batch(40, 32) == 20 and D.balanced_batch(40, 40) == 40 and D.balanced_batch(41, 40) == 21
     assert D.balanced_batch(7, 3) == 3 and D.balanced_batch(1, 8) == 1 and D.balanced_batch(60, 40) == 30
+
+
+def test_quadrature_orders_reach_the_mesh(mesh_file):
+    """quadrature_order / singular_order of the request select the regular rule (3 points for order 2, as the
+    reference's triangle_rule) and the Duffy order; the diagnostics report the order used; a Duffy order outside
+    1..4 and the wavelength mode are refused as the reference refuses them on a GPU backend."""
+    seen = {}
+
+    class Probe(FakeEngine):
+        def __init__(self, prep, qunit, **kw):
+            seen["qp"] = prep.qpoints.shape[1]
+            seen["orders"] = (prep.regular_order, prep.singular_order)
+            super().__init__(prep, qunit, **kw)
+
+    events = []
+    D.run_job(_request(mesh_file, quadrature_order=2, singular_order=3), Probe, events.append, batch=1)
+    assert seen == {"qp": 3, "orders": (2, 3)}
+    res = [e for e in events if e["type"] == "result"][0]["result"]
+    assert res["diagnostics"]["regular_quadrature_order"] == 2
+    for cfg, msg in (({"singular_order": 5}, "between 1 and 4"), ({"regular_quadrature_mode": "wavelength"},
+                                                                  "only for the BEAT CPU backend")):
+        with pytest.raises(ValueError, match=msg):
+            D.run_job(_request(mesh_file, **cfg), FakeEngine, lambda e: None, batch=1)
