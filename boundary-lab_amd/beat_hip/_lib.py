@@ -20,6 +20,8 @@ BEM_MAX_CHANNELS = 4
 BEM_NOALLOC_SCRATCH_BYTES = 64 << 20
 BEM_LU_FP32 = 0
 BEM_LU_SPLIT_BF16 = 1
+BEM_SING_WORKING = 0
+BEM_SING_FP64 = 1
 BEM_LU_MAX_N = 24576
 TIMER_REGULAR, TIMER_SINGULAR, TIMER_GATHER, TIMER_RHS, TIMER_LU, TIMER_FIELD, TIMER_SOLVE = range(7)
 TIMER_NAMES = ("regular", "singular", "gather", "rhs", "lu", "field", "solve")
@@ -93,6 +95,7 @@ SIGNATURES = {
     "bem_ctx_stream": (_i, [_p, C.POINTER(_p)]),
     "bem_ctx_set_fanout": (_i, [_p, _i]),
     "bem_ctx_set_lu_products": (_i, [_p, _i]),
+    "bem_ctx_set_singular_precision": (_i, [_p, _i]),
     "bem_ctx_set_tuning": (_i, [_p, C.c_void_p]),
     "bem_ctx_get_tuning": (_i, [_p, C.c_void_p]),
     "bem_ctx_timers": (_i, [_p, _i]),

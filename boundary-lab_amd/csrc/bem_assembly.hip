@@ -10,8 +10,9 @@
 // b = -(S + (i/k)(D* + 1/2 M10)) q for every channel, for a batch of wavenumbers at once.
 //
 // Data flow per frequency f:
-//   k_singular    (FP64)   one wave per vertex-sharing element pair: Duffy-rule blocks of the LHS (9 complex)
-//                          and of the RHS operator (3 complex), identity terms folded into coincident pairs.
+//   k_singular    (T)      one lane per vertex-sharing element pair and group of wavenumbers: Duffy-rule blocks
+//                          of the LHS (9 complex) and of the RHS operator (3 complex), written as FP64, identity
+//                          terms folded into coincident pairs (FP32 sums for mixed batches unless FP64 is asked).
 //   k_regular_lhs (T)      one wave per test element t; lanes sweep trial elements in locality-ordered chunks of
 //                          64. Each lane integrates its (t,s) pair with the 6x6 order-4 rule into a 3x3 block
 //                          (or picks up the FP64 singular block for adjacent pairs). Blocks go to LDS, then lanes
@@ -688,7 +689,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 }
 
 // ------------------------------------------------------------------------------------------------
-// Singular (vertex-sharing) pairs, FP64 Duffy rules: BeatEngineCpuAssembly.jl:151-218,
+// Singular (vertex-sharing) pairs, Duffy rules: BeatEngineCpuAssembly.jl:151-218,
 // BeatEngineRocmSingular.jl:84-289. Identity terms 1/2 M11 and 1/2 (i/k) M10 are added to the
 // coincident pair of each element (BeatEngineCore.jl:731-749 with the regular rule).
 // ------------------------------------------------------------------------------------------------
@@ -697,9 +698,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // One lane per (pair, group of NF frequencies) item; items run pair-major over pairs sorted by Duffy rule,
 // frequency group fastest, so the lanes of a wave share one rule (uniform trip count, broadcast rule-point loads)
 // and need no cross-lane reduction: each lane integrates its pair at its NF wavenumbers over all Duffy points in
-// FP64. The point geometry (r, 1/r, the normal projections, the basis products) is formed once for the NF
-// wavenumbers: NF = 2 halves that part of the work at the same occupancy (two waves per SIMD).
-template <int NF>
+// T arithmetic (the element geometry is formed in FP64 and rounded to T once; the block epilogue runs in FP64).
+// The point geometry (r, 1/r, the normal projections, the basis products) is formed once for the NF
+// wavenumbers. T = double: 1/r from the hardware estimate with two Newton steps and the FP64 phase series;
+// T = float: hardware reciprocal square root and sin/cos of the phase in revolutions, as the regular kernels.
+template <typename T, int NF>
 __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
   const int ngr = (nf + NF - 1) / NF;  // frequency groups per pair
   const long long item = (long long)blockIdx.x * 64 + threadIdx.x;
@@ -713,37 +716,50 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
   const double sg[3] = {double(p.sgn[3 * tr]), double(p.sgn[3 * tr + 1]), double(p.sgn[3 * tr + 2])};
   const double det = sg[0] * sg[1] * sg[2];
   double k[NF];
+  T kt[NF], kap[NF];  // wavenumber and wavenumber / 2 pi (revolutions per metre) in T
 #pragma unroll
-  for (int j = 0; j < NF; ++j) k[j] = p.k[f0 + min(j, nfv - 1)];
+  for (int j = 0; j < NF; ++j) {
+    k[j] = p.k[f0 + min(j, nfv - 1)];
+    kt[j] = T(k[j]);
+    kap[j] = T(k[j] / (2.0 * BEM_PI));
+  }
   // y - x = d0 + sxi se1 + set se2 - txi te1 - tet te2; both elements are flat, so (y - x).n_s = pa0 + txi pa1
   // + tet pa2 and (y - x).n_t = pb0 + sxi pb1 + set pb2 (the in-plane edges of an element are normal to its n)
-  double d0[3], te1[3], te2[3], se1[3], se2[3];
-  double pa0 = 0, pa1 = 0, pa2 = 0, pb0 = 0, pb1 = 0, pb2 = 0, nn = 0;
+  T d0[3], te1[3], te2[3], se1[3], se2[3];
+  T pa0, pa1, pa2, pb0, pb1, pb2;
+  double nn = 0;
+  {
+    double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double t0 = p.fvtx[9 * t + c], s0 = sg[c] * p.fvtx[9 * s + c];
-    te1[c] = p.fvtx[9 * t + 3 + c] - t0;
-    te2[c] = p.fvtx[9 * t + 6 + c] - t0;
-    se1[c] = sg[c] * p.fvtx[9 * s + 3 + c] - s0;
-    se2[c] = sg[c] * p.fvtx[9 * s + 6 + c] - s0;
-    d0[c] = s0 - t0;
-    const double ntc = p.fnrm[3 * t + c], nsc = sg[c] * p.fnrm[3 * s + c];
-    pa0 += d0[c] * nsc;
-    pa1 -= te1[c] * nsc;
-    pa2 -= te2[c] * nsc;
-    pb0 += d0[c] * ntc;
-    pb1 += se1[c] * ntc;
-    pb2 += se2[c] * ntc;
-    nn += ntc * nsc;
+    for (int c = 0; c < 3; ++c) {
+      const double t0 = p.fvtx[9 * t + c], s0 = sg[c] * p.fvtx[9 * s + c];
+      const double e1 = p.fvtx[9 * t + 3 + c] - t0, e2 = p.fvtx[9 * t + 6 + c] - t0;
+      const double f1 = sg[c] * p.fvtx[9 * s + 3 + c] - s0, f2 = sg[c] * p.fvtx[9 * s + 6 + c] - s0;
+      const double dd = s0 - t0;
+      const double ntc = p.fnrm[3 * t + c], nsc = sg[c] * p.fnrm[3 * s + c];
+      a0 += dd * nsc;
+      a1 -= e1 * nsc;
+      a2 -= e2 * nsc;
+      b0 += dd * ntc;
+      b1 += f1 * ntc;
+      b2 += f2 * ntc;
+      nn += ntc * nsc;
+      d0[c] = T(dd);
+      te1[c] = T(e1);
+      te2[c] = T(e2);
+      se1[c] = T(f1);
+      se2[c] = T(f2);
+    }
+    pa0 = T(a0), pa1 = T(a1), pa2 = T(a2), pb0 = T(b0), pb1 = T(b1), pb2 = T(b2);
   }
-  double Ar[NF][3], Ai[NF][3];
-  double Dr[NF][9], Di[NF][9], Gr[NF][9], Gi[NF][9];
+  T Ar[NF][3], Ai[NF][3];
+  T Dr[NF][9], Di[NF][9], Gr[NF][9], Gi[NF][9];
 #pragma unroll
   for (int j = 0; j < NF; ++j) {
 #pragma unroll
-    for (int l = 0; l < 3; ++l) Ar[j][l] = Ai[j][l] = 0.0;
+    for (int l = 0; l < 3; ++l) Ar[j][l] = Ai[j][l] = T(0);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Dr[j][i] = Di[j][i] = Gr[j][i] = Gi[j][i] = 0.0;
+    for (int i = 0; i < 9; ++i) Dr[j][i] = Di[j][i] = Gr[j][i] = Gi[j][i] = T(0);
   }
   const int off = p.rule_off[ru], len = p.rule_len[ru];
   const double* rp = p.rpts + (size_t)off * 5;
@@ -753,34 +769,31 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
 #pragma unroll
     for (int i = 0; i < 5; ++i) nxt[i] = rp[i];
   for (int q = 0; q < len; ++q) {
-    const double txi = nxt[0], tet = nxt[1], sxi = nxt[2], set = nxt[3], w = nxt[4];
+    const T txi = T(nxt[0]), tet = T(nxt[1]), sxi = T(nxt[2]), set = T(nxt[3]), w = T(nxt[4]);
     if (q + 1 < len) {
       rp += 5;
 #pragma unroll
       for (int i = 0; i < 5; ++i) nxt[i] = rp[i];
     }
-    double d[3];
+    T d[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = d0[c] + sxi * se1[c] + set * se2[c] - txi * te1[c] - tet * te2[c];
-    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    if (r2 == 0.0) continue;
-    // 1/r from the hardware reciprocal square root and two Newton steps (full FP64 accuracy)
-    double rinv = __builtin_amdgcn_rsq(r2);
-    rinv = rinv * fma(-0.5 * r2 * rinv, rinv, 1.5);
-    rinv = rinv * fma(-0.5 * r2 * rinv, rinv, 1.5);
-    const double r = r2 * rinv;
-    const double td = (pa0 + txi * pa1 + tet * pa2) * rinv;
-    const double ad = -(pb0 + sxi * pb1 + set * pb2) * rinv;
-    const double bt[3] = {1.0 - txi - tet, txi, tet};
-    const double bs[3] = {1.0 - sxi - set, sxi, set};
+    const T r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (r2 == T(0)) continue;
+    const T rinv = Phase<T>::rsqrt_(r2);
+    const T r = r2 * rinv;
+    const T td = (pa0 + txi * pa1 + tet * pa2) * rinv;
+    const T ad = -(pb0 + sxi * pb1 + set * pb2) * rinv;
+    const T bt[3] = {T(1) - txi - tet, txi, tet};
+    const T bs[3] = {T(1) - sxi - set, sxi, set};
 #pragma unroll
     for (int j = 0; j < NF; ++j) {
-      double sn, cs;
-      sincos_phase(k[j] * r, &sn, &cs);
-      const double gr = cs * rinv * w, gi = sn * rinv * w;  // 4 pi G w
-      const double hr = -gr * rinv - gi * k[j], hi = gr * k[j] - gi * rinv;
-      const double dgr = hr * td, dgi = hi * td;
-      const double har = hr * ad, hai = hi * ad;
+      T sn, cs;
+      Phase<T>::sincos_kr(kap[j], kt[j], r, &sn, &cs);
+      const T gr = cs * rinv * w, gi = sn * rinv * w;  // 4 pi G w
+      const T hr = -gr * rinv - gi * kt[j], hi = gr * kt[j] - gi * rinv;
+      const T dgr = hr * td, dgi = hi * td;
+      const T har = hr * ad, hai = hi * ad;
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
         Ar[j][l] += bt[l] * har;
@@ -790,7 +803,7 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
       for (int l = 0; l < 3; ++l)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          const double bb = bt[l] * bs[m];
+          const T bb = bt[l] * bs[m];
           Dr[j][3 * l + m] += bb * dgr;
           Di[j][3 * l + m] += bb * dgi;
           Gr[j][3 * l + m] += bb * gr;
@@ -812,8 +825,8 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
     double gsr = 0, gsi = 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      gsr += Gr[j][i];
-      gsi += Gi[j][i];
+      gsr += double(Gr[j][i]);
+      gsi += double(Gi[j][i]);
     }
     cplx<double>* out = p.lhs + ((size_t)f * p.P + pi) * 9;
 #pragma unroll
@@ -822,10 +835,10 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
       for (int m = 0; m < 3; ++m) {
         const double cp = ctp[3 * l] * cs0 * csp[3 * m] + ctp[3 * l + 1] * cs1 * csp[3 * m + 1] +
                           ctp[3 * l + 2] * cs2 * csp[3 * m + 2];
-        const double Hr = cp * gsr - k[j] * k[j] * nn * Gr[j][3 * l + m];
-        const double Hi = cp * gsi - k[j] * k[j] * nn * Gi[j][3 * l + m];
-        double re = scale * (-Dr[j][3 * l + m] - Hi * kinv);
-        const double im = scale * (-Di[j][3 * l + m] + Hr * kinv);
+        const double Hr = cp * gsr - k[j] * k[j] * nn * double(Gr[j][3 * l + m]);
+        const double Hi = cp * gsi - k[j] * k[j] * nn * double(Gi[j][3 * l + m]);
+        double re = scale * (-double(Dr[j][3 * l + m]) - Hi * kinv);
+        const double im = scale * (-double(Di[j][3 * l + m]) + Hr * kinv);
         if (self) re += 0.5 * p.m11[3 * l + m] * a2;
         out[3 * l + m] = mk<double>(re, im);
       }
@@ -835,10 +848,10 @@ __global__ __launch_bounds__(64) void k_singular(SingParams p, int nf) {
     for (int l = 0; l < 3; ++l) {
       // -(S + (i/k) D*) ; then -(1/2)(i/k) M10 on the coincident pair. S_l = sum_m G_lm: the trial
       // barycentric weights sum to one at every point.
-      const double Sr = Gr[j][3 * l] + Gr[j][3 * l + 1] + Gr[j][3 * l + 2];
-      const double Si = Gi[j][3 * l] + Gi[j][3 * l + 1] + Gi[j][3 * l + 2];
-      const double re = scale * (-Sr + Ai[j][l] * kinv);
-      double im = scale * (-Si - Ar[j][l] * kinv);
+      const double Sr = double(Gr[j][3 * l]) + double(Gr[j][3 * l + 1]) + double(Gr[j][3 * l + 2]);
+      const double Si = double(Gi[j][3 * l]) + double(Gi[j][3 * l + 1]) + double(Gi[j][3 * l + 2]);
+      const double re = scale * (-Sr + double(Ai[j][l]) * kinv);
+      double im = scale * (-Si - double(Ar[j][l]) * kinv);
       if (self) im -= 0.5 * kinv * p.m10[l] * a2;
       ro[l] = mk<double>(re, im);
     }
@@ -1193,14 +1206,21 @@ hipError_t launch_regular(const RegParams<T>& p, int nf, hipStream_t st) {
   }
   return hipGetLastError();
 }
-hipError_t launch_singular(const SingParams& p, int nf, hipStream_t st) {
+hipError_t launch_singular(const SingParams& p, int nf, bool fp32, hipStream_t st) {
   if (p.P == 0) return hipSuccess;
-  if (nf >= 2) {
-    const long long items = (long long)p.P * ((nf + 1) / 2);
-    hipLaunchKernelGGL(k_singular<2>, dim3((unsigned)((items + 63) / 64)), dim3(64), 0, st, p, nf);
+  const int NF = nf >= 2 ? 2 : 1;
+  const long long items = (long long)p.P * ((nf + NF - 1) / NF);
+  const dim3 grid((unsigned)((items + 63) / 64));
+  if (fp32) {
+    if (NF == 2)
+      hipLaunchKernelGGL((k_singular<float, 2>), grid, dim3(64), 0, st, p, nf);
+    else
+      hipLaunchKernelGGL((k_singular<float, 1>), grid, dim3(64), 0, st, p, nf);
   } else {
-    const long long items = (long long)p.P * nf;
-    hipLaunchKernelGGL(k_singular<1>, dim3((unsigned)((items + 63) / 64)), dim3(64), 0, st, p, nf);
+    if (NF == 2)
+      hipLaunchKernelGGL((k_singular<double, 2>), grid, dim3(64), 0, st, p, nf);
+    else
+      hipLaunchKernelGGL((k_singular<double, 1>), grid, dim3(64), 0, st, p, nf);
   }
   return hipGetLastError();
 }

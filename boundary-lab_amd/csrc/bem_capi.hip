@@ -75,6 +75,7 @@ struct bem_ctx {
   int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
   hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
   int lu_products;      // BEM_LU_FP32 or BEM_LU_SPLIT_BF16 for the large updates of FP32 factorizations
+  int sing_precision;   // BEM_SING_WORKING or BEM_SING_FP64: arithmetic of the Duffy sums of mixed batches
   int num_cus;          // compute units of the device
   std::vector<bem_mesh*> capture_meshes;  // meshes used by the sequence being captured
   bem_tuning tune;                        // launch-shape choices (bem_ctx_set_tuning)
@@ -239,6 +240,7 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
   c->timing = false;
   c->fanout = 1 + BEM_AUX_STREAMS;
   c->lu_products = BEM_LU_FP32;
+  c->sing_precision = BEM_SING_WORKING;
   c->tune = bem_tuning{0, BEM_TSW_MIN_BATCH_DEFAULT, 0, 0};
   c->num_cus = 0;
   if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
@@ -315,6 +317,14 @@ int bem_ctx_set_lu_products(bem_ctx* c, int mode) {
   if (!c) return fail(BEM_E_ARG, "ctx is NULL");
   if (mode != BEM_LU_FP32 && mode != BEM_LU_SPLIT_BF16) return fail(BEM_E_ARG, "unknown LU product mode");
   c->lu_products = mode;
+  return BEM_OK;
+}
+
+int bem_ctx_set_singular_precision(bem_ctx* c, int mode) {
+  if (!c) return fail(BEM_E_ARG, "ctx is NULL");
+  if (mode != BEM_SING_WORKING && mode != BEM_SING_FP64) return fail(BEM_E_ARG, "unknown singular precision mode");
+  if (c->capturing) return fail(BEM_E_STATE, "cannot change the singular precision during graph capture");
+  c->sing_precision = mode;
   return BEM_OK;
 }
 
@@ -760,7 +770,8 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
   std::memcpy(sp.sgn, m->sgn, sizeof(sp.sgn));
   {
     ScopedTimer tm(c, BEM_TIMER_SINGULAR, (long long)nf);
-    HIPCHK(launch_singular(sp, nf, st), "k_singular launch");
+    HIPCHK(launch_singular(sp, nf, precision == BEM_PREC_MIXED && c->sing_precision == BEM_SING_WORKING, st),
+           "k_singular launch");
   }
   // rhs partial sums
   RhsParams<T> rp;

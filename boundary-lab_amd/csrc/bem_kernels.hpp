@@ -130,7 +130,8 @@ struct MatBatch {
 
 template <typename T>
 hipError_t launch_regular(const RegParams<T>&, int nf, hipStream_t);
-hipError_t launch_singular(const SingParams&, int nf, hipStream_t);
+// fp32: Duffy sums in FP32 arithmetic (blocks still written as FP64); else FP64 throughout
+hipError_t launch_singular(const SingParams&, int nf, bool fp32, hipStream_t);
 template <typename T>
 hipError_t launch_rhs(const RhsParams<T>&, int nf, hipStream_t);
 // p.out is [nf][3F][p.n_src]: the RHS-operator block of every (test element, listed source face)

@@ -77,6 +77,15 @@ int bem_ctx_set_fanout(bem_ctx* ctx, int streams);
 #define BEM_LU_SPLIT_BF16 1
 int bem_ctx_set_lu_products(bem_ctx* ctx, int mode);
 
+/* Arithmetic of the Duffy-rule sums of the vertex-sharing element pairs in BEM_PREC_MIXED assemblies, per context.
+ * BEM_SING_WORKING (default): FP32, the working precision, as the reference's ROCm singular-correction kernels
+ * (JS/BeatEngineRocmSingular.jl:1-289, instantiated with the assembly's element type); the element geometry is
+ * formed in FP64 and rounded once, the blocks are written as FP64. BEM_SING_FP64: FP64 sums. FP64 assemblies always
+ * sum in FP64. Refused during graph capture. */
+#define BEM_SING_WORKING 0
+#define BEM_SING_FP64 1
+int bem_ctx_set_singular_precision(bem_ctx* ctx, int mode);
+
 /* Launch-shape choices (no reference counterpart: tuning), per context; the project's tuner (beat_hip/autotune.py)
  * times candidates on the device and the engine applies its table per (precision, N, batch). Captured graphs keep
  * the shapes of the tuning in force when they were captured.

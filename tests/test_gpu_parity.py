@@ -65,6 +65,38 @@ def test_assembly_matches_oracle(gpu_device, sphere, oracle_sphere, precision, t
     eng.close()
 
 
+def test_singular_precision_of_mixed_assembly(gpu_device, sphere, oracle_sphere):
+    """Mixed assemblies sum the Duffy rules in FP32 by default (as the reference's ROCm singular kernels in the
+    working precision) or in FP64 on request: both within the mixed tolerance of the oracle, and within 2e-6 of each
+    other (FP32 rounding of the singular blocks only); unknown modes are refused."""
+    from beat_hip import _lib
+
+    lib, ctx = gpu_device.lib, gpu_device.ctx
+    eng = _engine(gpu_device, sphere)
+    vp = sphere.vperm
+    lhs_ref = oracle_sphere["lhs"][np.ix_(vp, vp)]
+    N = sphere.n_vertices
+    A = {}
+    try:
+        for mode in (_lib.BEM_SING_FP64, _lib.BEM_SING_WORKING):
+            _lib.check(lib.bem_ctx_set_singular_precision(ctx, mode), "bem_ctx_set_singular_precision")
+            b = eng.batch("mixed", 2, None, RHO, C0)
+            b.set_frequencies([FREQ, 3 * FREQ])
+            with gpu_device.torch.cuda.stream(gpu_device.stream):
+                b._assemble()
+            gpu_device.sync()
+            A[mode] = b.A0[:, :, :N].cpu().numpy().astype(np.complex128)
+            b.close()
+            print("singular mode", mode, "A rel", _rel(A[mode][0], lhs_ref))
+            assert _rel(A[mode][0], lhs_ref) < 1e-5
+        for f in range(2):
+            assert _rel(A[_lib.BEM_SING_WORKING][f], A[_lib.BEM_SING_FP64][f]) < 2e-6
+        assert lib.bem_ctx_set_singular_precision(ctx, 2) == _lib.BEM_E_ARG
+    finally:
+        lib.bem_ctx_set_singular_precision(ctx, _lib.BEM_SING_WORKING)
+        eng.close()
+
+
 @pytest.fixture(scope="module")
 def fan_sphere():
     """Latitude/longitude sphere whose two poles are shared by 80 triangles each (1,120 triangles): star rows
