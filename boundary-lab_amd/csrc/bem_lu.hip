@@ -317,10 +317,10 @@ __device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, uns
 // updates use 4 x 4 (256 x 128, 1024 threads, one workgroup per CU): every staged A and B element feeds twice
 // as many products as with 2 x 2 (128 x 64), which cut a 2,746-row K = 256 update over 40 matrices from 3.31
 // to 2.56 ms; updates of one 128-column block keep 2 x 2 (more workgroups).
-template <int WY, int WX>
+template <int WY, int WX, int TN>
 __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, const GemmShape& g, int mat, int tx,
                                                  int ty) {
-  constexpr int BM = 64 * WY, BN = 32 * WX, KC = 16;  // rows, complex columns, complex k per stage
+  constexpr int BM = 64 * WY, BN = 32 * WX * TN, KC = 16;  // rows, complex columns, complex k per stage
   constexpr int NT = 64 * WY * WX;                     // threads
   constexpr int QA = BM * KC / NT, QB = KC * BN / NT;  // staged A and B elements per thread and stage
   constexpr int RW = KC + 4;  // LDS row: KC (re, im) bf16 pairs + 16 B of padding, in dwords
@@ -331,23 +331,25 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
   const int wy = w / WX, wx = w % WX;
   const int bm = ty * BM, bn = tx * BN;
   const int half = lane >> 5, l32 = lane & 31;
-  __shared__ __attribute__((aligned(16))) unsigned Ah[BM][RW], Al[BM][RW], Bh[BN][RW], Bl[BN][RW];
+  // two stage buffers: the stage after the current one is written while the current one is multiplied, so a
+  // stage needs one barrier (the 4 x 4 tile: 2 x 60 KB)
+  __shared__ __attribute__((aligned(16))) unsigned Ah[2][BM][RW], Al[2][BM][RW], Bh[2][BN][RW], Bl[2][BN][RW];
 
-  // accumulators: block ti, register v holds row 32 ti + (v/4)*8 + half*4 + v%4 of the wave's 64 rows and
-  // complex column l32 of its 32. They start at zero and C is added in the epilogue: a C tile read before the
-  // first product stalls the workgroup on HBM latency (measured 10 % slower on the 2,874-row updates).
-  const int rbase = bm + 64 * wy + half * 4;
-  const int col = bn + 32 * wx + l32;
-  f32x16 acr[2], aci[2];
+  // accumulators: block (ti, tj), register v holds row 32 ti + (v/4)*8 + half*4 + v%4 of the wave's 64 rows and
+  // complex column 32 tj + l32 of its 32 TN. They start at zero and C is added in the epilogue: a C tile read
+  // before the first product stalls the workgroup on HBM latency (measured 10 % slower on the 2,874-row updates).
+  f32x16 acr[2][TN], aci[2][TN];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acr[ti][v] = aci[ti][v] = 0.f;
+    for (int tj = 0; tj < TN; ++tj)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acr[ti][tj][v] = aci[ti][tj][v] = 0.f;
 
   // staging per stage: A = BM x 16 complex, B = 16 x BN complex, through buffer descriptors with scalar bases at
   // the tile's first A row and first B row: 32-bit byte offsets (at most 256 rows x lda x 8 B) instead of 64-bit
   // addresses, and an element outside the update gets an offset past the descriptor's range, which loads zero
-  // without a branch
+  // (and drops a store) without a branch
   constexpr int OOB = 0x7fffffff;
   const unsigned a_bytes = (unsigned)min((long long)BM * lda * 8, (long long)0x7ffffff0);
   const unsigned b_bytes = (unsigned)min((long long)g.K * lda * 8, (long long)0x7ffffff0);
@@ -356,49 +358,57 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
   const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
       (void*)(A + (size_t)g.b_r0 * lda + g.b_c0 + bn), (short)0, (int)b_bytes, 0x00020000);
   const int ldab = (int)lda * 8;
+  // element e = tid + NT q of a stage: A row e / 16 and k e % 16, B k e / BN and column e % BN; the lane's part of
+  // the byte offset is the vector operand, the part of q and of the stage (the same for all lanes) the scalar one
+  static_assert(NT % BN == 0 && NT % 16 == 0, "staging assumes whole rows per q");
+  const int a_r = tid >> 4, a_k = tid & 15, b_k = tid / BN, b_j = tid % BN;
+  const int a_loff = a_r * ldab + a_k * 8, b_loff = b_k * ldab + b_j * 8;
+  const int a_rlim = g.M - bm, b_jlim = g.Nc - bn;
   cplx<float> ra[QA], rb[QB];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
-      const int e = tid + NT * q, r = e >> 4, kk = e & 15;
-      const int off = (bm + r < g.M && k0 + kk < g.K) ? r * ldab + (k0 + kk) * 8 : OOB;
-      const gather_u2 v = __builtin_amdgcn_raw_buffer_load_b64(rsa, off, 0, 0);
+      const int dr = (NT / 16) * q;
+      const int off = (a_r + dr < a_rlim && k0 + a_k < g.K) ? a_loff : OOB;
+      const gather_u2 v = __builtin_amdgcn_raw_buffer_load_b64(rsa, off, dr * ldab + k0 * 8, 0);
       ra[q] = mk<float>(__uint_as_float(v.x), __uint_as_float(v.y));
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-      const int e = tid + NT * q, kk = e / BN, j = e % BN;
-      const int off = (k0 + kk < g.K && bn + j < g.Nc) ? (k0 + kk) * ldab + j * 8 : OOB;
-      const gather_u2 v = __builtin_amdgcn_raw_buffer_load_b64(rsb, off, 0, 0);
+      const int dk = (NT / BN) * q;
+      const int off = (k0 + b_k + dk < g.K && b_j < b_jlim) ? b_loff : OOB;
+      const gather_u2 v = __builtin_amdgcn_raw_buffer_load_b64(rsb, off, (k0 + dk) * ldab, 0);
       rb[q] = mk<float>(__uint_as_float(v.x), __uint_as_float(v.y));
     }
   };
-  auto stash = [&]() {
+  auto stash = [&](int sb) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int e = tid + NT * q;
       unsigned hi, lo;
       split_bf16(-ra[q].re, -ra[q].im, hi, lo);
-      Ah[e >> 4][e & 15] = hi;
-      Al[e >> 4][e & 15] = lo;
+      Ah[sb][e >> 4][e & 15] = hi;
+      Al[sb][e >> 4][e & 15] = lo;
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int e = tid + NT * q;
       unsigned hi, lo;
       split_bf16(rb[q].re, rb[q].im, hi, lo);
-      Bh[e % BN][e / BN] = hi;
-      Bl[e % BN][e / BN] = lo;
+      Bh[sb][e % BN][e / BN] = hi;
+      Bl[sb][e % BN][e / BN] = lo;
     }
   };
-  auto multiply = [&]() {
+  // the products of one group of eight complex k (the MFMA's sixteen bf16) of stage buffer sb
+  auto multiply_kg = [&](int sb, int kg) {
+    const int d0 = 8 * kg + 4 * half;  // first complex k of this lane's eight bf16
+    bf16x8 Brh[TN], Brl[TN], Bih[TN], Bil[TN];  // (br, -bi) and (bi, br) per complex k
 #pragma unroll
-    for (int kg = 0; kg < KC / 8; ++kg) {
-      const int d0 = 8 * kg + 4 * half;  // first complex k of this lane's eight bf16
-      const int cl = 32 * wx + l32;
-      const u32x4 bh = *reinterpret_cast<const u32x4*>(&Bh[cl][d0]);
-      const u32x4 bl = *reinterpret_cast<const u32x4*>(&Bl[cl][d0]);
-      u32x4 brh, brl, bih, bil;  // (br, -bi) and (bi, br) per complex k
+    for (int tj = 0; tj < TN; ++tj) {
+      const int cl = 32 * (TN * wx + tj) + l32;
+      const u32x4 bh = *reinterpret_cast<const u32x4*>(&Bh[sb][cl][d0]);
+      const u32x4 bl = *reinterpret_cast<const u32x4*>(&Bl[sb][cl][d0]);
+      u32x4 brh, brl, bih, bil;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         brh[i] = bh[i] ^ 0x80000000u;
@@ -406,53 +416,107 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
         bih[i] = (bh[i] >> 16) | (bh[i] << 16);
         bil[i] = (bl[i] >> 16) | (bl[i] << 16);
       }
-      const bf16x8 Brh = __builtin_bit_cast(bf16x8, brh), Brl = __builtin_bit_cast(bf16x8, brl);
-      const bf16x8 Bih = __builtin_bit_cast(bf16x8, bih), Bil = __builtin_bit_cast(bf16x8, bil);
-      // one 32-row block's A operands at a time (8 registers live instead of 16)
+      Brh[tj] = __builtin_bit_cast(bf16x8, brh);
+      Brl[tj] = __builtin_bit_cast(bf16x8, brl);
+      Bih[tj] = __builtin_bit_cast(bf16x8, bih);
+      Bil[tj] = __builtin_bit_cast(bf16x8, bil);
+    }
+    // one 32-row block's A operands at a time (8 registers live instead of 16)
 #pragma unroll
-      for (int ti = 0; ti < 2; ++ti) {
-        const int r = 64 * wy + 32 * ti + l32;
-        const bf16x8 Ahh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Ah[r][d0]));
-        const bf16x8 All = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Al[r][d0]));
-        acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brh, acr[ti], 0, 0, 0);
-        aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bih, aci[ti], 0, 0, 0);
-        acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brl, acr[ti], 0, 0, 0);
-        aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bil, aci[ti], 0, 0, 0);
-        acr[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Brh, acr[ti], 0, 0, 0);
-        aci[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Bih, aci[ti], 0, 0, 0);
+    for (int ti = 0; ti < 2; ++ti) {
+      const int r = 64 * wy + 32 * ti + l32;
+      const bf16x8 Ahh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Ah[sb][r][d0]));
+      const bf16x8 All = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Al[sb][r][d0]));
+#pragma unroll
+      for (int tj = 0; tj < TN; ++tj) {
+        acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brh[tj], acr[ti][tj], 0, 0, 0);
+        aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bih[tj], aci[ti][tj], 0, 0, 0);
+        acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brl[tj], acr[ti][tj], 0, 0, 0);
+        aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bil[tj], aci[ti][tj], 0, 0, 0);
+        acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Brh[tj], acr[ti][tj], 0, 0, 0);
+        aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Bih[tj], aci[ti][tj], 0, 0, 0);
       }
     }
   };
-  for (int k0 = 0; k0 < g.K; k0 += KC) {
-    if (k0 == 0) fetch(0);
-    stash();
-    __syncthreads();
-    if (k0 + KC < g.K) fetch(k0 + KC);  // next stage in flight during the multiply
-    multiply();
+  static_assert(KC == 16, "a stage is two groups of eight complex k");
+  // Stage s multiplies buffer s & 1 in two groups; between them the staged registers (stage s + 1, loaded during
+  // the previous stage) are split into buffer (s + 1) & 1, whose last reader was stage s - 1 before the previous
+  // barrier, and then refilled with stage s + 2: the loads have the second group of this stage and the first of
+  // the next (two groups of products) to arrive, with one set of staging registers. A persistent workgroup's
+  // previous tile ended with a barrier after its last products.
+  const int nst = (g.K + KC - 1) / KC;
+  fetch(0);
+  stash(0);
+  if (nst > 1) fetch(KC);
+  __syncthreads();
+  for (int s = 0; s < nst; ++s) {
+    multiply_kg(s & 1, 0);
+    if (s + 1 < nst) stash((s + 1) & 1);
+    if (s + 2 < nst) fetch((s + 2) * KC);
+    multiply_kg(s & 1, 1);
     __syncthreads();
   }
+  // epilogue: C += acc (the accumulators hold -A B). The C loads of eight rows are issued together through a
+  // descriptor at the tile's first C row (eight loads in flight per thread, then the stores), not as one dependent
+  // load-store round trip per element. The lane's offset is the vector operand (past the range for an element
+  // outside the update); the row's part, the same for all lanes, is the scalar offset.
+  const unsigned c_bytes = (unsigned)min((long long)BM * lda * 8, (long long)0x7ffffff0);
+  const __amdgpu_buffer_rsrc_t rsc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(A + (size_t)(g.c_r0 + bm) * lda + g.c_c0 + bn), (short)0, (int)c_bytes, 0x00020000);
+  const int lrow = 64 * wy + half * 4;  // the lane's first row in the tile
+  constexpr int EG = 8;
 #pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
+  for (int tj = 0; tj < TN; ++tj) {
+    const int cl = 32 * (TN * wx + tj) + l32;
+    const int loff = lrow * ldab + cl * 8;
+    const int rlim = bn + cl < g.Nc ? g.M - bm : 0;  // tile rows below rlim are in the update
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int r = rbase + 32 * ti + (v >> 2) * 8 + (v & 3);
-      if (r < g.M && col < g.Nc) {
-        cplx<float>* cp = A + (size_t)(g.c_r0 + r) * lda + g.c_c0 + col;
-        const cplx<float> c = *cp;
-        *cp = mk<float>(c.re + acr[ti][v], c.im + aci[ti][v]);
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int v0 = 0; v0 < 16; v0 += EG) {
+        gather_u2 c[EG];
+#pragma unroll
+        for (int i = 0; i < EG; ++i) {
+          const int v = v0 + i, dr = 32 * ti + (v >> 2) * 8 + (v & 3);
+          c[i] = __builtin_amdgcn_raw_buffer_load_b64(rsc, lrow + dr < rlim ? loff : OOB, dr * ldab, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < EG; ++i) {
+          const int v = v0 + i, dr = 32 * ti + (v >> 2) * 8 + (v & 3);
+          gather_u2 o;
+          o.x = __float_as_uint(__uint_as_float(c[i].x) + acr[ti][tj][v]);
+          o.y = __float_as_uint(__uint_as_float(c[i].y) + aci[ti][tj][v]);
+          __builtin_amdgcn_raw_buffer_store_b64(o, rsc, lrow + dr < rlim ? loff : OOB, dr * ldab, 0);
+        }
       }
-    }
+  }
 }
 
-// Tiles in the order x (columns) fastest, then rows, then matrices; a launch with fewer workgroups than tiles
-// walks them persistently, so a large update can leave CUs free for a concurrent panel chain.
-template <int WY, int WX>
+// Tile order. A launch's workgroups go to the eight XCDs in turn (workgroup b to XCD b % 8), each XCD with its own
+// L2, so the tiles are split into eight contiguous ranges, one per XCD, walked by that XCD's workgroups in turn (a
+// launch with fewer workgroups than tiles walks them persistently, so a large update can leave CUs free for a
+// concurrent panel chain). Within a matrix the tiles go column-major over groups of GR row bands: the A rows of a
+// group stay in the XCD's L2 while its column bands stream past. Against the plain order (columns fastest, tile
+// b + k gridDim.x) this reads 36 % fewer bytes past L2 on the 3,002-unknown factorizations, at the same speed.
+template <int WY, int WX, int TN>
 __global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
     MatBatch<float> mb, GemmShape g, const int* skip, int nx, int ny, int ntiles) {
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int tx = tile % nx, ty = (tile / nx) % ny, mat = tile / (nx * ny);
+  constexpr int NXCD = 8, GR = 4;
+  const int nwg = gridDim.x, b = blockIdx.x;
+  int first = b, step = nwg, end = ntiles;
+  if (nwg >= NXCD) {
+    const int x = b % NXCD, per = ntiles / NXCD, rem = ntiles % NXCD;
+    first = x * per + min(x, rem) + b / NXCD;
+    step = nwg / NXCD + (x < nwg % NXCD ? 1 : 0);  // workgroups on XCD x
+    end = (x + 1) * per + min(x + 1, rem);
+  }
+  const int per_mat = nx * ny;
+  for (int tile = first; tile < end; tile += step) {
+    const int mat = tile / per_mat, tm = tile % per_mat;
+    const int grp = tm / (GR * nx), tg = tm % (GR * nx), rows = min(GR, ny - grp * GR);
+    const int ty = grp * GR + tg % rows, tx = tg / rows;
     if (skip && skip[mat]) continue;
-    gemm_bf16x3_tile<WY, WX>(mb, g, mat, tx, ty);
+    gemm_bf16x3_tile<WY, WX, TN>(mb, g, mat, tx, ty);
   }
 }
 
@@ -1633,10 +1697,10 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
       if (split_tile == 0 && g.M >= 512 && g.Nc >= 256) {
         const int nx = (g.Nc + 127) / 128, ny = (g.M + 255) / 256, nt = nx * ny * batch;
         const int nwg = max_wgs > 0 ? min(nt, max_wgs) : nt;
-        hipLaunchKernelGGL((k_gemm_bf16x3<4, 4>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
+        hipLaunchKernelGGL((k_gemm_bf16x3<4, 4, 1>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
       } else {
         const int nx = (g.Nc + 63) / 64, ny = (g.M + 127) / 128, nt = nx * ny * batch;
-        hipLaunchKernelGGL((k_gemm_bf16x3<2, 2>), dim3(nt), dim3(256), 0, st, mb, g, skip, nx, ny, nt);
+        hipLaunchKernelGGL((k_gemm_bf16x3<2, 2, 1>), dim3(nt), dim3(256), 0, st, mb, g, skip, nx, ny, nt);
       }
       return hipGetLastError();
     }
