@@ -145,6 +145,40 @@ def field_probe():
     dev.close()
 
 
+def lu_probe(workload="horn"):
+    """Workload of the LU counter passes at the batch shape of a job (horn: 40 matrices of N = 3,002; scale: 24 of
+    N = 10,069), random well-conditioned complex matrices, the tuned launch shapes, one stream: a 1 GiB copy
+    (calibration, the run's first copy kernel), then one split-bf16 factorization (after the copy of the matrices
+    into place) with the FP64 refinement after it. Each step prints before it runs."""
+    import torch
+
+    from beat_hip import _lib
+    from beat_hip import autotune as AT
+    from beat_hip import tuning as TU
+    from beat_hip.engine import Device
+
+    n, batch = (3002, 40) if workload == "horn" else (10069, 24)
+    dev = Device(0)
+    _lib.check(dev.lib.bem_ctx_set_fanout(dev.ctx, 1), "bem_ctx_set_fanout")
+    with torch.cuda.stream(dev.stream):  # the calibration copy is the run's first copy kernel
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev.device)
+        b = torch.empty_like(a)
+        b.copy_(a)
+    dev.sync()
+    del a, b
+    log(f"probe: LU of {batch} random matrices of N = {n}")
+    run = AT.lu_workload(dev, "mixed", n, batch)
+    AT._set(dev, TU.table().lu_choice("mixed", n, batch))
+    dev.sync()
+    log("probe: factorization and refinement")
+    run()
+    dev.sync()
+    info, rel = run.check()
+    log(f"probe: done (info {info}, residual {rel:.1e})")
+    del run
+    dev.close()
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -568,9 +602,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the workload of the counter passes (copy calibration + one eager job of the "
                          "chosen --workload)")
-    ap.add_argument("--probe", choices=["job", "field"], default="job",
-                    help="--traffic-probe workload: one eager job of --workload, or the field evaluation of the "
-                         "20k-triangle job alone")
+    ap.add_argument("--probe", choices=["job", "field", "lu"], default="job",
+                    help="--traffic-probe workload: one eager job of --workload, the field evaluation of the "
+                         "20k-triangle job alone, or the LU of --workload's batch shape on random matrices")
     ap.add_argument("--workload", choices=["horn", "scale", "coupled"], default="horn",
                     help="horn: the 6k-triangle horn-in-box job (headline); scale: the 20k-triangle two-way "
                          "loudspeaker job; coupled: the FEM-BEM shell with a piston")
@@ -581,7 +615,12 @@ def main():
         sys.exit(2)
 
     if args.traffic_probe:
-        field_probe() if args.probe == "field" else traffic_probe(args.workload)
+        if args.probe == "field":
+            field_probe()
+        elif args.probe == "lu":
+            lu_probe(args.workload)
+        else:
+            traffic_probe(args.workload)
         return
     if args.workload == "scale":
         run_scale(args)
