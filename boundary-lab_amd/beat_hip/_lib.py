@@ -20,6 +20,7 @@ BEM_MAX_CHANNELS = 4
 BEM_NOALLOC_SCRATCH_BYTES = 64 << 20
 BEM_LU_FP32 = 0
 BEM_LU_SPLIT_BF16 = 1
+BEM_LU_BF16 = 2
 BEM_SING_WORKING = 0
 BEM_SING_FP64 = 1
 BEM_LU_MAX_N = 24576

@@ -74,7 +74,7 @@ struct bem_ctx {
   hipEvent_t fork_ev, join_ev[BEM_AUX_STREAMS];
   int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
   hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
-  int lu_products;      // BEM_LU_FP32 or BEM_LU_SPLIT_BF16 for the large updates of FP32 factorizations
+  int lu_products;      // BEM_LU_FP32, BEM_LU_SPLIT_BF16 or BEM_LU_BF16: large updates of FP32 factorizations
   int sing_precision;   // BEM_SING_WORKING or BEM_SING_FP64: arithmetic of the Duffy sums of mixed batches
   int num_cus;          // compute units of the device
   std::vector<bem_mesh*> capture_meshes;  // meshes used by the sequence being captured
@@ -315,7 +315,8 @@ int bem_ctx_set_fanout(bem_ctx* c, int streams) {
 
 int bem_ctx_set_lu_products(bem_ctx* c, int mode) {
   if (!c) return fail(BEM_E_ARG, "ctx is NULL");
-  if (mode != BEM_LU_FP32 && mode != BEM_LU_SPLIT_BF16) return fail(BEM_E_ARG, "unknown LU product mode");
+  if (mode != BEM_LU_FP32 && mode != BEM_LU_SPLIT_BF16 && mode != BEM_LU_BF16)
+    return fail(BEM_E_ARG, "unknown LU product mode");
   c->lu_products = mode;
   return BEM_OK;
 }
@@ -1233,11 +1234,11 @@ int bem_lu_factor(bem_ctx* c, int precision, int n, int batch, void* A, long lon
   if (precision == BEM_PREC_MIXED) {
     MatBatch<float> mb = {static_cast<cplx<float>*>(A), lda, strideA, n};
     e = lu_factor_batch<float>(mb, batch, ipiv, perm, strideP, info, nullptr, static_cast<cplx<float>*>(work),
-                               c->stream, &la, c->lu_products == BEM_LU_SPLIT_BF16);
+                               c->stream, &la, c->lu_products);
   } else {
     MatBatch<double> mb = {static_cast<cplx<double>*>(A), lda, strideA, n};
     e = lu_factor_batch<double>(mb, batch, ipiv, perm, strideP, info, nullptr, static_cast<cplx<double>*>(work),
-                                c->stream, &la, false);
+                                c->stream, &la, BEM_LU_FP32);
   }
   if (e != hipSuccess) return hipfail(e, "lu_factor launch");
   return BEM_OK;

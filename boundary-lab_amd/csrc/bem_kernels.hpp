@@ -180,7 +180,7 @@ struct LuLookahead {
 };
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>&, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, cplx<T>* W, hipStream_t, const LuLookahead* la, bool split_bf16);
+                           const int* skip, cplx<T>* W, hipStream_t, const LuLookahead* la, int products);
 // Triangular solves: one workgroup per matrix when N <= 4096 and batch >= wg_min_batch (0: never), else the
 // blocked multi-workgroup sweeps.
 constexpr int BEM_TSW_MIN_BATCH_DEFAULT = 8;

@@ -25,6 +25,7 @@
 // v_mfma_f32_32x32x2f32 (k_gemm_tile32), FP64 v_mfma_f64_16x16x4f64 (k_gemm_sub).
 
 #include "bem_kernels.hpp"
+#include "bemhip.h"
 
 #define NBO BEM_NBO
 #define MAX_CH BEM_MAX_CH
@@ -317,7 +318,7 @@ __device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, uns
 // updates use 4 x 4 (256 x 128, 1024 threads, one workgroup per CU): every staged A and B element feeds twice
 // as many products as with 2 x 2 (128 x 64), which cut a 2,746-row K = 256 update over 40 matrices from 3.31
 // to 2.56 ms; updates of one 128-column block keep 2 x 2 (more workgroups).
-template <int WY, int WX, int TN>
+template <int WY, int WX, int TN, int NP>
 __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, const GemmShape& g, int mat, int tx,
                                                  int ty) {
   constexpr int BM = 64 * WY, BN = 32 * WX * TN, KC = 16;  // rows, complex columns, complex k per stage
@@ -333,7 +334,9 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
   const int half = lane >> 5, l32 = lane & 31;
   // two stage buffers: the stage after the current one is written while the current one is multiplied, so a
   // stage needs one barrier (the 4 x 4 tile: 2 x 60 KB)
-  __shared__ __attribute__((aligned(16))) unsigned Ah[2][BM][RW], Al[2][BM][RW], Bh[2][BN][RW], Bl[2][BN][RW];
+  // (NP = 1: no lo rows; their arrays shrink to one row)
+  constexpr int LM = NP == 3 ? BM : 1, LN = NP == 3 ? BN : 1;
+  __shared__ __attribute__((aligned(16))) unsigned Ah[2][BM][RW], Al[2][LM][RW], Bh[2][BN][RW], Bl[2][LN][RW];
 
   // accumulators: block (ti, tj), register v holds row 32 ti + (v/4)*8 + half*4 + v%4 of the wave's 64 rows and
   // complex column 32 tj + l32 of its 32 TN. They start at zero and C is added in the epilogue: a C tile read
@@ -388,7 +391,7 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
       unsigned hi, lo;
       split_bf16(-ra[q].re, -ra[q].im, hi, lo);
       Ah[sb][e >> 4][e & 15] = hi;
-      Al[sb][e >> 4][e & 15] = lo;
+      if constexpr (NP == 3) Al[sb][e >> 4][e & 15] = lo;
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -396,7 +399,7 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
       unsigned hi, lo;
       split_bf16(rb[q].re, rb[q].im, hi, lo);
       Bh[sb][e % BN][e / BN] = hi;
-      Bl[sb][e % BN][e / BN] = lo;
+      if constexpr (NP == 3) Bl[sb][e % BN][e / BN] = lo;
     }
   };
   // the products of one group of eight complex k (the MFMA's sixteen bf16) of stage buffer sb
@@ -407,7 +410,7 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
     for (int tj = 0; tj < TN; ++tj) {
       const int cl = 32 * (TN * wx + tj) + l32;
       const u32x4 bh = *reinterpret_cast<const u32x4*>(&Bh[sb][cl][d0]);
-      const u32x4 bl = *reinterpret_cast<const u32x4*>(&Bl[sb][cl][d0]);
+      const u32x4 bl = NP == 3 ? *reinterpret_cast<const u32x4*>(&Bl[sb][NP == 3 ? cl : 0][d0]) : bh;
       u32x4 brh, brl, bih, bil;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -426,15 +429,20 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
     for (int ti = 0; ti < 2; ++ti) {
       const int r = 64 * wy + 32 * ti + l32;
       const bf16x8 Ahh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Ah[sb][r][d0]));
-      const bf16x8 All = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Al[sb][r][d0]));
 #pragma unroll
       for (int tj = 0; tj < TN; ++tj) {
         acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brh[tj], acr[ti][tj], 0, 0, 0);
         aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bih[tj], aci[ti][tj], 0, 0, 0);
-        acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brl[tj], acr[ti][tj], 0, 0, 0);
-        aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bil[tj], aci[ti][tj], 0, 0, 0);
-        acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Brh[tj], acr[ti][tj], 0, 0, 0);
-        aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Bih[tj], aci[ti][tj], 0, 0, 0);
+      }
+      if constexpr (NP == 3) {
+        const bf16x8 All = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Al[sb][r][d0]));
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj) {
+          acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Brl[tj], acr[ti][tj], 0, 0, 0);
+          aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahh, Bil[tj], aci[ti][tj], 0, 0, 0);
+          acr[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Brh[tj], acr[ti][tj], 0, 0, 0);
+          aci[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(All, Bih[tj], aci[ti][tj], 0, 0, 0);
+        }
       }
     }
   };
@@ -498,7 +506,7 @@ __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, cons
 // concurrent panel chain). Within a matrix the tiles go column-major over groups of GR row bands: the A rows of a
 // group stay in the XCD's L2 while its column bands stream past. Against the plain order (columns fastest, tile
 // b + k gridDim.x) this reads 36 % fewer bytes past L2 on the 3,002-unknown factorizations, at the same speed.
-template <int WY, int WX, int TN>
+template <int WY, int WX, int TN, int NP>
 __global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16x3(
     MatBatch<float> mb, GemmShape g, const int* skip, int nx, int ny, int ntiles) {
   constexpr int NXCD = 8, GR = 4;
@@ -516,7 +524,7 @@ __global__ __launch_bounds__(64 * WY * WX) __attribute__((amdgpu_waves_per_eu(2)
     const int grp = tm / (GR * nx), tg = tm % (GR * nx), rows = min(GR, ny - grp * GR);
     const int ty = grp * GR + tg % rows, tx = tg / rows;
     if (skip && skip[mat]) continue;
-    gemm_bf16x3_tile<WY, WX, TN>(mb, g, mat, tx, ty);
+    gemm_bf16x3_tile<WY, WX, TN, NP>(mb, g, mat, tx, ty);
   }
 }
 
@@ -1675,11 +1683,12 @@ int panel_width(int N, int c0) {
   return mr <= 3 ? 8 : (mr <= 6 ? 4 : (mr <= 12 ? 2 : 1));
 }
 
-// split: FP32 products as three bf16 products (k_gemm_bf16x3) for the large updates; max_wgs > 0 caps the
-// workgroups of a large split update (persistent tiles)
+// split: the large updates of FP32 factorizations on bf16 MFMA (k_gemm_bf16x3), BEM_LU_SPLIT_BF16 with three bf16
+// products per FP32 product, BEM_LU_BF16 with one (BEM_LU_FP32: FP32 MFMA); max_wgs > 0 caps the workgroups of a
+// large update (persistent tiles)
 template <typename T>
 static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape& g, const int* skip,
-                              hipStream_t st, bool split = false, int max_wgs = 0, int split_tile = 0) {
+                              hipStream_t st, int split = BEM_LU_FP32, int max_wgs = 0, int split_tile = 0) {
   if (g.M <= 0 || g.Nc <= 0 || g.K <= 0) return hipSuccess;
   if (g.Nc <= 16 && g.K <= BEM_NBO) {
     // as many accumulator columns as the update has (the narrow panels of large N update 4 or 8 columns)
@@ -1694,13 +1703,20 @@ static hipError_t launch_gemm(const MatBatch<T>& mb, int batch, const GemmShape&
   }
   if constexpr (sizeof(T) == 4) {
     if (split) {
+      const bool one = split == BEM_LU_BF16;
       if (split_tile == 0 && g.M >= 512 && g.Nc >= 256) {
         const int nx = (g.Nc + 127) / 128, ny = (g.M + 255) / 256, nt = nx * ny * batch;
         const int nwg = max_wgs > 0 ? min(nt, max_wgs) : nt;
-        hipLaunchKernelGGL((k_gemm_bf16x3<4, 4, 1>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
+        if (one)
+          hipLaunchKernelGGL((k_gemm_bf16x3<4, 4, 1, 1>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
+        else
+          hipLaunchKernelGGL((k_gemm_bf16x3<4, 4, 1, 3>), dim3(nwg), dim3(1024), 0, st, mb, g, skip, nx, ny, nt);
       } else {
         const int nx = (g.Nc + 63) / 64, ny = (g.M + 127) / 128, nt = nx * ny * batch;
-        hipLaunchKernelGGL((k_gemm_bf16x3<2, 2, 1>), dim3(nt), dim3(256), 0, st, mb, g, skip, nx, ny, nt);
+        if (one)
+          hipLaunchKernelGGL((k_gemm_bf16x3<2, 2, 1, 1>), dim3(nt), dim3(256), 0, st, mb, g, skip, nx, ny, nt);
+        else
+          hipLaunchKernelGGL((k_gemm_bf16x3<2, 2, 1, 3>), dim3(nt), dim3(256), 0, st, mb, g, skip, nx, ny, nt);
       }
       return hipGetLastError();
     }
@@ -1751,10 +1767,10 @@ static hipError_t solve_upper_rows(const MatBatch<T>& mb, int batch, int r0, int
 
 template <typename T>
 hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* perm, long long strideP, int* info,
-                           const int* skip, cplx<T>* W, hipStream_t st, const LuLookahead* la, bool split_bf16) {
+                           const int* skip, cplx<T>* W, hipStream_t st, const LuLookahead* la, int products) {
   const int tile = la ? la->split_tile : 0;
   const int N = mb.N;
-  const bool split = sizeof(T) == 4 && split_bf16;  // three bf16 products per FP32 product in the large updates
+  const int split = sizeof(T) == 4 ? products : BEM_LU_FP32;  // bf16 products in the large updates
   const long long strideW = lu_workspace_elems(N);
   const bool look = la && la->stream;
   hipLaunchKernelGGL(k_iota, dim3((N + 255) / 256, batch), dim3(256), 0, st, perm, strideP, N, info);
@@ -2008,9 +2024,9 @@ hipError_t solve_status_batch(const int* info, const cplx<double>* x, long long 
 }
 
 template hipError_t lu_factor_batch<float>(const MatBatch<float>&, int, int*, int*, long long, int*, const int*,
-                                           cplx<float>*, hipStream_t, const LuLookahead*, bool);
+                                           cplx<float>*, hipStream_t, const LuLookahead*, int);
 template hipError_t lu_factor_batch<double>(const MatBatch<double>&, int, int*, int*, long long, int*, const int*,
-                                            cplx<double>*, hipStream_t, const LuLookahead*, bool);
+                                            cplx<double>*, hipStream_t, const LuLookahead*, int);
 template hipError_t lu_solve_batch<float>(const MatBatch<float>&, int, const int*, long long, const cplx<double>*,
                                           long long, long long, cplx<float>*, long long, long long, int, const int*,
                                           const cplx<float>*, hipStream_t, int);

@@ -72,9 +72,12 @@ int bem_ctx_set_fanout(bem_ctx* ctx, int streams);
  * the reference factors in its working precision). BEM_LU_FP32 (default): FP32 MFMA. BEM_LU_SPLIT_BF16: each
  * FP32 product as three bf16 MFMA products (x = hi + lo, a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32
  * accumulation), relative error near 2^-16 per product, for factors that precondition the FP64 refinement of
- * bem_refine. The sub-panel updates, panels and triangular solves stay FP32. Ignored for BEM_PREC_FP64. */
+ * bem_refine. BEM_LU_BF16: one bf16 MFMA product per FP32 product (relative error near 2^-8 per product, about
+ * one more refinement step, a third of the MFMA work). The sub-panel updates, panels and triangular solves stay
+ * FP32. Ignored for BEM_PREC_FP64. */
 #define BEM_LU_FP32 0
 #define BEM_LU_SPLIT_BF16 1
+#define BEM_LU_BF16 2
 int bem_ctx_set_lu_products(bem_ctx* ctx, int mode);
 
 /* Arithmetic of the Duffy-rule sums of the vertex-sharing element pairs in BEM_PREC_MIXED assemblies, per context.

@@ -234,7 +234,8 @@ def test_lu_random(gpu_device, precision, n, batch):
 def test_lu_split_bf16_products_precondition_refinement(gpu_device, n, batch):
     """Split-bf16 trailing updates (three bf16 MFMA products per FP32 product): the factors of a matrix with a
     condition number in the tens have a backward error near 2^-16 sqrt(N), and FP64 refinement with them reaches
-    a relative residual of 1e-10 in as many steps as with FP32 factors (at most three here)."""
+    a relative residual of 1e-10 in as many steps as with FP32 factors (at most three here). One bf16 product
+    per FP32 product (BEM_LU_BF16): a backward error near 2^-8, refinement to 1e-10 in at most six steps."""
     from beat_hip import _lib
     from beat_hip.engine import PRECISIONS, _ptr
 
@@ -248,7 +249,7 @@ def test_lu_split_bf16_products_precondition_refinement(gpu_device, n, batch):
     lib, dev = gpu_device.lib, gpu_device.device
     results = {}
     try:
-        for mode in (_lib.BEM_LU_FP32, _lib.BEM_LU_SPLIT_BF16):
+        for mode in (_lib.BEM_LU_FP32, _lib.BEM_LU_SPLIT_BF16, _lib.BEM_LU_BF16):
             _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, mode), "products")
             with torch.cuda.stream(gpu_device.stream):
                 A = torch.zeros((batch, n, lda), dtype=torch.complex64, device=dev)
@@ -277,7 +278,7 @@ def test_lu_split_bf16_products_precondition_refinement(gpu_device, n, batch):
             bwd = [np.linalg.norm(B[m, 0] - M[m] @ X[m, 0]) / (np.linalg.norm(M[m], 2) * np.linalg.norm(X[m, 0]))
                    for m in range(batch)]
             hist = []
-            for it in range(4):
+            for it in range(6):
                 R = B - np.einsum("mij,mj->mi", M, X[:, 0])[:, None]
                 rel = max(np.linalg.norm(R[m]) / np.linalg.norm(B[m]) for m in range(batch))
                 hist.append(rel)
@@ -290,12 +291,15 @@ def test_lu_split_bf16_products_precondition_refinement(gpu_device, n, batch):
         _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, _lib.BEM_LU_FP32), "products")
     bwd32, hist32, f32 = results[_lib.BEM_LU_FP32]
     bwd16, hist16, f16 = results[_lib.BEM_LU_SPLIT_BF16]
-    assert not np.array_equal(f32, f16)  # the split products are in use
-    assert bwd32 < 1e-5 and bwd16 < 1e-4
+    bwd8, hist8, f8 = results[_lib.BEM_LU_BF16]
+    assert not np.array_equal(f32, f16) and not np.array_equal(f16, f8)  # the bf16 products are in use
+    assert bwd32 < 1e-5 and bwd16 < 1e-4 and bwd8 < 2e-2
     assert hist16[-1] < 1e-10 and len(hist16) <= 4 and len(hist16) <= len(hist32) + 1
+    assert hist8[-1] < 1e-10 and len(hist8) <= 6
 
 
-@pytest.mark.parametrize("precision,products", [("mixed", "fp32"), ("mixed", "split_bf16"), ("fp64", "fp32")])
+@pytest.mark.parametrize("precision,products", [("mixed", "fp32"), ("mixed", "split_bf16"), ("mixed", "bf16"),
+                                                ("fp64", "fp32")])
 def test_lu_streams_do_not_change_the_factors(gpu_device, precision, products):
     """Look-ahead (fan-out >= 2: a second stream for the trailing updates beyond the next block) computes every
     entry with the same operations in the same order as one stream: bitwise equal factors, pivots and
@@ -313,7 +317,7 @@ def test_lu_streams_do_not_change_the_factors(gpu_device, precision, products):
     prec = PRECISIONS[precision]
     lib = gpu_device.lib
     outs = []
-    mode = _lib.BEM_LU_SPLIT_BF16 if products == "split_bf16" else _lib.BEM_LU_FP32
+    mode = {"split_bf16": _lib.BEM_LU_SPLIT_BF16, "bf16": _lib.BEM_LU_BF16}.get(products, _lib.BEM_LU_FP32)
     try:
         _lib.check(lib.bem_ctx_set_lu_products(gpu_device.ctx, mode), "products")
         for fanout in (1, 4):
