@@ -19,7 +19,9 @@ rows of each P1 row's star (``vstar_ptr/vstar_row``).
 
 from __future__ import annotations
 
+import concurrent.futures
 import math
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -853,6 +855,19 @@ def plane_masks(pairs: SingularPairs, n_faces: int, n_transforms: int) -> np.nda
     return mask
 
 
+_HELPER = None
+_HELPER_LOCK = threading.Lock()
+
+
+def _helper():
+    """One process-wide helper thread for the independent parts of mesh preparation."""
+    global _HELPER
+    with _HELPER_LOCK:
+        if _HELPER is None:
+            _HELPER = concurrent.futures.ThreadPoolExecutor(max_workers=1, thread_name_prefix="bem-prepare")
+        return _HELPER
+
+
 def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True, symmetry: str = "off",
             regular_order: int = 4) -> PreparedMesh:
     """Device-ready tables. With ``symmetry`` x / xy the mesh is the fundamental domain (already snapped and
@@ -869,11 +884,15 @@ def prepare(mesh: BoundaryMesh, singular_order: int = 4, reorder: bool = True, s
     vinv = np.empty_like(vperm)
     vinv[vperm] = np.arange(len(vperm))
     work = BoundaryMesh(mesh.vertices[vperm], vinv[mesh.faces[fperm]], mesh.tags[fperm], mesh.mesh_ids[fperm])
+    signs = SYMMETRY_SIGNS[symmetry]
+    # the singular pairs (mostly the library's native builder, which releases the GIL) on a helper thread while
+    # this one forms the curls, rule points and slot tables
+    pending = _helper().submit(singular_pairs, work, singular_order, transforms=signs if len(signs) > 1 else None)
     curls = surface_curls(work.face_vertices, work.normals)
     qp = quadrature_points(work.face_vertices, regular_order)
     m11, m10 = mass_blocks(regular_order)
-    signs = SYMMETRY_SIGNS[symmetry]
-    pairs = singular_pairs(work, singular_order, transforms=signs if len(signs) > 1 else None)
-    return PreparedMesh(work, curls, qp, pairs, slot_tables(work), m11, m10, fperm, vperm, mesh, symmetry,
+    slots = slot_tables(work)
+    pairs = pending.result()
+    return PreparedMesh(work, curls, qp, pairs, slots, m11, m10, fperm, vperm, mesh, symmetry,
                         np.asarray(signs, dtype=np.int32), p1_orbit_weights(work, symmetry),
                         plane_masks(pairs, work.n_faces, len(signs)), int(regular_order), int(singular_order))

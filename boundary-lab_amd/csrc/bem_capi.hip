@@ -195,22 +195,40 @@ static int dev_alloc(bem_mesh* m, DevBuf& b, size_t bytes) {
   return BEM_OK;
 }
 
-template <typename U>
-static int upload(bem_mesh* m, U** dst, const U* src, size_t count) {
-  *dst = nullptr;
-  if (count == 0) return BEM_OK;
-  DevBuf b;
-  hipError_t e = hipMalloc(&b.p, count * sizeof(U));
-  if (e != hipSuccess) return fail(BEM_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-  b.bytes = count * sizeof(U);
-  m->owned.push_back(b);
-  if (src) {
-    e = hipMemcpy(b.p, src, count * sizeof(U), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hipfail(e, "hipMemcpy");
+// A mesh's tables go to the device as one allocation: each table is packed at a 256-byte aligned offset of a host
+// image, copied in one transfer, and the table pointers set into the allocation. One hipMalloc and one hipMemcpy
+// per mesh instead of one of each per table (37 tables: 4.7 ms per 6,000-triangle mesh, most of it per-call cost).
+struct TableArena {
+  struct Entry {
+    void** dst;
+    const void* src;
+    size_t bytes, off;
+  };
+  std::vector<Entry> entries;
+  size_t total = 0;
+  template <typename U>
+  void add(U** dst, const U* src, size_t count) {
+    *dst = nullptr;
+    if (count == 0) return;
+    entries.push_back({reinterpret_cast<void**>(dst), src, count * sizeof(U), total});
+    total += (count * sizeof(U) + 255) / 256 * 256;
   }
-  *dst = static_cast<U*>(b.p);
-  return BEM_OK;
-}
+  int commit(bem_mesh* m) {
+    if (total == 0) return BEM_OK;
+    std::vector<unsigned char> image(total, 0);
+    for (const Entry& e : entries)
+      if (e.src) std::memcpy(image.data() + e.off, e.src, e.bytes);
+    DevBuf b;
+    hipError_t err = hipMalloc(&b.p, total);
+    if (err != hipSuccess) return fail(BEM_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(err));
+    b.bytes = total;
+    m->owned.push_back(b);
+    err = hipMemcpy(b.p, image.data(), total, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return hipfail(err, "hipMemcpy");
+    for (const Entry& e : entries) *e.dst = static_cast<unsigned char*>(b.p) + e.off;
+    return BEM_OK;
+  }
+};
 
 extern "C" {
 
@@ -589,11 +607,12 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
   size_t total_rule_pts = 0;
   for (int r = 0; r < d->n_rules; ++r) total_rule_pts = std::max(total_rule_pts, (size_t)(d->rule_off[r] + d->rule_len[r]));
   size_t vs_nnz = (size_t)d->vstar_ptr[V];
-#define UP(dst, src, n)                      \
-  if ((rc = upload(m, &(dst), (src), (n)))) { \
-    bem_mesh_destroy(m);                     \
-    return rc;                               \
-  }
+  // (the arena copies at commit: every host table it is given lives until then)
+  TableArena arena;
+  std::vector<uint32_t> sb;
+  std::vector<int4> ent;
+  std::vector<int> by_rule;
+#define UP(dst, src, n) arena.add(&(dst), (src), (n))
   UP(m->rec32, r32.data(), r32.size());
   UP(m->rec64, r64.data(), r64.size());
   UP(m->src32, s32.data(), s32.size());
@@ -611,7 +630,7 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
   {
     // per chunk 512 bytes: incidence starts of the slots (bytes 0..192) and incidence codes (bytes 256..447), the
     // paired kernel's LDS copy of the chunk's slot table
-    std::vector<uint32_t> sb((size_t)d->n_chunks * 128, 0u);
+    sb.assign((size_t)d->n_chunks * 128, 0u);
     uint8_t* b = reinterpret_cast<uint8_t*>(sb.data());
     for (int ch = 0; ch < d->n_chunks; ++ch) {
       const uint16_t* st = d->slot_start + (size_t)ch * BEM_SLOT_STRIDE;
@@ -627,7 +646,7 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
   UP(m->vstar_ptr, d->vstar_ptr, (size_t)V + 1);
   UP(m->vstar_row, d->vstar_row, vs_nnz);
   {
-    std::vector<int4> ent(vs_nnz);
+    ent.resize(vs_nnz);
     for (int v = 0; v < V; ++v)
       for (int q = d->vstar_ptr[v]; q < d->vstar_ptr[v + 1]; ++q) {
         const int t = d->vstar_row[q] / 3;
@@ -644,11 +663,13 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
   UP(m->sing_rule, d->sing_rule, (size_t)d->n_pairs);
   UP(m->sing_ptr, d->sing_ptr, (size_t)F + 1);
   {
-    // pairs sorted by Duffy rule (stable), so a wave of the singular kernel shares one rule
-    std::vector<int> by_rule(d->n_pairs);
-    for (int i = 0; i < d->n_pairs; ++i) by_rule[i] = i;
-    std::stable_sort(by_rule.begin(), by_rule.end(),
-                     [d](int a, int b) { return d->sing_rule[a] < d->sing_rule[b]; });
+    // pairs sorted by Duffy rule (stable: a counting sort over the rules), so a wave of the singular kernel shares
+    // one rule
+    by_rule.resize(d->n_pairs);
+    std::vector<int> start((size_t)d->n_rules + 1, 0);
+    for (int i = 0; i < d->n_pairs; ++i) ++start[d->sing_rule[i] + 1];
+    for (int r = 0; r < d->n_rules; ++r) start[r + 1] += start[r];
+    for (int i = 0; i < d->n_pairs; ++i) by_rule[start[d->sing_rule[i]]++] = i;
     UP(m->sing_by_rule, by_rule.data(), (size_t)d->n_pairs);
   }
   UP(m->rule_off, d->rule_off, (size_t)d->n_rules);
@@ -660,6 +681,10 @@ static int mesh_create_impl(bem_ctx* c, const bem_mesh_desc* d, bem_mesh** out) 
   if (R > 1 && d->row_weights) UP(m->roww, d->row_weights, (size_t)V);
   if (R > 1 && d->plane_mask) UP(m->pmask, d->plane_mask, (size_t)F);
 #undef UP
+  if ((rc = arena.commit(m))) {
+    bem_mesh_destroy(m);
+    return rc;
+  }
   *out = m;
   return BEM_OK;
 }

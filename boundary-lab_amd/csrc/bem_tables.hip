@@ -82,14 +82,33 @@ int bem_host_vertex_pairs(int F, int V, const int* faces, long long capacity, in
         ++n;
       }
     }
-    std::vector<int> uniq(code_of);
-    std::sort(uniq.begin(), uniq.end());
-    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-    if ((int)uniq.size() > 64) return bem_fail_message(BEM_E_ARG, "more than 64 orientation codes");
-    for (size_t i = 0; i < uniq.size(); ++i) codes[i] = uniq[i];
-    *n_codes = (int)uniq.size();
-    for (long long p = 0; p < n; ++p)
-      rule[p] = (int)(std::lower_bound(uniq.begin(), uniq.end(), code_of[p]) - uniq.begin());
+    // rank of each code among the distinct codes, without sorting the pairs' codes: the codes map one to one and in
+    // order onto 91 slots (coincident; edge by (ia, ib, ja, jb); vertex by (ia, ja)), marked and prefix-counted
+    auto slot_of = [](int code) {
+      if (code == 100000) return 0;
+      if (code < 300000) {
+        const int d = code - 200000;
+        return 1 + ((d / 1000 * 3 + d / 100 % 10) * 3 + d / 10 % 10) * 3 + d % 10;
+      }
+      const int d = code - 300000;
+      return 82 + d / 1000 * 3 + d / 10 % 10;
+    };
+    int slot_code[91] = {0}, rank[91];
+    bool present[91] = {false};
+    for (long long q = 0; q < n; ++q) {
+      const int sl = slot_of(code_of[q]);
+      present[sl] = true;
+      slot_code[sl] = code_of[q];
+    }
+    int nu = 0;
+    for (int sl = 0; sl < 91; ++sl)
+      if (present[sl]) {
+        if (nu == 64) return bem_fail_message(BEM_E_ARG, "more than 64 orientation codes");
+        codes[nu] = slot_code[sl];
+        rank[sl] = nu++;
+      }
+    *n_codes = nu;
+    for (long long q = 0; q < n; ++q) rule[q] = rank[slot_of(code_of[q])];
     *n_pairs = n;
   } catch (const std::exception& ex) {
     return bem_fail_message(BEM_E_NOMEM, std::string("bem_host_vertex_pairs: ") + ex.what());
