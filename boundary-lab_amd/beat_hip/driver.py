@@ -14,6 +14,7 @@ Differences by design (documented in the result diagnostics):
 
 from __future__ import annotations
 
+import functools
 import math
 import os
 import time
@@ -351,7 +352,16 @@ def interpolate_complex_reference(values, angles_deg, ref_deg):
         ang, val = ang[:-1], val[:-1]
     ea = np.concatenate([ang - 360.0, ang, ang + 360.0])
     ev = np.concatenate([val, val, val])
-    for i in range(len(ea) - 1):
+    # the first bracket [ea[i], ea[i + 1]] holding ref in list order; with ascending angles it is found by bisection
+    # (the last i with ea[i] <= ref, stepped back while the bracket before also holds ref)
+    if np.all(np.diff(ea) > 0):
+        i = int(np.searchsorted(ea, ref, side="right")) - 1
+        while i > 0 and ea[i - 1] <= ref <= ea[i]:
+            i -= 1
+        candidates = range(max(i, 0), min(i + 1, len(ea) - 1))
+    else:  # angles out of order: every bracket, in list order
+        candidates = range(len(ea) - 1)
+    for i in candidates:
         lo, hi = ea[i], ea[i + 1]
         if lo <= ref <= hi:
             if math.isclose(lo, hi):
@@ -392,17 +402,27 @@ def synthesize_channel_basis(channel_names, hrows, vrows, srows, channels, freq,
             "weights": weights}
 
 
-def impedance_for_radiators(mesh, pressure, radiators, drives, symmetry_factor=1):
-    """solver.jl:607-631: F = 10 * sum p_avg A * factor; [Re(F/drive)/2, -Im(F/drive)/2]."""
+def radiator_selection(mesh, radiators):
+    """Per radiator, the vertex ids (F_r, 3) and areas of its faces (for impedance_for_radiators)."""
     out = []
-    f = mesh.faces
-    for r, drive in zip(radiators, drives):
+    for r in radiators:
+        sel = radiator_faces(mesh, r)
+        out.append((mesh.faces[sel], mesh.areas[sel]))
+    return out
+
+
+def impedance_for_radiators(mesh, pressure, radiators, drives, symmetry_factor=1, selection=None):
+    """solver.jl:607-631: F = 10 * sum p_avg A * factor; [Re(F/drive)/2, -Im(F/drive)/2]. ``selection``: the
+    radiators' faces from radiator_selection (formed here when not given)."""
+    out = []
+    if selection is None:
+        selection = radiator_selection(mesh, radiators)
+    for (fr, ar), drive in zip(selection, drives):
         if abs(drive) <= 0.0:
             out.append([float("nan"), float("nan")])
             continue
-        sel = radiator_faces(mesh, r)
-        pav = (pressure[f[sel, 0]] + pressure[f[sel, 1]] + pressure[f[sel, 2]]) / 3.0
-        force = 10.0 * np.sum(pav * mesh.areas[sel] * symmetry_factor)
+        pav = (pressure[fr[:, 0]] + pressure[fr[:, 1]] + pressure[fr[:, 2]]) / 3.0
+        force = 10.0 * np.sum(pav * ar * symmetry_factor)
         z = force / drive
         out.append([float(np.float32(z.real / 2)), float(np.float32(-z.imag / 2))])
     return out
@@ -440,6 +460,11 @@ class Job:
     fp64_check: bool
     regular_order: int = 4
     singular_order: int = 4
+
+    @functools.cached_property
+    def radiator_selection(self):
+        """The radiators' faces (vertex ids and areas), formed once per job."""
+        return radiator_selection(self.mesh, self.radiators)
 
     def prepare(self) -> G.PreparedMesh:
         """The device-ready tables of the job's mesh under its rules."""
@@ -524,7 +549,8 @@ def result_payload(job: Job, freq: float, p_vc: np.ndarray, field_oc: np.ndarray
                                                                                   1.0)
               for r in job.radiators]
     mixed = sum(p_vc[:, c] * syn["weights"][c] for c in range(C))
-    imp = impedance_for_radiators(job.mesh, mixed, job.radiators, drives, G.symmetry_reduction_factor(job.symmetry))
+    imp = impedance_for_radiators(job.mesh, mixed, job.radiators, drives, G.symmetry_reduction_factor(job.symmetry),
+                                  selection=job.radiator_selection)
     return {
         "freq_hz": float(np.float32(freq)),
         "horizontal_spl_norm_db": syn["horizontal_norm"].tolist(),

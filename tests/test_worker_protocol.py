@@ -358,3 +358,32 @@ def test_quadrature_orders_reach_the_mesh(mesh_file):
                                                                   "only for the BEAT CPU backend")):
         with pytest.raises(ValueError, match=msg):
             D.run_job(_request(mesh_file, **cfg), FakeEngine, lambda e: None, batch=1)
+
+
+def test_reference_interpolation_matches_the_bracket_scan():
+    """The flat-target reference value (solver.jl:499-544) takes the first bracket of the periodically extended
+    angles that holds the reference angle, scanning in list order; the driver bisects when the angles ascend. Both
+    give the same value, bit for bit, for ascending, random-ascending and unordered angle lists."""
+    import math
+
+    def scan(values, angles_deg, ref_deg):
+        ref = (float(ref_deg) + 180.0) % 360.0 - 180.0
+        ang, val = np.asarray(angles_deg, np.float64), np.asarray(values, np.complex128)
+        if math.isclose(ang[0], -180.0) and math.isclose(ang[-1], 180.0):
+            ang, val = ang[:-1], val[:-1]
+        ea, ev = np.concatenate([ang - 360.0, ang, ang + 360.0]), np.concatenate([val, val, val])
+        for i in range(len(ea) - 1):
+            if ea[i] <= ref <= ea[i + 1]:
+                if math.isclose(ea[i], ea[i + 1]):
+                    return ev[i]
+                frac = (ref - ea[i]) / (ea[i + 1] - ea[i])
+                return ev[i] * (1.0 - frac) + ev[i + 1] * frac
+        return ev[int(np.argmin(np.abs(ea - ref)))]
+
+    rng = np.random.default_rng(7)
+    for trial in range(600):
+        n = int(rng.integers(2, 80))
+        ang = [np.linspace(-180, 180, n), np.sort(rng.uniform(-180, 180, n)), rng.uniform(-180, 180, n)][trial % 3]
+        vals = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        for ref in (0.0, float(rng.uniform(-400, 400)), float(ang[int(rng.integers(0, n))]), 180.0, -180.0):
+            assert D.interpolate_complex_reference(vals, ang, ref) == scan(vals, ang, ref)
