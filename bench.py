@@ -370,14 +370,14 @@ def horn_request(path, precision, freqs):
 
 def run_request(req):
     """One request through driver.run_job on the worker's shared device; returns (wall s, result events). Every
-    event is JSON-encoded as the worker writes it."""
+    event is JSON-encoded as the worker writes it (worker.encode_event)."""
     from beat_hip import driver as D
     from beat_hip import worker as W
 
     lines, results = [], []
 
     def emit(ev):
-        lines.append(json.dumps(ev, separators=(",", ":")))
+        lines.append(W.encode_event(ev))
         if ev["type"] == "result":
             results.append(ev["result"])
         elif ev["type"] == "failed":

@@ -324,7 +324,7 @@ def surface_curls(face_vertices, normals):
 
 def quadrature_points(face_vertices, order: int = 4):
     """(F, n, 3) points of the regular rule of ``order``."""
-    return np.einsum("ql,flc->fqc", p1_basis(regular_rule(order)[0]), face_vertices)
+    return np.matmul(p1_basis(regular_rule(order)[0]), face_vertices)  # (n, 3) @ (F, 3, 3) -> (F, n, 3)
 
 
 # ---------------------------------------------------------------------------------------------

@@ -387,3 +387,34 @@ def test_reference_interpolation_matches_the_bracket_scan():
         vals = rng.standard_normal(n) + 1j * rng.standard_normal(n)
         for ref in (0.0, float(rng.uniform(-400, 400)), float(ang[int(rng.integers(0, n))]), 180.0, -180.0):
             assert D.interpolate_complex_reference(vals, ang, ref) == scan(vals, ang, ref)
+
+
+def test_event_encoding_round_trips_float32_rows():
+    """The worker writes lists of floats with nine significant digits: read back as float32 (as the consumer reads
+    the result rows) every value is the one json.dumps would give (an integral value may be written without a
+    decimal point, the same JSON number); non-finite values, strings, nesting and other types come out as
+    json.dumps writes them."""
+    import math
+
+    from beat_hip import worker as W
+
+    rng = np.random.default_rng(3)
+    rows = (rng.standard_normal(500) * 10.0 ** rng.integers(-30, 30, 500)).astype(np.float32).tolist()
+    ev = {"type": "result", "n": 3, "ok": True, "none": None, "name": 'a "q" \\ b',
+          "result": {"rows": rows, "nested": {"real": [rows[:7], [-0.0, 1e-45, 3.4e38]]},
+                     "bad": [1.0, float("nan"), float("-inf")], "empty": [], "ints": [1, 2, 3]}}
+    text = W.encode_event(ev)
+    assert "\n" not in text and len(text) < len(json.dumps(ev))
+    got, want = json.loads(text), json.loads(json.dumps(ev))
+
+    def same(a, b):
+        if isinstance(a, dict):
+            return a.keys() == b.keys() and all(same(a[k], b[k]) for k in a)
+        if isinstance(a, list):
+            return len(a) == len(b) and all(same(x, y) for x, y in zip(a, b))
+        if isinstance(b, float):  # a JSON number (an integral value may read back as int)
+            return isinstance(a, (int, float)) and ((math.isnan(a) and math.isnan(b)) or np.float32(a) == np.float32(b))
+        return a == b and type(a) is type(b)
+
+    assert same(got, want)
+    assert '"bad":[1.0,NaN,-Infinity]' in text
