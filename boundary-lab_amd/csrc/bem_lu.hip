@@ -314,10 +314,11 @@ __device__ __forceinline__ void split_bf16(float re, float im, unsigned& hi, uns
   lo = __builtin_bit_cast(unsigned, l);
 }
 
-// WY x WX waves of 64 rows x 32 complex columns each: a tile of 64 WY rows x 32 WX complex columns. The large
+// WY x WX waves of 64 rows x 32 TN complex columns each: a tile of 64 WY rows x 32 WX TN complex columns. The large
 // updates use 4 x 4 (256 x 128, 1024 threads, one workgroup per CU): every staged A and B element feeds twice
 // as many products as with 2 x 2 (128 x 64), which cut a 2,746-row K = 256 update over 40 matrices from 3.31
-// to 2.56 ms; updates of one 128-column block keep 2 x 2 (more workgroups).
+// to 2.56 ms; updates of one 128-column block keep 2 x 2 (more workgroups). NP: bf16 products per FP32 product,
+// 3 (the hi / lo split above) or 1 (the hi parts only: relative error near 2^-8 per product, no lo rows).
 template <int WY, int WX, int TN, int NP>
 __device__ __forceinline__ void gemm_bf16x3_tile(const MatBatch<float>& mb, const GemmShape& g, int mat, int tx,
                                                  int ty) {
