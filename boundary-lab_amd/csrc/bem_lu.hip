@@ -293,11 +293,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32. The dropped a_lo b_lo term and the rounding of
 // the lo parts leave a relative error near 2^-16 per product: more than FP32's, but the factors only precondition
 // the FP64 refinement that follows, which converges in the same number of steps (the LU of a Burton-Miller
-// matrix has a condition number of tens to hundreds). Each wave owns 64 rows x 32 complex columns of C as two
-// 32-row blocks with separate real and imaginary accumulators. Complex arithmetic as a real product with K doubled: the A operand is A itself ((re, im)
-// interleaved along k), the B operand for the real part is (br, -bi) per k and for the imaginary part (bi, br),
-// both formed from the staged (br, bi) pairs by a sign flip or a 16-bit rotation of each dword. A is staged
-// negated, split into hi and lo rows in LDS; B transposed (one row per column of C) and split the same way.
+// matrix has a condition number of tens to hundreds). Each wave owns 64 rows x 32 TN complex columns of C as
+// 32 x 32 blocks with separate real and imaginary accumulators. Complex arithmetic as a real product with K
+// doubled: the A operand is A itself ((re, im) interleaved along k), the B operand for the real part is
+// (br, -bi) per k and for the imaginary part (bi, br), both formed from the staged (br, bi) pairs by a sign flip
+// or a 16-bit rotation of each dword. A is staged negated, split into hi and lo rows in LDS; B transposed (one
+// row per column of C) and split the same way.
 // Operand layout of the 32x32x16 MFMA: lane l holds row (or column) l%32 and k = 8(l/32) .. 8(l/32)+7.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
