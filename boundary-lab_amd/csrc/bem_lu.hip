@@ -800,6 +800,13 @@ __device__ int compose_swaps(const int* piv, int c0, int nb, int* rows, int* src
 // permuted once per block by k_laswp.
 // ------------------------------------------------------------------------------------------------
 
+// A row's NBI sub-panel entries as 16-byte granules of CPG complex values.
+template <typename T>
+struct alignas(16) Granule {
+  static constexpr int CPG = 16 / (int)sizeof(cplx<T>);
+  cplx<T> c[CPG];
+};
+
 template <typename T, int NBI, int MR, int MRL = 0>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lu_panel(
     MatBatch<T> mb, int c0, int nb, int k0, int k1, int* ipiv, int* perm, long long strideP, int* info,
@@ -829,7 +836,27 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   // 16-column sub-panel cover 3,072 rows without spilling
   constexpr int MRR = MR - MRL;
   T Pr[MRR > 0 ? MRR : 1][NBI], Pi[MRR > 0 ? MRR : 1][NBI];
-  __shared__ T Xr[MRL > 0 ? MRL * NBI : 1][MRL > 0 ? 1024 : 1], Xi[MRL > 0 ? MRL * NBI : 1][MRL > 0 ? 1024 : 1];
+  // Rows move between HBM and the threads as whole 16-byte granules: the LPR lanes of a row chunk load (store)
+  // its consecutive granules, so one wave instruction covers RPI whole rows (one cache line touched per granule
+  // group instead of one per lane), and the chunks change hands between the lanes of their granules and the
+  // thread of their row through a wave-private LDS slice of 64 rows (granule g of row q at q LPR + (g ^ q % LPR):
+  // conflict-free in both directions). The slices share the LDS of the row sets Xr / Xi when there are any:
+  // the register row sets pass through them before the LDS row sets are written and after they are read.
+  using G = Granule<T>;
+  constexpr int CPG = G::CPG;
+  constexpr int LPR = NBI / CPG;  // granules per row chunk: 8, 4, 2 or 1
+  constexpr int RPI = 64 / LPR;   // rows per wave instruction
+  static_assert(NBI % CPG == 0 && (LPR & (LPR - 1)) == 0, "row chunks of whole granules");
+  constexpr int XN = MRL > 0 ? MRL * NBI * 1024 : 0;  // entries of each of Xr, Xi
+  constexpr int XBYTES = 2 * XN * (int)sizeof(T);
+  constexpr int SBYTES = LPR > 1 ? 16 * 64 * LPR * 16 : 0;
+  __shared__ G lds_g[(XBYTES > SBYTES ? XBYTES : SBYTES) / 16 + 1];
+  T (*Xr)[1024] = reinterpret_cast<T (*)[1024]>(lds_g);
+  T (*Xi)[1024] = reinterpret_cast<T (*)[1024]>(reinterpret_cast<T*>(lds_g) + XN);
+  G* stg = lds_g + w * 64 * LPR;
+  // granule transfers need 16-byte aligned row chunks
+  const bool gran = LPR > 1 && ((reinterpret_cast<uintptr_t>(A) | (uintptr_t)lda * sizeof(cplx<T>) |
+                                 (uintptr_t)c0 * sizeof(cplx<T>)) & 15) == 0;
   auto getP = [&](int m, int j) {
     if (m < MRR) return mk<T>(Pr[m][j], Pi[m][j]);
     return mk<T>(Xr[(m - MRR) * NBI + j][tid], Xi[(m - MRR) * NBI + j][tid]);
@@ -843,12 +870,72 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
       Xi[(m - MRR) * NBI + j][tid] = v.im;
     }
   };
+  // granule i of this lane in a wave instruction: row RPI i + lane / LPR of the wave's 64, granule lane % LPR
+  const int gq = lane / LPR, gg = lane % LPR;
+  auto load_granules = [&](int m, G (&gv)[LPR]) {
+    const int rb = c0 + 1024 * m + 64 * w;
 #pragma unroll
-  for (int m = 0; m < MR; ++m) {
-    const int r = c0 + tid + 1024 * m;
+    for (int i = 0; i < LPR; ++i) {
+      const int r = rb + RPI * i + gq;
+      // a granule that starts before column nb lies within the row (a partial one ends at column N < lda)
+      if (r < N && gg * CPG < nb) {
+        gv[i] = *reinterpret_cast<const G*>(A + (size_t)r * lda + c0 + gg * CPG);
+      } else {
 #pragma unroll
-    for (int j = 0; j < NBI; ++j)
-      setP(m, j, (r < N && j < nb) ? A[(size_t)r * lda + c0 + j] : mk<T>(0, 0));
+        for (int c = 0; c < CPG; ++c) gv[i].c[c] = mk<T>(0, 0);
+      }
+#pragma unroll
+      for (int c = 0; c < CPG; ++c)
+        if (gg * CPG + c >= nb) gv[i].c[c] = mk<T>(0, 0);
+    }
+  };
+  if (gran) {
+#pragma unroll
+    for (int m = 0; m < MRR; ++m) {
+      G gv[LPR];
+      load_granules(m, gv);
+#pragma unroll
+      for (int i = 0; i < LPR; ++i) {
+        const int q = RPI * i + gq;
+        stg[q * LPR + (gg ^ (q & (LPR - 1)))] = gv[i];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int g = 0; g < LPR; ++g) {
+        const G v = stg[lane * LPR + (g ^ (lane & (LPR - 1)))];
+#pragma unroll
+        for (int c = 0; c < CPG; ++c) setP(m, g * CPG + c, v.c[c]);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    if constexpr (MRL > 0) {
+      __syncthreads();  // every slice read before the LDS row sets overwrite them
+#pragma unroll
+      for (int m = MRR; m < MR; ++m) {
+        G gv[LPR];
+        load_granules(m, gv);
+#pragma unroll
+        for (int i = 0; i < LPR; ++i) {
+          const int t = 64 * w + RPI * i + gq;
+#pragma unroll
+          for (int c = 0; c < CPG; ++c) {
+            Xr[(m - MRR) * NBI + gg * CPG + c][t] = gv[i].c[c].re;
+            Xi[(m - MRR) * NBI + gg * CPG + c][t] = gv[i].c[c].im;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int r = c0 + tid + 1024 * m;
+#pragma unroll
+      for (int j = 0; j < NBI; ++j)
+        setP(m, j, (r < N && j < nb) ? A[(size_t)r * lda + c0 + j] : mk<T>(0, 0));
+    }
   }
   int zero_col = 0;  // first breakdown column (1-based), tracked by thread 0
 
@@ -939,13 +1026,66 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
       }
     }
   }
+  if (gran) {
+    // granule i of this lane to HBM: whole granules as one store, the partial last one by entries
+    auto store_granule = [&](int r, const G& v) {
+      cplx<T>* dst = A + (size_t)r * lda + c0 + gg * CPG;
+      if (gg * CPG + CPG <= nb) {
+        *reinterpret_cast<G*>(dst) = v;
+      } else {
 #pragma unroll
-  for (int m = 0; m < MR; ++m) {
-    const int r = c0 + tid + 1024 * m;
-    if (r < N)
+        for (int c = 0; c < CPG; ++c)
+          if (gg * CPG + c < nb) dst[c] = v.c[c];
+      }
+    };
+    if constexpr (MRL > 0) {
 #pragma unroll
-      for (int j = 0; j < NBI; ++j)
-        if (j < nb) A[(size_t)r * lda + c0 + j] = getP(m, j);
+      for (int m = MRR; m < MR; ++m) {
+        const int rb = c0 + 1024 * m + 64 * w;
+#pragma unroll
+        for (int i = 0; i < LPR; ++i) {
+          const int q = RPI * i + gq;
+          if (rb + q < N && gg * CPG < nb) {
+            G v;
+#pragma unroll
+            for (int c = 0; c < CPG; ++c)
+              v.c[c] = mk<T>(Xr[(m - MRR) * NBI + gg * CPG + c][64 * w + q], Xi[(m - MRR) * NBI + gg * CPG + c][64 * w + q]);
+            store_granule(rb + q, v);
+          }
+        }
+      }
+      __syncthreads();  // every LDS row set read before the slices overwrite them
+    }
+#pragma unroll
+    for (int m = 0; m < MRR; ++m) {
+#pragma unroll
+      for (int g = 0; g < LPR; ++g) {
+        G v;
+#pragma unroll
+        for (int c = 0; c < CPG; ++c) v.c[c] = getP(m, g * CPG + c);
+        stg[lane * LPR + (g ^ (lane & (LPR - 1)))] = v;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int rb = c0 + 1024 * m + 64 * w;
+#pragma unroll
+      for (int i = 0; i < LPR; ++i) {
+        const int q = RPI * i + gq;
+        const G v = stg[q * LPR + (gg ^ (q & (LPR - 1)))];
+        if (rb + q < N && gg * CPG < nb) store_granule(rb + q, v);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int r = c0 + tid + 1024 * m;
+      if (r < N)
+#pragma unroll
+        for (int j = 0; j < NBI; ++j)
+          if (j < nb) A[(size_t)r * lda + c0 + j] = getP(m, j);
+    }
   }
   __syncthreads();
   if (w == 0) {
@@ -958,8 +1098,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     if (lane == 0 && zero_col != 0 && info[mat] == 0) info[mat] = zero_col;
   }
   __syncthreads();
-  // interchanges on the other columns of the trailing block [k0, k1): one (column, row) pair per thread, all
-  // loads first
+  // interchanges on the other columns of the trailing block [k0, k1): one (row, column) pair per thread,
+  // consecutive threads on consecutive columns of a row, all loads first
   const int na = s_na;
   const int ncols = (k1 - k0) - nb;
   constexpr int PER = (2 * NBI * NBT + 1023) / 1024;
@@ -967,7 +1107,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = tid + 1024 * i;
-    const int q = e % (2 * NBI), cc = e / (2 * NBI);
+    const int q = e / NBT, cc = e % NBT;
     vr[i] = vi[i] = T(0);
     if (cc < ncols && q < na) {
       const int col = k0 + cc + (k0 + cc >= c0 ? nb : 0);
@@ -984,7 +1124,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = tid + 1024 * i;
-    const int q = e % (2 * NBI), cc = e / (2 * NBI);
+    const int q = e / NBT, cc = e % NBT;
     if (cc < ncols && q < na && s_src[q] != q) {
       const int col = k0 + cc + (k0 + cc >= c0 ? nb : 0);
       A[(size_t)s_rows[q] * lda + col] = mk<T>(vr[i], vi[i]);
