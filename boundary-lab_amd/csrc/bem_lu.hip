@@ -819,13 +819,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   const long long lda = mb.lda;
   const int N = mb.N;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  // per-column exchange area, double-buffered by column parity so that one barrier per column suffices:
-  // every wave publishes its largest magnitude, the first row holding it and that row's data; the owner of
-  // row c publishes row c
-  __shared__ T s_val[2][16];
-  __shared__ int s_row[2][16];
-  __shared__ cplx<T> s_cand[2][16][NBI];
-  __shared__ cplx<T> s_crow[2][NBI];
+  // per-column exchange area: every wave publishes its largest magnitude and the first row holding it; after
+  // the choice, the owner of the pivot row publishes that row and the owner of row c publishes row c
+  __shared__ T s_val[16];
+  __shared__ int s_row[16];
   __shared__ int s_pivs[NBI];
   __shared__ int s_rows[2 * NBI];
   __shared__ int s_src[2 * NBI];
@@ -854,6 +851,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
   T (*Xr)[1024] = reinterpret_cast<T (*)[1024]>(lds_g);
   T (*Xi)[1024] = reinterpret_cast<T (*)[1024]>(reinterpret_cast<T*>(lds_g) + XN);
   G* stg = lds_g + w * 64 * LPR;
+  __shared__ G s_prow_g[LPR], s_crow_g[LPR];
+  cplx<T>* const s_prow = reinterpret_cast<cplx<T>*>(s_prow_g);
+  cplx<T>* const s_crow = reinterpret_cast<cplx<T>*>(s_crow_g);
   // granule transfers need 16-byte aligned row chunks
   const bool gran = LPR > 1 && ((reinterpret_cast<uintptr_t>(A) | (uintptr_t)lda * sizeof(cplx<T>) |
                                  (uintptr_t)c0 * sizeof(cplx<T>)) & 15) == 0;
@@ -938,11 +938,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     }
   }
   int zero_col = 0;  // first breakdown column (1-based), tracked by thread 0
+  // one row of this thread to an exchange row, as whole granules
+  auto publish = [&](int m, G* dst) {
+#pragma unroll
+    for (int g = 0; g < LPR; ++g) {
+      G v;
+#pragma unroll
+      for (int c = 0; c < CPG; ++c) v.c[c] = getP(m, g * CPG + c);
+      dst[g] = v;
+    }
+  };
 
 #pragma unroll
   for (int jj = 0; jj < NBI; ++jj) {
     if (jj < nb) {
-      const int buf = jj & 1;
       const int c = c0 + jj;
       // this thread's best candidate: strict > keeps the lowest of its rows among equal magnitudes
       T bv = T(-1);
@@ -972,55 +981,72 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
         }
       }
       if (lane == 0) {
-        s_val[buf][w] = wmax;
-        s_row[buf][w] = wrow;
-      }
-#pragma unroll
-      for (int m = 0; m < MR; ++m) {
-        const int r = c0 + tid + 1024 * m;
-        if (r == wrow)
-#pragma unroll
-          for (int j = 0; j < NBI; ++j) s_cand[buf][w][j] = getP(m, j);
-        if (r == c)
-#pragma unroll
-          for (int j = 0; j < NBI; ++j) s_crow[buf][j] = getP(m, j);
+        s_val[w] = wmax;
+        s_row[w] = wrow;
       }
       __syncthreads();
       // across the 16 waves: largest magnitude, then the lowest row holding it (LAPACK i?amax order)
-      const T vq = s_val[buf][lane & 15];
-      const int rq = s_row[buf][lane & 15];
+      const T vq = s_val[lane & 15];
+      const int rq = s_row[lane & 15];
       const T gmax = row16_max_dpp(vq);
       const int pick = row16_min_dpp(vq == gmax && gmax >= T(0) ? rq : 0x7fffffff);
       // no candidate at all cannot happen for c < N; keep row c then (the pivot test records a breakdown)
       const int pv = gmax >= T(0) ? pick : c;
-      const int gw = ((pv - c0) & 1023) >> 6;
-      const cplx<T>* prow = pv == c ? s_crow[buf] : s_cand[buf][gw];  // pivot row (LDS broadcast reads)
+#pragma unroll
+      for (int m = 0; m < MR; ++m) {
+        const int r = c0 + tid + 1024 * m;
+        if (r == pv) publish(m, s_prow_g);
+        if (r == c && pv != c) publish(m, s_crow_g);
+      }
+      __syncthreads();
       if (pv != c) {
 #pragma unroll
         for (int m = 0; m < MR; ++m) {
           const int r = c0 + tid + 1024 * m;
           if (r == c)
 #pragma unroll
-            for (int j = 0; j < NBI; ++j) setP(m, j, prow[j]);
+            for (int j = 0; j < NBI; ++j) setP(m, j, s_prow[j]);
           if (r == pv)
 #pragma unroll
-            for (int j = 0; j < NBI; ++j) setP(m, j, s_crow[buf][j]);
+            for (int j = 0; j < NBI; ++j) setP(m, j, s_crow[j]);
         }
       }
       if (tid == 0) s_pivs[jj] = pv;
-      const cplx<T> pivot = prow[jj];
+      const cplx<T> pivot = s_prow[jj];
       if (!(cabs1(pivot) > T(0))) {  // zero or NaN pivot: breakdown, no elimination
         if (tid == 0 && zero_col == 0) zero_col = c + 1;
       } else {
         const cplx<T> inv = cdiv(mk<T>(1, 0), pivot);
+        // multipliers of the rows below c; rows c0 .. c (U rows of this panel: thread jj and below of the
+        // first row set) take multiplier 0 and keep column jj. Rows at or beyond N compute on zeros and are
+        // neither searched nor stored.
+        auto multiplier = [&](int m) {
+          cplx<T> l = cmul(getP(m, jj), inv);
+          if (m == 0 && c0 + tid <= c) l = mk<T>(0, 0);
+          else setP(m, jj, l);
+          return l;
+        };
+        if constexpr (NBI >= 8) {
+          // few row sets: all multipliers first, then each pivot-row entry read once for all of them
+          cplx<T> l[MR];
 #pragma unroll
-        for (int m = 0; m < MR; ++m) {
-          const int r = c0 + tid + 1024 * m;
-          if (r > c && r < N) {
-            const cplx<T> l = cmul(getP(m, jj), inv);
-            setP(m, jj, l);
+          for (int m = 0; m < MR; ++m) l[m] = multiplier(m);
 #pragma unroll
-            for (int j2 = jj + 1; j2 < NBI; ++j2) setP(m, j2, cfms(getP(m, j2), l, prow[j2]));
+          for (int j2 = jj + 1; j2 < NBI; ++j2) {
+            const cplx<T> pj = s_prow[j2];
+#pragma unroll
+            for (int m = 0; m < MR; ++m) setP(m, j2, cfms(getP(m, j2), l[m], pj));
+          }
+        } else {
+          // many row sets, few columns: the pivot row in registers, one multiplier at a time
+          cplx<T> pr[NBI];
+#pragma unroll
+          for (int j2 = jj + 1; j2 < NBI; ++j2) pr[j2] = s_prow[j2];
+#pragma unroll
+          for (int m = 0; m < MR; ++m) {
+            const cplx<T> l = multiplier(m);
+#pragma unroll
+            for (int j2 = jj + 1; j2 < NBI; ++j2) setP(m, j2, cfms(getP(m, j2), l, pr[j2]));
           }
         }
       }
