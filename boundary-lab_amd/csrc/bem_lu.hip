@@ -754,42 +754,48 @@ __device__ __forceinline__ int row16_min_dpp(int v) {
   return __builtin_amdgcn_readlane(v, 15);
 }
 
-// Composite of the row interchanges (c0+jj, piv[jj]), jj < nb, applied in order (LAPACK laswp order),
+// Composite of the row interchanges (c0+jj, piv[jj]), jj < nb <= 64, applied in order (LAPACK laswp order),
 // restricted to the rows it touches: afterwards row rows[q] holds the former row rows[src[q]]. Run by one
-// wave; rows [c0, c0+nb) come first, pivot rows beyond are appended in first-seen order. Returns the count.
+// wave; rows [c0, c0+nb) come first, pivot rows beyond are appended in first-seen order. The slots (at most
+// 2 nb) live in registers, slot q in lane q % 64 of the low or high half, so each step is a few scalar, cross-lane
+// and select operations; rows and src are written to LDS at the end. Returns the count.
 __device__ int compose_swaps(const int* piv, int c0, int nb, int* rows, int* src, int lane) {
-  for (int q = lane; q < nb; q += 64) {
-    rows[q] = c0 + q;
-    src[q] = q;
-  }
+  int rlo = lane < nb ? c0 + lane : -1, rhi = -1;  // rows of slots lane and 64 + lane (-1: not in use)
+  int slo = lane, shi = 64 + lane;                  // src of slots lane and 64 + lane
+  const int pl = lane < nb ? piv[lane] : 0;
   int na = nb;
   for (int jj = 0; jj < nb; ++jj) {
-    const int pv = piv[jj];
+    const int pv = __builtin_amdgcn_readlane(pl, jj);
     int b;
     if (pv < c0 + nb) {
       b = pv - c0;
     } else {
-      // look among the appended rows (at most nb of them; 64 lanes per pass)
-      b = -1;
-      for (int q0 = nb; q0 < na && b < 0; q0 += 64) {
-        const int q = q0 + lane;
-        const unsigned long long hit = __ballot(q < na && rows[q] == pv);
-        if (hit) b = q0 + __builtin_ctzll(hit);
-      }
-      if (b < 0) {
+      const unsigned long long hlo = __ballot(rlo == pv), hhi = __ballot(rhi == pv);
+      if (hlo) {
+        b = (int)__builtin_ctzll(hlo);
+      } else if (hhi) {
+        b = 64 + (int)__builtin_ctzll(hhi);
+      } else {
         b = na++;
-        if (lane == 0) {
-          rows[b] = pv;
-          src[b] = b;
-        }
+        if (b < 64) rlo = lane == b ? pv : rlo;
+        else rhi = lane == b - 64 ? pv : rhi;
       }
     }
-    if (lane == 0 && b != jj) {
-      const int t = src[jj];
-      src[jj] = src[b];
-      src[b] = t;
+    if (b != jj) {
+      const int sj = __builtin_amdgcn_readlane(slo, jj);
+      const int sb = b < 64 ? __builtin_amdgcn_readlane(slo, b) : __builtin_amdgcn_readlane(shi, b - 64);
+      slo = lane == jj ? sb : slo;
+      if (b < 64) slo = lane == b ? sj : slo;
+      else shi = lane == b - 64 ? sj : shi;
     }
-    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane < na) {
+    rows[lane] = rlo;
+    src[lane] = slo;
+  }
+  if (64 + lane < na) {
+    rows[64 + lane] = rhi;
+    src[64 + lane] = shi;
   }
   return na;
 }
