@@ -816,7 +816,7 @@ struct alignas(16) Granule {
 template <typename T, int NBI, int MR, int MRL = 0>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lu_panel(
     MatBatch<T> mb, int c0, int nb, int k0, int k1, int* ipiv, int* perm, long long strideP, int* info,
-    const int* skip) {
+    const int* skip, cplx<T>* W, long long strideW, int h0, int nbn) {
   const int mat = blockIdx.x;
   if (skip && skip[mat]) return;
   cplx<T>* A = mb.A + (size_t)mat * mb.strideA;
@@ -1058,6 +1058,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
       }
     }
   }
+  // the sub-panel's unit-lower diagonal block (rows c0 .. c0 + nb are threads 0 .. nb - 1 of the first row set)
+  __shared__ cplx<T> s_ld[NBI][NBI + 1];
+  if (tid < nb)
+#pragma unroll
+    for (int j = 0; j < NBI; ++j) s_ld[tid][j] = getP(0, j);
   if (gran) {
     // granule i of this lane to HBM: whole granules as one store, the partial last one by entries
     auto store_granule = [&](int r, const G& v) {
@@ -1160,6 +1165,65 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) vo
     if (cc < ncols && q < na && s_src[q] != q) {
       const int col = k0 + cc + (k0 + cc >= c0 ? nb : 0);
       A[(size_t)s_rows[q] * lda + col] = mk<T>(vr[i], vi[i]);
+    }
+  }
+
+  // The half's unit-lower diagonal block L[h0:h1, h0:h1] inverted one sub-panel at a time (the lower W block of
+  // the half): with I = L[h0:c0, h0:c0]^-1 from the earlier sub-panels, this sub-panel's rows of the inverse are
+  // [-Ic Lb I, Ic] (Ic: its diagonal block inverted, Lb: its rows in the earlier columns, both final after the
+  // interchanges above). The U rows of the next sub-panel [c0 + nb, c0 + nb + nbn) of the half then follow as one
+  // product with the extended inverse: U' = I' A[h0:c0+nb, c0+nb : c0+nb+nbn]. The first sub-panel of a half
+  // writes the whole block (identity beyond its rows, as for a partial last half).
+  __syncthreads();  // interchanges stored; the LDS of the row sets is free
+  {
+    const int K = c0 - h0, Kn = K + nb;
+    cplx<T>* Wb = W + (size_t)mat * strideW + (size_t)(h0 / 64) * 2 * 4096;
+    cplx<T>* sm = reinterpret_cast<cplx<T>*>(lds_g);
+    cplx<T>(*Wi)[65] = reinterpret_cast<cplx<T>(*)[65]>(sm);               // 64 x 64 inverse rows so far
+    cplx<T>(*Lb)[65] = reinterpret_cast<cplx<T>(*)[65]>(sm + 64 * 65);     // nb x K
+    cplx<T>(*T1)[65] = reinterpret_cast<cplx<T>(*)[65]>(sm + 80 * 65);     // Lb I, nb x K
+    cplx<T>(*X)[17] = reinterpret_cast<cplx<T>(*)[17]>(sm + 96 * 65);      // next sub-panel's columns, Kn x nbn
+    for (int e = tid; e < K * K; e += 1024) Wi[e / K][e % K] = Wb[(size_t)(e / K) * 64 + e % K];
+    for (int e = tid; e < nb * K; e += 1024) Lb[e / K][e % K] = A[(size_t)(c0 + e / K) * lda + h0 + e % K];
+    for (int e = tid; e < Kn * nbn; e += 1024) X[e / nbn][e % nbn] = A[(size_t)(h0 + e / nbn) * lda + c0 + nb + e % nbn];
+    __syncthreads();
+    if (tid < nb) {  // Ic column tid by substitution, and the zero upper part of the column
+      const int j = tid;
+      for (int i = 0; i < j; ++i) Wi[K + i][K + j] = mk<T>(0, 0);
+      Wi[K + j][K + j] = mk<T>(1, 0);
+      for (int i = j + 1; i < nb; ++i) {
+        cplx<T> acc = mk<T>(0, 0);
+        for (int k = j; k < i; ++k) acc = cfma(acc, s_ld[i][k], Wi[K + k][K + j]);
+        Wi[K + i][K + j] = mk<T>(-acc.re, -acc.im);
+      }
+    }
+    for (int e = tid; e < nb * K; e += 1024) {  // T1 = Lb I (I lower triangular)
+      const int i = e / K, j = e % K;
+      cplx<T> acc = mk<T>(0, 0);
+      for (int k = j; k < K; ++k) acc = cfma(acc, Lb[i][k], Wi[k][j]);
+      T1[i][j] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < nb * K; e += 1024) {  // -Ic T1 (Ic lower triangular)
+      const int i = e / K, j = e % K;
+      cplx<T> acc = mk<T>(0, 0);
+      for (int k = 0; k <= i; ++k) acc = cfma(acc, Wi[K + i][K + k], T1[k][j]);
+      Wi[K + i][j] = mk<T>(-acc.re, -acc.im);
+    }
+    __syncthreads();
+    if (K == 0) {
+      for (int e = tid; e < 4096; e += 1024) {
+        const int i = e >> 6, j = e & 63;
+        Wb[e] = (i < nb && j < nb) ? Wi[i][j] : mk<T>(i == j ? T(1) : T(0), T(0));
+      }
+    } else {
+      for (int e = tid; e < nb * Kn; e += 1024) Wb[(size_t)(K + e / Kn) * 64 + e % Kn] = Wi[K + e / Kn][e % Kn];
+    }
+    if (tid < Kn * nbn) {  // U' = I' X, I' lower triangular
+      const int i = tid / nbn, j = tid % nbn;
+      cplx<T> acc = mk<T>(0, 0);
+      for (int k = 0; k <= i; ++k) acc = cfma(acc, Wi[i][k], X[k][j]);
+      A[(size_t)(h0 + i) * lda + c0 + nb + j] = acc;
     }
   }
 }
@@ -1821,12 +1885,13 @@ __global__ __launch_bounds__(256) void k_growth(const double* umax, const double
 
 template <typename T>
 static hipError_t launch_panel(const MatBatch<T>& mb, int batch, int c0, int nb, int k0, int k1, int* ipiv,
-                               int* perm, long long strideP, int* info, const int* skip, hipStream_t st) {
+                               int* perm, long long strideP, int* info, const int* skip, hipStream_t st,
+                               cplx<T>* W, long long strideW, int h0, int nbn) {
   const int H = mb.N - c0;
   const int mr = (H + 1023) / 1024;
 #define PANEL(NBI_, MR_, ...)                                                                                 \
   hipLaunchKernelGGL((k_lu_panel<T, NBI_, MR_, ##__VA_ARGS__>), dim3(batch), dim3(1024), 0, st, mb, c0, nb, k0, \
-                     k1, ipiv, perm, strideP, info, skip)
+                     k1, ipiv, perm, strideP, info, skip, W, strideW, h0, nbn)
   // (NBI, MR, MRL): the sub-panel's register rows stay within 128 VGPRs/lane without spilling; the last MRL row
   // sets per thread live in 128 KB of LDS, which doubles the width wherever more than two row sets are needed
   if constexpr (sizeof(T) == 4) {
@@ -1960,20 +2025,20 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
         GemmShape g = {h0, k0, k0, h0, h0, h0, N - h0, h1 - h0, h0 - k0};
         CHK(launch_gemm<T>(mb, batch, g, skip, st, split, 0, tile));
       }
+      // sub-panels: each one's U rows [h0, c0) come from the panel before it in the half, which also extends
+      // the half's inverted L block (final once the half is done: later interchanges touch rows >= h1 only)
       int c0 = h0;
       while (c0 < h1) {
         const int nb = min(panel_width<T>(N, c0), h1 - c0);
         if (c0 > h0) {
-          CHK(launch_trsm<T>(mb, batch, h0, c0 - h0, c0, nb, skip, st));
           GemmShape g = {c0, h0, h0, c0, c0, c0, N - c0, nb, c0 - h0};
           CHK(launch_gemm<T>(mb, batch, g, skip, st));
         }
-        CHK(launch_panel<T>(mb, batch, c0, nb, k0, k1, ipiv, perm, strideP, info, skip, st));
-        c0 += nb;
+        const int cn = c0 + nb;
+        const int nbn = cn < h1 ? min(panel_width<T>(N, cn), h1 - cn) : 0;
+        CHK(launch_panel<T>(mb, batch, c0, nb, k0, k1, ipiv, perm, strideP, info, skip, st, W, strideW, h0, nbn));
+        c0 = cn;
       }
-      // the diagonal block of this half is final (later interchanges touch rows >= h1 only): invert its L
-      hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(1, batch), dim3(256), 0, st, mb, W, strideW, h0 / NBO, skip);
-      CHK(hipGetLastError());
     }
     return hipSuccess;
   };
