@@ -1284,18 +1284,19 @@ __global__ __launch_bounds__(256) void k_laswp(MatBatch<T> mb, const int* ipiv, 
 // ------------------------------------------------------------------------------------------------
 // Inverses of the 64 x 64 diagonal blocks of L (unit lower) and U (upper), stored per matrix in the LU
 // workspace W[mat][block][0 = L, 1 = U][64][64] (identity-padded past N). They turn the U12 solves into MFMA
-// products and the diagonal steps of the triangular sweeps into matrix-vector products.
-// One workgroup per (block, matrix), block-recursive: the four 16 x 16 diagonal sub-blocks are inverted by
+// products and the diagonal steps of the triangular sweeps into matrix-vector products. The L blocks are built
+// by the sub-panel kernel during the factorization, one row block per sub-panel; the U blocks here, after it:
+// one workgroup per (block, matrix), block-recursive: the four 16 x 16 diagonal sub-blocks are inverted by
 // substitution (one thread per column), then pairs are joined at sizes 16 and 32 with
-//   [A 0; C B]^-1 = [A^-1 0; -B^-1 C A^-1  B^-1]     and     [A C; 0 B]^-1 = [A^-1  -A^-1 C B^-1; 0 B^-1].
+//   [A C; 0 B]^-1 = [A^-1  -A^-1 C B^-1; 0 B^-1].
 // ------------------------------------------------------------------------------------------------
 
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, long long strideW, int kb_first,
-                                                 const int* skip) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_tri_inv_upper(MatBatch<T> mb, cplx<T>* W, long long strideW,
+                                                       const int* skip) {
   const int mat = blockIdx.y;
   if (skip && skip[mat]) return;
-  const int kb = kb_first + blockIdx.x;
+  const int kb = blockIdx.x;
   const int N = mb.N;
   const int r0 = kb * 64, nr = min(64, N - r0);
   if (nr <= 0) return;
@@ -1308,11 +1309,7 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
   for (int e = tid; e < 64 * 64; e += 256) {
     const int i = e >> 6, j = e & 63;
     cplx<T> v = mk<T>(i == j ? T(1) : T(0), T(0));
-    if (i < nr && j < nr) {
-      if (LOWER ? (j < i) : (j >= i)) v = A[(size_t)(r0 + i) * lda + r0 + j];
-      else if (LOWER && i == j) v = mk<T>(T(1), T(0));
-      else v = mk<T>(T(0), T(0));
-    }
+    if (i < nr && j < nr) v = j >= i ? A[(size_t)(r0 + i) * lda + r0 + j] : mk<T>(T(0), T(0));
     Bk[i][j] = v;
     X[i][j] = mk<T>(T(0), T(0));
   }
@@ -1320,21 +1317,12 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
   // 16 x 16 diagonal sub-blocks by substitution, one thread per column
   if (tid < 64) {
     const int o = tid & 48, j = tid & 15;
-    if (LOWER) {
-      X[o + j][o + j] = mk<T>(T(1), T(0));
-      for (int i = j + 1; i < 16; ++i) {
-        cplx<T> acc = mk<T>(T(0), T(0));
-        for (int k = j; k < i; ++k) acc = cfma(acc, Bk[o + i][o + k], X[o + k][o + j]);
-        X[o + i][o + j] = mk<T>(-acc.re, -acc.im);
-      }
-    } else {
-      X[o + j][o + j] = cdiv(mk<T>(T(1), T(0)), Bk[o + j][o + j]);
-      for (int i = j - 1; i >= 0; --i) {
-        cplx<T> acc = mk<T>(T(0), T(0));
-        for (int k = i + 1; k <= j; ++k) acc = cfma(acc, Bk[o + i][o + k], X[o + k][o + j]);
-        const cplx<T> q = cdiv(acc, Bk[o + i][o + i]);
-        X[o + i][o + j] = mk<T>(-q.re, -q.im);
-      }
+    X[o + j][o + j] = cdiv(mk<T>(T(1), T(0)), Bk[o + j][o + j]);
+    for (int i = j - 1; i >= 0; --i) {
+      cplx<T> acc = mk<T>(T(0), T(0));
+      for (int k = i + 1; k <= j; ++k) acc = cfma(acc, Bk[o + i][o + k], X[o + k][o + j]);
+      const cplx<T> q = cdiv(acc, Bk[o + i][o + i]);
+      X[o + i][o + j] = mk<T>(-q.re, -q.im);
     }
   }
   __syncthreads();
@@ -1342,7 +1330,7 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
   for (int h = 16; h < 64; h <<= 1) {
     const int E = 64 * h / 2;  // output entries over all pairs: (64 / 2h) pairs x h x h
     cplx<T> t[4];
-    // T = C A^-1 (lower: C = L21) or C B^-1 (upper: C = U12), h x h per pair
+    // T = C B^-1 (C = U12), h x h per pair
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u;
@@ -1350,9 +1338,7 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
       if (e < E) {
         const int p = e / (h * h), r = (e / h) % h, c = e % h, o = 2 * h * p;
         cplx<T> acc = mk<T>(T(0), T(0));
-        for (int k = 0; k < h; ++k)
-          acc = LOWER ? cfma(acc, Bk[o + h + r][o + k], X[o + k][o + c])
-                      : cfma(acc, Bk[o + r][o + h + k], X[o + h + k][o + h + c]);
+        for (int k = 0; k < h; ++k) acc = cfma(acc, Bk[o + r][o + h + k], X[o + h + k][o + h + c]);
         t[u] = acc;
       }
     }
@@ -1362,32 +1348,24 @@ __global__ __launch_bounds__(256) void k_tri_inv(MatBatch<T> mb, cplx<T>* W, lon
       const int e = tid + 256 * u;
       if (e < E) {
         const int p = e / (h * h), r = (e / h) % h, c = e % h, o = 2 * h * p;
-        if (LOWER)
-          Bk[o + h + r][o + c] = t[u];
-        else
-          Bk[o + r][o + h + c] = t[u];
+        Bk[o + r][o + h + c] = t[u];
       }
     }
     __syncthreads();
-    // lower: X21 = -B^-1 T ; upper: X12 = -A^-1 T
+    // X12 = -A^-1 T
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u;
       if (e < E) {
         const int p = e / (h * h), r = (e / h) % h, c = e % h, o = 2 * h * p;
         cplx<T> acc = mk<T>(T(0), T(0));
-        for (int k = 0; k < h; ++k)
-          acc = LOWER ? cfma(acc, X[o + h + r][o + h + k], Bk[o + h + k][o + c])
-                      : cfma(acc, X[o + r][o + k], Bk[o + k][o + h + c]);
-        if (LOWER)
-          X[o + h + r][o + c] = mk<T>(-acc.re, -acc.im);
-        else
-          X[o + r][o + h + c] = mk<T>(-acc.re, -acc.im);
+        for (int k = 0; k < h; ++k) acc = cfma(acc, X[o + r][o + k], Bk[o + k][o + h + c]);
+        X[o + r][o + h + c] = mk<T>(-acc.re, -acc.im);
       }
     }
     __syncthreads();
   }
-  cplx<T>* Wb = W + (size_t)mat * strideW + ((size_t)kb * 2 + (LOWER ? 0 : 1)) * 4096;
+  cplx<T>* Wb = W + (size_t)mat * strideW + ((size_t)kb * 2 + 1) * 4096;
   for (int e = tid; e < 64 * 64; e += 256) Wb[e] = X[e >> 6][e & 63];
 }
 
@@ -2143,7 +2121,7 @@ hipError_t lu_factor_batch(const MatBatch<T>& mb, int batch, int* ipiv, int* per
   }
   if (pending) CHK(hipStreamWaitEvent(st, la->done, 0));
   // inverses of the diagonal blocks of U for the backward sweeps
-  hipLaunchKernelGGL((k_tri_inv<T, false>), dim3((N + 63) / 64, batch), dim3(256), 0, st, mb, W, strideW, 0, skip);
+  hipLaunchKernelGGL(k_tri_inv_upper<T>, dim3((N + 63) / 64, batch), dim3(256), 0, st, mb, W, strideW, skip);
   CHK(hipGetLastError());
   return hipSuccess;
 }
