@@ -104,6 +104,7 @@ SIGNATURES = {
     "bem_mesh_validate": (_i, [C.POINTER(MeshDesc)]),
     "bem_host_vertex_pairs": (_i, [_i, _i, _ip, _ll, _ip, _ip, _ip, _ip, _ip, C.POINTER(_ll)]),
     "bem_host_slot_tables": (_i, [_i, _i, _ip, _ip, _i, _ip, _u16p, _u16p, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip]),
+    "bem_host_format_f32_json": (_ll, [_p, _ll, _p, _ll]),
     "bem_mesh_create": (_i, [_p, C.POINTER(MeshDesc), C.POINTER(_p)]),
     "bem_mesh_live_graphs": (_i, [_p, C.POINTER(_i)]),
     "bem_assemble_rhs": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _ll, _ll]),

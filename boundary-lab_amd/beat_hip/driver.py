@@ -23,6 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import geometry as G
+from .wire import Float32Row
 
 # ---------------------------------------------------------------------------------------------
 # config parsing (solver.jl:46-250)
@@ -336,8 +337,7 @@ def pressure_to_spl(p):
 
 
 def complex_rows_to_wire(rows):
-    return {"real": [np.asarray(r.real, np.float32).tolist() for r in rows],
-            "imag": [np.asarray(r.imag, np.float32).tolist() for r in rows]}
+    return {"real": [Float32Row(r.real) for r in rows], "imag": [Float32Row(r.imag) for r in rows]}
 
 
 def interpolate_complex_reference(values, angles_deg, ref_deg):
@@ -553,12 +553,12 @@ def result_payload(job: Job, freq: float, p_vc: np.ndarray, field_oc: np.ndarray
                                   selection=job.radiator_selection)
     return {
         "freq_hz": float(np.float32(freq)),
-        "horizontal_spl_norm_db": syn["horizontal_norm"].tolist(),
-        "vertical_spl_norm_db": syn["vertical_norm"].tolist(),
+        "horizontal_spl_norm_db": Float32Row(syn["horizontal_norm"]),
+        "vertical_spl_norm_db": Float32Row(syn["vertical_norm"]),
         "impedance": imp,
-        "horizontal_spl_db": syn["horizontal_spl"].tolist(),
-        "vertical_spl_db": syn["vertical_spl"].tolist(),
-        "sphere_spl_norm_db": None if syn["sphere_norm"] is None else syn["sphere_norm"].tolist(),
+        "horizontal_spl_db": Float32Row(syn["horizontal_spl"]),
+        "vertical_spl_db": Float32Row(syn["vertical_spl"]),
+        "sphere_spl_norm_db": None if syn["sphere_norm"] is None else Float32Row(syn["sphere_norm"]),
         "channel_names": list(job.channel_names),
         "horizontal_pressure": complex_rows_to_wire(hrows),
         "vertical_pressure": complex_rows_to_wire(vrows),

@@ -17,30 +17,55 @@ import os
 import sys
 import threading
 
+from .wire import Float32Row, row_json
+
 _DEVICE = None
 _DEVICE_LOCK = threading.Lock()
 
 
 _FLOAT_LIST_FORMATS: dict[int, str] = {}
+_KEYS: dict[str, str] = {}
+_FLOAT_ONLY = {float}
+_FINITE_BOUND = 1.7976931348623157e308
 
 
 def encode_event(event) -> str:
     """One event as compact JSON text. Lists of floats (the Float32 result rows) are written with nine significant
     digits, which return every float32 value exactly (the reference's worker writes its Float32 arrays in their
     float32 form; the consumer reads them back as float32); everything else, and any list holding a non-finite
-    value, as json.dumps writes it. About twice as fast as json.dumps on a result event."""
+    value, as json.dumps writes it. Result rows (Float32Row) go through the library's row formatter (the same
+    digits); dict keys are encoded once per process, scalars by their repr where that is their JSON text."""
     t = type(event)
     if t is dict:
-        return "{" + ",".join(json.dumps(k) + ":" + encode_event(v) for k, v in event.items()) + "}"
+        parts = []
+        for k, v in event.items():
+            kt = _KEYS.get(k) if type(k) is str else None
+            if kt is None:
+                if type(k) is not str:
+                    return json.dumps(event, separators=(",", ":"))
+                kt = _KEYS.setdefault(k, json.dumps(k))
+            parts.append(kt + ":" + encode_event(v))
+        return "{" + ",".join(parts) + "}"
+    if t is Float32Row:
+        text = row_json(event)
+        if text is not None:
+            return text
+        t = list
     if t is list:
-        if event and all(type(v) is float for v in event):
+        if event and set(map(type, event)) == _FLOAT_ONLY:
             fmt = _FLOAT_LIST_FORMATS.get(len(event))
             if fmt is None:
                 fmt = _FLOAT_LIST_FORMATS.setdefault(len(event), "[" + ",".join(["%.9g"] * len(event)) + "]")
             text = fmt % tuple(event)
             if "n" not in text:  # no nan / inf
                 return text
-        return "[" + ",".join(encode_event(v) for v in event) + "]"
+        return "[" + ",".join([encode_event(v) for v in event]) + "]"
+    if t is float and -_FINITE_BOUND <= event <= _FINITE_BOUND:
+        return float.__repr__(event)
+    if t is int:
+        return int.__repr__(event)
+    if event is None:
+        return "null"
     return json.dumps(event, separators=(",", ":"))
 
 

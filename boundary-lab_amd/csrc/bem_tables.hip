@@ -3,6 +3,7 @@
 // a request's host work.
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <queue>
@@ -247,6 +248,97 @@ int bem_host_slot_tables(int F, int V, const int* faces, const int* order, int p
     return bem_fail_message(BEM_E_NOMEM, std::string("bem_host_slot_tables: ") + ex.what());
   }
   return BEM_OK;
+}
+
+// Float32 rows as JSON text. A value is written with nine significant digits, which is enough for every float32
+// value to read back exactly even when the last digit is off by one (the nine-digit quantum is at most 1e-8 of
+// the value, under half the float32 spacing of 6e-8): the digits come from one double product with a power of
+// ten, rounded to the nearest integer, and are printed in %g style (exponent form below 1e-4 and from 1e9).
+namespace {
+double pow10_table(int k) {  // 10^k for |k| <= 60, from exact powers (10^22 is the largest exact double)
+  static const double kPos[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                                  1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+  double r = 1.0;
+  int a = k < 0 ? -k : k;
+  while (a > 22) {
+    r *= 1e22;
+    a -= 22;
+  }
+  r *= kPos[a];
+  return k < 0 ? 1.0 / r : r;
+}
+
+char* put_float32(char* o, float x) {
+  if (x == 0.0f) {
+    if (std::signbit(x)) *o++ = '-';
+    *o++ = '0';
+    return o;
+  }
+  double a = (double)x;
+  if (a < 0) {
+    *o++ = '-';
+    a = -a;
+  }
+  int e = (int)std::floor(std::log10(a));
+  long long m = std::llround(a * pow10_table(8 - e));
+  if (m >= 1000000000LL) {
+    ++e;
+    m = std::llround(a * pow10_table(8 - e));
+  } else if (m < 100000000LL) {
+    --e;
+    m = std::llround(a * pow10_table(8 - e));
+  }
+  if (m >= 1000000000LL) {  // rounded up to the next decade
+    m /= 10;
+    ++e;
+  }
+  char d[9];
+  for (int i = 8; i >= 0; --i) {
+    d[i] = (char)('0' + m % 10);
+    m /= 10;
+  }
+  int nd = 9;
+  while (nd > 1 && d[nd - 1] == '0') --nd;
+  if (e < -4 || e >= 9) {
+    *o++ = d[0];
+    if (nd > 1) {
+      *o++ = '.';
+      for (int i = 1; i < nd; ++i) *o++ = d[i];
+    }
+    *o++ = 'e';
+    *o++ = e < 0 ? '-' : '+';
+    const int ae = e < 0 ? -e : e;
+    if (ae >= 100) *o++ = (char)('0' + ae / 100);
+    *o++ = (char)('0' + (ae / 10) % 10);
+    *o++ = (char)('0' + ae % 10);
+  } else if (e >= 0) {
+    for (int i = 0; i <= e; ++i) *o++ = i < nd ? d[i] : '0';
+    if (nd > e + 1) {
+      *o++ = '.';
+      for (int i = e + 1; i < nd; ++i) *o++ = d[i];
+    }
+  } else {
+    *o++ = '0';
+    *o++ = '.';
+    for (int i = 0; i < -e - 1; ++i) *o++ = '0';
+    for (int i = 0; i < nd; ++i) *o++ = d[i];
+  }
+  return o;
+}
+}  // namespace
+
+long long bem_host_format_f32_json(const float* v, long long n, char* out, long long cap) {
+  if (n < 0 || (n > 0 && !v) || !out) return -1;
+  if (cap < 16 * n + 2) return -1;
+  char* o = out;
+  *o++ = '[';
+  for (long long i = 0; i < n; ++i) {
+    if (!std::isfinite(v[i])) return -2;
+    if (i) *o++ = ',';
+    o = put_float32(o, v[i]);
+  }
+  *o++ = ']';
+  return (long long)(o - out);
 }
 
 }  // extern "C"

@@ -205,6 +205,12 @@ int bem_host_slot_tables(int n_faces, int n_vertices, const int* faces, const in
                          int* slot_count, uint16_t* slot_start, uint16_t* slot_inc, int* slot_vertex, int* slot_pend,
                          int* slot_prev, int* elem_pos, int* chunk_start, int* vertex_last, int* vstar_ptr,
                          int* vstar_row);
+/* n float32 values as a JSON array (HOST arrays) in `out` (cap >= 16 n + 2 bytes, no terminator), each with nine
+ * significant digits, so that every value reads back as the same float32. Returns the length written, -1 for bad
+ * arguments or too small a buffer, -2 if a value is not finite (the caller writes those rows itself). The worker's
+ * result rows; replaces the JSON writer of the reference's worker for its Float32 arrays (emit_event,
+ * julia_local/solver.jl:12-17). */
+long long bem_host_format_f32_json(const float* v, long long n, char* out, long long cap);
 
 /* Upload a prepared mesh: SoA geometry, locality/slot tables, singular pairs and Duffy rules, field sources.
  * Replaces build_rocm_regular_assembly_cache (JS/BeatEngineRocmRegular.jl:134-205),

@@ -14,7 +14,7 @@ HEADER = ROOT / "include" / "bemhip.h"
 def declared_functions():
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*int\s+(bem_[a-z_]+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long long)\s+(bem_[a-z0-9_]+)\s*\(", text, flags=re.M)))
 
 
 @pytest.fixture(scope="module")

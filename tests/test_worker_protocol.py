@@ -418,3 +418,29 @@ def test_event_encoding_round_trips_float32_rows():
 
     assert same(got, want)
     assert '"bad":[1.0,NaN,-Infinity]' in text
+
+
+def test_result_rows_formatted_by_the_library_read_back_exactly():
+    """Result rows (Float32Row) are written by the library's row formatter: every finite float32 value, from the
+    smallest subnormal to the largest normal, at decade edges and rounding carries, reads back as the same
+    float32 (parsed as float, then rounded to float32; -0 reads back as 0 as with json.dumps' -0.0 -> "-0.0"
+    parsed by float); non-finite rows fall back to the Python writer."""
+    from beat_hip import worker as W
+    from beat_hip.wire import Float32Row, row_json
+
+    rng = np.random.default_rng(11)
+    bits = rng.integers(0, 0x7F800000, 200000, dtype=np.int64).astype(np.uint32)  # every finite magnitude
+    vals = bits.view(np.float32) * np.where(rng.random(bits.size) < 0.5, -1, 1).astype(np.float32)
+    edges = np.array([1e-45, 1.1754942e-38, 1.17549435e-38, 3.4028235e38, 1.0, 0.1, 1e-4, 9.99999975e-05,
+                      999999999.0, 1e9, 123456789.0, 0.99999994, 9.9999999e8, 1e-5, 5e-324 * 0, 0.0], np.float32)
+    vals = np.concatenate([vals, edges, -edges])
+    text = W.encode_event(Float32Row(vals))
+    assert text == row_json(Float32Row(vals))  # the library path was taken
+    back = np.array([float(t) for t in text[1:-1].split(",")]).astype(np.float32)
+    assert np.array_equal(back, vals)  # -0 == 0 numerically
+    assert np.array_equal(np.array(json.loads(text), np.float64).astype(np.float32), vals)  # valid JSON
+    # the same digits as the Python writer for lists of floats
+    py = ("[" + ",".join(["%.9g"] * len(vals)) + "]") % tuple(vals.astype(np.float64).tolist())
+    assert np.array_equal(np.array([float(t) for t in py[1:-1].split(",")]).astype(np.float32), back)
+    odd = Float32Row(np.array([1.0, np.nan, -np.inf], np.float32))
+    assert row_json(odd) is None and W.encode_event(odd) == "[1.0,NaN,-Infinity]"
