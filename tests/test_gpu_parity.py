@@ -171,14 +171,17 @@ def test_solve_and_field_match_oracle(gpu_device, sphere, oracle_sphere, precisi
 # blocks (190, 200) and the register-panel thresholds (1500, 2100), and the panels with row sets in LDS (3,500 to
 # 6,500 rows: three to seven row sets per thread); batches of 8 at N <= 4096 take the one-workgroup-per-matrix
 # solve, the others the blocked multi-workgroup sweeps; above 12,288 rows the panels hold 24 row sets per thread
-# (checked by the backward error only: a CPU solve of that size takes minutes)
-@pytest.mark.parametrize("precision,n,batch", [("mixed", 40, 2), ("fp64", 100, 1), ("mixed", 100, 2),
-                                               ("mixed", 190, 2), ("fp64", 200, 2), ("mixed", 200, 3),
-                                               ("fp64", 1500, 1), ("mixed", 2100, 2), ("fp64", 3500, 1),
-                                               ("mixed", 4500, 1), ("mixed", 6500, 1), ("fp64", 6500, 1),
-                                               ("fp64", 300, 8), ("mixed", 1100, 8), ("fp64", 12400, 1),
-                                               ("mixed", 13000, 1)])
-def test_lu_random(gpu_device, precision, n, batch):
+# (checked by the backward error only: a CPU solve of that size takes minutes). Leading dimensions: padded to 16
+# (the sub-panels move whole 16-byte granules) except where given: an odd one (entry by entry), and odd orders whose
+# last sub-panel ends inside a granule; 3,073 and 6,145 rows change the sub-panel width inside the first half
+# (8 to 16 and 4 to 8 columns), so the U rows and inverse blocks of one sub-panel feed a sub-panel of another width.
+@pytest.mark.parametrize("precision,n,batch,lda", [
+    ("mixed", 40, 2, None), ("fp64", 100, 1, None), ("mixed", 100, 2, None), ("mixed", 190, 2, None),
+    ("fp64", 200, 2, None), ("mixed", 200, 3, None), ("fp64", 1500, 1, None), ("mixed", 2100, 2, None),
+    ("fp64", 3500, 1, None), ("mixed", 4500, 1, None), ("mixed", 6500, 1, None), ("fp64", 6500, 1, None),
+    ("fp64", 300, 8, None), ("mixed", 1100, 8, None), ("fp64", 12400, 1, None), ("mixed", 13000, 1, None),
+    ("mixed", 1001, 2, 1001), ("fp64", 517, 2, 517), ("mixed", 3073, 1, 3074), ("mixed", 6145, 1, 6146)])
+def test_lu_random(gpu_device, precision, n, batch, lda):
     import ctypes as C
 
     from beat_hip import _lib
@@ -186,7 +189,7 @@ def test_lu_random(gpu_device, precision, n, batch):
 
     torch = gpu_device.torch
     rng = np.random.default_rng(n + batch)
-    lda = ((n + 15) // 16) * 16
+    lda = ((n + 15) // 16) * 16 if lda is None else lda
     M = rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))
     M += 4 * np.sqrt(n) * np.eye(n)[None] * (rng.standard_normal(batch)[:, None, None] > 5)  # usually no boost
     B = rng.standard_normal((batch, 2, n)) + 1j * rng.standard_normal((batch, 2, n))
