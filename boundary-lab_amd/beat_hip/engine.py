@@ -196,7 +196,6 @@ def _lda(n):
 class BatchResult:
     freqs: np.ndarray  # (nf,)
     pressure: np.ndarray  # (nf, V, C) complex128
-    q: np.ndarray  # (nf, F, C) complex128 (i rho omega * unit drives)
     field: np.ndarray  # (nf, n_obs, C) complex128 or None
     residual: np.ndarray  # (nf,) relative FP64 residual of the solved system (max over channels)
     growth: np.ndarray  # (nf,) pivot growth
@@ -206,6 +205,15 @@ class BatchResult:
     t_solve: float
     t_field: float
     flags: np.ndarray | None = None  # (nf,) solve-status bits (see Batch.status), None if not checked
+    # (nf, F, C) complex128 Neumann data i rho omega * unit drives, or a zero-argument callable that returns it
+    # (downloaded from the device on first use of .q: the result payloads do not need it)
+    q_source: object = None
+
+    @property
+    def q(self) -> np.ndarray:
+        if callable(self.q_source):
+            self.q_source = self.q_source()
+        return self.q_source
 
 
 # solve-status bits (bem_solve_status)
@@ -449,12 +457,19 @@ class Batch:
         prep = self.dmesh.prep
         # back to the caller's vertex / face numbering
         p = prep.unpermute_vertices(self.x[:n, :, :N].cpu().numpy().transpose(0, 2, 1), axis=1)
-        qi = self.q[:n].cpu().numpy().transpose(0, 2, 1)
-        q = np.empty_like(qi)
-        q[:, prep.fperm, :] = qi
+        with self.dev.torch.cuda.stream(self.dev.stream):  # ordered before the batch's next run reuses its buffer
+            qdev = self.q[:n].clone()
+
+        def neumann():
+            with self.dev.torch.cuda.stream(self.dev.stream):
+                qi = qdev.cpu().numpy().transpose(0, 2, 1)
+            q = np.empty_like(qi)
+            q[:, prep.fperm, :] = qi
+            return q
+
         field = self.field[:n].cpu().numpy().transpose(0, 2, 1) if self.n_obs else None
         return BatchResult(
-            freqs=freqs, pressure=p, q=q, field=field,
+            freqs=freqs, pressure=p, q_source=neumann, field=field,
             residual=self.rel[:, :n].cpu().numpy().max(axis=0), growth=self.growth[:n].cpu().numpy(),
             iters=self.iters[:, :n].cpu().numpy().max(axis=0) if self.precision == "mixed" else np.zeros(n, np.int32),
             info=self.info[:n].cpu().numpy(), t_assembly=t_as, t_solve=t_so, t_field=t_fi, flags=flags,

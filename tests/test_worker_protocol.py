@@ -33,7 +33,7 @@ class FakeBatch:
         field = np.stack([np.exp(1j * kk * r)[:, None] / r[:, None] * np.arange(1, C + 1)[None, :] for kk in k])
         p = np.ones((n, V, C), complex)
         q = (1j * self.rho * om)[:, None, None] * self.qunit[None]
-        return BatchResult(freqs=np.asarray(freqs), pressure=p, q=q, field=field, residual=np.full(n, 1e-12),
+        return BatchResult(freqs=np.asarray(freqs), pressure=p, q_source=q, field=field, residual=np.full(n, 1e-12),
                            growth=np.ones(n), iters=np.ones(n, int), info=np.zeros(n, int), t_assembly=0.1,
                            t_solve=0.2, t_field=0.05)
 
