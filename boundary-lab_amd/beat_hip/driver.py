@@ -340,18 +340,16 @@ def complex_rows_to_wire(rows):
     return {"real": [Float32Row(r.real) for r in rows], "imag": [Float32Row(r.imag) for r in rows]}
 
 
-def interpolate_complex_reference(values, angles_deg, ref_deg):
-    if len(values) == 0:
-        return 0j
-    if len(values) == 1:
-        return complex(values[0])
+@functools.lru_cache(maxsize=64)
+def _reference_bracket(angles: tuple, n_values: int, ref_deg: float):
+    """Where the flat-target reference angle falls among the polar angles: (i, j, frac) with the interpolated value
+    v[i] (1 - frac) + v[j] frac (indices into the values), computed once per angle set."""
     ref = (float(ref_deg) + 180.0) % 360.0 - 180.0
-    ang = np.asarray(angles_deg, dtype=np.float64)
-    val = np.asarray(values, dtype=np.complex128)
+    ang = np.asarray(angles, dtype=np.float64)
+    n = n_values
     if math.isclose(ang[0], -180.0) and math.isclose(ang[-1], 180.0):
-        ang, val = ang[:-1], val[:-1]
+        ang, n = ang[:-1], n - 1
     ea = np.concatenate([ang - 360.0, ang, ang + 360.0])
-    ev = np.concatenate([val, val, val])
     # the first bracket [ea[i], ea[i + 1]] holding ref in list order; with ascending angles it is found by bisection
     # (the last i with ea[i] <= ref, stepped back while the bracket before also holds ref)
     if np.all(np.diff(ea) > 0):
@@ -365,10 +363,23 @@ def interpolate_complex_reference(values, angles_deg, ref_deg):
         lo, hi = ea[i], ea[i + 1]
         if lo <= ref <= hi:
             if math.isclose(lo, hi):
-                return ev[i]
-            frac = (ref - lo) / (hi - lo)
-            return ev[i] * (1.0 - frac) + ev[i + 1] * frac
-    return ev[int(np.argmin(np.abs(ea - ref)))]
+                return i % n, i % n, None
+            return i % n, (i + 1) % n, (ref - lo) / (hi - lo)
+    k = int(np.argmin(np.abs(ea - ref))) % n
+    return k, k, None
+
+
+def interpolate_complex_reference(values, angles_deg, ref_deg):
+    if len(values) == 0:
+        return 0j
+    if len(values) == 1:
+        return complex(values[0])
+    key = angles_deg if type(angles_deg) is tuple else tuple(np.asarray(angles_deg, np.float64).tolist())
+    i, j, frac = _reference_bracket(key, len(values), float(ref_deg))
+    val = np.asarray(values, dtype=np.complex128)
+    if frac is None:
+        return val[i]
+    return val[i] * (1.0 - frac) + val[j] * frac
 
 
 def flat_target_corrections(channel_names, horizontal_rows, angles_deg, ref_deg, flat_target):
