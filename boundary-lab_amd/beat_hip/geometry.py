@@ -67,7 +67,12 @@ class BoundaryMesh:
 def read_gmsh22(path, scale: float = 1.0, translation=(0.0, 0.0, 0.0)):
     """Gmsh 2.2 ASCII: nodes and type-2 (triangle) elements; tag = first tag token (BeatEngineCore.jl:193-234)."""
     with open(path, "r", encoding="utf-8") as handle:
-        lines = handle.read().splitlines()
+        text = handle.read()
+    fast = _gmsh22_text(text, scale)
+    if fast is not None:
+        v, faces, tags = fast
+        return v + np.asarray(translation, dtype=np.float64), faces, tags
+    lines = text.splitlines()
     stripped = [line.strip() for line in lines]
     try:
         ns, ne = stripped.index("$Nodes"), stripped.index("$EndNodes")
@@ -114,6 +119,62 @@ def _uniform_block(lines, dtype):
     return flat.reshape(len(lines), k)
 
 
+def _marker_line(text, name):
+    """(start, end) of the first line of `text` that is `$name` between blanks, or None."""
+    key = "$" + name
+    i = text.find(key)
+    while i >= 0:
+        ls = text.rfind("\n", 0, i) + 1
+        le = text.find("\n", i)
+        le = len(text) if le < 0 else le
+        if not text[ls:i].strip() and not text[i + len(key) : le].strip():
+            return ls, le
+        i = text.find(key, i + 1)
+    return None
+
+
+def _text_block(text, begin, end):
+    """The data lines of a section (after its marker and count lines, before its end marker) as one string and
+    their number, or None."""
+    mb, me = _marker_line(text, begin), _marker_line(text, end)
+    if mb is None or me is None:
+        return None
+    count_end = text.find("\n", mb[1] + 1)  # the count line follows the marker line
+    if count_end < 0:
+        return None
+    if count_end >= me[0]:
+        return "", 0
+    block = text[count_end + 1 : me[0]]
+    return block, block.count("\n")
+
+
+def _uniform_text(block, nlines, dtype):
+    """_uniform_block for a section given as one string of nlines lines."""
+    if nlines == 0:
+        return None
+    first = block[: block.find("\n")]
+    k = len(first.split())
+    flat = np.fromstring(block, dtype=dtype, sep=" ")  # noqa: NPY201 (text mode)
+    if k == 0 or flat.size != k * nlines:
+        return None
+    return flat.reshape(nlines, k)
+
+
+def _gmsh22_text(text, scale):
+    """The vectorised read of _gmsh22_blocks on the whole file text (no split into lines); None when a section is
+    missing or irregular (the caller then reads line by line and reports what is wrong)."""
+    import warnings
+
+    nb, eb = _text_block(text, "Nodes", "EndNodes"), _text_block(text, "Elements", "EndElements")
+    if nb is None or eb is None:
+        return None
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", DeprecationWarning)
+        nodes = _uniform_text(nb[0], nb[1], np.float64)
+        elems = _uniform_text(eb[0], eb[1], np.int64) if eb[1] else np.zeros((0, 8), np.int64)
+    return _gmsh22_arrays(nodes, elems, scale)
+
+
 def _gmsh22_blocks(node_lines, elem_lines, scale):
     """Vectorised read of the Nodes and Elements blocks (the same semantics as the line loop of read_gmsh22:
     triangles only, element nodes are the last three tokens, tag is the first tag token). None when a block has
@@ -124,6 +185,10 @@ def _gmsh22_blocks(node_lines, elem_lines, scale):
         warnings.simplefilter("ignore", DeprecationWarning)
         nodes = _uniform_block(node_lines, np.float64)
         elems = _uniform_block(elem_lines, np.int64) if elem_lines else np.zeros((0, 8), np.int64)
+    return _gmsh22_arrays(nodes, elems, scale)
+
+
+def _gmsh22_arrays(nodes, elems, scale):
     if nodes is None or elems is None or nodes.shape[1] != 4 or elems.shape[1] < 8:
         return None
     ids = nodes[:, 0].astype(np.int64)

@@ -355,8 +355,9 @@ def test_paired_strips_reproduce_dense_assembly(pend_slots):
 
 
 def test_gmsh_block_reader_matches_line_reader(tmp_path):
-    # the vectorised reader and the line loop give the same mesh; files it cannot take as uniform blocks (mixed
-    # element lines, sparse node ids with unknown references) fall back to the loop with the same result
+    # the vectorised readers (on the whole text, on the list of lines) and the line loop give the same mesh; files
+    # they cannot take as uniform blocks (mixed element lines, sparse node ids with unknown references) fall back to
+    # the loop with the same result
     from beat_hip import geometry as G
     from beat_hip import synthetic as syn
 
@@ -372,15 +373,22 @@ def test_gmsh_block_reader_matches_line_reader(tmp_path):
     vm, fm, tm = G.read_gmsh22(mixed)
     assert fm.tolist() == [[0, 2, 1], [0, 1, 3], [1, 2, 3]] and tm.tolist() == [3, 4, 5]
     assert np.allclose(vm[4], [9, 9, 9])
-    orig = G._gmsh22_blocks
+    # Windows line ends and blanks around the section markers: the same mesh
+    crlf = tmp_path / "crlf.msh"
+    crlf.write_bytes(p.read_text().replace("$Nodes\n", "  $Nodes \n").replace("\n", "\r\n").encode())
+    fast_crlf = G.read_gmsh22(crlf, 0.001, (0.1, 0.0, -0.2))
+    orig_text, orig_blocks = G._gmsh22_text, G._gmsh22_blocks
     try:
-        G._gmsh22_blocks = lambda *a: None
+        G._gmsh22_text = lambda *a: None  # the line-list block reader
+        blocks = G.read_gmsh22(p, 0.001, (0.1, 0.0, -0.2))
+        G._gmsh22_blocks = lambda *a: None  # the line loop
         loop = G.read_gmsh22(p, 0.001, (0.1, 0.0, -0.2))
         loop_mixed = G.read_gmsh22(mixed)
+        loop_crlf = G.read_gmsh22(crlf, 0.001, (0.1, 0.0, -0.2))
     finally:
-        G._gmsh22_blocks = orig
-    for a, b in zip(fast, loop):
-        assert np.array_equal(a, b)
+        G._gmsh22_text, G._gmsh22_blocks = orig_text, orig_blocks
+    for a, b, c, d, e in zip(fast, blocks, loop, fast_crlf, loop_crlf):
+        assert np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d) and np.array_equal(a, e)
     for a, b in zip((vm, fm, tm), loop_mixed):
         assert np.array_equal(a, b)
 
