@@ -14,10 +14,11 @@
 // to the matrix beyond a pair of blocks at once (K = 256 on MFMA). Each block is two halves of NBO = 64: the
 // second half is brought up to date with the first by an MFMA product (K = 64) before it is factored.
 // Left-looking inside a half: sub-panels of NBI columns are factored by one 1024-thread workgroup that holds
-// the sub-panel in registers (k_lu_panel), after a top TRSM and an update with the sub-panels of the half
-// already done. Row interchanges are applied inside
-// the trailing block by the panel kernel and outside it by k_laswp once the block is factored. The 64 x 64
-// diagonal blocks of L and U are inverted into a workspace; the U12 solves and the diagonal steps of the
+// the sub-panel in registers (k_lu_panel), after an update with the sub-panels of the half already done; each
+// sub-panel kernel also extends the half's inverted L block by its rows and writes the next sub-panel's U rows
+// (one product with that inverse). Row interchanges are applied inside the trailing block by the panel kernel
+// and outside it by k_laswp once the block is factored. The 64 x 64 diagonal blocks of L (built by the sub-panels)
+// and U (inverted after the factorization) live in a workspace; the U12 solves and the diagonal steps of the
 // triangular sweeps multiply by those inverses.
 //
 // Complex GEMM on real MFMA: C (M x 2Nc reals, interleaved) = A (M x 2K reals) * B'' where B'' is the 2x2 real
