@@ -804,7 +804,8 @@ __device__ int compose_swaps(const int* piv, int c0, int nb, int* rows, int* src
 // ------------------------------------------------------------------------------------------------
 // Sub-panel factorization in registers: rows [c0, N), columns [c0, c0+nb), nb <= NBI. The row interchanges
 // are applied to the other columns of the trailing block [k0, k1) here; columns outside the block are
-// permuted once per block by k_laswp.
+// permuted once per block by k_laswp. Then the sub-panel's rows of the half's inverted L block (half [h0, h0+64),
+// in W) and the U rows [h0, c0+nb) of the next sub-panel's nbn columns.
 // ------------------------------------------------------------------------------------------------
 
 // A row's NBI sub-panel entries as 16-byte granules of CPG complex values.
