@@ -43,7 +43,9 @@ def estimate_footprint(n_vertices: int, n_faces: int, n_pairs: int, batch: int, 
     lu_work = batch * ((n_vertices + 63) // 64) * 2 * 4096 * cb  # inverted diagonal blocks
     matrices = batch * 2 * n_vertices * lda * cb + lu_work
     vectors = batch * 8 * n_ch * lda * 16
-    strip = min(batch, 4) * 3 * n_faces * strip_len * cb * (2 if precision == "mixed" else 1)  # row + column strips
+    # row + column strips; two sets of up to 4 GiB when there is more than one frequency group
+    strip = min(batch, 4) * 3 * n_faces * strip_len * cb * (2 if precision == "mixed" else 1)
+    strip *= 2 if batch > 4 and strip <= 4 << 30 else 1
     singular = batch * n_pairs * 12 * 16
     mesh = n_faces * (2 * 128 + 6 * 6 * 12 + 200) + n_pairs * 16
     field_ws = batch * n_ch * 6 * n_faces * 2 * cb + batch * n_obs * n_ch * 16 * 64

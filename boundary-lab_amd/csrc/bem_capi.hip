@@ -74,6 +74,7 @@ struct bem_ctx {
   hipEvent_t fork_ev, join_ev[BEM_AUX_STREAMS];
   int fanout;  // streams used for batch fan-out (1 .. 1 + BEM_AUX_STREAMS)
   hipEvent_t lu_ev[2];  // look-ahead LU: ready (context stream), done (look-ahead stream)
+  hipEvent_t asm_ev[4];  // assembly: strip set s written (2 s, context stream), gathered (2 s + 1, aux[0])
   int lu_products;      // BEM_LU_FP32, BEM_LU_SPLIT_BF16 or BEM_LU_BF16: large updates of FP32 factorizations
   int sing_precision;   // BEM_SING_WORKING or BEM_SING_FP64: arithmetic of the Duffy sums of mixed batches
   int num_cus;          // compute units of the device
@@ -163,6 +164,7 @@ struct bem_mesh {
   // workspace
   DevBuf strip, sing_lhs, sing_rhs, rhs_part, wsrc, partial, cols_part;
   int strip_nf = 0;
+  int strip_sets = 1;  // 2: the frequency groups alternate between two strip sets
   // captured graphs whose nodes address this mesh's buffers; while any exists the workspace may not move
   int live_graphs = 0;
   bool closed = false;  // bem_mesh_destroy was called while graphs were live
@@ -286,6 +288,11 @@ int bem_ctx_create(int device, void* stream, bem_ctx** out) {
       delete c;
       return fail(BEM_E_HIP, "hipEventCreate failed");
     }
+  for (int i = 0; i < 4; ++i)
+    if (hipEventCreateWithFlags(&c->asm_ev[i], hipEventDisableTiming) != hipSuccess) {
+      delete c;
+      return fail(BEM_E_HIP, "hipEventCreate failed");
+    }
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
     c->own_stream = false;
@@ -309,6 +316,7 @@ int bem_ctx_destroy(bem_ctx* c) {
     (void)hipEventDestroy(c->join_ev[i]);
   }
   for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c->lu_ev[i]);
+  for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->asm_ev[i]);
   (void)hipEventDestroy(c->fork_ev);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -729,9 +737,15 @@ static int reserve_impl(bem_mesh* m, int precision, int nf, int strip_nf, int n_
   const size_t cb = precision == BEM_PREC_MIXED ? 8 : 16;
   strip_nf = std::max(1, std::min(strip_nf, nf));
   int rc;
-  // FP32 without images assembles pairs once: a row strip and a column strip per frequency
+  // FP32 without images assembles pairs once: a row strip and a column strip per frequency; two strip sets of up
+  // to 4 GiB each when there is more than one frequency group (a group's gather overlaps the next group's assembly)
   const size_t nstrips = (precision == BEM_PREC_MIXED && m->R == 1) ? 2 : 1;
-  if ((rc = dev_alloc(m, m->strip, nstrips * strip_nf * 3 * m->F * (size_t)m->L * cb))) return rc;
+  const size_t set_bytes = nstrips * strip_nf * 3 * m->F * (size_t)m->L * cb;
+  // the bound is judged on strip rows of one column per vertex (as the host's footprint estimate counts them)
+  const size_t set_bound = nstrips * strip_nf * 3 * m->F * (((size_t)m->V + 63) / 64 * 64) * cb;
+  const int nsets = (nf > strip_nf && set_bound <= (4ull << 30)) ? 2 : 1;
+  if ((rc = dev_alloc(m, m->strip, nsets * set_bytes))) return rc;
+  m->strip_sets = nsets;
   m->strip_nf = strip_nf;
   if ((rc = dev_alloc(m, m->sing_lhs, (size_t)nf * std::max(1, m->P) * 9 * 16))) return rc;
   if ((rc = dev_alloc(m, m->sing_rhs, (size_t)nf * std::max(1, m->P) * 3 * 16))) return rc;
@@ -821,10 +835,18 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     ScopedTimer tm(c, BEM_TIMER_RHS, (long long)nf);
     HIPCHK(launch_rhs<T>(rp, nf, st), "k_rhs_part launch");
   }
-  // regular strips + gather, in strip-sized frequency groups
+  // regular strips + gather, in strip-sized frequency groups. With more than one group the groups alternate
+  // between two strip sets and each group's gather runs on an auxiliary stream, beside the next group's
+  // assembly (the VALU-bound strips and the HBM-bound gather share the device); a strip set is reused once its
+  // gather is done. Forks from and joins into the context stream only (capturable). Timed runs keep one stream.
   const long long fstride = 3LL * m->F * m->L;
-  for (int f0 = 0; f0 < nf; f0 += m->strip_nf) {
+  const size_t set_stride = (size_t)((sizeof(T) == 4 && m->R == 1) ? 2 : 1) * m->strip_nf * fstride;
+  const int ngroups = (nf + m->strip_nf - 1) / m->strip_nf;
+  const bool overlap = ngroups > 1 && m->strip_sets == 2 && !c->timing && c->fanout >= 2;
+  for (int f0 = 0, gi = 0; f0 < nf; f0 += m->strip_nf, ++gi) {
     const int g = std::min(m->strip_nf, nf - f0);
+    const int set = m->strip_sets == 2 ? (gi & 1) : 0;
+    if (overlap && gi >= 2) HIPCHK(hipStreamWaitEvent(st, c->asm_ev[2 * set + 1], 0), "strip set reuse");
     RegParams<T> p;
     p.rec = rp.rec;
     p.fv = m->fv;
@@ -842,7 +864,7 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     p.sing_trial = m->sing_trial;
     p.sing_lhs = static_cast<const cplx<double>*>(m->sing_lhs.p) + (size_t)f0 * m->P * 9;
     p.k = k + f0;
-    p.strip = static_cast<cplx<T>*>(m->strip.p);
+    p.strip = static_cast<cplx<T>*>(m->strip.p) + set * set_stride;
     // paired assembly (FP32, no images, order-4 rule): the column strip follows the row strips of the group
     p.strip2 = (sizeof(T) == 4 && m->R == 1 && m->nq == 6) ? p.strip + (size_t)m->strip_nf * fstride : nullptr;
     p.nq = m->nq;
@@ -882,10 +904,20 @@ static int assemble_impl(bem_ctx* c, bem_mesh* m, int nf, const double* k, const
     gp.F = m->F;
     gp.n_ch = n_ch;
     gp.roww = m->roww;
-    {
+    if (overlap) {
+      HIPCHK(hipEventRecord(c->asm_ev[2 * set], st), "strip set written");
+      HIPCHK(hipStreamWaitEvent(c->aux[0], c->asm_ev[2 * set], 0), "gather fork");
+      HIPCHK(launch_gather<T>(gp, g, c->aux[0]), "k_gather launch");
+      HIPCHK(hipEventRecord(c->asm_ev[2 * set + 1], c->aux[0]), "strip set gathered");
+    } else {
       ScopedTimer tm(c, BEM_TIMER_GATHER, (long long)g);
       HIPCHK(launch_gather<T>(gp, g, st), "k_gather launch");
     }
+  }
+  if (overlap) {  // join: the last group's gather (the one before it finished before that group's assembly)
+    const int last = (ngroups - 1) & 1;
+    HIPCHK(hipStreamWaitEvent(st, c->asm_ev[2 * last + 1], 0), "gather join");
+    if (ngroups >= 2) HIPCHK(hipStreamWaitEvent(st, c->asm_ev[2 * (last ^ 1) + 1], 0), "gather join");
   }
   return BEM_OK;
 }

@@ -134,9 +134,16 @@ def test_backend_contract(tmp_path, monkeypatch):
 
 
 def test_footprint_grows_with_n_squared():
-    a = S.estimate_footprint(1000, 2000, 27000, 8, 146)
-    b = S.estimate_footprint(4000, 8000, 108000, 8, 146)
+    a = S.estimate_footprint(1000, 2000, 27000, 4, 146)
+    b = S.estimate_footprint(4000, 8000, 108000, 4, 146)
     assert 10 < b / a < 17
+    # more than one frequency group (8 frequencies, strips of 4): a second strip set while a set stays within
+    # 4 GiB, so doubling the batch doubles everything but the mesh; a larger set stays single
+    small_set = 4 * 3 * 2000 * 1024 * 8 * 2
+    assert S.estimate_footprint(1000, 2000, 27000, 8, 146) - 2 * a > -small_set / 2
+    big_set = 4 * 3 * 8000 * 4032 * 8 * 2
+    assert big_set > 4 << 30
+    assert S.estimate_footprint(4000, 8000, 108000, 8, 146) - 2 * b < -big_set / 2
 
 
 ARTIFACT_KEYS = {"freq_hz", "polar_angle_deg", "horizontal_spl_db", "vertical_spl_db", "horizontal_spl_norm_db",
