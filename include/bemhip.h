@@ -226,7 +226,10 @@ int bem_mesh_destroy(bem_mesh* mesh);
  * of moving buffers the graphs address. */
 int bem_mesh_live_graphs(bem_mesh* mesh, int* count);
 /* Reserve library workspace for up to `nf` frequencies per call at `precision` (device allocations happen
- * here, never inside a captured graph). `strip_nf` bounds how many frequencies share one strip buffer. */
+ * here, never inside a captured graph). `strip_nf` bounds how many frequencies share one strip buffer; when nf
+ * exceeds it and a strip set stays within 4 GiB (counted with one column per vertex), two sets are reserved and
+ * bem_assemble gathers one frequency group on an auxiliary stream while the next group's strips are assembled
+ * (not when the context's fan-out is 1 or its timers are on). */
 int bem_mesh_reserve(bem_mesh* mesh, int precision, int nf, int strip_nf, int n_obs, int n_ch);
 /* Bytes of device memory held by the mesh (geometry + workspace). */
 int bem_mesh_device_bytes(bem_mesh* mesh, long long* bytes);
